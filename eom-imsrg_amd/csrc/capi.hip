@@ -1,0 +1,893 @@
+// C-ABI, device plan and nested-commutator drivers of the scalar-commutator
+// path.  The drivers restate the control flow of IMSRG_MAGNUS.f90
+// (BCH_EXPAND 149-234, MAGNUS_EXPAND 620-691, euler_step 754-764,
+// magnus_decouple 15-128) on device-resident operators.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <string>
+#include <tuple>
+#include <type_traits>
+#include <vector>
+
+#include "basis.hpp"
+#include "imsrg_hip.h"
+#include "kernels.hpp"
+
+using namespace imsrg;
+
+namespace {
+thread_local std::string g_err;
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+}  // namespace
+
+#define HIPCHK(x)                                                                        \
+  do {                                                                                   \
+    hipError_t e_ = (x);                                                                 \
+    if (e_ != hipSuccess) return fail(IMSRG_EHIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+struct imsrg_basis_s {
+  Basis b;
+  int path = IMSRG_PATH_GROUPED;
+  bool dev_ready = false;
+  DevBasis D{};
+  std::vector<void*> allocs;
+  // task lists (grouped by channel / cross-coupled block)
+  TaskList<SegTask> seg_tri, seg_full, seg_g3, seg_epi;
+  TaskList<CCTask> cc;
+  TaskList<PairTask> pairs;
+  TaskList<GemmTask> k2, k3, ph;
+  std::vector<GemmProb> k2p_h, k3p_h, php_h;
+  GemmProb *k2p = nullptr, *k3p = nullptr, *php = nullptr;
+  std::vector<OneTask> one_h;
+  OneTask* one = nullptr;
+  // norm chunks
+  std::vector<i64> ch_off_h;
+  std::vector<int> ch_len_h;
+  std::vector<double> ch_w_h;
+  i64* ch_off = nullptr;
+  int* ch_len = nullptr;
+  double* ch_w = nullptr;
+  // scratch
+  std::vector<i64> poff_h;
+  i64* poff = nullptr;
+  double* Pscr = nullptr;
+  double* part = nullptr;
+  int npart = 0;
+  double* M = nullptr;       // monopole table N x N
+  double* dval = nullptr;    // device scalars
+  double* hval = nullptr;    // pinned host scalars
+
+  ~imsrg_basis_s() {
+    for (void* p : allocs) (void)hipFree(p);
+    if (hval) (void)hipHostFree(hval);
+  }
+};
+
+namespace {
+
+template <class T>
+int upload(imsrg_basis_s* B, const std::vector<T>& v, T** out) {
+  *out = nullptr;
+  if (v.empty()) return 0;
+  void* p = nullptr;
+  HIPCHK(hipMalloc(&p, v.size() * sizeof(T)));
+  B->allocs.push_back(p);
+  HIPCHK(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  *out = (T*)p;
+  return 0;
+}
+
+int dalloc(imsrg_basis_s* B, size_t bytes, void** out) {
+  void* p = nullptr;
+  HIPCHK(hipMalloc(&p, std::max<size_t>(bytes, 8)));
+  B->allocs.push_back(p);
+  *out = p;
+  return 0;
+}
+
+// channels in decreasing pppp cost, so a grouped launch issues its largest
+// workgroups first (longest-processing-time order)
+std::vector<int> channel_order(const Basis& b) {
+  std::vector<int> o(b.nblocks);
+  for (int q = 0; q < b.nblocks; ++q) o[q] = q;
+  std::stable_sort(o.begin(), o.end(), [&](int x, int y) {
+    const i64 cx = (i64)b.dims[x * 3] + b.dims[x * 3 + 1], cy = (i64)b.dims[y * 3] + b.dims[y * 3 + 1];
+    return cx > cy;
+  });
+  return o;
+}
+
+enum SegMode { SEG_TRI, SEG_FULL, SEG_G3, SEG_EPI };
+
+void build_seg(const Basis& b, TaskList<SegTask>& tl, SegMode mode, const std::vector<int>& order) {
+  tl.host.clear();
+  tl.chan_begin.assign(1, 0);
+  for (int q : order) {
+    const int np = b.dims[q * 3], nb = b.dims[q * 3 + 1], nh = b.dims[q * 3 + 2];
+    for (int g = 0; g < 6; ++g) {
+      i64 cnt = 0;
+      const int n1 = b.rows(q, g), n2 = b.cols(q, g);
+      switch (mode) {
+        case SEG_TRI: cnt = kSquare[g] ? (i64)n1 * (n1 + 1) / 2 : (i64)n1 * n2; break;
+        case SEG_FULL: cnt = (i64)n1 * n2; break;
+        case SEG_G3: cnt = (g == 2) ? (i64)n1 * n2 : 0; break;
+        case SEG_EPI:
+          if (g == 1) cnt = (i64)np * (nb + nh);
+          else if (g == 3) cnt = (i64)nb * nb;
+          else if (g == 4) cnt = (i64)nh * nh;
+          else if (g == 5) cnt = (i64)nb * nh;
+          break;
+      }
+      for (i64 s = 0; s < cnt; s += kTile) tl.host.push_back(SegTask{q, g, (int)s, (int)cnt});
+    }
+    tl.chan_begin.push_back((int)tl.host.size());
+  }
+  tl.n = (int)tl.host.size();
+}
+
+int tiles(int n) { return (n + 63) / 64; }
+
+int build_plan(imsrg_basis_s* B) {
+  const Basis& b = B->b;
+  const std::vector<int> order = channel_order(b);
+  build_seg(b, B->seg_tri, SEG_TRI, order);
+  build_seg(b, B->seg_full, SEG_FULL, order);
+  build_seg(b, B->seg_g3, SEG_G3, order);
+  build_seg(b, B->seg_epi, SEG_EPI, order);
+  // cross-coupled element tiles, largest blocks first
+  {
+    std::vector<int> co(b.ncc);
+    for (int q = 0; q < b.ncc; ++q) co[q] = q;
+    std::stable_sort(co.begin(), co.end(),
+                     [&](int x, int y) { return (i64)b.rlen[x] * b.ncol[x] > (i64)b.rlen[y] * b.ncol[y]; });
+    B->cc.host.clear();
+    B->cc.chan_begin.assign(1, 0);
+    B->ph.host.clear();
+    B->ph.chan_begin.assign(1, 0);
+    B->php_h.clear();
+    std::vector<int> co2 = co;   // WCC GEMM: largest r^2 * nb first
+    for (int q : co) {
+      const i64 cnt = (i64)b.rlen[q] * b.ncol[q];
+      for (i64 s = 0; s < cnt; s += kTile) B->cc.host.push_back(CCTask{q, (int)s});
+      B->cc.chan_begin.push_back((int)B->cc.host.size());
+    }
+    std::stable_sort(co2.begin(), co2.end(), [&](int x, int y) {
+      return (i64)b.rlen[x] * b.rlen[x] * (b.ncol[x] + 1) > (i64)b.rlen[y] * b.rlen[y] * (b.ncol[y] + 1);
+    });
+    for (int q : co2) {
+      const int r = b.rlen[q];
+      if (r > 0) {
+        GemmProb p{};
+        p.a1 = b.ccoff[q]; p.b1 = b.ccoff[q]; p.c = b.wcoff[q];
+        p.m = r; p.n = r; p.k = b.ncol[q];
+        p.lda = r; p.ldb = r; p.ldc = r;
+        p.nprod = 1; p.ta1 = 0; p.tb1 = 1;
+        p.sa1 = 0; p.sb1 = 1; p.sc = 2;
+        p.alpha = 1.0; p.s2 = 0.0; p.beta = 0.0;
+        const int pid = (int)B->php_h.size();
+        B->php_h.push_back(p);
+        for (int tm = 0; tm < tiles(r); ++tm)
+          for (int tn = 0; tn < tiles(r); ++tn) B->ph.host.push_back(GemmTask{pid, tm, tn, 0});
+      }
+      B->ph.chan_begin.push_back((int)B->ph.host.size());
+    }
+    B->cc.n = (int)B->cc.host.size();
+    B->ph.n = (int)B->ph.host.size();
+  }
+  // 222_pp_hh: pppp tile pairs, K2 = pp x (ph|hh) dual product, K3 = (ph|hh)^T (ph|hh)
+  B->pairs.host.clear(); B->pairs.chan_begin.assign(1, 0);
+  B->k2.host.clear(); B->k2.chan_begin.assign(1, 0);
+  B->k3.host.clear(); B->k3.chan_begin.assign(1, 0);
+  B->k2p_h.clear(); B->k3p_h.clear();
+  B->poff_h.assign(b.nblocks, 0);
+  i64 pofs = 0;
+  for (int q = 0; q < b.nblocks; ++q) {
+    const int nb = b.dims[q * 3 + 1], nh = b.dims[q * 3 + 2];
+    B->poff_h[q] = pofs;
+    pofs += (i64)(nb + nh) * (nb + nh);
+  }
+  for (int q : order) {
+    const int np = b.dims[q * 3], nb = b.dims[q * 3 + 1], nh = b.dims[q * 3 + 2];
+    const int T = tiles(np);
+    for (int ti = 0; ti < T; ++ti)
+      for (int tj = ti; tj < T; ++tj) B->pairs.host.push_back(PairTask{q, ti, tj, 0});
+    B->pairs.chan_begin.push_back((int)B->pairs.host.size());
+    if (np > 0 && nb + nh > 0) {
+      GemmProb p{};
+      p.a1 = b.goff[q * 6 + 0]; p.b1 = b.goff[q * 6 + 1];
+      p.a2 = b.goff[q * 6 + 0]; p.b2 = b.goff[q * 6 + 1];
+      p.c = b.goff[q * 6 + 1];
+      p.m = np; p.n = nb + nh; p.k = np;
+      p.lda = np; p.ldb = np; p.ldc = np;
+      p.nprod = 2;
+      p.sa1 = 0; p.sb1 = 1; p.sa2 = 1; p.sb2 = 0; p.sc = 2;
+      p.alpha = 1.0; p.s2 = -1.0; p.beta = 0.0;
+      const int pid = (int)B->k2p_h.size();
+      B->k2p_h.push_back(p);
+      for (int tm = 0; tm < tiles(np); ++tm)
+        for (int tn = 0; tn < tiles(nb + nh); ++tn) B->k2.host.push_back(GemmTask{pid, tm, tn, 0});
+    }
+    B->k2.chan_begin.push_back((int)B->k2.host.size());
+    if (nb + nh > 0) {
+      GemmProb p{};
+      p.a1 = b.goff[q * 6 + 1]; p.b1 = b.goff[q * 6 + 1];
+      p.c = B->poff_h[q];
+      p.m = nb + nh; p.n = nb + nh; p.k = np;
+      p.lda = std::max(np, 1); p.ldb = std::max(np, 1); p.ldc = nb + nh;
+      p.nprod = 1; p.ta1 = 1; p.tb1 = 0;
+      p.sa1 = 0; p.sb1 = 1; p.sc = 2;
+      p.alpha = 1.0; p.s2 = 0.0; p.beta = 0.0;
+      const int pid = (int)B->k3p_h.size();
+      B->k3p_h.push_back(p);
+      for (int tm = 0; tm < tiles(nb + nh); ++tm)
+        for (int tn = 0; tn < tiles(nb + nh); ++tn) B->k3.host.push_back(GemmTask{pid, tm, tn, 0});
+    }
+    B->k3.chan_begin.push_back((int)B->k3.host.size());
+  }
+  B->pairs.n = (int)B->pairs.host.size();
+  B->k2.n = (int)B->k2.host.size();
+  B->k3.n = (int)B->k3.host.size();
+  // one-body outputs of 121 / 221 (hh p<=q, pp p<=q, ph), same (l, j, tz)
+  B->one_h.clear();
+  auto same = [&](int x, int y) { return b.jj[x] == b.jj[y] && b.ll[x] == b.ll[y] && b.tz[x] == b.tz[y]; };
+  for (int p = 0; p < b.A; ++p)
+    for (int q = p; q < b.A; ++q)
+      if (same(b.holes[p], b.holes[q])) B->one_h.push_back(OneTask{0, b.holes[p], b.holes[q], p, q});
+  for (int p = 0; p < b.P; ++p)
+    for (int q = p; q < b.P; ++q)
+      if (same(b.parts[p], b.parts[q])) B->one_h.push_back(OneTask{1, b.parts[p], b.parts[q], p, q});
+  for (int p = 0; p < b.P; ++p)
+    for (int q = 0; q < b.A; ++q)
+      if (same(b.parts[p], b.holes[q])) B->one_h.push_back(OneTask{2, b.parts[p], b.holes[q], p, q});
+  // norm chunks: fhh (w 1), fph (w 2), fpp (w 1), gam blocks (w 1 square, 2 rectangular)
+  B->ch_off_h.clear(); B->ch_len_h.clear(); B->ch_w_h.clear();
+  auto add_range = [&](i64 off, i64 len, double w) {
+    for (i64 s = 0; s < len; s += 8192) {
+      B->ch_off_h.push_back(off + s);
+      B->ch_len_h.push_back((int)std::min<i64>(8192, len - s));
+      B->ch_w_h.push_back(w);
+    }
+  };
+  add_range(b.off_fhh, (i64)b.A * b.A, 1.0);
+  add_range(b.off_fph, (i64)b.P * b.A, 2.0);
+  add_range(b.off_fpp, (i64)b.P * b.P, 1.0);
+  for (int q = 0; q < b.nblocks; ++q)
+    for (int g = 0; g < 6; ++g) add_range(b.goff[q * 6 + g], b.blocksize(q, g), kSquare[g] ? 1.0 : 2.0);
+
+  // upload
+  int rc;
+  if ((rc = upload(B, B->seg_tri.host, &B->seg_tri.dev))) return rc;
+  if ((rc = upload(B, B->seg_full.host, &B->seg_full.dev))) return rc;
+  if ((rc = upload(B, B->seg_g3.host, &B->seg_g3.dev))) return rc;
+  if ((rc = upload(B, B->seg_epi.host, &B->seg_epi.dev))) return rc;
+  if ((rc = upload(B, B->cc.host, &B->cc.dev))) return rc;
+  if ((rc = upload(B, B->ph.host, &B->ph.dev))) return rc;
+  if ((rc = upload(B, B->php_h, &B->php))) return rc;
+  if ((rc = upload(B, B->pairs.host, &B->pairs.dev))) return rc;
+  if ((rc = upload(B, B->k2.host, &B->k2.dev))) return rc;
+  if ((rc = upload(B, B->k2p_h, &B->k2p))) return rc;
+  if ((rc = upload(B, B->k3.host, &B->k3.dev))) return rc;
+  if ((rc = upload(B, B->k3p_h, &B->k3p))) return rc;
+  if ((rc = upload(B, B->one_h, &B->one))) return rc;
+  if ((rc = upload(B, B->ch_off_h, &B->ch_off))) return rc;
+  if ((rc = upload(B, B->ch_len_h, &B->ch_len))) return rc;
+  if ((rc = upload(B, B->ch_w_h, &B->ch_w))) return rc;
+  if ((rc = upload(B, B->poff_h, &B->poff))) return rc;
+  if ((rc = dalloc(B, std::max<i64>(pofs, 1) * sizeof(double), (void**)&B->Pscr))) return rc;
+  B->npart = std::max<int>(std::max<int>((int)B->ch_off_h.size(), B->seg_g3.n), b.nblocks + 1);
+  if ((rc = dalloc(B, (size_t)B->npart * sizeof(double), (void**)&B->part))) return rc;
+  if ((rc = dalloc(B, (size_t)b.N * b.N * sizeof(double), (void**)&B->M))) return rc;
+  if ((rc = dalloc(B, 64 * sizeof(double), (void**)&B->dval))) return rc;
+  HIPCHK(hipHostMalloc((void**)&B->hval, 64 * sizeof(double), hipHostMallocDefault));
+  return 0;
+}
+
+int ensure_dev(imsrg_basis_s* B) {
+  if (B->dev_ready) return 0;
+  const Basis& b = B->b;
+  DevBasis& D = B->D;
+  int rc;
+  D.N = b.N; D.A = b.A; D.P = b.P; D.JTM = b.JTM; D.nJ = b.nJ;
+  D.nblocks = b.nblocks; D.ncc = b.ncc;
+  D.off_fhh = b.off_fhh; D.off_fph = b.off_fph; D.off_fpp = b.off_fpp;
+  D.oplen = b.oplen; D.cclen = b.cclen; D.wclen = b.wclen;
+  D.nh6 = b.nh6; D.nJ6 = b.nJ6;
+#define UP(field, vec)                                                   \
+  if ((rc = upload(B, b.vec, (std::remove_const_t<std::remove_pointer_t<decltype(D.field)>>**)&D.field))) return rc;
+  UP(jj, jj) UP(ll, ll) UP(tz, tz) UP(con, con) UP(cidx, cidx) UP(holes, holes) UP(parts, parts)
+  UP(spstart, spstart) UP(spmem, spmem) UP(spblk, spblk) UP(xmap, xmap) UP(dims, dims) UP(chJ, chJ)
+  UP(goff, goff) UP(pairs, pairs) UP(pairoff, pairoff) UP(rlen, rlen) UP(ncol, ncol) UP(ccJ, ccJ)
+  UP(ccoff, ccoff) UP(wcoff, wcoff) UP(rmap, rmap) UP(nbmap, nbmap) UP(ccrows, ccrows) UP(ccrowoff, ccrowoff)
+  UP(cccols, cccols) UP(cccoloff, cccoloff) UP(six, six)
+#undef UP
+  if ((rc = build_plan(B))) return rc;
+  B->dev_ready = true;
+  return 0;
+}
+
+// launch a task list either in one grouped launch or one launch per group
+template <class T, class F>
+void run(imsrg_basis_s* B, TaskList<T>& tl, F&& launch) {
+  if (tl.n == 0) return;
+  if (B->path == IMSRG_PATH_GROUPED) {
+    launch(tl.dev, tl.n);
+    return;
+  }
+  for (size_t g = 0; g + 1 < tl.chan_begin.size(); ++g) {
+    const int b0 = tl.chan_begin[g], e0 = tl.chan_begin[g + 1];
+    if (e0 > b0) launch(tl.dev + b0, e0 - b0);
+  }
+}
+
+hipStream_t S(imsrg_stream s) { return (hipStream_t)s; }
+
+// ------------------------------------------------------------ term bodies
+void t_e0(imsrg_basis_s* B, const double* L, int Lh, const double* R, int Rh, double* dst, hipStream_t s) {
+  launch_e0_partials(B->D, L, Lh, R, Rh, B->part, s);
+  launch_e0_final(B->part, B->b.nblocks, (Lh * Rh == -1) ? 1 : 0, 1, dst, 0, s);
+}
+void t_111(imsrg_basis_s* B, const double* L, int Lh, const double* R, int Rh, double* RES, hipStream_t s) {
+  launch_comm111(B->D, L, Lh, R, Rh, RES, s);
+}
+void t_121(imsrg_basis_s* B, const double* L, int Lh, const double* R, int Rh, double* RES, int RESh,
+           hipStream_t s) {
+  launch_comm121(B->D, B->one, (int)B->one_h.size(), L, Lh, R, Rh, RES, RESh, s);
+}
+void t_122(imsrg_basis_s* B, const double* L, int Lh, const double* R, int Rh, double* RES, int RESh,
+           hipStream_t s) {
+  run(B, B->seg_tri, [&](const SegTask* t, int n) { launch_comm122(B->D, t, n, L, Lh, R, Rh, RES, RESh, s); });
+}
+void t_pphh(imsrg_basis_s* B, const double* L, int Lh, const double* R, int Rh, double* RES, int RESh,
+            double* w1, double* w2, hipStream_t s) {
+  (void)RESh;
+  run(B, B->pairs,
+      [&](const PairTask* t, int n) { launch_pppp_pairs(B->D, t, n, L, Lh, R, Rh, RES, w1, w2, 1, s); });
+  Slots s2{};
+  s2.p[0] = L; s2.p[1] = R; s2.p[2] = w1;
+  run(B, B->k2, [&](const GemmTask* t, int n) { launch_gemm(s2, B->k2p, t, n, s); });
+  Slots s3{};
+  s3.p[0] = L; s3.p[1] = R; s3.p[2] = B->Pscr;
+  run(B, B->k3, [&](const GemmTask* t, int n) { launch_gemm(s3, B->k3p, t, n, s); });
+  run(B, B->seg_epi, [&](const SegTask* t, int n) {
+    launch_pphh_epilogue(B->D, t, n, L, Lh, R, Rh, RES, w1, w2, B->Pscr, B->poff, s);
+  });
+}
+void t_221(imsrg_basis_s* B, int Lh, int Rh, double* RES, int RESh, const double* w1, const double* w2,
+           hipStream_t s) {
+  launch_comm221(B->D, B->one, (int)B->one_h.size(), Lh * Rh, w1, w2, RES, RESh, s);
+}
+void t_pandya(imsrg_basis_s* B, const double* op, int herm, double* CC, hipStream_t s) {
+  run(B, B->cc, [&](const CCTask* t, int n) { launch_pandya(B->D, t, n, op, herm, CC, s); });
+}
+void t_ph(imsrg_basis_s* B, const double* LCC, int Lh, const double* RCC, int Rh, double* RES, int RESh,
+          double* WCC, hipStream_t s) {
+  Slots sl{};
+  sl.p[0] = LCC; sl.p[1] = RCC; sl.p[2] = WCC;
+  run(B, B->ph, [&](const GemmTask* t, int n) { launch_gemm(sl, B->php, t, n, s); });
+  run(B, B->seg_tri,
+      [&](const SegTask* t, int n) { launch_comm222ph_scatter(B->D, t, n, WCC, Lh, Rh, RES, RESh, s); });
+}
+// device norm of op into dst (device)
+void t_norm(imsrg_basis_s* B, const double* op, double* dst, hipStream_t s) {
+  const int nch = (int)B->ch_off_h.size();
+  launch_norm_partials(op, B->ch_off, B->ch_len, B->ch_w, nch, B->part, s);
+  launch_sum_sqrt(B->part, nch, dst, 1, s);
+}
+
+// RES = [L, R]: the term sequence of BCH_EXPAND / MAGNUS_EXPAND
+// (IMSRG_MAGNUS.f90:207-219, 677-683).  LCC must already hold the
+// cross-coupled L; RCC is computed here.
+void t_full(imsrg_basis_s* B, const double* L, int Lh, const double* R, int Rh, double* RES, int RESh, bool e0,
+            double* w1, double* w2, const double* LCC, double* RCC, double* WCC, hipStream_t s) {
+  t_pandya(B, R, Rh, RCC, s);
+  if (e0) t_e0(B, L, Lh, R, Rh, RES, s);
+  t_111(B, L, Lh, R, Rh, RES, s);
+  t_121(B, L, Lh, R, Rh, RES, RESh, s);
+  t_122(B, L, Lh, R, Rh, RES, RESh, s);
+  t_pphh(B, L, Lh, R, Rh, RES, RESh, w1, w2, s);
+  t_221(B, Lh, Rh, RES, RESh, w1, w2, s);
+  t_ph(B, LCC, Lh, RCC, Rh, RES, RESh, WCC, s);
+}
+
+}  // namespace
+
+// =================================================================== flow
+struct imsrg_flow_s {
+  imsrg_basis_s* B = nullptr;
+  double *buf[2] = {nullptr, nullptr};   // INT2 / AD ping-pong
+  double *w1 = nullptr, *w2 = nullptr, *GCC = nullptr, *ADCC = nullptr, *WCC = nullptr;
+  double *int1 = nullptr, *Hbare = nullptr;
+  double* coefs = nullptr;   // device coefficient table for graph terms
+  bool graphs = false;
+  struct GraphKey {
+    int kind, iw;
+    const void *a, *b, *c;
+    int h1, h2;
+    bool operator<(const GraphKey& o) const {
+      return std::tie(kind, iw, a, b, c, h1, h2) < std::tie(o.kind, o.iw, o.a, o.b, o.c, o.h1, o.h2);
+    }
+  };
+  std::map<GraphKey, hipGraphExec_t> execs;
+  std::vector<void*> allocs;
+  ~imsrg_flow_s() {
+    for (auto& kv : execs) (void)hipGraphExecDestroy(kv.second);
+    for (void* p : allocs) (void)hipFree(p);
+  }
+};
+
+namespace {
+
+int falloc(imsrg_flow_s* F, i64 n, double** out) {
+  void* p = nullptr;
+  HIPCHK(hipMalloc(&p, std::max<i64>(n, 1) * sizeof(double)));
+  F->allocs.push_back(p);
+  *out = (double*)p;
+  return 0;
+}
+
+// Run `body` on stream s, through a cached hipGraph when graphs are on.
+template <class Body>
+int run_term(imsrg_flow_s* F, const imsrg_flow_s::GraphKey& key, hipStream_t s, Body&& body) {
+  if (!F->graphs || F->B->path != IMSRG_PATH_GROUPED) {
+    body(s);
+    HIPCHK(hipGetLastError());
+    return 0;
+  }
+  auto it = F->execs.find(key);
+  if (it == F->execs.end()) {
+    hipStream_t cs;
+    HIPCHK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+    hipGraph_t g;
+    HIPCHK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+    body(cs);
+    HIPCHK(hipStreamEndCapture(cs, &g));
+    hipGraphExec_t ex;
+    HIPCHK(hipGraphInstantiate(&ex, g, nullptr, nullptr, 0));
+    HIPCHK(hipGraphDestroy(g));
+    HIPCHK(hipStreamDestroy(cs));
+    it = F->execs.emplace(key, ex).first;
+  }
+  HIPCHK(hipGraphLaunch(it->second, s));
+  return 0;
+}
+
+int read_scalar(imsrg_basis_s* B, const double* dev, double* out, hipStream_t s) {
+  HIPCHK(hipMemcpyAsync(B->hval, dev, sizeof(double), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  *out = B->hval[0];
+  return 0;
+}
+
+// BCH_EXPAND, IMSRG_MAGNUS.f90:149-234
+int bch(imsrg_flow_s* F, double* HS, const double* G, int Gh, const double* H, int Hh, double conv, int maxit,
+        int* nterms, double* advals, hipStream_t s) {
+  imsrg_basis_s* B = F->B;
+  const i64 n = B->b.oplen;
+  HIPCHK(hipMemcpyAsync(HS, H, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+  HIPCHK(hipMemcpyAsync(F->buf[0], H, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+  // G is constant over the expansion: its cross-coupled form once
+  t_pandya(B, G, Gh, F->GCC, s);
+  double coef = 1.0;
+  int iw;
+  for (iw = 2; iw <= maxit; ++iw) {
+    coef = coef / (iw - 1.0);
+    double* AD = F->buf[iw % 2];
+    double* INT2 = F->buf[(iw + 1) % 2];
+    const double c = coef;
+    imsrg_flow_s::GraphKey key{0, iw, HS, G, H, Gh, Hh};
+    int rc = run_term(F, key, s, [&](hipStream_t cs) {
+      t_full(B, G, Gh, AD, Hh, INT2, Hh, true, F->w1, F->w2, F->GCC, F->ADCC, F->WCC, cs);
+      launch_axpy(n, HS, 1.0, INT2, c, HS, cs);
+      t_norm(B, INT2, B->dval, cs);
+    });
+    if (rc) return rc;
+    double nrm = 0.0;
+    if ((rc = read_scalar(B, B->dval, &nrm, s))) return rc;
+    const double adv = nrm * coef;
+    if (advals) advals[iw - 2] = adv;
+    if (adv < conv) break;
+  }
+  if (nterms) *nterms = iw;
+  if (iw > maxit) return fail(IMSRG_ECLIP, "clipping BCH");
+  return 0;
+}
+
+// MAGNUS_EXPAND, IMSRG_MAGNUS.f90:620-691
+int magnus(imsrg_flow_s* F, double* DG, int DGh, const double* G, int Gh, double conv, int* ncomm,
+           hipStream_t s) {
+  imsrg_basis_s* B = F->B;
+  const i64 n = B->b.oplen;
+  const double cof[7] = {1.0, -0.5, .0833333333, 0.0, -0.00138888888, 0.0, 3.306878e-5};
+  double advals[7] = {0, 0, 0, 0, 0, 0, 0};
+  int rc;
+  // INT2 = DG (both ping-pong buffers carry DG's E0, which no term writes)
+  HIPCHK(hipMemcpyAsync(F->buf[0], DG, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+  HIPCHK(hipMemcpyAsync(F->buf[1], DG, sizeof(double), hipMemcpyDeviceToDevice, s));
+  t_norm(B, DG, B->dval, s);
+  t_norm(B, G, B->dval + 1, s);
+  HIPCHK(hipMemcpyAsync(B->hval, B->dval, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  advals[0] = B->hval[0];
+  const double fullnorm = B->hval[1];
+  if (ncomm) *ncomm = 0;
+  if (fullnorm < (double)1e-9f) return 0;
+  t_pandya(B, G, Gh, F->GCC, s);
+  int q = 1, cnt = 0;
+  for (int i = 2; i <= 7; ++i) {
+    double* AD = F->buf[i % 2];
+    double* INT2 = F->buf[(i + 1) % 2];
+    const double adnorm = advals[i - q - 1];
+    q = (std::fabs(cof[i - 1]) > (double)1e-6f) ? 1 : 2;
+    if (std::fabs(adnorm / fullnorm) < conv) break;
+    const double c = cof[i - 1];
+    imsrg_flow_s::GraphKey key{1, i, DG, G, nullptr, Gh, DGh};
+    rc = run_term(F, key, s, [&](hipStream_t cs) {
+      t_full(B, G, Gh, AD, DGh, INT2, DGh, false, F->w1, F->w2, F->GCC, F->ADCC, F->WCC, cs);
+      launch_axpy(n, DG, 1.0, INT2, c, DG, cs);
+      t_norm(B, INT2, B->dval, cs);
+    });
+    if (rc) return rc;
+    ++cnt;
+    double nrm = 0.0;
+    if ((rc = read_scalar(B, B->dval, &nrm, s))) return rc;
+    advals[i - 1] = nrm * std::fabs(cof[i - 1]);
+  }
+  if (ncomm) *ncomm = cnt;
+  return 0;
+}
+
+int white(imsrg_basis_s* B, const double* H, double* ETA, hipStream_t s) {
+  launch_monopole(B->D, H, B->M, s);
+  launch_white_1b(B->D, H, ETA, s);
+  launch_white_2b(B->D, B->seg_full.dev, B->seg_full.n, H, B->M, ETA, s);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int mbpt2_dev(imsrg_basis_s* B, const double* H, double* dst, hipStream_t s) {
+  launch_mbpt2_partials(B->D, B->seg_g3.dev, B->seg_g3.n, H, B->part, s);
+  launch_sum_sqrt(B->part, B->seg_g3.n, dst, 0, s);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int magnus_step(imsrg_flow_s* F, double* HS, double* G, double* DG, const double* H, double ds, double bch_conv,
+                double mag_conv, double* out, hipStream_t s) {
+  imsrg_basis_s* B = F->B;
+  int rc, nm = 0, nb = 0;
+  if ((rc = magnus(F, DG, -1, G, -1, mag_conv, &nm, s))) return rc;
+  launch_axpy(B->b.oplen, G, 1.0, DG, ds, G, s);   // euler_step
+  if ((rc = bch(F, HS, G, -1, H, 1, bch_conv, 40, &nb, nullptr, s))) return rc;
+  if ((rc = white(B, HS, DG, s))) return rc;
+  if ((rc = mbpt2_dev(B, HS, B->dval + 2, s))) return rc;
+  HIPCHK(hipMemcpyAsync(B->dval + 3, HS, sizeof(double), hipMemcpyDeviceToDevice, s));
+  HIPCHK(hipMemcpyAsync(B->hval, B->dval + 2, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  if (out) {
+    out[0] = B->hval[1];
+    out[1] = B->hval[0];
+    out[2] = nm;
+    out[3] = nb;
+  }
+  return 0;
+}
+
+}  // namespace
+
+// ================================================================== C-ABI
+extern "C" {
+
+const char* imsrg_last_error(void) { return g_err.c_str(); }
+
+int imsrg_basis_create(int norb, const int* n, const int* l, const int* jj2, const int* tz2, const int* hole,
+                       imsrg_basis* out) {
+  if (!out || !n || !l || !jj2 || !tz2 || !hole) return fail(IMSRG_EINVAL, "null argument");
+  imsrg_basis_s* B = new imsrg_basis_s();
+  std::string e = build_basis(B->b, norb, n, l, jj2, tz2, hole);
+  if (!e.empty()) {
+    delete B;
+    return fail(IMSRG_EINVAL, e);
+  }
+  *out = B;
+  return 0;
+}
+
+int imsrg_basis_destroy(imsrg_basis B) {
+  delete B;
+  return 0;
+}
+
+int64_t imsrg_basis_size(imsrg_basis B, int what) {
+  if (!B) return -1;
+  const Basis& b = B->b;
+  switch (what) {
+    case IMSRG_SIZE_NORB: return b.N;
+    case IMSRG_SIZE_NHOLE: return b.A;
+    case IMSRG_SIZE_NPART: return b.P;
+    case IMSRG_SIZE_JTOTAL_MAX: return b.JTM;
+    case IMSRG_SIZE_NBLOCKS: return b.nblocks;
+    case IMSRG_SIZE_OP: return b.oplen;
+    case IMSRG_SIZE_NCC: return b.ncc;
+    case IMSRG_SIZE_CC: return b.cclen;
+    case IMSRG_SIZE_WCC: return b.wclen;
+  }
+  return -1;
+}
+
+int imsrg_basis_channels(imsrg_basis B, int* dims, int64_t* goff, int* jtp) {
+  if (!B) return fail(IMSRG_EINVAL, "null basis");
+  const Basis& b = B->b;
+  for (int q = 0; q < b.nblocks; ++q) {
+    for (int c = 0; c < 3; ++c) if (dims) dims[q * 3 + c] = b.dims[q * 3 + c];
+    for (int g = 0; g < 6; ++g) if (goff) goff[q * 6 + g] = b.goff[q * 6 + g];
+    if (jtp) { jtp[q * 3] = b.chJ[q]; jtp[q * 3 + 1] = b.chT[q]; jtp[q * 3 + 2] = b.chP[q]; }
+  }
+  return 0;
+}
+
+int imsrg_basis_pairs(imsrg_basis B, int q, int cls, int* out) {
+  if (!B || q < 0 || q >= B->b.nblocks || cls < 0 || cls > 2) return fail(IMSRG_EINVAL, "bad channel/class");
+  const std::vector<int>& v = B->b.pl[q * 3 + cls];
+  if (out) std::copy(v.begin(), v.end(), out);
+  return (int)v.size() / 2;
+}
+
+int imsrg_basis_cc(imsrg_basis B, int* rlen, int* ncol, int64_t* ccoff, int64_t* wcoff, int* jtp) {
+  if (!B) return fail(IMSRG_EINVAL, "null basis");
+  const Basis& b = B->b;
+  for (int q = 0; q < b.ncc; ++q) {
+    if (rlen) rlen[q] = b.rlen[q];
+    if (ncol) ncol[q] = b.ncol[q];
+    if (ccoff) ccoff[q] = b.ccoff[q];
+    if (wcoff) wcoff[q] = b.wcoff[q];
+    if (jtp) { jtp[q * 3] = b.ccJ[q]; jtp[q * 3 + 1] = b.ccT[q]; jtp[q * 3 + 2] = b.ccP[q]; }
+  }
+  return 0;
+}
+
+double imsrg_basis_sixj(imsrg_basis B, int j1, int j2, int j3, int j4, int j5, int j6) {
+  (void)B;
+  return sixj_value(j1, j2, j3, j4, j5, j6);
+}
+
+int imsrg_set_path(imsrg_basis B, int path) {
+  if (!B || (path != IMSRG_PATH_GROUPED && path != IMSRG_PATH_PLAIN)) return fail(IMSRG_EINVAL, "bad path");
+  B->path = path;
+  return 0;
+}
+
+#define PREP()                          \
+  if (!B) return fail(IMSRG_EINVAL, "null basis"); \
+  {                                     \
+    int rc_ = ensure_dev(B);            \
+    if (rc_) return rc_;                \
+  }                                     \
+  hipStream_t s = S(stream);
+
+int imsrg_commutator_110(imsrg_basis B, const double* L, int Lh, const double* R, int Rh, double* value,
+                         imsrg_stream stream) {
+  PREP();
+  launch_e0_partials(B->D, L, Lh, R, Rh, B->part, s);
+  launch_e0_final(B->part, B->b.nblocks, 0, 1, B->dval, 0, s);
+  HIPCHK(hipGetLastError());
+  return read_scalar(B, B->dval, value, s);
+}
+
+int imsrg_commutator_220(imsrg_basis B, const double* L, int Lh, const double* R, int Rh, double* value,
+                         imsrg_stream stream) {
+  PREP();
+  launch_e0_partials(B->D, L, Lh, R, Rh, B->part, s);
+  launch_e0_final(B->part, B->b.nblocks, (Lh * Rh == -1) ? 1 : 0, 0, B->dval, 0, s);
+  HIPCHK(hipGetLastError());
+  return read_scalar(B, B->dval, value, s);
+}
+
+int imsrg_commutator_e0(imsrg_basis B, const double* L, int Lh, const double* R, int Rh, double* RES,
+                        imsrg_stream stream) {
+  PREP();
+  t_e0(B, L, Lh, R, Rh, RES, s);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int imsrg_commutator_111(imsrg_basis B, const double* L, int Lh, const double* R, int Rh, double* RES, int RESh,
+                         imsrg_stream stream) {
+  PREP();
+  (void)RESh;
+  t_111(B, L, Lh, R, Rh, RES, s);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int imsrg_commutator_121(imsrg_basis B, const double* L, int Lh, const double* R, int Rh, double* RES, int RESh,
+                         imsrg_stream stream) {
+  PREP();
+  t_121(B, L, Lh, R, Rh, RES, RESh, s);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int imsrg_commutator_122(imsrg_basis B, const double* L, int Lh, const double* R, int Rh, double* RES, int RESh,
+                         imsrg_stream stream) {
+  PREP();
+  t_122(B, L, Lh, R, Rh, RES, RESh, s);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int imsrg_commutator_222_pp_hh(imsrg_basis B, const double* L, int Lh, const double* R, int Rh, double* RES,
+                               int RESh, double* w1, double* w2, imsrg_stream stream) {
+  PREP();
+  t_pphh(B, L, Lh, R, Rh, RES, RESh, w1, w2, s);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int imsrg_commutator_221(imsrg_basis B, const double* L, int Lh, const double* R, int Rh, double* RES, int RESh,
+                         const double* w1, const double* w2, imsrg_stream stream) {
+  PREP();
+  (void)L; (void)R;
+  t_221(B, Lh, Rh, RES, RESh, w1, w2, s);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int imsrg_calculate_cross_coupled(imsrg_basis B, const double* op, int herm, double* CC, imsrg_stream stream) {
+  PREP();
+  t_pandya(B, op, herm, CC, s);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int imsrg_commutator_222_ph(imsrg_basis B, const double* LCC, int Lh, const double* RCC, int Rh, double* RES,
+                            int RESh, double* WCC, imsrg_stream stream) {
+  PREP();
+  t_ph(B, LCC, Lh, RCC, Rh, RES, RESh, WCC, s);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int imsrg_add_sq_op(imsrg_basis B, const double* A, double ax, const double* Bv, double bx, double* C,
+                    imsrg_stream stream) {
+  PREP();
+  launch_axpy(B->b.oplen, A, ax, Bv, bx, C, s);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int imsrg_mat_frob_norm(imsrg_basis B, const double* op, double* value, imsrg_stream stream) {
+  PREP();
+  t_norm(B, op, B->dval, s);
+  HIPCHK(hipGetLastError());
+  return read_scalar(B, B->dval, value, s);
+}
+
+int imsrg_build_gs_white(imsrg_basis B, const double* H, double* ETA, imsrg_stream stream) {
+  PREP();
+  return white(B, H, ETA, s);
+}
+
+int imsrg_mbpt2(imsrg_basis B, const double* H, double* value, imsrg_stream stream) {
+  PREP();
+  int rc = mbpt2_dev(B, H, B->dval, s);
+  if (rc) return rc;
+  return read_scalar(B, B->dval, value, s);
+}
+
+int imsrg_normal_order(imsrg_basis B, double* H, imsrg_stream stream) {
+  PREP();
+  launch_normal_order(B->D, H, B->dval, s);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int imsrg_flow_create(imsrg_basis B, imsrg_flow* out) {
+  if (!B || !out) return fail(IMSRG_EINVAL, "null argument");
+  int rc = ensure_dev(B);
+  if (rc) return rc;
+  imsrg_flow_s* F = new imsrg_flow_s();
+  F->B = B;
+  const Basis& b = B->b;
+  if ((rc = falloc(F, b.oplen, &F->buf[0])) || (rc = falloc(F, b.oplen, &F->buf[1])) ||
+      (rc = falloc(F, b.oplen, &F->w1)) || (rc = falloc(F, b.oplen, &F->w2)) ||
+      (rc = falloc(F, b.cclen, &F->GCC)) || (rc = falloc(F, b.cclen, &F->ADCC)) ||
+      (rc = falloc(F, b.wclen, &F->WCC)) || (rc = falloc(F, b.oplen, &F->Hbare))) {
+    delete F;
+    return rc;
+  }
+  *out = F;
+  return 0;
+}
+
+int imsrg_flow_destroy(imsrg_flow F) {
+  delete F;
+  return 0;
+}
+
+int imsrg_flow_set_graphs(imsrg_flow F, int on) {
+  if (!F) return fail(IMSRG_EINVAL, "null flow");
+  F->graphs = (on != 0);
+  return 0;
+}
+
+int imsrg_bch_expand(imsrg_flow F, double* HS, const double* G, int Gh, const double* H, int Hh, double conv,
+                     int maxit, int* nterms, double* advals, imsrg_stream stream) {
+  if (!F) return fail(IMSRG_EINVAL, "null flow");
+  return bch(F, HS, G, Gh, H, Hh, conv, maxit, nterms, advals, S(stream));
+}
+
+int imsrg_magnus_expand(imsrg_flow F, double* DG, int DGh, const double* G, int Gh, double conv, int* ncomm,
+                        imsrg_stream stream) {
+  if (!F) return fail(IMSRG_EINVAL, "null flow");
+  return magnus(F, DG, DGh, G, Gh, conv, ncomm, S(stream));
+}
+
+int imsrg_euler_step(imsrg_basis B, double* G, const double* DG, double ds, imsrg_stream stream) {
+  PREP();
+  launch_axpy(B->b.oplen, G, 1.0, DG, ds, G, s);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int imsrg_magnus_step(imsrg_flow F, double* HS, double* G, double* DG, const double* H, double ds,
+                      double bch_conv, double mag_conv, double* out, imsrg_stream stream) {
+  if (!F) return fail(IMSRG_EINVAL, "null flow");
+  return magnus_step(F, HS, G, DG, H, ds, bch_conv, mag_conv, out, S(stream));
+}
+
+int imsrg_magnus_decouple(imsrg_flow F, double* HS, double* G, double ds, double bch_conv, double mag_conv,
+                          int crit_mode, double crit_tol, int maxsteps, double* log, int* steps,
+                          imsrg_stream stream) {
+  if (!F) return fail(IMSRG_EINVAL, "null flow");
+  imsrg_basis_s* B = F->B;
+  hipStream_t s = S(stream);
+  const i64 n = B->b.oplen;
+  int rc;
+  double* DG = nullptr;
+  if ((rc = falloc(F, n, &DG))) return rc;
+  launch_fill(DG, n, 0.0, s);
+  HIPCHK(hipMemcpyAsync(F->Hbare, HS, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+  if ((rc = white(B, HS, DG, s))) return rc;
+  if ((rc = mbpt2_dev(B, HS, B->dval + 2, s))) return rc;
+  HIPCHK(hipMemcpyAsync(B->dval + 3, HS, sizeof(double), hipMemcpyDeviceToDevice, s));
+  HIPCHK(hipMemcpyAsync(B->hval, B->dval + 2, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  double crit = std::fabs(B->hval[0]);
+  double Eold = B->hval[1];
+  int st = 0;
+  double s_flow = 0.0;
+  while (true) {
+    if (crit_mode == 0 && !(crit > crit_tol)) break;
+    if (st >= maxsteps) {
+      if (steps) *steps = st;
+      return fail(IMSRG_ESTEPS, "magnus_decouple: step limit reached");
+    }
+    ++st;
+    double out[4];
+    if ((rc = magnus_step(F, HS, G, DG, F->Hbare, ds, bch_conv, mag_conv, out, s))) return rc;
+    s_flow += ds;
+    crit = std::fabs(out[1]);
+    if (log) {
+      log[(st - 1) * 4 + 0] = s_flow;
+      log[(st - 1) * 4 + 1] = out[0];
+      log[(st - 1) * 4 + 2] = out[0] + out[1];
+      log[(st - 1) * 4 + 3] = out[2] + (out[3] - 1);
+    }
+    if (crit_mode == 1) {
+      if (std::fabs(out[0] - Eold) < crit_tol) break;
+      Eold = out[0];
+    }
+  }
+  if (steps) *steps = st;
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,105 @@
+// Device-side accessors shared by the kernels: one-body and two-body element
+// fetches with the reference's phase / normalisation conventions, the 6j
+// table lookup and small index helpers.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "basis.hpp"
+
+namespace imsrg {
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+// 1/sqrt(2) exactly as the reference types it (commutators.f90:834,847)
+#define IMSRG_INV_SQRT2_REF 0.70710678118
+
+__device__ __forceinline__ int m1pow(int e) { return (e & 1) ? -1 : 1; }
+__device__ __forceinline__ bool dtri(int a, int b, int c) { return c <= a + b && c >= abs(a - b); }
+__device__ __forceinline__ int row_cls(int g) { return g < 3 ? 0 : (g == 4 ? 2 : 1); }
+__device__ __forceinline__ int col_cls(int g) { return g == 0 ? 0 : (g == 1 || g == 3 ? 1 : 2); }
+__device__ __forceinline__ bool is_square(int g) { return g == 0 || g == 3 || g == 4; }
+__device__ __forceinline__ int brows(const DevBasis& B, int q, int g) { return B.dims[q * 3 + row_cls(g)]; }
+__device__ __forceinline__ int bcols(const DevBasis& B, int q, int g) { return B.dims[q * 3 + col_cls(g)]; }
+__device__ __forceinline__ void unpack(int p, int& a, int& b) { a = p & 0xffff; b = p >> 16; }
+
+// f_elem, basic_IMSRG.f90:1191-1221
+__device__ __forceinline__ double f_elem(const DevBasis& B, const double* op, int herm, int a, int c) {
+  const int c1 = B.con[a], c2 = B.con[c];
+  if (c1 + c2 == 0) return op[B.off_fpp + B.cidx[a] + (i64)B.cidx[c] * B.P];
+  if (c1 + c2 == 2) return op[B.off_fhh + B.cidx[a] + (i64)B.cidx[c] * B.A];
+  if (c1 > c2) return op[B.off_fph + B.cidx[c] + (i64)B.cidx[a] * B.P] * herm;
+  return op[B.off_fph + B.cidx[a] + (i64)B.cidx[c] * B.P];
+}
+
+// Position of the two-body element <ab|V|cd>_J: the lookup half of v_elem
+// (basic_IMSRG.f90:1314-1429).  Reading it from two operators gives the
+// reference's v_elem / v_same pair (basic_IMSRG.f90:2051-2072).
+struct VPos {
+  i64 off;      // < 0: element is zero by symmetry
+  double pre;   // phase (-1)^(1+(ja+jb-J)/2) for swapped pairs, sqrt(2) for a=b
+  int swp;      // stored as the transposed element times herm
+};
+
+__device__ __forceinline__ VPos vpos(const DevBasis& B, int a, int b, int c, int d, int J) {
+  VPos p;
+  p.off = -1; p.pre = 0.0; p.swp = 0;
+  const int ja = B.jj[a], jb = B.jj[b], jc = B.jj[c], jd = B.jj[d];
+  if (!(dtri(ja, jb, J) && dtri(jc, jd, J))) return p;
+  const int P = (B.ll[a] + B.ll[b]) & 1;
+  if (((B.ll[c] + B.ll[d]) & 1) != P) return p;
+  const int T2 = B.tz[a] + B.tz[b];
+  if (B.tz[c] + B.tz[d] != T2) return p;
+  const int q = 3 * J + 2 * (T2 / 2 + 1) + P;
+  const int C1 = B.con[a] + B.con[b], C2 = B.con[c] + B.con[d];
+  const int lo = C1 < C2 ? C1 : C2, hi = C1 < C2 ? C2 : C1;
+  // (lo,hi): (0,0) gam1, (0,1) gam2, (0,2) gam3, (1,1) gam4, (2,2) gam5, (1,2) gam6
+  int g;
+  if (lo == 0) g = hi;
+  else if (lo == 1) g = (hi == 1) ? 3 : 5;
+  else g = 4;
+  double pre = 1.0;
+  const int i1 = B.xmap[((i64)a * B.N + b) * B.nJ + (J >> 1)];
+  if (a > b) pre = (double)m1pow(1 + (ja + jb - J) / 2);
+  else if (a == b) pre = pre * 1.4142135623730951;
+  const int i2 = B.xmap[((i64)c * B.N + d) * B.nJ + (J >> 1)];
+  if (c > d) pre = pre * m1pow(1 + (jc + jd - J) / 2);
+  else if (c == d) pre = pre * 1.4142135623730951;
+  const int ld = B.dims[q * 3 + row_cls(g)];
+  if (C1 > C2) { p.off = B.goff[q * 6 + g] + i2 + (i64)i1 * ld; p.swp = 1; }
+  else { p.off = B.goff[q * 6 + g] + i1 + (i64)i2 * ld; p.swp = 0; }
+  p.pre = pre;
+  return p;
+}
+
+__device__ __forceinline__ double vat(const VPos& p, const double* op, int herm) {
+  if (p.off < 0) return 0.0;
+  double x = op[p.off];
+  if (p.swp) x = x * herm;
+  return x * p.pre;
+}
+
+__device__ __forceinline__ double v_elem(const DevBasis& B, const double* op, int herm, int a, int b, int c,
+                                         int d, int J) {
+  return vat(vpos(B, a, b, c, d, J), op, herm);
+}
+
+// sixj, basic_IMSRG.f90:2553-2599: j1,j2,j4,j5 half-integer; j3,j6 integer
+__device__ __forceinline__ double sixj(const DevBasis& B, int j1, int j2, int j3, int j4, int j5, int j6) {
+  if (!(dtri(j1, j2, j3) && dtri(j4, j5, j3) && dtri(j1, j5, j6) && dtri(j4, j2, j6))) return 0.0;
+  const int nh = B.nh6;
+  const i64 idx = (((((i64)(j3 >> 1) * nh + (j1 >> 1)) * nh + (j2 >> 1)) * nh + (j4 >> 1)) * nh + (j5 >> 1)) *
+                      B.nJ6 + (j6 >> 1);
+  return B.six[idx];
+}
+
+// (row, col) of element t of the upper triangle (row <= col) of an n x n
+// block, enumerated column by column.
+__device__ __forceinline__ void upper_decode(i64 t, int& row, int& col) {
+  int c = (int)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+  while ((i64)c * (c + 1) / 2 > t) --c;
+  while ((i64)(c + 1) * (c + 2) / 2 <= t) ++c;
+  col = c;
+  row = (int)(t - (i64)c * (c + 1) / 2);
+}
+
+}  // namespace imsrg

@@ -1,0 +1,939 @@
+// HIP kernels of the IMSRG(2) scalar-commutator path for gfx950 (CDNA4).
+//
+//  * k_comm122, k_comm222ph_scatter, k_pandya: per-element gathers, one
+//    thread per output element, 256-element tiles from a task list that
+//    spans every channel (grouped) or one channel (plain path).
+//  * k_gemm, k_pppp_pairs: FP64 MFMA (v_mfma_f64_16x16x4f64) tiles staged
+//    through LDS.  k_pppp_pairs computes the tile pair (I,J)/(J,I) of the
+//    pppp product so the reference's  W - pm W^T  epilogue of 222_pp_hh is
+//    applied without a second pass over HBM.
+//  * one-body terms, norms, axpy, generator, mbpt2 and normal ordering.
+#include <hip/hip_runtime.h>
+
+#include "device.hpp"
+#include "kernels.hpp"
+
+namespace imsrg {
+
+// ============================================================ commutator_122
+// commutators.f90:283-396.  RES two-body = 1b x 2b part (assigned).
+__global__ __launch_bounds__(256) void k_comm122(DevBasis B, const SegTask* tasks, const double* L, int Lh,
+                                                 const double* R, int Rh, double* RES, int RESh) {
+  const SegTask t = tasks[blockIdx.x];
+  const i64 e = (i64)t.start + threadIdx.x;
+  if (e >= t.cnt) return;
+  const int q = t.q, g = t.g;
+  const int n1 = brows(B, q, g);
+  const bool sq = is_square(g);
+  int IX, JX;
+  if (sq) upper_decode(e, IX, JX);
+  else { IX = (int)(e % n1); JX = (int)(e / n1); }
+  int a, b, c, d;
+  unpack(B.pairs[B.pairoff[q * 3 + row_cls(g)] + IX], a, b);
+  unpack(B.pairs[B.pairoff[q * 3 + col_cls(g)] + JX], c, d);
+  const int JT = B.chJ[q];
+  double sm = 0.0;
+  {
+    const int k = B.spblk[a];
+    for (int x = B.spstart[k]; x < B.spstart[k + 1]; ++x) {
+      const int i = B.spmem[x];
+      const VPos p = vpos(B, i, b, c, d, JT);
+      sm = sm + f_elem(B, L, Lh, a, i) * vat(p, R, Rh) - f_elem(B, R, Rh, a, i) * vat(p, L, Lh);
+    }
+  }
+  {
+    const int k = B.spblk[b];
+    for (int x = B.spstart[k]; x < B.spstart[k + 1]; ++x) {
+      const int i = B.spmem[x];
+      const VPos p = vpos(B, a, i, c, d, JT);
+      sm = sm + f_elem(B, L, Lh, b, i) * vat(p, R, Rh) - f_elem(B, R, Rh, b, i) * vat(p, L, Lh);
+    }
+  }
+  {
+    const int k = B.spblk[c];
+    for (int x = B.spstart[k]; x < B.spstart[k + 1]; ++x) {
+      const int i = B.spmem[x];
+      const VPos p = vpos(B, a, b, i, d, JT);
+      sm = sm - f_elem(B, L, Lh, i, c) * vat(p, R, Rh) + f_elem(B, R, Rh, i, c) * vat(p, L, Lh);
+    }
+  }
+  {
+    const int k = B.spblk[d];
+    for (int x = B.spstart[k]; x < B.spstart[k + 1]; ++x) {
+      const int i = B.spmem[x];
+      const VPos p = vpos(B, a, b, c, i, JT);
+      sm = sm - f_elem(B, L, Lh, i, d) * vat(p, R, Rh) + f_elem(B, R, Rh, i, d) * vat(p, L, Lh);
+    }
+  }
+  sm = sm / sqrt(1.0 + (a == b ? 1.0 : 0.0)) / sqrt(1.0 + (c == d ? 1.0 : 0.0));
+  const i64 base = B.goff[q * 6 + g];
+  RES[base + IX + (i64)JX * n1] = sm;
+  if (sq) RES[base + JX + (i64)IX * n1] = sm * RESh;
+}
+
+void launch_comm122(const DevBasis& B, const SegTask* t, int nt, const double* L, int Lh, const double* R,
+                    int Rh, double* RES, int RESh, hipStream_t s) {
+  if (nt <= 0) return;
+  hipLaunchKernelGGL(k_comm122, dim3(nt), dim3(kTile), 0, s, B, t, L, Lh, R, Rh, RES, RESh);
+}
+
+// ================================================== commutator_222_ph scatter
+// commutators.f90:804-923.  Inverse Pandya of WCC = LCC RCC^T into RES.
+__global__ __launch_bounds__(256) void k_comm222ph_scatter(DevBasis B, const SegTask* tasks, const double* WCC,
+                                                           int Lh, int Rh, double* RES, int RESh) {
+  const SegTask t = tasks[blockIdx.x];
+  const i64 e = (i64)t.start + threadIdx.x;
+  if (e >= t.cnt) return;
+  const int q = t.q, g = t.g;
+  const int n1 = brows(B, q, g);
+  const bool sq = is_square(g);
+  int IX, JX;
+  if (sq) upper_decode(e, IX, JX);
+  else { IX = (int)(e % n1); JX = (int)(e / n1); }
+  int i, j, k, l;
+  unpack(B.pairs[B.pairoff[q * 3 + row_cls(g)] + IX], i, j);
+  unpack(B.pairs[B.pairoff[q * 3 + col_cls(g)] + JX], k, l);
+  const double pre = (i == j) ? IMSRG_INV_SQRT2_REF : 1.0;
+  const double pre2 = (k == l) ? IMSRG_INV_SQRT2_REF : 1.0;
+  const int Jtot = B.chJ[q];
+  const int ji = B.jj[i], jj = B.jj[j], jk = B.jj[k], jl = B.jj[l];
+  const int li = B.ll[i], lj = B.ll[j], lk = B.ll[k], ll = B.ll[l];
+  const int ti = B.tz[i], tj = B.tz[j], tk = B.tz[k], tl = B.tz[l];
+  const int N = B.N, nJ = B.nJ, JTM = B.JTM;
+  double sm = 0.0;
+  // "inside" recoupling
+  int Tz = abs(ti - tk) / 2;
+  if (abs(tl - tj) != Tz * 2) return;
+  int PAR = (li + lk) % 2;
+  if ((ll + lj) % 2 != PAR) return;
+  int jmin = max(abs(jj - jl), abs(ji - jk));
+  int jmax = min(jj + jl, ji + jk);
+  int phase = m1pow((ji + jj + jk + jl) / 2);
+  for (int JP = jmin; JP <= jmax; JP += 2) {
+    const int qx = JP / 2 + Tz * (JTM + 1) + 2 * PAR * (JTM + 1);
+    const double* W = WCC + B.wcoff[qx];
+    const i64 rr = B.rlen[qx];
+    const int jo = JP >> 1;
+    const int rjl = B.rmap[((i64)j * N + l) * nJ + jo], rik = B.rmap[((i64)i * N + k) * nJ + jo];
+    const int gjl = B.rmap[((i64)l * N + j) * nJ + jo], gik = B.rmap[((i64)k * N + i) * nJ + jo];
+    sm = sm - m1pow((jl + jk + Jtot) / 2) *
+                  ((W[gik + rjl * rr] + phase * W[gjl + rik * rr]) * Lh -
+                   (phase * W[rik + gjl * rr] + W[rjl + gik * rr]) * Rh) *
+                  sixj(B, ji, jj, Jtot, jl, jk, JP);
+  }
+  // "outside" recoupling
+  Tz = abs(ti - tl) / 2;
+  if (abs(tk - tj) != Tz * 2) return;
+  PAR = (li + ll) % 2;
+  if ((lk + lj) % 2 != PAR) return;
+  jmin = max(abs(ji - jl), abs(jj - jk));
+  jmax = min(ji + jl, jj + jk);
+  phase = m1pow((ji + jj + jk + jl) / 2);
+  for (int JP = jmin; JP <= jmax; JP += 2) {
+    const int qx = JP / 2 + Tz * (JTM + 1) + 2 * PAR * (JTM + 1);
+    const double* W = WCC + B.wcoff[qx];
+    const i64 rr = B.rlen[qx];
+    const int jo = JP >> 1;
+    const int ril = B.rmap[((i64)i * N + l) * nJ + jo], rjk = B.rmap[((i64)j * N + k) * nJ + jo];
+    const int gil = B.rmap[((i64)l * N + i) * nJ + jo], gjk = B.rmap[((i64)k * N + j) * nJ + jo];
+    sm = sm + ((W[gil + rjk * rr] + phase * W[gjk + ril * rr]) * Lh -
+               (phase * W[ril + gjk * rr] + W[rjk + gil * rr]) * Rh) *
+                  sixj(B, ji, jj, Jtot, jk, jl, JP);
+  }
+  double* X = RES + B.goff[q * 6 + g];
+  const i64 o = IX + (i64)JX * n1;
+  X[o] = X[o] + sm * pre * pre2;
+  if (sq) X[JX + (i64)IX * n1] = X[o] * RESh;
+}
+
+void launch_comm222ph_scatter(const DevBasis& B, const SegTask* t, int nt, const double* W, int Lh, int Rh,
+                              double* RES, int RESh, hipStream_t s) {
+  if (nt <= 0) return;
+  hipLaunchKernelGGL(k_comm222ph_scatter, dim3(nt), dim3(kTile), 0, s, B, t, W, Lh, Rh, RES, RESh);
+}
+
+// =========================================================== Pandya forward
+// calculate_cross_coupled, cross_coupled.f90:1190-1300.  One thread per
+// CCX(q1)(r, nb) element; r = ordered pair (a,b), nb = (p,h).
+__global__ __launch_bounds__(256) void k_pandya(DevBasis B, const CCTask* tasks, const double* op, int herm,
+                                                double* CC) {
+  const CCTask t = tasks[blockIdx.x];
+  const int q1 = t.q1;
+  const int r = B.rlen[q1], nc = B.ncol[q1];
+  const i64 e = (i64)t.start + threadIdx.x;
+  if (e >= (i64)r * nc) return;
+  const int R = (int)(e % r), NB = (int)(e / r);
+  int a, b, p, h;
+  unpack(B.ccrows[B.ccrowoff[q1] + R], a, b);
+  unpack(B.cccols[B.cccoloff[q1] + NB], p, h);
+  const int JC = B.ccJ[q1];
+  const int ja = B.jj[a], jb = B.jj[b], jp = B.jj[p], jh = B.jj[h];
+  const int la = B.ll[a], lb = B.ll[b], lp = B.ll[p], lh = B.ll[h];
+  const int ta = B.tz[a], tb = B.tz[b], tp = B.tz[p], th = B.tz[h];
+  double val = 0.0;
+  if ((((la + lh) & 1) == ((lb + lp) & 1)) && (ta + th == tb + tp)) {
+    const int Jmin = max(abs(jp - jb), abs(ja - jh));
+    const int Jmax = min(jp + jb, ja + jh);
+    double sm = 0.0;
+    for (int JT = Jmin; JT <= Jmax; JT += 2)
+      sm = sm + (double)(m1pow(JT / 2) * (JT + 1)) * sixj(B, jp, jh, JC, ja, jb, JT) *
+                    v_elem(B, op, herm, h, a, p, b, JT);
+    val = sm * m1pow((jh + jb + JC) / 2) * 1.0 * sqrt(JC + 1.0);
+  }
+  CC[B.ccoff[q1] + R + (i64)NB * r] = val;
+}
+
+void launch_pandya(const DevBasis& B, const CCTask* t, int nt, const double* op, int herm, double* CC,
+                   hipStream_t s) {
+  if (nt <= 0) return;
+  hipLaunchKernelGGL(k_pandya, dim3(nt), dim3(kTile), 0, s, B, t, op, herm, CC);
+}
+
+// ============================================================ commutator_111
+// commutators.f90:36-80.  One thread per one-body output; assigns.
+__global__ __launch_bounds__(256) void k_comm111(DevBasis B, const double* L, int Lh, const double* R, int Rh,
+                                                 double* RES) {
+  const int A = B.A, P = B.P;
+  const i64 idx = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  const double *Lhh = L + B.off_fhh, *Lph = L + B.off_fph, *Lpp = L + B.off_fpp;
+  const double *Rhh = R + B.off_fhh, *Rph = R + B.off_fph, *Rpp = R + B.off_fpp;
+  if (idx < (i64)A * A) {
+    const int i = (int)(idx % A), j = (int)(idx / A);
+    double th1ij = 0, th1ji = 0, th2ij = 0, th2ji = 0;
+    for (int k = 0; k < A; ++k) {
+      th1ij += Lhh[i + k * A] * Rhh[k + j * A];
+      th1ji += Lhh[j + k * A] * Rhh[k + i * A];
+    }
+    for (int k = 0; k < P; ++k) {
+      th2ij += Lph[k + (i64)i * P] * Rph[k + (i64)j * P];
+      th2ji += Lph[k + (i64)j * P] * Rph[k + (i64)i * P];
+    }
+    RES[B.off_fhh + idx] = th1ij + th2ij * Lh - (th1ji * Lh + th2ji) * Rh;
+    return;
+  }
+  if (idx < (i64)A * A + (i64)P * A) {
+    const i64 x = idx - (i64)A * A;
+    const int a = (int)(x % P), i = (int)(x / P);
+    double tb1 = 0, tb2 = 0, tc1 = 0, tc2 = 0;
+    for (int k = 0; k < A; ++k) {
+      tb1 += Lph[a + (i64)k * P] * Rhh[k + i * A];
+      tc1 += Rph[a + (i64)k * P] * Lhh[k + i * A];
+    }
+    for (int k = 0; k < P; ++k) {
+      tb2 += Lpp[a + (i64)k * P] * Rph[k + (i64)i * P];
+      tc2 += Rpp[a + (i64)k * P] * Lph[k + (i64)i * P];
+    }
+    RES[B.off_fph + x] = (tb1 + tb2) - tc1 - tc2;
+    return;
+  }
+  if (idx < (i64)A * A + (i64)P * A + (i64)P * P) {
+    const i64 x = idx - (i64)A * A - (i64)P * A;
+    const int a = (int)(x % P), b = (int)(x / P);
+    double tp1ab = 0, tp1ba = 0, tp2ab = 0, tp2ba = 0;
+    for (int k = 0; k < A; ++k) {
+      tp1ab += Lph[a + (i64)k * P] * Rph[b + (i64)k * P];
+      tp1ba += Lph[b + (i64)k * P] * Rph[a + (i64)k * P];
+    }
+    for (int k = 0; k < P; ++k) {
+      tp2ab += Lpp[a + (i64)k * P] * Rpp[k + (i64)b * P];
+      tp2ba += Lpp[b + (i64)k * P] * Rpp[k + (i64)a * P];
+    }
+    RES[B.off_fpp + x] = tp1ab * Rh + tp2ab - (tp1ba + tp2ba * Rh) * Lh;
+  }
+}
+
+void launch_comm111(const DevBasis& B, const double* L, int Lh, const double* R, int Rh, double* RES,
+                    hipStream_t s) {
+  const i64 n = (i64)B.A * B.A + (i64)B.P * B.A + (i64)B.P * B.P;
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_comm111, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, B, L, Lh, R, Rh, RES);
+}
+
+// ============================================================ commutator_121
+// commutators.f90:83-280 (accumulates).
+__global__ __launch_bounds__(64) void k_comm121(DevBasis B, const OneTask* tasks, int nt, const double* L,
+                                                int Lh, const double* R, int Rh, double* RES, int RESh) {
+  const int tix = blockIdx.x * blockDim.x + threadIdx.x;
+  if (tix >= nt) return;
+  const OneTask t = tasks[tix];
+  const int pk = t.pk, qk = t.qk, jq = B.jj[qk];
+  const double* Lph = L + B.off_fph;
+  const double* Rph = R + B.off_fph;
+  double sm = 0.0;
+  for (int a = 0; a < B.P; ++a) {
+    const int ak = B.parts[a];
+    for (int i = 0; i < B.A; ++i) {
+      const int ik = B.holes[i];
+      if (B.jj[ik] != B.jj[ak] || B.ll[ik] != B.ll[ak] || B.tz[ik] != B.tz[ak]) continue;
+      const int ji = B.jj[ik];
+      double smx = 0, smy = 0, smx2 = 0, smy2 = 0;
+      for (int JT = abs(ji - jq); JT <= ji + jq; JT += 2) {
+        const VPos p1 = vpos(B, ak, pk, ik, qk, JT);
+        smx = smx + vat(p1, R, Rh) * (JT + 1);
+        smx2 = smx2 + vat(p1, L, Lh) * (JT + 1);
+        const VPos p2 = vpos(B, ik, pk, ak, qk, JT);
+        smy = smy + vat(p2, R, Rh) * (JT + 1);
+        smy2 = smy2 + vat(p2, L, Lh) * (JT + 1);
+      }
+      sm = sm + Lph[a + (i64)i * B.P] * (Lh * smx - smy) - Rph[a + (i64)i * B.P] * (Rh * smx2 - smy2);
+    }
+  }
+  const double add = sm / (B.jj[pk] + 1.0);
+  if (t.kind == 0) {
+    double* F = RES + B.off_fhh;
+    const i64 o = t.pi + (i64)t.qi * B.A;
+    F[o] = F[o] + add;
+    F[t.qi + (i64)t.pi * B.A] = F[o] * RESh;
+  } else if (t.kind == 1) {
+    double* F = RES + B.off_fpp;
+    const i64 o = t.pi + (i64)t.qi * B.P;
+    F[o] = F[o] + add;
+    F[t.qi + (i64)t.pi * B.P] = F[o] * RESh;
+  } else {
+    double* F = RES + B.off_fph;
+    const i64 o = t.pi + (i64)t.qi * B.P;
+    F[o] = F[o] + add;
+  }
+}
+
+void launch_comm121(const DevBasis& B, const OneTask* t, int nt, const double* L, int Lh, const double* R,
+                    int Rh, double* RES, int RESh, hipStream_t s) {
+  if (nt <= 0) return;
+  hipLaunchKernelGGL(k_comm121, dim3((nt + 63) / 64), dim3(64), 0, s, B, t, nt, L, Lh, R, Rh, RES, RESh);
+}
+
+// ============================================================ commutator_221
+// commutators.f90:435-578 (accumulates; reads w1/w2 of 222_pp_hh, herm 1).
+__global__ __launch_bounds__(64) void k_comm221(DevBasis B, const OneTask* tasks, int nt, int pm,
+                                                const double* w1, const double* w2, double* RES, int RESh) {
+  const int tix = blockIdx.x * blockDim.x + threadIdx.x;
+  if (tix >= nt) return;
+  const OneTask t = tasks[tix];
+  const int ik = t.pk, jk = t.qk, ji = B.jj[ik];
+  const bool both = (t.kind != 2);
+  double sm = 0.0;
+  for (int pass = 0; pass < 2; ++pass) {
+    const double* w = pass == 0 ? w1 : w2;   // holes c with w1, then particles c with w2
+    const int nc = pass == 0 ? B.A : B.P;
+    const int* cl = pass == 0 ? B.holes : B.parts;
+    for (int c = 0; c < nc; ++c) {
+      const int ck = cl[c];
+      const int jc = B.jj[ck];
+      for (int JT = abs(jc - ji); JT <= jc + ji; JT += 2) {
+        if (both)
+          sm = sm + (v_elem(B, w, 1, ck, ik, ck, jk, JT) - pm * v_elem(B, w, 1, ck, jk, ck, ik, JT)) * (JT + 1);
+        else
+          sm = sm + v_elem(B, w, 1, ck, ik, ck, jk, JT) * (JT + 1);
+      }
+    }
+  }
+  const double add = sm / (ji + 1.0);
+  if (t.kind == 0) {
+    double* F = RES + B.off_fhh;
+    const i64 o = t.pi + (i64)t.qi * B.A;
+    F[o] = F[o] + add;
+    F[t.qi + (i64)t.pi * B.A] = F[o] * RESh;
+  } else if (t.kind == 1) {
+    double* F = RES + B.off_fpp;
+    const i64 o = t.pi + (i64)t.qi * B.P;
+    F[o] = F[o] + add;
+    F[t.qi + (i64)t.pi * B.P] = F[o] * RESh;
+  } else {
+    double* F = RES + B.off_fph;
+    const i64 o = t.pi + (i64)t.qi * B.P;
+    F[o] = F[o] + add;
+  }
+}
+
+void launch_comm221(const DevBasis& B, const OneTask* t, int nt, int pm, const double* w1, const double* w2,
+                    double* RES, int RESh, hipStream_t s) {
+  if (nt <= 0) return;
+  hipLaunchKernelGGL(k_comm221, dim3((nt + 63) / 64), dim3(64), 0, s, B, t, nt, pm, w1, w2, RES, RESh);
+}
+
+// ================================================= commutator_110 + 220 (E0)
+// commutators.f90:10-33 and 399-432.  Workgroup q < nblocks: channel q's
+// 220 sum times (2J+1); workgroup nblocks: the 110 sum.
+__device__ double block_sum(double v, double* red) {
+  const int tid = threadIdx.x;
+  red[tid] = v;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (tid < s) red[tid] += red[tid + s];
+    __syncthreads();
+  }
+  const double r = red[0];
+  __syncthreads();
+  return r;
+}
+
+__global__ __launch_bounds__(256) void k_e0_partials(DevBasis B, const double* L, int Lh, const double* R, int Rh,
+                                                     double* part) {
+  __shared__ double red[256];
+  const int q = blockIdx.x;
+  double v = 0.0;
+  if (q < B.nblocks) {
+    const int np = B.dims[q * 3], nh = B.dims[q * 3 + 2];
+    const double* L3 = L + B.goff[q * 6 + 2];
+    const double* R3 = R + B.goff[q * 6 + 2];
+    for (i64 x = threadIdx.x; x < (i64)np * nh; x += blockDim.x) v = v - L3[x] * R3[x] * Rh;
+    const double s = block_sum(v, red);
+    if (threadIdx.x == 0) part[q] = s * (B.chJ[q] + 1);
+  } else {
+    const double* Lph = L + B.off_fph;
+    const double* Rph = R + B.off_fph;
+    for (i64 x = threadIdx.x; x < (i64)B.P * B.A; x += blockDim.x) {
+      const int hk = B.holes[(int)(x / B.P)];
+      v = v + (Lph[x] * Rph[x] * Lh - Rph[x] * Lph[x] * Rh) * (B.jj[hk] + 1);
+    }
+    const double s = block_sum(v, red);
+    if (threadIdx.x == 0) part[B.nblocks] = s;
+  }
+}
+
+// dst = [use110] part[n] + [use220] 2 * sum(part[0..n-1])   (one workgroup)
+__global__ __launch_bounds__(256) void k_e0_final(const double* part, int n, int use220, int use110, double* dst,
+                                                  int accumulate) {
+  __shared__ double red[256];
+  double v = 0.0;
+  for (int x = threadIdx.x; x < n; x += blockDim.x) v += part[x];
+  const double s = block_sum(v, red);
+  if (threadIdx.x == 0) {
+    const double e110 = use110 ? part[n] : 0.0;
+    const double e220 = use220 ? s * 2.0 : 0.0;
+    const double e = e110 + e220;
+    dst[0] = accumulate ? dst[0] + e : e;
+  }
+}
+
+void launch_e0_partials(const DevBasis& B, const double* L, int Lh, const double* R, int Rh, double* part,
+                        hipStream_t s) {
+  hipLaunchKernelGGL(k_e0_partials, dim3(B.nblocks + 1), dim3(256), 0, s, B, L, Lh, R, Rh, part);
+}
+
+void launch_e0_final(const double* part, int nblocks, int use220, int use110, double* dst, int accumulate,
+                     hipStream_t s) {
+  hipLaunchKernelGGL(k_e0_final, dim3(1), dim3(256), 0, s, part, nblocks, use220, use110, dst, accumulate);
+}
+
+// ========================================================= FP64 MFMA GEMM
+// 64 x 64 output tile per 256-thread workgroup (4 waves, 2 x 2 of 32 x 32),
+// K staged through LDS in steps of 16.  v_mfma_f64_16x16x4f64 operand
+// layout on gfx950 (measured with a one-wave probe): A lane l ->
+// A[l%16][l/16], B lane l -> B[l/16][l%16], D lane l, element r ->
+// D[l/16 + 4r][l%16].
+constexpr int BM = 64, BN = 64, BK = 16;
+
+__device__ __forceinline__ void load_a(double (*As)[BM + 1], const double* A, int lda, int tA, int M, int K, int m0,
+                                       int k0, double scale) {
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int r = 0; r < (BM * BK) / 256; ++r) {
+    const int e = tid + 256 * r;
+    int i, kk;
+    if (!tA) { i = e % BM; kk = e / BM; }
+    else { kk = e % BK; i = e / BK; }
+    const int gi = m0 + i, gk = k0 + kk;
+    double v = 0.0;
+    if (gi < M && gk < K) v = tA ? A[gk + (i64)gi * lda] : A[gi + (i64)gk * lda];
+    As[kk][i] = v * scale;
+  }
+}
+
+__device__ __forceinline__ void load_b(double (*Bs)[BN + 1], const double* Bm, int ldb, int tB, int N, int K, int n0,
+                                       int k0) {
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int r = 0; r < (BN * BK) / 256; ++r) {
+    const int e = tid + 256 * r;
+    int j, kk;
+    if (!tB) { kk = e % BK; j = e / BK; }
+    else { j = e % BN; kk = e / BN; }
+    const int gj = n0 + j, gk = k0 + kk;
+    double v = 0.0;
+    if (gj < N && gk < K) v = tB ? Bm[gj + (i64)gk * ldb] : Bm[gk + (i64)gj * ldb];
+    Bs[kk][j] = v;
+  }
+}
+
+// acc[2][2] += op(A)[m0:m0+64, :] * op(B)[:, n0:n0+64]
+__device__ __forceinline__ void tile_mma(v4d (&acc)[2][2], double (*As)[BM + 1], double (*Bs)[BN + 1],
+                                         const double* A, int lda, int tA, const double* Bm, int ldb, int tB, int M,
+                                         int N, int K, int m0, int n0, double scale) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int lr = lane & 15, lk = lane >> 4;
+  for (int k0 = 0; k0 < K; k0 += BK) {
+    load_a(As, A, lda, tA, M, K, m0, k0, scale);
+    load_b(Bs, Bm, ldb, tB, N, K, n0, k0);
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 4) {
+      const double a0 = As[kk + lk][wm * 32 + lr];
+      const double a1 = As[kk + lk][wm * 32 + 16 + lr];
+      const double b0 = Bs[kk + lk][wn * 32 + lr];
+      const double b1 = Bs[kk + lk][wn * 32 + 16 + lr];
+      acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+      acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+      acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+      acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(256) void k_gemm(Slots sl, const GemmProb* probs, const GemmTask* tasks) {
+  __shared__ double As[BK][BM + 1];
+  __shared__ double Bs[BK][BN + 1];
+  const GemmTask t = tasks[blockIdx.x];
+  const GemmProb pr = probs[t.prob];
+  const int m0 = t.tm * BM, n0 = t.tn * BN;
+  v4d acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) acc[a][b] = v4d{0.0, 0.0, 0.0, 0.0};
+  tile_mma(acc, As, Bs, sl.p[pr.sa1] + pr.a1, pr.lda, pr.ta1, sl.p[pr.sb1] + pr.b1, pr.ldb, pr.tb1, pr.m, pr.n,
+           pr.k, m0, n0, 1.0);
+  if (pr.nprod == 2)
+    tile_mma(acc, As, Bs, sl.p[pr.sa2] + pr.a2, pr.lda, pr.ta2, sl.p[pr.sb2] + pr.b2, pr.ldb, pr.tb2, pr.m, pr.n,
+             pr.k, m0, n0, pr.s2);
+  double* C = const_cast<double*>(sl.p[pr.sc]) + pr.c;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+#pragma unroll
+  for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+    for (int ni = 0; ni < 2; ++ni)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = m0 + wm * 32 + mi * 16 + (lane >> 4) + 4 * r;
+        const int col = n0 + wn * 32 + ni * 16 + (lane & 15);
+        if (row < pr.m && col < pr.n) {
+          const i64 o = row + (i64)col * pr.ldc;
+          const double v = pr.alpha * acc[mi][ni][r];
+          C[o] = (pr.beta == 0.0) ? v : v + pr.beta * C[o];
+        }
+      }
+}
+
+void launch_gemm(const Slots& sl, const GemmProb* probs, const GemmTask* tasks, int nt, hipStream_t s) {
+  if (nt <= 0) return;
+  hipLaunchKernelGGL(k_gemm, dim3(nt), dim3(256), 0, s, sl, probs, tasks);
+}
+
+// ======================================== 222_pp_hh pppp block (tile pairs)
+// commutators.f90:611-668: W1 = L1 R1, W2 = Rh L3 R3^T and
+// RES1 += W1 - W2 - (W1 - W2)^T pm.  The workgroup computes the raw
+// product of tile (I,J) into LDS and of tile (J,I) into registers, so both
+// RES tiles are finished in one pass.
+__device__ __forceinline__ double w2pp(const double* L3, const double* R3, int np, int nh, int Rh, int x, int y) {
+  double s = 0.0;
+  for (int h = 0; h < nh; ++h) s += L3[x + (i64)h * np] * R3[y + (i64)h * np];
+  return s * Rh;
+}
+
+__global__ __launch_bounds__(256) void k_pppp_pairs(DevBasis B, const PairTask* tasks, const double* L, int Lh,
+                                                    const double* R, int Rh, double* RES, double* w1, double* w2,
+                                                    int write_w1) {
+  __shared__ double As[BK][BM + 1];
+  __shared__ double Bs[BK][BN + 1];
+  __shared__ double S1[BM][BN + 1];
+  const PairTask t = tasks[blockIdx.x];
+  const int q = t.q;
+  const int np = B.dims[q * 3], nh = B.dims[q * 3 + 2];
+  const i64 o1 = B.goff[q * 6 + 0], o3 = B.goff[q * 6 + 2];
+  const double *L1 = L + o1, *R1 = R + o1, *L3 = L + o3, *R3 = R + o3;
+  const int pm = Lh * Rh;
+  const int I0 = t.ti * BM, J0 = t.tj * BN;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  v4d acc[2][2];
+  // raw product of tile (I, J) -> S1[x][y]
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) acc[a][b] = v4d{0.0, 0.0, 0.0, 0.0};
+  tile_mma(acc, As, Bs, L1, np, 0, R1, np, 0, np, np, np, I0, J0, 1.0);
+#pragma unroll
+  for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+    for (int ni = 0; ni < 2; ++ni)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        S1[wm * 32 + mi * 16 + (lane >> 4) + 4 * r][wn * 32 + ni * 16 + (lane & 15)] = acc[mi][ni][r];
+  __syncthreads();
+  const bool diag = (t.ti == t.tj);
+  if (!diag) {
+    // raw product of tile (J, I) in registers
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b) acc[a][b] = v4d{0.0, 0.0, 0.0, 0.0};
+    tile_mma(acc, As, Bs, L1, np, 0, R1, np, 0, np, np, np, J0, I0, 1.0);
+  }
+  double* X1 = RES + o1;
+  double* W11 = w1 + o1;
+  double* W21 = w2 + o1;
+#pragma unroll
+  for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+    for (int ni = 0; ni < 2; ++ni)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        // register element: row y of the (J,I) tile, column x (for a
+        // diagonal tile: row y, column x of the (I,I) tile)
+        const int y = wm * 32 + mi * 16 + (lane >> 4) + 4 * r;
+        const int x = wn * 32 + ni * 16 + (lane & 15);
+        const int gy = J0 + y, gx = I0 + x;
+        if (gy >= np || gx >= np) continue;
+        const double w1yx = diag ? S1[y][x] : acc[mi][ni][r];
+        const double w1xy = S1[x][y];
+        const double w2yx = w2pp(L3, R3, np, nh, Rh, gy, gx);
+        const double w2xy = w2pp(L3, R3, np, nh, Rh, gx, gy);
+        const i64 oyx = gy + (i64)gx * np;
+        X1[oyx] = X1[oyx] + w1yx - w2yx - (w1xy - w2xy) * pm;
+        if (write_w1) W11[oyx] = w1yx;
+        W21[oyx] = w2yx;
+        if (!diag) {
+          const i64 oxy = gx + (i64)gy * np;
+          X1[oxy] = X1[oxy] + w1xy - w2xy - (w1yx - w2yx) * pm;
+          if (write_w1) W11[oxy] = w1xy;
+          W21[oxy] = w2xy;
+        }
+      }
+}
+
+void launch_pppp_pairs(const DevBasis& B, const PairTask* t, int nt, const double* L, int Lh, const double* R,
+                       int Rh, double* RES, double* w1, double* w2, int write_w1, hipStream_t s) {
+  if (nt <= 0) return;
+  hipLaunchKernelGGL(k_pppp_pairs, dim3(nt), dim3(256), 0, s, B, t, L, Lh, R, Rh, RES, w1, w2, write_w1);
+}
+
+// ================================ 222_pp_hh epilogue of the small blocks
+// commutators.f90:636-764 after the grouped GEMMs:
+//   g == 1: region gam2|gam3 (np x (nb+nh)); w1 already holds
+//           L1 [R2|R3] - R1 [L2|L3]; adds the rank-nh parts W2 and RES.
+//   g == 3 / 4 / 5: gam4 / gam5 / gam6 from P = [L2|L3]^T [R2|R3].
+__global__ __launch_bounds__(256) void k_pphh_epilogue(DevBasis B, const SegTask* tasks, const double* L, int Lh,
+                                                       const double* R, int Rh, double* RES, double* w1,
+                                                       double* w2, const double* Pscr, const i64* poff) {
+  const SegTask t = tasks[blockIdx.x];
+  const i64 e = (i64)t.start + threadIdx.x;
+  if (e >= t.cnt) return;
+  const int q = t.q;
+  const int np = B.dims[q * 3], nb = B.dims[q * 3 + 1], nh = B.dims[q * 3 + 2];
+  const int pm = Lh * Rh;
+  const i64* go = B.goff + q * 6;
+  const double *L3 = L + go[2], *R3 = R + go[2], *L5 = L + go[4], *R5 = R + go[4], *L6 = L + go[5],
+               *R6 = R + go[5];
+  if (t.g == 1) {
+    const int x = (int)(e % np), y = (int)(e / np);
+    double w2v;
+    if (y < nb) {
+      double s1 = 0.0, s2 = 0.0;
+      for (int h = 0; h < nh; ++h) {
+        s1 += L3[x + (i64)h * np] * R6[y + (i64)h * nb];
+        s2 += R3[x + (i64)h * np] * L6[y + (i64)h * nb];
+      }
+      w2v = Rh * s1 + (-1.0 * Lh) * s2;
+    } else {
+      const int hp = y - nb;
+      double s1 = 0.0, s2 = 0.0;
+      for (int h = 0; h < nh; ++h) {
+        s1 += L3[x + (i64)h * np] * R5[h + (i64)hp * nh];
+        s2 += R3[x + (i64)h * np] * L5[h + (i64)hp * nh];
+      }
+      w2v = s1 + (-1.0) * s2;
+    }
+    const i64 o = go[1] + e;
+    RES[o] = RES[o] + w1[o] - w2v;
+    w2[o] = w2v;
+    return;
+  }
+  const int nbh = nb + nh;
+  const double* Pq = Pscr + poff[q];
+  if (t.g == 3) {   // gam4, nb x nb
+    const int x = (int)(e % nb), y = (int)(e / nb);
+    double sxy = 0.0, syx = 0.0;
+    for (int h = 0; h < nh; ++h) {
+      sxy += L6[x + (i64)h * nb] * R6[y + (i64)h * nb];
+      syx += L6[y + (i64)h * nb] * R6[x + (i64)h * nb];
+    }
+    const double w1xy = Pq[x + (i64)y * nbh] * Lh, w1yx = Pq[y + (i64)x * nbh] * Lh;
+    const double w2xy = sxy * Rh, w2yx = syx * Rh;
+    const i64 o = go[3] + e;
+    RES[o] = RES[o] + w1xy - w2xy - (w1yx - w2yx) * pm;
+    w1[o] = w1xy;
+    w2[o] = w2xy;
+    return;
+  }
+  if (t.g == 4) {   // gam5, nh x nh
+    const int a = (int)(e % nh), b = (int)(e / nh);
+    double sab = 0.0, sba = 0.0;
+    for (int h = 0; h < nh; ++h) {
+      sab += L5[a + (i64)h * nh] * R5[h + (i64)b * nh];
+      sba += L5[b + (i64)h * nh] * R5[h + (i64)a * nh];
+    }
+    const double w1ab = Pq[(nb + a) + (i64)(nb + b) * nbh] * Lh, w1ba = Pq[(nb + b) + (i64)(nb + a) * nbh] * Lh;
+    const i64 o = go[4] + e;
+    RES[o] = RES[o] + w1ab - sab - (w1ba - sba) * pm;
+    w1[o] = w1ab;
+    w2[o] = sab;
+    return;
+  }
+  {   // gam6, nb x nh
+    const int x = (int)(e % nb), h = (int)(e / nb);
+    double s1 = 0.0, s2 = 0.0;
+    for (int hp = 0; hp < nh; ++hp) {
+      s1 += L6[x + (i64)hp * nb] * R5[hp + (i64)h * nh];
+      s2 += R6[x + (i64)hp * nb] * L5[hp + (i64)h * nh];
+    }
+    const double w1v = Lh * Pq[x + (i64)(nb + h) * nbh] + (-1.0 * Rh) * Pq[(nb + h) + (i64)x * nbh];
+    const double w2v = s1 + (-1.0) * s2;
+    const i64 o = go[5] + e;
+    RES[o] = RES[o] + w1v - w2v;
+    w1[o] = w1v;
+    w2[o] = w2v;
+  }
+}
+
+void launch_pphh_epilogue(const DevBasis& B, const SegTask* t, int nt, const double* L, int Lh, const double* R,
+                          int Rh, double* RES, double* w1, double* w2, const double* Pscr, const i64* poff,
+                          hipStream_t s) {
+  if (nt <= 0) return;
+  hipLaunchKernelGGL(k_pphh_epilogue, dim3(nt), dim3(256), 0, s, B, t, L, Lh, R, Rh, RES, w1, w2, Pscr, poff);
+}
+
+// ================================================ streaming algebra / norm
+// add_sq_op, basic_IMSRG.f90:3551-3583:  C = A*ax + B*bx
+__global__ __launch_bounds__(256) void k_axpy(i64 n, const double* A, double ax, const double* Bv, double bx,
+                                              double* C) {
+  for (i64 x = (i64)blockIdx.x * blockDim.x + threadIdx.x; x < n; x += (i64)gridDim.x * blockDim.x)
+    C[x] = A[x] * ax + Bv[x] * bx;
+}
+__global__ __launch_bounds__(256) void k_axpy_dev(i64 n, const double* A, const double* Bv, const double* coef,
+                                                  double* C) {
+  const double bx = coef[0];
+  for (i64 x = (i64)blockIdx.x * blockDim.x + threadIdx.x; x < n; x += (i64)gridDim.x * blockDim.x)
+    C[x] = A[x] * 1.0 + Bv[x] * bx;
+}
+
+static unsigned stream_grid(i64 n) {
+  i64 g = (n + 255) / 256;
+  if (g > 8192) g = 8192;
+  if (g < 1) g = 1;
+  return (unsigned)g;
+}
+
+void launch_axpy(i64 n, const double* A, double ax, const double* Bv, double bx, double* C, hipStream_t s) {
+  hipLaunchKernelGGL(k_axpy, dim3(stream_grid(n)), dim3(256), 0, s, n, A, ax, Bv, bx, C);
+}
+void launch_axpy_devcoef(i64 n, const double* A, const double* Bv, const double* coef, double* C,
+                         hipStream_t s) {
+  hipLaunchKernelGGL(k_axpy_dev, dim3(stream_grid(n)), dim3(256), 0, s, n, A, Bv, coef, C);
+}
+
+// mat_frob_norm, basic_IMSRG.f90:4056-4087: weighted sum of squares per chunk
+__global__ __launch_bounds__(256) void k_norm_partials(const double* op, const i64* coff, const int* clen,
+                                                       const double* cw, double* part) {
+  __shared__ double red[256];
+  const int c = blockIdx.x;
+  const double* x = op + coff[c];
+  const int n = clen[c];
+  double v = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) v += x[i] * x[i];
+  const double s = block_sum(v, red);
+  if (threadIdx.x == 0) part[c] = s * cw[c];
+}
+__global__ __launch_bounds__(1024) void k_sum_sqrt(const double* part, int n, double* dst, int do_sqrt) {
+  __shared__ double red[1024];
+  double v = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) v += part[i];
+  const double s = block_sum(v, red);
+  if (threadIdx.x == 0) dst[0] = do_sqrt ? sqrt(s) : s;
+}
+
+void launch_norm_partials(const double* op, const i64* chunk_off, const int* chunk_len, const double* chunk_w,
+                          int nchunk, double* part, hipStream_t s) {
+  if (nchunk <= 0) return;
+  hipLaunchKernelGGL(k_norm_partials, dim3(nchunk), dim3(256), 0, s, op, chunk_off, chunk_len, chunk_w, part);
+}
+void launch_sum_sqrt(const double* part, int n, double* dst, int do_sqrt, hipStream_t s) {
+  hipLaunchKernelGGL(k_sum_sqrt, dim3(1), dim3(1024), 0, s, part, n, dst, do_sqrt);
+}
+
+// ======================================================== white generator
+// twobody_monopole, basic_IMSRG.f90:2075-2090, for every orbit pair
+__global__ __launch_bounds__(256) void k_monopole(DevBasis B, const double* H, double* M) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= B.N * B.N) return;
+  const int a = idx % B.N, b = idx / B.N;
+  const int ja = B.jj[a], jb = B.jj[b];
+  double sm = 0.0;
+  for (int JT = abs(ja - jb); JT <= ja + jb; JT += 2) sm = sm + (JT + 1) * v_elem(B, H, 1, a, b, a, b, JT);
+  M[idx] = sm / (ja + 1.0) / (jb + 1.0);
+}
+void launch_monopole(const DevBasis& B, const double* H, double* M, hipStream_t s) {
+  hipLaunchKernelGGL(k_monopole, dim3((B.N * B.N + 255) / 256), dim3(256), 0, s, B, H, M);
+}
+
+// build_gs_white one-body part, generators.f90:19-55 (also zeroes fhh, fpp)
+__global__ __launch_bounds__(256) void k_white_1b(DevBasis B, const double* H, double* ETA) {
+  const i64 idx = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  const i64 nA = (i64)B.A * B.A, nPA = (i64)B.P * B.A, nP = (i64)B.P * B.P;
+  if (idx < nA) { ETA[B.off_fhh + idx] = 0.0; return; }
+  if (idx < nA + nP) { ETA[B.off_fpp + (idx - nA)] = 0.0; return; }
+  if (idx >= nA + nP + nPA) return;
+  const i64 x = idx - nA - nP;
+  const int a = (int)(x % B.P), i = (int)(x / B.P);
+  const int ak = B.parts[a], ik = B.holes[i];
+  if (B.jj[ik] != B.jj[ak] || B.ll[ik] != B.ll[ak] || B.tz[ik] != B.tz[ak]) return;
+  const int ji = B.jj[ik];
+  double Eden = 0.0;
+  for (int JT = 0; JT <= 2 * ji; JT += 2) Eden = Eden - (JT + 1) * v_elem(B, H, 1, ak, ik, ak, ik, JT);
+  Eden = Eden / (ji + 1.0) / (ji + 1.0);
+  Eden = Eden + H[B.off_fpp + a + (i64)a * B.P] - H[B.off_fhh + i + (i64)i * B.A];
+  ETA[B.off_fph + x] = H[B.off_fph + x] / Eden;
+}
+void launch_white_1b(const DevBasis& B, const double* H, double* ETA, hipStream_t s) {
+  const i64 n = (i64)B.A * B.A + (i64)B.P * B.P + (i64)B.P * B.A;
+  hipLaunchKernelGGL(k_white_1b, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, B, H, ETA);
+}
+
+// build_gs_white two-body part, generators.f90:57-104: full-segment tasks
+__global__ __launch_bounds__(256) void k_white_2b(DevBasis B, const SegTask* tasks, const double* H,
+                                                  const double* M, double* ETA) {
+  const SegTask t = tasks[blockIdx.x];
+  const i64 e = (i64)t.start + threadIdx.x;
+  if (e >= t.cnt) return;
+  const int q = t.q, g = t.g;
+  const i64 o = B.goff[q * 6 + g] + e;
+  if (g != 2) { ETA[o] = 0.0; return; }
+  const int np = B.dims[q * 3];
+  const int IX = (int)(e % np), JX = (int)(e / np);
+  int a, b, i, j;
+  unpack(B.pairs[B.pairoff[q * 3 + 0] + IX], a, b);
+  unpack(B.pairs[B.pairoff[q * 3 + 2] + JX], i, j);
+  const int N = B.N;
+  double Eden = 0.0;
+  Eden = Eden + M[a + b * N];
+  Eden = Eden + M[i + j * N];
+  Eden = Eden - M[a + i * N];
+  Eden = Eden - M[a + j * N];
+  Eden = Eden - M[i + b * N];
+  Eden = Eden - M[j + b * N];
+  Eden = Eden + f_elem(B, H, 1, a, a) + f_elem(B, H, 1, b, b) - f_elem(B, H, 1, i, i) - f_elem(B, H, 1, j, j);
+  ETA[o] = H[o] / Eden;
+}
+void launch_white_2b(const DevBasis& B, const SegTask* t, int nt, const double* H, const double* M, double* ETA,
+                     hipStream_t s) {
+  if (nt <= 0) return;
+  hipLaunchKernelGGL(k_white_2b, dim3(nt), dim3(256), 0, s, B, t, H, M, ETA);
+}
+
+// mbpt2, basic_IMSRG.f90:4165-4199: one partial per gam3 tile
+__global__ __launch_bounds__(256) void k_mbpt2_partials(DevBasis B, const SegTask* tasks, const double* H,
+                                                        double* part) {
+  __shared__ double red[256];
+  const SegTask t = tasks[blockIdx.x];
+  const i64 e = (i64)t.start + threadIdx.x;
+  double v = 0.0;
+  const int q = t.q;
+  if (e < t.cnt) {
+    const int np = B.dims[q * 3];
+    const int II = (int)(e % np), JJ = (int)(e / np);
+    int i, j, k, l;
+    unpack(B.pairs[B.pairoff[q * 3 + 0] + II], i, j);
+    unpack(B.pairs[B.pairoff[q * 3 + 2] + JJ], k, l);
+    const double fi = f_elem(B, H, 1, i, i), fj = f_elem(B, H, 1, j, j);
+    const double fk = f_elem(B, H, 1, k, k), fl = f_elem(B, H, 1, l, l);
+    const double eden = fk + fl - fi - fj;
+    const double x = H[B.goff[q * 6 + 2] + e];
+    v = x * x / eden;
+  }
+  const double s = block_sum(v, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = s * (B.chJ[q] + 1.0);
+}
+void launch_mbpt2_partials(const DevBasis& B, const SegTask* t, int nt, const double* H, double* part,
+                           hipStream_t s) {
+  if (nt <= 0) return;
+  hipLaunchKernelGGL(k_mbpt2_partials, dim3(nt), dim3(256), 0, s, B, t, H, part);
+}
+
+// ============================================================ normal_order
+// basic_IMSRG.f90:2419-2550.  scratch[0] receives E0; then the one-body
+// corrections are applied in place (every thread reads only the element it
+// writes, so the in-place update equals the reference's TEMP copy).
+__global__ __launch_bounds__(256) void k_no_e0(DevBasis B, const double* H, double* E0) {
+  __shared__ double red[256];
+  double v = 0.0;
+  // 2 * sum_h f_hh (2j+1)
+  for (int i = threadIdx.x; i < B.N; i += blockDim.x)
+    if (B.con[i] == 1) v += f_elem(B, H, 1, i, i) * (B.jj[i] + 1);
+  double s1 = block_sum(v, red) * 2.0;
+  v = 0.0;
+  for (int x = threadIdx.x; x < B.N * B.N; x += blockDim.x) {
+    const int i = x / B.N, j = x % B.N;
+    if (B.con[i] * B.con[j] == 0) continue;
+    for (int JT = abs(B.jj[i] - B.jj[j]); JT <= B.jj[i] + B.jj[j]; JT += 2)
+      v += v_elem(B, H, 1, i, j, i, j, JT) * (JT + 1);
+  }
+  const double s2 = block_sum(v, red);
+  if (threadIdx.x == 0) E0[0] = (s1 + s2) * 0.5;
+}
+
+__global__ __launch_bounds__(256) void k_no_1b(DevBasis B, double* H) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= B.N * B.N) return;
+  const int ax = idx % B.N, bx = idx / B.N;
+  if (B.jj[ax] != B.jj[bx] || B.ll[ax] != B.ll[bx] || B.tz[ax] != B.tz[bx]) return;
+  const int ca = B.con[ax], cb = B.con[bx];
+  // the reference visits hh (a<=b), ph (a particle, b hole), pp (a<=b)
+  if (ca == 1 && cb == 1 && B.cidx[ax] > B.cidx[bx]) return;
+  if (ca == 0 && cb == 0 && B.cidx[ax] > B.cidx[bx]) return;
+  if (ca == 1 && cb == 0) return;
+  double sm = 0.0;
+  for (int ix = 0; ix < B.N; ++ix) {
+    if (B.con[ix] != 1) continue;
+    for (int JT = abs(B.jj[ix] - B.jj[ax]); JT <= B.jj[ix] + B.jj[ax]; JT += 2)
+      sm = sm + (JT + 1.0) / (B.jj[ax] + 1) * v_elem(B, H, 1, ax, ix, bx, ix, JT);
+  }
+  const double v = f_elem(B, H, 1, ax, bx) + sm;
+  const int a = B.cidx[ax], b = B.cidx[bx];
+  if (ca == 1) {
+    H[B.off_fhh + a + (i64)b * B.A] = v;
+    H[B.off_fhh + b + (i64)a * B.A] = v;
+  } else if (cb == 1) {
+    H[B.off_fph + a + (i64)b * B.P] = v;
+  } else {
+    H[B.off_fpp + a + (i64)b * B.P] = v;
+    H[B.off_fpp + b + (i64)a * B.P] = v;
+  }
+}
+
+__global__ void k_set_e0(double* H, const double* E0) { H[0] = E0[0]; }
+
+void launch_normal_order(const DevBasis& B, double* H, double* scratch, hipStream_t s) {
+  hipLaunchKernelGGL(k_no_e0, dim3(1), dim3(256), 0, s, B, H, scratch);
+  hipLaunchKernelGGL(k_no_1b, dim3((B.N * B.N + 255) / 256), dim3(256), 0, s, B, H);
+  hipLaunchKernelGGL(k_set_e0, dim3(1), dim3(1), 0, s, H, scratch);
+}
+
+// ================================================================ helpers
+__global__ void k_fill(double* p, i64 n, double v) {
+  for (i64 x = (i64)blockIdx.x * blockDim.x + threadIdx.x; x < n; x += (i64)gridDim.x * blockDim.x) p[x] = v;
+}
+__global__ void k_copy(double* d, const double* s, i64 n) {
+  for (i64 x = (i64)blockIdx.x * blockDim.x + threadIdx.x; x < n; x += (i64)gridDim.x * blockDim.x) d[x] = s[x];
+}
+void launch_fill(double* p, i64 n, double v, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_fill, dim3(stream_grid(n)), dim3(256), 0, s, p, n, v);
+}
+void launch_copy(double* dst, const double* src, i64 n, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_copy, dim3(stream_grid(n)), dim3(256), 0, s, dst, src, n);
+}
+
+}  // namespace imsrg

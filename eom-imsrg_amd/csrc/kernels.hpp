@@ -1,0 +1,92 @@
+// Work descriptors and kernel launchers of the scalar-commutator path.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "basis.hpp"
+
+namespace imsrg {
+
+constexpr int kTile = 256;   // work items per workgroup of the element kernels
+
+// A tile of up to kTile elements of block g of channel q; `cnt` is the
+// number of elements of the (triangular or full) segment.
+struct SegTask { int q; int g; int start; int cnt; };
+
+// One-body output (kind 0 hh, 1 pp, 2 ph): orbits p, q (p <= q for hh, pp)
+// plus their indices in fhh / fpp / fph.
+struct OneTask { int kind; int pk; int qk; int pi; int qi; };
+
+// Tile of up to kTile CCX elements of cross-coupled block q1
+struct CCTask { int q1; int start; };
+
+// Operand table of a grouped GEMM launch: problems address operands as
+// (slot, offset) so one problem list serves every call.
+struct Slots { const double* p[6]; };
+
+// C(slotC) = alpha * (op(A1) op(B1) + s2 * op(A2) op(B2)) + beta * C
+struct GemmProb {
+  i64 a1, b1, a2, b2, c;
+  int m, n, k;
+  int lda, ldb, ldc;
+  int nprod;
+  unsigned char ta1, tb1, ta2, tb2;
+  unsigned char sa1, sb1, sa2, sb2;
+  unsigned char sc, pad0, pad1, pad2;
+  double alpha, s2, beta;
+};
+struct GemmTask { int prob; int tm; int tn; int pad; };
+
+// Tile pair (I,J), I <= J, of the pppp block of channel q
+struct PairTask { int q; int ti; int tj; int pad; };
+
+// Task list on the device with per-channel ranges (the plain path launches
+// one range at a time).
+template <class T>
+struct TaskList {
+  T* dev = nullptr;
+  int n = 0;
+  std::vector<T> host;
+  std::vector<int> chan_begin;   // [groups+1]
+};
+
+// launchers (all asynchronous on `s`)
+void launch_comm122(const DevBasis& B, const SegTask* t, int nt, const double* L, int Lh, const double* R,
+                    int Rh, double* RES, int RESh, hipStream_t s);
+void launch_comm222ph_scatter(const DevBasis& B, const SegTask* t, int nt, const double* W, int Lh, int Rh,
+                              double* RES, int RESh, hipStream_t s);
+void launch_pandya(const DevBasis& B, const CCTask* t, int nt, const double* op, int herm, double* CC,
+                   hipStream_t s);
+void launch_comm111(const DevBasis& B, const double* L, int Lh, const double* R, int Rh, double* RES,
+                    hipStream_t s);
+void launch_comm121(const DevBasis& B, const OneTask* t, int nt, const double* L, int Lh, const double* R,
+                    int Rh, double* RES, int RESh, hipStream_t s);
+void launch_comm221(const DevBasis& B, const OneTask* t, int nt, int pm, const double* w1, const double* w2,
+                    double* RES, int RESh, hipStream_t s);
+void launch_e0_partials(const DevBasis& B, const double* L, int Lh, const double* R, int Rh, double* part,
+                        hipStream_t s);
+void launch_e0_final(const double* part, int nblocks, int use220, int use110, double* dst, int accumulate,
+                     hipStream_t s);
+void launch_gemm(const Slots& sl, const GemmProb* probs, const GemmTask* tasks, int nt, hipStream_t s);
+void launch_pppp_pairs(const DevBasis& B, const PairTask* t, int nt, const double* L, int Lh, const double* R,
+                       int Rh, double* RES, double* w1, double* w2, int write_w1, hipStream_t s);
+void launch_pphh_epilogue(const DevBasis& B, const SegTask* t, int nt, const double* L, int Lh, const double* R,
+                          int Rh, double* RES, double* w1, double* w2, const double* Pscr, const i64* poff,
+                          hipStream_t s);
+void launch_axpy(i64 n, const double* A, double ax, const double* B, double bx, double* C, hipStream_t s);
+void launch_axpy_devcoef(i64 n, const double* A, const double* B, const double* coef, double* C, hipStream_t s);
+void launch_norm_partials(const double* op, const i64* chunk_off, const int* chunk_len, const double* chunk_w,
+                          int nchunk, double* part, hipStream_t s);
+void launch_sum_sqrt(const double* part, int n, double* dst, int do_sqrt, hipStream_t s);
+void launch_monopole(const DevBasis& B, const double* H, double* M, hipStream_t s);
+void launch_white_1b(const DevBasis& B, const double* H, double* ETA, hipStream_t s);
+void launch_white_2b(const DevBasis& B, const SegTask* t, int nt, const double* H, const double* M, double* ETA,
+                     hipStream_t s);
+void launch_mbpt2_partials(const DevBasis& B, const SegTask* t, int nt, const double* H, double* part,
+                           hipStream_t s);
+void launch_normal_order(const DevBasis& B, double* H, double* scratch, hipStream_t s);
+void launch_fill(double* p, i64 n, double v, hipStream_t s);
+void launch_copy(double* dst, const double* src, i64 n, hipStream_t s);
+
+}  // namespace imsrg

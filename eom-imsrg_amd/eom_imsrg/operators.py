@@ -1,0 +1,90 @@
+"""Operator containers of the reference (``type(sq_op)``, ``type(cc_mat)``)
+as device FP64 buffers.
+
+``SqOp`` holds E0, fhh, fph, fpp and the six gam blocks of every channel in
+the flat column-major layout of include/imsrg_hip.h, plus the herm flag
+(basic_IMSRG.f90:65-76).  ``CCMat`` is the cross-coupled storage
+(cross_coupled.f90:11-15) and ``CCWork`` the WCC workspace
+(allocate_CC_wkspc, cross_coupled.f90:1075-1096).  torch provides the
+device memory and the stream; all arithmetic is in libimsrg_hip.so.
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from .basis import Basis, ROW_CLS, COL_CLS
+
+
+def _stream() -> int:
+    return torch.cuda.current_stream().cuda_stream
+
+
+class SqOp:
+    def __init__(self, basis: Basis, data: torch.Tensor, herm: int = 1):
+        if data.dtype != torch.float64 or data.numel() != basis.oplen or not data.is_cuda:
+            raise ValueError("SqOp needs a CUDA float64 tensor of basis.oplen elements")
+        self.basis = basis
+        self.data = data.contiguous()
+        self.herm = int(herm)
+
+    # allocate_blocks / duplicate_sq_op (basic_IMSRG.f90:521-685, 2797-2860): zero operator
+    @classmethod
+    def zeros(cls, basis: Basis, herm: int = 1, device="cuda") -> "SqOp":
+        return cls(basis, torch.zeros(basis.oplen, dtype=torch.float64, device=device), herm)
+
+    @classmethod
+    def from_numpy(cls, basis: Basis, arr: np.ndarray, herm: int = 1, device="cuda") -> "SqOp":
+        t = torch.from_numpy(np.ascontiguousarray(arr, np.float64)).to(device)
+        return cls(basis, t, herm)
+
+    def numpy(self) -> np.ndarray:
+        return self.data.detach().cpu().numpy()
+
+    @property
+    def ptr(self) -> int:
+        return self.data.data_ptr()
+
+    @property
+    def E0(self) -> float:
+        return float(self.data[0].item())
+
+    def duplicate(self, herm: int | None = None) -> "SqOp":
+        return SqOp.zeros(self.basis, self.herm if herm is None else herm, self.data.device)
+
+    def clone(self) -> "SqOp":
+        return SqOp(self.basis, self.data.clone(), self.herm)
+
+    def gam(self, q: int, g: int) -> torch.Tensor:
+        """view of gam(g+1) of channel q as an (rows, cols) matrix"""
+        n1, n2 = self.basis.block_shape(q, g)
+        o = int(self.basis.goff[q, g])
+        return self.data[o: o + n1 * n2].view(n2, n1).t()
+
+
+class CCMat:
+    """cross-coupled matrices CCX(q) (rlen x nph) of one operator"""
+
+    def __init__(self, basis: Basis, herm: int = 1, device="cuda"):
+        self.basis = basis
+        self.herm = herm
+        self.data = torch.zeros(max(basis.cclen, 1), dtype=torch.float64, device=device)
+
+    @property
+    def ptr(self) -> int:
+        return self.data.data_ptr()
+
+
+class CCWork:
+    """WCC workspace: rlen x rlen per cross-coupled block"""
+
+    def __init__(self, basis: Basis, device="cuda"):
+        self.basis = basis
+        self.data = torch.zeros(max(basis.wcclen, 1), dtype=torch.float64, device=device)
+
+    @property
+    def ptr(self) -> int:
+        return self.data.data_ptr()
+
+
+__all__ = ["SqOp", "CCMat", "CCWork", "ROW_CLS", "COL_CLS"]

@@ -1,0 +1,163 @@
+/*
+ * C-ABI of the MI355X-native IMSRG(2) scalar-commutator / BCH / Magnus path.
+ *
+ * Drop-in boundary for n-parzuchowski/EOM-IMSRG: every entry point below
+ * replaces one reference routine (cited per declaration) with the same
+ * argument meaning.  Operators are device buffers in one flat FP64 layout
+ * that is the reference's own storage concatenated:
+ *
+ *   [E0 | fhh (A x A) | fph (P x A) | fpp (P x P) |
+ *    for each channel q = 3*(2J) + 2*(Tz+1) + parity:
+ *      gam1 (npp x npp) gam2 (npp x nph) gam3 (npp x nhh)
+ *      gam4 (nph x nph) gam5 (nhh x nhh) gam6 (nph x nhh) ]
+ *
+ * every block column-major exactly as the Fortran sq_op holds it
+ * (basic_IMSRG.f90:50-76,619-628).  The herm flag (+1 / -1) of each operand
+ * travels beside its pointer, as sq_op%herm does.  Cross-coupled matrices
+ * (type cc_mat) are device buffers of imsrg_basis_size(IMSRG_SIZE_CC)
+ * doubles; the WCC workspace has imsrg_basis_size(IMSRG_SIZE_WCC) doubles.
+ *
+ * All compute entry points are asynchronous on `stream` (a hipStream_t; 0 is
+ * the default stream) unless they return a host value.  They return 0 on
+ * success or an IMSRG_E* code; imsrg_last_error() describes the failure.
+ * The reference reports errors with STOP; a Fortran shim maps non-zero
+ * codes back to the same STOP text.  RES must not alias L or R.
+ */
+#ifndef IMSRG_HIP_H
+#define IMSRG_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct imsrg_basis_s* imsrg_basis;
+typedef struct imsrg_flow_s* imsrg_flow;
+typedef void* imsrg_stream;
+
+#define IMSRG_OK 0
+#define IMSRG_EINVAL 1
+#define IMSRG_EHIP 2
+#define IMSRG_ECLIP 3 /* BCH_EXPAND did not converge ('clipping BCH') */
+#define IMSRG_ESTEPS 4 /* magnus_decouple hit the step limit */
+
+/* which kernels execute a commutator term */
+#define IMSRG_PATH_GROUPED 0 /* one grouped launch per term over all channels */
+#define IMSRG_PATH_PLAIN 1   /* one launch per channel per term */
+
+/* imsrg_basis_size selectors */
+#define IMSRG_SIZE_NORB 0
+#define IMSRG_SIZE_NHOLE 1
+#define IMSRG_SIZE_NPART 2
+#define IMSRG_SIZE_JTOTAL_MAX 3
+#define IMSRG_SIZE_NBLOCKS 4
+#define IMSRG_SIZE_OP 5   /* doubles per operator */
+#define IMSRG_SIZE_NCC 6  /* cross-coupled blocks */
+#define IMSRG_SIZE_CC 7   /* doubles per cross-coupled matrix set (cc_mat) */
+#define IMSRG_SIZE_WCC 8  /* doubles of the WCC workspace */
+
+const char* imsrg_last_error(void);
+
+/* ---------------------------------------------------------------- basis */
+/* read_sp_basis + find_holes + allocate_blocks + allocate_CCMAT + store_6j
+ * (basic_IMSRG.f90:192-440,521-685,2227-2332; cross_coupled.f90:61-177):
+ * orbit table (n, l, 2j, 2tz, hole flag) in the .sps order.  Builds host
+ * tables only; device tables are uploaded on first device use. */
+int imsrg_basis_create(int norb, const int* n, const int* l, const int* jj2, const int* tz2, const int* hole,
+                       imsrg_basis* out);
+int imsrg_basis_destroy(imsrg_basis b);
+int64_t imsrg_basis_size(imsrg_basis b, int what);
+/* per channel: dims[3q+c] (npp, nph, nhh), goff[6q+g], jtp[3q+{0,1,2}] = (2J, Tz, parity) */
+int imsrg_basis_channels(imsrg_basis b, int* dims, int64_t* goff, int* jtp);
+/* pair list of class cls (0 pp, 1 ph, 2 hh) of channel q: out[2i], out[2i+1] (0-based orbits) */
+int imsrg_basis_pairs(imsrg_basis b, int q, int cls, int* out);
+/* cross-coupled layout: rlen, ncol, ccoff, wcoff, jtp[3q+{0,1,2}] = (2J, |Tz|, parity) */
+int imsrg_basis_cc(imsrg_basis b, int* rlen, int* ncol, int64_t* ccoff, int64_t* wcoff, int* jtp);
+/* host 6j symbol of the basis table (twice-j arguments), for tests */
+double imsrg_basis_sixj(imsrg_basis b, int j1, int j2, int j3, int j4, int j5, int j6);
+int imsrg_set_path(imsrg_basis b, int path);
+
+/* ------------------------------------------------ commutator terms */
+/* commutator_110, commutators.f90:10-33 (returns the value; synchronises) */
+int imsrg_commutator_110(imsrg_basis b, const double* L, int Lherm, const double* R, int Rherm, double* value,
+                         imsrg_stream stream);
+/* commutator_220, commutators.f90:399-432 (returns the value; synchronises) */
+int imsrg_commutator_220(imsrg_basis b, const double* L, int Lherm, const double* R, int Rherm, double* value,
+                         imsrg_stream stream);
+/* RES%E0 = commutator_110 + commutator_220 written on the device
+ * (IMSRG_MAGNUS.f90:210) */
+int imsrg_commutator_e0(imsrg_basis b, const double* L, int Lherm, const double* R, int Rherm, double* RES,
+                        imsrg_stream stream);
+/* commutator_111, commutators.f90:36-80 (assigns RES one-body) */
+int imsrg_commutator_111(imsrg_basis b, const double* L, int Lherm, const double* R, int Rherm, double* RES,
+                         int RESherm, imsrg_stream stream);
+/* commutator_121, commutators.f90:83-280 (accumulates into RES one-body) */
+int imsrg_commutator_121(imsrg_basis b, const double* L, int Lherm, const double* R, int Rherm, double* RES,
+                         int RESherm, imsrg_stream stream);
+/* commutator_122, commutators.f90:283-396 (assigns RES two-body) */
+int imsrg_commutator_122(imsrg_basis b, const double* L, int Lherm, const double* R, int Rherm, double* RES,
+                         int RESherm, imsrg_stream stream);
+/* commutator_222_pp_hh, commutators.f90:581-768 (accumulates; fills w1, w2) */
+int imsrg_commutator_222_pp_hh(imsrg_basis b, const double* L, int Lherm, const double* R, int Rherm, double* RES,
+                               int RESherm, double* w1, double* w2, imsrg_stream stream);
+/* commutator_221, commutators.f90:435-578 (accumulates; after 222_pp_hh) */
+int imsrg_commutator_221(imsrg_basis b, const double* L, int Lherm, const double* R, int Rherm, double* RES,
+                         int RESherm, const double* w1, const double* w2, imsrg_stream stream);
+/* calculate_cross_coupled, cross_coupled.f90:1190-1300 */
+int imsrg_calculate_cross_coupled(imsrg_basis b, const double* op, int herm, double* CC, imsrg_stream stream);
+/* commutator_222_ph, commutators.f90:771-925 (accumulates; fills WCC) */
+int imsrg_commutator_222_ph(imsrg_basis b, const double* LCC, int Lherm, const double* RCC, int Rherm,
+                            double* RES, int RESherm, double* WCC, imsrg_stream stream);
+
+/* ------------------------------------------------ operator algebra */
+/* add_sq_op, basic_IMSRG.f90:3551-3583: C = ax*A + bx*B (C may alias A or B) */
+int imsrg_add_sq_op(imsrg_basis b, const double* A, double ax, const double* B, double bx, double* C,
+                    imsrg_stream stream);
+/* mat_frob_norm, basic_IMSRG.f90:4056-4087 (returns the value; synchronises) */
+int imsrg_mat_frob_norm(imsrg_basis b, const double* op, double* value, imsrg_stream stream);
+/* build_gs_white, generators.f90:4-106 (ETA herm = -1) */
+int imsrg_build_gs_white(imsrg_basis b, const double* H, double* ETA, imsrg_stream stream);
+/* mbpt2, basic_IMSRG.f90:4165-4199 (returns the value; synchronises) */
+int imsrg_mbpt2(imsrg_basis b, const double* H, double* value, imsrg_stream stream);
+/* normal_order, basic_IMSRG.f90:2419-2550 (in place) */
+int imsrg_normal_order(imsrg_basis b, double* H, imsrg_stream stream);
+
+/* ------------------------------------------------ step level */
+/* Workspace of the nested-commutator drivers (INT1/INT2/AD/w1/w2 and the
+ * cross-coupled buffers that BCH_EXPAND and MAGNUS_EXPAND allocate). */
+int imsrg_flow_create(imsrg_basis b, imsrg_flow* out);
+int imsrg_flow_destroy(imsrg_flow f);
+/* capture each nested commutator term as a hipGraph (grouped path only) */
+int imsrg_flow_set_graphs(imsrg_flow f, int on);
+/* BCH_EXPAND, IMSRG_MAGNUS.f90:149-234: HS = exp(G) H exp(-G), terms until
+ * ||term|| < conv (reference: 1e-6 as a single-precision literal), at most
+ * maxit-1 nested commutators.  nterms = the loop index iw of the exit.
+ * Gherm / Hherm are G%herm and H%herm (HS and every term inherit Hherm).
+ * advals (host, optional, length maxit) receives ||term||*coef. */
+int imsrg_bch_expand(imsrg_flow f, double* HS, const double* G, int Gherm, const double* H, int Hherm,
+                     double conv, int maxit, int* nterms, double* advals, imsrg_stream stream);
+/* MAGNUS_EXPAND, IMSRG_MAGNUS.f90:620-691: DG <- dOmega/ds from eta (in DG)
+ * and Omega G.  ncomm = nested commutators evaluated. */
+int imsrg_magnus_expand(imsrg_flow f, double* DG, int DGherm, const double* G, int Gherm, double conv,
+                        int* ncomm, imsrg_stream stream);
+/* euler_step, IMSRG_MAGNUS.f90:754-764: G = G + ds*DG */
+int imsrg_euler_step(imsrg_basis b, double* G, const double* DG, double ds, imsrg_stream stream);
+/* One iteration of magnus_decouple's loop (IMSRG_MAGNUS.f90:100-125) with the
+ * white generator: MAGNUS_EXPAND, euler_step, BCH_EXPAND, build_gs_white,
+ * mbpt2.  out[0..3] = E0, mbpt2, Magnus commutators, BCH loop index. */
+int imsrg_magnus_step(imsrg_flow f, double* HS, double* G, double* DG, const double* H, double ds,
+                      double bch_conv, double mag_conv, double* out, imsrg_stream stream);
+/* magnus_decouple, IMSRG_MAGNUS.f90:15-128 with build_gs_white: HS in = the
+ * bare normal-ordered H, out = decoupled H; G out = Omega.  crit_mode 0:
+ * loop while |mbpt2| > crit_tol (reference); 1: until |dE0| < crit_tol.
+ * log (host, optional) receives 4 doubles per step: s, E0, E0+mbpt2,
+ * nested commutators.  *steps = Magnus steps taken. */
+int imsrg_magnus_decouple(imsrg_flow f, double* HS, double* G, double ds, double bch_conv, double mag_conv,
+                          int crit_mode, double crit_tol, int maxsteps, double* log, int* steps,
+                          imsrg_stream stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,219 @@
+"""TEST INFRASTRUCTURE ONLY: ctypes wrapper of the CPU oracle
+(oracle/imsrg_oracle.cpp, built into oracle/_build/liboracle.so).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+import this module.  Operators are numpy float64 arrays in the same flat
+layout as the product (E0, fhh, fph, fpp, then gam1..6 per channel).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "liboracle.so")
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} missing: run `make -C oracle`")
+        L = C.CDLL(LIB_PATH)
+        vp, d, i, dp, ip, lp = C.c_void_p, C.c_double, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int), \
+            C.POINTER(C.c_int64)
+        sig = {
+            "orc_set_blas": (i, [C.c_char_p]), "orc_unset_blas": (None, []), "orc_set_threads": (i, [i]),
+            "orc_basis_new": (vp, [i, ip, ip, ip, ip, ip]), "orc_basis_free": (None, [vp]),
+            "orc_basis_sizes": (None, [vp, lp]), "orc_basis_channels": (None, [vp, ip, lp, ip]),
+            "orc_basis_pairs": (i, [vp, i, i, ip]), "orc_basis_cc": (None, [vp, ip, ip, lp, lp, ip]),
+            "orc_cc_row": (i, [vp, i, i, i]), "orc_cc_col": (i, [vp, i, i, i]),
+            "orc_sixj": (d, [i] * 6), "orc_ninej": (d, [i] * 9), "orc_sixj_table": (d, [vp] + [i] * 6),
+            "orc_v_elem": (d, [vp, dp, i, i, i, i, i, i]), "orc_f_elem": (d, [vp, dp, i, i, i]),
+            "orc_comm110": (d, [vp, dp, i, dp, i]), "orc_comm220": (d, [vp, dp, i, dp, i]),
+            "orc_comm111": (None, [vp, dp, i, dp, i, dp, i]), "orc_comm121": (None, [vp, dp, i, dp, i, dp, i]),
+            "orc_comm122": (None, [vp, dp, i, dp, i, dp, i]),
+            "orc_comm222pphh": (None, [vp, dp, i, dp, i, dp, i, dp, dp]),
+            "orc_comm221": (None, [vp, dp, i, dp, i, dp, i, dp, dp]),
+            "orc_cross_coupled": (None, [vp, dp, i, dp]),
+            "orc_comm222ph": (None, [vp, dp, i, dp, i, dp, i, dp]),
+            "orc_frob_norm": (d, [vp, dp]), "orc_add": (None, [vp, dp, d, dp, d, dp]),
+            "orc_white": (None, [vp, dp, dp]), "orc_mbpt2": (d, [vp, dp]), "orc_normal_order": (None, [vp, dp]),
+            "orc_monopole": (d, [vp, dp, i, i]),
+            "orc_commutator": (None, [vp, dp, i, dp, i, dp, i]),
+            "orc_bch": (i, [vp, dp, dp, dp, d, i, dp]), "orc_magnus": (i, [vp, dp, dp, d]),
+            "orc_magnus_decouple": (i, [vp, dp, dp, d, d, d, i, d, i, dp]),
+            "orc_bf_0body": (d, [vp, dp, i, dp, i]), "orc_bf_1body": (d, [vp, dp, i, dp, i, i, i]),
+            "orc_bf_2body": (d, [vp, dp, i, dp, i, i, i, i, i, i, dp]),
+        }
+        for k, (r, a) in sig.items():
+            f = getattr(L, k)
+            f.restype = r
+            f.argtypes = a
+        _lib = L
+    return _lib
+
+
+def use_host_blas() -> str | None:
+    """route the oracle's dgemm to numpy's bundled OpenBLAS when present"""
+    import glob
+    import numpy
+    libs = glob.glob(os.path.join(os.path.dirname(numpy.__file__) + ".libs", "libscipy_openblas64_*.so"))
+    for p in libs:
+        if lib().orc_set_blas(p.encode()) == 0:
+            return p
+    return None
+
+
+def _dp(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class OracleBasis:
+    def __init__(self, n, l, jj, tz, hole):
+        ip = lambda a: np.ascontiguousarray(a, np.int32).ctypes.data_as(C.POINTER(C.c_int))
+        self._keep = [np.ascontiguousarray(x, np.int32) for x in (n, l, jj, tz, hole)]
+        self.h = lib().orc_basis_new(len(n), *[x.ctypes.data_as(C.POINTER(C.c_int)) for x in self._keep])
+        s = np.zeros(9, np.int64)
+        lib().orc_basis_sizes(self.h, s.ctypes.data_as(C.POINTER(C.c_int64)))
+        (self.N, self.A, self.P, self.JTM, self.nblocks, self.oplen, self.ncc, self.cclen,
+         self.wcclen) = [int(x) for x in s]
+        nb = self.nblocks
+        self.dims = np.zeros(3 * nb, np.int32)
+        self.goff = np.zeros(6 * nb, np.int64)
+        self.jtp = np.zeros(3 * nb, np.int32)
+        lib().orc_basis_channels(self.h, self.dims.ctypes.data_as(C.POINTER(C.c_int)),
+                                 self.goff.ctypes.data_as(C.POINTER(C.c_int64)),
+                                 self.jtp.ctypes.data_as(C.POINTER(C.c_int)))
+        self.dims = self.dims.reshape(nb, 3)
+        self.goff = self.goff.reshape(nb, 6)
+        self.jtp = self.jtp.reshape(nb, 3)
+
+    def pairs(self, q, c):
+        buf = np.zeros(2 * max(int(self.dims[q, c]), 1), np.int32)
+        k = lib().orc_basis_pairs(self.h, q, c, buf.ctypes.data_as(C.POINTER(C.c_int)))
+        return buf[: 2 * k].reshape(k, 2)
+
+    def cc_layout(self):
+        m = self.ncc
+        rl, nc = np.zeros(m, np.int32), np.zeros(m, np.int32)
+        co, wo, jtp = np.zeros(m, np.int64), np.zeros(m, np.int64), np.zeros(3 * m, np.int32)
+        lib().orc_basis_cc(self.h, rl.ctypes.data_as(C.POINTER(C.c_int)), nc.ctypes.data_as(C.POINTER(C.c_int)),
+                           co.ctypes.data_as(C.POINTER(C.c_int64)), wo.ctypes.data_as(C.POINTER(C.c_int64)),
+                           jtp.ctypes.data_as(C.POINTER(C.c_int)))
+        return rl, nc, co, wo, jtp.reshape(m, 3)
+
+    # --- reference routines on host arrays
+    def v_elem(self, op, herm, a, b, c, d, J):
+        return lib().orc_v_elem(self.h, _dp(op), herm, a, b, c, d, J)
+
+    def f_elem(self, op, herm, a, b):
+        return lib().orc_f_elem(self.h, _dp(op), herm, a, b)
+
+    def comm110(self, L, Lh, R, Rh):
+        return lib().orc_comm110(self.h, _dp(L), Lh, _dp(R), Rh)
+
+    def comm220(self, L, Lh, R, Rh):
+        return lib().orc_comm220(self.h, _dp(L), Lh, _dp(R), Rh)
+
+    def comm111(self, L, Lh, R, Rh, RES, RESh):
+        lib().orc_comm111(self.h, _dp(L), Lh, _dp(R), Rh, _dp(RES), RESh)
+
+    def comm121(self, L, Lh, R, Rh, RES, RESh):
+        lib().orc_comm121(self.h, _dp(L), Lh, _dp(R), Rh, _dp(RES), RESh)
+
+    def comm122(self, L, Lh, R, Rh, RES, RESh):
+        lib().orc_comm122(self.h, _dp(L), Lh, _dp(R), Rh, _dp(RES), RESh)
+
+    def comm222pphh(self, L, Lh, R, Rh, RES, RESh, w1, w2):
+        lib().orc_comm222pphh(self.h, _dp(L), Lh, _dp(R), Rh, _dp(RES), RESh, _dp(w1), _dp(w2))
+
+    def comm221(self, L, Lh, R, Rh, RES, RESh, w1, w2):
+        lib().orc_comm221(self.h, _dp(L), Lh, _dp(R), Rh, _dp(RES), RESh, _dp(w1), _dp(w2))
+
+    def cross_coupled(self, op, herm):
+        cc = np.zeros(max(self.cclen, 1))
+        lib().orc_cross_coupled(self.h, _dp(op), herm, _dp(cc))
+        return cc
+
+    def comm222ph(self, LCC, Lh, RCC, Rh, RES, RESh, WCC=None):
+        W = WCC if WCC is not None else np.zeros(max(self.wcclen, 1))
+        lib().orc_comm222ph(self.h, _dp(LCC), Lh, _dp(RCC), Rh, _dp(RES), RESh, _dp(W))
+        return W
+
+    def frob_norm(self, op):
+        return lib().orc_frob_norm(self.h, _dp(op))
+
+    def add(self, A, ax, B, bx):
+        C_ = np.zeros(self.oplen)
+        lib().orc_add(self.h, _dp(A), ax, _dp(B), bx, _dp(C_))
+        return C_
+
+    def white(self, H):
+        eta = np.zeros(self.oplen)
+        lib().orc_white(self.h, _dp(H), _dp(eta))
+        return eta
+
+    def mbpt2(self, H):
+        return lib().orc_mbpt2(self.h, _dp(H))
+
+    def normal_order(self, H):
+        out = H.copy()
+        lib().orc_normal_order(self.h, _dp(out))
+        return out
+
+    def monopole(self, H, a, c):
+        return lib().orc_monopole(self.h, _dp(H), a, c)
+
+    def commutator(self, L, Lh, R, Rh, RESh):
+        RES = np.zeros(self.oplen)
+        lib().orc_commutator(self.h, _dp(L), Lh, _dp(R), Rh, _dp(RES), RESh)
+        return RES
+
+    def bch(self, G, H, conv, maxit=40):
+        HS = np.zeros(self.oplen)
+        adv = np.zeros(maxit)
+        it = lib().orc_bch(self.h, _dp(HS), _dp(G), _dp(H), conv, maxit, _dp(adv))
+        return HS, it, adv
+
+    def magnus(self, DG, G, conv):
+        out = DG.copy()
+        n = lib().orc_magnus(self.h, _dp(out), _dp(G), conv)
+        return out, n
+
+    def magnus_decouple(self, H, ds, bch_conv, mag_conv, crit_mode, crit_tol, maxsteps=200):
+        HS = H.copy()
+        G = np.zeros(self.oplen)
+        log = np.zeros(4 * maxsteps)
+        st = lib().orc_magnus_decouple(self.h, _dp(HS), _dp(G), ds, bch_conv, mag_conv, crit_mode, crit_tol,
+                                       maxsteps, _dp(log))
+        return HS, G, log[: 4 * max(st, 0)].reshape(-1, 4), st
+
+    def bf_0body(self, A, Ah, B, Bh):
+        return lib().orc_bf_0body(self.h, _dp(A), Ah, _dp(B), Bh)
+
+    def bf_1body(self, A, Ah, B, Bh, a, b):
+        return lib().orc_bf_1body(self.h, _dp(A), Ah, _dp(B), Bh, a, b)
+
+    def bf_2body(self, A, Ah, B, Bh, a, b, c, d, J):
+        parts = np.zeros(3)
+        v = lib().orc_bf_2body(self.h, _dp(A), Ah, _dp(B), Bh, a, b, c, d, J, _dp(parts))
+        return v, parts
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_basis_free(self.h)
+            self.h = None
+
+
+def sixj(*a):
+    return lib().orc_sixj(*a)
+
+
+def ninej(*a):
+    return lib().orc_ninej(*a)

@@ -1,0 +1,62 @@
+import os
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+for p in (os.path.join(ROOT, "eom-imsrg_amd"), ROOT):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP kernels)")
+    config.addinivalue_line("markers", "slow: larger bases, minutes of CPU")
+
+
+_BASES = {}
+
+
+def basis(emax: int):
+    from eom_imsrg import Basis
+    if emax not in _BASES:
+        _BASES[emax] = Basis.oxygen16(emax)
+    return _BASES[emax]
+
+
+_ORACLES = {}
+
+
+def oracle_basis(emax: int):
+    from oracle.oracle import OracleBasis
+    if emax not in _ORACLES:
+        b = basis(emax)
+        _ORACLES[emax] = OracleBasis(b.n, b.l, b.jj, b.tz, b.hole)
+    return _ORACLES[emax]
+
+
+def assert_rel(got, ref, rtol=1e-10, floor=1e-3, what=""):
+    """|got - ref| <= rtol * (|ref| + floor * max|ref|), element by element.
+
+    FP64 sums of K terms carry ~K*1e-16 absolute error relative to the
+    largest terms, so elements cancelled far below the block's scale are
+    held to rtol*floor*max|ref| instead of their own magnitude."""
+    got = np.asarray(got, np.float64)
+    ref = np.asarray(ref, np.float64)
+    scale = float(np.max(np.abs(ref))) if ref.size else 0.0
+    bound = rtol * (np.abs(ref) + floor * scale) + 1e-300
+    d = np.abs(got - ref)
+    bad = d > bound
+    if bad.any():
+        k = int(np.argmax(d / bound))
+        raise AssertionError(f"{what}: {int(bad.sum())} of {d.size} elements off; worst at {k}: got {got.flat[k]!r}"
+                             f" ref {ref.flat[k]!r} (|d|={d.flat[k]:.3e}, bound {bound.flat[k]:.3e}, scale {scale:.3e})")
+
+
+@pytest.fixture
+def gpu():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch.device("cuda:0")
