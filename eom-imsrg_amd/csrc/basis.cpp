@@ -176,20 +176,21 @@ std::string build_basis(Basis& b, int N, const int* n, const int* l, const int* 
   b.cclen = co;
   b.wclen = wo;
 
-  // 6j table indexed [J3/2][j1/2][j2/2][j4/2][j5/2][J6/2]: for a fixed
-  // channel 2J (= J3) the symbols of one channel form one contiguous slab.
+  // 6j table indexed [j1/2][j2/2][j4/2][j5/2][J3/2][J6/2]: the symbols that
+  // share their four half-integer arguments form one contiguous
+  // (J3, J6) tile, which is what one orbit quadruple reads.
   b.nh6 = (b.JTM + 1) / 2;
   b.nJ6 = b.JTM + 1;
   const int nh = b.nh6, nJ6 = b.nJ6;
   b.six.assign((size_t)nJ6 * nh * nh * nh * nh * nJ6, 0.0);
-  for (int J3 = 0; J3 < nJ6; ++J3)
-    for (int a1 = 0; a1 < nh; ++a1)
-      for (int a2 = 0; a2 < nh; ++a2)
-        for (int a4 = 0; a4 < nh; ++a4)
-          for (int a5 = 0; a5 < nh; ++a5)
+  for (int a1 = 0; a1 < nh; ++a1)
+    for (int a2 = 0; a2 < nh; ++a2)
+      for (int a4 = 0; a4 < nh; ++a4)
+        for (int a5 = 0; a5 < nh; ++a5)
+          for (int J3 = 0; J3 < nJ6; ++J3)
             for (int J6 = 0; J6 < nJ6; ++J6) {
               const size_t idx =
-                  ((((((size_t)J3 * nh + a1) * nh + a2) * nh + a4) * nh + a5) * nJ6) + J6;
+                  ((((((size_t)a1 * nh + a2) * nh + a4) * nh + a5) * nJ6 + J3) * nJ6) + J6;
               b.six[idx] = sixj_value(2 * a1 + 1, 2 * a2 + 1, 2 * J3, 2 * a4 + 1, 2 * a5 + 1, 2 * J6);
             }
   return "";

@@ -44,6 +44,12 @@ struct imsrg_basis_s {
   TaskList<CCTask> cc;
   TaskList<PairTask> pairs;
   TaskList<GemmTask> k2, k3, ph;
+  std::vector<Partner> part_h;      // 122 partner lists (CSR by global pair id)
+  std::vector<int> rowp_h, colp_h;
+  Partner* partners = nullptr;
+  int *rowp = nullptr, *colp = nullptr;
+  std::vector<QuadItem> quad_h;   // 222_ph outputs, quadruple-major (grouped path)
+  QuadItem* quad = nullptr;
   std::vector<GemmProb> k2p_h, k3p_h, php_h;
   GemmProb *k2p = nullptr, *k3p = nullptr, *php = nullptr;
   std::vector<OneTask> one_h;
@@ -247,6 +253,102 @@ int build_plan(imsrg_basis_s* B) {
   for (int p = 0; p < b.P; ++p)
     for (int q = 0; q < b.A; ++q)
       if (same(b.parts[p], b.holes[q])) B->one_h.push_back(OneTask{2, b.parts[p], b.holes[q], p, q});
+  // 122 partner lists.  For the stored pair (a,b) of channel q: row role =
+  // (a -> i, f(a,i)) then (b -> i, f(b,i)); column role = (a -> i, f(i,a))
+  // then (b -> i, f(i,b)); i runs over the (l,j,tz) block of the replaced
+  // orbit.  Phase / sqrt(2) of the resulting pair as v_elem applies them.
+  {
+    B->part_h.clear();
+    const size_t npairs = b.pairs.size();
+    B->rowp_h.assign(npairs + 1, 0);
+    B->colp_h.assign(npairs + 1, 0);
+    auto fref = [&](int x, int y, int& foff, int& fsw) {   // f_elem(x, y) location
+      const int c1 = b.con[x], c2 = b.con[y];
+      fsw = 0;
+      if (c1 + c2 == 0) foff = (int)(b.off_fpp + b.cidx[x] + (i64)b.cidx[y] * b.P);
+      else if (c1 + c2 == 2) foff = (int)(b.off_fhh + b.cidx[x] + (i64)b.cidx[y] * b.A);
+      else if (c1 > c2) { foff = (int)(b.off_fph + b.cidx[y] + (i64)b.cidx[x] * b.P); fsw = 1; }
+      else foff = (int)(b.off_fph + b.cidx[x] + (i64)b.cidx[y] * b.P);
+    };
+    auto pair_code = [&](int x, int y, int J) {        // pair (x,y) as v_elem orders it
+      int neg = 0, s2 = 0, lo = x, hi = y;
+      if (x > y) { lo = y; hi = x; neg = ((1 + (b.jj[x] + b.jj[y] - J) / 2) & 1); }
+      else if (x == y) s2 = 1;
+      const int cl = b.con[lo] + b.con[hi];
+      const int idx = b.xmap[((size_t)lo * b.N + hi) * b.nJ + J / 2];
+      return idx | (cl << 22) | (neg << 25) | (s2 << 26);
+    };
+    std::vector<std::vector<Partner>> rowl(npairs), coll(npairs);
+    for (int q = 0; q < b.nblocks; ++q) {
+      const int J = b.chJ[q];
+      for (int c = 0; c < 3; ++c)
+        for (int ix = 0; ix < b.dims[q * 3 + c]; ++ix) {
+          const size_t gid = (size_t)b.pairoff[q * 3 + c] + ix;
+          const int x = b.pl[q * 3 + c][2 * ix], y = b.pl[q * 3 + c][2 * ix + 1];
+          int foff, fsw;
+          for (int k = b.spstart[b.spblk[x]]; k < b.spstart[b.spblk[x] + 1]; ++k) {
+            const int i = b.spmem[k];
+            fref(x, i, foff, fsw);
+            rowl[gid].push_back(Partner{pair_code(i, y, J) | (fsw << 24), foff});
+            fref(i, x, foff, fsw);
+            coll[gid].push_back(Partner{pair_code(i, y, J) | (fsw << 24), foff});
+          }
+          for (int k = b.spstart[b.spblk[y]]; k < b.spstart[b.spblk[y] + 1]; ++k) {
+            const int i = b.spmem[k];
+            fref(y, i, foff, fsw);
+            rowl[gid].push_back(Partner{pair_code(x, i, J) | (fsw << 24), foff});
+            fref(i, y, foff, fsw);
+            coll[gid].push_back(Partner{pair_code(x, i, J) | (fsw << 24), foff});
+          }
+        }
+    }
+    for (size_t gi = 0; gi < npairs; ++gi) {
+      B->rowp_h[gi] = (int)B->part_h.size();
+      B->part_h.insert(B->part_h.end(), rowl[gi].begin(), rowl[gi].end());
+    }
+    B->rowp_h[npairs] = (int)B->part_h.size();
+    for (size_t gi = 0; gi < npairs; ++gi) {
+      B->colp_h[gi] = (int)B->part_h.size();
+      B->part_h.insert(B->part_h.end(), coll[gi].begin(), coll[gi].end());
+    }
+    B->colp_h[npairs] = (int)B->part_h.size();
+  }
+  // 222_ph outputs as (orbit quadruple, 2J) items, J fastest: every upper
+  // element of every channel exactly once (checked against seg_tri)
+  {
+    B->quad_h.clear();
+    if (b.N > 256) return fail(IMSRG_EINVAL, "more than 256 orbits");
+    static const int rcls[6] = {0, 0, 0, 1, 2, 1}, ccls[6] = {0, 1, 2, 1, 2, 2};
+    for (int Pr = 0; Pr <= 1; ++Pr)
+      for (int T = -1; T <= 1; ++T)
+        for (int g = 0; g < 6; ++g) {
+          std::vector<int> rows, cols;
+          for (int i = 0; i < b.N; ++i)
+            for (int j = i; j < b.N; ++j) {
+              if ((b.ll[i] + b.ll[j]) % 2 != Pr || (b.tz[i] + b.tz[j]) / 2 != T) continue;
+              const int c = b.con[i] + b.con[j];
+              // key (i << 8) | j orders pairs as the channel lists do (i, then j)
+              if (c == rcls[g]) rows.push_back((i << 8) | j);
+              if (c == ccls[g]) cols.push_back((i << 8) | j);
+            }
+          for (int rp : rows) {
+            const int i = rp >> 8, j = rp & 255;
+            for (int cp : cols) {
+              if (kSquare[g] && cp < rp) continue;   // (k,l) >= (i,j): the upper triangle
+              const int k = cp >> 8, l = cp & 255;
+              const int lo = std::max(std::abs(b.jj[i] - b.jj[j]), std::abs(b.jj[k] - b.jj[l]));
+              const int hi = std::min(b.jj[i] + b.jj[j], b.jj[k] + b.jj[l]);
+              for (int J = lo; J <= hi; J += 2)
+                B->quad_h.push_back(QuadItem{i | (j << 8) | (k << 16) | (l << 24), J});
+            }
+          }
+        }
+    i64 tri = 0;
+    for (int q = 0; q < b.nblocks; ++q)
+      for (int g = 0; g < 6; ++g)
+        tri += kSquare[g] ? (i64)b.rows(q, g) * (b.rows(q, g) + 1) / 2 : b.blocksize(q, g);
+    if ((i64)B->quad_h.size() != tri) return fail(IMSRG_EINVAL, "quadruple enumeration does not cover the channels");
+  }
   // norm chunks: fhh (w 1), fph (w 2), fpp (w 1), gam blocks (w 1 square, 2 rectangular)
   B->ch_off_h.clear(); B->ch_len_h.clear(); B->ch_w_h.clear();
   auto add_range = [&](i64 off, i64 len, double w) {
@@ -277,6 +379,10 @@ int build_plan(imsrg_basis_s* B) {
   if ((rc = upload(B, B->k3.host, &B->k3.dev))) return rc;
   if ((rc = upload(B, B->k3p_h, &B->k3p))) return rc;
   if ((rc = upload(B, B->one_h, &B->one))) return rc;
+  if ((rc = upload(B, B->quad_h, &B->quad))) return rc;
+  if ((rc = upload(B, B->part_h, &B->partners))) return rc;
+  if ((rc = upload(B, B->rowp_h, &B->rowp))) return rc;
+  if ((rc = upload(B, B->colp_h, &B->colp))) return rc;
   if ((rc = upload(B, B->ch_off_h, &B->ch_off))) return rc;
   if ((rc = upload(B, B->ch_len_h, &B->ch_len))) return rc;
   if ((rc = upload(B, B->ch_w_h, &B->ch_w))) return rc;
@@ -343,7 +449,9 @@ void t_121(imsrg_basis_s* B, const double* L, int Lh, const double* R, int Rh, d
 }
 void t_122(imsrg_basis_s* B, const double* L, int Lh, const double* R, int Rh, double* RES, int RESh,
            hipStream_t s) {
-  run(B, B->seg_tri, [&](const SegTask* t, int n) { launch_comm122(B->D, t, n, L, Lh, R, Rh, RES, RESh, s); });
+  run(B, B->seg_tri, [&](const SegTask* t, int n) {
+    launch_comm122p(B->D, t, n, B->partners, B->rowp, B->colp, L, Lh, R, Rh, RES, RESh, s);
+  });
 }
 void t_pphh(imsrg_basis_s* B, const double* L, int Lh, const double* R, int Rh, double* RES, int RESh,
             double* w1, double* w2, hipStream_t s) {
@@ -372,8 +480,11 @@ void t_ph(imsrg_basis_s* B, const double* LCC, int Lh, const double* RCC, int Rh
   Slots sl{};
   sl.p[0] = LCC; sl.p[1] = RCC; sl.p[2] = WCC;
   run(B, B->ph, [&](const GemmTask* t, int n) { launch_gemm(sl, B->php, t, n, s); });
-  run(B, B->seg_tri,
-      [&](const SegTask* t, int n) { launch_comm222ph_scatter(B->D, t, n, WCC, Lh, Rh, RES, RESh, s); });
+  if (B->path == IMSRG_PATH_GROUPED)
+    launch_comm222ph_quad(B->D, B->quad, (int)B->quad_h.size(), WCC, Lh, Rh, RES, RESh, s);
+  else
+    run(B, B->seg_tri,
+        [&](const SegTask* t, int n) { launch_comm222ph_scatter(B->D, t, n, WCC, Lh, Rh, RES, RESh, s); });
 }
 // device norm of op into dst (device)
 void t_norm(imsrg_basis_s* B, const double* op, double* dst, hipStream_t s) {

@@ -77,22 +77,88 @@ void launch_comm122(const DevBasis& B, const SegTask* t, int nt, const double* L
   hipLaunchKernelGGL(k_comm122, dim3(nt), dim3(kTile), 0, s, B, t, L, Lh, R, Rh, RES, RESh);
 }
 
-// ================================================== commutator_222_ph scatter
-// commutators.f90:804-923.  Inverse Pandya of WCC = LCC RCC^T into RES.
-__global__ __launch_bounds__(256) void k_comm222ph_scatter(DevBasis B, const SegTask* tasks, const double* WCC,
-                                                           int Lh, int Rh, double* RES, int RESh) {
+// commutator_122 with precomputed partner lists: the same sums as k_comm122
+// (a-, b-, c-, d-replaced, in that order) without per-term index searches.
+__device__ __forceinline__ int blk_of(int lo, int hi) { return lo == 0 ? hi : (lo == 1 ? (hi == 1 ? 3 : 5) : 4); }
+
+__global__ __launch_bounds__(256) void k_comm122p(DevBasis B, const SegTask* tasks, const Partner* part,
+                                                  const int* rowp, const int* colp, const double* L, int Lh,
+                                                  const double* R, int Rh, double* RES, int RESh) {
   const SegTask t = tasks[blockIdx.x];
   const i64 e = (i64)t.start + threadIdx.x;
   if (e >= t.cnt) return;
   const int q = t.q, g = t.g;
-  const int n1 = brows(B, q, g);
+  const int rc = row_cls(g), cc = col_cls(g);
+  const int n1 = B.dims[q * 3 + rc];
   const bool sq = is_square(g);
   int IX, JX;
   if (sq) upper_decode(e, IX, JX);
   else { IX = (int)(e % n1); JX = (int)(e / n1); }
-  int i, j, k, l;
-  unpack(B.pairs[B.pairoff[q * 3 + row_cls(g)] + IX], i, j);
-  unpack(B.pairs[B.pairoff[q * 3 + col_cls(g)] + JX], k, l);
+  const i64 gr = B.pairoff[q * 3 + rc] + IX, gc = B.pairoff[q * 3 + cc] + JX;
+  int a, b, c, d;
+  unpack(B.pairs[gr], a, b);
+  unpack(B.pairs[gc], c, d);
+  const double pre_ab = (a == b) ? 1.4142135623730951 : 1.0;
+  const double pre_cd = (c == d) ? 1.4142135623730951 : 1.0;
+  const i64* go = B.goff + q * 6;
+  const int* dm = B.dims + q * 3;
+  double sm = 0.0;
+  for (int x = rowp[gr]; x < rowp[gr + 1]; ++x) {
+    const Partner p = part[x];
+    const int idx = p.code & 0x3fffff, cl = (p.code >> 22) & 3;
+    double pre = ((p.code >> 26) & 1) ? 1.4142135623730951 : 1.0;
+    if ((p.code >> 25) & 1) pre = -pre;
+    pre = pre * pre_cd;
+    double vr, vl;
+    if (cl <= cc) {
+      const i64 o = go[blk_of(cl, cc)] + idx + (i64)JX * dm[cl];
+      vr = R[o] * pre; vl = L[o] * pre;
+    } else {
+      const i64 o = go[blk_of(cc, cl)] + JX + (i64)idx * dm[cc];
+      vr = R[o] * Rh * pre; vl = L[o] * Lh * pre;
+    }
+    double fl = L[p.foff], fr = R[p.foff];
+    if ((p.code >> 24) & 1) { fl = fl * Lh; fr = fr * Rh; }
+    sm = sm + fl * vr - fr * vl;
+  }
+  for (int x = colp[gc]; x < colp[gc + 1]; ++x) {
+    const Partner p = part[x];
+    const int idx = p.code & 0x3fffff, cl = (p.code >> 22) & 3;
+    double pre = ((p.code >> 26) & 1) ? 1.4142135623730951 : 1.0;
+    if ((p.code >> 25) & 1) pre = -pre;
+    pre = pre_ab * pre;
+    double vr, vl;
+    if (rc <= cl) {
+      const i64 o = go[blk_of(rc, cl)] + IX + (i64)idx * dm[rc];
+      vr = R[o] * pre; vl = L[o] * pre;
+    } else {
+      const i64 o = go[blk_of(cl, rc)] + idx + (i64)IX * dm[cl];
+      vr = R[o] * Rh * pre; vl = L[o] * Lh * pre;
+    }
+    double fl = L[p.foff], fr = R[p.foff];
+    if ((p.code >> 24) & 1) { fl = fl * Lh; fr = fr * Rh; }
+    sm = sm - fl * vr + fr * vl;
+  }
+  sm = sm / sqrt(1.0 + (a == b ? 1.0 : 0.0)) / sqrt(1.0 + (c == d ? 1.0 : 0.0));
+  const i64 base = go[g];
+  RES[base + IX + (i64)JX * n1] = sm;
+  if (sq) RES[base + JX + (i64)IX * n1] = sm * RESh;
+}
+
+void launch_comm122p(const DevBasis& B, const SegTask* t, int nt, const Partner* part, const int* rowp,
+                     const int* colp, const double* L, int Lh, const double* R, int Rh, double* RES, int RESh,
+                     hipStream_t s) {
+  if (nt <= 0) return;
+  hipLaunchKernelGGL(k_comm122p, dim3(nt), dim3(kTile), 0, s, B, t, part, rowp, colp, L, Lh, R, Rh, RES, RESh);
+}
+
+// ================================================== commutator_222_ph scatter
+// commutators.f90:804-923.  Inverse Pandya of WCC = LCC RCC^T into element
+// (IX, JX) of block g of channel q, whose pairs are (i,j) and (k,l).
+__device__ __forceinline__ void ph_element(const DevBasis& B, const double* WCC, int Lh, int Rh, double* RES,
+                                           int RESh, int q, int g, int IX, int JX, int i, int j, int k, int l) {
+  const int n1 = brows(B, q, g);
+  const bool sq = is_square(g);
   const double pre = (i == j) ? IMSRG_INV_SQRT2_REF : 1.0;
   const double pre2 = (k == l) ? IMSRG_INV_SQRT2_REF : 1.0;
   const int Jtot = B.chJ[q];
@@ -146,10 +212,52 @@ __global__ __launch_bounds__(256) void k_comm222ph_scatter(DevBasis B, const Seg
   if (sq) X[JX + (i64)IX * n1] = X[o] * RESh;
 }
 
+// per-channel form: segment tiles of one channel (plain path)
+__global__ __launch_bounds__(256) void k_comm222ph_scatter(DevBasis B, const SegTask* tasks, const double* WCC,
+                                                           int Lh, int Rh, double* RES, int RESh) {
+  const SegTask t = tasks[blockIdx.x];
+  const i64 e = (i64)t.start + threadIdx.x;
+  if (e >= t.cnt) return;
+  const int q = t.q, g = t.g;
+  const int n1 = brows(B, q, g);
+  int IX, JX;
+  if (is_square(g)) upper_decode(e, IX, JX);
+  else { IX = (int)(e % n1); JX = (int)(e / n1); }
+  int i, j, k, l;
+  unpack(B.pairs[B.pairoff[q * 3 + row_cls(g)] + IX], i, j);
+  unpack(B.pairs[B.pairoff[q * 3 + col_cls(g)] + JX], k, l);
+  ph_element(B, WCC, Lh, Rh, RES, RESh, q, g, IX, JX, i, j, k, l);
+}
+
 void launch_comm222ph_scatter(const DevBasis& B, const SegTask* t, int nt, const double* W, int Lh, int Rh,
                               double* RES, int RESh, hipStream_t s) {
   if (nt <= 0) return;
   hipLaunchKernelGGL(k_comm222ph_scatter, dim3(nt), dim3(kTile), 0, s, B, t, W, Lh, Rh, RES, RESh);
+}
+
+// grouped form: one thread per (orbit quadruple, J), quadruple-major with
+// J fastest, so the lanes of one quadruple read identical WCC addresses
+// (one request per wave load) and neighbouring 6j symbols.
+__global__ __launch_bounds__(256) void k_comm222ph_quad(DevBasis B, const QuadItem* items, int n, const double* WCC,
+                                                        int Lh, int Rh, double* RES, int RESh) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const QuadItem it = items[t];
+  const int i = it.ijkl & 255, j = (it.ijkl >> 8) & 255, k = (it.ijkl >> 16) & 255, l = (it.ijkl >> 24) & 255;
+  const int J = it.J;
+  const int P = (B.ll[i] + B.ll[j]) & 1;
+  const int q = 3 * J + 2 * ((B.tz[i] + B.tz[j]) / 2 + 1) + P;
+  const int rc = B.con[i] + B.con[j], cc = B.con[k] + B.con[l];
+  const int g = (rc == 0) ? cc : (rc == 1 ? (cc == 1 ? 3 : 5) : 4);
+  const int IX = B.xmap[((i64)i * B.N + j) * B.nJ + (J >> 1)];
+  const int JX = B.xmap[((i64)k * B.N + l) * B.nJ + (J >> 1)];
+  ph_element(B, WCC, Lh, Rh, RES, RESh, q, g, IX, JX, i, j, k, l);
+}
+
+void launch_comm222ph_quad(const DevBasis& B, const QuadItem* items, int n, const double* W, int Lh, int Rh,
+                           double* RES, int RESh, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_comm222ph_quad, dim3((n + 255) / 256), dim3(256), 0, s, B, items, n, W, Lh, Rh, RES, RESh);
 }
 
 // =========================================================== Pandya forward
@@ -249,36 +357,11 @@ void launch_comm111(const DevBasis& B, const double* L, int Lh, const double* R,
   hipLaunchKernelGGL(k_comm111, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, B, L, Lh, R, Rh, RES);
 }
 
-// ============================================================ commutator_121
-// commutators.f90:83-280 (accumulates).
-__global__ __launch_bounds__(64) void k_comm121(DevBasis B, const OneTask* tasks, int nt, const double* L,
-                                                int Lh, const double* R, int Rh, double* RES, int RESh) {
-  const int tix = blockIdx.x * blockDim.x + threadIdx.x;
-  if (tix >= nt) return;
-  const OneTask t = tasks[tix];
-  const int pk = t.pk, qk = t.qk, jq = B.jj[qk];
-  const double* Lph = L + B.off_fph;
-  const double* Rph = R + B.off_fph;
-  double sm = 0.0;
-  for (int a = 0; a < B.P; ++a) {
-    const int ak = B.parts[a];
-    for (int i = 0; i < B.A; ++i) {
-      const int ik = B.holes[i];
-      if (B.jj[ik] != B.jj[ak] || B.ll[ik] != B.ll[ak] || B.tz[ik] != B.tz[ak]) continue;
-      const int ji = B.jj[ik];
-      double smx = 0, smy = 0, smx2 = 0, smy2 = 0;
-      for (int JT = abs(ji - jq); JT <= ji + jq; JT += 2) {
-        const VPos p1 = vpos(B, ak, pk, ik, qk, JT);
-        smx = smx + vat(p1, R, Rh) * (JT + 1);
-        smx2 = smx2 + vat(p1, L, Lh) * (JT + 1);
-        const VPos p2 = vpos(B, ik, pk, ak, qk, JT);
-        smy = smy + vat(p2, R, Rh) * (JT + 1);
-        smy2 = smy2 + vat(p2, L, Lh) * (JT + 1);
-      }
-      sm = sm + Lph[a + (i64)i * B.P] * (Lh * smx - smy) - Rph[a + (i64)i * B.P] * (Rh * smx2 - smy2);
-    }
-  }
-  const double add = sm / (B.jj[pk] + 1.0);
+__device__ double block_sum(double v, double* red);
+
+// one-body result of 121 / 221: RES(p,q) += add and the herm-symmetric fill
+__device__ __forceinline__ void onebody_update(const DevBasis& B, const OneTask& t, double add, double* RES,
+                                               int RESh) {
   if (t.kind == 0) {
     double* F = RES + B.off_fhh;
     const i64 o = t.pi + (i64)t.qi * B.A;
@@ -294,61 +377,76 @@ __global__ __launch_bounds__(64) void k_comm121(DevBasis B, const OneTask* tasks
     const i64 o = t.pi + (i64)t.qi * B.P;
     F[o] = F[o] + add;
   }
+}
+
+// ============================================================ commutator_121
+// commutators.f90:83-280 (accumulates).  One workgroup per one-body output;
+// its threads split the (particle a, hole i) intermediates, tree-reduced.
+__global__ __launch_bounds__(128) void k_comm121(DevBasis B, const OneTask* tasks, const double* L, int Lh,
+                                                 const double* R, int Rh, double* RES, int RESh) {
+  __shared__ double red[128];
+  const OneTask t = tasks[blockIdx.x];
+  const int pk = t.pk, qk = t.qk, jq = B.jj[qk];
+  const double* Lph = L + B.off_fph;
+  const double* Rph = R + B.off_fph;
+  double v = 0.0;
+  for (int x = threadIdx.x; x < B.P * B.A; x += blockDim.x) {
+    const int a = x / B.A, i = x % B.A;
+    const int ak = B.parts[a], ik = B.holes[i];
+    if (B.jj[ik] != B.jj[ak] || B.ll[ik] != B.ll[ak] || B.tz[ik] != B.tz[ak]) continue;
+    const int ji = B.jj[ik];
+    double smx = 0, smy = 0, smx2 = 0, smy2 = 0;
+    for (int JT = abs(ji - jq); JT <= ji + jq; JT += 2) {
+      const VPos p1 = vpos(B, ak, pk, ik, qk, JT);
+      smx = smx + vat(p1, R, Rh) * (JT + 1);
+      smx2 = smx2 + vat(p1, L, Lh) * (JT + 1);
+      const VPos p2 = vpos(B, ik, pk, ak, qk, JT);
+      smy = smy + vat(p2, R, Rh) * (JT + 1);
+      smy2 = smy2 + vat(p2, L, Lh) * (JT + 1);
+    }
+    v = v + Lph[a + (i64)i * B.P] * (Lh * smx - smy) - Rph[a + (i64)i * B.P] * (Rh * smx2 - smy2);
+  }
+  const double sm = block_sum(v, red);
+  if (threadIdx.x == 0) onebody_update(B, t, sm / (B.jj[pk] + 1.0), RES, RESh);
 }
 
 void launch_comm121(const DevBasis& B, const OneTask* t, int nt, const double* L, int Lh, const double* R,
                     int Rh, double* RES, int RESh, hipStream_t s) {
   if (nt <= 0) return;
-  hipLaunchKernelGGL(k_comm121, dim3((nt + 63) / 64), dim3(64), 0, s, B, t, nt, L, Lh, R, Rh, RES, RESh);
+  hipLaunchKernelGGL(k_comm121, dim3(nt), dim3(128), 0, s, B, t, L, Lh, R, Rh, RES, RESh);
 }
 
 // ============================================================ commutator_221
 // commutators.f90:435-578 (accumulates; reads w1/w2 of 222_pp_hh, herm 1).
-__global__ __launch_bounds__(64) void k_comm221(DevBasis B, const OneTask* tasks, int nt, int pm,
-                                                const double* w1, const double* w2, double* RES, int RESh) {
-  const int tix = blockIdx.x * blockDim.x + threadIdx.x;
-  if (tix >= nt) return;
-  const OneTask t = tasks[tix];
+// One workgroup per one-body output; thread x takes intermediate orbit c
+// (holes first with w1, then particles with w2) and its J sum.
+__global__ __launch_bounds__(128) void k_comm221(DevBasis B, const OneTask* tasks, int pm, const double* w1,
+                                                 const double* w2, double* RES, int RESh) {
+  __shared__ double red[128];
+  const OneTask t = tasks[blockIdx.x];
   const int ik = t.pk, jk = t.qk, ji = B.jj[ik];
   const bool both = (t.kind != 2);
-  double sm = 0.0;
-  for (int pass = 0; pass < 2; ++pass) {
-    const double* w = pass == 0 ? w1 : w2;   // holes c with w1, then particles c with w2
-    const int nc = pass == 0 ? B.A : B.P;
-    const int* cl = pass == 0 ? B.holes : B.parts;
-    for (int c = 0; c < nc; ++c) {
-      const int ck = cl[c];
-      const int jc = B.jj[ck];
-      for (int JT = abs(jc - ji); JT <= jc + ji; JT += 2) {
-        if (both)
-          sm = sm + (v_elem(B, w, 1, ck, ik, ck, jk, JT) - pm * v_elem(B, w, 1, ck, jk, ck, ik, JT)) * (JT + 1);
-        else
-          sm = sm + v_elem(B, w, 1, ck, ik, ck, jk, JT) * (JT + 1);
-      }
+  double v = 0.0;
+  for (int x = threadIdx.x; x < B.N; x += blockDim.x) {
+    const bool hole = x < B.A;
+    const int ck = hole ? B.holes[x] : B.parts[x - B.A];
+    const double* w = hole ? w1 : w2;
+    const int jc = B.jj[ck];
+    for (int JT = abs(jc - ji); JT <= jc + ji; JT += 2) {
+      if (both)
+        v = v + (v_elem(B, w, 1, ck, ik, ck, jk, JT) - pm * v_elem(B, w, 1, ck, jk, ck, ik, JT)) * (JT + 1);
+      else
+        v = v + v_elem(B, w, 1, ck, ik, ck, jk, JT) * (JT + 1);
     }
   }
-  const double add = sm / (ji + 1.0);
-  if (t.kind == 0) {
-    double* F = RES + B.off_fhh;
-    const i64 o = t.pi + (i64)t.qi * B.A;
-    F[o] = F[o] + add;
-    F[t.qi + (i64)t.pi * B.A] = F[o] * RESh;
-  } else if (t.kind == 1) {
-    double* F = RES + B.off_fpp;
-    const i64 o = t.pi + (i64)t.qi * B.P;
-    F[o] = F[o] + add;
-    F[t.qi + (i64)t.pi * B.P] = F[o] * RESh;
-  } else {
-    double* F = RES + B.off_fph;
-    const i64 o = t.pi + (i64)t.qi * B.P;
-    F[o] = F[o] + add;
-  }
+  const double sm = block_sum(v, red);
+  if (threadIdx.x == 0) onebody_update(B, t, sm / (ji + 1.0), RES, RESh);
 }
 
 void launch_comm221(const DevBasis& B, const OneTask* t, int nt, int pm, const double* w1, const double* w2,
                     double* RES, int RESh, hipStream_t s) {
   if (nt <= 0) return;
-  hipLaunchKernelGGL(k_comm221, dim3((nt + 63) / 64), dim3(64), 0, s, B, t, nt, pm, w1, w2, RES, RESh);
+  hipLaunchKernelGGL(k_comm221, dim3(nt), dim3(128), 0, s, B, t, pm, w1, w2, RES, RESh);
 }
 
 // ================================================= commutator_110 + 220 (E0)

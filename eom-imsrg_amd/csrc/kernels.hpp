@@ -18,6 +18,17 @@ struct SegTask { int q; int g; int start; int cnt; };
 // plus their indices in fhh / fpp / fph.
 struct OneTask { int kind; int pk; int qk; int pi; int qi; };
 
+// One two-body output element named by its orbits (i | j<<8 | k<<16 | l<<24,
+// pairs i<=j, k<=l) and its 2J.
+struct QuadItem { int ijkl; int J; };
+
+// A pair reached from a stored pair by replacing one orbit inside its
+// (l, j, tz) block (the 1b x 2b sums of commutator_122).  code: index in
+// its class list (bits 0-21) | class << 22 | f stored transposed << 24 |
+// negative phase << 25 | sqrt(2) factor << 26.  foff: offset of the
+// one-body element in the operator.
+struct Partner { int code; int foff; };
+
 // Tile of up to kTile CCX elements of cross-coupled block q1
 struct CCTask { int q1; int start; };
 
@@ -56,6 +67,11 @@ void launch_comm122(const DevBasis& B, const SegTask* t, int nt, const double* L
                     int Rh, double* RES, int RESh, hipStream_t s);
 void launch_comm222ph_scatter(const DevBasis& B, const SegTask* t, int nt, const double* W, int Lh, int Rh,
                               double* RES, int RESh, hipStream_t s);
+void launch_comm222ph_quad(const DevBasis& B, const QuadItem* items, int n, const double* W, int Lh, int Rh,
+                           double* RES, int RESh, hipStream_t s);
+void launch_comm122p(const DevBasis& B, const SegTask* t, int nt, const Partner* part, const int* rowp,
+                     const int* colp, const double* L, int Lh, const double* R, int Rh, double* RES, int RESh,
+                     hipStream_t s);
 void launch_pandya(const DevBasis& B, const CCTask* t, int nt, const double* op, int herm, double* CC,
                    hipStream_t s);
 void launch_comm111(const DevBasis& B, const double* L, int Lh, const double* R, int Rh, double* RES,
