@@ -62,9 +62,13 @@ struct imsrg_basis_s {
   int* ch_len = nullptr;
   double* ch_w = nullptr;
   // scratch
-  std::vector<i64> poff_h;
+  std::vector<i64> poff_h, k2off_h;   // split-K partial slabs of K3 (P) and K2, per channel
+  std::vector<int> k3ks_h, k2ks_h;
   i64* poff = nullptr;
+  i64* k2off = nullptr;
+  int *k3ks = nullptr, *k2ks = nullptr;
   double* Pscr = nullptr;
+  double* K2scr = nullptr;
   double* part = nullptr;
   int npart = 0;
   double* M = nullptr;       // monopole table N x N
@@ -193,12 +197,19 @@ int build_plan(imsrg_basis_s* B) {
   B->k2.host.clear(); B->k2.chan_begin.assign(1, 0);
   B->k3.host.clear(); B->k3.chan_begin.assign(1, 0);
   B->k2p_h.clear(); B->k3p_h.clear();
-  B->poff_h.assign(b.nblocks, 0);
-  i64 pofs = 0;
+  // split K (deterministic: partial slabs summed in order by the epilogue)
+  const int KC2 = 128, KC3 = 64;
+  B->poff_h.assign(b.nblocks, 0); B->k3ks_h.assign(b.nblocks, 1);
+  B->k2off_h.assign(b.nblocks, 0); B->k2ks_h.assign(b.nblocks, 1);
+  i64 pofs = 0, k2ofs = 0;
   for (int q = 0; q < b.nblocks; ++q) {
-    const int nb = b.dims[q * 3 + 1], nh = b.dims[q * 3 + 2];
+    const int np = b.dims[q * 3], nb = b.dims[q * 3 + 1], nh = b.dims[q * 3 + 2];
+    B->k3ks_h[q] = std::max(1, (np + KC3 - 1) / KC3);
     B->poff_h[q] = pofs;
-    pofs += (i64)(nb + nh) * (nb + nh);
+    pofs += (i64)B->k3ks_h[q] * (nb + nh) * (nb + nh);
+    B->k2ks_h[q] = std::max(1, (np + KC2 - 1) / KC2);
+    B->k2off_h[q] = k2ofs;
+    k2ofs += (i64)B->k2ks_h[q] * np * (nb + nh);
   }
   for (int q : order) {
     const int np = b.dims[q * 3], nb = b.dims[q * 3 + 1], nh = b.dims[q * 3 + 2];
@@ -210,16 +221,18 @@ int build_plan(imsrg_basis_s* B) {
       GemmProb p{};
       p.a1 = b.goff[q * 6 + 0]; p.b1 = b.goff[q * 6 + 1];
       p.a2 = b.goff[q * 6 + 0]; p.b2 = b.goff[q * 6 + 1];
-      p.c = b.goff[q * 6 + 1];
+      p.c = B->k2off_h[q];
       p.m = np; p.n = nb + nh; p.k = np;
       p.lda = np; p.ldb = np; p.ldc = np;
       p.nprod = 2;
       p.sa1 = 0; p.sb1 = 1; p.sa2 = 1; p.sb2 = 0; p.sc = 2;
       p.alpha = 1.0; p.s2 = -1.0; p.beta = 0.0;
+      p.kchunk = KC2; p.cstride = (i64)np * (nb + nh);
       const int pid = (int)B->k2p_h.size();
       B->k2p_h.push_back(p);
-      for (int tm = 0; tm < tiles(np); ++tm)
-        for (int tn = 0; tn < tiles(nb + nh); ++tn) B->k2.host.push_back(GemmTask{pid, tm, tn, 0});
+      for (int ks = 0; ks < B->k2ks_h[q]; ++ks)
+        for (int tm = 0; tm < tiles(np); ++tm)
+          for (int tn = 0; tn < tiles(nb + nh); ++tn) B->k2.host.push_back(GemmTask{pid, tm, tn, ks});
     }
     B->k2.chan_begin.push_back((int)B->k2.host.size());
     if (nb + nh > 0) {
@@ -231,10 +244,12 @@ int build_plan(imsrg_basis_s* B) {
       p.nprod = 1; p.ta1 = 1; p.tb1 = 0;
       p.sa1 = 0; p.sb1 = 1; p.sc = 2;
       p.alpha = 1.0; p.s2 = 0.0; p.beta = 0.0;
+      p.kchunk = KC3; p.cstride = (i64)(nb + nh) * (nb + nh);
       const int pid = (int)B->k3p_h.size();
       B->k3p_h.push_back(p);
-      for (int tm = 0; tm < tiles(nb + nh); ++tm)
-        for (int tn = 0; tn < tiles(nb + nh); ++tn) B->k3.host.push_back(GemmTask{pid, tm, tn, 0});
+      for (int ks = 0; ks < B->k3ks_h[q]; ++ks)
+        for (int tm = 0; tm < tiles(nb + nh); ++tm)
+          for (int tn = 0; tn < tiles(nb + nh); ++tn) B->k3.host.push_back(GemmTask{pid, tm, tn, ks});
     }
     B->k3.chan_begin.push_back((int)B->k3.host.size());
   }
@@ -387,7 +402,11 @@ int build_plan(imsrg_basis_s* B) {
   if ((rc = upload(B, B->ch_len_h, &B->ch_len))) return rc;
   if ((rc = upload(B, B->ch_w_h, &B->ch_w))) return rc;
   if ((rc = upload(B, B->poff_h, &B->poff))) return rc;
+  if ((rc = upload(B, B->k2off_h, &B->k2off))) return rc;
+  if ((rc = upload(B, B->k3ks_h, &B->k3ks))) return rc;
+  if ((rc = upload(B, B->k2ks_h, &B->k2ks))) return rc;
   if ((rc = dalloc(B, std::max<i64>(pofs, 1) * sizeof(double), (void**)&B->Pscr))) return rc;
+  if ((rc = dalloc(B, std::max<i64>(k2ofs, 1) * sizeof(double), (void**)&B->K2scr))) return rc;
   B->npart = std::max<int>(std::max<int>((int)B->ch_off_h.size(), B->seg_g3.n), b.nblocks + 1);
   if ((rc = dalloc(B, (size_t)B->npart * sizeof(double), (void**)&B->part))) return rc;
   if ((rc = dalloc(B, (size_t)b.N * b.N * sizeof(double), (void**)&B->M))) return rc;
@@ -459,13 +478,14 @@ void t_pphh(imsrg_basis_s* B, const double* L, int Lh, const double* R, int Rh, 
   run(B, B->pairs,
       [&](const PairTask* t, int n) { launch_pppp_pairs(B->D, t, n, L, Lh, R, Rh, RES, w1, w2, 1, s); });
   Slots s2{};
-  s2.p[0] = L; s2.p[1] = R; s2.p[2] = w1;
+  s2.p[0] = L; s2.p[1] = R; s2.p[2] = B->K2scr;
   run(B, B->k2, [&](const GemmTask* t, int n) { launch_gemm(s2, B->k2p, t, n, s); });
   Slots s3{};
   s3.p[0] = L; s3.p[1] = R; s3.p[2] = B->Pscr;
   run(B, B->k3, [&](const GemmTask* t, int n) { launch_gemm(s3, B->k3p, t, n, s); });
   run(B, B->seg_epi, [&](const SegTask* t, int n) {
-    launch_pphh_epilogue(B->D, t, n, L, Lh, R, Rh, RES, w1, w2, B->Pscr, B->poff, s);
+    launch_pphh_epilogue(B->D, t, n, L, Lh, R, Rh, RES, w1, w2, B->Pscr, B->poff, B->k3ks, B->K2scr, B->k2off,
+                         B->k2ks, s);
   });
 }
 void t_221(imsrg_basis_s* B, int Lh, int Rh, double* RES, int RESh, const double* w1, const double* w2,
@@ -998,6 +1018,75 @@ int imsrg_magnus_decouple(imsrg_flow F, double* HS, double* G, double ds, double
     }
   }
   if (steps) *steps = st;
+  return 0;
+}
+
+int imsrg_profile_kernel(imsrg_basis B, int which, const double* L, int Lh, const double* R, int Rh, double* RES,
+                         int RESh, double* w1, double* w2, double* LCC, double* RCC, double* WCC, int iters,
+                         double* ms, imsrg_stream stream) {
+  PREP();
+  if (iters < 1 || which < 0 || which > IMSRG_K_COMMUTATOR) return fail(IMSRG_EINVAL, "bad profile request");
+  auto body = [&](int k) {
+    switch (k) {
+      case IMSRG_K_PPPP:
+        run(B, B->pairs, [&](const PairTask* t, int n) { launch_pppp_pairs(B->D, t, n, L, Lh, R, Rh, RES, w1, w2, 1, s); });
+        break;
+      case IMSRG_K_PPHH_SMALL: {
+        Slots s2{};
+        s2.p[0] = L; s2.p[1] = R; s2.p[2] = B->K2scr;
+        run(B, B->k2, [&](const GemmTask* t, int n) { launch_gemm(s2, B->k2p, t, n, s); });
+        Slots s3{};
+        s3.p[0] = L; s3.p[1] = R; s3.p[2] = B->Pscr;
+        run(B, B->k3, [&](const GemmTask* t, int n) { launch_gemm(s3, B->k3p, t, n, s); });
+        run(B, B->seg_epi, [&](const SegTask* t, int n) {
+          launch_pphh_epilogue(B->D, t, n, L, Lh, R, Rh, RES, w1, w2, B->Pscr, B->poff, B->k3ks, B->K2scr, B->k2off,
+                               B->k2ks, s);
+        });
+        break;
+      }
+      case IMSRG_K_PH_GEMM: {
+        Slots sl{};
+        sl.p[0] = LCC; sl.p[1] = RCC; sl.p[2] = WCC;
+        run(B, B->ph, [&](const GemmTask* t, int n) { launch_gemm(sl, B->php, t, n, s); });
+        break;
+      }
+      case IMSRG_K_PH_SCATTER:
+        if (B->path == IMSRG_PATH_GROUPED)
+          launch_comm222ph_quad(B->D, B->quad, (int)B->quad_h.size(), WCC, Lh, Rh, RES, RESh, s);
+        else
+          run(B, B->seg_tri,
+              [&](const SegTask* t, int n) { launch_comm222ph_scatter(B->D, t, n, WCC, Lh, Rh, RES, RESh, s); });
+        break;
+      case IMSRG_K_122: t_122(B, L, Lh, R, Rh, RES, RESh, s); break;
+      case IMSRG_K_PANDYA: t_pandya(B, R, Rh, RCC, s); break;
+      case IMSRG_K_ONEBODY:
+        t_111(B, L, Lh, R, Rh, RES, s);
+        t_121(B, L, Lh, R, Rh, RES, RESh, s);
+        t_221(B, Lh, Rh, RES, RESh, w1, w2, s);
+        break;
+      case IMSRG_K_COMMUTATOR:
+        t_pandya(B, L, Lh, LCC, s);
+        t_full(B, L, Lh, R, Rh, RES, RESh, true, w1, w2, LCC, RCC, WCC, s);
+        break;
+    }
+  };
+  // inputs of every class once (w1/w2, LCC/RCC, WCC), then the timed loop
+  t_pandya(B, L, Lh, LCC, s);
+  t_full(B, L, Lh, R, Rh, RES, RESh, true, w1, w2, LCC, RCC, WCC, s);
+  body(which);
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  HIPCHK(hipEventRecord(e0, s));
+  for (int it = 0; it < iters; ++it) body(which);
+  HIPCHK(hipEventRecord(e1, s));
+  HIPCHK(hipEventSynchronize(e1));
+  float t = 0.f;
+  HIPCHK(hipEventElapsedTime(&t, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  HIPCHK(hipGetLastError());
+  *ms = (double)t / iters;
   return 0;
 }
 

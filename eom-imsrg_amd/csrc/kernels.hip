@@ -554,16 +554,23 @@ __device__ __forceinline__ void load_b(double (*Bs)[BN + 1], const double* Bm, i
   }
 }
 
-// acc[2][2] += op(A)[m0:m0+64, :] * op(B)[:, n0:n0+64]
+// acc[2][2] += op(A)[m0:m0+64, k0:k1] * op(B)[k0:k1, n0:n0+64].  The MFMA
+// is fed op(B)^T as its A operand and op(A)^T as its B operand, so register
+// r of lane l holds C[m0 + wm*32 + mi*16 + l%16][n0 + wn*32 + ni*16 + l/16 + 4r]:
+// the 16 lanes of a row group hold 16 consecutive rows of one column, which
+// makes the column-major stores of C contiguous.
+__device__ __forceinline__ int c_row(int wm, int mi, int lane, int r) { (void)r; return wm * 32 + mi * 16 + (lane & 15); }
+__device__ __forceinline__ int c_col(int wn, int ni, int lane, int r) { return wn * 32 + ni * 16 + (lane >> 4) + 4 * r; }
+
 __device__ __forceinline__ void tile_mma(v4d (&acc)[2][2], double (*As)[BM + 1], double (*Bs)[BN + 1],
                                          const double* A, int lda, int tA, const double* Bm, int ldb, int tB, int M,
-                                         int N, int K, int m0, int n0, double scale) {
+                                         int N, int k0s, int k1, int m0, int n0, double scale) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wm = wave >> 1, wn = wave & 1;
   const int lr = lane & 15, lk = lane >> 4;
-  for (int k0 = 0; k0 < K; k0 += BK) {
-    load_a(As, A, lda, tA, M, K, m0, k0, scale);
-    load_b(Bs, Bm, ldb, tB, N, K, n0, k0);
+  for (int k0 = k0s; k0 < k1; k0 += BK) {
+    load_a(As, A, lda, tA, M, k1, m0, k0, scale);
+    load_b(Bs, Bm, ldb, tB, N, k1, n0, k0);
     __syncthreads();
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 4) {
@@ -571,10 +578,10 @@ __device__ __forceinline__ void tile_mma(v4d (&acc)[2][2], double (*As)[BM + 1],
       const double a1 = As[kk + lk][wm * 32 + 16 + lr];
       const double b0 = Bs[kk + lk][wn * 32 + lr];
       const double b1 = Bs[kk + lk][wn * 32 + 16 + lr];
-      acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
-      acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
-      acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
-      acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+      acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(b0, a0, acc[0][0], 0, 0, 0);
+      acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(b1, a0, acc[0][1], 0, 0, 0);
+      acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(b0, a1, acc[1][0], 0, 0, 0);
+      acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(b1, a1, acc[1][1], 0, 0, 0);
     }
     __syncthreads();
   }
@@ -586,17 +593,20 @@ __global__ __launch_bounds__(256) void k_gemm(Slots sl, const GemmProb* probs, c
   const GemmTask t = tasks[blockIdx.x];
   const GemmProb pr = probs[t.prob];
   const int m0 = t.tm * BM, n0 = t.tn * BN;
+  // split K: chunk t.ks covers [ks*kchunk, (ks+1)*kchunk) and lands in slab ks of C
+  const int kb = pr.kchunk > 0 ? t.ks * pr.kchunk : 0;
+  const int ke = pr.kchunk > 0 ? min(pr.k, kb + pr.kchunk) : pr.k;
   v4d acc[2][2];
 #pragma unroll
   for (int a = 0; a < 2; ++a)
 #pragma unroll
     for (int b = 0; b < 2; ++b) acc[a][b] = v4d{0.0, 0.0, 0.0, 0.0};
   tile_mma(acc, As, Bs, sl.p[pr.sa1] + pr.a1, pr.lda, pr.ta1, sl.p[pr.sb1] + pr.b1, pr.ldb, pr.tb1, pr.m, pr.n,
-           pr.k, m0, n0, 1.0);
+           kb, ke, m0, n0, 1.0);
   if (pr.nprod == 2)
     tile_mma(acc, As, Bs, sl.p[pr.sa2] + pr.a2, pr.lda, pr.ta2, sl.p[pr.sb2] + pr.b2, pr.ldb, pr.tb2, pr.m, pr.n,
-             pr.k, m0, n0, pr.s2);
-  double* C = const_cast<double*>(sl.p[pr.sc]) + pr.c;
+             kb, ke, m0, n0, pr.s2);
+  double* C = const_cast<double*>(sl.p[pr.sc]) + pr.c + (i64)t.ks * pr.cstride;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wm = wave >> 1, wn = wave & 1;
 #pragma unroll
@@ -605,8 +615,8 @@ __global__ __launch_bounds__(256) void k_gemm(Slots sl, const GemmProb* probs, c
     for (int ni = 0; ni < 2; ++ni)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int row = m0 + wm * 32 + mi * 16 + (lane >> 4) + 4 * r;
-        const int col = n0 + wn * 32 + ni * 16 + (lane & 15);
+        const int row = m0 + c_row(wm, mi, lane, r);
+        const int col = n0 + c_col(wn, ni, lane, r);
         if (row < pr.m && col < pr.n) {
           const i64 o = row + (i64)col * pr.ldc;
           const double v = pr.alpha * acc[mi][ni][r];
@@ -652,14 +662,14 @@ __global__ __launch_bounds__(256) void k_pppp_pairs(DevBasis B, const PairTask* 
   for (int a = 0; a < 2; ++a)
 #pragma unroll
     for (int b = 0; b < 2; ++b) acc[a][b] = v4d{0.0, 0.0, 0.0, 0.0};
-  tile_mma(acc, As, Bs, L1, np, 0, R1, np, 0, np, np, np, I0, J0, 1.0);
+  tile_mma(acc, As, Bs, L1, np, 0, R1, np, 0, np, np, 0, np, I0, J0, 1.0);
 #pragma unroll
   for (int mi = 0; mi < 2; ++mi)
 #pragma unroll
     for (int ni = 0; ni < 2; ++ni)
 #pragma unroll
       for (int r = 0; r < 4; ++r)
-        S1[wm * 32 + mi * 16 + (lane >> 4) + 4 * r][wn * 32 + ni * 16 + (lane & 15)] = acc[mi][ni][r];
+        S1[c_row(wm, mi, lane, r)][c_col(wn, ni, lane, r)] = acc[mi][ni][r];
   __syncthreads();
   const bool diag = (t.ti == t.tj);
   if (!diag) {
@@ -668,7 +678,7 @@ __global__ __launch_bounds__(256) void k_pppp_pairs(DevBasis B, const PairTask* 
     for (int a = 0; a < 2; ++a)
 #pragma unroll
       for (int b = 0; b < 2; ++b) acc[a][b] = v4d{0.0, 0.0, 0.0, 0.0};
-    tile_mma(acc, As, Bs, L1, np, 0, R1, np, 0, np, np, np, J0, I0, 1.0);
+    tile_mma(acc, As, Bs, L1, np, 0, R1, np, 0, np, np, 0, np, J0, I0, 1.0);
   }
   double* X1 = RES + o1;
   double* W11 = w1 + o1;
@@ -681,8 +691,8 @@ __global__ __launch_bounds__(256) void k_pppp_pairs(DevBasis B, const PairTask* 
       for (int r = 0; r < 4; ++r) {
         // register element: row y of the (J,I) tile, column x (for a
         // diagonal tile: row y, column x of the (I,I) tile)
-        const int y = wm * 32 + mi * 16 + (lane >> 4) + 4 * r;
-        const int x = wn * 32 + ni * 16 + (lane & 15);
+        const int y = c_row(wm, mi, lane, r);
+        const int x = c_col(wn, ni, lane, r);
         const int gy = J0 + y, gx = I0 + x;
         if (gy >= np || gx >= np) continue;
         const double w1yx = diag ? S1[y][x] : acc[mi][ni][r];
@@ -713,9 +723,18 @@ void launch_pppp_pairs(const DevBasis& B, const PairTask* t, int nt, const doubl
 //   g == 1: region gam2|gam3 (np x (nb+nh)); w1 already holds
 //           L1 [R2|R3] - R1 [L2|L3]; adds the rank-nh parts W2 and RES.
 //   g == 3 / 4 / 5: gam4 / gam5 / gam6 from P = [L2|L3]^T [R2|R3].
+// P = [L2|L3]^T [R2|R3] (nbh x nbh) element (x, y): sum of the split-K slabs
+__device__ __forceinline__ double pval(const double* Pq, int nbh, int ks, int x, int y) {
+  double v = 0.0;
+  for (int s = 0; s < ks; ++s) v += Pq[(i64)s * nbh * nbh + x + (i64)y * nbh];
+  return v;
+}
+
 __global__ __launch_bounds__(256) void k_pphh_epilogue(DevBasis B, const SegTask* tasks, const double* L, int Lh,
                                                        const double* R, int Rh, double* RES, double* w1,
-                                                       double* w2, const double* Pscr, const i64* poff) {
+                                                       double* w2, const double* Pscr, const i64* poff,
+                                                       const int* k3ks, const double* K2scr, const i64* k2off,
+                                                       const int* k2ks) {
   const SegTask t = tasks[blockIdx.x];
   const i64 e = (i64)t.start + threadIdx.x;
   if (e >= t.cnt) return;
@@ -744,13 +763,20 @@ __global__ __launch_bounds__(256) void k_pphh_epilogue(DevBasis B, const SegTask
       }
       w2v = s1 + (-1.0) * s2;
     }
+    // W1 = L1 [R2|R3] - R1 [L2|L3]: sum of the split-K slabs
+    double w1v = 0.0;
+    const double* K2q = K2scr + k2off[q];
+    const i64 slab = (i64)np * (nb + nh);
+    for (int s = 0; s < k2ks[q]; ++s) w1v += K2q[s * slab + e];
     const i64 o = go[1] + e;
-    RES[o] = RES[o] + w1[o] - w2v;
+    RES[o] = RES[o] + w1v - w2v;
+    w1[o] = w1v;
     w2[o] = w2v;
     return;
   }
   const int nbh = nb + nh;
   const double* Pq = Pscr + poff[q];
+  const int ks = k3ks[q];
   if (t.g == 3) {   // gam4, nb x nb
     const int x = (int)(e % nb), y = (int)(e / nb);
     double sxy = 0.0, syx = 0.0;
@@ -758,7 +784,7 @@ __global__ __launch_bounds__(256) void k_pphh_epilogue(DevBasis B, const SegTask
       sxy += L6[x + (i64)h * nb] * R6[y + (i64)h * nb];
       syx += L6[y + (i64)h * nb] * R6[x + (i64)h * nb];
     }
-    const double w1xy = Pq[x + (i64)y * nbh] * Lh, w1yx = Pq[y + (i64)x * nbh] * Lh;
+    const double w1xy = pval(Pq, nbh, ks, x, y) * Lh, w1yx = pval(Pq, nbh, ks, y, x) * Lh;
     const double w2xy = sxy * Rh, w2yx = syx * Rh;
     const i64 o = go[3] + e;
     RES[o] = RES[o] + w1xy - w2xy - (w1yx - w2yx) * pm;
@@ -773,7 +799,7 @@ __global__ __launch_bounds__(256) void k_pphh_epilogue(DevBasis B, const SegTask
       sab += L5[a + (i64)h * nh] * R5[h + (i64)b * nh];
       sba += L5[b + (i64)h * nh] * R5[h + (i64)a * nh];
     }
-    const double w1ab = Pq[(nb + a) + (i64)(nb + b) * nbh] * Lh, w1ba = Pq[(nb + b) + (i64)(nb + a) * nbh] * Lh;
+    const double w1ab = pval(Pq, nbh, ks, nb + a, nb + b) * Lh, w1ba = pval(Pq, nbh, ks, nb + b, nb + a) * Lh;
     const i64 o = go[4] + e;
     RES[o] = RES[o] + w1ab - sab - (w1ba - sba) * pm;
     w1[o] = w1ab;
@@ -787,7 +813,7 @@ __global__ __launch_bounds__(256) void k_pphh_epilogue(DevBasis B, const SegTask
       s1 += L6[x + (i64)hp * nb] * R5[hp + (i64)h * nh];
       s2 += R6[x + (i64)hp * nb] * L5[hp + (i64)h * nh];
     }
-    const double w1v = Lh * Pq[x + (i64)(nb + h) * nbh] + (-1.0 * Rh) * Pq[(nb + h) + (i64)x * nbh];
+    const double w1v = Lh * pval(Pq, nbh, ks, x, nb + h) + (-1.0 * Rh) * pval(Pq, nbh, ks, nb + h, x);
     const double w2v = s1 + (-1.0) * s2;
     const i64 o = go[5] + e;
     RES[o] = RES[o] + w1v - w2v;
@@ -798,9 +824,10 @@ __global__ __launch_bounds__(256) void k_pphh_epilogue(DevBasis B, const SegTask
 
 void launch_pphh_epilogue(const DevBasis& B, const SegTask* t, int nt, const double* L, int Lh, const double* R,
                           int Rh, double* RES, double* w1, double* w2, const double* Pscr, const i64* poff,
-                          hipStream_t s) {
+                          const int* k3ks, const double* K2scr, const i64* k2off, const int* k2ks, hipStream_t s) {
   if (nt <= 0) return;
-  hipLaunchKernelGGL(k_pphh_epilogue, dim3(nt), dim3(256), 0, s, B, t, L, Lh, R, Rh, RES, w1, w2, Pscr, poff);
+  hipLaunchKernelGGL(k_pphh_epilogue, dim3(nt), dim3(256), 0, s, B, t, L, Lh, R, Rh, RES, w1, w2, Pscr, poff, k3ks,
+                     K2scr, k2off, k2ks);
 }
 
 // ================================================ streaming algebra / norm

@@ -46,8 +46,11 @@ struct GemmProb {
   unsigned char sa1, sb1, sa2, sb2;
   unsigned char sc, pad0, pad1, pad2;
   double alpha, s2, beta;
+  int kchunk;        // > 0: split K in chunks of kchunk, chunk ks written to C + ks*cstride
+  int pad3;
+  i64 cstride;
 };
-struct GemmTask { int prob; int tm; int tn; int pad; };
+struct GemmTask { int prob; int tm; int tn; int ks; };
 
 // Tile pair (I,J), I <= J, of the pppp block of channel q
 struct PairTask { int q; int ti; int tj; int pad; };
@@ -89,7 +92,7 @@ void launch_pppp_pairs(const DevBasis& B, const PairTask* t, int nt, const doubl
                        int Rh, double* RES, double* w1, double* w2, int write_w1, hipStream_t s);
 void launch_pphh_epilogue(const DevBasis& B, const SegTask* t, int nt, const double* L, int Lh, const double* R,
                           int Rh, double* RES, double* w1, double* w2, const double* Pscr, const i64* poff,
-                          hipStream_t s);
+                          const int* k3ks, const double* K2scr, const i64* k2off, const int* k2ks, hipStream_t s);
 void launch_axpy(i64 n, const double* A, double ax, const double* B, double bx, double* C, hipStream_t s);
 void launch_axpy_devcoef(i64 n, const double* A, const double* B, const double* coef, double* C, hipStream_t s);
 void launch_norm_partials(const double* op, const i64* chunk_off, const int* chunk_len, const double* chunk_w,

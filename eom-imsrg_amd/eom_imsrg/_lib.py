@@ -77,7 +77,10 @@ SIGNATURES = {
     "imsrg_magnus_step": (_i, [_p, _dp, _dp, _dp, _dp, C.c_double, C.c_double, C.c_double, _d, _p]),
     "imsrg_magnus_decouple": (_i, [_p, _dp, _dp, C.c_double, C.c_double, C.c_double, _i, C.c_double, _i, _d,
                                    _ip, _p]),
+    "imsrg_profile_kernel": (_i, [_p, _i, _dp, _i, _dp, _i, _dp, _i, _dp, _dp, _dp, _dp, _dp, _i, _d, _p]),
 }
+
+KERNEL = dict(PPPP=0, PPHH_SMALL=1, PH_GEMM=2, PH_SCATTER=3, K122=4, PANDYA=5, ONEBODY=6, COMMUTATOR=7)
 
 for _name, (_res, _args) in SIGNATURES.items():
     _f = getattr(lib, _name)

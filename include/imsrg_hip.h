@@ -157,6 +157,24 @@ int imsrg_magnus_decouple(imsrg_flow f, double* HS, double* G, double ds, double
                           int crit_mode, double crit_tol, int maxsteps, double* log, int* steps,
                           imsrg_stream stream);
 
+/* ------------------------------------------------ instrumentation */
+/* No counterpart in the reference: times `iters` launches of one kernel
+ * class of [L, R] between hipEvents recorded on `stream` and returns the
+ * mean milliseconds per launch.  Inputs as for the term routines; every
+ * kernel class runs once untimed first (fills w1/w2, LCC/RCC, WCC).
+ * which: IMSRG_K_* below. */
+#define IMSRG_K_PPPP 0      /* 222_pp_hh pppp tile pairs (MFMA) */
+#define IMSRG_K_PPHH_SMALL 1 /* 222_pp_hh pp x (ph|hh) and (ph|hh)^2 GEMMs + epilogue */
+#define IMSRG_K_PH_GEMM 2   /* 222_ph WCC = LCC RCC^T (MFMA) */
+#define IMSRG_K_PH_SCATTER 3 /* 222_ph inverse Pandya */
+#define IMSRG_K_122 4
+#define IMSRG_K_PANDYA 5
+#define IMSRG_K_ONEBODY 6   /* 111 + 121 + 221 */
+#define IMSRG_K_COMMUTATOR 7 /* the whole [L, R] */
+int imsrg_profile_kernel(imsrg_basis b, int which, const double* L, int Lherm, const double* R, int Rherm,
+                         double* RES, int RESherm, double* w1, double* w2, double* LCC, double* RCC, double* WCC,
+                         int iters, double* ms, imsrg_stream stream);
+
 #ifdef __cplusplus
 }
 #endif
