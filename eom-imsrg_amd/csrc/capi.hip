@@ -175,17 +175,19 @@ int build_plan(imsrg_basis_s* B) {
     for (int q : co2) {
       const int r = b.rlen[q];
       if (r > 0) {
+        // W' = Lh LCC RCC^T - Rh RCC LCC^T, upper tiles only (alpha = Lh and
+        // s2 = -Rh/Lh are applied at launch)
         GemmProb p{};
-        p.a1 = b.ccoff[q]; p.b1 = b.ccoff[q]; p.c = b.wcoff[q];
+        p.a1 = b.ccoff[q]; p.b1 = b.ccoff[q]; p.a2 = b.ccoff[q]; p.b2 = b.ccoff[q]; p.c = b.wcoff[q];
         p.m = r; p.n = r; p.k = b.ncol[q];
         p.lda = r; p.ldb = r; p.ldc = r;
-        p.nprod = 1; p.ta1 = 0; p.tb1 = 1;
-        p.sa1 = 0; p.sb1 = 1; p.sc = 2;
-        p.alpha = 1.0; p.s2 = 0.0; p.beta = 0.0;
+        p.nprod = 2; p.ta1 = 0; p.tb1 = 1; p.ta2 = 0; p.tb2 = 1;
+        p.sa1 = 0; p.sb1 = 1; p.sa2 = 1; p.sb2 = 0; p.sc = 2;
+        p.alpha = 1.0; p.s2 = 1.0; p.beta = 0.0;
         const int pid = (int)B->php_h.size();
         B->php_h.push_back(p);
         for (int tm = 0; tm < tiles(r); ++tm)
-          for (int tn = 0; tn < tiles(r); ++tn) B->ph.host.push_back(GemmTask{pid, tm, tn, 0});
+          for (int tn = tm; tn < tiles(r); ++tn) B->ph.host.push_back(GemmTask{pid, tm, tn, 0});
       }
       B->ph.chan_begin.push_back((int)B->ph.host.size());
     }
@@ -346,11 +348,17 @@ int build_plan(imsrg_basis_s* B) {
               if (c == rcls[g]) rows.push_back((i << 8) | j);
               if (c == ccls[g]) cols.push_back((i << 8) | j);
             }
-          for (int rp : rows) {
-            const int i = rp >> 8, j = rp & 255;
-            for (int cp : cols) {
+          // order: column pair (k,l), then j, then i fastest, then 2J: the
+          // WCC rows (k,i) and (l,i) of neighbouring lanes are neighbours
+          std::vector<int> rows_j;   // row pairs sorted by (j, i)
+          rows_j = rows;
+          std::stable_sort(rows_j.begin(), rows_j.end(),
+                           [](int x, int y) { return ((x & 255) << 8 | (x >> 8)) < ((y & 255) << 8 | (y >> 8)); });
+          for (int cp : cols) {
+            const int k = cp >> 8, l = cp & 255;
+            for (int rp : rows_j) {
               if (kSquare[g] && cp < rp) continue;   // (k,l) >= (i,j): the upper triangle
-              const int k = cp >> 8, l = cp & 255;
+              const int i = rp >> 8, j = rp & 255;
               const int lo = std::max(std::abs(b.jj[i] - b.jj[j]), std::abs(b.jj[k] - b.jj[l]));
               const int hi = std::min(b.jj[i] + b.jj[j], b.jj[k] + b.jj[l]);
               for (int J = lo; J <= hi; J += 2)
@@ -499,7 +507,7 @@ void t_ph(imsrg_basis_s* B, const double* LCC, int Lh, const double* RCC, int Rh
           double* WCC, hipStream_t s) {
   Slots sl{};
   sl.p[0] = LCC; sl.p[1] = RCC; sl.p[2] = WCC;
-  run(B, B->ph, [&](const GemmTask* t, int n) { launch_gemm(sl, B->php, t, n, s); });
+  run(B, B->ph, [&](const GemmTask* t, int n) { launch_gemm(sl, B->php, t, n, s, (double)Lh, -(double)(Rh * Lh)); });
   if (B->path == IMSRG_PATH_GROUPED)
     launch_comm222ph_quad(B->D, B->quad, (int)B->quad_h.size(), WCC, Lh, Rh, RES, RESh, s);
   else
@@ -1047,7 +1055,8 @@ int imsrg_profile_kernel(imsrg_basis B, int which, const double* L, int Lh, cons
       case IMSRG_K_PH_GEMM: {
         Slots sl{};
         sl.p[0] = LCC; sl.p[1] = RCC; sl.p[2] = WCC;
-        run(B, B->ph, [&](const GemmTask* t, int n) { launch_gemm(sl, B->php, t, n, s); });
+        run(B, B->ph,
+            [&](const GemmTask* t, int n) { launch_gemm(sl, B->php, t, n, s, (double)Lh, -(double)(Rh * Lh)); });
         break;
       }
       case IMSRG_K_PH_SCATTER:

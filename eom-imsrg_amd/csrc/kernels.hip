@@ -153,14 +153,25 @@ void launch_comm122p(const DevBasis& B, const SegTask* t, int nt, const Partner*
 }
 
 // ================================================== commutator_222_ph scatter
-// commutators.f90:804-923.  Inverse Pandya of WCC = LCC RCC^T into element
-// (IX, JX) of block g of channel q, whose pairs are (i,j) and (k,l).
+// commutators.f90:804-923.  Every J' term of the reference reads WCC and
+// its transpose only in the combination
+//     W'(x,y) = Lh W(x,y) - Rh W(y,x),    W = LCC RCC^T,
+// inside:  (W(gik,rjl) + ph W(gjl,rik)) Lh - (ph W(rik,gjl) + W(rjl,gik)) Rh
+//        =  W'(gik,rjl) + ph W'(gjl,rik)
+// outside: W'(gil,rjk) + ph W'(gjk,ril).
+// W' satisfies W'(y,x) = -Lh Rh W'(x,y), so the GEMM stores its upper
+// triangle only and the scatter reads the element (min, max).
+__device__ __forceinline__ double wprime(const double* W, i64 rr, int x, int y, double mpm) {
+  return x <= y ? W[x + (i64)y * rr] : mpm * W[y + (i64)x * rr];
+}
+
 __device__ __forceinline__ void ph_element(const DevBasis& B, const double* WCC, int Lh, int Rh, double* RES,
                                            int RESh, int q, int g, int IX, int JX, int i, int j, int k, int l) {
   const int n1 = brows(B, q, g);
   const bool sq = is_square(g);
   const double pre = (i == j) ? IMSRG_INV_SQRT2_REF : 1.0;
   const double pre2 = (k == l) ? IMSRG_INV_SQRT2_REF : 1.0;
+  const double mpm = -(double)(Lh * Rh);
   const int Jtot = B.chJ[q];
   const int ji = B.jj[i], jj = B.jj[j], jk = B.jj[k], jl = B.jj[l];
   const int li = B.ll[i], lj = B.ll[j], lk = B.ll[k], ll = B.ll[l];
@@ -182,9 +193,7 @@ __device__ __forceinline__ void ph_element(const DevBasis& B, const double* WCC,
     const int jo = JP >> 1;
     const int rjl = B.rmap[((i64)j * N + l) * nJ + jo], rik = B.rmap[((i64)i * N + k) * nJ + jo];
     const int gjl = B.rmap[((i64)l * N + j) * nJ + jo], gik = B.rmap[((i64)k * N + i) * nJ + jo];
-    sm = sm - m1pow((jl + jk + Jtot) / 2) *
-                  ((W[gik + rjl * rr] + phase * W[gjl + rik * rr]) * Lh -
-                   (phase * W[rik + gjl * rr] + W[rjl + gik * rr]) * Rh) *
+    sm = sm - m1pow((jl + jk + Jtot) / 2) * (wprime(W, rr, gik, rjl, mpm) + phase * wprime(W, rr, gjl, rik, mpm)) *
                   sixj(B, ji, jj, Jtot, jl, jk, JP);
   }
   // "outside" recoupling
@@ -202,8 +211,7 @@ __device__ __forceinline__ void ph_element(const DevBasis& B, const double* WCC,
     const int jo = JP >> 1;
     const int ril = B.rmap[((i64)i * N + l) * nJ + jo], rjk = B.rmap[((i64)j * N + k) * nJ + jo];
     const int gil = B.rmap[((i64)l * N + i) * nJ + jo], gjk = B.rmap[((i64)k * N + j) * nJ + jo];
-    sm = sm + ((W[gil + rjk * rr] + phase * W[gjk + ril * rr]) * Lh -
-               (phase * W[ril + gjk * rr] + W[rjk + gil * rr]) * Rh) *
+    sm = sm + (wprime(W, rr, gil, rjk, mpm) + phase * wprime(W, rr, gjk, ril, mpm)) *
                   sixj(B, ji, jj, Jtot, jk, jl, JP);
   }
   double* X = RES + B.goff[q * 6 + g];
@@ -587,7 +595,8 @@ __device__ __forceinline__ void tile_mma(v4d (&acc)[2][2], double (*As)[BM + 1],
   }
 }
 
-__global__ __launch_bounds__(256) void k_gemm(Slots sl, const GemmProb* probs, const GemmTask* tasks) {
+__global__ __launch_bounds__(256) void k_gemm(Slots sl, const GemmProb* probs, const GemmTask* tasks,
+                                              double ascale, double s2scale) {
   __shared__ double As[BK][BM + 1];
   __shared__ double Bs[BK][BN + 1];
   const GemmTask t = tasks[blockIdx.x];
@@ -605,7 +614,7 @@ __global__ __launch_bounds__(256) void k_gemm(Slots sl, const GemmProb* probs, c
            kb, ke, m0, n0, 1.0);
   if (pr.nprod == 2)
     tile_mma(acc, As, Bs, sl.p[pr.sa2] + pr.a2, pr.lda, pr.ta2, sl.p[pr.sb2] + pr.b2, pr.ldb, pr.tb2, pr.m, pr.n,
-             kb, ke, m0, n0, pr.s2);
+             kb, ke, m0, n0, pr.s2 * s2scale);
   double* C = const_cast<double*>(sl.p[pr.sc]) + pr.c + (i64)t.ks * pr.cstride;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wm = wave >> 1, wn = wave & 1;
@@ -619,15 +628,16 @@ __global__ __launch_bounds__(256) void k_gemm(Slots sl, const GemmProb* probs, c
         const int col = n0 + c_col(wn, ni, lane, r);
         if (row < pr.m && col < pr.n) {
           const i64 o = row + (i64)col * pr.ldc;
-          const double v = pr.alpha * acc[mi][ni][r];
+          const double v = pr.alpha * ascale * acc[mi][ni][r];
           C[o] = (pr.beta == 0.0) ? v : v + pr.beta * C[o];
         }
       }
 }
 
-void launch_gemm(const Slots& sl, const GemmProb* probs, const GemmTask* tasks, int nt, hipStream_t s) {
+void launch_gemm(const Slots& sl, const GemmProb* probs, const GemmTask* tasks, int nt, hipStream_t s,
+                 double ascale, double s2scale) {
   if (nt <= 0) return;
-  hipLaunchKernelGGL(k_gemm, dim3(nt), dim3(256), 0, s, sl, probs, tasks);
+  hipLaunchKernelGGL(k_gemm, dim3(nt), dim3(256), 0, s, sl, probs, tasks, ascale, s2scale);
 }
 
 // ======================================== 222_pp_hh pppp block (tile pairs)

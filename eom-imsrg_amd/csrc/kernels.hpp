@@ -87,7 +87,9 @@ void launch_e0_partials(const DevBasis& B, const double* L, int Lh, const double
                         hipStream_t s);
 void launch_e0_final(const double* part, int nblocks, int use220, int use110, double* dst, int accumulate,
                      hipStream_t s);
-void launch_gemm(const Slots& sl, const GemmProb* probs, const GemmTask* tasks, int nt, hipStream_t s);
+// alpha and s2 of every problem are multiplied by ascale / s2scale
+void launch_gemm(const Slots& sl, const GemmProb* probs, const GemmTask* tasks, int nt, hipStream_t s,
+                 double ascale = 1.0, double s2scale = 1.0);
 void launch_pppp_pairs(const DevBasis& B, const PairTask* t, int nt, const double* L, int Lh, const double* R,
                        int Rh, double* RES, double* w1, double* w2, int write_w1, hipStream_t s);
 void launch_pphh_epilogue(const DevBasis& B, const SegTask* t, int nt, const double* L, int Lh, const double* R,

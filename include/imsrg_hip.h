@@ -106,7 +106,10 @@ int imsrg_commutator_221(imsrg_basis b, const double* L, int Lherm, const double
                          int RESherm, const double* w1, const double* w2, imsrg_stream stream);
 /* calculate_cross_coupled, cross_coupled.f90:1190-1300 */
 int imsrg_calculate_cross_coupled(imsrg_basis b, const double* op, int herm, double* CC, imsrg_stream stream);
-/* commutator_222_ph, commutators.f90:771-925 (accumulates; fills WCC) */
+/* commutator_222_ph, commutators.f90:771-925 (accumulates).  WCC is the
+ * caller's workspace as in the reference; on return each block holds, in
+ * its upper triangle (row <= column), Lh*W - Rh*W^T with W = LCC RCC^T,
+ * the only combination the inverse Pandya transform reads. */
 int imsrg_commutator_222_ph(imsrg_basis b, const double* LCC, int Lherm, const double* RCC, int Rherm,
                             double* RES, int RESherm, double* WCC, imsrg_stream stream);
 

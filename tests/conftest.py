@@ -36,7 +36,7 @@ def oracle_basis(emax: int):
     return _ORACLES[emax]
 
 
-def assert_rel(got, ref, rtol=1e-10, floor=1e-3, what=""):
+def assert_rel(got, ref, rtol=1e-10, floor=1e-3, what="", scale=None):
     """|got - ref| <= rtol * (|ref| + floor * max|ref|), element by element.
 
     FP64 sums of K terms carry ~K*1e-16 absolute error relative to the
@@ -44,7 +44,8 @@ def assert_rel(got, ref, rtol=1e-10, floor=1e-3, what=""):
     held to rtol*floor*max|ref| instead of their own magnitude."""
     got = np.asarray(got, np.float64)
     ref = np.asarray(ref, np.float64)
-    scale = float(np.max(np.abs(ref))) if ref.size else 0.0
+    if scale is None:
+        scale = float(np.max(np.abs(ref))) if ref.size else 0.0
     bound = rtol * (np.abs(ref) + floor * scale) + 1e-300
     d = np.abs(got - ref)
     bad = d > bound

@@ -105,7 +105,18 @@ def test_terms_vs_oracle(gpu, emax, hL, hR, path):
     RCCd.data.copy_(torch.from_numpy(RCCr))
     cm.commutator_222_ph(LCCd, RCCd, RES, W)
     _sync()
-    assert_rel(W.data.cpu().numpy(), Wr, what="222_ph WCC")
+    # WCC holds W' = Lh W - Rh W^T (upper triangle of every block)
+    Wg = W.data.cpu().numpy()
+    wscale = 2 * float(np.max(np.abs(Wr)))
+    for q in range(b.ncc):
+        r, off = int(b.rlen[q]), int(b.wcoff[q])
+        if r == 0:
+            continue
+        Wq = Wr[off: off + r * r].reshape(r, r, order="F")
+        Wp = hL * Wq - hR * Wq.T
+        iu = np.triu_indices(r)
+        assert_rel(Wg[off: off + r * r].reshape(r, r, order="F")[iu], Wp[iu], what=f"222_ph WCC block {q}",
+                   scale=wscale)
     assert_rel(RES.numpy(), ref, what="commutator_222_ph RES")
     b.set_path(0)
 
