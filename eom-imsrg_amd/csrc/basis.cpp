@@ -176,9 +176,10 @@ std::string build_basis(Basis& b, int N, const int* n, const int* l, const int* 
   b.cclen = co;
   b.wclen = wo;
 
-  // 6j table indexed [j1/2][j2/2][j4/2][j5/2][J3/2][J6/2]: the symbols that
-  // share their four half-integer arguments form one contiguous
-  // (J3, J6) tile, which is what one orbit quadruple reads.
+  // 6j table indexed [j1/2][j2/2][j4/2][j5/2][J6/2][J3/2]: the symbols that
+  // share their four half-integer arguments form one contiguous tile, and
+  // for fixed J6 the channel 2J (= J3) is the fastest index, which is what
+  // the lanes of one orbit quadruple read together.
   b.nh6 = (b.JTM + 1) / 2;
   b.nJ6 = b.JTM + 1;
   const int nh = b.nh6, nJ6 = b.nJ6;
@@ -187,12 +188,24 @@ std::string build_basis(Basis& b, int N, const int* n, const int* l, const int* 
     for (int a2 = 0; a2 < nh; ++a2)
       for (int a4 = 0; a4 < nh; ++a4)
         for (int a5 = 0; a5 < nh; ++a5)
-          for (int J3 = 0; J3 < nJ6; ++J3)
-            for (int J6 = 0; J6 < nJ6; ++J6) {
+          for (int J6 = 0; J6 < nJ6; ++J6)
+            for (int J3 = 0; J3 < nJ6; ++J3) {
               const size_t idx =
-                  ((((((size_t)a1 * nh + a2) * nh + a4) * nh + a5) * nJ6 + J3) * nJ6) + J6;
+                  ((((((size_t)a1 * nh + a2) * nh + a4) * nh + a5) * nJ6 + J6) * nJ6) + J3;
               b.six[idx] = sixj_value(2 * a1 + 1, 2 * a2 + 1, 2 * J3, 2 * a4 + 1, 2 * a5 + 1, 2 * J6);
             }
+  // cross-coupled row maps with 2J/2 slowest: rmapJ[(J*N + a)*N + b] and the
+  // transposed rmapJT[(J*N + b)*N + a] (the inverse Pandya reads rows (k,i)
+  // and (i,k) for neighbouring i)
+  b.rmapJ.assign((size_t)N * N * nJ, -1);
+  b.rmapJT.assign((size_t)N * N * nJ, -1);
+  for (int x = 0; x < N; ++x)
+    for (int y = 0; y < N; ++y)
+      for (int jo = 0; jo < nJ; ++jo) {
+        const int v = b.rmap[((size_t)x * N + y) * nJ + jo];
+        b.rmapJ[((size_t)jo * N + x) * N + y] = v;
+        b.rmapJT[((size_t)jo * N + y) * N + x] = v;
+      }
   return "";
 }
 

@@ -52,11 +52,13 @@ struct DevBasis {
   const i64* wcoff;                // [ncc] offset of WCC(q) (rlen x rlen, col-major)
   const int* rmap;                 // [N*N*nJ] ordered pair (i,j) at 2J -> CC row, -1
   const int* nbmap;                // [N*N*nJ] ordered (p,h) at 2J -> CC column, -1
+  const int* rmapJ;                // [(J/2*N + a)*N + b] -> CC row of (a,b)
+  const int* rmapJT;               // [(J/2*N + b)*N + a] -> CC row of (a,b)
   const int* ccrows;               // packed row pairs per CC block
   const i64* ccrowoff;             // [ncc]
   const int* cccols;               // packed column (p,h) pairs per CC block
   const i64* cccoloff;             // [ncc]
-  const double* six;               // 6j table, see sixj_index
+  const double* six;               // 6j table [j1/2][j2/2][j4/2][j5/2][J6/2][J3/2]
   int nh6, nJ6;
 };
 
@@ -79,7 +81,7 @@ struct Basis {
   int ncc = 0;
   std::vector<int> ccJ, ccT, ccP, rlen, ncol;
   std::vector<i64> ccoff, wcoff, ccrowoff, cccoloff;
-  std::vector<int> ccrows, cccols, rmap, nbmap;
+  std::vector<int> ccrows, cccols, rmap, nbmap, rmapJ, rmapJT;
   i64 cclen = 0, wclen = 0;
   // 6j
   int nh6 = 0, nJ6 = 0;

@@ -33,25 +33,38 @@ int fail(int code, const std::string& msg) {
     if (e_ != hipSuccess) return fail(IMSRG_EHIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
+// One family of grouped GEMM problems (one launch on the grouped path, one
+// per channel / cross-coupled block on the plain path) and its tile tasks.
+struct GemmFamily {
+  int tile = 0;                   // 0: 64x64, 1: 128x64, 2: 128x128
+  bool upper = false;             // only tiles with tm <= tn (symmetric results)
+  std::vector<GemmProb> probs_h;
+  std::vector<int> nks;           // split-K chunks per problem
+  std::vector<int> group_end;     // problems of group g: [group_end[g-1], group_end[g])
+  GemmProb* probs = nullptr;
+  TaskList<GemmTask> tasks;
+  GemmTask* tasks_alloc = nullptr;
+  bool xcd_aware = false;         // pack groups onto XCDs (else one largest-first list)
+};
+
 struct imsrg_basis_s {
   Basis b;
   int path = IMSRG_PATH_GROUPED;
   bool dev_ready = false;
+  int tile_req[4] = {0, 0, 0, 0};   // GEMM tile shapes for pp1, k2, k3, ph (64x64, measured best)
   DevBasis D{};
   std::vector<void*> allocs;
   // task lists (grouped by channel / cross-coupled block)
   TaskList<SegTask> seg_tri, seg_full, seg_g3, seg_epi;
   TaskList<CCTask> cc;
   TaskList<PairTask> pairs;
-  TaskList<GemmTask> k2, k3, ph;
+  GemmFamily pp1, k2, k3, ph;       // pppp L1 R1, pp x (ph|hh), (ph|hh)^2, cross-coupled W'
   std::vector<Partner> part_h;      // 122 partner lists (CSR by global pair id)
   std::vector<int> rowp_h, colp_h;
   Partner* partners = nullptr;
   int *rowp = nullptr, *colp = nullptr;
   std::vector<QuadItem> quad_h;   // 222_ph outputs, quadruple-major (grouped path)
   QuadItem* quad = nullptr;
-  std::vector<GemmProb> k2p_h, k3p_h, php_h;
-  GemmProb *k2p = nullptr, *k3p = nullptr, *php = nullptr;
   std::vector<OneTask> one_h;
   OneTask* one = nullptr;
   // norm chunks
@@ -145,6 +158,83 @@ void build_seg(const Basis& b, TaskList<SegTask>& tl, SegMode mode, const std::v
 
 int tiles(int n) { return (n + 63) / 64; }
 
+// (re)build the tile tasks of a GEMM family for tile shape `tile`.  Groups
+// (channels / cross-coupled blocks) are packed onto the XCDs by cost
+// (largest first onto the least loaded XCD); each XCD's range lists its
+// groups largest first, so per-group ranges stay contiguous for the plain
+// path and the grouped launch keeps each group inside one L2.
+int family_tasks(imsrg_basis_s* B, GemmFamily& f, int tile) {
+  f.tile = tile;
+  const int bm = tile_rows(tile), bn = tile_cols(tile);
+  const int ng = (int)f.group_end.size();
+  std::vector<std::vector<GemmTask>> gt(ng);
+  std::vector<double> gcost(ng, 0.0);
+  int p0 = 0;
+  for (int g = 0; g < ng; ++g) {
+    const int pe = f.group_end[g];
+    for (int pid = p0; pid < pe; ++pid) {
+      const GemmProb& p = f.probs_h[pid];
+      const int tm_n = (p.m + bm - 1) / bm, tn_n = (p.n + bn - 1) / bn;
+      for (int ks = 0; ks < f.nks[pid]; ++ks) {
+        const int kb = p.kchunk > 0 ? ks * p.kchunk : 0;
+        const int ke = p.kchunk > 0 ? std::min(p.k, kb + p.kchunk) : p.k;
+        for (int tm = 0; tm < tm_n; ++tm)
+          for (int tn = 0; tn < tn_n; ++tn) {
+            if (f.upper && (tn + 1) * bn <= tm * bm) continue;   // tile entirely below the diagonal
+            gt[g].push_back(GemmTask{pid, tm, tn, ks});
+            gcost[g] += 8.0 + (ke - kb) * (p.nprod == 2 ? 2.0 : 1.0);
+          }
+      }
+    }
+    p0 = pe;
+  }
+  std::vector<int> gord(ng);
+  for (int g = 0; g < ng; ++g) gord[g] = g;
+  std::stable_sort(gord.begin(), gord.end(), [&](int x, int y) { return gcost[x] > gcost[y]; });
+  f.tasks.host.clear();
+  f.tasks.chan_begin.assign(1, 0);
+  if (f.xcd_aware) {
+    std::vector<std::vector<int>> xg(kXcds);
+    double load[kXcds] = {0};
+    for (int g : gord) {
+      if (gt[g].empty()) continue;
+      int best = 0;
+      for (int x = 1; x < kXcds; ++x)
+        if (load[x] < load[best]) best = x;
+      xg[best].push_back(g);
+      load[best] += gcost[g];
+    }
+    int maxc = 0;
+    for (int x = 0; x < kXcds; ++x) {
+      f.tasks.xmap.start[x] = (int)f.tasks.host.size();
+      for (int g : xg[x]) {
+        f.tasks.host.insert(f.tasks.host.end(), gt[g].begin(), gt[g].end());
+        f.tasks.chan_begin.push_back((int)f.tasks.host.size());
+      }
+      f.tasks.xmap.count[x] = (int)f.tasks.host.size() - f.tasks.xmap.start[x];
+      maxc = std::max(maxc, f.tasks.xmap.count[x]);
+    }
+    f.tasks.grid = kXcds * maxc;
+  } else {
+    for (int g : gord) {
+      f.tasks.host.insert(f.tasks.host.end(), gt[g].begin(), gt[g].end());
+      f.tasks.chan_begin.push_back((int)f.tasks.host.size());
+    }
+    for (int x = 0; x < kXcds; ++x) { f.tasks.xmap.start[x] = 0; f.tasks.xmap.count[x] = -1; }
+    f.tasks.grid = (int)f.tasks.host.size();
+  }
+  f.tasks.n = (int)f.tasks.host.size();
+  if (f.tasks_alloc) {
+    for (auto it = B->allocs.begin(); it != B->allocs.end(); ++it)
+      if (*it == (void*)f.tasks_alloc) { B->allocs.erase(it); break; }
+    (void)hipFree(f.tasks_alloc);
+    f.tasks_alloc = nullptr;
+  }
+  int rc = upload(B, f.tasks.host, &f.tasks.dev);
+  f.tasks_alloc = f.tasks.dev;
+  return rc;
+}
+
 int build_plan(imsrg_basis_s* B) {
   const Basis& b = B->b;
   const std::vector<int> order = channel_order(b);
@@ -160,10 +250,9 @@ int build_plan(imsrg_basis_s* B) {
                      [&](int x, int y) { return (i64)b.rlen[x] * b.ncol[x] > (i64)b.rlen[y] * b.ncol[y]; });
     B->cc.host.clear();
     B->cc.chan_begin.assign(1, 0);
-    B->ph.host.clear();
-    B->ph.chan_begin.assign(1, 0);
-    B->php_h.clear();
-    std::vector<int> co2 = co;   // WCC GEMM: largest r^2 * nb first
+    B->ph.probs_h.clear(); B->ph.nks.clear(); B->ph.group_end.clear();
+    B->ph.upper = true;
+    std::vector<int> co2 = co;   // W' GEMM: largest r^2 * nb first
     for (int q : co) {
       const i64 cnt = (i64)b.rlen[q] * b.ncol[q];
       for (i64 s = 0; s < cnt; s += kTile) B->cc.host.push_back(CCTask{q, (int)s});
@@ -184,21 +273,17 @@ int build_plan(imsrg_basis_s* B) {
         p.nprod = 2; p.ta1 = 0; p.tb1 = 1; p.ta2 = 0; p.tb2 = 1;
         p.sa1 = 0; p.sb1 = 1; p.sa2 = 1; p.sb2 = 0; p.sc = 2;
         p.alpha = 1.0; p.s2 = 1.0; p.beta = 0.0;
-        const int pid = (int)B->php_h.size();
-        B->php_h.push_back(p);
-        for (int tm = 0; tm < tiles(r); ++tm)
-          for (int tn = tm; tn < tiles(r); ++tn) B->ph.host.push_back(GemmTask{pid, tm, tn, 0});
+        B->ph.probs_h.push_back(p);
+        B->ph.nks.push_back(1);
       }
-      B->ph.chan_begin.push_back((int)B->ph.host.size());
+      B->ph.group_end.push_back((int)B->ph.probs_h.size());
     }
     B->cc.n = (int)B->cc.host.size();
-    B->ph.n = (int)B->ph.host.size();
   }
-  // 222_pp_hh: pppp tile pairs, K2 = pp x (ph|hh) dual product, K3 = (ph|hh)^T (ph|hh)
+  // 222_pp_hh: pppp product L1 R1 (-> w1) + tile-pair epilogue, K2 = pp x
+  // (ph|hh) dual product, K3 = (ph|hh)^T (ph|hh)
   B->pairs.host.clear(); B->pairs.chan_begin.assign(1, 0);
-  B->k2.host.clear(); B->k2.chan_begin.assign(1, 0);
-  B->k3.host.clear(); B->k3.chan_begin.assign(1, 0);
-  B->k2p_h.clear(); B->k3p_h.clear();
+  for (GemmFamily* f : {&B->pp1, &B->k2, &B->k3}) { f->probs_h.clear(); f->nks.clear(); f->group_end.clear(); }
   // split K (deterministic: partial slabs summed in order by the epilogue)
   const int KC2 = 128, KC3 = 64;
   B->poff_h.assign(b.nblocks, 0); B->k3ks_h.assign(b.nblocks, 1);
@@ -219,6 +304,21 @@ int build_plan(imsrg_basis_s* B) {
     for (int ti = 0; ti < T; ++ti)
       for (int tj = ti; tj < T; ++tj) B->pairs.host.push_back(PairTask{q, ti, tj, 0});
     B->pairs.chan_begin.push_back((int)B->pairs.host.size());
+    if (np > 0) {
+      // C = L1 R1 + s2 L3 R3^T with s2 = -Rh applied at launch (K2 = nh)
+      GemmProb p{};
+      p.a1 = b.goff[q * 6 + 0]; p.b1 = b.goff[q * 6 + 0]; p.c = b.goff[q * 6 + 0];
+      p.a2 = b.goff[q * 6 + 2]; p.b2 = b.goff[q * 6 + 2];
+      p.m = np; p.n = np; p.k = np;
+      p.lda = np; p.ldb = np; p.ldc = np;
+      p.nprod = nh > 0 ? 2 : 1; p.k2 = nh;
+      p.ta2 = 0; p.tb2 = 1;
+      p.sa1 = 0; p.sb1 = 1; p.sa2 = 0; p.sb2 = 1; p.sc = 2;
+      p.alpha = 1.0; p.s2 = 1.0; p.beta = 0.0;
+      B->pp1.probs_h.push_back(p);
+      B->pp1.nks.push_back(1);
+    }
+    B->pp1.group_end.push_back((int)B->pp1.probs_h.size());
     if (np > 0 && nb + nh > 0) {
       GemmProb p{};
       p.a1 = b.goff[q * 6 + 0]; p.b1 = b.goff[q * 6 + 1];
@@ -230,13 +330,10 @@ int build_plan(imsrg_basis_s* B) {
       p.sa1 = 0; p.sb1 = 1; p.sa2 = 1; p.sb2 = 0; p.sc = 2;
       p.alpha = 1.0; p.s2 = -1.0; p.beta = 0.0;
       p.kchunk = KC2; p.cstride = (i64)np * (nb + nh);
-      const int pid = (int)B->k2p_h.size();
-      B->k2p_h.push_back(p);
-      for (int ks = 0; ks < B->k2ks_h[q]; ++ks)
-        for (int tm = 0; tm < tiles(np); ++tm)
-          for (int tn = 0; tn < tiles(nb + nh); ++tn) B->k2.host.push_back(GemmTask{pid, tm, tn, ks});
+      B->k2.probs_h.push_back(p);
+      B->k2.nks.push_back(B->k2ks_h[q]);
     }
-    B->k2.chan_begin.push_back((int)B->k2.host.size());
+    B->k2.group_end.push_back((int)B->k2.probs_h.size());
     if (nb + nh > 0) {
       GemmProb p{};
       p.a1 = b.goff[q * 6 + 1]; p.b1 = b.goff[q * 6 + 1];
@@ -247,17 +344,14 @@ int build_plan(imsrg_basis_s* B) {
       p.sa1 = 0; p.sb1 = 1; p.sc = 2;
       p.alpha = 1.0; p.s2 = 0.0; p.beta = 0.0;
       p.kchunk = KC3; p.cstride = (i64)(nb + nh) * (nb + nh);
-      const int pid = (int)B->k3p_h.size();
-      B->k3p_h.push_back(p);
-      for (int ks = 0; ks < B->k3ks_h[q]; ++ks)
-        for (int tm = 0; tm < tiles(nb + nh); ++tm)
-          for (int tn = 0; tn < tiles(nb + nh); ++tn) B->k3.host.push_back(GemmTask{pid, tm, tn, ks});
+      B->k3.probs_h.push_back(p);
+      B->k3.nks.push_back(B->k3ks_h[q]);
     }
-    B->k3.chan_begin.push_back((int)B->k3.host.size());
+    B->k3.group_end.push_back((int)B->k3.probs_h.size());
   }
   B->pairs.n = (int)B->pairs.host.size();
-  B->k2.n = (int)B->k2.host.size();
-  B->k3.n = (int)B->k3.host.size();
+  B->pp1.tile = B->tile_req[0] & 7; B->k2.tile = B->tile_req[1] & 7; B->k3.tile = B->tile_req[2] & 7;
+  B->ph.tile = B->tile_req[3] & 7;
   // one-body outputs of 121 / 221 (hh p<=q, pp p<=q, ph), same (l, j, tz)
   B->one_h.clear();
   auto same = [&](int x, int y) { return b.jj[x] == b.jj[y] && b.ll[x] == b.ll[y] && b.tz[x] == b.tz[y]; };
@@ -394,13 +488,11 @@ int build_plan(imsrg_basis_s* B) {
   if ((rc = upload(B, B->seg_g3.host, &B->seg_g3.dev))) return rc;
   if ((rc = upload(B, B->seg_epi.host, &B->seg_epi.dev))) return rc;
   if ((rc = upload(B, B->cc.host, &B->cc.dev))) return rc;
-  if ((rc = upload(B, B->ph.host, &B->ph.dev))) return rc;
-  if ((rc = upload(B, B->php_h, &B->php))) return rc;
   if ((rc = upload(B, B->pairs.host, &B->pairs.dev))) return rc;
-  if ((rc = upload(B, B->k2.host, &B->k2.dev))) return rc;
-  if ((rc = upload(B, B->k2p_h, &B->k2p))) return rc;
-  if ((rc = upload(B, B->k3.host, &B->k3.dev))) return rc;
-  if ((rc = upload(B, B->k3p_h, &B->k3p))) return rc;
+  for (GemmFamily* f : {&B->pp1, &B->k2, &B->k3, &B->ph}) {
+    if ((rc = upload(B, f->probs_h, &f->probs))) return rc;
+    if ((rc = family_tasks(B, *f, f->tile))) return rc;
+  }
   if ((rc = upload(B, B->one_h, &B->one))) return rc;
   if ((rc = upload(B, B->quad_h, &B->quad))) return rc;
   if ((rc = upload(B, B->part_h, &B->partners))) return rc;
@@ -438,7 +530,7 @@ int ensure_dev(imsrg_basis_s* B) {
   UP(jj, jj) UP(ll, ll) UP(tz, tz) UP(con, con) UP(cidx, cidx) UP(holes, holes) UP(parts, parts)
   UP(spstart, spstart) UP(spmem, spmem) UP(spblk, spblk) UP(xmap, xmap) UP(dims, dims) UP(chJ, chJ)
   UP(goff, goff) UP(pairs, pairs) UP(pairoff, pairoff) UP(rlen, rlen) UP(ncol, ncol) UP(ccJ, ccJ)
-  UP(ccoff, ccoff) UP(wcoff, wcoff) UP(rmap, rmap) UP(nbmap, nbmap) UP(ccrows, ccrows) UP(ccrowoff, ccrowoff)
+  UP(ccoff, ccoff) UP(wcoff, wcoff) UP(rmap, rmap) UP(nbmap, nbmap) UP(rmapJ, rmapJ) UP(rmapJT, rmapJT) UP(ccrows, ccrows) UP(ccrowoff, ccrowoff)
   UP(cccols, cccols) UP(cccoloff, cccoloff) UP(six, six)
 #undef UP
   if ((rc = build_plan(B))) return rc;
@@ -462,6 +554,21 @@ void run(imsrg_basis_s* B, TaskList<T>& tl, F&& launch) {
 
 hipStream_t S(imsrg_stream s) { return (hipStream_t)s; }
 
+// one GEMM family: one XCD-mapped launch (grouped) or one launch per group
+void run_gemm(imsrg_basis_s* B, GemmFamily& f, const Slots& sl, hipStream_t s, double ascale = 1.0,
+              double s2scale = 1.0) {
+  if (f.tasks.n == 0) return;
+  if (B->path == IMSRG_PATH_GROUPED) {
+    launch_gemm_tiled(f.tile, sl, f.probs, f.tasks.dev, f.tasks.grid, f.tasks.xmap, s, ascale, s2scale);
+    return;
+  }
+  const XcdMap ident{{0}, {-1}};
+  for (size_t g = 0; g + 1 < f.tasks.chan_begin.size(); ++g) {
+    const int b0 = f.tasks.chan_begin[g], e0 = f.tasks.chan_begin[g + 1];
+    if (e0 > b0) launch_gemm_tiled(f.tile, sl, f.probs, f.tasks.dev + b0, e0 - b0, ident, s, ascale, s2scale);
+  }
+}
+
 // ------------------------------------------------------------ term bodies
 void t_e0(imsrg_basis_s* B, const double* L, int Lh, const double* R, int Rh, double* dst, hipStream_t s) {
   launch_e0_partials(B->D, L, Lh, R, Rh, B->part, s);
@@ -481,24 +588,27 @@ void t_122(imsrg_basis_s* B, const double* L, int Lh, const double* R, int Rh, d
   });
 }
 void t_pphh(imsrg_basis_s* B, const double* L, int Lh, const double* R, int Rh, double* RES, int RESh,
-            double* w1, double* w2, hipStream_t s) {
+            double* w1, double* w2, hipStream_t s, int ref_outputs = 1) {
   (void)RESh;
+  Slots s1{};
+  s1.p[0] = L; s1.p[1] = R; s1.p[2] = w1;
+  run_gemm(B, B->pp1, s1, s, 1.0, -(double)Rh);
   run(B, B->pairs,
-      [&](const PairTask* t, int n) { launch_pppp_pairs(B->D, t, n, L, Lh, R, Rh, RES, w1, w2, 1, s); });
+      [&](const PairTask* t, int n) { launch_pppp_sym(B->D, t, n, L, Lh, R, Rh, RES, w1, w2, ref_outputs, s); });
   Slots s2{};
   s2.p[0] = L; s2.p[1] = R; s2.p[2] = B->K2scr;
-  run(B, B->k2, [&](const GemmTask* t, int n) { launch_gemm(s2, B->k2p, t, n, s); });
+  run_gemm(B, B->k2, s2, s);
   Slots s3{};
   s3.p[0] = L; s3.p[1] = R; s3.p[2] = B->Pscr;
-  run(B, B->k3, [&](const GemmTask* t, int n) { launch_gemm(s3, B->k3p, t, n, s); });
+  run_gemm(B, B->k3, s3, s);
   run(B, B->seg_epi, [&](const SegTask* t, int n) {
     launch_pphh_epilogue(B->D, t, n, L, Lh, R, Rh, RES, w1, w2, B->Pscr, B->poff, B->k3ks, B->K2scr, B->k2off,
                          B->k2ks, s);
   });
 }
-void t_221(imsrg_basis_s* B, int Lh, int Rh, double* RES, int RESh, const double* w1, const double* w2,
-           hipStream_t s) {
-  launch_comm221(B->D, B->one, (int)B->one_h.size(), Lh * Rh, w1, w2, RES, RESh, s);
+void t_221(imsrg_basis_s* B, const double* L, int Lh, const double* R, int Rh, double* RES, int RESh,
+           const double* w1, const double* w2, hipStream_t s) {
+  launch_comm221(B->D, B->one, (int)B->one_h.size(), L, Lh, R, Rh, w1, w2, RES, RESh, s);
 }
 void t_pandya(imsrg_basis_s* B, const double* op, int herm, double* CC, hipStream_t s) {
   run(B, B->cc, [&](const CCTask* t, int n) { launch_pandya(B->D, t, n, op, herm, CC, s); });
@@ -507,7 +617,7 @@ void t_ph(imsrg_basis_s* B, const double* LCC, int Lh, const double* RCC, int Rh
           double* WCC, hipStream_t s) {
   Slots sl{};
   sl.p[0] = LCC; sl.p[1] = RCC; sl.p[2] = WCC;
-  run(B, B->ph, [&](const GemmTask* t, int n) { launch_gemm(sl, B->php, t, n, s, (double)Lh, -(double)(Rh * Lh)); });
+  run_gemm(B, B->ph, sl, s, (double)Lh, -(double)(Rh * Lh));
   if (B->path == IMSRG_PATH_GROUPED)
     launch_comm222ph_quad(B->D, B->quad, (int)B->quad_h.size(), WCC, Lh, Rh, RES, RESh, s);
   else
@@ -531,8 +641,8 @@ void t_full(imsrg_basis_s* B, const double* L, int Lh, const double* R, int Rh, 
   t_111(B, L, Lh, R, Rh, RES, s);
   t_121(B, L, Lh, R, Rh, RES, RESh, s);
   t_122(B, L, Lh, R, Rh, RES, RESh, s);
-  t_pphh(B, L, Lh, R, Rh, RES, RESh, w1, w2, s);
-  t_221(B, Lh, Rh, RES, RESh, w1, w2, s);
+  t_pphh(B, L, Lh, R, Rh, RES, RESh, w1, w2, s, 0);
+  t_221(B, L, Lh, R, Rh, RES, RESh, w1, w2, s);
   t_ph(B, LCC, Lh, RCC, Rh, RES, RESh, WCC, s);
 }
 
@@ -873,8 +983,7 @@ int imsrg_commutator_222_pp_hh(imsrg_basis B, const double* L, int Lh, const dou
 int imsrg_commutator_221(imsrg_basis B, const double* L, int Lh, const double* R, int Rh, double* RES, int RESh,
                          const double* w1, const double* w2, imsrg_stream stream) {
   PREP();
-  (void)L; (void)R;
-  t_221(B, Lh, Rh, RES, RESh, w1, w2, s);
+  t_221(B, L, Lh, R, Rh, RES, RESh, w1, w2, s);
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -1029,23 +1138,49 @@ int imsrg_magnus_decouple(imsrg_flow F, double* HS, double* G, double ds, double
   return 0;
 }
 
+int imsrg_set_gemm_tile(imsrg_basis B, int family, int tile) {
+  const int shape = tile & 7;
+  if (!B || family < 0 || family > 3 || shape > 2 || tile < 0 || tile > 10) return fail(IMSRG_EINVAL, "bad gemm family / tile");
+  GemmFamily* f[4] = {&B->pp1, &B->k2, &B->k3, &B->ph};
+  f[family]->xcd_aware = (tile & IMSRG_TILE_XCD) != 0;
+  B->tile_req[family] = tile;
+  if (!B->dev_ready) return 0;   // applied when the plan is built
+  return family_tasks(B, *f[family], shape);
+}
+
+int imsrg_get_gemm_tile(imsrg_basis B, int family) {
+  if (!B || family < 0 || family > 3) return -1;
+  return B->tile_req[family];
+}
+
 int imsrg_profile_kernel(imsrg_basis B, int which, const double* L, int Lh, const double* R, int Rh, double* RES,
                          int RESh, double* w1, double* w2, double* LCC, double* RCC, double* WCC, int iters,
                          double* ms, imsrg_stream stream) {
   PREP();
-  if (iters < 1 || which < 0 || which > IMSRG_K_COMMUTATOR) return fail(IMSRG_EINVAL, "bad profile request");
+  if (iters < 1 || which < 0 || which > IMSRG_K_COPY) return fail(IMSRG_EINVAL, "bad profile request");
   auto body = [&](int k) {
     switch (k) {
-      case IMSRG_K_PPPP:
-        run(B, B->pairs, [&](const PairTask* t, int n) { launch_pppp_pairs(B->D, t, n, L, Lh, R, Rh, RES, w1, w2, 1, s); });
+      case IMSRG_K_PPPP: {
+        Slots s1{};
+        s1.p[0] = L; s1.p[1] = R; s1.p[2] = w1;
+        run_gemm(B, B->pp1, s1, s, 1.0, -(double)Rh);
+        run(B, B->pairs,
+            [&](const PairTask* t, int n) { launch_pppp_sym(B->D, t, n, L, Lh, R, Rh, RES, w1, w2, 0, s); });
         break;
+      }
+      case IMSRG_K_PPPP_GEMM: {
+        Slots s1{};
+        s1.p[0] = L; s1.p[1] = R; s1.p[2] = w1;
+        run_gemm(B, B->pp1, s1, s, 1.0, -(double)Rh);
+        break;
+      }
       case IMSRG_K_PPHH_SMALL: {
         Slots s2{};
         s2.p[0] = L; s2.p[1] = R; s2.p[2] = B->K2scr;
-        run(B, B->k2, [&](const GemmTask* t, int n) { launch_gemm(s2, B->k2p, t, n, s); });
+        run_gemm(B, B->k2, s2, s);
         Slots s3{};
         s3.p[0] = L; s3.p[1] = R; s3.p[2] = B->Pscr;
-        run(B, B->k3, [&](const GemmTask* t, int n) { launch_gemm(s3, B->k3p, t, n, s); });
+        run_gemm(B, B->k3, s3, s);
         run(B, B->seg_epi, [&](const SegTask* t, int n) {
           launch_pphh_epilogue(B->D, t, n, L, Lh, R, Rh, RES, w1, w2, B->Pscr, B->poff, B->k3ks, B->K2scr, B->k2off,
                                B->k2ks, s);
@@ -1055,8 +1190,7 @@ int imsrg_profile_kernel(imsrg_basis B, int which, const double* L, int Lh, cons
       case IMSRG_K_PH_GEMM: {
         Slots sl{};
         sl.p[0] = LCC; sl.p[1] = RCC; sl.p[2] = WCC;
-        run(B, B->ph,
-            [&](const GemmTask* t, int n) { launch_gemm(sl, B->php, t, n, s, (double)Lh, -(double)(Rh * Lh)); });
+        run_gemm(B, B->ph, sl, s, (double)Lh, -(double)(Rh * Lh));
         break;
       }
       case IMSRG_K_PH_SCATTER:
@@ -1068,10 +1202,11 @@ int imsrg_profile_kernel(imsrg_basis B, int which, const double* L, int Lh, cons
         break;
       case IMSRG_K_122: t_122(B, L, Lh, R, Rh, RES, RESh, s); break;
       case IMSRG_K_PANDYA: t_pandya(B, R, Rh, RCC, s); break;
+      case IMSRG_K_COPY: launch_copy(RES, R, B->b.oplen, s); break;
       case IMSRG_K_ONEBODY:
         t_111(B, L, Lh, R, Rh, RES, s);
         t_121(B, L, Lh, R, Rh, RES, RESh, s);
-        t_221(B, Lh, Rh, RES, RESh, w1, w2, s);
+        t_221(B, L, Lh, R, Rh, RES, RESh, w1, w2, s);
         break;
       case IMSRG_K_COMMUTATOR:
         t_pandya(B, L, Lh, LCC, s);

@@ -87,8 +87,8 @@ __device__ __forceinline__ double v_elem(const DevBasis& B, const double* op, in
 __device__ __forceinline__ double sixj(const DevBasis& B, int j1, int j2, int j3, int j4, int j5, int j6) {
   if (!(dtri(j1, j2, j3) && dtri(j4, j5, j3) && dtri(j1, j5, j6) && dtri(j4, j2, j6))) return 0.0;
   const int nh = B.nh6;
-  const i64 idx = (((((i64)(j1 >> 1) * nh + (j2 >> 1)) * nh + (j4 >> 1)) * nh + (j5 >> 1)) * B.nJ6 + (j3 >> 1)) *
-                      B.nJ6 + (j6 >> 1);
+  const i64 idx = (((((i64)(j1 >> 1) * nh + (j2 >> 1)) * nh + (j4 >> 1)) * nh + (j5 >> 1)) * B.nJ6 + (j6 >> 1)) *
+                      B.nJ6 + (j3 >> 1);
   return B.six[idx];
 }
 

@@ -1,0 +1,184 @@
+// Grouped FP64 MFMA GEMM for gfx950 (v_mfma_f64_16x16x4f64).
+//
+// One launch runs a list of tile tasks over a list of problems
+//   C = alpha * (op(A1) op(B1) + s2 * op(A2) op(B2))  (+ beta C),
+// operands addressed as (slot, offset) so a plan serves every call.  Tile
+// shapes 64x64, 128x64, 128x128 (256 threads = 4 waves in 2 x 2, each wave
+// a (TBM/2) x (TBN/2) block of 16x16 MFMA tiles), K in steps of 16 through
+// two LDS buffers with the next K-tile prefetched into registers.  Split K:
+// task.ks selects K-chunk [ks*kchunk, (ks+1)*kchunk) and slab C + ks*cstride.
+#include <hip/hip_runtime.h>
+
+#include "device.hpp"
+#include "kernels.hpp"
+
+namespace imsrg {
+
+namespace {
+
+constexpr int GBK = 16;
+
+// per thread and K-tile: TBM*16/256 elements of op(A), TBN*16/256 of op(B)
+template <int TBM, int TBN>
+struct Regs {
+  double a[TBM * GBK / 256];
+  double b[TBN * GBK / 256];
+};
+
+template <int TBM, int TBN>
+__device__ __forceinline__ void fetch(Regs<TBM, TBN>& t, const double* A, int lda, int tA, const double* Bm, int ldb,
+                                      int tB, int M, int N, int K, int m0, int n0, int k0, double scale) {
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int r = 0; r < TBM * GBK / 256; ++r) {
+    const int e = tid + 256 * r;
+    int i, kk;
+    if (!tA) { i = e % TBM; kk = e / TBM; }
+    else { kk = e % GBK; i = e / GBK; }
+    const int gi = m0 + i, gk = k0 + kk;
+    double v = 0.0;
+    if (gi < M && gk < K) v = tA ? A[gk + (i64)gi * lda] : A[gi + (i64)gk * lda];
+    t.a[r] = v * scale;
+  }
+#pragma unroll
+  for (int r = 0; r < TBN * GBK / 256; ++r) {
+    const int e = tid + 256 * r;
+    int j, kk;
+    if (!tB) { kk = e % GBK; j = e / GBK; }
+    else { j = e % TBN; kk = e / TBN; }
+    const int gj = n0 + j, gk = k0 + kk;
+    double w = 0.0;
+    if (gj < N && gk < K) w = tB ? Bm[gj + (i64)gk * ldb] : Bm[gk + (i64)gj * ldb];
+    t.b[r] = w;
+  }
+}
+
+template <int TBM, int TBN>
+__device__ __forceinline__ void stash(const Regs<TBM, TBN>& t, double (*As)[TBM + 1], double (*Bs)[TBN + 1], int tA,
+                                      int tB) {
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int r = 0; r < TBM * GBK / 256; ++r) {
+    const int e = tid + 256 * r;
+    if (!tA) As[e / TBM][e % TBM] = t.a[r];
+    else As[e % GBK][e / GBK] = t.a[r];
+  }
+#pragma unroll
+  for (int r = 0; r < TBN * GBK / 256; ++r) {
+    const int e = tid + 256 * r;
+    if (!tB) Bs[e % GBK][e / GBK] = t.b[r];
+    else Bs[e / TBN][e % TBN] = t.b[r];
+  }
+}
+
+// acc += op(A)[m0:m0+TBM, k0:k1] op(B)[k0:k1, n0:n0+TBN].  The MFMA takes
+// op(B)^T as its A operand and op(A)^T as its B operand: register r of lane
+// l of tile (mi, ni) holds C row wm*TBM/2 + mi*16 + l%16, column
+// wn*TBN/2 + ni*16 + l/16 + 4r (16 consecutive rows per row group, so the
+// column-major stores are contiguous).
+template <int TBM, int TBN>
+__device__ __forceinline__ void mma(v4d* acc, double (*As)[GBK][TBM + 1], double (*Bs)[GBK][TBN + 1], const double* A,
+                                    int lda, int tA, const double* Bm, int ldb, int tB, int M, int N, int k0s, int k1,
+                                    int m0, int n0, double scale) {
+  constexpr int MI = TBM / 32, NI = TBN / 32;   // acc[mi * NI + ni]
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int lr = lane & 15, lk = lane >> 4;
+  if (k0s >= k1) return;
+  Regs<TBM, TBN> t;
+  fetch<TBM, TBN>(t, A, lda, tA, Bm, ldb, tB, M, N, k1, m0, n0, k0s, scale);
+  stash<TBM, TBN>(t, As[0], Bs[0], tA, tB);
+  __syncthreads();
+  int cur = 0;
+  for (int k0 = k0s; k0 < k1; k0 += GBK) {
+    const bool more = (k0 + GBK < k1);
+    if (more) fetch<TBM, TBN>(t, A, lda, tA, Bm, ldb, tB, M, N, k1, m0, n0, k0 + GBK, scale);
+#pragma unroll
+    for (int kk = 0; kk < GBK; kk += 4) {
+      double a[MI], b[NI];
+#pragma unroll
+      for (int mi = 0; mi < MI; ++mi) a[mi] = As[cur][kk + lk][wm * (TBM / 2) + mi * 16 + lr];
+#pragma unroll
+      for (int ni = 0; ni < NI; ++ni) b[ni] = Bs[cur][kk + lk][wn * (TBN / 2) + ni * 16 + lr];
+#pragma unroll
+      for (int mi = 0; mi < MI; ++mi)
+#pragma unroll
+        for (int ni = 0; ni < NI; ++ni)
+          acc[mi * NI + ni] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[ni], a[mi], acc[mi * NI + ni], 0, 0, 0);
+    }
+    if (more) stash<TBM, TBN>(t, As[cur ^ 1], Bs[cur ^ 1], tA, tB);
+    __syncthreads();
+    cur ^= 1;
+  }
+}
+
+template <int TBM, int TBN>
+__global__ __launch_bounds__(256) void k_gemm_t(Slots sl, const GemmProb* probs, const GemmTask* tasks, XcdMap xm,
+                                                double ascale, double s2scale) {
+  constexpr int MI = TBM / 32, NI = TBN / 32;
+  __shared__ double As[2][GBK][TBM + 1];
+  __shared__ double Bs[2][GBK][TBN + 1];
+  const int ti = xcd_task(xm, blockIdx.x);
+  if (ti < 0) return;
+  const GemmTask t = tasks[ti];
+  const GemmProb pr = probs[t.prob];
+  const int m0 = t.tm * TBM, n0 = t.tn * TBN;
+  const int kb = pr.kchunk > 0 ? t.ks * pr.kchunk : 0;
+  const int ke = pr.kchunk > 0 ? min(pr.k, kb + pr.kchunk) : pr.k;
+  v4d acc[MI * NI];
+#pragma unroll
+  for (int a = 0; a < MI * NI; ++a) acc[a] = v4d{0.0, 0.0, 0.0, 0.0};
+  mma<TBM, TBN>(acc, As, Bs, sl.p[pr.sa1] + pr.a1, pr.lda, pr.ta1, sl.p[pr.sb1] + pr.b1, pr.ldb, pr.tb1, pr.m, pr.n,
+                kb, ke, m0, n0, 1.0);
+  if (pr.nprod == 2) {
+    // second product: same K range, or its own K (k2 > 0) with its own
+    // leading dimensions packed in a2/b2's problem (lda2 = ldb2 = m here)
+    if (pr.k2 > 0)
+      mma<TBM, TBN>(acc, As, Bs, sl.p[pr.sa2] + pr.a2, pr.m, pr.ta2, sl.p[pr.sb2] + pr.b2, pr.n, pr.tb2, pr.m, pr.n,
+                    0, pr.k2, m0, n0, pr.s2 * s2scale);
+    else
+      mma<TBM, TBN>(acc, As, Bs, sl.p[pr.sa2] + pr.a2, pr.lda, pr.ta2, sl.p[pr.sb2] + pr.b2, pr.ldb, pr.tb2, pr.m,
+                    pr.n, kb, ke, m0, n0, pr.s2 * s2scale);
+  }
+  double* C = const_cast<double*>(sl.p[pr.sc]) + pr.c + (i64)t.ks * pr.cstride;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const double al = pr.alpha * ascale;
+#pragma unroll
+  for (int mi = 0; mi < MI; ++mi)
+#pragma unroll
+    for (int ni = 0; ni < NI; ++ni)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = m0 + wm * (TBM / 2) + mi * 16 + (lane & 15);
+        const int col = n0 + wn * (TBN / 2) + ni * 16 + (lane >> 4) + 4 * r;
+        if (row < pr.m && col < pr.n) {
+          const i64 o = row + (i64)col * pr.ldc;
+          const double v = al * acc[mi * NI + ni][r];
+          C[o] = (pr.beta == 0.0) ? v : v + pr.beta * C[o];
+        }
+      }
+}
+
+}  // namespace
+
+int tile_rows(int tile) { return tile == 0 ? 64 : 128; }
+int tile_cols(int tile) { return tile == 2 ? 128 : 64; }
+
+void launch_gemm_tiled(int tile, const Slots& sl, const GemmProb* probs, const GemmTask* tasks, int grid,
+                       const XcdMap& xm, hipStream_t s, double ascale, double s2scale) {
+  if (grid <= 0) return;
+  switch (tile) {
+    case 0:
+      hipLaunchKernelGGL((k_gemm_t<64, 64>), dim3(grid), dim3(256), 0, s, sl, probs, tasks, xm, ascale, s2scale);
+      break;
+    case 1:
+      hipLaunchKernelGGL((k_gemm_t<128, 64>), dim3(grid), dim3(256), 0, s, sl, probs, tasks, xm, ascale, s2scale);
+      break;
+    default:
+      hipLaunchKernelGGL((k_gemm_t<128, 128>), dim3(grid), dim3(256), 0, s, sl, probs, tasks, xm, ascale, s2scale);
+      break;
+  }
+}
+
+}  // namespace imsrg

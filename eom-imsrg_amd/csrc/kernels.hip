@@ -3,10 +3,8 @@
 //  * k_comm122, k_comm222ph_scatter, k_pandya: per-element gathers, one
 //    thread per output element, 256-element tiles from a task list that
 //    spans every channel (grouped) or one channel (plain path).
-//  * k_gemm, k_pppp_pairs: FP64 MFMA (v_mfma_f64_16x16x4f64) tiles staged
-//    through LDS.  k_pppp_pairs computes the tile pair (I,J)/(J,I) of the
-//    pppp product so the reference's  W - pm W^T  epilogue of 222_pp_hh is
-//    applied without a second pass over HBM.
+//  * the FP64 MFMA GEMMs live in gemm.hip; k_pppp_sym applies the
+//    reference's  W - pm W^T  epilogue of 222_pp_hh to the pppp product.
 //  * one-body terms, norms, axpy, generator, mbpt2 and normal ordering.
 #include <hip/hip_runtime.h>
 
@@ -176,7 +174,7 @@ __device__ __forceinline__ void ph_element(const DevBasis& B, const double* WCC,
   const int ji = B.jj[i], jj = B.jj[j], jk = B.jj[k], jl = B.jj[l];
   const int li = B.ll[i], lj = B.ll[j], lk = B.ll[k], ll = B.ll[l];
   const int ti = B.tz[i], tj = B.tz[j], tk = B.tz[k], tl = B.tz[l];
-  const int N = B.N, nJ = B.nJ, JTM = B.JTM;
+  const int N = B.N, JTM = B.JTM;
   double sm = 0.0;
   // "inside" recoupling
   int Tz = abs(ti - tk) / 2;
@@ -191,8 +189,10 @@ __device__ __forceinline__ void ph_element(const DevBasis& B, const double* WCC,
     const double* W = WCC + B.wcoff[qx];
     const i64 rr = B.rlen[qx];
     const int jo = JP >> 1;
-    const int rjl = B.rmap[((i64)j * N + l) * nJ + jo], rik = B.rmap[((i64)i * N + k) * nJ + jo];
-    const int gjl = B.rmap[((i64)l * N + j) * nJ + jo], gik = B.rmap[((i64)k * N + i) * nJ + jo];
+    const int* rJ = B.rmapJ + (i64)jo * N * N;
+    const int* rT = B.rmapJT + (i64)jo * N * N;
+    const int rjl = rJ[j * N + l], rik = rT[k * N + i];
+    const int gjl = rJ[l * N + j], gik = rJ[k * N + i];
     sm = sm - m1pow((jl + jk + Jtot) / 2) * (wprime(W, rr, gik, rjl, mpm) + phase * wprime(W, rr, gjl, rik, mpm)) *
                   sixj(B, ji, jj, Jtot, jl, jk, JP);
   }
@@ -209,8 +209,10 @@ __device__ __forceinline__ void ph_element(const DevBasis& B, const double* WCC,
     const double* W = WCC + B.wcoff[qx];
     const i64 rr = B.rlen[qx];
     const int jo = JP >> 1;
-    const int ril = B.rmap[((i64)i * N + l) * nJ + jo], rjk = B.rmap[((i64)j * N + k) * nJ + jo];
-    const int gil = B.rmap[((i64)l * N + i) * nJ + jo], gjk = B.rmap[((i64)k * N + j) * nJ + jo];
+    const int* rJ = B.rmapJ + (i64)jo * N * N;
+    const int* rT = B.rmapJT + (i64)jo * N * N;
+    const int ril = rT[l * N + i], rjk = rJ[j * N + k];
+    const int gil = rJ[l * N + i], gjk = rJ[k * N + j];
     sm = sm + (wprime(W, rr, gil, rjk, mpm) + phase * wprime(W, rr, gjk, ril, mpm)) *
                   sixj(B, ji, jj, Jtot, jk, jl, JP);
   }
@@ -427,34 +429,61 @@ void launch_comm121(const DevBasis& B, const OneTask* t, int nt, const double* L
 // ============================================================ commutator_221
 // commutators.f90:435-578 (accumulates; reads w1/w2 of 222_pp_hh, herm 1).
 // One workgroup per one-body output; thread x takes intermediate orbit c
-// (holes first with w1, then particles with w2) and its J sum.
-__global__ __launch_bounds__(128) void k_comm221(DevBasis B, const OneTask* tasks, int pm, const double* w1,
-                                                 const double* w2, double* RES, int RESh) {
+// (holes first with w1, then particles with w2) and its J sum.  Elements
+// of w2's pppp blocks (W2 = Rh L3 R3^T, rank nh) are formed from L and R
+// here, so 222_pp_hh need not store that block.
+__device__ __forceinline__ double w2_elem(const DevBasis& B, const double* w2, const double* L, const double* R,
+                                          int Rh, int a, int b, int c, int d, int J) {
+  const VPos p = vpos(B, a, b, c, d, J);
+  if (p.off < 0) return 0.0;
+  const int C1 = B.con[a] + B.con[b], C2 = B.con[c] + B.con[d];
+  if (C1 != 0 || C2 != 0) return vat(p, w2, 1);
+  const int q = 3 * J + 2 * ((B.tz[a] + B.tz[b]) / 2 + 1) + ((B.ll[a] + B.ll[b]) & 1);
+  const int np = B.dims[q * 3], nh = B.dims[q * 3 + 2];
+  const i64 o3 = B.goff[q * 6 + 2];
+  const i64 rel = p.off - B.goff[q * 6 + 0];
+  const int x = (int)(rel % np), y = (int)(rel / np);
+  double s = 0.0;
+  for (int h = 0; h < nh; ++h) s += L[o3 + x + (i64)h * np] * R[o3 + y + (i64)h * np];
+  return (s * Rh) * p.pre;
+}
+
+__global__ __launch_bounds__(128) void k_comm221(DevBasis B, const OneTask* tasks, const double* L, int Lh,
+                                                 const double* R, int Rh, const double* w1, const double* w2,
+                                                 double* RES, int RESh) {
   __shared__ double red[128];
   const OneTask t = tasks[blockIdx.x];
   const int ik = t.pk, jk = t.qk, ji = B.jj[ik];
+  const int pm = Lh * Rh;
   const bool both = (t.kind != 2);
   double v = 0.0;
   for (int x = threadIdx.x; x < B.N; x += blockDim.x) {
     const bool hole = x < B.A;
     const int ck = hole ? B.holes[x] : B.parts[x - B.A];
-    const double* w = hole ? w1 : w2;
     const int jc = B.jj[ck];
     for (int JT = abs(jc - ji); JT <= jc + ji; JT += 2) {
-      if (both)
-        v = v + (v_elem(B, w, 1, ck, ik, ck, jk, JT) - pm * v_elem(B, w, 1, ck, jk, ck, ik, JT)) * (JT + 1);
-      else
-        v = v + v_elem(B, w, 1, ck, ik, ck, jk, JT) * (JT + 1);
+      if (hole) {
+        if (both)
+          v = v + (v_elem(B, w1, 1, ck, ik, ck, jk, JT) - pm * v_elem(B, w1, 1, ck, jk, ck, ik, JT)) * (JT + 1);
+        else
+          v = v + v_elem(B, w1, 1, ck, ik, ck, jk, JT) * (JT + 1);
+      } else {
+        if (both)
+          v = v + (w2_elem(B, w2, L, R, Rh, ck, ik, ck, jk, JT) - pm * w2_elem(B, w2, L, R, Rh, ck, jk, ck, ik, JT)) *
+                      (JT + 1);
+        else
+          v = v + w2_elem(B, w2, L, R, Rh, ck, ik, ck, jk, JT) * (JT + 1);
+      }
     }
   }
   const double sm = block_sum(v, red);
   if (threadIdx.x == 0) onebody_update(B, t, sm / (ji + 1.0), RES, RESh);
 }
 
-void launch_comm221(const DevBasis& B, const OneTask* t, int nt, int pm, const double* w1, const double* w2,
-                    double* RES, int RESh, hipStream_t s) {
+void launch_comm221(const DevBasis& B, const OneTask* t, int nt, const double* L, int Lh, const double* R, int Rh,
+                    const double* w1, const double* w2, double* RES, int RESh, hipStream_t s) {
   if (nt <= 0) return;
-  hipLaunchKernelGGL(k_comm221, dim3(nt), dim3(128), 0, s, B, t, pm, w1, w2, RES, RESh);
+  hipLaunchKernelGGL(k_comm221, dim3(nt), dim3(128), 0, s, B, t, L, Lh, R, Rh, w1, w2, RES, RESh);
 }
 
 // ================================================= commutator_110 + 220 (E0)
@@ -522,210 +551,75 @@ void launch_e0_final(const double* part, int nblocks, int use220, int use110, do
   hipLaunchKernelGGL(k_e0_final, dim3(1), dim3(256), 0, s, part, nblocks, use220, use110, dst, accumulate);
 }
 
-// ========================================================= FP64 MFMA GEMM
-// 64 x 64 output tile per 256-thread workgroup (4 waves, 2 x 2 of 32 x 32),
-// K staged through LDS in steps of 16.  v_mfma_f64_16x16x4f64 operand
-// layout on gfx950 (measured with a one-wave probe): A lane l ->
-// A[l%16][l/16], B lane l -> B[l/16][l%16], D lane l, element r ->
-// D[l/16 + 4r][l%16].
-constexpr int BM = 64, BN = 64, BK = 16;
-
-__device__ __forceinline__ void load_a(double (*As)[BM + 1], const double* A, int lda, int tA, int M, int K, int m0,
-                                       int k0, double scale) {
-  const int tid = threadIdx.x;
-#pragma unroll
-  for (int r = 0; r < (BM * BK) / 256; ++r) {
-    const int e = tid + 256 * r;
-    int i, kk;
-    if (!tA) { i = e % BM; kk = e / BM; }
-    else { kk = e % BK; i = e / BK; }
-    const int gi = m0 + i, gk = k0 + kk;
-    double v = 0.0;
-    if (gi < M && gk < K) v = tA ? A[gk + (i64)gi * lda] : A[gi + (i64)gk * lda];
-    As[kk][i] = v * scale;
-  }
-}
-
-__device__ __forceinline__ void load_b(double (*Bs)[BN + 1], const double* Bm, int ldb, int tB, int N, int K, int n0,
-                                       int k0) {
-  const int tid = threadIdx.x;
-#pragma unroll
-  for (int r = 0; r < (BN * BK) / 256; ++r) {
-    const int e = tid + 256 * r;
-    int j, kk;
-    if (!tB) { kk = e % BK; j = e / BK; }
-    else { j = e % BN; kk = e / BN; }
-    const int gj = n0 + j, gk = k0 + kk;
-    double v = 0.0;
-    if (gj < N && gk < K) v = tB ? Bm[gj + (i64)gk * ldb] : Bm[gk + (i64)gj * ldb];
-    Bs[kk][j] = v;
-  }
-}
-
-// acc[2][2] += op(A)[m0:m0+64, k0:k1] * op(B)[k0:k1, n0:n0+64].  The MFMA
-// is fed op(B)^T as its A operand and op(A)^T as its B operand, so register
-// r of lane l holds C[m0 + wm*32 + mi*16 + l%16][n0 + wn*32 + ni*16 + l/16 + 4r]:
-// the 16 lanes of a row group hold 16 consecutive rows of one column, which
-// makes the column-major stores of C contiguous.
-__device__ __forceinline__ int c_row(int wm, int mi, int lane, int r) { (void)r; return wm * 32 + mi * 16 + (lane & 15); }
-__device__ __forceinline__ int c_col(int wn, int ni, int lane, int r) { return wn * 32 + ni * 16 + (lane >> 4) + 4 * r; }
-
-__device__ __forceinline__ void tile_mma(v4d (&acc)[2][2], double (*As)[BM + 1], double (*Bs)[BN + 1],
-                                         const double* A, int lda, int tA, const double* Bm, int ldb, int tB, int M,
-                                         int N, int k0s, int k1, int m0, int n0, double scale) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int wm = wave >> 1, wn = wave & 1;
-  const int lr = lane & 15, lk = lane >> 4;
-  for (int k0 = k0s; k0 < k1; k0 += BK) {
-    load_a(As, A, lda, tA, M, k1, m0, k0, scale);
-    load_b(Bs, Bm, ldb, tB, N, k1, n0, k0);
-    __syncthreads();
-#pragma unroll
-    for (int kk = 0; kk < BK; kk += 4) {
-      const double a0 = As[kk + lk][wm * 32 + lr];
-      const double a1 = As[kk + lk][wm * 32 + 16 + lr];
-      const double b0 = Bs[kk + lk][wn * 32 + lr];
-      const double b1 = Bs[kk + lk][wn * 32 + 16 + lr];
-      acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(b0, a0, acc[0][0], 0, 0, 0);
-      acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(b1, a0, acc[0][1], 0, 0, 0);
-      acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(b0, a1, acc[1][0], 0, 0, 0);
-      acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(b1, a1, acc[1][1], 0, 0, 0);
-    }
-    __syncthreads();
-  }
-}
-
-__global__ __launch_bounds__(256) void k_gemm(Slots sl, const GemmProb* probs, const GemmTask* tasks,
-                                              double ascale, double s2scale) {
-  __shared__ double As[BK][BM + 1];
-  __shared__ double Bs[BK][BN + 1];
-  const GemmTask t = tasks[blockIdx.x];
-  const GemmProb pr = probs[t.prob];
-  const int m0 = t.tm * BM, n0 = t.tn * BN;
-  // split K: chunk t.ks covers [ks*kchunk, (ks+1)*kchunk) and lands in slab ks of C
-  const int kb = pr.kchunk > 0 ? t.ks * pr.kchunk : 0;
-  const int ke = pr.kchunk > 0 ? min(pr.k, kb + pr.kchunk) : pr.k;
-  v4d acc[2][2];
-#pragma unroll
-  for (int a = 0; a < 2; ++a)
-#pragma unroll
-    for (int b = 0; b < 2; ++b) acc[a][b] = v4d{0.0, 0.0, 0.0, 0.0};
-  tile_mma(acc, As, Bs, sl.p[pr.sa1] + pr.a1, pr.lda, pr.ta1, sl.p[pr.sb1] + pr.b1, pr.ldb, pr.tb1, pr.m, pr.n,
-           kb, ke, m0, n0, 1.0);
-  if (pr.nprod == 2)
-    tile_mma(acc, As, Bs, sl.p[pr.sa2] + pr.a2, pr.lda, pr.ta2, sl.p[pr.sb2] + pr.b2, pr.ldb, pr.tb2, pr.m, pr.n,
-             kb, ke, m0, n0, pr.s2 * s2scale);
-  double* C = const_cast<double*>(sl.p[pr.sc]) + pr.c + (i64)t.ks * pr.cstride;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int wm = wave >> 1, wn = wave & 1;
-#pragma unroll
-  for (int mi = 0; mi < 2; ++mi)
-#pragma unroll
-    for (int ni = 0; ni < 2; ++ni)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int row = m0 + c_row(wm, mi, lane, r);
-        const int col = n0 + c_col(wn, ni, lane, r);
-        if (row < pr.m && col < pr.n) {
-          const i64 o = row + (i64)col * pr.ldc;
-          const double v = pr.alpha * ascale * acc[mi][ni][r];
-          C[o] = (pr.beta == 0.0) ? v : v + pr.beta * C[o];
-        }
-      }
-}
-
-void launch_gemm(const Slots& sl, const GemmProb* probs, const GemmTask* tasks, int nt, hipStream_t s,
-                 double ascale, double s2scale) {
-  if (nt <= 0) return;
-  hipLaunchKernelGGL(k_gemm, dim3(nt), dim3(256), 0, s, sl, probs, tasks, ascale, s2scale);
-}
-
-// ======================================== 222_pp_hh pppp block (tile pairs)
-// commutators.f90:611-668: W1 = L1 R1, W2 = Rh L3 R3^T and
-// RES1 += W1 - W2 - (W1 - W2)^T pm.  The workgroup computes the raw
-// product of tile (I,J) into LDS and of tile (J,I) into registers, so both
-// RES tiles are finished in one pass.
+// ======================================== 222_pp_hh pppp block epilogue
+// commutators.f90:611-668.  The grouped GEMM leaves C = W1 - W2 =
+// L1 R1 - Rh L3 R3^T in w1's pppp block; RES1 += C - pm C^T.  Tile pair
+// (I,J), I <= J, of 64 x 64: both C tiles are staged through LDS so every
+// global access is a contiguous column run.  With ref_outputs the
+// reference's W1 and W2 are left in w1 / w2 (W2 formed here, rank nh).
 __device__ __forceinline__ double w2pp(const double* L3, const double* R3, int np, int nh, int Rh, int x, int y) {
   double s = 0.0;
   for (int h = 0; h < nh; ++h) s += L3[x + (i64)h * np] * R3[y + (i64)h * np];
   return s * Rh;
 }
 
-__global__ __launch_bounds__(256) void k_pppp_pairs(DevBasis B, const PairTask* tasks, const double* L, int Lh,
-                                                    const double* R, int Rh, double* RES, double* w1, double* w2,
-                                                    int write_w1) {
-  __shared__ double As[BK][BM + 1];
-  __shared__ double Bs[BK][BN + 1];
-  __shared__ double S1[BM][BN + 1];
+__global__ __launch_bounds__(256) void k_pppp_sym(DevBasis B, const PairTask* tasks, const double* L, int Lh,
+                                                  const double* R, int Rh, double* RES, double* w1, double* w2,
+                                                  int ref_outputs) {
+  __shared__ double T1[64][65];   // C(I0 + x, J0 + y)
+  __shared__ double T2[64][65];   // C(J0 + x, I0 + y)
   const PairTask t = tasks[blockIdx.x];
   const int q = t.q;
   const int np = B.dims[q * 3], nh = B.dims[q * 3 + 2];
   const i64 o1 = B.goff[q * 6 + 0], o3 = B.goff[q * 6 + 2];
-  const double *L1 = L + o1, *R1 = R + o1, *L3 = L + o3, *R3 = R + o3;
-  const int pm = Lh * Rh;
-  const int I0 = t.ti * BM, J0 = t.tj * BN;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int wm = wave >> 1, wn = wave & 1;
-  v4d acc[2][2];
-  // raw product of tile (I, J) -> S1[x][y]
-#pragma unroll
-  for (int a = 0; a < 2; ++a)
-#pragma unroll
-    for (int b = 0; b < 2; ++b) acc[a][b] = v4d{0.0, 0.0, 0.0, 0.0};
-  tile_mma(acc, As, Bs, L1, np, 0, R1, np, 0, np, np, 0, np, I0, J0, 1.0);
-#pragma unroll
-  for (int mi = 0; mi < 2; ++mi)
-#pragma unroll
-    for (int ni = 0; ni < 2; ++ni)
-#pragma unroll
-      for (int r = 0; r < 4; ++r)
-        S1[c_row(wm, mi, lane, r)][c_col(wn, ni, lane, r)] = acc[mi][ni][r];
-  __syncthreads();
-  const bool diag = (t.ti == t.tj);
-  if (!diag) {
-    // raw product of tile (J, I) in registers
-#pragma unroll
-    for (int a = 0; a < 2; ++a)
-#pragma unroll
-      for (int b = 0; b < 2; ++b) acc[a][b] = v4d{0.0, 0.0, 0.0, 0.0};
-    tile_mma(acc, As, Bs, L1, np, 0, R1, np, 0, np, np, 0, np, J0, I0, 1.0);
-  }
+  const double *L3 = L + o3, *R3 = R + o3;
+  double* Cm = w1 + o1;
   double* X1 = RES + o1;
-  double* W11 = w1 + o1;
   double* W21 = w2 + o1;
-#pragma unroll
-  for (int mi = 0; mi < 2; ++mi)
-#pragma unroll
-    for (int ni = 0; ni < 2; ++ni)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        // register element: row y of the (J,I) tile, column x (for a
-        // diagonal tile: row y, column x of the (I,I) tile)
-        const int y = c_row(wm, mi, lane, r);
-        const int x = c_col(wn, ni, lane, r);
-        const int gy = J0 + y, gx = I0 + x;
-        if (gy >= np || gx >= np) continue;
-        const double w1yx = diag ? S1[y][x] : acc[mi][ni][r];
-        const double w1xy = S1[x][y];
-        const double w2yx = w2pp(L3, R3, np, nh, Rh, gy, gx);
-        const double w2xy = w2pp(L3, R3, np, nh, Rh, gx, gy);
-        const i64 oyx = gy + (i64)gx * np;
-        X1[oyx] = X1[oyx] + w1yx - w2yx - (w1xy - w2xy) * pm;
-        if (write_w1) W11[oyx] = w1yx;
-        W21[oyx] = w2yx;
-        if (!diag) {
-          const i64 oxy = gx + (i64)gy * np;
-          X1[oxy] = X1[oxy] + w1xy - w2xy - (w1yx - w2yx) * pm;
-          if (write_w1) W11[oxy] = w1xy;
-          W21[oxy] = w2xy;
+  const int pm = Lh * Rh;
+  const int I0 = t.ti * 64, J0 = t.tj * 64;
+  const bool diag = (t.ti == t.tj);
+  for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+    const int x = e & 63, y = e >> 6;
+    T1[x][y] = (I0 + x < np && J0 + y < np) ? Cm[(I0 + x) + (i64)(J0 + y) * np] : 0.0;
+    if (!diag) T2[x][y] = (J0 + x < np && I0 + y < np) ? Cm[(J0 + x) + (i64)(I0 + y) * np] : 0.0;
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+    const int x = e & 63, y = e >> 6;
+    const int gx = I0 + x, gy = J0 + y;
+    if (gx < np && gy < np) {
+      const double cxy = T1[x][y];
+      const double cyx = diag ? T1[y][x] : T2[y][x];
+      const i64 o = gx + (i64)gy * np;
+      X1[o] = X1[o] + cxy - cyx * pm;
+      if (ref_outputs) {
+        const double w2v = w2pp(L3, R3, np, nh, Rh, gx, gy);
+        Cm[o] = cxy + w2v;
+        W21[o] = w2v;
+      }
+    }
+    if (!diag) {
+      const int hx = J0 + x, hy = I0 + y;   // element (J0 + x, I0 + y) of tile (J, I)
+      if (hx < np && hy < np) {
+        const double cxy = T2[x][y];
+        const double cyx = T1[y][x];
+        const i64 o = hx + (i64)hy * np;
+        X1[o] = X1[o] + cxy - cyx * pm;
+        if (ref_outputs) {
+          const double w2v = w2pp(L3, R3, np, nh, Rh, hx, hy);
+          Cm[o] = cxy + w2v;
+          W21[o] = w2v;
         }
       }
+    }
+  }
 }
 
-void launch_pppp_pairs(const DevBasis& B, const PairTask* t, int nt, const double* L, int Lh, const double* R,
-                       int Rh, double* RES, double* w1, double* w2, int write_w1, hipStream_t s) {
+void launch_pppp_sym(const DevBasis& B, const PairTask* t, int nt, const double* L, int Lh, const double* R, int Rh,
+                     double* RES, double* w1, double* w2, int ref_outputs, hipStream_t s) {
   if (nt <= 0) return;
-  hipLaunchKernelGGL(k_pppp_pairs, dim3(nt), dim3(256), 0, s, B, t, L, Lh, R, Rh, RES, w1, w2, write_w1);
+  hipLaunchKernelGGL(k_pppp_sym, dim3(nt), dim3(256), 0, s, B, t, L, Lh, R, Rh, RES, w1, w2, ref_outputs);
 }
 
 // ================================ 222_pp_hh epilogue of the small blocks

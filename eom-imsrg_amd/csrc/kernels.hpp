@@ -47,7 +47,7 @@ struct GemmProb {
   unsigned char sc, pad0, pad1, pad2;
   double alpha, s2, beta;
   int kchunk;        // > 0: split K in chunks of kchunk, chunk ks written to C + ks*cstride
-  int pad3;
+  int k2;            // > 0: K of the second product when it differs (no split K then)
   i64 cstride;
 };
 struct GemmTask { int prob; int tm; int tn; int ks; };
@@ -55,14 +55,29 @@ struct GemmTask { int prob; int tm; int tn; int ks; };
 // Tile pair (I,J), I <= J, of the pppp block of channel q
 struct PairTask { int q; int ti; int tj; int pad; };
 
-// Task list on the device with per-channel ranges (the plain path launches
-// one range at a time).
+// Workgroups are dispatched to the 8 XCDs round-robin (blockIdx mod 8), each
+// XCD with its own L2.  An XcdMap gives every XCD its own contiguous task
+// range so the tiles of one channel share an L2: block b runs task
+// start[b % 8] + b / 8 (if below count).  count[0] < 0: identity mapping.
+constexpr int kXcds = 8;
+struct XcdMap { int start[kXcds]; int count[kXcds]; };
+
+__device__ __forceinline__ int xcd_task(const XcdMap& m, int block) {
+  if (m.count[0] < 0) return block;
+  const int x = block & (kXcds - 1), loc = block / kXcds;
+  return loc < m.count[x] ? m.start[x] + loc : -1;
+}
+
+// Task list on the device with per-group ranges (the plain path launches
+// one range at a time; the grouped path launches everything at once).
 template <class T>
 struct TaskList {
   T* dev = nullptr;
   int n = 0;
   std::vector<T> host;
-  std::vector<int> chan_begin;   // [groups+1]
+  std::vector<int> chan_begin;   // [groups+1] (groups in list order)
+  XcdMap xmap{{0}, {-1}};        // grouped launch mapping
+  int grid = 0;                  // grouped launch size
 };
 
 // launchers (all asynchronous on `s`)
@@ -81,17 +96,23 @@ void launch_comm111(const DevBasis& B, const double* L, int Lh, const double* R,
                     hipStream_t s);
 void launch_comm121(const DevBasis& B, const OneTask* t, int nt, const double* L, int Lh, const double* R,
                     int Rh, double* RES, int RESh, hipStream_t s);
-void launch_comm221(const DevBasis& B, const OneTask* t, int nt, int pm, const double* w1, const double* w2,
-                    double* RES, int RESh, hipStream_t s);
+void launch_comm221(const DevBasis& B, const OneTask* t, int nt, const double* L, int Lh, const double* R, int Rh,
+                    const double* w1, const double* w2, double* RES, int RESh, hipStream_t s);
 void launch_e0_partials(const DevBasis& B, const double* L, int Lh, const double* R, int Rh, double* part,
                         hipStream_t s);
 void launch_e0_final(const double* part, int nblocks, int use220, int use110, double* dst, int accumulate,
                      hipStream_t s);
-// alpha and s2 of every problem are multiplied by ascale / s2scale
-void launch_gemm(const Slots& sl, const GemmProb* probs, const GemmTask* tasks, int nt, hipStream_t s,
-                 double ascale = 1.0, double s2scale = 1.0);
-void launch_pppp_pairs(const DevBasis& B, const PairTask* t, int nt, const double* L, int Lh, const double* R,
-                       int Rh, double* RES, double* w1, double* w2, int write_w1, hipStream_t s);
+// grouped MFMA GEMM (gemm.hip); tile 0: 64x64, 1: 128x64, 2: 128x128.
+// alpha and s2 of every problem are multiplied by ascale / s2scale.
+int tile_rows(int tile);
+int tile_cols(int tile);
+void launch_gemm_tiled(int tile, const Slots& sl, const GemmProb* probs, const GemmTask* tasks, int grid,
+                       const XcdMap& xm, hipStream_t s, double ascale = 1.0, double s2scale = 1.0);
+// RES1 += C - pm C^T with C = L1 R1 - Rh L3 R3^T held in w1's pppp blocks;
+// ref_outputs: also leave the reference's W1 = L1 R1 and W2 = Rh L3 R3^T
+// in w1 / w2 (otherwise w1's pppp blocks keep C and w2's are untouched)
+void launch_pppp_sym(const DevBasis& B, const PairTask* t, int nt, const double* L, int Lh, const double* R, int Rh,
+                     double* RES, double* w1, double* w2, int ref_outputs, hipStream_t s);
 void launch_pphh_epilogue(const DevBasis& B, const SegTask* t, int nt, const double* L, int Lh, const double* R,
                           int Rh, double* RES, double* w1, double* w2, const double* Pscr, const i64* poff,
                           const int* k3ks, const double* K2scr, const i64* k2off, const int* k2ks, hipStream_t s);

@@ -77,10 +77,14 @@ SIGNATURES = {
     "imsrg_magnus_step": (_i, [_p, _dp, _dp, _dp, _dp, C.c_double, C.c_double, C.c_double, _d, _p]),
     "imsrg_magnus_decouple": (_i, [_p, _dp, _dp, C.c_double, C.c_double, C.c_double, _i, C.c_double, _i, _d,
                                    _ip, _p]),
+    "imsrg_set_gemm_tile": (_i, [_p, _i, _i]),
+    "imsrg_get_gemm_tile": (_i, [_p, _i]),
     "imsrg_profile_kernel": (_i, [_p, _i, _dp, _i, _dp, _i, _dp, _i, _dp, _dp, _dp, _dp, _dp, _i, _d, _p]),
 }
 
-KERNEL = dict(PPPP=0, PPHH_SMALL=1, PH_GEMM=2, PH_SCATTER=3, K122=4, PANDYA=5, ONEBODY=6, COMMUTATOR=7)
+GEMM = dict(PPPP=0, PPX=1, XX=2, PH=3)
+KERNEL = dict(PPPP=0, PPHH_SMALL=1, PH_GEMM=2, PH_SCATTER=3, K122=4, PANDYA=5, ONEBODY=6, COMMUTATOR=7,
+              PPPP_GEMM=8, COPY=9)
 
 for _name, (_res, _args) in SIGNATURES.items():
     _f = getattr(lib, _name)

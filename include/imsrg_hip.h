@@ -160,13 +160,25 @@ int imsrg_magnus_decouple(imsrg_flow f, double* HS, double* G, double ds, double
                           int crit_mode, double crit_tol, int maxsteps, double* log, int* steps,
                           imsrg_stream stream);
 
+/* ------------------------------------------------ tuning */
+/* No counterpart in the reference: selects the MFMA tile shape of one GEMM
+ * family (IMSRG_GEMM_*): tile 0 = 64x64, 1 = 128x64, 2 = 128x128, plus
+ * IMSRG_TILE_XCD to pack each channel's tiles onto one XCD (one L2). */
+#define IMSRG_TILE_XCD 8
+#define IMSRG_GEMM_PPPP 0   /* 222_pp_hh L1 R1 */
+#define IMSRG_GEMM_PPX 1    /* 222_pp_hh pp x (ph|hh) */
+#define IMSRG_GEMM_XX 2     /* 222_pp_hh (ph|hh)^T (ph|hh) */
+#define IMSRG_GEMM_PH 3     /* 222_ph W' = Lh LCC RCC^T - Rh RCC LCC^T */
+int imsrg_set_gemm_tile(imsrg_basis b, int family, int tile);
+int imsrg_get_gemm_tile(imsrg_basis b, int family);
+
 /* ------------------------------------------------ instrumentation */
 /* No counterpart in the reference: times `iters` launches of one kernel
  * class of [L, R] between hipEvents recorded on `stream` and returns the
  * mean milliseconds per launch.  Inputs as for the term routines; every
  * kernel class runs once untimed first (fills w1/w2, LCC/RCC, WCC).
  * which: IMSRG_K_* below. */
-#define IMSRG_K_PPPP 0      /* 222_pp_hh pppp tile pairs (MFMA) */
+#define IMSRG_K_PPPP 0      /* 222_pp_hh pppp block: L1 R1 MFMA GEMM + symmetric epilogue */
 #define IMSRG_K_PPHH_SMALL 1 /* 222_pp_hh pp x (ph|hh) and (ph|hh)^2 GEMMs + epilogue */
 #define IMSRG_K_PH_GEMM 2   /* 222_ph WCC = LCC RCC^T (MFMA) */
 #define IMSRG_K_PH_SCATTER 3 /* 222_ph inverse Pandya */
@@ -174,6 +186,8 @@ int imsrg_magnus_decouple(imsrg_flow f, double* HS, double* G, double ds, double
 #define IMSRG_K_PANDYA 5
 #define IMSRG_K_ONEBODY 6   /* 111 + 121 + 221 */
 #define IMSRG_K_COMMUTATOR 7 /* the whole [L, R] */
+#define IMSRG_K_PPPP_GEMM 8 /* the pppp L1 R1 MFMA GEMM alone */
+#define IMSRG_K_COPY 9      /* plain copy R -> RES of one operator (HBM calibration) */
 int imsrg_profile_kernel(imsrg_basis b, int which, const double* L, int Lherm, const double* R, int Rherm,
                          double* RES, int RESherm, double* w1, double* w2, double* LCC, double* RCC, double* WCC,
                          int iters, double* ms, imsrg_stream stream);
