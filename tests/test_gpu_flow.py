@@ -89,3 +89,28 @@ def test_magnus_decouple_emax4(gpu, graphs):
     assert np.max(np.abs(log[:, 1] - logr[:, 1])) < 1e-8
     assert_rel(HS.numpy(), HSr, rtol=1e-9, what="decoupled H")
     assert_rel(G.numpy(), Gr, rtol=1e-9, what="Omega")
+
+
+def test_flow_matches_committed_fixture(gpu):
+    """the eMax-4 flow from the product's own normal ordering against the
+    committed fixture (tests/golden/oracle_emax4.json): same step count and
+    commutators per step, E0 per step within 1e-8 MeV"""
+    import json
+    import os
+    from eom_imsrg import Flow, SqOp, commutators as cm, synth
+    g = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "oracle_emax4.json")))
+    b = basis(4)
+    print("[fixture flow] emax=4", flush=True)
+    H = SqOp.from_numpy(b, synth.hamiltonian_vacuum(b, seed=0, hw=20.0), 1)
+    cm.normal_order(H)
+    torch.cuda.synchronize()
+    assert abs(H.E0 - g["E0_bare_normal_ordered"]) < 1e-9
+    assert abs(cm.mbpt2(H) - g["mbpt2_bare"]) < 1e-9
+    f = Flow(b, graphs=True)
+    G = SqOp.zeros(b, -1)
+    log = f.magnus_decouple(H, G, ds=0.5)
+    torch.cuda.synchronize()
+    assert len(log) == g["flow"]["steps"]
+    assert [int(x) for x in log[:, 3]] == g["flow"]["commutators"]
+    assert np.abs(log[:, 1] - np.array(g["flow"]["E0"])).max() < 1e-8
+    assert abs(H.E0 - g["flow"]["final_E0"]) < 1e-8
