@@ -132,7 +132,8 @@ def main() -> int:
     bound, work, unit, peak, _ = cand[dom]
     achieved = work / (kms[dom] / 1e3)
     roof = {"kernel": dom, "bound": bound, "achieved": round(achieved, 3), "peak": peak, "unit": unit,
-            "frac": round(achieved / peak, 4), "traffic": None, "launch_ms": round(kms[dom], 4),
+            "frac": round(achieved / peak, 4), "traffic": measured_traffic(dom, a.emax),
+            "launch_ms": round(kms[dom], 4),
             "work_per_launch": work * (1e12 if unit == "TFLOP/s" else 1e9),
             "work_unit": "flop" if unit == "TFLOP/s" else "byte"}
 
@@ -161,6 +162,20 @@ def main() -> int:
     }
     print(json.dumps(res))
     return 0
+
+
+def measured_traffic(kernel: str, emax: int):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summary (FETCH_SIZE and WRITE_SIZE passes scaled by a copy kernel of known
+    size), or None when no summary covers this kernel and basis"""
+    p = os.path.join(ROOT, "profiles", f"hbm_traffic_emax{emax}.md")
+    if not os.path.exists(p):
+        return None
+    for line in open(p):
+        f = line.split()
+        if len(f) == 3 and f[0] == "traffic_bytes_per_launch" and f[1] == kernel:
+            return float(f[2])
+    return None
 
 
 def cpu_baseline(b, emax, H, Omega, comm_per_step):

@@ -1202,7 +1202,9 @@ int imsrg_profile_kernel(imsrg_basis B, int which, const double* L, int Lh, cons
         break;
       case IMSRG_K_122: t_122(B, L, Lh, R, Rh, RES, RESh, s); break;
       case IMSRG_K_PANDYA: t_pandya(B, R, Rh, RCC, s); break;
-      case IMSRG_K_COPY: launch_copy(RES, R, B->b.oplen, s); break;
+      case IMSRG_K_COPY:   // first half of WCC into its second half: wclen/2 doubles each way
+        launch_copy(WCC + B->b.wclen / 2, WCC, B->b.wclen / 2, s);
+        break;
       case IMSRG_K_ONEBODY:
         t_111(B, L, Lh, R, Rh, RES, s);
         t_121(B, L, Lh, R, Rh, RES, RESh, s);

@@ -187,7 +187,7 @@ int imsrg_get_gemm_tile(imsrg_basis b, int family);
 #define IMSRG_K_ONEBODY 6   /* 111 + 121 + 221 */
 #define IMSRG_K_COMMUTATOR 7 /* the whole [L, R] */
 #define IMSRG_K_PPPP_GEMM 8 /* the pppp L1 R1 MFMA GEMM alone */
-#define IMSRG_K_COPY 9      /* plain copy R -> RES of one operator (HBM calibration) */
+#define IMSRG_K_COPY 9      /* plain copy of the first half of WCC into its second half (HBM calibration) */
 int imsrg_profile_kernel(imsrg_basis b, int which, const double* L, int Lherm, const double* R, int Rherm,
                          double* RES, int RESherm, double* w1, double* w2, double* LCC, double* RCC, double* WCC,
                          int iters, double* ms, imsrg_stream stream);
