@@ -907,6 +907,56 @@ double imsrg_basis_sixj(imsrg_basis B, int j1, int j2, int j3, int j4, int j5, i
   return sixj_value(j1, j2, j3, j4, j5, j6);
 }
 
+int imsrg_alloc(int64_t count, double** out) {
+  if (!out || count < 0) return fail(IMSRG_EINVAL, "bad allocation request");
+  void* p = nullptr;
+  HIPCHK(hipMalloc(&p, std::max<int64_t>(count, 1) * sizeof(double)));
+  HIPCHK(hipMemset(p, 0, std::max<int64_t>(count, 1) * sizeof(double)));
+  *out = (double*)p;
+  return 0;
+}
+
+int imsrg_free(double* p) {
+  if (p) HIPCHK(hipFree(p));
+  return 0;
+}
+
+namespace {
+// offset and length (doubles) of one sq_op component in the flat layout
+int component(const Basis& b, int q, int g, i64* off, i64* len) {
+  const i64 A = b.A, P = b.P;
+  if (q == -1) {
+    switch (g) {
+      case 0: *off = 0; *len = 1; return 0;
+      case 1: *off = 1; *len = A * A; return 0;
+      case 2: *off = 1 + A * A; *len = P * A; return 0;
+      case 3: *off = 1 + A * A + P * A; *len = P * P; return 0;
+      default: return fail(IMSRG_EINVAL, "one-body component g must be 0..3");
+    }
+  }
+  if (q < 0 || q >= b.nblocks || g < 0 || g > 5) return fail(IMSRG_EINVAL, "bad channel / block");
+  *off = b.goff[q * 6 + g];
+  *len = (i64)b.rows(q, g) * b.cols(q, g);
+  return 0;
+}
+}  // namespace
+
+int imsrg_op_put(imsrg_basis B, double* op, int q, int g, const double* host) {
+  if (!B || !op || !host) return fail(IMSRG_EINVAL, "null argument");
+  i64 off, len;
+  if (int rc = component(B->b, q, g, &off, &len)) return rc;
+  if (len > 0) HIPCHK(hipMemcpy(op + off, host, len * sizeof(double), hipMemcpyHostToDevice));
+  return 0;
+}
+
+int imsrg_op_get(imsrg_basis B, const double* op, int q, int g, double* host) {
+  if (!B || !op || !host) return fail(IMSRG_EINVAL, "null argument");
+  i64 off, len;
+  if (int rc = component(B->b, q, g, &off, &len)) return rc;
+  if (len > 0) HIPCHK(hipMemcpy(host, op + off, len * sizeof(double), hipMemcpyDeviceToHost));
+  return 0;
+}
+
 int imsrg_set_path(imsrg_basis B, int path) {
   if (!B || (path != IMSRG_PATH_GROUPED && path != IMSRG_PATH_PLAIN)) return fail(IMSRG_EINVAL, "bad path");
   B->path = path;

@@ -53,6 +53,10 @@ SIGNATURES = {
     "imsrg_basis_cc": (_i, [_p, _ip, _ip, _lp, _lp, _ip]),
     "imsrg_basis_sixj": (C.c_double, [_p, _i, _i, _i, _i, _i, _i]),
     "imsrg_set_path": (_i, [_p, _i]),
+    "imsrg_alloc": (_i, [C.c_int64, C.POINTER(_dp)]),
+    "imsrg_free": (_i, [_dp]),
+    "imsrg_op_put": (_i, [_p, _dp, _i, _i, _d]),
+    "imsrg_op_get": (_i, [_p, _dp, _i, _i, _d]),
     "imsrg_commutator_110": (_i, [_p, _dp, _i, _dp, _i, _d, _p]),
     "imsrg_commutator_220": (_i, [_p, _dp, _i, _dp, _i, _d, _p]),
     "imsrg_commutator_e0": (_i, [_p, _dp, _i, _dp, _i, _dp, _p]),

@@ -78,6 +78,22 @@ int imsrg_basis_cc(imsrg_basis b, int* rlen, int* ncol, int64_t* ccoff, int64_t*
 double imsrg_basis_sixj(imsrg_basis b, int j1, int j2, int j3, int j4, int j5, int j6);
 int imsrg_set_path(imsrg_basis b, int path);
 
+/* ------------------------------------------------- operator storage */
+/* allocate_blocks / duplicate_sq_op / allocate_CCMAT (basic_IMSRG.f90:521-685,
+ * 2797-2850; cross_coupled.f90:61-177): a zeroed device buffer of `count`
+ * doubles (an operator: IMSRG_SIZE_OP, a cc_mat: IMSRG_SIZE_CC, the WCC
+ * workspace: IMSRG_SIZE_WCC).  Synchronous. */
+int imsrg_alloc(int64_t count, double** out);
+int imsrg_free(double* p);
+/* copy one component of a host sq_op to / from a device operator, so a
+ * Fortran caller can mirror op%E0, op%fhh, op%fph, op%fpp and
+ * op%mat(q)%gam(g)%X (basic_IMSRG.f90:50-76) without repacking:
+ *   q = -1: g = 0 E0 (one double), 1 fhh, 2 fph, 3 fpp;
+ *   q >= 0: gam(g+1) of channel q (0-based: Fortran mat(q+1)%gam(g+1)).
+ * Column-major, exactly the Fortran array.  Synchronous. */
+int imsrg_op_put(imsrg_basis b, double* op, int q, int g, const double* host);
+int imsrg_op_get(imsrg_basis b, const double* op, int q, int g, double* host);
+
 /* ------------------------------------------------ commutator terms */
 /* commutator_110, commutators.f90:10-33 (returns the value; synchronises) */
 int imsrg_commutator_110(imsrg_basis b, const double* L, int Lherm, const double* R, int Rherm, double* value,
