@@ -999,6 +999,8 @@ int magnus(Op& DG, const Op& G, double conv, Work& w) {
   return ncomm;
 }
 
+#include "imsrg_adjoint.inc"
+
 }  // namespace
 
 // =============================================================== C interface
@@ -1121,6 +1123,35 @@ void orc_white(void* h, const double* H, double* ETA) {
 double orc_mbpt2(void* h, const double* H) { OPS(h) return mbpt2(Op{&b, (double*)H, 1}); }
 void orc_normal_order(void* h, double* H) { OPS(h) Op o{&b, H, 1}; normal_order(o); }
 double orc_monopole(void* h, const double* H, int a, int c) { OPS(h) return monopole(a, c, Op{&b, (double*)H, 1}); }
+
+// ---- reverse-mode derivatives (imsrg_adjoint.inc); gradients accumulate
+void orc_commutator_vjp(void* h, const double* L, int Lh, const double* R, int Rh, const double* RESb, int RESh,
+                        double* Lb, double* Rb) {
+  OPS(h) Op lb{&b, Lb, Lh}, rb{&b, Rb, Rh};
+  adj_commutator(Op{&b, (double*)L, Lh}, Op{&b, (double*)R, Rh}, RESb, RESh, lb, rb);
+}
+int orc_bch_vjp(void* h, const double* G, const double* H, double conv, int maxit, const double* HSb, double* Gb,
+                double* Hb) {
+  OPS(h) Op gb{&b, Gb, -1}, hb{&b, Hb, 1};
+  return adj_bch(Op{&b, (double*)G, -1}, Op{&b, (double*)H, 1}, conv, maxit, HSb, gb, hb);
+}
+int orc_magnus_vjp(void* h, const double* ETA, const double* G, double conv, const double* DGb, double* ETAb,
+                   double* Gb) {
+  OPS(h) Op eb{&b, ETAb, -1}, gb{&b, Gb, -1};
+  return adj_magnus(Op{&b, (double*)ETA, -1}, Op{&b, (double*)G, -1}, conv, DGb, eb, gb);
+}
+void orc_white_vjp(void* h, const double* H, const double* ETAb, double* Hb) {
+  OPS(h) Op hb{&b, Hb, 1};
+  adj_white(Op{&b, (double*)H, 1}, Op{&b, (double*)ETAb, -1}, hb);
+}
+void orc_normal_order_vjp(void* h, const double* Hb, double* Tb) {
+  OPS(h) Op tb{&b, Tb, 1};
+  adj_normal_order(Op{&b, (double*)Hb, 1}, tb);
+}
+int orc_flow_grad(void* h, const double* T, double ds, double bch_conv, double mag_conv, double crit_tol,
+                  int maxsteps, int ckpt, double* Tbar, double* E0) {
+  OPS(h) return flow_grad(b, T, ds, bch_conv, mag_conv, crit_tol, maxsteps, ckpt, Tbar, E0);
+}
 
 // One full scalar commutator RES = [L, R] in the BCH term order
 // (E0 = 110 + 220, then 111, 121, 122, 222_pp_hh, 221, 222_ph).

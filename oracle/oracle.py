@@ -47,6 +47,12 @@ def lib():
             "orc_commutator": (None, [vp, dp, i, dp, i, dp, i]),
             "orc_bch": (i, [vp, dp, dp, dp, d, i, dp]), "orc_magnus": (i, [vp, dp, dp, d]),
             "orc_magnus_decouple": (i, [vp, dp, dp, d, d, d, i, d, i, dp]),
+            "orc_commutator_vjp": (None, [vp, dp, i, dp, i, dp, i, dp, dp]),
+            "orc_bch_vjp": (i, [vp, dp, dp, d, i, dp, dp, dp]),
+            "orc_magnus_vjp": (i, [vp, dp, dp, d, dp, dp, dp]),
+            "orc_white_vjp": (None, [vp, dp, dp, dp]),
+            "orc_normal_order_vjp": (None, [vp, dp, dp]),
+            "orc_flow_grad": (i, [vp, dp, d, d, d, d, i, i, dp, dp]),
             "orc_bf_0body": (d, [vp, dp, i, dp, i]), "orc_bf_1body": (d, [vp, dp, i, dp, i, i, i]),
             "orc_bf_2body": (d, [vp, dp, i, dp, i, i, i, i, i, i, dp]),
         }
@@ -193,6 +199,40 @@ class OracleBasis:
         st = lib().orc_magnus_decouple(self.h, _dp(HS), _dp(G), ds, bch_conv, mag_conv, crit_mode, crit_tol,
                                        maxsteps, _dp(log))
         return HS, G, log[: 4 * max(st, 0)].reshape(-1, 4), st
+
+    # ---- reverse-mode derivatives (flat-array gradients)
+    def commutator_vjp(self, L, Lh, R, Rh, RESb, RESh):
+        Lb, Rb = np.zeros(self.oplen), np.zeros(self.oplen)
+        lib().orc_commutator_vjp(self.h, _dp(L), Lh, _dp(R), Rh, _dp(np.ascontiguousarray(RESb)), RESh, _dp(Lb),
+                                 _dp(Rb))
+        return Lb, Rb
+
+    def bch_vjp(self, G, H, conv, HSb, maxit=40):
+        Gb, Hb = np.zeros(self.oplen), np.zeros(self.oplen)
+        it = lib().orc_bch_vjp(self.h, _dp(G), _dp(H), conv, maxit, _dp(np.ascontiguousarray(HSb)), _dp(Gb), _dp(Hb))
+        return Gb, Hb, it
+
+    def magnus_vjp(self, ETA, G, conv, DGb):
+        Eb, Gb = np.zeros(self.oplen), np.zeros(self.oplen)
+        k = lib().orc_magnus_vjp(self.h, _dp(ETA), _dp(G), conv, _dp(np.ascontiguousarray(DGb)), _dp(Eb), _dp(Gb))
+        return Eb, Gb, k
+
+    def white_vjp(self, H, ETAb):
+        Hb = np.zeros(self.oplen)
+        lib().orc_white_vjp(self.h, _dp(H), _dp(np.ascontiguousarray(ETAb)), _dp(Hb))
+        return Hb
+
+    def normal_order_vjp(self, Hb):
+        Tb = np.zeros(self.oplen)
+        lib().orc_normal_order_vjp(self.h, _dp(np.ascontiguousarray(Hb)), _dp(Tb))
+        return Tb
+
+    def flow_grad(self, T, ds, bch_conv, mag_conv, crit_tol, maxsteps=200, ckpt=4):
+        """(E0, dE0/dT, steps) for magnus_decouple started from normal_order(T)"""
+        Tb = np.zeros(self.oplen)
+        e0 = np.zeros(1)
+        st = lib().orc_flow_grad(self.h, _dp(T), ds, bch_conv, mag_conv, crit_tol, maxsteps, ckpt, _dp(Tb), _dp(e0))
+        return float(e0[0]), Tb, st
 
     def bf_0body(self, A, Ah, B, Bh):
         return lib().orc_bf_0body(self.h, _dp(A), Ah, _dp(B), Bh)
