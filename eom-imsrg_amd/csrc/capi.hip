@@ -87,6 +87,17 @@ struct imsrg_basis_s {
   double* M = nullptr;       // monopole table N x N
   double* dval = nullptr;    // device scalars
   double* hval = nullptr;    // pinned host scalars
+  // reverse mode, built on first use: 222_pp_hh transposed products in
+  // rounds of distinct outputs, the W' transpose products, upper-triangle
+  // tiles of WCC, quadruple starts, the compact one-body map of 122
+  bool adj_ready = false;
+  GemmFamily pb[6], phb;
+  TaskList<CCTask> ccu;
+  std::vector<int> qstart_h, f1map_h, f1off_h;
+  int *qstart = nullptr, *f1map = nullptr, *f1off = nullptr;
+  int nquads = 0, nf1 = 0;
+  double *Wb1 = nullptr, *Wb2 = nullptr, *cbar = nullptr, *Y = nullptr, *U = nullptr;
+  double *aLCC = nullptr, *aRCC = nullptr, *aLCCb = nullptr, *aRCCb = nullptr, *Mb = nullptr;
 
   ~imsrg_basis_s() {
     for (void* p : allocs) (void)hipFree(p);
@@ -646,6 +657,184 @@ void t_full(imsrg_basis_s* B, const double* L, int Lh, const double* R, int Rh, 
   t_ph(B, LCC, Lh, RCC, Rh, RES, RESh, WCC, s);
 }
 
+
+// ============================================================ reverse mode
+// One transposed 222_pp_hh product  C(out) += alpha op(A) op(B), slots
+// 0 L, 1 R, 2 Lb, 3 Rb, 4 Wb1, 5 Wb2 (cotangents of the w1 / w2 blocks),
+// dims 'p' np, 'b' nb, 'h' nh; mode bit0 * Lh, bit1 * Rh.
+struct BwdProd {
+  int out_slot, out_g, a_slot, a_g, ta, b_slot, b_g, tb;
+  char m, n, k;
+  double alpha;
+  int mode;
+};
+// transposes of W11 = L1 R1, W25 = L5 R5, W15 = Lh L3^T R3, W21 = Rh L3 R3^T,
+// W13 = L1 R3 - R1 L3, W23 = L3 R5 - R3 L5, W24 = Rh L6 R6^T, W26 = L6 R5 -
+// R6 L5, W14 = Lh L2^T R2, W12 = L1 R2 - R1 L2, W16 = Lh L2^T R3 - Rh R2^T L3,
+// W22 = Rh L3 R6^T - Lh R3 L6^T  (commutators.f90:611-755)
+const BwdProd kBwd[36] = {
+    {2, 0, 4, 0, 0, 1, 0, 1, 'p', 'p', 'p', 1, 0},  {3, 0, 0, 0, 1, 4, 0, 0, 'p', 'p', 'p', 1, 0},
+    {2, 4, 5, 4, 0, 1, 4, 1, 'h', 'h', 'h', 1, 0},  {3, 4, 0, 4, 1, 5, 4, 0, 'h', 'h', 'h', 1, 0},
+    {2, 2, 1, 2, 0, 4, 4, 1, 'p', 'h', 'h', 1, 1},  {3, 2, 0, 2, 0, 4, 4, 0, 'p', 'h', 'h', 1, 1},
+    {2, 2, 5, 0, 0, 1, 2, 0, 'p', 'h', 'p', 1, 2},  {3, 2, 5, 0, 1, 0, 2, 0, 'p', 'h', 'p', 1, 2},
+    {2, 0, 4, 2, 0, 1, 2, 1, 'p', 'p', 'h', 1, 0},  {3, 2, 0, 0, 1, 4, 2, 0, 'p', 'h', 'p', 1, 0},
+    {3, 0, 4, 2, 0, 0, 2, 1, 'p', 'p', 'h', -1, 0}, {2, 2, 1, 0, 1, 4, 2, 0, 'p', 'h', 'p', -1, 0},
+    {2, 2, 5, 2, 0, 1, 4, 1, 'p', 'h', 'h', 1, 0},  {3, 4, 0, 2, 1, 5, 2, 0, 'h', 'h', 'p', 1, 0},
+    {3, 2, 5, 2, 0, 0, 4, 1, 'p', 'h', 'h', -1, 0}, {2, 4, 1, 2, 1, 5, 2, 0, 'h', 'h', 'p', -1, 0},
+    {2, 5, 5, 3, 0, 1, 5, 0, 'b', 'h', 'b', 1, 2},  {3, 5, 5, 3, 1, 0, 5, 0, 'b', 'h', 'b', 1, 2},
+    {2, 5, 5, 5, 0, 1, 4, 1, 'b', 'h', 'h', 1, 0},  {3, 4, 0, 5, 1, 5, 5, 0, 'h', 'h', 'b', 1, 0},
+    {3, 5, 5, 5, 0, 0, 4, 1, 'b', 'h', 'h', -1, 0}, {2, 4, 1, 5, 1, 5, 5, 0, 'h', 'h', 'b', -1, 0},
+    {2, 1, 1, 1, 0, 4, 3, 1, 'p', 'b', 'b', 1, 1},  {3, 1, 0, 1, 0, 4, 3, 0, 'p', 'b', 'b', 1, 1},
+    {2, 0, 4, 1, 0, 1, 1, 1, 'p', 'p', 'b', 1, 0},  {3, 1, 0, 0, 1, 4, 1, 0, 'p', 'b', 'p', 1, 0},
+    {3, 0, 4, 1, 0, 0, 1, 1, 'p', 'p', 'b', -1, 0}, {2, 1, 1, 0, 1, 4, 1, 0, 'p', 'b', 'p', -1, 0},
+    {2, 1, 1, 2, 0, 4, 5, 1, 'p', 'b', 'h', 1, 1},  {3, 2, 0, 1, 0, 4, 5, 0, 'p', 'h', 'b', 1, 1},
+    {3, 1, 0, 2, 0, 4, 5, 1, 'p', 'b', 'h', -1, 2}, {2, 2, 1, 1, 0, 4, 5, 0, 'p', 'h', 'b', -1, 2},
+    {2, 2, 5, 1, 0, 1, 5, 0, 'p', 'h', 'b', 1, 2},  {3, 5, 5, 1, 1, 0, 2, 0, 'b', 'h', 'p', 1, 2},
+    {3, 2, 5, 1, 0, 0, 5, 0, 'p', 'h', 'b', -1, 1}, {2, 5, 5, 1, 1, 1, 2, 0, 'b', 'h', 'p', -1, 1},
+};
+
+int ensure_adj(imsrg_basis_s* B) {
+  if (B->adj_ready) return 0;
+  int rc = ensure_dev(B);
+  if (rc) return rc;
+  const Basis& b = B->b;
+  const std::vector<int> order = channel_order(b);
+  static const int rcls[6] = {0, 0, 0, 1, 2, 1}, ccls[6] = {0, 1, 2, 1, 2, 2};
+  for (GemmFamily& f : B->pb) { f.probs_h.clear(); f.nks.clear(); f.group_end.clear(); f.tile = 0; }
+  for (int q : order) {
+    const int dd[3] = {b.dims[q * 3], b.dims[q * 3 + 1], b.dims[q * 3 + 2]};
+    auto dim = [&](char c) { return c == 'p' ? dd[0] : (c == 'b' ? dd[1] : dd[2]); };
+    int used[4][6] = {{0}};
+    for (const BwdProd& x : kBwd) {
+      const int m = dim(x.m), n = dim(x.n), k = dim(x.k);
+      if (m == 0 || n == 0 || k == 0) continue;
+      const int r = used[x.out_slot][x.out_g]++;
+      GemmProb p{};
+      p.a1 = b.goff[q * 6 + x.a_g]; p.b1 = b.goff[q * 6 + x.b_g]; p.c = b.goff[q * 6 + x.out_g];
+      p.m = m; p.n = n; p.k = k;
+      p.lda = std::max(dd[rcls[x.a_g]], 1); p.ldb = std::max(dd[rcls[x.b_g]], 1); p.ldc = std::max(dd[rcls[x.out_g]], 1);
+      p.nprod = 1; p.ta1 = (unsigned char)x.ta; p.tb1 = (unsigned char)x.tb;
+      p.sa1 = (unsigned char)x.a_slot; p.sb1 = (unsigned char)x.b_slot; p.sc = (unsigned char)x.out_slot;
+      p.alpha = x.alpha; p.beta = 1.0; p.mode = (unsigned char)(4 | x.mode);
+      B->pb[r].probs_h.push_back(p);
+      B->pb[r].nks.push_back(1);
+    }
+    (void)ccls;
+    for (GemmFamily& f : B->pb) f.group_end.push_back((int)f.probs_h.size());
+  }
+  // W' transpose: LCCb += M RCC, RCCb += M^T LCC per cross-coupled block;
+  // upper-triangle tiles of every WCC block
+  B->phb.probs_h.clear(); B->phb.nks.clear(); B->phb.group_end.clear(); B->phb.tile = 0;
+  B->ccu.host.clear(); B->ccu.chan_begin.assign(1, 0);
+  for (int q = 0; q < b.ncc; ++q) {
+    const int r = b.rlen[q], nb = b.ncol[q];
+    const i64 cnt = (i64)r * (r + 1) / 2;
+    for (i64 st = 0; st < cnt; st += kTile) B->ccu.host.push_back(CCTask{q, (int)st});
+    B->ccu.chan_begin.push_back((int)B->ccu.host.size());
+    if (r > 0 && nb > 0) {
+      for (int side = 0; side < 2; ++side) {
+        GemmProb p{};
+        p.a1 = b.wcoff[q]; p.b1 = b.ccoff[q]; p.c = b.ccoff[q];
+        p.m = r; p.n = nb; p.k = r;
+        p.lda = r; p.ldb = r; p.ldc = r;
+        p.nprod = 1; p.ta1 = (unsigned char)side; p.tb1 = 0;
+        p.sa1 = 4; p.sb1 = side ? 0 : 1; p.sc = side ? 3 : 2;
+        p.alpha = 1.0; p.beta = 1.0; p.mode = 4;
+        B->phb.probs_h.push_back(p);
+        B->phb.nks.push_back(1);
+      }
+    }
+    B->phb.group_end.push_back((int)B->phb.probs_h.size());
+  }
+  B->ccu.n = (int)B->ccu.host.size();
+  // quadruple starts of the 222_ph item list
+  B->qstart_h.clear();
+  for (size_t x = 0; x < B->quad_h.size(); ++x)
+    if (x == 0 || B->quad_h[x].ijkl != B->quad_h[x - 1].ijkl) B->qstart_h.push_back((int)x);
+  B->nquads = (int)B->qstart_h.size();
+  B->qstart_h.push_back((int)B->quad_h.size());
+  // compact index of the one-body elements the 122 partner sums read
+  B->f1map_h.assign((size_t)(b.off_fpp + (i64)b.P * b.P), -1);
+  B->f1off_h.clear();
+  for (const Partner& p : B->part_h)
+    if (B->f1map_h[p.foff] < 0) { B->f1map_h[p.foff] = (int)B->f1off_h.size(); B->f1off_h.push_back(p.foff); }
+  B->nf1 = (int)B->f1off_h.size();
+  if (B->nf1 > 4000) return fail(IMSRG_EINVAL, "one-body partner set exceeds the LDS of the 122 transpose");
+  for (GemmFamily* f : {&B->pb[0], &B->pb[1], &B->pb[2], &B->pb[3], &B->pb[4], &B->pb[5], &B->phb}) {
+    if ((rc = upload(B, f->probs_h, &f->probs))) return rc;
+    if ((rc = family_tasks(B, *f, 0))) return rc;
+  }
+  if ((rc = upload(B, B->ccu.host, &B->ccu.dev))) return rc;
+  if ((rc = upload(B, B->qstart_h, &B->qstart))) return rc;
+  if ((rc = upload(B, B->f1map_h, &B->f1map))) return rc;
+  if ((rc = upload(B, B->f1off_h, &B->f1off))) return rc;
+  const size_t op = (size_t)std::max<i64>(b.oplen, 1) * sizeof(double);
+  const size_t cc = (size_t)std::max<i64>(b.cclen, 1) * sizeof(double);
+  if ((rc = dalloc(B, op, (void**)&B->Wb1)) || (rc = dalloc(B, op, (void**)&B->Wb2)) ||
+      (rc = dalloc(B, op, (void**)&B->cbar)) || (rc = dalloc(B, op, (void**)&B->Y)) ||
+      (rc = dalloc(B, (size_t)std::max<i64>(b.wclen, 1) * sizeof(double), (void**)&B->U)) ||
+      (rc = dalloc(B, cc, (void**)&B->aLCC)) || (rc = dalloc(B, cc, (void**)&B->aRCC)) ||
+      (rc = dalloc(B, cc, (void**)&B->aLCCb)) || (rc = dalloc(B, cc, (void**)&B->aRCCb)) ||
+      (rc = dalloc(B, (size_t)b.N * b.N * sizeof(double), (void**)&B->Mb)))
+    return rc;
+  B->adj_ready = true;
+  return 0;
+}
+
+// Lb += d<RESb, [L,R]>/dL, Rb += d<RESb, [L,R]>/dR (the term sequence of
+// t_full; with e0 also 110 + 220).  LCCin: the cross-coupled L if already
+// formed; LCCb_acc: accumulate L's cross-coupled cotangent there instead of
+// transposing the Pandya transform here (the caller does it once).
+void comm_vjp(imsrg_basis_s* B, const double* L, int Lh, const double* R, int Rh, const double* RESb, int RESh,
+              double* Lb, double* Rb, bool e0, const double* LCCin, double* LCCb_acc, hipStream_t s) {
+  const Basis& b = B->b;
+  const DevBasis& D = B->D;
+  const int pm = Lh * Rh;
+  if (e0) launch_e0_vjp(D, B->seg_g3.dev, B->seg_g3.n, L, Lh, R, Rh, RESb, Lb, Rb, s);
+  launch_111_vjp(D, L, Lh, R, Rh, RESb, Lb, Rb, s);
+  launch_121_vjp(D, B->one, (int)B->one_h.size(), L, Lh, R, Rh, RESb, RESh, Lb, Rb, s);
+  // 222_pp_hh and 221: cotangents of the w1 / w2 products, then the
+  // transposed products in rounds of distinct outputs
+  launch_pphh_wbar(D, B->seg_full.dev, B->seg_full.n, RESb, pm, B->Wb1, B->Wb2, s);
+  launch_221_vjp(D, B->one, (int)B->one_h.size(), Lh, Rh, RESb, RESh, B->Wb1, B->Wb2, s);
+  Slots sp{};
+  sp.p[0] = L; sp.p[1] = R; sp.p[2] = Lb; sp.p[3] = Rb; sp.p[4] = B->Wb1; sp.p[5] = B->Wb2;
+  for (GemmFamily& f : B->pb)
+    if (f.tasks.n) launch_gemm_tiled(f.tile, sp, f.probs, f.tasks.dev, f.tasks.grid, f.tasks.xmap, s, (double)Lh,
+                                     (double)Rh);
+  // 122: two-body parts through the forward kernel on the symmetrised
+  // cotangent, one-body part by the partner sums
+  launch_fill(B->cbar, b.off_fpp + (i64)b.P * b.P, 0.0, s);
+  launch_to_dense(D, B->seg_full.dev, B->seg_full.n, RESb, RESh, B->cbar, s);
+  launch_comm122p(D, B->seg_tri.dev, B->seg_tri.n, B->partners, B->rowp, B->colp, R, Rh, B->cbar, RESh, B->Y, Lh, s);
+  launch_from_dense(D, B->seg_full.dev, B->seg_full.n, B->Y, -(double)Rh, Lb, s);
+  launch_comm122p(D, B->seg_tri.dev, B->seg_tri.n, B->partners, B->rowp, B->colp, L, Lh, B->cbar, RESh, B->Y, Rh, s);
+  launch_from_dense(D, B->seg_full.dev, B->seg_full.n, B->Y, (double)Lh, Rb, s);
+  launch_122_vjp_1b(D, B->seg_tri.dev, B->seg_tri.n, B->partners, B->rowp, B->colp, B->f1map, B->f1off, B->nf1, L,
+                    Lh, R, Rh, RESb, RESh, Lb, Rb, s);
+  // 222_ph: scatter transpose into the W' cotangent, M = Lh U - Rh U^T,
+  // cross-coupled cotangents, Pandya transposes
+  launch_pandya(D, B->cc.dev, B->cc.n, R, Rh, B->aRCC, s);
+  const double* LCC = LCCin;
+  if (!LCC) {
+    launch_pandya(D, B->cc.dev, B->cc.n, L, Lh, B->aLCC, s);
+    LCC = B->aLCC;
+  }
+  launch_fill(B->U, b.wclen, 0.0, s);
+  launch_ph_vjp(D, B->quad, B->qstart, B->nquads, RESb, RESh, Lh, Rh, B->U, s);
+  launch_ph_mform(D, B->ccu.dev, B->ccu.n, B->U, Lh, Rh, s);
+  double* LCCb = LCCb_acc ? LCCb_acc : B->aLCCb;
+  if (!LCCb_acc) launch_fill(B->aLCCb, b.cclen, 0.0, s);
+  launch_fill(B->aRCCb, b.cclen, 0.0, s);
+  Slots sc{};
+  sc.p[0] = LCC; sc.p[1] = B->aRCC; sc.p[2] = LCCb; sc.p[3] = B->aRCCb; sc.p[4] = B->U;
+  if (B->phb.tasks.n)
+    launch_gemm_tiled(B->phb.tile, sc, B->phb.probs, B->phb.tasks.dev, B->phb.tasks.grid, B->phb.tasks.xmap, s, 1.0,
+                      1.0);
+  launch_pandya_vjp(D, B->cc.dev, B->cc.n, B->aRCCb, Rh, Rb, s);
+  if (!LCCb_acc) launch_pandya_vjp(D, B->cc.dev, B->cc.n, B->aLCCb, Lh, Lb, s);
+}
+
 }  // namespace
 
 // =================================================================== flow
@@ -666,6 +855,15 @@ struct imsrg_flow_s {
   };
   std::map<GraphKey, hipGraphExec_t> execs;
   std::vector<void*> allocs;
+  // reverse mode: nested-commutator terms of one BCH / Magnus expansion,
+  // cotangent ping-pong, Omega's cross-coupled cotangent, and the flow
+  // gradient's checkpoints and segment states
+  std::vector<double*> terms;
+  double *Ab = nullptr, *Prev = nullptr, *GCCb = nullptr;
+  std::vector<double*> ckpts, segG, segH, segE;
+  double* decDG = nullptr;   // generator buffer of imsrg_magnus_decouple
+  double *gH = nullptr, *gHS = nullptr, *gG = nullptr, *gDG = nullptr, *gHSb = nullptr, *gGb = nullptr,
+         *gHb = nullptr, *gdOb = nullptr, *getab = nullptr;
   ~imsrg_flow_s() {
     for (auto& kv : execs) (void)hipGraphExecDestroy(kv.second);
     for (void* p : allocs) (void)hipFree(p);
@@ -826,6 +1024,242 @@ int magnus_step(imsrg_flow_s* F, double* HS, double* G, double* DG, const double
     out[2] = nm;
     out[3] = nb;
   }
+  return 0;
+}
+
+
+// ------------------------------------------------- reverse mode (flow)
+int fbuf(imsrg_flow_s* F, double** slot, i64 n) {
+  if (*slot) return 0;
+  return falloc(F, n, slot);
+}
+int term_buf(imsrg_flow_s* F, size_t k, double** out) {
+  while (F->terms.size() <= k) {
+    double* p = nullptr;
+    if (int rc = falloc(F, F->B->b.oplen, &p)) return rc;
+    F->terms.push_back(p);
+  }
+  *out = F->terms[k];
+  return 0;
+}
+int grow(imsrg_flow_s* F, std::vector<double*>& v, size_t k) {
+  while (v.size() <= k) {
+    double* p = nullptr;
+    if (int rc = falloc(F, F->B->b.oplen, &p)) return rc;
+    v.push_back(p);
+  }
+  return 0;
+}
+
+// BCH_EXPAND transpose (IMSRG_MAGNUS.f90:149-234): Gb += d<HSb, HS>/dG,
+// Hb += d<HSb, HS>/dH for HS = BCH(G, H).  The nested commutators are
+// recomputed with the same exit, kept, and swept in reverse.
+int bch_vjp(imsrg_flow_s* F, const double* G, const double* H, int Hh, double conv, int maxit, const double* HSb,
+            double* Gb, double* Hb, int* nterms, hipStream_t s) {
+  imsrg_basis_s* B = F->B;
+  const i64 n = B->b.oplen;
+  int rc;
+  if ((rc = ensure_adj(B))) return rc;
+  if ((rc = fbuf(F, &F->Ab, n)) || (rc = fbuf(F, &F->Prev, n)) || (rc = fbuf(F, &F->GCCb, B->b.cclen))) return rc;
+  double* A0 = nullptr;
+  if ((rc = term_buf(F, 0, &A0))) return rc;
+  HIPCHK(hipMemcpyAsync(A0, H, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+  t_pandya(B, G, -1, F->GCC, s);
+  std::vector<double> coefs(1, 1.0);
+  double coef = 1.0;
+  int iw;
+  for (iw = 2; iw <= maxit; ++iw) {
+    coef = coef / (iw - 1.0);
+    double* nx = nullptr;
+    if ((rc = term_buf(F, (size_t)iw - 1, &nx))) return rc;
+    t_full(B, G, -1, F->terms[iw - 2], Hh, nx, Hh, true, F->w1, F->w2, F->GCC, F->ADCC, F->WCC, s);
+    t_norm(B, nx, B->dval, s);
+    double nrm = 0.0;
+    if ((rc = read_scalar(B, B->dval, &nrm, s))) return rc;
+    coefs.push_back(coef);
+    if (nrm * coef < conv) break;
+  }
+  if (nterms) *nterms = iw;
+  if (iw > maxit) return fail(IMSRG_ECLIP, "clipping BCH");
+  const int K = (int)coefs.size() - 1;
+  launch_axpy(n, HSb, coefs[K], HSb, 0.0, F->Ab, s);
+  launch_fill(F->GCCb, B->b.cclen, 0.0, s);
+  for (int k = K; k >= 1; --k) {
+    launch_fill(F->Prev, n, 0.0, s);
+    comm_vjp(B, G, -1, F->terms[k - 1], Hh, F->Ab, Hh, Gb, F->Prev, true, F->GCC, F->GCCb, s);
+    launch_axpy(n, F->Prev, 1.0, HSb, coefs[k - 1], F->Ab, s);
+  }
+  launch_axpy(n, Hb, 1.0, F->Ab, 1.0, Hb, s);
+  launch_pandya_vjp(B->D, B->cc.dev, B->cc.n, F->GCCb, -1, Gb, s);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// MAGNUS_EXPAND transpose (IMSRG_MAGNUS.f90:620-691): ETAb += d<DGb, dOmega>/d eta,
+// Gb += d<DGb, dOmega>/dOmega (eta, Omega, dOmega antihermitian)
+int magnus_vjp(imsrg_flow_s* F, const double* ETA, const double* G, double conv, const double* DGb, double* ETAb,
+               double* Gb, int* ncomm, hipStream_t s) {
+  imsrg_basis_s* B = F->B;
+  const i64 n = B->b.oplen;
+  const double cof[7] = {1.0, -0.5, .0833333333, 0.0, -0.00138888888, 0.0, 3.306878e-5};
+  double advals[7] = {0, 0, 0, 0, 0, 0, 0};
+  int rc;
+  if ((rc = ensure_adj(B))) return rc;
+  if ((rc = fbuf(F, &F->Ab, n)) || (rc = fbuf(F, &F->Prev, n)) || (rc = fbuf(F, &F->GCCb, B->b.cclen))) return rc;
+  double* A0 = nullptr;
+  if ((rc = term_buf(F, 0, &A0))) return rc;
+  HIPCHK(hipMemcpyAsync(A0, ETA, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+  t_norm(B, ETA, B->dval, s);
+  t_norm(B, G, B->dval + 1, s);
+  HIPCHK(hipMemcpyAsync(B->hval, B->dval, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  advals[0] = B->hval[0];
+  const double fullnorm = B->hval[1];
+  std::vector<double> cs(1, 1.0);
+  if (!(fullnorm < (double)1e-9f)) {
+    t_pandya(B, G, -1, F->GCC, s);
+    int q = 1;
+    for (int i = 2; i <= 7; ++i) {
+      const double adnorm = advals[i - q - 1];
+      q = (std::fabs(cof[i - 1]) > (double)1e-6f) ? 1 : 2;
+      if (std::fabs(adnorm / fullnorm) < conv) break;
+      const size_t K = cs.size() - 1;
+      double* nx = nullptr;
+      if ((rc = term_buf(F, K + 1, &nx))) return rc;
+      HIPCHK(hipMemcpyAsync(nx, F->terms[K], sizeof(double), hipMemcpyDeviceToDevice, s));
+      t_full(B, G, -1, F->terms[K], -1, nx, -1, false, F->w1, F->w2, F->GCC, F->ADCC, F->WCC, s);
+      t_norm(B, nx, B->dval, s);
+      double nrm = 0.0;
+      if ((rc = read_scalar(B, B->dval, &nrm, s))) return rc;
+      advals[i - 1] = nrm * std::fabs(cof[i - 1]);
+      cs.push_back(cof[i - 1]);
+    }
+  }
+  const int K = (int)cs.size() - 1;
+  double e0sum = 0.0;
+  for (double c : cs) e0sum += c;
+  launch_axpy(n, DGb, cs[K], DGb, 0.0, F->Ab, s);
+  launch_fill(F->Ab, 1, 0.0, s);
+  if (K > 0) launch_fill(F->GCCb, B->b.cclen, 0.0, s);
+  for (int k = K; k >= 1; --k) {
+    launch_fill(F->Prev, n, 0.0, s);
+    comm_vjp(B, G, -1, F->terms[k - 1], -1, F->Ab, -1, Gb, F->Prev, false, F->GCC, F->GCCb, s);
+    launch_axpy(n, F->Prev, 1.0, DGb, cs[k - 1], F->Ab, s);
+    launch_fill(F->Ab, 1, 0.0, s);
+  }
+  launch_axpy(n, ETAb, 1.0, F->Ab, 1.0, ETAb, s);
+  launch_add_e0(ETAb, DGb, e0sum, s);
+  if (K > 0) launch_pandya_vjp(B->D, B->cc.dev, B->cc.n, F->GCCb, -1, Gb, s);
+  if (ncomm) *ncomm = K;
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// build_gs_white transpose (generators.f90:4-106): Hb += d<ETAb, white(H)>/dH
+int white_vjp(imsrg_basis_s* B, const double* H, const double* ETAb, double* Hb, hipStream_t s) {
+  int rc;
+  if ((rc = ensure_adj(B))) return rc;
+  launch_monopole(B->D, H, B->M, s);
+  launch_fill(B->Mb, (i64)B->b.N * B->b.N, 0.0, s);
+  launch_white_vjp(B->D, B->seg_g3.dev, B->seg_g3.n, H, B->M, ETAb, Hb, B->Mb, s);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// normal_order transpose (basic_IMSRG.f90:2419-2550): Tb = d<Hb, NO(T)>/dT
+int normal_order_vjp(imsrg_basis_s* B, const double* Hb, double* Tb, hipStream_t s) {
+  HIPCHK(hipMemcpyAsync(Tb, Hb, B->b.oplen * sizeof(double), hipMemcpyDeviceToDevice, s));
+  launch_normal_order_vjp(B->D, Hb, Tb, s);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// dE0/dT of the flow of magnus_decouple (white generator, |mbpt2| criterion)
+// started from H = normal_order(T).  Omega is checkpointed every `ckpt`
+// steps; each segment is recomputed from its checkpoint (Omega, H(s), eta)
+// and swept in reverse through BCH_EXPAND, euler_step, MAGNUS_EXPAND and the
+// generator.
+int flow_grad(imsrg_flow_s* F, const double* T, double ds, double bch_conv, double mag_conv, double crit_tol,
+              int maxsteps, int ckpt, double* Tb, double* E0, int* nsteps, hipStream_t s) {
+  imsrg_basis_s* B = F->B;
+  const i64 n = B->b.oplen;
+  int rc, nt = 0, nc = 0;
+  if (ckpt < 1) return fail(IMSRG_EINVAL, "checkpoint interval must be >= 1");
+  if ((rc = ensure_adj(B))) return rc;
+  for (double** p : {&F->gH, &F->gHS, &F->gG, &F->gDG, &F->gHSb, &F->gGb, &F->gHb, &F->gdOb, &F->getab})
+    if ((rc = fbuf(F, p, n))) return rc;
+  HIPCHK(hipMemcpyAsync(F->gH, T, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+  launch_normal_order(B->D, F->gH, B->dval + 8, s);
+  HIPCHK(hipMemcpyAsync(F->gHS, F->gH, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+  launch_fill(F->gG, n, 0.0, s);
+  launch_fill(F->gDG, n, 0.0, s);
+  if ((rc = white(B, F->gHS, F->gDG, s)) || (rc = mbpt2_dev(B, F->gHS, B->dval + 2, s))) return rc;
+  double crit = 0.0;
+  if ((rc = read_scalar(B, B->dval + 2, &crit, s))) return rc;
+  crit = std::fabs(crit);
+  int steps = 0;
+  while (crit > crit_tol) {
+    if (steps >= maxsteps) return fail(IMSRG_ESTEPS, "flow gradient: step limit reached");
+    if (steps % ckpt == 0) {
+      if ((rc = grow(F, F->ckpts, (size_t)(steps / ckpt)))) return rc;
+      HIPCHK(hipMemcpyAsync(F->ckpts[steps / ckpt], F->gG, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    }
+    if ((rc = magnus(F, F->gDG, -1, F->gG, -1, mag_conv, &nc, s))) return rc;
+    launch_axpy(n, F->gG, 1.0, F->gDG, ds, F->gG, s);
+    if ((rc = bch(F, F->gHS, F->gG, -1, F->gH, 1, bch_conv, 40, &nt, nullptr, s))) return rc;
+    if ((rc = white(B, F->gHS, F->gDG, s)) || (rc = mbpt2_dev(B, F->gHS, B->dval + 2, s))) return rc;
+    if ((rc = read_scalar(B, B->dval + 2, &crit, s))) return rc;
+    crit = std::fabs(crit);
+    ++steps;
+  }
+  double e0 = 0.0;
+  if ((rc = read_scalar(B, F->gHS, &e0, s))) return rc;
+  // reverse sweep
+  launch_fill(F->gHSb, n, 0.0, s);
+  launch_fill(F->gHSb, 1, 1.0, s);
+  launch_fill(F->gGb, n, 0.0, s);
+  launch_fill(F->gHb, n, 0.0, s);
+  if ((rc = grow(F, F->segG, (size_t)ckpt)) || (rc = grow(F, F->segH, (size_t)ckpt)) ||
+      (rc = grow(F, F->segE, (size_t)ckpt)))
+    return rc;
+  const int nck = (steps + ckpt - 1) / ckpt;
+  for (int seg = nck - 1; seg >= 0; --seg) {
+    const int n0 = seg * ckpt, n1 = std::min(steps, n0 + ckpt);
+    HIPCHK(hipMemcpyAsync(F->segG[0], F->ckpts[seg], n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    if (n0 == 0) {
+      HIPCHK(hipMemcpyAsync(F->segH[0], F->gH, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    } else if ((rc = bch(F, F->segH[0], F->segG[0], -1, F->gH, 1, bch_conv, 40, &nt, nullptr, s))) {
+      return rc;
+    }
+    launch_fill(F->segE[0], n, 0.0, s);
+    if ((rc = white(B, F->segH[0], F->segE[0], s))) return rc;
+    for (int m = n0; m < n1; ++m) {
+      const int k = m - n0;
+      HIPCHK(hipMemcpyAsync(F->gDG, F->segE[k], n * sizeof(double), hipMemcpyDeviceToDevice, s));
+      if ((rc = magnus(F, F->gDG, -1, F->segG[k], -1, mag_conv, &nc, s))) return rc;
+      launch_axpy(n, F->segG[k], 1.0, F->gDG, ds, F->segG[k + 1], s);
+      if ((rc = bch(F, F->segH[k + 1], F->segG[k + 1], -1, F->gH, 1, bch_conv, 40, &nt, nullptr, s))) return rc;
+      launch_fill(F->segE[k + 1], n, 0.0, s);
+      if ((rc = white(B, F->segH[k + 1], F->segE[k + 1], s))) return rc;
+    }
+    for (int m = n1 - 1; m >= n0; --m) {
+      const int k = m - n0;
+      if ((rc = bch_vjp(F, F->segG[k + 1], F->gH, 1, bch_conv, 40, F->gHSb, F->gGb, F->gHb, &nt, s))) return rc;
+      launch_axpy(n, F->gGb, ds, F->gGb, 0.0, F->gdOb, s);
+      launch_fill(F->getab, n, 0.0, s);
+      if ((rc = magnus_vjp(F, F->segE[k], F->segG[k], mag_conv, F->gdOb, F->getab, F->gGb, &nc, s))) return rc;
+      launch_fill(F->gHSb, n, 0.0, s);
+      if ((rc = white_vjp(B, F->segH[k], F->getab, F->gHSb, s))) return rc;
+      if (m == 0) {
+        launch_axpy(n, F->gHb, 1.0, F->gHSb, 1.0, F->gHb, s);
+        launch_fill(F->gHSb, n, 0.0, s);
+      }
+    }
+  }
+  if ((rc = normal_order_vjp(B, F->gHb, Tb, s))) return rc;
+  HIPCHK(hipStreamSynchronize(s));
+  if (E0) *E0 = e0;
+  if (nsteps) *nsteps = steps;
   return 0;
 }
 
@@ -1149,8 +1583,8 @@ int imsrg_magnus_decouple(imsrg_flow F, double* HS, double* G, double ds, double
   hipStream_t s = S(stream);
   const i64 n = B->b.oplen;
   int rc;
-  double* DG = nullptr;
-  if ((rc = falloc(F, n, &DG))) return rc;
+  if ((rc = fbuf(F, &F->decDG, n))) return rc;
+  double* DG = F->decDG;
   launch_fill(DG, n, 0.0, s);
   HIPCHK(hipMemcpyAsync(F->Hbare, HS, n * sizeof(double), hipMemcpyDeviceToDevice, s));
   if ((rc = white(B, HS, DG, s))) return rc;
@@ -1186,6 +1620,47 @@ int imsrg_magnus_decouple(imsrg_flow F, double* HS, double* G, double ds, double
   }
   if (steps) *steps = st;
   return 0;
+}
+
+// ------------------------------------------------------- reverse mode
+int imsrg_commutator_vjp(imsrg_basis B, const double* L, int Lh, const double* R, int Rh, const double* RESb,
+                         int RESh, double* Lb, double* Rb, imsrg_stream stream) {
+  if (!B || !L || !R || !RESb || !Lb || !Rb) return fail(IMSRG_EINVAL, "null argument");
+  int rc = ensure_adj(B);
+  if (rc) return rc;
+  comm_vjp(B, L, Lh, R, Rh, RESb, RESh, Lb, Rb, true, nullptr, nullptr, S(stream));
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int imsrg_bch_vjp(imsrg_flow F, const double* G, const double* H, int Hherm, double conv, int maxit,
+                  const double* HSb, double* Gb, double* Hb, int* nterms, imsrg_stream stream) {
+  if (!F || !G || !H || !HSb || !Gb || !Hb) return fail(IMSRG_EINVAL, "null argument");
+  return bch_vjp(F, G, H, Hherm, conv, maxit, HSb, Gb, Hb, nterms, S(stream));
+}
+
+int imsrg_magnus_vjp(imsrg_flow F, const double* ETA, const double* G, double conv, const double* DGb, double* ETAb,
+                     double* Gb, int* ncomm, imsrg_stream stream) {
+  if (!F || !ETA || !G || !DGb || !ETAb || !Gb) return fail(IMSRG_EINVAL, "null argument");
+  return magnus_vjp(F, ETA, G, conv, DGb, ETAb, Gb, ncomm, S(stream));
+}
+
+int imsrg_white_vjp(imsrg_basis B, const double* H, const double* ETAb, double* Hb, imsrg_stream stream) {
+  if (!B || !H || !ETAb || !Hb) return fail(IMSRG_EINVAL, "null argument");
+  return white_vjp(B, H, ETAb, Hb, S(stream));
+}
+
+int imsrg_normal_order_vjp(imsrg_basis B, const double* Hb, double* Tb, imsrg_stream stream) {
+  if (!B || !Hb || !Tb) return fail(IMSRG_EINVAL, "null argument");
+  int rc = ensure_dev(B);
+  if (rc) return rc;
+  return normal_order_vjp(B, Hb, Tb, S(stream));
+}
+
+int imsrg_flow_grad(imsrg_flow F, const double* T, double ds, double bch_conv, double mag_conv, double crit_tol,
+                    int maxsteps, int ckpt, double* Tb, double* E0, int* steps, imsrg_stream stream) {
+  if (!F || !T || !Tb) return fail(IMSRG_EINVAL, "null argument");
+  return flow_grad(F, T, ds, bch_conv, mag_conv, crit_tol, maxsteps, ckpt, Tb, E0, steps, S(stream));
 }
 
 int imsrg_set_gemm_tile(imsrg_basis B, int family, int tile) {

@@ -143,7 +143,8 @@ __global__ __launch_bounds__(256) void k_gemm_t(Slots sl, const GemmProb* probs,
   double* C = const_cast<double*>(sl.p[pr.sc]) + pr.c + (i64)t.ks * pr.cstride;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wm = wave >> 1, wn = wave & 1;
-  const double al = pr.alpha * ascale;
+  const double al = pr.mode == 0 ? pr.alpha * ascale
+                                  : pr.alpha * ((pr.mode & 1) ? ascale : 1.0) * ((pr.mode & 2) ? s2scale : 1.0);
 #pragma unroll
   for (int mi = 0; mi < MI; ++mi)
 #pragma unroll

@@ -44,7 +44,8 @@ struct GemmProb {
   int nprod;
   unsigned char ta1, tb1, ta2, tb2;
   unsigned char sa1, sb1, sa2, sb2;
-  unsigned char sc, pad0, pad1, pad2;
+  unsigned char sc, mode, pad1, pad2;   // mode 0: alpha*ascale, s2*s2scale;
+                                        // else bit0: *ascale, bit1: *s2scale (one product)
   double alpha, s2, beta;
   int kchunk;        // > 0: split K in chunks of kchunk, chunk ks written to C + ks*cstride
   int k2;            // > 0: K of the second product when it differs (no split K then)
@@ -130,5 +131,33 @@ void launch_mbpt2_partials(const DevBasis& B, const SegTask* t, int nt, const do
 void launch_normal_order(const DevBasis& B, double* H, double* scratch, hipStream_t s);
 void launch_fill(double* p, i64 n, double v, hipStream_t s);
 void launch_copy(double* dst, const double* src, i64 n, hipStream_t s);
+
+// reverse-mode kernels (kernels_adj.hip)
+void launch_e0_vjp(const DevBasis& B, const SegTask* g3, int ng3, const double* L, int Lh, const double* R, int Rh,
+                   const double* RESb, double* Lb, double* Rb, hipStream_t s);
+void launch_111_vjp(const DevBasis& B, const double* L, int Lh, const double* R, int Rh, const double* RESb,
+                    double* Lb, double* Rb, hipStream_t s);
+void launch_121_vjp(const DevBasis& B, const OneTask* t, int nt, const double* L, int Lh, const double* R, int Rh,
+                    const double* RESb, int RESh, double* Lb, double* Rb, hipStream_t s);
+void launch_221_vjp(const DevBasis& B, const OneTask* t, int nt, int Lh, int Rh, const double* RESb, int RESh,
+                    double* Wb1, double* Wb2, hipStream_t s);
+void launch_pphh_wbar(const DevBasis& B, const SegTask* t, int nt, const double* RESb, int pm, double* Wb1,
+                      double* Wb2, hipStream_t s);
+void launch_to_dense(const DevBasis& B, const SegTask* t, int nt, const double* X, int herm, double* cbar,
+                     hipStream_t s);
+void launch_from_dense(const DevBasis& B, const SegTask* t, int nt, const double* Y, double alpha, double* dst,
+                       hipStream_t s);
+void launch_122_vjp_1b(const DevBasis& B, const SegTask* t, int nt, const Partner* part, const int* rowp,
+                       const int* colp, const int* f1map, const int* f1off, int nf1, const double* L, int Lh,
+                       const double* R, int Rh, const double* RESb, int RESh, double* Lb, double* Rb, hipStream_t s);
+void launch_ph_vjp(const DevBasis& B, const QuadItem* items, const int* qstart, int nquads, const double* RESb,
+                   int RESh, int Lh, int Rh, double* U, hipStream_t s);
+void launch_ph_mform(const DevBasis& B, const CCTask* t, int nt, double* U, int Lh, int Rh, hipStream_t s);
+void launch_pandya_vjp(const DevBasis& B, const CCTask* t, int nt, const double* CCb, int herm, double* opb,
+                       hipStream_t s);
+void launch_white_vjp(const DevBasis& B, const SegTask* g3, int ng3, const double* H, const double* M,
+                      const double* ETAb, double* Hb, double* Mb, hipStream_t s);
+void launch_normal_order_vjp(const DevBasis& B, const double* Hb, double* Tb, hipStream_t s);
+void launch_add_e0(double* dst, const double* src, double factor, hipStream_t s);
 
 }  // namespace imsrg

@@ -1,0 +1,690 @@
+// Reverse-mode (vector-Jacobian) kernels of the scalar-commutator path, the
+// transposes of the forward kernels in kernels.hip.  The reference has no
+// adjoint; these give d<RESbar, [L,R]>/dL and /dR on the (anti)hermitian,
+// Pauli-respecting operators the flow produces.  Every term's transpose reads
+// the same commutator cotangent RESb: each forward term maps such operators
+// to such operators, so its cotangent is RESb restricted to them.
+//
+//  * element gathers of the forward pass (121, 221, Pandya, 222_ph scatter)
+//    become scatters with FP64 atomics into their sparse sources;
+//  * 222_pp_hh's products are transposed by the grouped MFMA GEMM;
+//  * 122's two-body part reuses the forward 122 kernel on the symmetrised,
+//    Pauli-masked cotangent (a 1b x 2b commutator is its own transpose up
+//    to herm factors); its one-body part sums per workgroup in LDS.
+#include <hip/hip_runtime.h>
+
+#include "device.hpp"
+#include "kernels.hpp"
+
+namespace imsrg {
+
+namespace {
+
+__device__ __forceinline__ void aadd(double* p, double v) { unsafeAtomicAdd(p, v); }
+
+// add v into the storage position vat(p, op, herm) reads
+__device__ __forceinline__ void vadj(const VPos& p, double* bar, int herm, double v) {
+  if (p.off < 0) return;
+  aadd(bar + p.off, (p.swp ? v * herm : v) * p.pre);
+}
+
+// add v into the storage position f_elem(B, op, herm, a, c) reads
+__device__ __forceinline__ void fadj(const DevBasis& B, double* bar, int herm, int a, int c, double v) {
+  const int c1 = B.con[a], c2 = B.con[c];
+  i64 o;
+  double s = 1.0;
+  if (c1 + c2 == 0) o = B.off_fpp + B.cidx[a] + (i64)B.cidx[c] * B.P;
+  else if (c1 + c2 == 2) o = B.off_fhh + B.cidx[a] + (i64)B.cidx[c] * B.A;
+  else if (c1 > c2) { o = B.off_fph + B.cidx[c] + (i64)B.cidx[a] * B.P; s = herm; }
+  else o = B.off_fph + B.cidx[a] + (i64)B.cidx[c] * B.P;
+  aadd(bar + o, v * s);
+}
+
+// cotangent of the value u written at (IX,JX) of block g, the lower partner
+// of a square block being filled with u*herm afterwards
+__device__ __forceinline__ double fold2(const DevBasis& B, const double* Xb, int q, int g, int IX, int JX, int herm) {
+  const int n1 = brows(B, q, g);
+  const double* X = Xb + B.goff[q * 6 + g];
+  if (!is_square(g)) return X[IX + (i64)JX * n1];
+  if (IX == JX) return X[IX + (i64)IX * n1] * herm;
+  return X[IX + (i64)JX * n1] + herm * X[JX + (i64)IX * n1];
+}
+
+// cotangent of the one-body value added by onebody_update
+__device__ __forceinline__ double fold1(const DevBasis& B, const OneTask& t, const double* Xb, int herm) {
+  if (t.kind == 2) return Xb[B.off_fph + t.pi + (i64)t.qi * B.P];
+  const double* F = Xb + (t.kind == 0 ? B.off_fhh : B.off_fpp);
+  const int ld = (t.kind == 0) ? B.A : B.P;
+  if (t.pi == t.qi) return F[t.pi + (i64)t.pi * ld] * herm;
+  return F[t.pi + (i64)t.qi * ld] + herm * F[t.qi + (i64)t.pi * ld];
+}
+
+__device__ double bsum(double v, double* red) {
+  const int tid = threadIdx.x;
+  red[tid] = v;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (tid < s) red[tid] += red[tid + s];
+    __syncthreads();
+  }
+  const double r = red[0];
+  __syncthreads();
+  return r;
+}
+
+}  // namespace
+
+// ------------------------------------------------ 110 / 220 (E0) transpose
+__global__ __launch_bounds__(256) void k_e0_vjp_1b(DevBasis B, const double* L, int Lh, const double* R, int Rh,
+                                                   const double* RESb, double* Lb, double* Rb) {
+  const i64 x = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= (i64)B.P * B.A) return;
+  const double f = RESb[0] * (Lh - Rh) * (B.jj[B.holes[(int)(x / B.P)]] + 1);
+  const i64 o = B.off_fph + x;
+  Lb[o] += f * R[o];
+  Rb[o] += f * L[o];
+}
+
+__global__ __launch_bounds__(256) void k_e0_vjp_2b(DevBasis B, const SegTask* tasks, const double* L, int Lh,
+                                                   const double* R, int Rh, const double* RESb, double* Lb,
+                                                   double* Rb) {
+  const SegTask t = tasks[blockIdx.x];
+  const i64 e = (i64)t.start + threadIdx.x;
+  if (e >= t.cnt) return;
+  const i64 o = B.goff[t.q * 6 + 2] + e;
+  const double c = -2.0 * RESb[0] * (B.chJ[t.q] + 1) * Rh;
+  Lb[o] += c * R[o];
+  Rb[o] += c * L[o];
+}
+
+void launch_e0_vjp(const DevBasis& B, const SegTask* g3, int ng3, const double* L, int Lh, const double* R, int Rh,
+                   const double* RESb, double* Lb, double* Rb, hipStream_t s) {
+  const i64 n = (i64)B.P * B.A;
+  if (n > 0)
+    hipLaunchKernelGGL(k_e0_vjp_1b, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, B, L, Lh, R, Rh, RESb, Lb,
+                       Rb);
+  if (Lh * Rh == -1 && ng3 > 0)
+    hipLaunchKernelGGL(k_e0_vjp_2b, dim3(ng3), dim3(256), 0, s, B, g3, L, Lh, R, Rh, RESb, Lb, Rb);
+}
+
+// ------------------------------------------------------- 111 transpose
+// One thread per one-body element of Lb (first E1 threads) or Rb.
+__global__ __launch_bounds__(256) void k_111_vjp(DevBasis B, const double* L, int Lh, const double* R, int Rh,
+                                                 const double* RESb, double* Lb, double* Rb) {
+  const int A = B.A, P = B.P;
+  const i64 E1 = (i64)A * A + (i64)P * A + (i64)P * P;
+  const i64 idx = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= 2 * E1) return;
+  const bool side = idx >= E1;   // false: Lb, true: Rb
+  const i64 x = side ? idx - E1 : idx;
+  const double *Lhh = L + B.off_fhh, *Lph = L + B.off_fph, *Lpp = L + B.off_fpp;
+  const double *Rhh = R + B.off_fhh, *Rph = R + B.off_fph, *Rpp = R + B.off_fpp;
+  const double *Bhh = RESb + B.off_fhh, *Bph = RESb + B.off_fph, *Bpp = RESb + B.off_fpp;
+  auto th1b = [&](int i, int j) { return Bhh[i + j * A] - Lh * Rh * Bhh[j + i * A]; };
+  auto th2b = [&](int i, int j) { return Lh * Bhh[i + j * A] - Rh * Bhh[j + i * A]; };
+  auto tp1b = [&](int p, int r) { return Rh * Bpp[p + (i64)r * P] - Lh * Bpp[r + (i64)p * P]; };
+  auto tp2b = [&](int p, int r) { return Bpp[p + (i64)r * P] - Rh * Lh * Bpp[r + (i64)p * P]; };
+  double s = 0.0;
+  if (x < (i64)A * A) {
+    const int i = (int)(x % A), j = (int)(x / A);
+    if (!side) {
+      for (int k = 0; k < A; ++k) s += th1b(i, k) * Rhh[j + k * A];
+      for (int p = 0; p < P; ++p) s -= Rph[p + (i64)i * P] * Bph[p + (i64)j * P];
+      Lb[B.off_fhh + x] += s;
+    } else {
+      for (int k = 0; k < A; ++k) s += Lhh[k + i * A] * th1b(k, j);
+      for (int p = 0; p < P; ++p) s += Lph[p + (i64)i * P] * Bph[p + (i64)j * P];
+      Rb[B.off_fhh + x] += s;
+    }
+  } else if (x < (i64)A * A + (i64)P * A) {
+    const i64 y = x - (i64)A * A;
+    const int p = (int)(y % P), i = (int)(y / P);
+    if (!side) {
+      for (int j = 0; j < A; ++j) s += Rph[p + (i64)j * P] * th2b(i, j) + Bph[p + (i64)j * P] * Rhh[i + j * A];
+      for (int r = 0; r < P; ++r) s += -Rpp[r + (i64)p * P] * Bph[r + (i64)i * P] + tp1b(p, r) * Rph[r + (i64)i * P];
+      Lb[B.off_fph + y] += s;
+    } else {
+      for (int k = 0; k < A; ++k) s += Lph[p + (i64)k * P] * th2b(k, i) - Bph[p + (i64)k * P] * Lhh[i + k * A];
+      for (int r = 0; r < P; ++r) s += Lpp[r + (i64)p * P] * Bph[r + (i64)i * P] + tp1b(r, p) * Lph[r + (i64)i * P];
+      Rb[B.off_fph + y] += s;
+    }
+  } else {
+    const i64 y = x - (i64)A * A - (i64)P * A;
+    const int p = (int)(y % P), r = (int)(y / P);
+    if (!side) {
+      for (int i = 0; i < A; ++i) s += Bph[p + (i64)i * P] * Rph[r + (i64)i * P];
+      for (int k = 0; k < P; ++k) s += tp2b(p, k) * Rpp[r + (i64)k * P];
+      Lb[B.off_fpp + y] += s;
+    } else {
+      for (int i = 0; i < A; ++i) s -= Bph[p + (i64)i * P] * Lph[r + (i64)i * P];
+      for (int k = 0; k < P; ++k) s += Lpp[k + (i64)p * P] * tp2b(k, r);
+      Rb[B.off_fpp + y] += s;
+    }
+  }
+}
+
+void launch_111_vjp(const DevBasis& B, const double* L, int Lh, const double* R, int Rh, const double* RESb,
+                    double* Lb, double* Rb, hipStream_t s) {
+  const i64 n = 2 * ((i64)B.A * B.A + (i64)B.P * B.A + (i64)B.P * B.P);
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_111_vjp, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, B, L, Lh, R, Rh, RESb, Lb, Rb);
+}
+
+// ------------------------------------------------------- 121 transpose
+__global__ __launch_bounds__(128) void k_121_vjp(DevBasis B, const OneTask* tasks, const double* L, int Lh,
+                                                 const double* R, int Rh, const double* RESb, int RESh, double* Lb,
+                                                 double* Rb) {
+  const OneTask t = tasks[blockIdx.x];
+  const int pk = t.pk, qk = t.qk, jq = B.jj[qk];
+  const double sb = fold1(B, t, RESb, RESh) / (B.jj[pk] + 1.0);
+  if (sb == 0.0) return;
+  const double* Lph = L + B.off_fph;
+  const double* Rph = R + B.off_fph;
+  for (int x = threadIdx.x; x < B.P * B.A; x += blockDim.x) {
+    const int a = x / B.A, i = x % B.A;
+    const int ak = B.parts[a], ik = B.holes[i];
+    if (B.jj[ik] != B.jj[ak] || B.ll[ik] != B.ll[ak] || B.tz[ik] != B.tz[ak]) continue;
+    const int ji = B.jj[ik];
+    const i64 o = a + (i64)i * B.P;
+    const double lf = Lph[o], rf = Rph[o];
+    double smx = 0, smy = 0, smx2 = 0, smy2 = 0;
+    for (int JT = abs(ji - jq); JT <= ji + jq; JT += 2) {
+      const VPos p1 = vpos(B, ak, pk, ik, qk, JT);
+      const VPos p2 = vpos(B, ik, pk, ak, qk, JT);
+      smx += vat(p1, R, Rh) * (JT + 1);
+      smx2 += vat(p1, L, Lh) * (JT + 1);
+      smy += vat(p2, R, Rh) * (JT + 1);
+      smy2 += vat(p2, L, Lh) * (JT + 1);
+      vadj(p1, Rb, Rh, sb * lf * Lh * (JT + 1));
+      vadj(p2, Rb, Rh, -sb * lf * (JT + 1));
+      vadj(p1, Lb, Lh, -sb * rf * Rh * (JT + 1));
+      vadj(p2, Lb, Lh, sb * rf * (JT + 1));
+    }
+    aadd(Lb + B.off_fph + o, sb * (Lh * smx - smy));
+    aadd(Rb + B.off_fph + o, -sb * (Rh * smx2 - smy2));
+  }
+}
+
+void launch_121_vjp(const DevBasis& B, const OneTask* t, int nt, const double* L, int Lh, const double* R, int Rh,
+                    const double* RESb, int RESh, double* Lb, double* Rb, hipStream_t s) {
+  if (nt <= 0) return;
+  hipLaunchKernelGGL(k_121_vjp, dim3(nt), dim3(128), 0, s, B, t, L, Lh, R, Rh, RESb, RESh, Lb, Rb);
+}
+
+// ------------------------------------------------------- 221 transpose
+// RESb (one-body) -> cotangents of the w1 / w2 elements 221 reads, added
+// into Wb1 / Wb2 (which already hold the 222_pp_hh part)
+__global__ __launch_bounds__(128) void k_221_vjp(DevBasis B, const OneTask* tasks, int Lh, int Rh,
+                                                 const double* RESb, int RESh, double* Wb1, double* Wb2) {
+  const OneTask t = tasks[blockIdx.x];
+  const int ik = t.pk, jk = t.qk, ji = B.jj[ik];
+  const int pm = Lh * Rh;
+  const bool both = (t.kind != 2);
+  const double sb = fold1(B, t, RESb, RESh) / (ji + 1.0);
+  if (sb == 0.0) return;
+  for (int x = threadIdx.x; x < B.N; x += blockDim.x) {
+    const bool hole = x < B.A;
+    const int ck = hole ? B.holes[x] : B.parts[x - B.A];
+    double* W = hole ? Wb1 : Wb2;
+    const int jc = B.jj[ck];
+    for (int JT = abs(jc - ji); JT <= jc + ji; JT += 2) {
+      vadj(vpos(B, ck, ik, ck, jk, JT), W, 1, sb * (JT + 1));
+      if (both) vadj(vpos(B, ck, jk, ck, ik, JT), W, 1, -pm * sb * (JT + 1));
+    }
+  }
+}
+
+void launch_221_vjp(const DevBasis& B, const OneTask* t, int nt, int Lh, int Rh, const double* RESb, int RESh,
+                    double* Wb1, double* Wb2, hipStream_t s) {
+  if (nt <= 0) return;
+  hipLaunchKernelGGL(k_221_vjp, dim3(nt), dim3(128), 0, s, B, t, Lh, Rh, RESb, RESh, Wb1, Wb2);
+}
+
+// ------------------------------------------- 222_pp_hh cotangent blocks
+// Wb1 / Wb2 block g = cotangent of the w1 / w2 product of block g:
+// square g: Y = X - pm X^T (X = RESb block), Wb1 = Y, Wb2 = -Y;
+// rectangular g: Wb1 = X, Wb2 = -X.
+__global__ __launch_bounds__(256) void k_pphh_wbar(DevBasis B, const SegTask* tasks, const double* RESb, int pm,
+                                                   double* Wb1, double* Wb2) {
+  const SegTask t = tasks[blockIdx.x];
+  const i64 e = (i64)t.start + threadIdx.x;
+  if (e >= t.cnt) return;
+  const int q = t.q, g = t.g;
+  const i64 base = B.goff[q * 6 + g];
+  double y;
+  if (is_square(g)) {
+    const int n1 = brows(B, q, g);
+    const int IX = (int)(e % n1), JX = (int)(e / n1);
+    y = RESb[base + e] - pm * RESb[base + JX + (i64)IX * n1];
+  } else {
+    y = RESb[base + e];
+  }
+  Wb1[base + e] = y;
+  Wb2[base + e] = -y;
+}
+
+void launch_pphh_wbar(const DevBasis& B, const SegTask* t, int nt, const double* RESb, int pm, double* Wb1,
+                      double* Wb2, hipStream_t s) {
+  if (nt <= 0) return;
+  hipLaunchKernelGGL(k_pphh_wbar, dim3(nt), dim3(256), 0, s, B, t, RESb, pm, Wb1, Wb2);
+}
+
+// ----------------------------------------------- 122 transpose helpers
+// cbar = symmetrised cotangent: square blocks (X + h X^T)/2, rectangular
+// X/2, rows and columns of Pauli-forbidden pairs (a,a) at odd J zeroed
+__global__ __launch_bounds__(256) void k_to_dense(DevBasis B, const SegTask* tasks, const double* X, int herm,
+                                                  double* cbar) {
+  const SegTask t = tasks[blockIdx.x];
+  const i64 e = (i64)t.start + threadIdx.x;
+  if (e >= t.cnt) return;
+  const int q = t.q, g = t.g;
+  const int n1 = brows(B, q, g);
+  const int IX = (int)(e % n1), JX = (int)(e / n1);
+  const i64 base = B.goff[q * 6 + g];
+  double v;
+  if (is_square(g)) v = 0.5 * (X[base + e] + herm * X[base + JX + (i64)IX * n1]);
+  else v = 0.5 * X[base + e];
+  if ((B.chJ[q] >> 1) & 1) {
+    int a, b, c, d;
+    unpack(B.pairs[B.pairoff[q * 3 + row_cls(g)] + IX], a, b);
+    unpack(B.pairs[B.pairoff[q * 3 + col_cls(g)] + JX], c, d);
+    if (a == b || c == d) v = 0.0;
+  }
+  cbar[base + e] = v;
+}
+
+// dst += alpha * Y on square blocks, 2 alpha * Y on rectangular blocks
+__global__ __launch_bounds__(256) void k_from_dense(DevBasis B, const SegTask* tasks, const double* Y, double alpha,
+                                                    double* dst) {
+  const SegTask t = tasks[blockIdx.x];
+  const i64 e = (i64)t.start + threadIdx.x;
+  if (e >= t.cnt) return;
+  const i64 o = B.goff[t.q * 6 + t.g] + e;
+  dst[o] += (is_square(t.g) ? alpha : 2.0 * alpha) * Y[o];
+}
+
+void launch_to_dense(const DevBasis& B, const SegTask* t, int nt, const double* X, int herm, double* cbar,
+                     hipStream_t s) {
+  if (nt <= 0) return;
+  hipLaunchKernelGGL(k_to_dense, dim3(nt), dim3(256), 0, s, B, t, X, herm, cbar);
+}
+void launch_from_dense(const DevBasis& B, const SegTask* t, int nt, const double* Y, double alpha, double* dst,
+                       hipStream_t s) {
+  if (nt <= 0) return;
+  hipLaunchKernelGGL(k_from_dense, dim3(nt), dim3(256), 0, s, B, t, Y, alpha, dst);
+}
+
+// one-body part of the 122 transpose: the partner sums of k_comm122p with
+// the cotangent of each upper element; contributions to the few one-body
+// elements are summed per workgroup in LDS (compact index f1map[foff])
+__global__ __launch_bounds__(256) void k_122_vjp_1b(DevBasis B, const SegTask* tasks, int ntasks,
+                                                    const Partner* part, const int* rowp, const int* colp,
+                                                    const int* f1map, const int* f1off, int nf1, const double* L,
+                                                    int Lh, const double* R, int Rh, const double* RESb, int RESh,
+                                                    double* Lb, double* Rb) {
+  extern __shared__ double sh[];
+  double* lbar = sh;
+  double* rbar = sh + nf1;
+  for (int k = threadIdx.x; k < 2 * nf1; k += blockDim.x) sh[k] = 0.0;
+  __syncthreads();
+  for (int ti = blockIdx.x; ti < ntasks; ti += gridDim.x) {
+    const SegTask t = tasks[ti];
+    const i64 e = (i64)t.start + threadIdx.x;
+    if (e >= t.cnt) continue;
+    const int q = t.q, g = t.g;
+    const int rc = row_cls(g), cc = col_cls(g);
+    const int n1 = B.dims[q * 3 + rc];
+    int IX, JX;
+    if (is_square(g)) upper_decode(e, IX, JX);
+    else { IX = (int)(e % n1); JX = (int)(e / n1); }
+    const i64 gr = B.pairoff[q * 3 + rc] + IX, gc = B.pairoff[q * 3 + cc] + JX;
+    int a, b, c, d;
+    unpack(B.pairs[gr], a, b);
+    unpack(B.pairs[gc], c, d);
+    double sb = fold2(B, RESb, q, g, IX, JX, RESh);
+    sb = sb / sqrt(1.0 + (a == b ? 1.0 : 0.0)) / sqrt(1.0 + (c == d ? 1.0 : 0.0));
+    if (sb == 0.0) continue;
+    const double pre_ab = (a == b) ? 1.4142135623730951 : 1.0;
+    const double pre_cd = (c == d) ? 1.4142135623730951 : 1.0;
+    const i64* go = B.goff + q * 6;
+    const int* dm = B.dims + q * 3;
+    for (int x = rowp[gr]; x < rowp[gr + 1]; ++x) {
+      const Partner p = part[x];
+      const int idx = p.code & 0x3fffff, cl = (p.code >> 22) & 3;
+      double pre = ((p.code >> 26) & 1) ? 1.4142135623730951 : 1.0;
+      if ((p.code >> 25) & 1) pre = -pre;
+      pre = pre * pre_cd;
+      double vr, vl;
+      if (cl <= cc) {
+        const i64 o = go[cl == 0 ? cc : (cl == 1 ? (cc == 1 ? 3 : 5) : 4)] + idx + (i64)JX * dm[cl];
+        vr = R[o] * pre; vl = L[o] * pre;
+      } else {
+        const i64 o = go[cc == 0 ? cl : (cc == 1 ? (cl == 1 ? 3 : 5) : 4)] + JX + (i64)idx * dm[cc];
+        vr = R[o] * Rh * pre; vl = L[o] * Lh * pre;
+      }
+      const bool fs = (p.code >> 24) & 1;
+      const int k = f1map[p.foff];
+      unsafeAtomicAdd(lbar + k, sb * (fs ? Lh : 1) * vr);
+      unsafeAtomicAdd(rbar + k, -sb * (fs ? Rh : 1) * vl);
+    }
+    for (int x = colp[gc]; x < colp[gc + 1]; ++x) {
+      const Partner p = part[x];
+      const int idx = p.code & 0x3fffff, cl = (p.code >> 22) & 3;
+      double pre = ((p.code >> 26) & 1) ? 1.4142135623730951 : 1.0;
+      if ((p.code >> 25) & 1) pre = -pre;
+      pre = pre_ab * pre;
+      double vr, vl;
+      if (rc <= cl) {
+        const i64 o = go[rc == 0 ? cl : (rc == 1 ? (cl == 1 ? 3 : 5) : 4)] + IX + (i64)idx * dm[rc];
+        vr = R[o] * pre; vl = L[o] * pre;
+      } else {
+        const i64 o = go[cl == 0 ? rc : (cl == 1 ? (rc == 1 ? 3 : 5) : 4)] + idx + (i64)IX * dm[cl];
+        vr = R[o] * Rh * pre; vl = L[o] * Lh * pre;
+      }
+      const bool fs = (p.code >> 24) & 1;
+      const int k = f1map[p.foff];
+      unsafeAtomicAdd(lbar + k, -sb * (fs ? Lh : 1) * vr);
+      unsafeAtomicAdd(rbar + k, sb * (fs ? Rh : 1) * vl);
+    }
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < nf1; k += blockDim.x) {
+    if (lbar[k] != 0.0) aadd(Lb + f1off[k], lbar[k]);
+    if (rbar[k] != 0.0) aadd(Rb + f1off[k], rbar[k]);
+  }
+}
+
+void launch_122_vjp_1b(const DevBasis& B, const SegTask* t, int nt, const Partner* part, const int* rowp,
+                       const int* colp, const int* f1map, const int* f1off, int nf1, const double* L, int Lh,
+                       const double* R, int Rh, const double* RESb, int RESh, double* Lb, double* Rb, hipStream_t s) {
+  if (nt <= 0 || nf1 <= 0) return;
+  const int grid = nt < 2048 ? nt : 2048;
+  hipLaunchKernelGGL(k_122_vjp_1b, dim3(grid), dim3(256), 2 * nf1 * sizeof(double), s, B, t, nt, part, rowp, colp,
+                     f1map, f1off, nf1, L, Lh, R, Rh, RESb, RESh, Lb, Rb);
+}
+
+// -------------------------------------------------- 222_ph transposes
+// add c at the W' element the scatter reads as wprime(x, y): the upper
+// triangle holds W'; x > y reads -Lh Rh W'(y, x)
+__device__ __forceinline__ void wadj(double* U, i64 rr, int x, int y, double mpm, double c) {
+  if (x <= y) aadd(U + x + (i64)y * rr, c);
+  else aadd(U + y + (i64)x * rr, mpm * c);
+}
+
+// One thread per orbit quadruple (its J items are contiguous in `items`):
+// for every J' the contributions of all its J are summed before the
+// atomic adds into U (the cotangent of the upper triangle of W').
+__global__ __launch_bounds__(256) void k_ph_vjp(DevBasis B, const QuadItem* items, const int* qstart, int nquads,
+                                                const double* RESb, int RESh, int Lh, int Rh, double* U) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nquads) return;
+  const int i0 = qstart[t], i1 = qstart[t + 1];
+  const int ijkl = items[i0].ijkl;
+  const int i = ijkl & 255, j = (ijkl >> 8) & 255, k = (ijkl >> 16) & 255, l = (ijkl >> 24) & 255;
+  const int ji = B.jj[i], jj = B.jj[j], jk = B.jj[k], jl = B.jj[l];
+  const int li = B.ll[i], lj = B.ll[j], lk = B.ll[k], ll = B.ll[l];
+  const int ti = B.tz[i], tj = B.tz[j], tk = B.tz[k], tl = B.tz[l];
+  const int Tz1 = abs(ti - tk) / 2;
+  if (abs(tl - tj) != Tz1 * 2) return;
+  const int PAR1 = (li + lk) % 2;
+  if ((ll + lj) % 2 != PAR1) return;
+  const int Tz2 = abs(ti - tl) / 2;
+  if (abs(tk - tj) != Tz2 * 2) return;
+  const int PAR2 = (li + ll) % 2;
+  if ((lk + lj) % 2 != PAR2) return;
+  const int P = (li + lj) & 1;
+  const int rc = B.con[i] + B.con[j], cc = B.con[k] + B.con[l];
+  const int g = (rc == 0) ? cc : (rc == 1 ? (cc == 1 ? 3 : 5) : 4);
+  const double pre = (i == j) ? IMSRG_INV_SQRT2_REF : 1.0;
+  const double pre2 = (k == l) ? IMSRG_INV_SQRT2_REF : 1.0;
+  constexpr int kMaxJ = 64;
+  double sb[kMaxJ];
+  int nj = i1 - i0;
+  if (nj > kMaxJ) nj = kMaxJ;   // 2J <= 2*jmax keeps nj far below this
+  for (int x = 0; x < nj; ++x) {
+    const int J = items[i0 + x].J;
+    const int q = 3 * J + 2 * ((ti + tj) / 2 + 1) + P;
+    const int IX = B.xmap[((i64)i * B.N + j) * B.nJ + (J >> 1)];
+    const int JX = B.xmap[((i64)k * B.N + l) * B.nJ + (J >> 1)];
+    sb[x] = fold2(B, RESb, q, g, IX, JX, RESh) * pre * pre2;
+  }
+  const double mpm = -(double)(Lh * Rh);
+  const int phase = m1pow((ji + jj + jk + jl) / 2);
+  const int N = B.N, JTM = B.JTM;
+  int jmin = max(abs(jj - jl), abs(ji - jk));
+  int jmax = min(jj + jl, ji + jk);
+  for (int JP = jmin; JP <= jmax; JP += 2) {
+    double c = 0.0;
+    for (int x = 0; x < nj; ++x) {
+      const int J = items[i0 + x].J;
+      c -= m1pow((jl + jk + J) / 2) * sixj(B, ji, jj, J, jl, jk, JP) * sb[x];
+    }
+    if (c == 0.0) continue;
+    const int qx = JP / 2 + Tz1 * (JTM + 1) + 2 * PAR1 * (JTM + 1);
+    double* W = U + B.wcoff[qx];
+    const i64 rr = B.rlen[qx];
+    const int* rJ = B.rmapJ + (i64)(JP >> 1) * N * N;
+    const int* rT = B.rmapJT + (i64)(JP >> 1) * N * N;
+    const int rjl = rJ[j * N + l], rik = rT[k * N + i], gjl = rJ[l * N + j], gik = rJ[k * N + i];
+    wadj(W, rr, gik, rjl, mpm, c);
+    wadj(W, rr, gjl, rik, mpm, phase * c);
+  }
+  jmin = max(abs(ji - jl), abs(jj - jk));
+  jmax = min(ji + jl, jj + jk);
+  for (int JP = jmin; JP <= jmax; JP += 2) {
+    double c = 0.0;
+    for (int x = 0; x < nj; ++x) {
+      const int J = items[i0 + x].J;
+      c += sixj(B, ji, jj, J, jk, jl, JP) * sb[x];
+    }
+    if (c == 0.0) continue;
+    const int qx = JP / 2 + Tz2 * (JTM + 1) + 2 * PAR2 * (JTM + 1);
+    double* W = U + B.wcoff[qx];
+    const i64 rr = B.rlen[qx];
+    const int* rJ = B.rmapJ + (i64)(JP >> 1) * N * N;
+    const int* rT = B.rmapJT + (i64)(JP >> 1) * N * N;
+    const int ril = rT[l * N + i], rjk = rJ[j * N + k], gil = rJ[l * N + i], gjk = rJ[k * N + j];
+    wadj(W, rr, gil, rjk, mpm, c);
+    wadj(W, rr, gjk, ril, mpm, phase * c);
+  }
+}
+
+void launch_ph_vjp(const DevBasis& B, const QuadItem* items, const int* qstart, int nquads, const double* RESb,
+                   int RESh, int Lh, int Rh, double* U, hipStream_t s) {
+  if (nquads <= 0) return;
+  hipLaunchKernelGGL(k_ph_vjp, dim3((nquads + 255) / 256), dim3(256), 0, s, B, items, qstart, nquads, RESb, RESh,
+                     Lh, Rh, U);
+}
+
+// U (upper-triangle cotangent of W') -> M = Lh U - Rh U^T in place, so that
+// LCCb = M RCC and RCCb = M^T LCC.  One thread per upper element.
+__global__ __launch_bounds__(256) void k_ph_mform(DevBasis B, const CCTask* tasks, double* U, int Lh, int Rh) {
+  const CCTask t = tasks[blockIdx.x];
+  const int r = B.rlen[t.q1];
+  const i64 e = (i64)t.start + threadIdx.x;
+  if (e >= (i64)r * (r + 1) / 2) return;
+  int x, y;
+  upper_decode(e, x, y);
+  double* W = U + B.wcoff[t.q1];
+  const double u = W[x + (i64)y * r];
+  if (x == y) {
+    W[x + (i64)x * r] = (Lh - Rh) * u;
+  } else {
+    W[x + (i64)y * r] = Lh * u;
+    W[y + (i64)x * r] = -Rh * u;
+  }
+}
+
+void launch_ph_mform(const DevBasis& B, const CCTask* t, int nt, double* U, int Lh, int Rh, hipStream_t s) {
+  if (nt <= 0) return;
+  hipLaunchKernelGGL(k_ph_mform, dim3(nt), dim3(256), 0, s, B, t, U, Lh, Rh);
+}
+
+// Pandya transpose: CCb -> opb (calculate_cross_coupled read positions)
+__global__ __launch_bounds__(256) void k_pandya_vjp(DevBasis B, const CCTask* tasks, const double* CCb, int herm,
+                                                    double* opb) {
+  const CCTask t = tasks[blockIdx.x];
+  const int q1 = t.q1;
+  const int r = B.rlen[q1], nc = B.ncol[q1];
+  const i64 e = (i64)t.start + threadIdx.x;
+  if (e >= (i64)r * nc) return;
+  const int Rr = (int)(e % r), NB = (int)(e / r);
+  const double cb = CCb[B.ccoff[q1] + Rr + (i64)NB * r];
+  if (cb == 0.0) return;
+  int a, b, p, h;
+  unpack(B.ccrows[B.ccrowoff[q1] + Rr], a, b);
+  unpack(B.cccols[B.cccoloff[q1] + NB], p, h);
+  const int JC = B.ccJ[q1];
+  const int ja = B.jj[a], jb = B.jj[b], jp = B.jj[p], jh = B.jj[h];
+  const int la = B.ll[a], lb = B.ll[b], lp = B.ll[p], lh = B.ll[h];
+  const int ta = B.tz[a], tb = B.tz[b], tp = B.tz[p], th = B.tz[h];
+  if (!((((la + lh) & 1) == ((lb + lp) & 1)) && (ta + th == tb + tp))) return;
+  const double c = cb * m1pow((jh + jb + JC) / 2) * sqrt(JC + 1.0);
+  const int Jmin = max(abs(jp - jb), abs(ja - jh));
+  const int Jmax = min(jp + jb, ja + jh);
+  for (int JT = Jmin; JT <= Jmax; JT += 2)
+    vadj(vpos(B, h, a, p, b, JT), opb, herm, c * (double)(m1pow(JT / 2) * (JT + 1)) * sixj(B, jp, jh, JC, ja, jb, JT));
+}
+
+void launch_pandya_vjp(const DevBasis& B, const CCTask* t, int nt, const double* CCb, int herm, double* opb,
+                       hipStream_t s) {
+  if (nt <= 0) return;
+  hipLaunchKernelGGL(k_pandya_vjp, dim3(nt), dim3(kTile), 0, s, B, t, CCb, herm, opb);
+}
+
+// ------------------------------------------------- white / normal order
+__global__ __launch_bounds__(256) void k_white_vjp_1b(DevBasis B, const double* H, const double* ETAb, double* Hb) {
+  const i64 x = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= (i64)B.P * B.A) return;
+  const int a = (int)(x % B.P), i = (int)(x / B.P);
+  const int ak = B.parts[a], ik = B.holes[i];
+  if (B.jj[ik] != B.jj[ak] || B.ll[ik] != B.ll[ak] || B.tz[ik] != B.tz[ak]) return;
+  const double g = ETAb[B.off_fph + x];
+  if (g == 0.0) return;
+  const int ji = B.jj[ik];
+  double Eden = 0.0;
+  for (int JT = 0; JT <= 2 * ji; JT += 2) Eden = Eden - (JT + 1) * v_elem(B, H, 1, ak, ik, ak, ik, JT);
+  Eden = Eden / (ji + 1.0) / (ji + 1.0);
+  Eden = Eden + H[B.off_fpp + a + (i64)a * B.P] - H[B.off_fhh + i + (i64)i * B.A];
+  aadd(Hb + B.off_fph + x, g / Eden);
+  const double Eb = -g * H[B.off_fph + x] / (Eden * Eden);
+  aadd(Hb + B.off_fpp + a + (i64)a * B.P, Eb);
+  aadd(Hb + B.off_fhh + i + (i64)i * B.A, -Eb);
+  for (int JT = 0; JT <= 2 * ji; JT += 2)
+    vadj(vpos(B, ak, ik, ak, ik, JT), Hb, 1, -Eb * (JT + 1) / (ji + 1.0) / (ji + 1.0));
+}
+
+// two-body part; the cotangents of the monopoles go to Mb (N x N)
+__global__ __launch_bounds__(256) void k_white_vjp_2b(DevBasis B, const SegTask* tasks, const double* H,
+                                                      const double* M, const double* ETAb, double* Hb, double* Mb) {
+  const SegTask t = tasks[blockIdx.x];
+  const i64 e = (i64)t.start + threadIdx.x;
+  if (e >= t.cnt) return;
+  const int q = t.q;
+  const i64 o = B.goff[q * 6 + 2] + e;
+  const double g = ETAb[o];
+  if (g == 0.0) return;
+  const int np = B.dims[q * 3];
+  const int IX = (int)(e % np), JX = (int)(e / np);
+  int a, b, i, j;
+  unpack(B.pairs[B.pairoff[q * 3 + 0] + IX], a, b);
+  unpack(B.pairs[B.pairoff[q * 3 + 2] + JX], i, j);
+  const int N = B.N;
+  double Eden = M[a + b * N] + M[i + j * N] - M[a + i * N] - M[a + j * N] - M[i + b * N] - M[j + b * N];
+  Eden = Eden + f_elem(B, H, 1, a, a) + f_elem(B, H, 1, b, b) - f_elem(B, H, 1, i, i) - f_elem(B, H, 1, j, j);
+  aadd(Hb + o, g / Eden);
+  const double Eb = -g * H[o] / (Eden * Eden);
+  aadd(Mb + a + b * N, Eb);
+  aadd(Mb + i + j * N, Eb);
+  aadd(Mb + a + i * N, -Eb);
+  aadd(Mb + a + j * N, -Eb);
+  aadd(Mb + i + b * N, -Eb);
+  aadd(Mb + j + b * N, -Eb);
+  fadj(B, Hb, 1, a, a, Eb);
+  fadj(B, Hb, 1, b, b, Eb);
+  fadj(B, Hb, 1, i, i, -Eb);
+  fadj(B, Hb, 1, j, j, -Eb);
+}
+
+__global__ __launch_bounds__(256) void k_monopole_vjp(DevBasis B, const double* Mb, double* Hb) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= B.N * B.N) return;
+  const double m = Mb[idx];
+  if (m == 0.0) return;
+  const int a = idx % B.N, b = idx / B.N;
+  const int ja = B.jj[a], jb = B.jj[b];
+  for (int JT = abs(ja - jb); JT <= ja + jb; JT += 2)
+    vadj(vpos(B, a, b, a, b, JT), Hb, 1, m * (JT + 1) / (ja + 1.0) / (jb + 1.0));
+}
+
+void launch_white_vjp(const DevBasis& B, const SegTask* g3, int ng3, const double* H, const double* M,
+                      const double* ETAb, double* Hb, double* Mb, hipStream_t s) {
+  const i64 n = (i64)B.P * B.A;
+  if (n > 0)
+    hipLaunchKernelGGL(k_white_vjp_1b, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, B, H, ETAb, Hb);
+  if (ng3 > 0) hipLaunchKernelGGL(k_white_vjp_2b, dim3(ng3), dim3(256), 0, s, B, g3, H, M, ETAb, Hb, Mb);
+  hipLaunchKernelGGL(k_monopole_vjp, dim3((B.N * B.N + 255) / 256), dim3(256), 0, s, B, Mb, Hb);
+}
+
+// normal_order transpose.  Tb must hold a copy of Hb on entry.  Pass 1: the
+// one-body elements normal_order replaced (their T values do not reach H);
+// pass 2: E0.
+__global__ __launch_bounds__(256) void k_no_vjp_1b(DevBasis B, const double* Hb, double* Tb) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= B.N * B.N) return;
+  const int ax = idx % B.N, bx = idx / B.N;
+  if (B.jj[ax] != B.jj[bx] || B.ll[ax] != B.ll[bx] || B.tz[ax] != B.tz[bx]) return;
+  const int ca = B.con[ax], cb = B.con[bx];
+  if (ca == 1 && cb == 1 && B.cidx[ax] > B.cidx[bx]) return;
+  if (ca == 0 && cb == 0 && B.cidx[ax] > B.cidx[bx]) return;
+  if (ca == 1 && cb == 0) return;
+  const int a = B.cidx[ax], b = B.cidx[bx];
+  double vb;
+  if (ca == 1) {
+    const i64 o1 = B.off_fhh + a + (i64)b * B.A, o2 = B.off_fhh + b + (i64)a * B.A;
+    vb = Hb[o1] + (a != b ? Hb[o2] : 0.0);
+    Tb[o2] = 0.0;
+    Tb[o1] = vb;          // f_elem(ax, bx) of T reads fhh(a, b)
+  } else if (cb == 1) {
+    const i64 o1 = B.off_fph + a + (i64)b * B.P;
+    vb = Hb[o1];
+    Tb[o1] = vb;
+  } else {
+    const i64 o1 = B.off_fpp + a + (i64)b * B.P, o2 = B.off_fpp + b + (i64)a * B.P;
+    vb = Hb[o1] + (a != b ? Hb[o2] : 0.0);
+    Tb[o2] = 0.0;
+    Tb[o1] = vb;
+  }
+  if (vb == 0.0) return;
+  for (int ix = 0; ix < B.N; ++ix) {
+    if (B.con[ix] != 1) continue;
+    for (int JT = abs(B.jj[ix] - B.jj[ax]); JT <= B.jj[ix] + B.jj[ax]; JT += 2)
+      vadj(vpos(B, ax, ix, bx, ix, JT), Tb, 1, vb * (JT + 1.0) / (B.jj[ax] + 1));
+  }
+}
+
+__global__ __launch_bounds__(256) void k_no_vjp_e0(DevBasis B, const double* Hb, double* Tb) {
+  const double e0 = Hb[0];
+  if (threadIdx.x == 0 && blockIdx.x == 0) Tb[0] = 0.0;
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= B.N * B.N || e0 == 0.0) return;
+  const int i = idx % B.N, j = idx / B.N;
+  if (i == j && B.con[i] == 1) fadj(B, Tb, 1, i, i, e0 * (B.jj[i] + 1));
+  if (B.con[i] * B.con[j] == 0) return;
+  for (int JT = abs(B.jj[i] - B.jj[j]); JT <= B.jj[i] + B.jj[j]; JT += 2)
+    vadj(vpos(B, i, j, i, j, JT), Tb, 1, 0.5 * e0 * (JT + 1));
+}
+
+void launch_normal_order_vjp(const DevBasis& B, const double* Hb, double* Tb, hipStream_t s) {
+  const int n = B.N * B.N;
+  hipLaunchKernelGGL(k_no_vjp_1b, dim3((n + 255) / 256), dim3(256), 0, s, B, Hb, Tb);
+  hipLaunchKernelGGL(k_no_vjp_e0, dim3((n + 255) / 256), dim3(256), 0, s, B, Hb, Tb);
+}
+
+// dst[0] += factor * src[0]
+__global__ void k_add_e0(double* dst, const double* src, double factor) { dst[0] += factor * src[0]; }
+void launch_add_e0(double* dst, const double* src, double factor, hipStream_t s) {
+  hipLaunchKernelGGL(k_add_e0, dim3(1), dim3(1), 0, s, dst, src, factor);
+}
+
+}  // namespace imsrg

@@ -81,6 +81,12 @@ SIGNATURES = {
     "imsrg_magnus_step": (_i, [_p, _dp, _dp, _dp, _dp, C.c_double, C.c_double, C.c_double, _d, _p]),
     "imsrg_magnus_decouple": (_i, [_p, _dp, _dp, C.c_double, C.c_double, C.c_double, _i, C.c_double, _i, _d,
                                    _ip, _p]),
+    "imsrg_commutator_vjp": (_i, [_p, _dp, _i, _dp, _i, _dp, _i, _dp, _dp, _p]),
+    "imsrg_bch_vjp": (_i, [_p, _dp, _dp, _i, C.c_double, _i, _dp, _dp, _dp, _ip, _p]),
+    "imsrg_magnus_vjp": (_i, [_p, _dp, _dp, C.c_double, _dp, _dp, _dp, _ip, _p]),
+    "imsrg_white_vjp": (_i, [_p, _dp, _dp, _dp, _p]),
+    "imsrg_normal_order_vjp": (_i, [_p, _dp, _dp, _p]),
+    "imsrg_flow_grad": (_i, [_p, _dp, C.c_double, C.c_double, C.c_double, C.c_double, _i, _i, _dp, _d, _ip, _p]),
     "imsrg_set_gemm_tile": (_i, [_p, _i, _i]),
     "imsrg_get_gemm_tile": (_i, [_p, _i]),
     "imsrg_profile_kernel": (_i, [_p, _i, _dp, _i, _dp, _i, _dp, _i, _dp, _dp, _dp, _dp, _dp, _i, _d, _p]),

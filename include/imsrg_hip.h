@@ -176,6 +176,41 @@ int imsrg_magnus_decouple(imsrg_flow f, double* HS, double* G, double ds, double
                           int crit_mode, double crit_tol, int maxsteps, double* log, int* steps,
                           imsrg_stream stream);
 
+/* ------------------------------------------------ reverse mode */
+/* The reference has no derivatives.  These are hand-written transposes of
+ * the routines above (vector-Jacobian products) for dE0/d(input matrix
+ * elements) of the Magnus flow.  Gradients are w.r.t. the flat operator
+ * arrays and ACCUMULATE into the outputs (Lb, Rb, Gb, Hb, ETAb) unless
+ * stated; they are exact for perturbations that keep the operators'
+ * herm symmetry and the zero Pauli-forbidden (a,a) odd-J elements, which
+ * is every operator of the flow.  Asynchronous on `stream` except the
+ * drivers that need the reference's data-dependent exits (bch / magnus /
+ * flow), which synchronise like their forward counterparts. */
+
+/* transpose of one [L, R] (E0 = 110 + 220, then the BCH term order of
+ * IMSRG_MAGNUS.f90:207-219): Lb += d<RESb,[L,R]>/dL, Rb += d<RESb,[L,R]>/dR */
+int imsrg_commutator_vjp(imsrg_basis b, const double* L, int Lherm, const double* R, int Rherm, const double* RESb,
+                         int RESherm, double* Lb, double* Rb, imsrg_stream stream);
+/* transpose of BCH_EXPAND (IMSRG_MAGNUS.f90:149-234) with G antihermitian:
+ * Gb += d<HSb,HS>/dG, Hb += d<HSb,HS>/dH; nterms as imsrg_bch_expand */
+int imsrg_bch_vjp(imsrg_flow f, const double* G, const double* H, int Hherm, double conv, int maxit,
+                  const double* HSb, double* Gb, double* Hb, int* nterms, imsrg_stream stream);
+/* transpose of MAGNUS_EXPAND (IMSRG_MAGNUS.f90:620-691), eta and Omega
+ * antihermitian: ETAb += d<DGb,dOmega>/d eta, Gb += d<DGb,dOmega>/dOmega */
+int imsrg_magnus_vjp(imsrg_flow f, const double* ETA, const double* G, double conv, const double* DGb,
+                     double* ETAb, double* Gb, int* ncomm, imsrg_stream stream);
+/* transpose of build_gs_white (generators.f90:4-106): Hb += d<ETAb,eta(H)>/dH */
+int imsrg_white_vjp(imsrg_basis b, const double* H, const double* ETAb, double* Hb, imsrg_stream stream);
+/* transpose of normal_order (basic_IMSRG.f90:2419-2550): Tb = d<Hb,H(T)>/dT
+ * (overwritten) */
+int imsrg_normal_order_vjp(imsrg_basis b, const double* Hb, double* Tb, imsrg_stream stream);
+/* dE0/dT for magnus_decouple (IMSRG_MAGNUS.f90:15-128, white generator,
+ * |mbpt2| > crit_tol loop) started from H = normal_order(T): Tb receives the
+ * gradient (overwritten), E0 the flowed energy, steps the step count.  Omega
+ * is checkpointed every `ckpt` steps and the steps in between recomputed. */
+int imsrg_flow_grad(imsrg_flow f, const double* T, double ds, double bch_conv, double mag_conv, double crit_tol,
+                    int maxsteps, int ckpt, double* Tb, double* E0, int* steps, imsrg_stream stream);
+
 /* ------------------------------------------------ tuning */
 /* No counterpart in the reference: selects the MFMA tile shape of one GEMM
  * family (IMSRG_GEMM_*): tile 0 = 64x64, 1 = 128x64, 2 = 128x128, plus

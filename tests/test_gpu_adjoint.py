@@ -1,0 +1,154 @@
+"""Reverse mode on the GPU: the hand-written transposes (imsrg_*_vjp,
+imsrg_flow_grad) against the oracle's exact transposes on the same inputs,
+against directional derivatives of the GPU forward commutator (bilinear, so
+exact), and the flow gradient against central differences of the GPU flow.
+
+Gradients are compared after folding onto the independent (anti)hermitian
+matrix elements (adjoint.symmetric_gradient): the flow only ever perturbs
+operators along those directions.  Tolerance 1e-9 relative per element for
+single maps (atomic adds change the summation order), 1e-8 for the flow.
+"""
+import numpy as np
+import pytest
+import torch
+
+from conftest import assert_rel, basis, oracle_basis
+
+pytestmark = pytest.mark.gpu
+
+HERMS = [(-1, 1), (-1, -1), (1, 1), (1, -1)]
+REF = float(np.float32(1e-6))
+
+
+def _rand(b, h, seed, sigma=1.0):
+    from eom_imsrg import synth
+    return synth.hermitian_random(b, seed, sigma) if h == 1 else synth.antihermitian(b, seed, sigma)
+
+
+def _t(x):
+    return torch.from_numpy(np.ascontiguousarray(x, np.float64)).cuda()
+
+
+def _sym(b, g, h):
+    from eom_imsrg import adjoint
+    return adjoint.symmetric_gradient(b, np.asarray(g), h)
+
+
+@pytest.mark.parametrize("emax", [2, 4])
+@pytest.mark.parametrize("hL,hR", HERMS)
+def test_commutator_vjp(gpu, emax, hL, hR):
+    from eom_imsrg import SqOp, adjoint, commutators as cm
+    b, o = basis(emax), oracle_basis(emax)
+    print(f"[commutator_vjp] emax={emax} herm=({hL},{hR})", flush=True)
+    L, R = _rand(b, hL, 3), _rand(b, hR, 4)
+    hres = -hL * hR
+    x = np.random.default_rng(5).normal(size=b.oplen)
+    Lb_o, Rb_o = o.commutator_vjp(L, hL, R, hR, x, hres)
+    Ld, Rd = SqOp.from_numpy(b, L, hL), SqOp.from_numpy(b, R, hR)
+    Lb, Rb = torch.zeros(b.oplen, dtype=torch.float64, device="cuda"), torch.zeros(b.oplen, dtype=torch.float64,
+                                                                                     device="cuda")
+    adjoint.commutator_vjp(Ld, Rd, _t(x), hres, Lb, Rb)
+    torch.cuda.synchronize()
+    lb, rb = Lb.cpu().numpy(), Rb.cpu().numpy()
+    assert_rel(_sym(b, lb, hL), _sym(b, Lb_o, hL), rtol=1e-9, what="Lbar")
+    assert_rel(_sym(b, rb, hR), _sym(b, Rb_o, hR), rtol=1e-9, what="Rbar")
+    # directional derivatives of the GPU forward commutator
+    for s in range(2):
+        dL, dR = _rand(b, hL, 70 + s), _rand(b, hR, 80 + s)
+        RES = SqOp.zeros(b, hres)
+        cm.commutator(SqOp.from_numpy(b, dL, hL), Rd, RES)
+        r1 = x @ RES.numpy()
+        cm.commutator(Ld, SqOp.from_numpy(b, dR, hR), RES)
+        r2 = x @ RES.numpy()
+        assert abs(lb @ dL - r1) <= 1e-10 * (abs(r1) + np.abs(x).sum() * 1e-3), (lb @ dL, r1)
+        assert abs(rb @ dR - r2) <= 1e-10 * (abs(r2) + np.abs(x).sum() * 1e-3), (rb @ dR, r2)
+
+
+def test_bch_magnus_white_normal_order_vjp(gpu):
+    from eom_imsrg import Flow, SqOp, adjoint, synth
+    emax = 4
+    b, o = basis(emax), oracle_basis(emax)
+    print("[flow maps vjp] emax=4", flush=True)
+    T = synth.hamiltonian_vacuum(b, 0)
+    H = o.normal_order(T)
+    G = synth.antihermitian(b, 3, 0.05)
+    rng = np.random.default_rng(11)
+    f = Flow(b)
+    zeros = lambda: torch.zeros(b.oplen, dtype=torch.float64, device="cuda")
+    # BCH
+    HSb = rng.normal(size=b.oplen)
+    Gb_o, Hb_o, it_o = o.bch_vjp(G, H, 1e-10, HSb)
+    Gb, Hb = zeros(), zeros()
+    it = adjoint.bch_vjp(f, SqOp.from_numpy(b, G, -1), SqOp.from_numpy(b, H, 1), _t(HSb), Gb, Hb, conv=1e-10)
+    torch.cuda.synchronize()
+    assert it == it_o
+    assert_rel(_sym(b, Gb.cpu().numpy(), -1), _sym(b, Gb_o, -1), rtol=1e-9, what="BCH Gbar")
+    assert_rel(_sym(b, Hb.cpu().numpy(), 1), _sym(b, Hb_o, 1), rtol=1e-9, what="BCH Hbar")
+    # Magnus
+    eta = o.white(H)
+    G2 = synth.antihermitian(b, 6, 0.3)
+    DGb = rng.normal(size=b.oplen)
+    Eb_o, Gb2_o, k_o = o.magnus_vjp(eta, G2, 1e-12, DGb)
+    Eb, Gb2 = zeros(), zeros()
+    k = adjoint.magnus_vjp(f, SqOp.from_numpy(b, eta, -1), SqOp.from_numpy(b, G2, -1), _t(DGb), Eb, Gb2, conv=1e-12)
+    torch.cuda.synchronize()
+    assert k == k_o
+    assert_rel(_sym(b, Eb.cpu().numpy(), -1), _sym(b, Eb_o, -1), rtol=1e-9, what="Magnus etabar")
+    assert_rel(_sym(b, Gb2.cpu().numpy(), -1), _sym(b, Gb2_o, -1), rtol=1e-9, what="Magnus Gbar")
+    # white
+    ETAb = rng.normal(size=b.oplen)
+    Hb3 = zeros()
+    adjoint.white_vjp(SqOp.from_numpy(b, H, 1), _t(ETAb), Hb3)
+    torch.cuda.synchronize()
+    assert_rel(_sym(b, Hb3.cpu().numpy(), 1), _sym(b, o.white_vjp(H, ETAb), 1), rtol=1e-9, what="white Hbar")
+    # normal ordering
+    Hb4 = rng.normal(size=b.oplen)
+    Tb = zeros()
+    adjoint.normal_order_vjp(b, _t(Hb4), Tb)
+    torch.cuda.synchronize()
+    assert_rel(_sym(b, Tb.cpu().numpy(), 1), _sym(b, o.normal_order_vjp(Hb4), 1), rtol=1e-12, what="normal order")
+
+
+@pytest.mark.parametrize("emax", [2, 4])
+def test_flow_grad_vs_oracle(gpu, emax):
+    from eom_imsrg import Flow, adjoint, synth
+    b, o = basis(emax), oracle_basis(emax)
+    print(f"[flow_grad] emax={emax}", flush=True)
+    T = synth.hamiltonian_vacuum(b, 0)
+    e0_o, Tb_o, st_o = o.flow_grad(T, 0.5, REF, REF, REF, ckpt=4)
+    e0, Tb, st = adjoint.flow_grad(Flow(b, graphs=True), _t(T), ckpt=4)
+    assert st == st_o and abs(e0 - e0_o) < 1e-8
+    assert_rel(_sym(b, Tb.cpu().numpy(), 1), _sym(b, Tb_o, 1), rtol=1e-8, what="dE0/dT")
+
+
+def test_flow_energy_autograd_and_central_differences(gpu):
+    """torch.autograd.Function over the flow; central differences (h = 1e-5)
+    of the GPU flow on hermitian two-body matrix elements, eMax 4"""
+    from eom_imsrg import Flow, flow_energy, synth
+    from jscheme import each_element
+    b = basis(4)
+    print("[flow_energy] emax=4", flush=True)
+    f = Flow(b, graphs=True)
+    T0 = _t(synth.hamiltonian_vacuum(b, 0))
+    T = T0.clone().requires_grad_(True)
+    E = flow_energy(T, f, ckpt=4)
+    E.backward()
+    g = T.grad.cpu().numpy()
+    gs = _sym(b, g, 1)
+    h = 1e-5
+    rng = np.random.default_rng(3)
+    elems = [e for e in each_element(b) if e[1] in (0, 2, 3, 4)]
+    big = [e for e in elems if abs(gs[int(b.goff[e[0], e[1]]) + e[2] + e[3] * b.block_shape(e[0], e[1])[0]]) > 1e-3]
+    for k in rng.choice(len(big), 4, replace=False):
+        q, gg, ix, jx, _, _, _ = big[k]
+        n1, _ = b.block_shape(q, gg)
+        d = torch.zeros(b.oplen, dtype=torch.float64, device="cuda")
+        d[int(b.goff[q, gg]) + ix + jx * n1] = 1.0
+        if gg != 2:
+            d[int(b.goff[q, gg]) + jx + ix * n1] = 1.0
+        with torch.no_grad():
+            ep = flow_energy(T0 + h * d, f).item()
+            em = flow_energy(T0 - h * d, f).item()
+        fd = (ep - em) / (2 * h)
+        an = float(g @ d.cpu().numpy())
+        assert abs(fd - an) < 1e-6 * abs(fd) + 5e-8, (fd, an)
