@@ -35,6 +35,9 @@ def _parse():
     ap.add_argument("--emax", type=int, default=8)
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-grad", action="store_true")
+    ap.add_argument("--grad-check", type=int, default=20,
+                    help="central-difference check of dE0/dV on this many seeded two-body elements")
     ap.add_argument("--kernel-only", default=None,
                     help="launch one kernel class repeatedly and exit (for rocprofv3 counter runs)")
     return ap.parse_args()
@@ -122,6 +125,11 @@ def main() -> int:
         out_plain = {"plain_ms_per_step": round(p_ms, 3), "speedup_vs_plain": round(p_ms / g_ms, 3),
                      "dE0_grouped_vs_plain_MeV": dE0, "dE0_ok": bool(dE0 < 1e-8)}
 
+    # ---- gradient of the flowed E0 with respect to the vacuum two-body elements
+    grad = None
+    if not a.no_grad:
+        grad = gradient_bench(b, Hv, a.grad_check)
+
     # ---- roofline of the dominant kernel
     cand = {
         "PPPP": ("mfma", wl.pppp_flops(b) / 1e12, "TFLOP/s", FP64_MFMA_PEAK_TFLOPS, wl.pppp_bytes(b)),
@@ -156,12 +164,104 @@ def main() -> int:
         "commutator_ms": {"grouped": round(kms["COMMUTATOR"], 4), "plain": round(comm_plain_ms, 4)},
         "commutator_tflops": round(F / (kms["COMMUTATOR"] / 1e3) / 1e12, 3),
         "kernel_ms": {k: round(v, 4) for k, v in kms.items()},
-        "grad_over_fwd": None,
+        "grad_over_fwd": grad["grad_over_fwd"] if grad else None,
+        "gradient": grad,
         "roofline": roof,
         "cpu_baseline": cpu,
     }
     print(json.dumps(res))
     return 0
+
+
+def gradient_bench(b, Hv, ncheck: int):
+    """dE0/dV of the full flow (|mbpt2| < 1e-6, Omega checkpointed every 4
+    steps): time of a plain forward flow, of the checkpointed forward plus the
+    reverse sweep, and a central-difference check (h = 1e-5 MeV) on `ncheck`
+    seeded hermitian two-body elements of the vacuum interaction"""
+    import numpy as np
+    import torch
+    from eom_imsrg import Flow, SqOp, adjoint, commutators as cm
+
+    T = torch.from_numpy(Hv).cuda()
+    f = Flow(b, graphs=True)
+
+    def plain_flow(Tv):
+        H = SqOp(b, Tv.clone(), 1)
+        cm.normal_order(H)
+        G = SqOp.zeros(b, -1)
+        log = f.magnus_decouple(H, G, ds=0.5)
+        return H.E0, len(log)
+
+    plain_flow(T)                                    # graph capture
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e0, steps = plain_flow(T)
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+    Tb = torch.empty_like(T)
+    adjoint.flow_forward(f, T, ckpt=4)
+    adjoint.flow_backward(f, Tb)                     # warm-up of the reverse graphs / buffers
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e0g, st = adjoint.flow_forward(f, T, ckpt=4)
+    torch.cuda.synchronize()
+    t_fw = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    adjoint.flow_backward(f, Tb)
+    torch.cuda.synchronize()
+    t_bw = time.perf_counter() - t0
+    g = Tb.cpu().numpy()
+    gs = adjoint.symmetric_gradient(b, g, 1)
+    out = {"steps": steps, "E0": e0, "forward_flow_s": round(t_fwd, 4),
+           "checkpointed_forward_s": round(t_fw, 4), "reverse_sweep_s": round(t_bw, 4),
+           "grad_over_fwd": round((t_fw + t_bw) / t_fwd, 3), "reverse_over_fwd": round(t_bw / t_fwd, 3),
+           "checkpoint_every": 4, "E0_forward_vs_gradient_run": abs(e0 - e0g)}
+    if ncheck > 0:
+        # elements whose derivative stands clear of the difference quotient's
+        # noise floor: the flowed E0 (~700 MeV) carries ~1e-12 MeV of
+        # rounding, so (E+ - E-)/2h is good to about 1e-7 absolute
+        rng = np.random.default_rng(2024)
+        two0 = b.off_fpp + b.npart * b.npart
+        cand = np.nonzero(np.abs(gs[two0:]) > 1e-2)[0] + two0
+        pick = rng.choice(cand, size=min(ncheck, len(cand)), replace=False)
+        mirror = mirror_index(b)
+        h = 1e-5
+        worst_rel, worst_abs, ok = 0.0, 0.0, True
+        for x in pick:
+            d = torch.zeros_like(T)
+            d[int(x)] = 1.0
+            if mirror[x] >= 0:
+                d[int(mirror[x])] = 1.0
+            ep, _ = plain_flow(T + h * d)
+            em, _ = plain_flow(T - h * d)
+            fd = (ep - em) / (2 * h)
+            an = float(gs[x])
+            ok = ok and abs(fd - an) <= 1e-6 * abs(an) + 2e-7
+            worst_rel = max(worst_rel, abs(fd - an) / abs(an))
+            worst_abs = max(worst_abs, abs(fd - an))
+        out["fd_check"] = {"elements": int(len(pick)), "candidates_abs_grad_gt_1e-2": int(len(cand)),
+                           "h_MeV": h, "max_rel_err": worst_rel, "max_abs_err": worst_abs,
+                           "criterion": "|fd - grad| <= 1e-6 |grad| + 2e-7", "pass": bool(ok)}
+    return out
+
+
+def mirror_index(b):
+    """flat index of the hermitian partner of every two-body element of a
+    square block (-1 elsewhere or on the diagonal)"""
+    import numpy as np
+    m = -np.ones(b.oplen, np.int64)
+    for q in range(b.nblocks):
+        for g in (0, 3, 4):
+            n1, _ = b.block_shape(q, g)
+            if n1 == 0:
+                continue
+            o = int(b.goff[q, g])
+            ix, jx = np.meshgrid(np.arange(n1), np.arange(n1), indexing="ij")
+            src = o + ix + jx * n1
+            dst = o + jx + ix * n1
+            sel = ix != jx
+            m[src[sel]] = dst[sel]
+    return m
 
 
 def measured_traffic(kernel: str, emax: int):

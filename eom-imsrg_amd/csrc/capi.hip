@@ -88,11 +88,10 @@ struct imsrg_basis_s {
   double* dval = nullptr;    // device scalars
   double* hval = nullptr;    // pinned host scalars
   // reverse mode, built on first use: 222_pp_hh transposed products in
-  // rounds of distinct outputs, the W' transpose products, upper-triangle
-  // tiles of WCC, quadruple starts, the compact one-body map of 122
+  // rounds of distinct outputs, the W' transpose products, quadruple
+  // starts, the compact one-body map of 122
   bool adj_ready = false;
   GemmFamily pb[6], phb;
-  TaskList<CCTask> ccu;
   std::vector<int> qstart_h, f1map_h, f1off_h;
   int *qstart = nullptr, *f1map = nullptr, *f1off = nullptr;
   int nquads = 0, nf1 = 0;
@@ -722,31 +721,28 @@ int ensure_adj(imsrg_basis_s* B) {
     (void)ccls;
     for (GemmFamily& f : B->pb) f.group_end.push_back((int)f.probs_h.size());
   }
-  // W' transpose: LCCb += M RCC, RCCb += M^T LCC per cross-coupled block;
-  // upper-triangle tiles of every WCC block
+  // W' transpose per cross-coupled block, U the upper-triangular cotangent
+  // of W' = Lh LCC RCC^T - Rh RCC LCC^T:
+  //   LCCb += Lh U RCC - Rh U^T RCC,   RCCb += Lh U^T LCC - Rh U LCC
+  // (alpha = Lh and s2 = -Rh Lh at launch; triangular A limits each tile's K)
   B->phb.probs_h.clear(); B->phb.nks.clear(); B->phb.group_end.clear(); B->phb.tile = 0;
-  B->ccu.host.clear(); B->ccu.chan_begin.assign(1, 0);
   for (int q = 0; q < b.ncc; ++q) {
     const int r = b.rlen[q], nb = b.ncol[q];
-    const i64 cnt = (i64)r * (r + 1) / 2;
-    for (i64 st = 0; st < cnt; st += kTile) B->ccu.host.push_back(CCTask{q, (int)st});
-    B->ccu.chan_begin.push_back((int)B->ccu.host.size());
     if (r > 0 && nb > 0) {
       for (int side = 0; side < 2; ++side) {
         GemmProb p{};
-        p.a1 = b.wcoff[q]; p.b1 = b.ccoff[q]; p.c = b.ccoff[q];
+        p.a1 = b.wcoff[q]; p.b1 = b.ccoff[q]; p.a2 = b.wcoff[q]; p.b2 = b.ccoff[q]; p.c = b.ccoff[q];
         p.m = r; p.n = nb; p.k = r;
         p.lda = r; p.ldb = r; p.ldc = r;
-        p.nprod = 1; p.ta1 = (unsigned char)side; p.tb1 = 0;
-        p.sa1 = 4; p.sb1 = side ? 0 : 1; p.sc = side ? 3 : 2;
-        p.alpha = 1.0; p.beta = 1.0; p.mode = 4;
+        p.nprod = 2; p.ta1 = (unsigned char)side; p.tb1 = 0; p.ta2 = (unsigned char)(1 - side); p.tb2 = 0;
+        p.sa1 = 4; p.sb1 = side ? 0 : 1; p.sa2 = 4; p.sb2 = side ? 0 : 1; p.sc = side ? 3 : 2;
+        p.alpha = 1.0; p.s2 = 1.0; p.beta = 1.0; p.mode = 1 | 4 | 8 | 16;
         B->phb.probs_h.push_back(p);
         B->phb.nks.push_back(1);
       }
     }
     B->phb.group_end.push_back((int)B->phb.probs_h.size());
   }
-  B->ccu.n = (int)B->ccu.host.size();
   // quadruple starts of the 222_ph item list
   B->qstart_h.clear();
   for (size_t x = 0; x < B->quad_h.size(); ++x)
@@ -764,7 +760,6 @@ int ensure_adj(imsrg_basis_s* B) {
     if ((rc = upload(B, f->probs_h, &f->probs))) return rc;
     if ((rc = family_tasks(B, *f, 0))) return rc;
   }
-  if ((rc = upload(B, B->ccu.host, &B->ccu.dev))) return rc;
   if ((rc = upload(B, B->qstart_h, &B->qstart))) return rc;
   if ((rc = upload(B, B->f1map_h, &B->f1map))) return rc;
   if ((rc = upload(B, B->f1off_h, &B->f1off))) return rc;
@@ -802,18 +797,16 @@ void comm_vjp(imsrg_basis_s* B, const double* L, int Lh, const double* R, int Rh
   for (GemmFamily& f : B->pb)
     if (f.tasks.n) launch_gemm_tiled(f.tile, sp, f.probs, f.tasks.dev, f.tasks.grid, f.tasks.xmap, s, (double)Lh,
                                      (double)Rh);
-  // 122: two-body parts through the forward kernel on the symmetrised
-  // cotangent, one-body part by the partner sums
+  // 122: two-body parts from the symmetrised cotangent, one-body part by
+  // the partner sums
   launch_fill(B->cbar, b.off_fpp + (i64)b.P * b.P, 0.0, s);
   launch_to_dense(D, B->seg_full.dev, B->seg_full.n, RESb, RESh, B->cbar, s);
-  launch_comm122p(D, B->seg_tri.dev, B->seg_tri.n, B->partners, B->rowp, B->colp, R, Rh, B->cbar, RESh, B->Y, Lh, s);
-  launch_from_dense(D, B->seg_full.dev, B->seg_full.n, B->Y, -(double)Rh, Lb, s);
-  launch_comm122p(D, B->seg_tri.dev, B->seg_tri.n, B->partners, B->rowp, B->colp, L, Lh, B->cbar, RESh, B->Y, Rh, s);
-  launch_from_dense(D, B->seg_full.dev, B->seg_full.n, B->Y, (double)Lh, Rb, s);
+  launch_122_vjp_2b(D, B->seg_tri.dev, B->seg_tri.n, B->partners, B->rowp, B->colp, L, Lh, R, Rh, B->cbar, RESh, Lb,
+                    Rb, s);
   launch_122_vjp_1b(D, B->seg_tri.dev, B->seg_tri.n, B->partners, B->rowp, B->colp, B->f1map, B->f1off, B->nf1, L,
                     Lh, R, Rh, RESb, RESh, Lb, Rb, s);
-  // 222_ph: scatter transpose into the W' cotangent, M = Lh U - Rh U^T,
-  // cross-coupled cotangents, Pandya transposes
+  // 222_ph: scatter transpose into the W' cotangent U, cross-coupled
+  // cotangents (triangular-U GEMMs), Pandya transposes
   launch_pandya(D, B->cc.dev, B->cc.n, R, Rh, B->aRCC, s);
   const double* LCC = LCCin;
   if (!LCC) {
@@ -822,15 +815,14 @@ void comm_vjp(imsrg_basis_s* B, const double* L, int Lh, const double* R, int Rh
   }
   launch_fill(B->U, b.wclen, 0.0, s);
   launch_ph_vjp(D, B->quad, B->qstart, B->nquads, RESb, RESh, Lh, Rh, B->U, s);
-  launch_ph_mform(D, B->ccu.dev, B->ccu.n, B->U, Lh, Rh, s);
   double* LCCb = LCCb_acc ? LCCb_acc : B->aLCCb;
   if (!LCCb_acc) launch_fill(B->aLCCb, b.cclen, 0.0, s);
   launch_fill(B->aRCCb, b.cclen, 0.0, s);
   Slots sc{};
   sc.p[0] = LCC; sc.p[1] = B->aRCC; sc.p[2] = LCCb; sc.p[3] = B->aRCCb; sc.p[4] = B->U;
   if (B->phb.tasks.n)
-    launch_gemm_tiled(B->phb.tile, sc, B->phb.probs, B->phb.tasks.dev, B->phb.tasks.grid, B->phb.tasks.xmap, s, 1.0,
-                      1.0);
+    launch_gemm_tiled(B->phb.tile, sc, B->phb.probs, B->phb.tasks.dev, B->phb.tasks.grid, B->phb.tasks.xmap, s,
+                      (double)Lh, -(double)(Rh * Lh));
   launch_pandya_vjp(D, B->cc.dev, B->cc.n, B->aRCCb, Rh, Rb, s);
   if (!LCCb_acc) launch_pandya_vjp(D, B->cc.dev, B->cc.n, B->aLCCb, Lh, Lb, s);
 }
@@ -858,10 +850,13 @@ struct imsrg_flow_s {
   // reverse mode: nested-commutator terms of one BCH / Magnus expansion,
   // cotangent ping-pong, Omega's cross-coupled cotangent, and the flow
   // gradient's checkpoints and segment states
-  std::vector<double*> terms;
-  double *Ab = nullptr, *Prev = nullptr, *GCCb = nullptr;
+  std::vector<double*> pool;   // recorded nested-commutator terms
+  double *Ab = nullptr, *Prev = nullptr, *GCCb = nullptr, *mDG = nullptr;
   std::vector<double*> ckpts, segG, segH, segE;
   double* decDG = nullptr;   // generator buffer of imsrg_magnus_decouple
+  bool fw_ok = false;        // a checkpointed forward sweep is held
+  int fw_steps = 0, fw_ckpt = 1;
+  double fw_ds = 0.5, fw_bch = 1e-6, fw_mag = 1e-6;
   double *gH = nullptr, *gHS = nullptr, *gG = nullptr, *gDG = nullptr, *gHSb = nullptr, *gGb = nullptr,
          *gHb = nullptr, *gdOb = nullptr, *getab = nullptr;
   ~imsrg_flow_s() {
@@ -1033,15 +1028,6 @@ int fbuf(imsrg_flow_s* F, double** slot, i64 n) {
   if (*slot) return 0;
   return falloc(F, n, slot);
 }
-int term_buf(imsrg_flow_s* F, size_t k, double** out) {
-  while (F->terms.size() <= k) {
-    double* p = nullptr;
-    if (int rc = falloc(F, F->B->b.oplen, &p)) return rc;
-    F->terms.push_back(p);
-  }
-  *out = F->terms[k];
-  return 0;
-}
 int grow(imsrg_flow_s* F, std::vector<double*>& v, size_t k) {
   while (v.size() <= k) {
     double* p = nullptr;
@@ -1051,108 +1037,177 @@ int grow(imsrg_flow_s* F, std::vector<double*>& v, size_t k) {
   return 0;
 }
 
-// BCH_EXPAND transpose (IMSRG_MAGNUS.f90:149-234): Gb += d<HSb, HS>/dG,
-// Hb += d<HSb, HS>/dH for HS = BCH(G, H).  The nested commutators are
-// recomputed with the same exit, kept, and swept in reverse.
-int bch_vjp(imsrg_flow_s* F, const double* G, const double* H, int Hh, double conv, int maxit, const double* HSb,
-            double* Gb, double* Hb, int* nterms, hipStream_t s) {
+// Nested-commutator terms of one BCH / Magnus expansion kept for its
+// transpose: A[0] is the expanded operator, A[k] = [Omega, A[k-1]], c[k] the
+// coefficient of A[k] in the sum.
+struct TermRec {
+  std::vector<const double*> A;
+  std::vector<double> c;
+};
+
+int pool_get(imsrg_flow_s* F, size_t& idx, double** out) {
+  while (F->pool.size() <= idx) {
+    double* p = nullptr;
+    if (int rc = falloc(F, F->B->b.oplen, &p)) return rc;
+    F->pool.push_back(p);
+  }
+  *out = F->pool[idx++];
+  return 0;
+}
+
+// BCH_EXPAND (IMSRG_MAGNUS.f90:149-234) keeping its terms; HS (optional)
+// receives the sum.  Same kernels and order as bch(), so HS is identical.
+int bch_record(imsrg_flow_s* F, const double* G, const double* H, int Hh, double conv, int maxit, TermRec& rec,
+               size_t& pidx, double* HS, int* nterms, hipStream_t s) {
   imsrg_basis_s* B = F->B;
   const i64 n = B->b.oplen;
   int rc;
-  if ((rc = ensure_adj(B))) return rc;
-  if ((rc = fbuf(F, &F->Ab, n)) || (rc = fbuf(F, &F->Prev, n)) || (rc = fbuf(F, &F->GCCb, B->b.cclen))) return rc;
-  double* A0 = nullptr;
-  if ((rc = term_buf(F, 0, &A0))) return rc;
-  HIPCHK(hipMemcpyAsync(A0, H, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+  rec.A.assign(1, H);
+  rec.c.assign(1, 1.0);
+  if (HS) HIPCHK(hipMemcpyAsync(HS, H, n * sizeof(double), hipMemcpyDeviceToDevice, s));
   t_pandya(B, G, -1, F->GCC, s);
-  std::vector<double> coefs(1, 1.0);
   double coef = 1.0;
   int iw;
   for (iw = 2; iw <= maxit; ++iw) {
     coef = coef / (iw - 1.0);
     double* nx = nullptr;
-    if ((rc = term_buf(F, (size_t)iw - 1, &nx))) return rc;
-    t_full(B, G, -1, F->terms[iw - 2], Hh, nx, Hh, true, F->w1, F->w2, F->GCC, F->ADCC, F->WCC, s);
+    if ((rc = pool_get(F, pidx, &nx))) return rc;
+    t_full(B, G, -1, rec.A.back(), Hh, nx, Hh, true, F->w1, F->w2, F->GCC, F->ADCC, F->WCC, s);
+    if (HS) launch_axpy(n, HS, 1.0, nx, coef, HS, s);
     t_norm(B, nx, B->dval, s);
     double nrm = 0.0;
     if ((rc = read_scalar(B, B->dval, &nrm, s))) return rc;
-    coefs.push_back(coef);
+    rec.A.push_back(nx);
+    rec.c.push_back(coef);
     if (nrm * coef < conv) break;
   }
   if (nterms) *nterms = iw;
   if (iw > maxit) return fail(IMSRG_ECLIP, "clipping BCH");
-  const int K = (int)coefs.size() - 1;
-  launch_axpy(n, HSb, coefs[K], HSb, 0.0, F->Ab, s);
-  launch_fill(F->GCCb, B->b.cclen, 0.0, s);
+  return 0;
+}
+
+// transpose of a recorded BCH_EXPAND: Gb += d<HSb,HS>/dG, Hb += d<HSb,HS>/dH
+int bch_reverse(imsrg_flow_s* F, const double* G, int Hh, const TermRec& rec, const double* HSb, double* Gb,
+                double* Hb, hipStream_t s) {
+  imsrg_basis_s* B = F->B;
+  const i64 n = B->b.oplen;
+  int rc;
+  if ((rc = fbuf(F, &F->Ab, n)) || (rc = fbuf(F, &F->Prev, n)) || (rc = fbuf(F, &F->GCCb, B->b.cclen))) return rc;
+  const int K = (int)rec.A.size() - 1;
+  launch_axpy(n, HSb, rec.c[K], HSb, 0.0, F->Ab, s);
+  if (K > 0) {
+    t_pandya(B, G, -1, F->GCC, s);
+    launch_fill(F->GCCb, B->b.cclen, 0.0, s);
+  }
   for (int k = K; k >= 1; --k) {
     launch_fill(F->Prev, n, 0.0, s);
-    comm_vjp(B, G, -1, F->terms[k - 1], Hh, F->Ab, Hh, Gb, F->Prev, true, F->GCC, F->GCCb, s);
-    launch_axpy(n, F->Prev, 1.0, HSb, coefs[k - 1], F->Ab, s);
+    comm_vjp(B, G, -1, rec.A[k - 1], Hh, F->Ab, Hh, Gb, F->Prev, true, F->GCC, F->GCCb, s);
+    launch_axpy(n, F->Prev, 1.0, HSb, rec.c[k - 1], F->Ab, s);
   }
   launch_axpy(n, Hb, 1.0, F->Ab, 1.0, Hb, s);
-  launch_pandya_vjp(B->D, B->cc.dev, B->cc.n, F->GCCb, -1, Gb, s);
+  if (K > 0) launch_pandya_vjp(B->D, B->cc.dev, B->cc.n, F->GCCb, -1, Gb, s);
   HIPCHK(hipGetLastError());
   return 0;
 }
 
-// MAGNUS_EXPAND transpose (IMSRG_MAGNUS.f90:620-691): ETAb += d<DGb, dOmega>/d eta,
-// Gb += d<DGb, dOmega>/dOmega (eta, Omega, dOmega antihermitian)
-int magnus_vjp(imsrg_flow_s* F, const double* ETA, const double* G, double conv, const double* DGb, double* ETAb,
-               double* Gb, int* ncomm, hipStream_t s) {
+// MAGNUS_EXPAND (IMSRG_MAGNUS.f90:620-691) keeping its terms: DG holds eta
+// on entry (kept unchanged in rec.A[0] = ETA) and receives dOmega.  Same
+// kernels, exits and order as magnus().
+int magnus_record(imsrg_flow_s* F, const double* ETA, double* DG, const double* G, double conv, TermRec& rec,
+                  size_t& pidx, int* ncomm, hipStream_t s) {
   imsrg_basis_s* B = F->B;
   const i64 n = B->b.oplen;
   const double cof[7] = {1.0, -0.5, .0833333333, 0.0, -0.00138888888, 0.0, 3.306878e-5};
   double advals[7] = {0, 0, 0, 0, 0, 0, 0};
   int rc;
-  if ((rc = ensure_adj(B))) return rc;
-  if ((rc = fbuf(F, &F->Ab, n)) || (rc = fbuf(F, &F->Prev, n)) || (rc = fbuf(F, &F->GCCb, B->b.cclen))) return rc;
-  double* A0 = nullptr;
-  if ((rc = term_buf(F, 0, &A0))) return rc;
-  HIPCHK(hipMemcpyAsync(A0, ETA, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+  rec.A.assign(1, ETA);
+  rec.c.assign(1, 1.0);
+  if (DG != ETA) HIPCHK(hipMemcpyAsync(DG, ETA, n * sizeof(double), hipMemcpyDeviceToDevice, s));
   t_norm(B, ETA, B->dval, s);
   t_norm(B, G, B->dval + 1, s);
   HIPCHK(hipMemcpyAsync(B->hval, B->dval, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
   advals[0] = B->hval[0];
   const double fullnorm = B->hval[1];
-  std::vector<double> cs(1, 1.0);
-  if (!(fullnorm < (double)1e-9f)) {
-    t_pandya(B, G, -1, F->GCC, s);
-    int q = 1;
-    for (int i = 2; i <= 7; ++i) {
-      const double adnorm = advals[i - q - 1];
-      q = (std::fabs(cof[i - 1]) > (double)1e-6f) ? 1 : 2;
-      if (std::fabs(adnorm / fullnorm) < conv) break;
-      const size_t K = cs.size() - 1;
-      double* nx = nullptr;
-      if ((rc = term_buf(F, K + 1, &nx))) return rc;
-      HIPCHK(hipMemcpyAsync(nx, F->terms[K], sizeof(double), hipMemcpyDeviceToDevice, s));
-      t_full(B, G, -1, F->terms[K], -1, nx, -1, false, F->w1, F->w2, F->GCC, F->ADCC, F->WCC, s);
-      t_norm(B, nx, B->dval, s);
-      double nrm = 0.0;
-      if ((rc = read_scalar(B, B->dval, &nrm, s))) return rc;
-      advals[i - 1] = nrm * std::fabs(cof[i - 1]);
-      cs.push_back(cof[i - 1]);
-    }
+  if (ncomm) *ncomm = 0;
+  if (fullnorm < (double)1e-9f) return 0;
+  t_pandya(B, G, -1, F->GCC, s);
+  int q = 1;
+  for (int i = 2; i <= 7; ++i) {
+    const double adnorm = advals[i - q - 1];
+    q = (std::fabs(cof[i - 1]) > (double)1e-6f) ? 1 : 2;
+    if (std::fabs(adnorm / fullnorm) < conv) break;
+    double* nx = nullptr;
+    if ((rc = pool_get(F, pidx, &nx))) return rc;
+    HIPCHK(hipMemcpyAsync(nx, rec.A.back(), sizeof(double), hipMemcpyDeviceToDevice, s));
+    t_full(B, G, -1, rec.A.back(), -1, nx, -1, false, F->w1, F->w2, F->GCC, F->ADCC, F->WCC, s);
+    launch_axpy(n, DG, 1.0, nx, cof[i - 1], DG, s);
+    t_norm(B, nx, B->dval, s);
+    double nrm = 0.0;
+    if ((rc = read_scalar(B, B->dval, &nrm, s))) return rc;
+    advals[i - 1] = nrm * std::fabs(cof[i - 1]);
+    rec.A.push_back(nx);
+    rec.c.push_back(cof[i - 1]);
   }
-  const int K = (int)cs.size() - 1;
+  if (ncomm) *ncomm = (int)rec.A.size() - 1;
+  return 0;
+}
+
+// transpose of a recorded MAGNUS_EXPAND: ETAb += d<DGb,dOmega>/d eta,
+// Gb += d<DGb,dOmega>/dOmega (E0 of every term is eta's E0)
+int magnus_reverse(imsrg_flow_s* F, const double* G, const TermRec& rec, const double* DGb, double* ETAb, double* Gb,
+                   hipStream_t s) {
+  imsrg_basis_s* B = F->B;
+  const i64 n = B->b.oplen;
+  int rc;
+  if ((rc = fbuf(F, &F->Ab, n)) || (rc = fbuf(F, &F->Prev, n)) || (rc = fbuf(F, &F->GCCb, B->b.cclen))) return rc;
+  const int K = (int)rec.A.size() - 1;
   double e0sum = 0.0;
-  for (double c : cs) e0sum += c;
-  launch_axpy(n, DGb, cs[K], DGb, 0.0, F->Ab, s);
+  for (double c : rec.c) e0sum += c;
+  launch_axpy(n, DGb, rec.c[K], DGb, 0.0, F->Ab, s);
   launch_fill(F->Ab, 1, 0.0, s);
-  if (K > 0) launch_fill(F->GCCb, B->b.cclen, 0.0, s);
+  if (K > 0) {
+    t_pandya(B, G, -1, F->GCC, s);
+    launch_fill(F->GCCb, B->b.cclen, 0.0, s);
+  }
   for (int k = K; k >= 1; --k) {
     launch_fill(F->Prev, n, 0.0, s);
-    comm_vjp(B, G, -1, F->terms[k - 1], -1, F->Ab, -1, Gb, F->Prev, false, F->GCC, F->GCCb, s);
-    launch_axpy(n, F->Prev, 1.0, DGb, cs[k - 1], F->Ab, s);
+    comm_vjp(B, G, -1, rec.A[k - 1], -1, F->Ab, -1, Gb, F->Prev, false, F->GCC, F->GCCb, s);
+    launch_axpy(n, F->Prev, 1.0, DGb, rec.c[k - 1], F->Ab, s);
     launch_fill(F->Ab, 1, 0.0, s);
   }
   launch_axpy(n, ETAb, 1.0, F->Ab, 1.0, ETAb, s);
   launch_add_e0(ETAb, DGb, e0sum, s);
   if (K > 0) launch_pandya_vjp(B->D, B->cc.dev, B->cc.n, F->GCCb, -1, Gb, s);
-  if (ncomm) *ncomm = K;
   HIPCHK(hipGetLastError());
   return 0;
+}
+
+// BCH_EXPAND transpose (IMSRG_MAGNUS.f90:149-234): Gb += d<HSb, HS>/dG,
+// Hb += d<HSb, HS>/dH for HS = BCH(G, H), G antihermitian
+int bch_vjp(imsrg_flow_s* F, const double* G, const double* H, int Hh, double conv, int maxit, const double* HSb,
+            double* Gb, double* Hb, int* nterms, hipStream_t s) {
+  int rc;
+  if ((rc = ensure_adj(F->B))) return rc;
+  TermRec rec;
+  size_t pidx = 0;
+  if ((rc = bch_record(F, G, H, Hh, conv, maxit, rec, pidx, nullptr, nterms, s))) return rc;
+  return bch_reverse(F, G, Hh, rec, HSb, Gb, Hb, s);
+}
+
+// MAGNUS_EXPAND transpose (IMSRG_MAGNUS.f90:620-691), eta and Omega
+// antihermitian: ETAb += d<DGb, dOmega>/d eta, Gb += d<DGb, dOmega>/dOmega
+int magnus_vjp(imsrg_flow_s* F, const double* ETA, const double* G, double conv, const double* DGb, double* ETAb,
+               double* Gb, int* ncomm, hipStream_t s) {
+  int rc;
+  if ((rc = ensure_adj(F->B))) return rc;
+  double* dg = nullptr;
+  if ((rc = fbuf(F, &F->mDG, F->B->b.oplen))) return rc;
+  dg = F->mDG;
+  TermRec rec;
+  size_t pidx = 0;
+  if ((rc = magnus_record(F, ETA, dg, G, conv, rec, pidx, ncomm, s))) return rc;
+  return magnus_reverse(F, G, rec, DGb, ETAb, Gb, s);
 }
 
 // build_gs_white transpose (generators.f90:4-106): Hb += d<ETAb, white(H)>/dH
@@ -1174,13 +1229,11 @@ int normal_order_vjp(imsrg_basis_s* B, const double* Hb, double* Tb, hipStream_t
   return 0;
 }
 
-// dE0/dT of the flow of magnus_decouple (white generator, |mbpt2| criterion)
-// started from H = normal_order(T).  Omega is checkpointed every `ckpt`
-// steps; each segment is recomputed from its checkpoint (Omega, H(s), eta)
-// and swept in reverse through BCH_EXPAND, euler_step, MAGNUS_EXPAND and the
-// generator.
-int flow_grad(imsrg_flow_s* F, const double* T, double ds, double bch_conv, double mag_conv, double crit_tol,
-              int maxsteps, int ckpt, double* Tb, double* E0, int* nsteps, hipStream_t s) {
+// Forward sweep of the flow gradient: magnus_decouple (white generator,
+// |mbpt2| > crit_tol loop) from H = normal_order(T), keeping Omega every
+// `ckpt` steps in the flow object for flow_backward.
+int flow_forward(imsrg_flow_s* F, const double* T, double ds, double bch_conv, double mag_conv, double crit_tol,
+                 int maxsteps, int ckpt, double* E0, int* nsteps, hipStream_t s) {
   imsrg_basis_s* B = F->B;
   const i64 n = B->b.oplen;
   int rc, nt = 0, nc = 0;
@@ -1188,6 +1241,7 @@ int flow_grad(imsrg_flow_s* F, const double* T, double ds, double bch_conv, doub
   if ((rc = ensure_adj(B))) return rc;
   for (double** p : {&F->gH, &F->gHS, &F->gG, &F->gDG, &F->gHSb, &F->gGb, &F->gHb, &F->gdOb, &F->getab})
     if ((rc = fbuf(F, p, n))) return rc;
+  F->fw_ok = false;
   HIPCHK(hipMemcpyAsync(F->gH, T, n * sizeof(double), hipMemcpyDeviceToDevice, s));
   launch_normal_order(B->D, F->gH, B->dval + 8, s);
   HIPCHK(hipMemcpyAsync(F->gHS, F->gH, n * sizeof(double), hipMemcpyDeviceToDevice, s));
@@ -1214,7 +1268,23 @@ int flow_grad(imsrg_flow_s* F, const double* T, double ds, double bch_conv, doub
   }
   double e0 = 0.0;
   if ((rc = read_scalar(B, F->gHS, &e0, s))) return rc;
-  // reverse sweep
+  F->fw_ok = true;
+  F->fw_steps = steps; F->fw_ckpt = ckpt; F->fw_ds = ds; F->fw_bch = bch_conv; F->fw_mag = mag_conv;
+  if (E0) *E0 = e0;
+  if (nsteps) *nsteps = steps;
+  return 0;
+}
+
+// Reverse sweep: each checkpoint segment is recomputed (Omega, H(s), eta)
+// and swept backwards through BCH_EXPAND, euler_step, MAGNUS_EXPAND and the
+// generator; Tb = dE0/dT.
+int flow_backward(imsrg_flow_s* F, double* Tb, hipStream_t s) {
+  imsrg_basis_s* B = F->B;
+  const i64 n = B->b.oplen;
+  int rc, nt = 0, nc = 0;
+  if (!F->fw_ok) return fail(IMSRG_EINVAL, "flow gradient: no forward sweep to reverse");
+  const int steps = F->fw_steps, ckpt = F->fw_ckpt;
+  const double ds = F->fw_ds, bch_conv = F->fw_bch, mag_conv = F->fw_mag;
   launch_fill(F->gHSb, n, 0.0, s);
   launch_fill(F->gHSb, 1, 1.0, s);
   launch_fill(F->gGb, n, 0.0, s);
@@ -1233,21 +1303,23 @@ int flow_grad(imsrg_flow_s* F, const double* T, double ds, double bch_conv, doub
     }
     launch_fill(F->segE[0], n, 0.0, s);
     if ((rc = white(B, F->segH[0], F->segE[0], s))) return rc;
+    std::vector<TermRec> mrec(n1 - n0), brec(n1 - n0);
+    size_t pidx = 0;
     for (int m = n0; m < n1; ++m) {
       const int k = m - n0;
-      HIPCHK(hipMemcpyAsync(F->gDG, F->segE[k], n * sizeof(double), hipMemcpyDeviceToDevice, s));
-      if ((rc = magnus(F, F->gDG, -1, F->segG[k], -1, mag_conv, &nc, s))) return rc;
+      if ((rc = magnus_record(F, F->segE[k], F->gDG, F->segG[k], mag_conv, mrec[k], pidx, &nc, s))) return rc;
       launch_axpy(n, F->segG[k], 1.0, F->gDG, ds, F->segG[k + 1], s);
-      if ((rc = bch(F, F->segH[k + 1], F->segG[k + 1], -1, F->gH, 1, bch_conv, 40, &nt, nullptr, s))) return rc;
+      if ((rc = bch_record(F, F->segG[k + 1], F->gH, 1, bch_conv, 40, brec[k], pidx, F->segH[k + 1], &nt, s)))
+        return rc;
       launch_fill(F->segE[k + 1], n, 0.0, s);
       if ((rc = white(B, F->segH[k + 1], F->segE[k + 1], s))) return rc;
     }
     for (int m = n1 - 1; m >= n0; --m) {
       const int k = m - n0;
-      if ((rc = bch_vjp(F, F->segG[k + 1], F->gH, 1, bch_conv, 40, F->gHSb, F->gGb, F->gHb, &nt, s))) return rc;
+      if ((rc = bch_reverse(F, F->segG[k + 1], 1, brec[k], F->gHSb, F->gGb, F->gHb, s))) return rc;
       launch_axpy(n, F->gGb, ds, F->gGb, 0.0, F->gdOb, s);
       launch_fill(F->getab, n, 0.0, s);
-      if ((rc = magnus_vjp(F, F->segE[k], F->segG[k], mag_conv, F->gdOb, F->getab, F->gGb, &nc, s))) return rc;
+      if ((rc = magnus_reverse(F, F->segG[k], mrec[k], F->gdOb, F->getab, F->gGb, s))) return rc;
       launch_fill(F->gHSb, n, 0.0, s);
       if ((rc = white_vjp(B, F->segH[k], F->getab, F->gHSb, s))) return rc;
       if (m == 0) {
@@ -1258,8 +1330,6 @@ int flow_grad(imsrg_flow_s* F, const double* T, double ds, double bch_conv, doub
   }
   if ((rc = normal_order_vjp(B, F->gHb, Tb, s))) return rc;
   HIPCHK(hipStreamSynchronize(s));
-  if (E0) *E0 = e0;
-  if (nsteps) *nsteps = steps;
   return 0;
 }
 
@@ -1657,10 +1727,23 @@ int imsrg_normal_order_vjp(imsrg_basis B, const double* Hb, double* Tb, imsrg_st
   return normal_order_vjp(B, Hb, Tb, S(stream));
 }
 
+int imsrg_flow_forward(imsrg_flow F, const double* T, double ds, double bch_conv, double mag_conv, double crit_tol,
+                       int maxsteps, int ckpt, double* E0, int* steps, imsrg_stream stream) {
+  if (!F || !T) return fail(IMSRG_EINVAL, "null argument");
+  return flow_forward(F, T, ds, bch_conv, mag_conv, crit_tol, maxsteps, ckpt, E0, steps, S(stream));
+}
+
+int imsrg_flow_backward(imsrg_flow F, double* Tb, imsrg_stream stream) {
+  if (!F || !Tb) return fail(IMSRG_EINVAL, "null argument");
+  return flow_backward(F, Tb, S(stream));
+}
+
 int imsrg_flow_grad(imsrg_flow F, const double* T, double ds, double bch_conv, double mag_conv, double crit_tol,
                     int maxsteps, int ckpt, double* Tb, double* E0, int* steps, imsrg_stream stream) {
   if (!F || !T || !Tb) return fail(IMSRG_EINVAL, "null argument");
-  return flow_grad(F, T, ds, bch_conv, mag_conv, crit_tol, maxsteps, ckpt, Tb, E0, steps, S(stream));
+  int rc = flow_forward(F, T, ds, bch_conv, mag_conv, crit_tol, maxsteps, ckpt, E0, steps, S(stream));
+  if (rc) return rc;
+  return flow_backward(F, Tb, S(stream));
 }
 
 int imsrg_set_gemm_tile(imsrg_basis B, int family, int tile) {

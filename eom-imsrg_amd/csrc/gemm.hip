@@ -128,8 +128,11 @@ __global__ __launch_bounds__(256) void k_gemm_t(Slots sl, const GemmProb* probs,
   v4d acc[MI * NI];
 #pragma unroll
   for (int a = 0; a < MI * NI; ++a) acc[a] = v4d{0.0, 0.0, 0.0, 0.0};
+  // mode bit3: A is upper triangular (as stored); rows m0.. of op(A) reach
+  // only k >= m0 (A as stored) or k < m0 + TBM (A transposed)
+  const bool tri = pr.mode & 8;
   mma<TBM, TBN>(acc, As, Bs, sl.p[pr.sa1] + pr.a1, pr.lda, pr.ta1, sl.p[pr.sb1] + pr.b1, pr.ldb, pr.tb1, pr.m, pr.n,
-                kb, ke, m0, n0, 1.0);
+                (tri && !pr.ta1) ? max(kb, m0) : kb, (tri && pr.ta1) ? min(ke, m0 + TBM) : ke, m0, n0, 1.0);
   if (pr.nprod == 2) {
     // second product: same K range, or its own K (k2 > 0) with its own
     // leading dimensions packed in a2/b2's problem (lda2 = ldb2 = m here)
@@ -138,7 +141,8 @@ __global__ __launch_bounds__(256) void k_gemm_t(Slots sl, const GemmProb* probs,
                     0, pr.k2, m0, n0, pr.s2 * s2scale);
     else
       mma<TBM, TBN>(acc, As, Bs, sl.p[pr.sa2] + pr.a2, pr.lda, pr.ta2, sl.p[pr.sb2] + pr.b2, pr.ldb, pr.tb2, pr.m,
-                    pr.n, kb, ke, m0, n0, pr.s2 * s2scale);
+                    pr.n, (tri && !pr.ta2) ? max(kb, m0) : kb, (tri && pr.ta2) ? min(ke, m0 + TBM) : ke, m0, n0,
+                    pr.s2 * ((pr.mode == 0 || (pr.mode & 16)) ? s2scale : 1.0));
   }
   double* C = const_cast<double*>(sl.p[pr.sc]) + pr.c + (i64)t.ks * pr.cstride;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;

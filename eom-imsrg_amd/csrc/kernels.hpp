@@ -44,8 +44,9 @@ struct GemmProb {
   int nprod;
   unsigned char ta1, tb1, ta2, tb2;
   unsigned char sa1, sb1, sa2, sb2;
-  unsigned char sc, mode, pad1, pad2;   // mode 0: alpha*ascale, s2*s2scale;
-                                        // else bit0: *ascale, bit1: *s2scale (one product)
+  unsigned char sc, mode, pad1, pad2;   // mode 0: alpha*ascale, s2*s2scale; else alpha is
+                                        // scaled by ascale (bit0) and s2scale (bit1), s2 by
+                                        // s2scale (bit4); bit3: A upper triangular
   double alpha, s2, beta;
   int kchunk;        // > 0: split K in chunks of kchunk, chunk ks written to C + ks*cstride
   int k2;            // > 0: K of the second product when it differs (no split K then)
@@ -152,7 +153,9 @@ void launch_122_vjp_1b(const DevBasis& B, const SegTask* t, int nt, const Partne
                        const double* R, int Rh, const double* RESb, int RESh, double* Lb, double* Rb, hipStream_t s);
 void launch_ph_vjp(const DevBasis& B, const QuadItem* items, const int* qstart, int nquads, const double* RESb,
                    int RESh, int Lh, int Rh, double* U, hipStream_t s);
-void launch_ph_mform(const DevBasis& B, const CCTask* t, int nt, double* U, int Lh, int Rh, hipStream_t s);
+void launch_122_vjp_2b(const DevBasis& B, const SegTask* t, int nt, const Partner* part, const int* rowp,
+                       const int* colp, const double* L, int Lh, const double* R, int Rh, const double* cbar, int ch,
+                       double* Lb, double* Rb, hipStream_t s);
 void launch_pandya_vjp(const DevBasis& B, const CCTask* t, int nt, const double* CCb, int herm, double* opb,
                        hipStream_t s);
 void launch_white_vjp(const DevBasis& B, const SegTask* g3, int ng3, const double* H, const double* M,

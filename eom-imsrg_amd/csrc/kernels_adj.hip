@@ -314,6 +314,88 @@ void launch_from_dense(const DevBasis& B, const SegTask* t, int nt, const double
   hipLaunchKernelGGL(k_from_dense, dim3(nt), dim3(256), 0, s, B, t, Y, alpha, dst);
 }
 
+// two-body part of the 122 transpose in one pass: with cbar the symmetrised
+// cotangent (one-body part zero), y1 = [R_1b, cbar] and y2 = [L_1b, cbar]
+// element by element (the partner sums of k_comm122p), then
+// Lb += -Rh y1 and Rb += Lh y2 on stored elements, the lower partners of
+// square blocks through their herm (Lh for y1, Rh for y2), rectangular
+// blocks twice (their transposed partners are not stored)
+__global__ __launch_bounds__(256) void k_122_vjp_2b(DevBasis B, const SegTask* tasks, const Partner* part,
+                                                    const int* rowp, const int* colp, const double* L, int Lh,
+                                                    const double* R, int Rh, const double* cbar, int ch, double* Lb,
+                                                    double* Rb) {
+  const SegTask t = tasks[blockIdx.x];
+  const i64 e = (i64)t.start + threadIdx.x;
+  if (e >= t.cnt) return;
+  const int q = t.q, g = t.g;
+  const int rc = row_cls(g), cc = col_cls(g);
+  const int n1 = B.dims[q * 3 + rc];
+  const bool sq = is_square(g);
+  int IX, JX;
+  if (sq) upper_decode(e, IX, JX);
+  else { IX = (int)(e % n1); JX = (int)(e / n1); }
+  const i64 gr = B.pairoff[q * 3 + rc] + IX, gc = B.pairoff[q * 3 + cc] + JX;
+  int a, b, c, d;
+  unpack(B.pairs[gr], a, b);
+  unpack(B.pairs[gc], c, d);
+  const double pre_ab = (a == b) ? 1.4142135623730951 : 1.0;
+  const double pre_cd = (c == d) ? 1.4142135623730951 : 1.0;
+  const i64* go = B.goff + q * 6;
+  const int* dm = B.dims + q * 3;
+  double y1 = 0.0, y2 = 0.0;
+  for (int x = rowp[gr]; x < rowp[gr + 1]; ++x) {
+    const Partner p = part[x];
+    const int idx = p.code & 0x3fffff, cl = (p.code >> 22) & 3;
+    double pre = ((p.code >> 26) & 1) ? 1.4142135623730951 : 1.0;
+    if ((p.code >> 25) & 1) pre = -pre;
+    pre = pre * pre_cd;
+    double v;
+    if (cl <= cc) v = cbar[go[cl == 0 ? cc : (cl == 1 ? (cc == 1 ? 3 : 5) : 4)] + idx + (i64)JX * dm[cl]] * pre;
+    else v = cbar[go[cc == 0 ? cl : (cc == 1 ? (cl == 1 ? 3 : 5) : 4)] + JX + (i64)idx * dm[cc]] * ch * pre;
+    const bool fs = (p.code >> 24) & 1;
+    y1 += R[p.foff] * (fs ? Rh : 1) * v;
+    y2 += L[p.foff] * (fs ? Lh : 1) * v;
+  }
+  for (int x = colp[gc]; x < colp[gc + 1]; ++x) {
+    const Partner p = part[x];
+    const int idx = p.code & 0x3fffff, cl = (p.code >> 22) & 3;
+    double pre = ((p.code >> 26) & 1) ? 1.4142135623730951 : 1.0;
+    if ((p.code >> 25) & 1) pre = -pre;
+    pre = pre_ab * pre;
+    double v;
+    if (rc <= cl) v = cbar[go[rc == 0 ? cl : (rc == 1 ? (cl == 1 ? 3 : 5) : 4)] + IX + (i64)idx * dm[rc]] * pre;
+    else v = cbar[go[cl == 0 ? rc : (cl == 1 ? (rc == 1 ? 3 : 5) : 4)] + idx + (i64)IX * dm[cl]] * ch * pre;
+    const bool fs = (p.code >> 24) & 1;
+    y1 -= R[p.foff] * (fs ? Rh : 1) * v;
+    y2 -= L[p.foff] * (fs ? Lh : 1) * v;
+  }
+  const double nrm = 1.0 / (sqrt(1.0 + (a == b ? 1.0 : 0.0)) * sqrt(1.0 + (c == d ? 1.0 : 0.0)));
+  y1 *= nrm;
+  y2 *= nrm;
+  const i64 o = go[g] + IX + (i64)JX * n1;
+  if (!sq) {
+    Lb[o] += -2.0 * Rh * y1;
+    Rb[o] += 2.0 * Lh * y2;
+  } else if (IX == JX) {
+    Lb[o] += -Rh * Lh * y1;
+    Rb[o] += Lh * Rh * y2;
+  } else {
+    const i64 ol = go[g] + JX + (i64)IX * n1;
+    Lb[o] += -Rh * y1;
+    Lb[ol] += -Rh * Lh * y1;
+    Rb[o] += Lh * y2;
+    Rb[ol] += Lh * Rh * y2;
+  }
+}
+
+void launch_122_vjp_2b(const DevBasis& B, const SegTask* t, int nt, const Partner* part, const int* rowp,
+                       const int* colp, const double* L, int Lh, const double* R, int Rh, const double* cbar, int ch,
+                       double* Lb, double* Rb, hipStream_t s) {
+  if (nt <= 0) return;
+  hipLaunchKernelGGL(k_122_vjp_2b, dim3(nt), dim3(kTile), 0, s, B, t, part, rowp, colp, L, Lh, R, Rh, cbar, ch, Lb,
+                     Rb);
+}
+
 // one-body part of the 122 transpose: the partner sums of k_comm122p with
 // the cotangent of each upper element; contributions to the few one-body
 // elements are summed per workgroup in LDS (compact index f1map[foff])
@@ -494,30 +576,6 @@ void launch_ph_vjp(const DevBasis& B, const QuadItem* items, const int* qstart, 
   if (nquads <= 0) return;
   hipLaunchKernelGGL(k_ph_vjp, dim3((nquads + 255) / 256), dim3(256), 0, s, B, items, qstart, nquads, RESb, RESh,
                      Lh, Rh, U);
-}
-
-// U (upper-triangle cotangent of W') -> M = Lh U - Rh U^T in place, so that
-// LCCb = M RCC and RCCb = M^T LCC.  One thread per upper element.
-__global__ __launch_bounds__(256) void k_ph_mform(DevBasis B, const CCTask* tasks, double* U, int Lh, int Rh) {
-  const CCTask t = tasks[blockIdx.x];
-  const int r = B.rlen[t.q1];
-  const i64 e = (i64)t.start + threadIdx.x;
-  if (e >= (i64)r * (r + 1) / 2) return;
-  int x, y;
-  upper_decode(e, x, y);
-  double* W = U + B.wcoff[t.q1];
-  const double u = W[x + (i64)y * r];
-  if (x == y) {
-    W[x + (i64)x * r] = (Lh - Rh) * u;
-  } else {
-    W[x + (i64)y * r] = Lh * u;
-    W[y + (i64)x * r] = -Rh * u;
-  }
-}
-
-void launch_ph_mform(const DevBasis& B, const CCTask* t, int nt, double* U, int Lh, int Rh, hipStream_t s) {
-  if (nt <= 0) return;
-  hipLaunchKernelGGL(k_ph_mform, dim3(nt), dim3(256), 0, s, B, t, U, Lh, Rh);
 }
 
 // Pandya transpose: CCb -> opb (calculate_cross_coupled read positions)

@@ -86,6 +86,8 @@ SIGNATURES = {
     "imsrg_magnus_vjp": (_i, [_p, _dp, _dp, C.c_double, _dp, _dp, _dp, _ip, _p]),
     "imsrg_white_vjp": (_i, [_p, _dp, _dp, _dp, _p]),
     "imsrg_normal_order_vjp": (_i, [_p, _dp, _dp, _p]),
+    "imsrg_flow_forward": (_i, [_p, _dp, C.c_double, C.c_double, C.c_double, C.c_double, _i, _i, _d, _ip, _p]),
+    "imsrg_flow_backward": (_i, [_p, _dp, _p]),
     "imsrg_flow_grad": (_i, [_p, _dp, C.c_double, C.c_double, C.c_double, C.c_double, _i, _i, _dp, _d, _ip, _p]),
     "imsrg_set_gemm_tile": (_i, [_p, _i, _i]),
     "imsrg_get_gemm_tile": (_i, [_p, _i]),

@@ -67,29 +67,43 @@ def flow_grad(flow: Flow, T: torch.Tensor, ds: float = 0.5, bch_conv: float = RE
     return e0.value, Tb, st.value
 
 
+def flow_forward(flow: Flow, T: torch.Tensor, ds: float = 0.5, bch_conv: float = REF_TOL,
+                 mag_conv: float = REF_TOL, crit_tol: float = REF_TOL, maxsteps: int = 200, ckpt: int = 4):
+    """run the flow from normal_order(T) keeping Omega every `ckpt` steps in
+    `flow`; returns (E0, steps)"""
+    e0 = C.c_double()
+    st = C.c_int()
+    _lib.check(_lib.lib.imsrg_flow_forward(flow.handle, T.data_ptr(), float(ds), float(bch_conv), float(mag_conv),
+                                           float(crit_tol), int(maxsteps), int(ckpt), C.byref(e0), C.byref(st),
+                                           _stream()))
+    return e0.value, st.value
+
+
+def flow_backward(flow: Flow, Tb: torch.Tensor) -> None:
+    """dE0/dT of the last flow_forward on `flow` into Tb"""
+    _lib.check(_lib.lib.imsrg_flow_backward(flow.handle, Tb.data_ptr(), _stream()))
+
+
 class _FlowEnergy(torch.autograd.Function):
     @staticmethod
     def forward(ctx, T, flow, ds, tol, ckpt):
-        b = flow.basis
-        H = SqOp(b, T.detach().clone(), 1)
-        from . import commutators as cm
-        cm.normal_order(H)
-        G = SqOp.zeros(b, -1)
-        flow.magnus_decouple(H, G, ds=ds, bch_conv=tol, mag_conv=tol, crit_tol=tol)
-        ctx.flow, ctx.ds, ctx.tol, ctx.ckpt = flow, ds, tol, ckpt
-        ctx.save_for_backward(T)
-        return H.data[0].clone()
+        Tc = T.detach().contiguous()
+        e0, _ = flow_forward(flow, Tc, ds, tol, tol, tol, ckpt=ckpt)
+        ctx.flow = flow
+        return torch.tensor(e0, dtype=torch.float64, device=T.device)
 
     @staticmethod
     def backward(ctx, gE):
-        (T,) = ctx.saved_tensors
-        _, Tb, _ = flow_grad(ctx.flow, T.detach().contiguous(), ctx.ds, ctx.tol, ctx.tol, ctx.tol, ckpt=ctx.ckpt)
+        Tb = torch.empty(ctx.flow.basis.oplen, dtype=torch.float64, device=gE.device)
+        flow_backward(ctx.flow, Tb)
         return Tb * gE, None, None, None, None
 
 
 def flow_energy(T: torch.Tensor, flow: Flow, ds: float = 0.5, tol: float = REF_TOL, ckpt: int = 4) -> torch.Tensor:
     """E0 after magnus_decouple from the vacuum operator T (CUDA float64,
-    basis.oplen elements), differentiable with respect to T"""
+    basis.oplen elements), differentiable with respect to T.  The flow object
+    holds the checkpoints of the forward sweep until the next call, so call
+    backward before running another flow_energy on the same `flow`."""
     return _FlowEnergy.apply(T, flow, ds, tol, ckpt)
 
 

@@ -205,9 +205,14 @@ int imsrg_white_vjp(imsrg_basis b, const double* H, const double* ETAb, double* 
  * (overwritten) */
 int imsrg_normal_order_vjp(imsrg_basis b, const double* Hb, double* Tb, imsrg_stream stream);
 /* dE0/dT for magnus_decouple (IMSRG_MAGNUS.f90:15-128, white generator,
- * |mbpt2| > crit_tol loop) started from H = normal_order(T): Tb receives the
- * gradient (overwritten), E0 the flowed energy, steps the step count.  Omega
- * is checkpointed every `ckpt` steps and the steps in between recomputed. */
+ * |mbpt2| > crit_tol loop) started from H = normal_order(T), in two sweeps:
+ * imsrg_flow_forward runs the flow, returns E0 and the step count and keeps
+ * Omega every `ckpt` steps in the flow object; imsrg_flow_backward then
+ * writes dE0/dT into Tb (overwritten), recomputing the steps between
+ * checkpoints.  imsrg_flow_grad does both. */
+int imsrg_flow_forward(imsrg_flow f, const double* T, double ds, double bch_conv, double mag_conv, double crit_tol,
+                       int maxsteps, int ckpt, double* E0, int* steps, imsrg_stream stream);
+int imsrg_flow_backward(imsrg_flow f, double* Tb, imsrg_stream stream);
 int imsrg_flow_grad(imsrg_flow f, const double* T, double ds, double bch_conv, double mag_conv, double crit_tol,
                     int maxsteps, int ckpt, double* Tb, double* E0, int* steps, imsrg_stream stream);
 
