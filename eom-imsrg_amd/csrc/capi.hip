@@ -92,9 +92,19 @@ struct imsrg_basis_s {
   // starts, the compact one-body map of 122
   bool adj_ready = false;
   GemmFamily pb[6], phb;
-  std::vector<int> qstart_h, f1map_h, f1off_h;
-  int *qstart = nullptr, *f1map = nullptr, *f1off = nullptr;
-  int nquads = 0, nf1 = 0;
+  std::vector<int> f1map_h, f1off_h;
+  int *f1map = nullptr, *f1off = nullptr;
+  int nf1 = 0;
+  // 222_ph transpose: quadruples (packed ijkl) with the start of their J
+  // items, one lane per (quadruple, recoupling, J'), the upper 64 x 64 tiles
+  // of the W' blocks and the (lane, position) terms of each tile
+  std::vector<int> phq_h, phqs_h, phl_h, pht0_h;
+  std::vector<PairTask> phtile_h;
+  std::vector<int2> phterm_h;
+  int *phq = nullptr, *phqs = nullptr, *phl = nullptr, *pht0 = nullptr;
+  PairTask* phtile = nullptr;
+  int2* phterm = nullptr;
+  double *SB = nullptr, *CV = nullptr;
   double *Wb1 = nullptr, *Wb2 = nullptr, *cbar = nullptr, *Y = nullptr, *U = nullptr;
   double *aLCC = nullptr, *aRCC = nullptr, *aLCCb = nullptr, *aRCCb = nullptr, *Mb = nullptr;
 
@@ -743,12 +753,68 @@ int ensure_adj(imsrg_basis_s* B) {
     }
     B->phb.group_end.push_back((int)B->phb.probs_h.size());
   }
-  // quadruple starts of the 222_ph item list
-  B->qstart_h.clear();
-  for (size_t x = 0; x < B->quad_h.size(); ++x)
-    if (x == 0 || B->quad_h[x].ijkl != B->quad_h[x - 1].ijkl) B->qstart_h.push_back((int)x);
-  B->nquads = (int)B->qstart_h.size();
-  B->qstart_h.push_back((int)B->quad_h.size());
+  // 222_ph transpose: lanes, tiles and the terms of each tile
+  {
+    B->phq_h.clear(); B->phqs_h.clear(); B->phl_h.clear(); B->phtile_h.clear();
+    std::vector<int> tbase(b.ncc + 1, 0);
+    for (int q = 0; q < b.ncc; ++q) {
+      const int nt = (b.rlen[q] + 63) / 64;
+      tbase[q] = (int)B->phtile_h.size();
+      for (int tj = 0; tj < nt; ++tj)
+        for (int ti = 0; ti <= tj; ++ti) B->phtile_h.push_back(PairTask{q, ti, tj, 0});
+    }
+    const int N = b.N, nJ = b.nJ, JTM = b.JTM;
+    std::vector<int> tid;   // tile of each term, in term order
+    std::vector<int2> term;
+    auto rmap = [&](int x, int y, int JP) { return b.rmap[((size_t)x * N + y) * nJ + JP / 2]; };
+    auto add = [&](int lane, int qx, int x, int y, bool neg) {
+      if (x < 0 || y < 0) return false;
+      int bits = neg ? 4096 : 0;
+      if (x > y) { std::swap(x, y); bits |= 8192; }
+      const int ti = x / 64, tj = y / 64;
+      tid.push_back(tbase[qx] + tj * (tj + 1) / 2 + ti);
+      term.push_back(int2{lane, bits | ((x - 64 * ti) + 64 * (y - 64 * tj))});
+      return true;
+    };
+    bool ok = true;
+    for (size_t x = 0; x < B->quad_h.size(); ++x) {
+      if (x > 0 && B->quad_h[x].ijkl == B->quad_h[x - 1].ijkl) continue;
+      const int u = B->quad_h[x].ijkl;
+      const int i = u & 255, j = (u >> 8) & 255, k = (u >> 16) & 255, l = (u >> 24) & 255;
+      const int ji = b.jj[i], jj = b.jj[j], jk = b.jj[k], jl = b.jj[l];
+      const int a1 = std::max(std::abs(jj - jl), std::abs(ji - jk)), b1 = std::min(jj + jl, ji + jk);
+      const int a2 = std::max(std::abs(ji - jl), std::abs(jj - jk)), b2 = std::min(ji + jl, jj + jk);
+      const int n1 = b1 >= a1 ? (b1 - a1) / 2 + 1 : 0, n2 = b2 >= a2 ? (b2 - a2) / 2 + 1 : 0;
+      const int qd = (int)B->phq_h.size();
+      if (n1 + n2 > 64 || qd >= (1 << 25)) return fail(IMSRG_EINVAL, "222_ph transpose lane index overflow");
+      const bool neg = ((ji + jj + jk + jl) / 2) % 2 != 0;
+      const int T1 = std::abs(b.tz[i] - b.tz[k]) / 2, P1 = (b.ll[i] + b.ll[k]) % 2;
+      const int T2 = std::abs(b.tz[i] - b.tz[l]) / 2, P2 = (b.ll[i] + b.ll[l]) % 2;
+      B->phq_h.push_back(u);
+      B->phqs_h.push_back((int)x);
+      for (int sl = 0; sl < n1 + n2; ++sl) {
+        const int lane = (int)B->phl_h.size();
+        B->phl_h.push_back(qd * 64 + sl);
+        if (sl < n1) {
+          const int JP = a1 + 2 * sl, qx = JP / 2 + T1 * (JTM + 1) + 2 * P1 * (JTM + 1);
+          ok = ok && add(lane, qx, rmap(k, i, JP), rmap(j, l, JP), false);
+          ok = ok && add(lane, qx, rmap(l, j, JP), rmap(i, k, JP), neg);
+        } else {
+          const int JP = a2 + 2 * (sl - n1), qx = JP / 2 + T2 * (JTM + 1) + 2 * P2 * (JTM + 1);
+          ok = ok && add(lane, qx, rmap(l, i, JP), rmap(j, k, JP), false);
+          ok = ok && add(lane, qx, rmap(k, j, JP), rmap(i, l, JP), neg);
+        }
+      }
+    }
+    if (!ok) return fail(IMSRG_EINVAL, "222_ph transpose: a read outside the cross-coupled rows");
+    const size_t nt = B->phtile_h.size();
+    B->pht0_h.assign(nt + 1, 0);
+    for (int t : tid) ++B->pht0_h[t + 1];
+    for (size_t t = 0; t < nt; ++t) B->pht0_h[t + 1] += B->pht0_h[t];
+    B->phterm_h.resize(term.size());
+    std::vector<int> fillp(B->pht0_h.begin(), B->pht0_h.end() - 1);
+    for (size_t e = 0; e < term.size(); ++e) B->phterm_h[fillp[tid[e]]++] = term[e];
+  }
   // compact index of the one-body elements the 122 partner sums read
   B->f1map_h.assign((size_t)(b.off_fpp + (i64)b.P * b.P), -1);
   B->f1off_h.clear();
@@ -760,7 +826,10 @@ int ensure_adj(imsrg_basis_s* B) {
     if ((rc = upload(B, f->probs_h, &f->probs))) return rc;
     if ((rc = family_tasks(B, *f, 0))) return rc;
   }
-  if ((rc = upload(B, B->qstart_h, &B->qstart))) return rc;
+  if ((rc = upload(B, B->phq_h, &B->phq)) || (rc = upload(B, B->phqs_h, &B->phqs)) ||
+      (rc = upload(B, B->phl_h, &B->phl)) || (rc = upload(B, B->phtile_h, &B->phtile)) ||
+      (rc = upload(B, B->pht0_h, &B->pht0)) || (rc = upload(B, B->phterm_h, &B->phterm)))
+    return rc;
   if ((rc = upload(B, B->f1map_h, &B->f1map))) return rc;
   if ((rc = upload(B, B->f1off_h, &B->f1off))) return rc;
   const size_t op = (size_t)std::max<i64>(b.oplen, 1) * sizeof(double);
@@ -770,8 +839,13 @@ int ensure_adj(imsrg_basis_s* B) {
       (rc = dalloc(B, (size_t)std::max<i64>(b.wclen, 1) * sizeof(double), (void**)&B->U)) ||
       (rc = dalloc(B, cc, (void**)&B->aLCC)) || (rc = dalloc(B, cc, (void**)&B->aRCC)) ||
       (rc = dalloc(B, cc, (void**)&B->aLCCb)) || (rc = dalloc(B, cc, (void**)&B->aRCCb)) ||
-      (rc = dalloc(B, (size_t)b.N * b.N * sizeof(double), (void**)&B->Mb)))
+      (rc = dalloc(B, (size_t)b.N * b.N * sizeof(double), (void**)&B->Mb)) ||
+      (rc = dalloc(B, std::max<size_t>(B->quad_h.size(), 1) * sizeof(double), (void**)&B->SB)) ||
+      (rc = dalloc(B, std::max<size_t>(B->phl_h.size(), 1) * sizeof(double), (void**)&B->CV)))
     return rc;
+  // the transpose stores every upper tile of U; the tiles below stay zero,
+  // as the triangular GEMMs read the lower half of their diagonal tiles
+  HIPCHK(hipMemset(B->U, 0, (size_t)std::max<i64>(b.wclen, 1) * sizeof(double)));
   B->adj_ready = true;
   return 0;
 }
@@ -813,8 +887,8 @@ void comm_vjp(imsrg_basis_s* B, const double* L, int Lh, const double* R, int Rh
     launch_pandya(D, B->cc.dev, B->cc.n, L, Lh, B->aLCC, s);
     LCC = B->aLCC;
   }
-  launch_fill(B->U, b.wclen, 0.0, s);
-  launch_ph_vjp(D, B->quad, B->qstart, B->nquads, RESb, RESh, Lh, Rh, B->U, s);
+  launch_ph_vjp(D, B->quad, (int)B->quad_h.size(), B->phl, (int)B->phl_h.size(), B->phq, B->phqs, B->phtile,
+                (int)B->phtile_h.size(), B->pht0, B->phterm, RESb, RESh, Lh, Rh, B->SB, B->CV, B->U, s);
   double* LCCb = LCCb_acc ? LCCb_acc : B->aLCCb;
   if (!LCCb_acc) launch_fill(B->aLCCb, b.cclen, 0.0, s);
   launch_fill(B->aRCCb, b.cclen, 0.0, s);

@@ -485,97 +485,104 @@ void launch_122_vjp_1b(const DevBasis& B, const SegTask* t, int nt, const Partne
                      f1map, f1off, nf1, L, Lh, R, Rh, RESb, RESh, Lb, Rb);
 }
 
-// -------------------------------------------------- 222_ph transposes
-// add c at the W' element the scatter reads as wprime(x, y): the upper
-// triangle holds W'; x > y reads -Lh Rh W'(y, x)
-__device__ __forceinline__ void wadj(double* U, i64 rr, int x, int y, double mpm, double c) {
-  if (x <= y) aadd(U + x + (i64)y * rr, c);
-  else aadd(U + y + (i64)x * rr, mpm * c);
-}
+// -------------------------------------------------- 222_ph transpose
+// The scatter of 222_ph (one output per quadruple (ij,kl) and J) reads W'
+// of the cross-coupled block at 2J' through its upper triangle U:
+// wprime(x, y) = U(x, y) for x <= y and -Lh Rh U(y, x) below.  Its
+// transpose runs in three steps:
+//   k_ph_sb:   the folded cotangent of every (quadruple, J) output, one lane
+//              per output as in the forward scatter;
+//   k_ph_coef: one lane per (quadruple, recoupling, J'): its J column of 6j
+//              symbols against those cotangents;
+//   k_ph_tile: one workgroup per 64 x 64 tile of U: the (lane, sign) terms
+//              that land in the tile, summed in LDS and stored once.
+// Where each term lands depends on the basis only, so the host sorts the
+// terms by tile once; no global atomics, and every upper tile of U is
+// written (no clearing pass).
 
-// One thread per orbit quadruple (its J items are contiguous in `items`):
-// for every J' the contributions of all its J are summed before the
-// atomic adds into U (the cotangent of the upper triangle of W').
-__global__ __launch_bounds__(256) void k_ph_vjp(DevBasis B, const QuadItem* items, const int* qstart, int nquads,
-                                                const double* RESb, int RESh, int Lh, int Rh, double* U) {
+__global__ __launch_bounds__(256) void k_ph_sb(DevBasis B, const QuadItem* items, int n, const double* RESb, int RESh,
+                                               double* SB) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nquads) return;
-  const int i0 = qstart[t], i1 = qstart[t + 1];
-  const int ijkl = items[i0].ijkl;
-  const int i = ijkl & 255, j = (ijkl >> 8) & 255, k = (ijkl >> 16) & 255, l = (ijkl >> 24) & 255;
-  const int ji = B.jj[i], jj = B.jj[j], jk = B.jj[k], jl = B.jj[l];
-  const int li = B.ll[i], lj = B.ll[j], lk = B.ll[k], ll = B.ll[l];
-  const int ti = B.tz[i], tj = B.tz[j], tk = B.tz[k], tl = B.tz[l];
-  const int Tz1 = abs(ti - tk) / 2;
-  if (abs(tl - tj) != Tz1 * 2) return;
-  const int PAR1 = (li + lk) % 2;
-  if ((ll + lj) % 2 != PAR1) return;
-  const int Tz2 = abs(ti - tl) / 2;
-  if (abs(tk - tj) != Tz2 * 2) return;
-  const int PAR2 = (li + ll) % 2;
-  if ((lk + lj) % 2 != PAR2) return;
-  const int P = (li + lj) & 1;
+  if (t >= n) return;
+  const QuadItem it = items[t];
+  const int i = it.ijkl & 255, j = (it.ijkl >> 8) & 255, k = (it.ijkl >> 16) & 255, l = (it.ijkl >> 24) & 255;
+  const int J = it.J;
+  const int P = (B.ll[i] + B.ll[j]) & 1;
+  const int q = 3 * J + 2 * ((B.tz[i] + B.tz[j]) / 2 + 1) + P;
   const int rc = B.con[i] + B.con[j], cc = B.con[k] + B.con[l];
   const int g = (rc == 0) ? cc : (rc == 1 ? (cc == 1 ? 3 : 5) : 4);
-  const double pre = (i == j) ? IMSRG_INV_SQRT2_REF : 1.0;
-  const double pre2 = (k == l) ? IMSRG_INV_SQRT2_REF : 1.0;
-  constexpr int kMaxJ = 64;
-  double sb[kMaxJ];
-  int nj = i1 - i0;
-  if (nj > kMaxJ) nj = kMaxJ;   // 2J <= 2*jmax keeps nj far below this
-  for (int x = 0; x < nj; ++x) {
-    const int J = items[i0 + x].J;
-    const int q = 3 * J + 2 * ((ti + tj) / 2 + 1) + P;
-    const int IX = B.xmap[((i64)i * B.N + j) * B.nJ + (J >> 1)];
-    const int JX = B.xmap[((i64)k * B.N + l) * B.nJ + (J >> 1)];
-    sb[x] = fold2(B, RESb, q, g, IX, JX, RESh) * pre * pre2;
-  }
-  const double mpm = -(double)(Lh * Rh);
-  const int phase = m1pow((ji + jj + jk + jl) / 2);
-  const int N = B.N, JTM = B.JTM;
-  int jmin = max(abs(jj - jl), abs(ji - jk));
-  int jmax = min(jj + jl, ji + jk);
-  for (int JP = jmin; JP <= jmax; JP += 2) {
-    double c = 0.0;
+  const int IX = B.xmap[((i64)i * B.N + j) * B.nJ + (J >> 1)];
+  const int JX = B.xmap[((i64)k * B.N + l) * B.nJ + (J >> 1)];
+  const double pre = ((i == j) ? IMSRG_INV_SQRT2_REF : 1.0) * ((k == l) ? IMSRG_INV_SQRT2_REF : 1.0);
+  SB[t] = fold2(B, RESb, q, g, IX, JX, RESh) * pre;
+}
+
+// lanes[t] = quadruple index * 64 + slot; slots [0, n1) are the J' of the
+// first recoupling (coefficient -(-1)^((jl+jk+J)/2) {ji jj J; jl jk J'}),
+// [n1, n1 + n2) those of the second ({ji jj J; jk jl J'})
+__global__ __launch_bounds__(256) void k_ph_coef(DevBasis B, const int* lanes, int nl, const int* quads,
+                                                 const int* qstart, const double* SB, double* CV) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nl) return;
+  const int v = lanes[t];
+  const int qd = v >> 6, slot = v & 63;
+  const int ijkl = quads[qd];
+  const double* sb = SB + qstart[qd];
+  const int i = ijkl & 255, j = (ijkl >> 8) & 255, k = (ijkl >> 16) & 255, l = (ijkl >> 24) & 255;
+  const int ji = B.jj[i], jj = B.jj[j], jk = B.jj[k], jl = B.jj[l];
+  const int lo = max(abs(ji - jj), abs(jk - jl));
+  const int nj = (min(ji + jj, jk + jl) - lo) / 2 + 1;
+  const int a1 = max(abs(jj - jl), abs(ji - jk)), b1 = min(jj + jl, ji + jk);
+  const int n1 = b1 >= a1 ? (b1 - a1) / 2 + 1 : 0;
+  double c = 0.0;
+  if (slot < n1) {
+    const int JP = a1 + 2 * slot;
     for (int x = 0; x < nj; ++x) {
-      const int J = items[i0 + x].J;
+      const int J = lo + 2 * x;
       c -= m1pow((jl + jk + J) / 2) * sixj(B, ji, jj, J, jl, jk, JP) * sb[x];
     }
-    if (c == 0.0) continue;
-    const int qx = JP / 2 + Tz1 * (JTM + 1) + 2 * PAR1 * (JTM + 1);
-    double* W = U + B.wcoff[qx];
-    const i64 rr = B.rlen[qx];
-    const int* rJ = B.rmapJ + (i64)(JP >> 1) * N * N;
-    const int* rT = B.rmapJT + (i64)(JP >> 1) * N * N;
-    const int rjl = rJ[j * N + l], rik = rT[k * N + i], gjl = rJ[l * N + j], gik = rJ[k * N + i];
-    wadj(W, rr, gik, rjl, mpm, c);
-    wadj(W, rr, gjl, rik, mpm, phase * c);
+  } else {
+    const int JP = max(abs(ji - jl), abs(jj - jk)) + 2 * (slot - n1);
+    for (int x = 0; x < nj; ++x) c += sixj(B, ji, jj, lo + 2 * x, jk, jl, JP) * sb[x];
   }
-  jmin = max(abs(ji - jl), abs(jj - jk));
-  jmax = min(ji + jl, jj + jk);
-  for (int JP = jmin; JP <= jmax; JP += 2) {
-    double c = 0.0;
-    for (int x = 0; x < nj; ++x) {
-      const int J = items[i0 + x].J;
-      c += sixj(B, ji, jj, J, jk, jl, JP) * sb[x];
-    }
-    if (c == 0.0) continue;
-    const int qx = JP / 2 + Tz2 * (JTM + 1) + 2 * PAR2 * (JTM + 1);
-    double* W = U + B.wcoff[qx];
-    const i64 rr = B.rlen[qx];
-    const int* rJ = B.rmapJ + (i64)(JP >> 1) * N * N;
-    const int* rT = B.rmapJT + (i64)(JP >> 1) * N * N;
-    const int ril = rT[l * N + i], rjk = rJ[j * N + k], gil = rJ[l * N + i], gjk = rJ[k * N + j];
-    wadj(W, rr, gil, rjk, mpm, c);
-    wadj(W, rr, gjk, ril, mpm, phase * c);
+  CV[t] = c;
+}
+
+// terms[tstart[w] .. tstart[w+1]) land in tile w: (lane, bits) with bits
+// 0-11 the position (x - x0) + 64 (y - y0), bit 12 a factor -1 (phase of the
+// second read of a recoupling), bit 13 a factor -Lh Rh (read below the
+// diagonal)
+__global__ __launch_bounds__(256) void k_ph_tile(DevBasis B, const PairTask* tiles, const int* tstart,
+                                                 const int2* terms, const double* CV, double mpm, double* U) {
+  __shared__ double acc[64 * 64];
+  for (int e = threadIdx.x; e < 64 * 64; e += 256) acc[e] = 0.0;
+  __syncthreads();
+  const int e0 = tstart[blockIdx.x], e1 = tstart[blockIdx.x + 1];
+  for (int e = e0 + threadIdx.x; e < e1; e += 256) {
+    const int2 tm = terms[e];
+    double v = CV[tm.x];
+    if (tm.y & 4096) v = -v;
+    if (tm.y & 8192) v *= mpm;
+    atomicAdd(&acc[tm.y & 4095], v);
+  }
+  __syncthreads();
+  const PairTask tk = tiles[blockIdx.x];
+  const int r = B.rlen[tk.q];
+  double* W = U + B.wcoff[tk.q];
+  const int x0 = tk.ti * 64, y0 = tk.tj * 64;
+  for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+    const int x = x0 + (e & 63), y = y0 + (e >> 6);
+    if (x < r && y < r) W[x + (i64)y * r] = acc[e];
   }
 }
 
-void launch_ph_vjp(const DevBasis& B, const QuadItem* items, const int* qstart, int nquads, const double* RESb,
-                   int RESh, int Lh, int Rh, double* U, hipStream_t s) {
-  if (nquads <= 0) return;
-  hipLaunchKernelGGL(k_ph_vjp, dim3((nquads + 255) / 256), dim3(256), 0, s, B, items, qstart, nquads, RESb, RESh,
-                     Lh, Rh, U);
+void launch_ph_vjp(const DevBasis& B, const QuadItem* items, int nitems, const int* lanes, int nl, const int* quads,
+                   const int* qstart, const PairTask* tiles, int ntiles, const int* tstart, const int2* terms,
+                   const double* RESb, int RESh, int Lh, int Rh, double* SB, double* CV, double* U, hipStream_t s) {
+  if (nitems <= 0 || nl <= 0 || ntiles <= 0) return;
+  hipLaunchKernelGGL(k_ph_sb, dim3((nitems + 255) / 256), dim3(256), 0, s, B, items, nitems, RESb, RESh, SB);
+  hipLaunchKernelGGL(k_ph_coef, dim3((nl + 255) / 256), dim3(256), 0, s, B, lanes, nl, quads, qstart, SB, CV);
+  hipLaunchKernelGGL(k_ph_tile, dim3(ntiles), dim3(256), 0, s, B, tiles, tstart, terms, CV, -(double)(Lh * Rh), U);
 }
 
 // Pandya transpose: CCb -> opb (calculate_cross_coupled read positions)
