@@ -60,6 +60,12 @@ struct DevBasis {
   const i64* cccoloff;             // [ncc]
   const double* six;               // 6j table [j1/2][j2/2][j4/2][j5/2][J6/2][J3/2]
   int nh6, nJ6;
+  // many-instance batches: blockIdx.y is the instance of a launch, and each
+  // array of an instance follows the previous instance's at a stride of one
+  // instance's length (oplen, cclen, wclen, N*N, and the lengths below)
+  int ninst;                       // grid.y of batched launches (1: one instance)
+  const int* act;                  // act[inst] == 0 skips the instance; null: all run
+  i64 spart, spscr, sk2scr;        // partial sums, 222_pp_hh split-K scratch (P, K2)
 };
 
 struct Basis {

@@ -527,8 +527,13 @@ int build_plan(imsrg_basis_s* B) {
   if ((rc = upload(B, B->k2ks_h, &B->k2ks))) return rc;
   if ((rc = dalloc(B, std::max<i64>(pofs, 1) * sizeof(double), (void**)&B->Pscr))) return rc;
   if ((rc = dalloc(B, std::max<i64>(k2ofs, 1) * sizeof(double), (void**)&B->K2scr))) return rc;
+  B->D.spscr = std::max<i64>(pofs, 1);
+  B->D.sk2scr = std::max<i64>(k2ofs, 1);
   B->npart = std::max<int>(std::max<int>((int)B->ch_off_h.size(), B->seg_g3.n), b.nblocks + 1);
   if ((rc = dalloc(B, (size_t)B->npart * sizeof(double), (void**)&B->part))) return rc;
+  B->D.spart = B->npart;
+  B->D.ninst = 1;
+  B->D.act = nullptr;
   if ((rc = dalloc(B, (size_t)b.N * b.N * sizeof(double), (void**)&B->M))) return rc;
   if ((rc = dalloc(B, 64 * sizeof(double), (void**)&B->dval))) return rc;
   HIPCHK(hipHostMalloc((void**)&B->hval, 64 * sizeof(double), hipHostMallocDefault));
@@ -590,80 +595,120 @@ void run_gemm(imsrg_basis_s* B, GemmFamily& f, const Slots& sl, hipStream_t s, d
 }
 
 // ------------------------------------------------------------ term bodies
-void t_e0(imsrg_basis_s* B, const double* L, int Lh, const double* R, int Rh, double* dst, hipStream_t s) {
-  launch_e0_partials(B->D, L, Lh, R, Rh, B->part, s);
-  launch_e0_final(B->part, B->b.nblocks, (Lh * Rh == -1) ? 1 : 0, 1, dst, 0, s);
+// The term sequence runs in an execution context: the device basis (whose
+// ninst / act select the instances of batched launches) and the scratch the
+// terms write (partial sums, 222_pp_hh split-K slabs, monopoles, scalars;
+// one copy per instance).  Scalars of instance k sit at dval + k * kDStride.
+constexpr i64 kDStride = 64;
+
+struct Exec {
+  imsrg_basis_s* B = nullptr;
+  DevBasis D{};
+  double *part = nullptr, *Pscr = nullptr, *K2scr = nullptr, *M = nullptr, *dval = nullptr;
+};
+
+Exec basis_exec(imsrg_basis_s* B) {
+  Exec X;
+  X.B = B;
+  X.D = B->D;
+  X.part = B->part; X.Pscr = B->Pscr; X.K2scr = B->K2scr; X.M = B->M; X.dval = B->dval;
+  return X;
 }
-void t_111(imsrg_basis_s* B, const double* L, int Lh, const double* R, int Rh, double* RES, hipStream_t s) {
-  launch_comm111(B->D, L, Lh, R, Rh, RES, s);
+
+// one GEMM family in the context's instances
+void run_gemm_x(const Exec& X, GemmFamily& f, Slots sl, hipStream_t s, double ascale = 1.0, double s2scale = 1.0) {
+  sl.act = X.D.act;
+  if (X.D.ninst > 1 && X.B->path == IMSRG_PATH_GROUPED) {
+    if (f.tasks.n)
+      launch_gemm_tiled(f.tile, sl, f.probs, f.tasks.dev, f.tasks.grid, f.tasks.xmap, s, ascale, s2scale, X.D.ninst);
+    return;
+  }
+  run_gemm(X.B, f, sl, s, ascale, s2scale);
 }
-void t_121(imsrg_basis_s* B, const double* L, int Lh, const double* R, int Rh, double* RES, int RESh,
+
+void t_e0(const Exec& X, const double* L, int Lh, const double* R, int Rh, double* dst, hipStream_t s) {
+  launch_e0_partials(X.D, L, Lh, R, Rh, X.part, s);
+  launch_e0_final_b(X.D, X.part, (Lh * Rh == -1) ? 1 : 0, 1, dst, 0, s);
+}
+void t_111(const Exec& X, const double* L, int Lh, const double* R, int Rh, double* RES, hipStream_t s) {
+  launch_comm111(X.D, L, Lh, R, Rh, RES, s);
+}
+void t_121(const Exec& X, const double* L, int Lh, const double* R, int Rh, double* RES, int RESh,
            hipStream_t s) {
-  launch_comm121(B->D, B->one, (int)B->one_h.size(), L, Lh, R, Rh, RES, RESh, s);
+  launch_comm121(X.D, X.B->one, (int)X.B->one_h.size(), L, Lh, R, Rh, RES, RESh, s);
 }
-void t_122(imsrg_basis_s* B, const double* L, int Lh, const double* R, int Rh, double* RES, int RESh,
+void t_122(const Exec& X, const double* L, int Lh, const double* R, int Rh, double* RES, int RESh,
            hipStream_t s) {
+  imsrg_basis_s* B = X.B;
   run(B, B->seg_tri, [&](const SegTask* t, int n) {
-    launch_comm122p(B->D, t, n, B->partners, B->rowp, B->colp, L, Lh, R, Rh, RES, RESh, s);
+    launch_comm122p(X.D, t, n, B->partners, B->rowp, B->colp, L, Lh, R, Rh, RES, RESh, s);
   });
 }
-void t_pphh(imsrg_basis_s* B, const double* L, int Lh, const double* R, int Rh, double* RES, int RESh,
+void t_pphh(const Exec& X, const double* L, int Lh, const double* R, int Rh, double* RES, int RESh,
             double* w1, double* w2, hipStream_t s, int ref_outputs = 1) {
   (void)RESh;
+  imsrg_basis_s* B = X.B;
+  const i64 op = B->b.oplen;
   Slots s1{};
   s1.p[0] = L; s1.p[1] = R; s1.p[2] = w1;
-  run_gemm(B, B->pp1, s1, s, 1.0, -(double)Rh);
+  s1.st[0] = op; s1.st[1] = op; s1.st[2] = op;
+  run_gemm_x(X, B->pp1, s1, s, 1.0, -(double)Rh);
   run(B, B->pairs,
-      [&](const PairTask* t, int n) { launch_pppp_sym(B->D, t, n, L, Lh, R, Rh, RES, w1, w2, ref_outputs, s); });
+      [&](const PairTask* t, int n) { launch_pppp_sym(X.D, t, n, L, Lh, R, Rh, RES, w1, w2, ref_outputs, s); });
   Slots s2{};
-  s2.p[0] = L; s2.p[1] = R; s2.p[2] = B->K2scr;
-  run_gemm(B, B->k2, s2, s);
+  s2.p[0] = L; s2.p[1] = R; s2.p[2] = X.K2scr;
+  s2.st[0] = op; s2.st[1] = op; s2.st[2] = X.D.sk2scr;
+  run_gemm_x(X, B->k2, s2, s);
   Slots s3{};
-  s3.p[0] = L; s3.p[1] = R; s3.p[2] = B->Pscr;
-  run_gemm(B, B->k3, s3, s);
+  s3.p[0] = L; s3.p[1] = R; s3.p[2] = X.Pscr;
+  s3.st[0] = op; s3.st[1] = op; s3.st[2] = X.D.spscr;
+  run_gemm_x(X, B->k3, s3, s);
   run(B, B->seg_epi, [&](const SegTask* t, int n) {
-    launch_pphh_epilogue(B->D, t, n, L, Lh, R, Rh, RES, w1, w2, B->Pscr, B->poff, B->k3ks, B->K2scr, B->k2off,
+    launch_pphh_epilogue(X.D, t, n, L, Lh, R, Rh, RES, w1, w2, X.Pscr, B->poff, B->k3ks, X.K2scr, B->k2off,
                          B->k2ks, s);
   });
 }
-void t_221(imsrg_basis_s* B, const double* L, int Lh, const double* R, int Rh, double* RES, int RESh,
+void t_221(const Exec& X, const double* L, int Lh, const double* R, int Rh, double* RES, int RESh,
            const double* w1, const double* w2, hipStream_t s) {
-  launch_comm221(B->D, B->one, (int)B->one_h.size(), L, Lh, R, Rh, w1, w2, RES, RESh, s);
+  launch_comm221(X.D, X.B->one, (int)X.B->one_h.size(), L, Lh, R, Rh, w1, w2, RES, RESh, s);
 }
-void t_pandya(imsrg_basis_s* B, const double* op, int herm, double* CC, hipStream_t s) {
-  run(B, B->cc, [&](const CCTask* t, int n) { launch_pandya(B->D, t, n, op, herm, CC, s); });
+void t_pandya(const Exec& X, const double* op, int herm, double* CC, hipStream_t s) {
+  run(X.B, X.B->cc, [&](const CCTask* t, int n) { launch_pandya(X.D, t, n, op, herm, CC, s); });
 }
-void t_ph(imsrg_basis_s* B, const double* LCC, int Lh, const double* RCC, int Rh, double* RES, int RESh,
+void t_ph(const Exec& X, const double* LCC, int Lh, const double* RCC, int Rh, double* RES, int RESh,
           double* WCC, hipStream_t s) {
+  imsrg_basis_s* B = X.B;
   Slots sl{};
   sl.p[0] = LCC; sl.p[1] = RCC; sl.p[2] = WCC;
-  run_gemm(B, B->ph, sl, s, (double)Lh, -(double)(Rh * Lh));
+  sl.st[0] = B->b.cclen; sl.st[1] = B->b.cclen; sl.st[2] = B->b.wclen;
+  run_gemm_x(X, B->ph, sl, s, (double)Lh, -(double)(Rh * Lh));
   if (B->path == IMSRG_PATH_GROUPED)
-    launch_comm222ph_quad(B->D, B->quad, (int)B->quad_h.size(), WCC, Lh, Rh, RES, RESh, s);
+    launch_comm222ph_quad(X.D, B->quad, (int)B->quad_h.size(), WCC, Lh, Rh, RES, RESh, s);
   else
     run(B, B->seg_tri,
-        [&](const SegTask* t, int n) { launch_comm222ph_scatter(B->D, t, n, WCC, Lh, Rh, RES, RESh, s); });
+        [&](const SegTask* t, int n) { launch_comm222ph_scatter(X.D, t, n, WCC, Lh, Rh, RES, RESh, s); });
 }
-// device norm of op into dst (device)
-void t_norm(imsrg_basis_s* B, const double* op, double* dst, hipStream_t s) {
+// device norm of op into dst (device; instance k at dst + k * kDStride)
+void t_norm(const Exec& X, const double* op, double* dst, hipStream_t s) {
+  imsrg_basis_s* B = X.B;
   const int nch = (int)B->ch_off_h.size();
-  launch_norm_partials(op, B->ch_off, B->ch_len, B->ch_w, nch, B->part, s);
-  launch_sum_sqrt(B->part, nch, dst, 1, s);
+  launch_norm_partials_b(X.D, op, B->ch_off, B->ch_len, B->ch_w, nch, X.part, s);
+  launch_sum_sqrt_b(X.D, X.part, nch, dst, kDStride, 1, s);
 }
 
 // RES = [L, R]: the term sequence of BCH_EXPAND / MAGNUS_EXPAND
 // (IMSRG_MAGNUS.f90:207-219, 677-683).  LCC must already hold the
 // cross-coupled L; RCC is computed here.
-void t_full(imsrg_basis_s* B, const double* L, int Lh, const double* R, int Rh, double* RES, int RESh, bool e0,
+void t_full(const Exec& X, const double* L, int Lh, const double* R, int Rh, double* RES, int RESh, bool e0,
             double* w1, double* w2, const double* LCC, double* RCC, double* WCC, hipStream_t s) {
-  t_pandya(B, R, Rh, RCC, s);
-  if (e0) t_e0(B, L, Lh, R, Rh, RES, s);
-  t_111(B, L, Lh, R, Rh, RES, s);
-  t_121(B, L, Lh, R, Rh, RES, RESh, s);
-  t_122(B, L, Lh, R, Rh, RES, RESh, s);
-  t_pphh(B, L, Lh, R, Rh, RES, RESh, w1, w2, s, 0);
-  t_221(B, L, Lh, R, Rh, RES, RESh, w1, w2, s);
-  t_ph(B, LCC, Lh, RCC, Rh, RES, RESh, WCC, s);
+  t_pandya(X, R, Rh, RCC, s);
+  if (e0) t_e0(X, L, Lh, R, Rh, RES, s);
+  t_111(X, L, Lh, R, Rh, RES, s);
+  t_121(X, L, Lh, R, Rh, RES, RESh, s);
+  t_122(X, L, Lh, R, Rh, RES, RESh, s);
+  t_pphh(X, L, Lh, R, Rh, RES, RESh, w1, w2, s, 0);
+  t_221(X, L, Lh, R, Rh, RES, RESh, w1, w2, s);
+  t_ph(X, LCC, Lh, RCC, Rh, RES, RESh, WCC, s);
 }
 
 
@@ -906,6 +951,7 @@ void comm_vjp(imsrg_basis_s* B, const double* L, int Lh, const double* R, int Rh
 // =================================================================== flow
 struct imsrg_flow_s {
   imsrg_basis_s* B = nullptr;
+  Exec X;                                // execution context of the flow's terms
   double *buf[2] = {nullptr, nullptr};   // INT2 / AD ping-pong
   double *w1 = nullptr, *w2 = nullptr, *GCC = nullptr, *ADCC = nullptr, *WCC = nullptr;
   double *int1 = nullptr, *Hbare = nullptr;
@@ -990,7 +1036,7 @@ int bch(imsrg_flow_s* F, double* HS, const double* G, int Gh, const double* H, i
   HIPCHK(hipMemcpyAsync(HS, H, n * sizeof(double), hipMemcpyDeviceToDevice, s));
   HIPCHK(hipMemcpyAsync(F->buf[0], H, n * sizeof(double), hipMemcpyDeviceToDevice, s));
   // G is constant over the expansion: its cross-coupled form once
-  t_pandya(B, G, Gh, F->GCC, s);
+  t_pandya(F->X, G, Gh, F->GCC, s);
   double coef = 1.0;
   int iw;
   for (iw = 2; iw <= maxit; ++iw) {
@@ -1000,9 +1046,9 @@ int bch(imsrg_flow_s* F, double* HS, const double* G, int Gh, const double* H, i
     const double c = coef;
     imsrg_flow_s::GraphKey key{0, iw, HS, G, H, Gh, Hh};
     int rc = run_term(F, key, s, [&](hipStream_t cs) {
-      t_full(B, G, Gh, AD, Hh, INT2, Hh, true, F->w1, F->w2, F->GCC, F->ADCC, F->WCC, cs);
+      t_full(F->X, G, Gh, AD, Hh, INT2, Hh, true, F->w1, F->w2, F->GCC, F->ADCC, F->WCC, cs);
       launch_axpy(n, HS, 1.0, INT2, c, HS, cs);
-      t_norm(B, INT2, B->dval, cs);
+      t_norm(F->X, INT2, B->dval, cs);
     });
     if (rc) return rc;
     double nrm = 0.0;
@@ -1027,15 +1073,15 @@ int magnus(imsrg_flow_s* F, double* DG, int DGh, const double* G, int Gh, double
   // INT2 = DG (both ping-pong buffers carry DG's E0, which no term writes)
   HIPCHK(hipMemcpyAsync(F->buf[0], DG, n * sizeof(double), hipMemcpyDeviceToDevice, s));
   HIPCHK(hipMemcpyAsync(F->buf[1], DG, sizeof(double), hipMemcpyDeviceToDevice, s));
-  t_norm(B, DG, B->dval, s);
-  t_norm(B, G, B->dval + 1, s);
+  t_norm(F->X, DG, B->dval, s);
+  t_norm(F->X, G, B->dval + 1, s);
   HIPCHK(hipMemcpyAsync(B->hval, B->dval, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
   advals[0] = B->hval[0];
   const double fullnorm = B->hval[1];
   if (ncomm) *ncomm = 0;
   if (fullnorm < (double)1e-9f) return 0;
-  t_pandya(B, G, Gh, F->GCC, s);
+  t_pandya(F->X, G, Gh, F->GCC, s);
   int q = 1, cnt = 0;
   for (int i = 2; i <= 7; ++i) {
     double* AD = F->buf[i % 2];
@@ -1046,9 +1092,9 @@ int magnus(imsrg_flow_s* F, double* DG, int DGh, const double* G, int Gh, double
     const double c = cof[i - 1];
     imsrg_flow_s::GraphKey key{1, i, DG, G, nullptr, Gh, DGh};
     rc = run_term(F, key, s, [&](hipStream_t cs) {
-      t_full(B, G, Gh, AD, DGh, INT2, DGh, false, F->w1, F->w2, F->GCC, F->ADCC, F->WCC, cs);
+      t_full(F->X, G, Gh, AD, DGh, INT2, DGh, false, F->w1, F->w2, F->GCC, F->ADCC, F->WCC, cs);
       launch_axpy(n, DG, 1.0, INT2, c, DG, cs);
-      t_norm(B, INT2, B->dval, cs);
+      t_norm(F->X, INT2, B->dval, cs);
     });
     if (rc) return rc;
     ++cnt;
@@ -1060,17 +1106,17 @@ int magnus(imsrg_flow_s* F, double* DG, int DGh, const double* G, int Gh, double
   return 0;
 }
 
-int white(imsrg_basis_s* B, const double* H, double* ETA, hipStream_t s) {
-  launch_monopole(B->D, H, B->M, s);
-  launch_white_1b(B->D, H, ETA, s);
-  launch_white_2b(B->D, B->seg_full.dev, B->seg_full.n, H, B->M, ETA, s);
+int white(const Exec& X, const double* H, double* ETA, hipStream_t s) {
+  launch_monopole(X.D, H, X.M, s);
+  launch_white_1b(X.D, H, ETA, s);
+  launch_white_2b(X.D, X.B->seg_full.dev, X.B->seg_full.n, H, X.M, ETA, s);
   HIPCHK(hipGetLastError());
   return 0;
 }
 
-int mbpt2_dev(imsrg_basis_s* B, const double* H, double* dst, hipStream_t s) {
-  launch_mbpt2_partials(B->D, B->seg_g3.dev, B->seg_g3.n, H, B->part, s);
-  launch_sum_sqrt(B->part, B->seg_g3.n, dst, 0, s);
+int mbpt2_dev(const Exec& X, const double* H, double* dst, hipStream_t s) {
+  launch_mbpt2_partials(X.D, X.B->seg_g3.dev, X.B->seg_g3.n, H, X.part, s);
+  launch_sum_sqrt_b(X.D, X.part, X.B->seg_g3.n, dst, kDStride, 0, s);
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -1082,8 +1128,8 @@ int magnus_step(imsrg_flow_s* F, double* HS, double* G, double* DG, const double
   if ((rc = magnus(F, DG, -1, G, -1, mag_conv, &nm, s))) return rc;
   launch_axpy(B->b.oplen, G, 1.0, DG, ds, G, s);   // euler_step
   if ((rc = bch(F, HS, G, -1, H, 1, bch_conv, 40, &nb, nullptr, s))) return rc;
-  if ((rc = white(B, HS, DG, s))) return rc;
-  if ((rc = mbpt2_dev(B, HS, B->dval + 2, s))) return rc;
+  if ((rc = white(F->X, HS, DG, s))) return rc;
+  if ((rc = mbpt2_dev(F->X, HS, B->dval + 2, s))) return rc;
   HIPCHK(hipMemcpyAsync(B->dval + 3, HS, sizeof(double), hipMemcpyDeviceToDevice, s));
   HIPCHK(hipMemcpyAsync(B->hval, B->dval + 2, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
@@ -1139,16 +1185,16 @@ int bch_record(imsrg_flow_s* F, const double* G, const double* H, int Hh, double
   rec.A.assign(1, H);
   rec.c.assign(1, 1.0);
   if (HS) HIPCHK(hipMemcpyAsync(HS, H, n * sizeof(double), hipMemcpyDeviceToDevice, s));
-  t_pandya(B, G, -1, F->GCC, s);
+  t_pandya(F->X, G, -1, F->GCC, s);
   double coef = 1.0;
   int iw;
   for (iw = 2; iw <= maxit; ++iw) {
     coef = coef / (iw - 1.0);
     double* nx = nullptr;
     if ((rc = pool_get(F, pidx, &nx))) return rc;
-    t_full(B, G, -1, rec.A.back(), Hh, nx, Hh, true, F->w1, F->w2, F->GCC, F->ADCC, F->WCC, s);
+    t_full(F->X, G, -1, rec.A.back(), Hh, nx, Hh, true, F->w1, F->w2, F->GCC, F->ADCC, F->WCC, s);
     if (HS) launch_axpy(n, HS, 1.0, nx, coef, HS, s);
-    t_norm(B, nx, B->dval, s);
+    t_norm(F->X, nx, B->dval, s);
     double nrm = 0.0;
     if ((rc = read_scalar(B, B->dval, &nrm, s))) return rc;
     rec.A.push_back(nx);
@@ -1170,7 +1216,7 @@ int bch_reverse(imsrg_flow_s* F, const double* G, int Hh, const TermRec& rec, co
   const int K = (int)rec.A.size() - 1;
   launch_axpy(n, HSb, rec.c[K], HSb, 0.0, F->Ab, s);
   if (K > 0) {
-    t_pandya(B, G, -1, F->GCC, s);
+    t_pandya(F->X, G, -1, F->GCC, s);
     launch_fill(F->GCCb, B->b.cclen, 0.0, s);
   }
   for (int k = K; k >= 1; --k) {
@@ -1197,15 +1243,15 @@ int magnus_record(imsrg_flow_s* F, const double* ETA, double* DG, const double* 
   rec.A.assign(1, ETA);
   rec.c.assign(1, 1.0);
   if (DG != ETA) HIPCHK(hipMemcpyAsync(DG, ETA, n * sizeof(double), hipMemcpyDeviceToDevice, s));
-  t_norm(B, ETA, B->dval, s);
-  t_norm(B, G, B->dval + 1, s);
+  t_norm(F->X, ETA, B->dval, s);
+  t_norm(F->X, G, B->dval + 1, s);
   HIPCHK(hipMemcpyAsync(B->hval, B->dval, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
   advals[0] = B->hval[0];
   const double fullnorm = B->hval[1];
   if (ncomm) *ncomm = 0;
   if (fullnorm < (double)1e-9f) return 0;
-  t_pandya(B, G, -1, F->GCC, s);
+  t_pandya(F->X, G, -1, F->GCC, s);
   int q = 1;
   for (int i = 2; i <= 7; ++i) {
     const double adnorm = advals[i - q - 1];
@@ -1214,9 +1260,9 @@ int magnus_record(imsrg_flow_s* F, const double* ETA, double* DG, const double* 
     double* nx = nullptr;
     if ((rc = pool_get(F, pidx, &nx))) return rc;
     HIPCHK(hipMemcpyAsync(nx, rec.A.back(), sizeof(double), hipMemcpyDeviceToDevice, s));
-    t_full(B, G, -1, rec.A.back(), -1, nx, -1, false, F->w1, F->w2, F->GCC, F->ADCC, F->WCC, s);
+    t_full(F->X, G, -1, rec.A.back(), -1, nx, -1, false, F->w1, F->w2, F->GCC, F->ADCC, F->WCC, s);
     launch_axpy(n, DG, 1.0, nx, cof[i - 1], DG, s);
-    t_norm(B, nx, B->dval, s);
+    t_norm(F->X, nx, B->dval, s);
     double nrm = 0.0;
     if ((rc = read_scalar(B, B->dval, &nrm, s))) return rc;
     advals[i - 1] = nrm * std::fabs(cof[i - 1]);
@@ -1241,7 +1287,7 @@ int magnus_reverse(imsrg_flow_s* F, const double* G, const TermRec& rec, const d
   launch_axpy(n, DGb, rec.c[K], DGb, 0.0, F->Ab, s);
   launch_fill(F->Ab, 1, 0.0, s);
   if (K > 0) {
-    t_pandya(B, G, -1, F->GCC, s);
+    t_pandya(F->X, G, -1, F->GCC, s);
     launch_fill(F->GCCb, B->b.cclen, 0.0, s);
   }
   for (int k = K; k >= 1; --k) {
@@ -1321,7 +1367,7 @@ int flow_forward(imsrg_flow_s* F, const double* T, double ds, double bch_conv, d
   HIPCHK(hipMemcpyAsync(F->gHS, F->gH, n * sizeof(double), hipMemcpyDeviceToDevice, s));
   launch_fill(F->gG, n, 0.0, s);
   launch_fill(F->gDG, n, 0.0, s);
-  if ((rc = white(B, F->gHS, F->gDG, s)) || (rc = mbpt2_dev(B, F->gHS, B->dval + 2, s))) return rc;
+  if ((rc = white(F->X, F->gHS, F->gDG, s)) || (rc = mbpt2_dev(F->X, F->gHS, B->dval + 2, s))) return rc;
   double crit = 0.0;
   if ((rc = read_scalar(B, B->dval + 2, &crit, s))) return rc;
   crit = std::fabs(crit);
@@ -1335,7 +1381,7 @@ int flow_forward(imsrg_flow_s* F, const double* T, double ds, double bch_conv, d
     if ((rc = magnus(F, F->gDG, -1, F->gG, -1, mag_conv, &nc, s))) return rc;
     launch_axpy(n, F->gG, 1.0, F->gDG, ds, F->gG, s);
     if ((rc = bch(F, F->gHS, F->gG, -1, F->gH, 1, bch_conv, 40, &nt, nullptr, s))) return rc;
-    if ((rc = white(B, F->gHS, F->gDG, s)) || (rc = mbpt2_dev(B, F->gHS, B->dval + 2, s))) return rc;
+    if ((rc = white(F->X, F->gHS, F->gDG, s)) || (rc = mbpt2_dev(F->X, F->gHS, B->dval + 2, s))) return rc;
     if ((rc = read_scalar(B, B->dval + 2, &crit, s))) return rc;
     crit = std::fabs(crit);
     ++steps;
@@ -1376,7 +1422,7 @@ int flow_backward(imsrg_flow_s* F, double* Tb, hipStream_t s) {
       return rc;
     }
     launch_fill(F->segE[0], n, 0.0, s);
-    if ((rc = white(B, F->segH[0], F->segE[0], s))) return rc;
+    if ((rc = white(F->X, F->segH[0], F->segE[0], s))) return rc;
     std::vector<TermRec> mrec(n1 - n0), brec(n1 - n0);
     size_t pidx = 0;
     for (int m = n0; m < n1; ++m) {
@@ -1386,7 +1432,7 @@ int flow_backward(imsrg_flow_s* F, double* Tb, hipStream_t s) {
       if ((rc = bch_record(F, F->segG[k + 1], F->gH, 1, bch_conv, 40, brec[k], pidx, F->segH[k + 1], &nt, s)))
         return rc;
       launch_fill(F->segE[k + 1], n, 0.0, s);
-      if ((rc = white(B, F->segH[k + 1], F->segE[k + 1], s))) return rc;
+      if ((rc = white(F->X, F->segH[k + 1], F->segE[k + 1], s))) return rc;
     }
     for (int m = n1 - 1; m >= n0; --m) {
       const int k = m - n0;
@@ -1570,7 +1616,7 @@ int imsrg_commutator_220(imsrg_basis B, const double* L, int Lh, const double* R
 int imsrg_commutator_e0(imsrg_basis B, const double* L, int Lh, const double* R, int Rh, double* RES,
                         imsrg_stream stream) {
   PREP();
-  t_e0(B, L, Lh, R, Rh, RES, s);
+  t_e0(basis_exec(B), L, Lh, R, Rh, RES, s);
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -1579,7 +1625,7 @@ int imsrg_commutator_111(imsrg_basis B, const double* L, int Lh, const double* R
                          imsrg_stream stream) {
   PREP();
   (void)RESh;
-  t_111(B, L, Lh, R, Rh, RES, s);
+  t_111(basis_exec(B), L, Lh, R, Rh, RES, s);
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -1587,7 +1633,7 @@ int imsrg_commutator_111(imsrg_basis B, const double* L, int Lh, const double* R
 int imsrg_commutator_121(imsrg_basis B, const double* L, int Lh, const double* R, int Rh, double* RES, int RESh,
                          imsrg_stream stream) {
   PREP();
-  t_121(B, L, Lh, R, Rh, RES, RESh, s);
+  t_121(basis_exec(B), L, Lh, R, Rh, RES, RESh, s);
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -1595,7 +1641,7 @@ int imsrg_commutator_121(imsrg_basis B, const double* L, int Lh, const double* R
 int imsrg_commutator_122(imsrg_basis B, const double* L, int Lh, const double* R, int Rh, double* RES, int RESh,
                          imsrg_stream stream) {
   PREP();
-  t_122(B, L, Lh, R, Rh, RES, RESh, s);
+  t_122(basis_exec(B), L, Lh, R, Rh, RES, RESh, s);
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -1603,7 +1649,7 @@ int imsrg_commutator_122(imsrg_basis B, const double* L, int Lh, const double* R
 int imsrg_commutator_222_pp_hh(imsrg_basis B, const double* L, int Lh, const double* R, int Rh, double* RES,
                                int RESh, double* w1, double* w2, imsrg_stream stream) {
   PREP();
-  t_pphh(B, L, Lh, R, Rh, RES, RESh, w1, w2, s);
+  t_pphh(basis_exec(B), L, Lh, R, Rh, RES, RESh, w1, w2, s);
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -1611,14 +1657,14 @@ int imsrg_commutator_222_pp_hh(imsrg_basis B, const double* L, int Lh, const dou
 int imsrg_commutator_221(imsrg_basis B, const double* L, int Lh, const double* R, int Rh, double* RES, int RESh,
                          const double* w1, const double* w2, imsrg_stream stream) {
   PREP();
-  t_221(B, L, Lh, R, Rh, RES, RESh, w1, w2, s);
+  t_221(basis_exec(B), L, Lh, R, Rh, RES, RESh, w1, w2, s);
   HIPCHK(hipGetLastError());
   return 0;
 }
 
 int imsrg_calculate_cross_coupled(imsrg_basis B, const double* op, int herm, double* CC, imsrg_stream stream) {
   PREP();
-  t_pandya(B, op, herm, CC, s);
+  t_pandya(basis_exec(B), op, herm, CC, s);
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -1626,7 +1672,7 @@ int imsrg_calculate_cross_coupled(imsrg_basis B, const double* op, int herm, dou
 int imsrg_commutator_222_ph(imsrg_basis B, const double* LCC, int Lh, const double* RCC, int Rh, double* RES,
                             int RESh, double* WCC, imsrg_stream stream) {
   PREP();
-  t_ph(B, LCC, Lh, RCC, Rh, RES, RESh, WCC, s);
+  t_ph(basis_exec(B), LCC, Lh, RCC, Rh, RES, RESh, WCC, s);
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -1641,19 +1687,19 @@ int imsrg_add_sq_op(imsrg_basis B, const double* A, double ax, const double* Bv,
 
 int imsrg_mat_frob_norm(imsrg_basis B, const double* op, double* value, imsrg_stream stream) {
   PREP();
-  t_norm(B, op, B->dval, s);
+  t_norm(basis_exec(B), op, B->dval, s);
   HIPCHK(hipGetLastError());
   return read_scalar(B, B->dval, value, s);
 }
 
 int imsrg_build_gs_white(imsrg_basis B, const double* H, double* ETA, imsrg_stream stream) {
   PREP();
-  return white(B, H, ETA, s);
+  return white(basis_exec(B), H, ETA, s);
 }
 
 int imsrg_mbpt2(imsrg_basis B, const double* H, double* value, imsrg_stream stream) {
   PREP();
-  int rc = mbpt2_dev(B, H, B->dval, s);
+  int rc = mbpt2_dev(basis_exec(B), H, B->dval, s);
   if (rc) return rc;
   return read_scalar(B, B->dval, value, s);
 }
@@ -1671,6 +1717,7 @@ int imsrg_flow_create(imsrg_basis B, imsrg_flow* out) {
   if (rc) return rc;
   imsrg_flow_s* F = new imsrg_flow_s();
   F->B = B;
+  F->X = basis_exec(B);
   const Basis& b = B->b;
   if ((rc = falloc(F, b.oplen, &F->buf[0])) || (rc = falloc(F, b.oplen, &F->buf[1])) ||
       (rc = falloc(F, b.oplen, &F->w1)) || (rc = falloc(F, b.oplen, &F->w2)) ||
@@ -1731,8 +1778,8 @@ int imsrg_magnus_decouple(imsrg_flow F, double* HS, double* G, double ds, double
   double* DG = F->decDG;
   launch_fill(DG, n, 0.0, s);
   HIPCHK(hipMemcpyAsync(F->Hbare, HS, n * sizeof(double), hipMemcpyDeviceToDevice, s));
-  if ((rc = white(B, HS, DG, s))) return rc;
-  if ((rc = mbpt2_dev(B, HS, B->dval + 2, s))) return rc;
+  if ((rc = white(F->X, HS, DG, s))) return rc;
+  if ((rc = mbpt2_dev(F->X, HS, B->dval + 2, s))) return rc;
   HIPCHK(hipMemcpyAsync(B->dval + 3, HS, sizeof(double), hipMemcpyDeviceToDevice, s));
   HIPCHK(hipMemcpyAsync(B->hval, B->dval + 2, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
@@ -1882,25 +1929,25 @@ int imsrg_profile_kernel(imsrg_basis B, int which, const double* L, int Lh, cons
           run(B, B->seg_tri,
               [&](const SegTask* t, int n) { launch_comm222ph_scatter(B->D, t, n, WCC, Lh, Rh, RES, RESh, s); });
         break;
-      case IMSRG_K_122: t_122(B, L, Lh, R, Rh, RES, RESh, s); break;
-      case IMSRG_K_PANDYA: t_pandya(B, R, Rh, RCC, s); break;
+      case IMSRG_K_122: t_122(basis_exec(B), L, Lh, R, Rh, RES, RESh, s); break;
+      case IMSRG_K_PANDYA: t_pandya(basis_exec(B), R, Rh, RCC, s); break;
       case IMSRG_K_COPY:   // first half of WCC into its second half: wclen/2 doubles each way
         launch_copy(WCC + B->b.wclen / 2, WCC, B->b.wclen / 2, s);
         break;
       case IMSRG_K_ONEBODY:
-        t_111(B, L, Lh, R, Rh, RES, s);
-        t_121(B, L, Lh, R, Rh, RES, RESh, s);
-        t_221(B, L, Lh, R, Rh, RES, RESh, w1, w2, s);
+        t_111(basis_exec(B), L, Lh, R, Rh, RES, s);
+        t_121(basis_exec(B), L, Lh, R, Rh, RES, RESh, s);
+        t_221(basis_exec(B), L, Lh, R, Rh, RES, RESh, w1, w2, s);
         break;
       case IMSRG_K_COMMUTATOR:
-        t_pandya(B, L, Lh, LCC, s);
-        t_full(B, L, Lh, R, Rh, RES, RESh, true, w1, w2, LCC, RCC, WCC, s);
+        t_pandya(basis_exec(B), L, Lh, LCC, s);
+        t_full(basis_exec(B), L, Lh, R, Rh, RES, RESh, true, w1, w2, LCC, RCC, WCC, s);
         break;
     }
   };
   // inputs of every class once (w1/w2, LCC/RCC, WCC), then the timed loop
-  t_pandya(B, L, Lh, LCC, s);
-  t_full(B, L, Lh, R, Rh, RES, RESh, true, w1, w2, LCC, RCC, WCC, s);
+  t_pandya(basis_exec(B), L, Lh, LCC, s);
+  t_full(basis_exec(B), L, Lh, R, Rh, RES, RESh, true, w1, w2, LCC, RCC, WCC, s);
   body(which);
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));

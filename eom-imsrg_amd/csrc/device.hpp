@@ -94,6 +94,17 @@ __device__ __forceinline__ double sixj(const DevBasis& B, int j1, int j2, int j3
 
 // (row, col) of element t of the upper triangle (row <= col) of an n x n
 // block, enumerated column by column.
+// instance of a batched launch; false when it is masked off (the whole
+// workgroup returns together)
+__device__ __forceinline__ bool batch_inst(const DevBasis& B, int& inst) {
+  inst = blockIdx.y;
+  return !(B.act && !B.act[inst]);
+}
+__device__ __forceinline__ bool batch_on(const int* act, int& inst) {
+  inst = blockIdx.y;
+  return !(act && !act[inst]);
+}
+
 __device__ __forceinline__ void upper_decode(i64 t, int& row, int& col) {
   int c = (int)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
   while ((i64)c * (c + 1) / 2 > t) --c;

@@ -120,6 +120,10 @@ __global__ __launch_bounds__(256) void k_gemm_t(Slots sl, const GemmProb* probs,
   __shared__ double Bs[2][GBK][TBN + 1];
   const int ti = xcd_task(xm, blockIdx.x);
   if (ti < 0) return;
+  int inst;
+  if (!batch_on(sl.act, inst)) return;
+  if (inst > 0)
+    for (int x = 0; x < 6; ++x) sl.p[x] += (i64)inst * sl.st[x];
   const GemmTask t = tasks[ti];
   const GemmProb pr = probs[t.prob];
   const int m0 = t.tm * TBM, n0 = t.tn * TBN;
@@ -171,17 +175,17 @@ int tile_rows(int tile) { return tile == 0 ? 64 : 128; }
 int tile_cols(int tile) { return tile == 2 ? 128 : 64; }
 
 void launch_gemm_tiled(int tile, const Slots& sl, const GemmProb* probs, const GemmTask* tasks, int grid,
-                       const XcdMap& xm, hipStream_t s, double ascale, double s2scale) {
+                       const XcdMap& xm, hipStream_t s, double ascale, double s2scale, int ninst) {
   if (grid <= 0) return;
   switch (tile) {
     case 0:
-      hipLaunchKernelGGL((k_gemm_t<64, 64>), dim3(grid), dim3(256), 0, s, sl, probs, tasks, xm, ascale, s2scale);
+      hipLaunchKernelGGL((k_gemm_t<64, 64>), dim3(grid, ninst), dim3(256), 0, s, sl, probs, tasks, xm, ascale, s2scale);
       break;
     case 1:
-      hipLaunchKernelGGL((k_gemm_t<128, 64>), dim3(grid), dim3(256), 0, s, sl, probs, tasks, xm, ascale, s2scale);
+      hipLaunchKernelGGL((k_gemm_t<128, 64>), dim3(grid, ninst), dim3(256), 0, s, sl, probs, tasks, xm, ascale, s2scale);
       break;
     default:
-      hipLaunchKernelGGL((k_gemm_t<128, 128>), dim3(grid), dim3(256), 0, s, sl, probs, tasks, xm, ascale, s2scale);
+      hipLaunchKernelGGL((k_gemm_t<128, 128>), dim3(grid, ninst), dim3(256), 0, s, sl, probs, tasks, xm, ascale, s2scale);
       break;
   }
 }

@@ -17,6 +17,11 @@ namespace imsrg {
 // commutators.f90:283-396.  RES two-body = 1b x 2b part (assigned).
 __global__ __launch_bounds__(256) void k_comm122(DevBasis B, const SegTask* tasks, const double* L, int Lh,
                                                  const double* R, int Rh, double* RES, int RESh) {
+  int inst;
+  if (!batch_inst(B, inst)) return;
+  L += (i64)inst * B.oplen;
+  R += (i64)inst * B.oplen;
+  RES += (i64)inst * B.oplen;
   const SegTask t = tasks[blockIdx.x];
   const i64 e = (i64)t.start + threadIdx.x;
   if (e >= t.cnt) return;
@@ -72,7 +77,7 @@ __global__ __launch_bounds__(256) void k_comm122(DevBasis B, const SegTask* task
 void launch_comm122(const DevBasis& B, const SegTask* t, int nt, const double* L, int Lh, const double* R,
                     int Rh, double* RES, int RESh, hipStream_t s) {
   if (nt <= 0) return;
-  hipLaunchKernelGGL(k_comm122, dim3(nt), dim3(kTile), 0, s, B, t, L, Lh, R, Rh, RES, RESh);
+  hipLaunchKernelGGL(k_comm122, dim3(nt, B.ninst), dim3(kTile), 0, s, B, t, L, Lh, R, Rh, RES, RESh);
 }
 
 // commutator_122 with precomputed partner lists: the same sums as k_comm122
@@ -82,6 +87,11 @@ __device__ __forceinline__ int blk_of(int lo, int hi) { return lo == 0 ? hi : (l
 __global__ __launch_bounds__(256) void k_comm122p(DevBasis B, const SegTask* tasks, const Partner* part,
                                                   const int* rowp, const int* colp, const double* L, int Lh,
                                                   const double* R, int Rh, double* RES, int RESh) {
+  int inst;
+  if (!batch_inst(B, inst)) return;
+  L += (i64)inst * B.oplen;
+  R += (i64)inst * B.oplen;
+  RES += (i64)inst * B.oplen;
   const SegTask t = tasks[blockIdx.x];
   const i64 e = (i64)t.start + threadIdx.x;
   if (e >= t.cnt) return;
@@ -147,7 +157,7 @@ void launch_comm122p(const DevBasis& B, const SegTask* t, int nt, const Partner*
                      const int* colp, const double* L, int Lh, const double* R, int Rh, double* RES, int RESh,
                      hipStream_t s) {
   if (nt <= 0) return;
-  hipLaunchKernelGGL(k_comm122p, dim3(nt), dim3(kTile), 0, s, B, t, part, rowp, colp, L, Lh, R, Rh, RES, RESh);
+  hipLaunchKernelGGL(k_comm122p, dim3(nt, B.ninst), dim3(kTile), 0, s, B, t, part, rowp, colp, L, Lh, R, Rh, RES, RESh);
 }
 
 // ================================================== commutator_222_ph scatter
@@ -225,6 +235,10 @@ __device__ __forceinline__ void ph_element(const DevBasis& B, const double* WCC,
 // per-channel form: segment tiles of one channel (plain path)
 __global__ __launch_bounds__(256) void k_comm222ph_scatter(DevBasis B, const SegTask* tasks, const double* WCC,
                                                            int Lh, int Rh, double* RES, int RESh) {
+  int inst;
+  if (!batch_inst(B, inst)) return;
+  WCC += (i64)inst * B.wclen;
+  RES += (i64)inst * B.oplen;
   const SegTask t = tasks[blockIdx.x];
   const i64 e = (i64)t.start + threadIdx.x;
   if (e >= t.cnt) return;
@@ -242,7 +256,7 @@ __global__ __launch_bounds__(256) void k_comm222ph_scatter(DevBasis B, const Seg
 void launch_comm222ph_scatter(const DevBasis& B, const SegTask* t, int nt, const double* W, int Lh, int Rh,
                               double* RES, int RESh, hipStream_t s) {
   if (nt <= 0) return;
-  hipLaunchKernelGGL(k_comm222ph_scatter, dim3(nt), dim3(kTile), 0, s, B, t, W, Lh, Rh, RES, RESh);
+  hipLaunchKernelGGL(k_comm222ph_scatter, dim3(nt, B.ninst), dim3(kTile), 0, s, B, t, W, Lh, Rh, RES, RESh);
 }
 
 // grouped form: one thread per (orbit quadruple, J), quadruple-major with
@@ -250,6 +264,10 @@ void launch_comm222ph_scatter(const DevBasis& B, const SegTask* t, int nt, const
 // (one request per wave load) and neighbouring 6j symbols.
 __global__ __launch_bounds__(256) void k_comm222ph_quad(DevBasis B, const QuadItem* items, int n, const double* WCC,
                                                         int Lh, int Rh, double* RES, int RESh) {
+  int inst;
+  if (!batch_inst(B, inst)) return;
+  WCC += (i64)inst * B.wclen;
+  RES += (i64)inst * B.oplen;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
   const QuadItem it = items[t];
@@ -267,7 +285,7 @@ __global__ __launch_bounds__(256) void k_comm222ph_quad(DevBasis B, const QuadIt
 void launch_comm222ph_quad(const DevBasis& B, const QuadItem* items, int n, const double* W, int Lh, int Rh,
                            double* RES, int RESh, hipStream_t s) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_comm222ph_quad, dim3((n + 255) / 256), dim3(256), 0, s, B, items, n, W, Lh, Rh, RES, RESh);
+  hipLaunchKernelGGL(k_comm222ph_quad, dim3((n + 255) / 256, B.ninst), dim3(256), 0, s, B, items, n, W, Lh, Rh, RES, RESh);
 }
 
 // =========================================================== Pandya forward
@@ -275,6 +293,10 @@ void launch_comm222ph_quad(const DevBasis& B, const QuadItem* items, int n, cons
 // CCX(q1)(r, nb) element; r = ordered pair (a,b), nb = (p,h).
 __global__ __launch_bounds__(256) void k_pandya(DevBasis B, const CCTask* tasks, const double* op, int herm,
                                                 double* CC) {
+  int inst;
+  if (!batch_inst(B, inst)) return;
+  op += (i64)inst * B.oplen;
+  CC += (i64)inst * B.cclen;
   const CCTask t = tasks[blockIdx.x];
   const int q1 = t.q1;
   const int r = B.rlen[q1], nc = B.ncol[q1];
@@ -304,13 +326,18 @@ __global__ __launch_bounds__(256) void k_pandya(DevBasis B, const CCTask* tasks,
 void launch_pandya(const DevBasis& B, const CCTask* t, int nt, const double* op, int herm, double* CC,
                    hipStream_t s) {
   if (nt <= 0) return;
-  hipLaunchKernelGGL(k_pandya, dim3(nt), dim3(kTile), 0, s, B, t, op, herm, CC);
+  hipLaunchKernelGGL(k_pandya, dim3(nt, B.ninst), dim3(kTile), 0, s, B, t, op, herm, CC);
 }
 
 // ============================================================ commutator_111
 // commutators.f90:36-80.  One thread per one-body output; assigns.
 __global__ __launch_bounds__(256) void k_comm111(DevBasis B, const double* L, int Lh, const double* R, int Rh,
                                                  double* RES) {
+  int inst;
+  if (!batch_inst(B, inst)) return;
+  L += (i64)inst * B.oplen;
+  R += (i64)inst * B.oplen;
+  RES += (i64)inst * B.oplen;
   const int A = B.A, P = B.P;
   const i64 idx = (i64)blockIdx.x * blockDim.x + threadIdx.x;
   const double *Lhh = L + B.off_fhh, *Lph = L + B.off_fph, *Lpp = L + B.off_fpp;
@@ -364,7 +391,7 @@ void launch_comm111(const DevBasis& B, const double* L, int Lh, const double* R,
                     hipStream_t s) {
   const i64 n = (i64)B.A * B.A + (i64)B.P * B.A + (i64)B.P * B.P;
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_comm111, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, B, L, Lh, R, Rh, RES);
+  hipLaunchKernelGGL(k_comm111, dim3((unsigned)((n + 255) / 256), B.ninst), dim3(256), 0, s, B, L, Lh, R, Rh, RES);
 }
 
 __device__ double block_sum(double v, double* red);
@@ -394,6 +421,11 @@ __device__ __forceinline__ void onebody_update(const DevBasis& B, const OneTask&
 // its threads split the (particle a, hole i) intermediates, tree-reduced.
 __global__ __launch_bounds__(128) void k_comm121(DevBasis B, const OneTask* tasks, const double* L, int Lh,
                                                  const double* R, int Rh, double* RES, int RESh) {
+  int inst;
+  if (!batch_inst(B, inst)) return;
+  L += (i64)inst * B.oplen;
+  R += (i64)inst * B.oplen;
+  RES += (i64)inst * B.oplen;
   __shared__ double red[128];
   const OneTask t = tasks[blockIdx.x];
   const int pk = t.pk, qk = t.qk, jq = B.jj[qk];
@@ -423,7 +455,7 @@ __global__ __launch_bounds__(128) void k_comm121(DevBasis B, const OneTask* task
 void launch_comm121(const DevBasis& B, const OneTask* t, int nt, const double* L, int Lh, const double* R,
                     int Rh, double* RES, int RESh, hipStream_t s) {
   if (nt <= 0) return;
-  hipLaunchKernelGGL(k_comm121, dim3(nt), dim3(128), 0, s, B, t, L, Lh, R, Rh, RES, RESh);
+  hipLaunchKernelGGL(k_comm121, dim3(nt, B.ninst), dim3(128), 0, s, B, t, L, Lh, R, Rh, RES, RESh);
 }
 
 // ============================================================ commutator_221
@@ -451,6 +483,13 @@ __device__ __forceinline__ double w2_elem(const DevBasis& B, const double* w2, c
 __global__ __launch_bounds__(128) void k_comm221(DevBasis B, const OneTask* tasks, const double* L, int Lh,
                                                  const double* R, int Rh, const double* w1, const double* w2,
                                                  double* RES, int RESh) {
+  int inst;
+  if (!batch_inst(B, inst)) return;
+  L += (i64)inst * B.oplen;
+  R += (i64)inst * B.oplen;
+  w1 += (i64)inst * B.oplen;
+  w2 += (i64)inst * B.oplen;
+  RES += (i64)inst * B.oplen;
   __shared__ double red[128];
   const OneTask t = tasks[blockIdx.x];
   const int ik = t.pk, jk = t.qk, ji = B.jj[ik];
@@ -483,7 +522,7 @@ __global__ __launch_bounds__(128) void k_comm221(DevBasis B, const OneTask* task
 void launch_comm221(const DevBasis& B, const OneTask* t, int nt, const double* L, int Lh, const double* R, int Rh,
                     const double* w1, const double* w2, double* RES, int RESh, hipStream_t s) {
   if (nt <= 0) return;
-  hipLaunchKernelGGL(k_comm221, dim3(nt), dim3(128), 0, s, B, t, L, Lh, R, Rh, w1, w2, RES, RESh);
+  hipLaunchKernelGGL(k_comm221, dim3(nt, B.ninst), dim3(128), 0, s, B, t, L, Lh, R, Rh, w1, w2, RES, RESh);
 }
 
 // ================================================= commutator_110 + 220 (E0)
@@ -504,6 +543,11 @@ __device__ double block_sum(double v, double* red) {
 
 __global__ __launch_bounds__(256) void k_e0_partials(DevBasis B, const double* L, int Lh, const double* R, int Rh,
                                                      double* part) {
+  int inst;
+  if (!batch_inst(B, inst)) return;
+  L += (i64)inst * B.oplen;
+  R += (i64)inst * B.oplen;
+  part += (i64)inst * B.spart;
   __shared__ double red[256];
   const int q = blockIdx.x;
   double v = 0.0;
@@ -528,8 +572,12 @@ __global__ __launch_bounds__(256) void k_e0_partials(DevBasis B, const double* L
 
 // dst = [use110] part[n] + [use220] 2 * sum(part[0..n-1])   (one workgroup)
 __global__ __launch_bounds__(256) void k_e0_final(const double* part, int n, int use220, int use110, double* dst,
-                                                  int accumulate) {
+                                                  int accumulate, i64 spart, i64 sdst, const int* act) {
   __shared__ double red[256];
+  int inst;
+  if (!batch_on(act, inst)) return;
+  part += (i64)inst * spart;
+  dst += (i64)inst * sdst;
   double v = 0.0;
   for (int x = threadIdx.x; x < n; x += blockDim.x) v += part[x];
   const double s = block_sum(v, red);
@@ -543,12 +591,18 @@ __global__ __launch_bounds__(256) void k_e0_final(const double* part, int n, int
 
 void launch_e0_partials(const DevBasis& B, const double* L, int Lh, const double* R, int Rh, double* part,
                         hipStream_t s) {
-  hipLaunchKernelGGL(k_e0_partials, dim3(B.nblocks + 1), dim3(256), 0, s, B, L, Lh, R, Rh, part);
+  hipLaunchKernelGGL(k_e0_partials, dim3(B.nblocks + 1, B.ninst), dim3(256), 0, s, B, L, Lh, R, Rh, part);
 }
 
 void launch_e0_final(const double* part, int nblocks, int use220, int use110, double* dst, int accumulate,
                      hipStream_t s) {
-  hipLaunchKernelGGL(k_e0_final, dim3(1), dim3(256), 0, s, part, nblocks, use220, use110, dst, accumulate);
+  hipLaunchKernelGGL(k_e0_final, dim3(1), dim3(256), 0, s, part, nblocks, use220, use110, dst, accumulate, (i64)0,
+                     (i64)0, (const int*)nullptr);
+}
+void launch_e0_final_b(const DevBasis& B, const double* part, int use220, int use110, double* dst, int accumulate,
+                       hipStream_t s) {
+  hipLaunchKernelGGL(k_e0_final, dim3(1, B.ninst), dim3(256), 0, s, part, B.nblocks, use220, use110, dst, accumulate,
+                     B.spart, B.oplen, B.act);
 }
 
 // ======================================== 222_pp_hh pppp block epilogue
@@ -566,6 +620,13 @@ __device__ __forceinline__ double w2pp(const double* L3, const double* R3, int n
 __global__ __launch_bounds__(256) void k_pppp_sym(DevBasis B, const PairTask* tasks, const double* L, int Lh,
                                                   const double* R, int Rh, double* RES, double* w1, double* w2,
                                                   int ref_outputs) {
+  int inst;
+  if (!batch_inst(B, inst)) return;
+  L += (i64)inst * B.oplen;
+  R += (i64)inst * B.oplen;
+  RES += (i64)inst * B.oplen;
+  w1 += (i64)inst * B.oplen;
+  w2 += (i64)inst * B.oplen;
   __shared__ double T1[64][65];   // C(I0 + x, J0 + y)
   __shared__ double T2[64][65];   // C(J0 + x, I0 + y)
   const PairTask t = tasks[blockIdx.x];
@@ -619,7 +680,7 @@ __global__ __launch_bounds__(256) void k_pppp_sym(DevBasis B, const PairTask* ta
 void launch_pppp_sym(const DevBasis& B, const PairTask* t, int nt, const double* L, int Lh, const double* R, int Rh,
                      double* RES, double* w1, double* w2, int ref_outputs, hipStream_t s) {
   if (nt <= 0) return;
-  hipLaunchKernelGGL(k_pppp_sym, dim3(nt), dim3(256), 0, s, B, t, L, Lh, R, Rh, RES, w1, w2, ref_outputs);
+  hipLaunchKernelGGL(k_pppp_sym, dim3(nt, B.ninst), dim3(256), 0, s, B, t, L, Lh, R, Rh, RES, w1, w2, ref_outputs);
 }
 
 // ================================ 222_pp_hh epilogue of the small blocks
@@ -639,6 +700,15 @@ __global__ __launch_bounds__(256) void k_pphh_epilogue(DevBasis B, const SegTask
                                                        double* w2, const double* Pscr, const i64* poff,
                                                        const int* k3ks, const double* K2scr, const i64* k2off,
                                                        const int* k2ks) {
+  int inst;
+  if (!batch_inst(B, inst)) return;
+  L += (i64)inst * B.oplen;
+  R += (i64)inst * B.oplen;
+  RES += (i64)inst * B.oplen;
+  w1 += (i64)inst * B.oplen;
+  w2 += (i64)inst * B.oplen;
+  Pscr += (i64)inst * B.spscr;
+  K2scr += (i64)inst * B.sk2scr;
   const SegTask t = tasks[blockIdx.x];
   const i64 e = (i64)t.start + threadIdx.x;
   if (e >= t.cnt) return;
@@ -730,24 +800,20 @@ void launch_pphh_epilogue(const DevBasis& B, const SegTask* t, int nt, const dou
                           int Rh, double* RES, double* w1, double* w2, const double* Pscr, const i64* poff,
                           const int* k3ks, const double* K2scr, const i64* k2off, const int* k2ks, hipStream_t s) {
   if (nt <= 0) return;
-  hipLaunchKernelGGL(k_pphh_epilogue, dim3(nt), dim3(256), 0, s, B, t, L, Lh, R, Rh, RES, w1, w2, Pscr, poff, k3ks,
+  hipLaunchKernelGGL(k_pphh_epilogue, dim3(nt, B.ninst), dim3(256), 0, s, B, t, L, Lh, R, Rh, RES, w1, w2, Pscr, poff, k3ks,
                      K2scr, k2off, k2ks);
 }
 
 // ================================================ streaming algebra / norm
 // add_sq_op, basic_IMSRG.f90:3551-3583:  C = A*ax + B*bx
 __global__ __launch_bounds__(256) void k_axpy(i64 n, const double* A, double ax, const double* Bv, double bx,
-                                              double* C) {
+                                              double* C, const int* act) {
+  int inst;
+  if (!batch_on(act, inst)) return;
+  A += (i64)inst * n; Bv += (i64)inst * n; C += (i64)inst * n;
   for (i64 x = (i64)blockIdx.x * blockDim.x + threadIdx.x; x < n; x += (i64)gridDim.x * blockDim.x)
     C[x] = A[x] * ax + Bv[x] * bx;
 }
-__global__ __launch_bounds__(256) void k_axpy_dev(i64 n, const double* A, const double* Bv, const double* coef,
-                                                  double* C) {
-  const double bx = coef[0];
-  for (i64 x = (i64)blockIdx.x * blockDim.x + threadIdx.x; x < n; x += (i64)gridDim.x * blockDim.x)
-    C[x] = A[x] * 1.0 + Bv[x] * bx;
-}
-
 static unsigned stream_grid(i64 n) {
   i64 g = (n + 255) / 256;
   if (g > 8192) g = 8192;
@@ -756,17 +822,25 @@ static unsigned stream_grid(i64 n) {
 }
 
 void launch_axpy(i64 n, const double* A, double ax, const double* Bv, double bx, double* C, hipStream_t s) {
-  hipLaunchKernelGGL(k_axpy, dim3(stream_grid(n)), dim3(256), 0, s, n, A, ax, Bv, bx, C);
+  hipLaunchKernelGGL(k_axpy, dim3(stream_grid(n)), dim3(256), 0, s, n, A, ax, Bv, bx, C, (const int*)nullptr);
 }
-void launch_axpy_devcoef(i64 n, const double* A, const double* Bv, const double* coef, double* C,
-                         hipStream_t s) {
-  hipLaunchKernelGGL(k_axpy_dev, dim3(stream_grid(n)), dim3(256), 0, s, n, A, Bv, coef, C);
+// the same on ninst instances back to back (stride n), act-masked
+void launch_axpy_b(i64 n, const double* A, double ax, const double* Bv, double bx, double* C, int ninst,
+                   const int* act, hipStream_t s) {
+  unsigned g = stream_grid(n);
+  if (ninst > 1 && g > 1024) g = 1024;
+  hipLaunchKernelGGL(k_axpy, dim3(g, ninst), dim3(256), 0, s, n, A, ax, Bv, bx, C, act);
 }
 
 // mat_frob_norm, basic_IMSRG.f90:4056-4087: weighted sum of squares per chunk
 __global__ __launch_bounds__(256) void k_norm_partials(const double* op, const i64* coff, const int* clen,
-                                                       const double* cw, double* part) {
+                                                       const double* cw, double* part, i64 sop, i64 spart,
+                                                       const int* act) {
   __shared__ double red[256];
+  int inst;
+  if (!batch_on(act, inst)) return;
+  op += (i64)inst * sop;
+  part += (i64)inst * spart;
   const int c = blockIdx.x;
   const double* x = op + coff[c];
   const int n = clen[c];
@@ -775,8 +849,13 @@ __global__ __launch_bounds__(256) void k_norm_partials(const double* op, const i
   const double s = block_sum(v, red);
   if (threadIdx.x == 0) part[c] = s * cw[c];
 }
-__global__ __launch_bounds__(1024) void k_sum_sqrt(const double* part, int n, double* dst, int do_sqrt) {
+__global__ __launch_bounds__(1024) void k_sum_sqrt(const double* part, int n, double* dst, int do_sqrt, i64 spart,
+                                                   i64 sdst, const int* act) {
   __shared__ double red[1024];
+  int inst;
+  if (!batch_on(act, inst)) return;
+  part += (i64)inst * spart;
+  dst += (i64)inst * sdst;
   double v = 0.0;
   for (int i = threadIdx.x; i < n; i += blockDim.x) v += part[i];
   const double s = block_sum(v, red);
@@ -786,15 +865,31 @@ __global__ __launch_bounds__(1024) void k_sum_sqrt(const double* part, int n, do
 void launch_norm_partials(const double* op, const i64* chunk_off, const int* chunk_len, const double* chunk_w,
                           int nchunk, double* part, hipStream_t s) {
   if (nchunk <= 0) return;
-  hipLaunchKernelGGL(k_norm_partials, dim3(nchunk), dim3(256), 0, s, op, chunk_off, chunk_len, chunk_w, part);
+  hipLaunchKernelGGL(k_norm_partials, dim3(nchunk), dim3(256), 0, s, op, chunk_off, chunk_len, chunk_w, part, (i64)0,
+                     (i64)0, (const int*)nullptr);
 }
 void launch_sum_sqrt(const double* part, int n, double* dst, int do_sqrt, hipStream_t s) {
-  hipLaunchKernelGGL(k_sum_sqrt, dim3(1), dim3(1024), 0, s, part, n, dst, do_sqrt);
+  hipLaunchKernelGGL(k_sum_sqrt, dim3(1), dim3(1024), 0, s, part, n, dst, do_sqrt, (i64)0, (i64)0,
+                     (const int*)nullptr);
+}
+void launch_norm_partials_b(const DevBasis& B, const double* op, const i64* chunk_off, const int* chunk_len,
+                            const double* chunk_w, int nchunk, double* part, hipStream_t s) {
+  if (nchunk <= 0) return;
+  hipLaunchKernelGGL(k_norm_partials, dim3(nchunk, B.ninst), dim3(256), 0, s, op, chunk_off, chunk_len, chunk_w, part,
+                     B.oplen, B.spart, B.act);
+}
+void launch_sum_sqrt_b(const DevBasis& B, const double* part, int n, double* dst, i64 sdst, int do_sqrt,
+                       hipStream_t s) {
+  hipLaunchKernelGGL(k_sum_sqrt, dim3(1, B.ninst), dim3(1024), 0, s, part, n, dst, do_sqrt, B.spart, sdst, B.act);
 }
 
 // ======================================================== white generator
 // twobody_monopole, basic_IMSRG.f90:2075-2090, for every orbit pair
 __global__ __launch_bounds__(256) void k_monopole(DevBasis B, const double* H, double* M) {
+  int inst;
+  if (!batch_inst(B, inst)) return;
+  H += (i64)inst * B.oplen;
+  M += (i64)inst * B.N * B.N;
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= B.N * B.N) return;
   const int a = idx % B.N, b = idx / B.N;
@@ -804,11 +899,15 @@ __global__ __launch_bounds__(256) void k_monopole(DevBasis B, const double* H, d
   M[idx] = sm / (ja + 1.0) / (jb + 1.0);
 }
 void launch_monopole(const DevBasis& B, const double* H, double* M, hipStream_t s) {
-  hipLaunchKernelGGL(k_monopole, dim3((B.N * B.N + 255) / 256), dim3(256), 0, s, B, H, M);
+  hipLaunchKernelGGL(k_monopole, dim3((B.N * B.N + 255) / 256, B.ninst), dim3(256), 0, s, B, H, M);
 }
 
 // build_gs_white one-body part, generators.f90:19-55 (also zeroes fhh, fpp)
 __global__ __launch_bounds__(256) void k_white_1b(DevBasis B, const double* H, double* ETA) {
+  int inst;
+  if (!batch_inst(B, inst)) return;
+  H += (i64)inst * B.oplen;
+  ETA += (i64)inst * B.oplen;
   const i64 idx = (i64)blockIdx.x * blockDim.x + threadIdx.x;
   const i64 nA = (i64)B.A * B.A, nPA = (i64)B.P * B.A, nP = (i64)B.P * B.P;
   if (idx < nA) { ETA[B.off_fhh + idx] = 0.0; return; }
@@ -827,12 +926,17 @@ __global__ __launch_bounds__(256) void k_white_1b(DevBasis B, const double* H, d
 }
 void launch_white_1b(const DevBasis& B, const double* H, double* ETA, hipStream_t s) {
   const i64 n = (i64)B.A * B.A + (i64)B.P * B.P + (i64)B.P * B.A;
-  hipLaunchKernelGGL(k_white_1b, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, B, H, ETA);
+  hipLaunchKernelGGL(k_white_1b, dim3((unsigned)((n + 255) / 256), B.ninst), dim3(256), 0, s, B, H, ETA);
 }
 
 // build_gs_white two-body part, generators.f90:57-104: full-segment tasks
 __global__ __launch_bounds__(256) void k_white_2b(DevBasis B, const SegTask* tasks, const double* H,
                                                   const double* M, double* ETA) {
+  int inst;
+  if (!batch_inst(B, inst)) return;
+  H += (i64)inst * B.oplen;
+  ETA += (i64)inst * B.oplen;
+  M += (i64)inst * B.N * B.N;
   const SegTask t = tasks[blockIdx.x];
   const i64 e = (i64)t.start + threadIdx.x;
   if (e >= t.cnt) return;
@@ -858,12 +962,16 @@ __global__ __launch_bounds__(256) void k_white_2b(DevBasis B, const SegTask* tas
 void launch_white_2b(const DevBasis& B, const SegTask* t, int nt, const double* H, const double* M, double* ETA,
                      hipStream_t s) {
   if (nt <= 0) return;
-  hipLaunchKernelGGL(k_white_2b, dim3(nt), dim3(256), 0, s, B, t, H, M, ETA);
+  hipLaunchKernelGGL(k_white_2b, dim3(nt, B.ninst), dim3(256), 0, s, B, t, H, M, ETA);
 }
 
 // mbpt2, basic_IMSRG.f90:4165-4199: one partial per gam3 tile
 __global__ __launch_bounds__(256) void k_mbpt2_partials(DevBasis B, const SegTask* tasks, const double* H,
                                                         double* part) {
+  int inst;
+  if (!batch_inst(B, inst)) return;
+  H += (i64)inst * B.oplen;
+  part += (i64)inst * B.spart;
   __shared__ double red[256];
   const SegTask t = tasks[blockIdx.x];
   const i64 e = (i64)t.start + threadIdx.x;
@@ -887,7 +995,7 @@ __global__ __launch_bounds__(256) void k_mbpt2_partials(DevBasis B, const SegTas
 void launch_mbpt2_partials(const DevBasis& B, const SegTask* t, int nt, const double* H, double* part,
                            hipStream_t s) {
   if (nt <= 0) return;
-  hipLaunchKernelGGL(k_mbpt2_partials, dim3(nt), dim3(256), 0, s, B, t, H, part);
+  hipLaunchKernelGGL(k_mbpt2_partials, dim3(nt, B.ninst), dim3(256), 0, s, B, t, H, part);
 }
 
 // ============================================================ normal_order
@@ -950,19 +1058,38 @@ void launch_normal_order(const DevBasis& B, double* H, double* scratch, hipStrea
 }
 
 // ================================================================ helpers
-__global__ void k_fill(double* p, i64 n, double v) {
+__global__ void k_fill(double* p, i64 n, double v, const int* act) {
+  int inst;
+  if (!batch_on(act, inst)) return;
+  p += (i64)inst * n;
   for (i64 x = (i64)blockIdx.x * blockDim.x + threadIdx.x; x < n; x += (i64)gridDim.x * blockDim.x) p[x] = v;
 }
-__global__ void k_copy(double* d, const double* s, i64 n) {
+__global__ void k_copy(double* d, const double* s, i64 n, const int* act) {
+  int inst;
+  if (!batch_on(act, inst)) return;
+  d += (i64)inst * n;
+  s += (i64)inst * n;
   for (i64 x = (i64)blockIdx.x * blockDim.x + threadIdx.x; x < n; x += (i64)gridDim.x * blockDim.x) d[x] = s[x];
 }
 void launch_fill(double* p, i64 n, double v, hipStream_t s) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_fill, dim3(stream_grid(n)), dim3(256), 0, s, p, n, v);
+  hipLaunchKernelGGL(k_fill, dim3(stream_grid(n)), dim3(256), 0, s, p, n, v, (const int*)nullptr);
 }
 void launch_copy(double* dst, const double* src, i64 n, hipStream_t s) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_copy, dim3(stream_grid(n)), dim3(256), 0, s, dst, src, n);
+  hipLaunchKernelGGL(k_copy, dim3(stream_grid(n)), dim3(256), 0, s, dst, src, n, (const int*)nullptr);
+}
+void launch_fill_b(double* p, i64 n, double v, int ninst, const int* act, hipStream_t s) {
+  if (n <= 0) return;
+  unsigned g = stream_grid(n);
+  if (ninst > 1 && g > 1024) g = 1024;
+  hipLaunchKernelGGL(k_fill, dim3(g, ninst), dim3(256), 0, s, p, n, v, act);
+}
+void launch_copy_b(double* dst, const double* src, i64 n, int ninst, const int* act, hipStream_t s) {
+  if (n <= 0) return;
+  unsigned g = stream_grid(n);
+  if (ninst > 1 && g > 1024) g = 1024;
+  hipLaunchKernelGGL(k_copy, dim3(g, ninst), dim3(256), 0, s, dst, src, n, act);
 }
 
 }  // namespace imsrg

@@ -34,7 +34,13 @@ struct CCTask { int q1; int start; };
 
 // Operand table of a grouped GEMM launch: problems address operands as
 // (slot, offset) so one problem list serves every call.
-struct Slots { const double* p[6]; };
+// operand slots of a GEMM launch; st[x] is slot x's stride between the
+// instances of a batched launch (grid.y), act the instance mask (null: all)
+struct Slots {
+  const double* p[6];
+  i64 st[6];
+  const int* act;
+};
 
 // C(slotC) = alpha * (op(A1) op(B1) + s2 * op(A2) op(B2)) + beta * C
 struct GemmProb {
@@ -109,7 +115,7 @@ void launch_e0_final(const double* part, int nblocks, int use220, int use110, do
 int tile_rows(int tile);
 int tile_cols(int tile);
 void launch_gemm_tiled(int tile, const Slots& sl, const GemmProb* probs, const GemmTask* tasks, int grid,
-                       const XcdMap& xm, hipStream_t s, double ascale = 1.0, double s2scale = 1.0);
+                       const XcdMap& xm, hipStream_t s, double ascale = 1.0, double s2scale = 1.0, int ninst = 1);
 // RES1 += C - pm C^T with C = L1 R1 - Rh L3 R3^T held in w1's pppp blocks;
 // ref_outputs: also leave the reference's W1 = L1 R1 and W2 = Rh L3 R3^T
 // in w1 / w2 (otherwise w1's pppp blocks keep C and w2's are untouched)
@@ -119,10 +125,22 @@ void launch_pphh_epilogue(const DevBasis& B, const SegTask* t, int nt, const dou
                           int Rh, double* RES, double* w1, double* w2, const double* Pscr, const i64* poff,
                           const int* k3ks, const double* K2scr, const i64* k2off, const int* k2ks, hipStream_t s);
 void launch_axpy(i64 n, const double* A, double ax, const double* B, double bx, double* C, hipStream_t s);
-void launch_axpy_devcoef(i64 n, const double* A, const double* B, const double* coef, double* C, hipStream_t s);
 void launch_norm_partials(const double* op, const i64* chunk_off, const int* chunk_len, const double* chunk_w,
                           int nchunk, double* part, hipStream_t s);
 void launch_sum_sqrt(const double* part, int n, double* dst, int do_sqrt, hipStream_t s);
+// many-instance forms: B.ninst instances back to back, masked by B.act
+// (axpy / fill / copy: stride n; norms: operand stride oplen, partials
+// stride spart, results stride sdst; e0_final: partials spart, dst oplen)
+void launch_axpy_b(i64 n, const double* A, double ax, const double* Bv, double bx, double* C, int ninst,
+                   const int* act, hipStream_t s);
+void launch_fill_b(double* p, i64 n, double v, int ninst, const int* act, hipStream_t s);
+void launch_copy_b(double* dst, const double* src, i64 n, int ninst, const int* act, hipStream_t s);
+void launch_norm_partials_b(const DevBasis& B, const double* op, const i64* chunk_off, const int* chunk_len,
+                            const double* chunk_w, int nchunk, double* part, hipStream_t s);
+void launch_sum_sqrt_b(const DevBasis& B, const double* part, int n, double* dst, i64 sdst, int do_sqrt,
+                       hipStream_t s);
+void launch_e0_final_b(const DevBasis& B, const double* part, int use220, int use110, double* dst, int accumulate,
+                       hipStream_t s);
 void launch_monopole(const DevBasis& B, const double* H, double* M, hipStream_t s);
 void launch_white_1b(const DevBasis& B, const double* H, double* ETA, hipStream_t s);
 void launch_white_2b(const DevBasis& B, const SegTask* t, int nt, const double* H, const double* M, double* ETA,
