@@ -979,9 +979,22 @@ struct imsrg_flow_s {
   double fw_ds = 0.5, fw_bch = 1e-6, fw_mag = 1e-6;
   double *gH = nullptr, *gHS = nullptr, *gG = nullptr, *gDG = nullptr, *gHSb = nullptr, *gGb = nullptr,
          *gHb = nullptr, *gdOb = nullptr, *getab = nullptr;
+  // many-instance batch (imsrg_batch_create): ninst instances back to back;
+  // X masks by the instances still flowing (actf), Xe by those still adding
+  // terms to the current BCH / Magnus expansion (acte)
+  bool batch = false;
+  int ninst = 1;
+  Exec Xe;
+  int *actf = nullptr, *acte = nullptr;      // device masks
+  int *hmf = nullptr, *hme = nullptr;        // pinned staging of the masks
+  double* hv = nullptr;                      // pinned per-instance scalars (kDStride each)
+  std::vector<int> hf;                       // host copy of the flow mask
   ~imsrg_flow_s() {
     for (auto& kv : execs) (void)hipGraphExecDestroy(kv.second);
     for (void* p : allocs) (void)hipFree(p);
+    if (hmf) (void)hipHostFree(hmf);
+    if (hme) (void)hipHostFree(hme);
+    if (hv) (void)hipHostFree(hv);
   }
 };
 
@@ -1142,6 +1155,156 @@ int magnus_step(imsrg_flow_s* F, double* HS, double* G, double* DG, const double
   return 0;
 }
 
+
+// ------------------------------------------------- many-instance batches
+// The step-level calls over F->ninst instances at once: every launch covers
+// all instances (grid.y) and skips masked ones, so each instance runs
+// exactly the kernels, and takes exactly the exits, of its own flow.
+
+// upload a mask through its pinned staging (synchronises first, so the
+// staging of an earlier upload is free)
+int set_mask(imsrg_flow_s* F, int* dev, int* stage, const std::vector<int>& m, hipStream_t s) {
+  HIPCHK(hipStreamSynchronize(s));
+  for (int k = 0; k < F->ninst; ++k) stage[k] = m[k];
+  HIPCHK(hipMemcpyAsync(dev, stage, F->ninst * sizeof(int), hipMemcpyHostToDevice, s));
+  return 0;
+}
+
+// per-instance scalars dval[k * kDStride + 0 .. nslot) to F->hv; with E0 of
+// op (optional) in slot 3
+int read_inst(imsrg_flow_s* F, int nslot, const double* op, hipStream_t s) {
+  HIPCHK(hipMemcpyAsync(F->hv, F->X.dval, (size_t)F->ninst * kDStride * sizeof(double), hipMemcpyDeviceToHost, s));
+  (void)nslot;
+  if (op)
+    HIPCHK(hipMemcpy2DAsync(F->hv + 3, kDStride * sizeof(double), op, (size_t)F->B->b.oplen * sizeof(double),
+                            sizeof(double), F->ninst, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return 0;
+}
+
+// BCH_EXPAND (IMSRG_MAGNUS.f90:149-234) per instance of the flow mask;
+// nterms[k] = the reference's iw at instance k's exit
+int bch_b(imsrg_flow_s* F, double* HS, const double* G, const double* H, double conv, int maxit, int* nterms,
+          hipStream_t s) {
+  imsrg_basis_s* B = F->B;
+  const int ni = F->ninst;
+  const i64 n = B->b.oplen;
+  int rc;
+  std::vector<int> he = F->hf;
+  if ((rc = set_mask(F, F->acte, F->hme, he, s))) return rc;
+  launch_copy_b(HS, H, n, ni, F->acte, s);
+  launch_copy_b(F->buf[0], H, n, ni, F->acte, s);
+  t_pandya(F->Xe, G, -1, F->GCC, s);
+  double coef = 1.0;
+  int iw;
+  for (iw = 2; iw <= maxit; ++iw) {
+    coef = coef / (iw - 1.0);
+    double* AD = F->buf[iw % 2];
+    double* INT2 = F->buf[(iw + 1) % 2];
+    const double c = coef;
+    imsrg_flow_s::GraphKey key{2, iw, HS, G, H, -1, 1};
+    rc = run_term(F, key, s, [&](hipStream_t cs) {
+      t_full(F->Xe, G, -1, AD, 1, INT2, 1, true, F->w1, F->w2, F->GCC, F->ADCC, F->WCC, cs);
+      launch_axpy_b(n, HS, 1.0, INT2, c, HS, ni, F->acte, cs);
+      t_norm(F->Xe, INT2, F->Xe.dval, cs);
+    });
+    if (rc) return rc;
+    if ((rc = read_inst(F, 1, nullptr, s))) return rc;
+    bool any = false, changed = false;
+    for (int k = 0; k < ni; ++k) {
+      if (!he[k]) continue;
+      if (F->hv[k * kDStride] * coef < conv) {
+        he[k] = 0;
+        if (nterms) nterms[k] = iw;
+        changed = true;
+      } else {
+        any = true;
+      }
+    }
+    if (!any) break;
+    if (changed && (rc = set_mask(F, F->acte, F->hme, he, s))) return rc;
+  }
+  if (iw > maxit) return fail(IMSRG_ECLIP, "clipping BCH");
+  return 0;
+}
+
+// MAGNUS_EXPAND (IMSRG_MAGNUS.f90:620-691) per instance of the flow mask:
+// DG holds eta on entry and dOmega on exit; ncomm[k] nested commutators
+int magnus_b(imsrg_flow_s* F, double* DG, const double* G, double conv, int* ncomm, hipStream_t s) {
+  imsrg_basis_s* B = F->B;
+  const int ni = F->ninst;
+  const i64 n = B->b.oplen;
+  const double cof[7] = {1.0, -0.5, .0833333333, 0.0, -0.00138888888, 0.0, 3.306878e-5};
+  int rc;
+  std::vector<int> he = F->hf, q(ni, 1);
+  std::vector<double> adv((size_t)ni * 7, 0.0), full(ni, 0.0);
+  // both ping-pong buffers carry DG (every term assigns all but E0)
+  launch_copy_b(F->buf[0], DG, n, ni, F->actf, s);
+  launch_copy_b(F->buf[1], DG, n, ni, F->actf, s);
+  t_norm(F->X, DG, F->X.dval, s);
+  t_norm(F->X, G, F->X.dval + 1, s);
+  if ((rc = read_inst(F, 2, nullptr, s))) return rc;
+  bool any = false;
+  for (int k = 0; k < ni; ++k) {
+    if (ncomm) ncomm[k] = 0;
+    if (!he[k]) continue;
+    adv[(size_t)k * 7] = F->hv[k * kDStride];
+    full[k] = F->hv[k * kDStride + 1];
+    if (full[k] < (double)1e-9f) he[k] = 0;
+    any = any || he[k];
+  }
+  if (!any) return 0;
+  if ((rc = set_mask(F, F->acte, F->hme, he, s))) return rc;
+  t_pandya(F->Xe, G, -1, F->GCC, s);
+  for (int i = 2; i <= 7; ++i) {
+    bool changed = false;
+    any = false;
+    for (int k = 0; k < ni; ++k) {
+      if (!he[k]) continue;
+      const double adnorm = adv[(size_t)k * 7 + (i - q[k] - 1)];
+      q[k] = (std::fabs(cof[i - 1]) > (double)1e-6f) ? 1 : 2;
+      if (std::fabs(adnorm / full[k]) < conv) {
+        he[k] = 0;
+        changed = true;
+      } else {
+        any = true;
+      }
+    }
+    if (!any) break;
+    if (changed && (rc = set_mask(F, F->acte, F->hme, he, s))) return rc;
+    double* AD = F->buf[i % 2];
+    double* INT2 = F->buf[(i + 1) % 2];
+    const double c = cof[i - 1];
+    imsrg_flow_s::GraphKey key{3, i, DG, G, nullptr, -1, -1};
+    rc = run_term(F, key, s, [&](hipStream_t cs) {
+      t_full(F->Xe, G, -1, AD, -1, INT2, -1, false, F->w1, F->w2, F->GCC, F->ADCC, F->WCC, cs);
+      launch_axpy_b(n, DG, 1.0, INT2, c, DG, ni, F->acte, cs);
+      t_norm(F->Xe, INT2, F->Xe.dval, cs);
+    });
+    if (rc) return rc;
+    if ((rc = read_inst(F, 1, nullptr, s))) return rc;
+    for (int k = 0; k < ni; ++k)
+      if (he[k]) {
+        if (ncomm) ++ncomm[k];
+        adv[(size_t)k * 7 + (i - 1)] = F->hv[k * kDStride] * std::fabs(cof[i - 1]);
+      }
+  }
+  return 0;
+}
+
+// one Magnus step per flowing instance; E0 and mbpt2 of instance k land in
+// F->hv[k * kDStride + 3] and [+ 2]
+int magnus_step_b(imsrg_flow_s* F, double* HS, double* G, double* DG, const double* H, double ds, double bch_conv,
+                  double mag_conv, int* nm, int* nb, hipStream_t s) {
+  imsrg_basis_s* B = F->B;
+  int rc;
+  if ((rc = magnus_b(F, DG, G, mag_conv, nm, s))) return rc;
+  launch_axpy_b(B->b.oplen, G, 1.0, DG, ds, G, F->ninst, F->actf, s);   // euler_step
+  if ((rc = bch_b(F, HS, G, H, bch_conv, 40, nb, s))) return rc;
+  if ((rc = white(F->X, HS, DG, s))) return rc;
+  if ((rc = mbpt2_dev(F->X, HS, F->X.dval + 2, s))) return rc;
+  return read_inst(F, 3, HS, s);
+}
 
 // ------------------------------------------------- reverse mode (flow)
 int fbuf(imsrg_flow_s* F, double** slot, i64 n) {
@@ -1744,12 +1907,14 @@ int imsrg_flow_set_graphs(imsrg_flow F, int on) {
 int imsrg_bch_expand(imsrg_flow F, double* HS, const double* G, int Gh, const double* H, int Hh, double conv,
                      int maxit, int* nterms, double* advals, imsrg_stream stream) {
   if (!F) return fail(IMSRG_EINVAL, "null flow");
+  if (F->batch) return fail(IMSRG_EINVAL, "batched flow: use the imsrg_batch_* calls");
   return bch(F, HS, G, Gh, H, Hh, conv, maxit, nterms, advals, S(stream));
 }
 
 int imsrg_magnus_expand(imsrg_flow F, double* DG, int DGh, const double* G, int Gh, double conv, int* ncomm,
                         imsrg_stream stream) {
   if (!F) return fail(IMSRG_EINVAL, "null flow");
+  if (F->batch) return fail(IMSRG_EINVAL, "batched flow: use the imsrg_batch_* calls");
   return magnus(F, DG, DGh, G, Gh, conv, ncomm, S(stream));
 }
 
@@ -1763,6 +1928,7 @@ int imsrg_euler_step(imsrg_basis B, double* G, const double* DG, double ds, imsr
 int imsrg_magnus_step(imsrg_flow F, double* HS, double* G, double* DG, const double* H, double ds,
                       double bch_conv, double mag_conv, double* out, imsrg_stream stream) {
   if (!F) return fail(IMSRG_EINVAL, "null flow");
+  if (F->batch) return fail(IMSRG_EINVAL, "batched flow: use the imsrg_batch_* calls");
   return magnus_step(F, HS, G, DG, H, ds, bch_conv, mag_conv, out, S(stream));
 }
 
@@ -1770,6 +1936,7 @@ int imsrg_magnus_decouple(imsrg_flow F, double* HS, double* G, double ds, double
                           int crit_mode, double crit_tol, int maxsteps, double* log, int* steps,
                           imsrg_stream stream) {
   if (!F) return fail(IMSRG_EINVAL, "null flow");
+  if (F->batch) return fail(IMSRG_EINVAL, "batched flow: use the imsrg_batch_* calls");
   imsrg_basis_s* B = F->B;
   hipStream_t s = S(stream);
   const i64 n = B->b.oplen;
@@ -1813,6 +1980,155 @@ int imsrg_magnus_decouple(imsrg_flow F, double* HS, double* G, double ds, double
   return 0;
 }
 
+// ------------------------------------------------ many-instance batches
+int imsrg_batch_create(imsrg_basis B, int ninst, imsrg_flow* out) {
+  if (!B || !out) return fail(IMSRG_EINVAL, "null argument");
+  if (ninst < 1 || ninst > 65535) return fail(IMSRG_EINVAL, "batch size out of range");
+  if (B->path != IMSRG_PATH_GROUPED) return fail(IMSRG_EINVAL, "batched flows run the grouped path");
+  int rc = ensure_dev(B);
+  if (rc) return rc;
+  imsrg_flow_s* F = new imsrg_flow_s();
+  F->B = B;
+  F->batch = true;
+  F->ninst = ninst;
+  const Basis& b = B->b;
+  const i64 ni = ninst;
+  Exec X = basis_exec(B);
+  X.D.ninst = ninst;
+  void* pm = nullptr;
+  if ((rc = falloc(F, ni * b.oplen, &F->buf[0])) || (rc = falloc(F, ni * b.oplen, &F->buf[1])) ||
+      (rc = falloc(F, ni * b.oplen, &F->w1)) || (rc = falloc(F, ni * b.oplen, &F->w2)) ||
+      (rc = falloc(F, ni * b.cclen, &F->GCC)) || (rc = falloc(F, ni * b.cclen, &F->ADCC)) ||
+      (rc = falloc(F, ni * b.wclen, &F->WCC)) || (rc = falloc(F, ni * b.oplen, &F->Hbare)) ||
+      (rc = falloc(F, ni * b.oplen, &F->decDG)) || (rc = falloc(F, ni * X.D.spart, &X.part)) ||
+      (rc = falloc(F, ni * X.D.spscr, &X.Pscr)) || (rc = falloc(F, ni * X.D.sk2scr, &X.K2scr)) ||
+      (rc = falloc(F, ni * b.N * b.N, &X.M)) || (rc = falloc(F, ni * kDStride, &X.dval))) {
+    delete F;
+    return rc;
+  }
+  if (hipMalloc(&pm, 2 * ni * sizeof(int)) != hipSuccess) {
+    delete F;
+    return fail(IMSRG_EHIP, "hipMalloc of the batch masks failed");
+  }
+  F->allocs.push_back(pm);
+  F->actf = (int*)pm;
+  F->acte = F->actf + ni;
+  if (hipHostMalloc((void**)&F->hmf, ni * sizeof(int), hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void**)&F->hme, ni * sizeof(int), hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void**)&F->hv, ni * kDStride * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+    delete F;
+    return fail(IMSRG_EHIP, "hipHostMalloc of the batch staging failed");
+  }
+  F->X = X;
+  F->X.D.act = F->actf;
+  F->Xe = X;
+  F->Xe.D.act = F->acte;
+  F->hf.assign(ninst, 1);
+  if ((rc = set_mask(F, F->actf, F->hmf, F->hf, nullptr))) {
+    delete F;
+    return rc;
+  }
+  if (hipStreamSynchronize(nullptr) != hipSuccess) {
+    delete F;
+    return fail(IMSRG_EHIP, "batch mask upload failed");
+  }
+  *out = F;
+  return 0;
+}
+
+int imsrg_batch_size(imsrg_flow F) { return F ? F->ninst : -1; }
+
+#define PREP_BATCH()                                                     \
+  if (!F || !F->batch) return fail(IMSRG_EINVAL, "not a batched flow"); \
+  hipStream_t s = S(stream);                                             \
+  imsrg_basis_s* B = F->B;                                               \
+  (void)B;
+
+int imsrg_batch_commutator(imsrg_flow F, const double* L, int Lh, const double* R, int Rh, double* RES,
+                           imsrg_stream stream) {
+  PREP_BATCH();
+  if (!L || !R || !RES) return fail(IMSRG_EINVAL, "null argument");
+  int rc;
+  F->hf.assign(F->ninst, 1);
+  if ((rc = set_mask(F, F->actf, F->hmf, F->hf, s))) return rc;
+  t_pandya(F->X, L, Lh, F->GCC, s);
+  t_full(F->X, L, Lh, R, Rh, RES, -Lh * Rh, true, F->w1, F->w2, F->GCC, F->ADCC, F->WCC, s);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int imsrg_batch_bch_expand(imsrg_flow F, double* HS, const double* G, const double* H, double conv, int maxit,
+                           int* nterms, imsrg_stream stream) {
+  PREP_BATCH();
+  if (!HS || !G || !H) return fail(IMSRG_EINVAL, "null argument");
+  int rc;
+  F->hf.assign(F->ninst, 1);
+  if ((rc = set_mask(F, F->actf, F->hmf, F->hf, s))) return rc;
+  return bch_b(F, HS, G, H, conv, maxit, nterms, s);
+}
+
+int imsrg_batch_magnus_expand(imsrg_flow F, double* DG, const double* G, double conv, int* ncomm,
+                              imsrg_stream stream) {
+  PREP_BATCH();
+  if (!DG || !G) return fail(IMSRG_EINVAL, "null argument");
+  int rc;
+  F->hf.assign(F->ninst, 1);
+  if ((rc = set_mask(F, F->actf, F->hmf, F->hf, s))) return rc;
+  return magnus_b(F, DG, G, conv, ncomm, s);
+}
+
+int imsrg_batch_magnus_decouple(imsrg_flow F, double* HS, double* G, double ds, double bch_conv, double mag_conv,
+                                double crit_tol, int maxsteps, double* E0, int* steps, int* nsteps_max,
+                                imsrg_stream stream) {
+  PREP_BATCH();
+  if (!HS || !G) return fail(IMSRG_EINVAL, "null argument");
+  const int ni = F->ninst;
+  const i64 n = B->b.oplen;
+  int rc;
+  F->hf.assign(ni, 1);
+  if ((rc = set_mask(F, F->actf, F->hmf, F->hf, s))) return rc;
+  double* DG = F->decDG;
+  launch_fill_b(DG, n, 0.0, ni, F->actf, s);
+  launch_copy_b(F->Hbare, HS, n, ni, F->actf, s);
+  if ((rc = white(F->X, HS, DG, s))) return rc;
+  if ((rc = mbpt2_dev(F->X, HS, F->X.dval + 2, s))) return rc;
+  if ((rc = read_inst(F, 3, HS, s))) return rc;
+  bool any = false;
+  for (int k = 0; k < ni; ++k) {
+    if (steps) steps[k] = 0;
+    if (E0) E0[k] = F->hv[k * kDStride + 3];
+    if (!(std::fabs(F->hv[k * kDStride + 2]) > crit_tol)) F->hf[k] = 0;
+    any = any || F->hf[k];
+  }
+  if ((rc = set_mask(F, F->actf, F->hmf, F->hf, s))) return rc;
+  std::vector<int> nm(ni), nb(ni);
+  int st = 0;
+  while (any) {
+    if (st >= maxsteps) {
+      if (nsteps_max) *nsteps_max = st;
+      return fail(IMSRG_ESTEPS, "batched magnus_decouple: step limit reached");
+    }
+    ++st;
+    if ((rc = magnus_step_b(F, HS, G, DG, F->Hbare, ds, bch_conv, mag_conv, nm.data(), nb.data(), s))) return rc;
+    bool changed = false;
+    any = false;
+    for (int k = 0; k < ni; ++k) {
+      if (!F->hf[k]) continue;
+      if (E0) E0[k] = F->hv[k * kDStride + 3];
+      if (steps) steps[k] = st;
+      if (!(std::fabs(F->hv[k * kDStride + 2]) > crit_tol)) {
+        F->hf[k] = 0;
+        changed = true;
+      } else {
+        any = true;
+      }
+    }
+    if (changed && (rc = set_mask(F, F->actf, F->hmf, F->hf, s))) return rc;
+  }
+  if (nsteps_max) *nsteps_max = st;
+  return 0;
+}
+
 // ------------------------------------------------------- reverse mode
 int imsrg_commutator_vjp(imsrg_basis B, const double* L, int Lh, const double* R, int Rh, const double* RESb,
                          int RESh, double* Lb, double* Rb, imsrg_stream stream) {
@@ -1827,12 +2143,14 @@ int imsrg_commutator_vjp(imsrg_basis B, const double* L, int Lh, const double* R
 int imsrg_bch_vjp(imsrg_flow F, const double* G, const double* H, int Hherm, double conv, int maxit,
                   const double* HSb, double* Gb, double* Hb, int* nterms, imsrg_stream stream) {
   if (!F || !G || !H || !HSb || !Gb || !Hb) return fail(IMSRG_EINVAL, "null argument");
+  if (F->batch) return fail(IMSRG_EINVAL, "batched flow: use the imsrg_batch_* calls");
   return bch_vjp(F, G, H, Hherm, conv, maxit, HSb, Gb, Hb, nterms, S(stream));
 }
 
 int imsrg_magnus_vjp(imsrg_flow F, const double* ETA, const double* G, double conv, const double* DGb, double* ETAb,
                      double* Gb, int* ncomm, imsrg_stream stream) {
   if (!F || !ETA || !G || !DGb || !ETAb || !Gb) return fail(IMSRG_EINVAL, "null argument");
+  if (F->batch) return fail(IMSRG_EINVAL, "batched flow: use the imsrg_batch_* calls");
   return magnus_vjp(F, ETA, G, conv, DGb, ETAb, Gb, ncomm, S(stream));
 }
 
@@ -1851,17 +2169,20 @@ int imsrg_normal_order_vjp(imsrg_basis B, const double* Hb, double* Tb, imsrg_st
 int imsrg_flow_forward(imsrg_flow F, const double* T, double ds, double bch_conv, double mag_conv, double crit_tol,
                        int maxsteps, int ckpt, double* E0, int* steps, imsrg_stream stream) {
   if (!F || !T) return fail(IMSRG_EINVAL, "null argument");
+  if (F->batch) return fail(IMSRG_EINVAL, "batched flow: use the imsrg_batch_* calls");
   return flow_forward(F, T, ds, bch_conv, mag_conv, crit_tol, maxsteps, ckpt, E0, steps, S(stream));
 }
 
 int imsrg_flow_backward(imsrg_flow F, double* Tb, imsrg_stream stream) {
   if (!F || !Tb) return fail(IMSRG_EINVAL, "null argument");
+  if (F->batch) return fail(IMSRG_EINVAL, "batched flow: use the imsrg_batch_* calls");
   return flow_backward(F, Tb, S(stream));
 }
 
 int imsrg_flow_grad(imsrg_flow F, const double* T, double ds, double bch_conv, double mag_conv, double crit_tol,
                     int maxsteps, int ckpt, double* Tb, double* E0, int* steps, imsrg_stream stream) {
   if (!F || !T || !Tb) return fail(IMSRG_EINVAL, "null argument");
+  if (F->batch) return fail(IMSRG_EINVAL, "batched flow: use the imsrg_batch_* calls");
   int rc = flow_forward(F, T, ds, bch_conv, mag_conv, crit_tol, maxsteps, ckpt, E0, steps, S(stream));
   if (rc) return rc;
   return flow_backward(F, Tb, S(stream));

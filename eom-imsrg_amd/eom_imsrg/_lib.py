@@ -81,6 +81,13 @@ SIGNATURES = {
     "imsrg_magnus_step": (_i, [_p, _dp, _dp, _dp, _dp, C.c_double, C.c_double, C.c_double, _d, _p]),
     "imsrg_magnus_decouple": (_i, [_p, _dp, _dp, C.c_double, C.c_double, C.c_double, _i, C.c_double, _i, _d,
                                    _ip, _p]),
+    "imsrg_batch_create": (_i, [_p, _i, C.POINTER(_p)]),
+    "imsrg_batch_size": (_i, [_p]),
+    "imsrg_batch_commutator": (_i, [_p, _dp, _i, _dp, _i, _dp, _p]),
+    "imsrg_batch_bch_expand": (_i, [_p, _dp, _dp, _dp, C.c_double, _i, _ip, _p]),
+    "imsrg_batch_magnus_expand": (_i, [_p, _dp, _dp, C.c_double, _ip, _p]),
+    "imsrg_batch_magnus_decouple": (_i, [_p, _dp, _dp, C.c_double, C.c_double, C.c_double, C.c_double, _i, _d, _ip,
+                                         _ip, _p]),
     "imsrg_commutator_vjp": (_i, [_p, _dp, _i, _dp, _i, _dp, _i, _dp, _dp, _p]),
     "imsrg_bch_vjp": (_i, [_p, _dp, _dp, _i, C.c_double, _i, _dp, _dp, _dp, _ip, _p]),
     "imsrg_magnus_vjp": (_i, [_p, _dp, _dp, C.c_double, _dp, _dp, _dp, _ip, _p]),

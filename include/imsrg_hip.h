@@ -176,6 +176,35 @@ int imsrg_magnus_decouple(imsrg_flow f, double* HS, double* G, double ds, double
                           int crit_mode, double crit_tol, int maxsteps, double* log, int* steps,
                           imsrg_stream stream);
 
+/* ------------------------------------------------ many-instance batches */
+/* Instance-array variants of the step-level calls (IMSRG_MAGNUS.f90:15-234,
+ * 620-691) for ninst independent problems on one basis, run together: every
+ * kernel launch covers all instances, and each instance takes the BCH,
+ * Magnus and convergence exits of its own flow (identical results to ninst
+ * separate flows).  Operator arguments hold ninst operators back to back
+ * (ninst * IMSRG_SIZE_OP doubles, instance k at offset k * IMSRG_SIZE_OP).
+ * A batched flow runs the grouped path and accepts only imsrg_batch_* and
+ * imsrg_flow_set_graphs / imsrg_flow_destroy. */
+int imsrg_batch_create(imsrg_basis b, int ninst, imsrg_flow* out);
+int imsrg_batch_size(imsrg_flow f);
+/* RES_k = [L_k, R_k] for every k (RES herm = -Lh*Rh; E0 included) */
+int imsrg_batch_commutator(imsrg_flow f, const double* L, int Lh, const double* R, int Rh, double* RES,
+                           imsrg_stream stream);
+/* BCH_EXPAND per instance (G antihermitian, H hermitian); nterms[k] = the
+ * reference's iw at instance k's exit */
+int imsrg_batch_bch_expand(imsrg_flow f, double* HS, const double* G, const double* H, double conv, int maxit,
+                           int* nterms, imsrg_stream stream);
+/* MAGNUS_EXPAND per instance: DG holds eta on entry, dOmega on exit */
+int imsrg_batch_magnus_expand(imsrg_flow f, double* DG, const double* G, double conv, int* ncomm,
+                              imsrg_stream stream);
+/* magnus_decouple per instance with build_gs_white and the |mbpt2| <
+ * crit_tol criterion: HS in = normal-ordered H, out = flowed H; G (Omega)
+ * must be zero on entry; E0[k], steps[k] per instance; nsteps_max the
+ * number of batched steps run */
+int imsrg_batch_magnus_decouple(imsrg_flow f, double* HS, double* G, double ds, double bch_conv, double mag_conv,
+                                double crit_tol, int maxsteps, double* E0, int* steps, int* nsteps_max,
+                                imsrg_stream stream);
+
 /* ------------------------------------------------ reverse mode */
 /* The reference has no derivatives.  These are hand-written transposes of
  * the routines above (vector-Jacobian products) for dE0/d(input matrix
