@@ -38,6 +38,10 @@ def _parse():
     ap.add_argument("--no-grad", action="store_true")
     ap.add_argument("--grad-check", type=int, default=20,
                     help="central-difference check of dE0/dV on this many seeded two-body elements")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 64-instance eMax 6 batch")
+    ap.add_argument("--batch-cpu-check", type=int, default=1,
+                    help="instances of the batch whose flowed E0 is checked against the oracle on the host")
+    ap.add_argument("--no-emax10", action="store_true", help="skip the eMax 10 grouped + hipGraph steps")
     ap.add_argument("--kernel-only", default=None,
                     help="launch one kernel class repeatedly and exit (for rocprofv3 counter runs)")
     return ap.parse_args()
@@ -130,6 +134,10 @@ def main() -> int:
     if not a.no_grad:
         grad = gradient_bench(b, Hv, a.grad_check)
 
+    # ---- many-instance batch (64 x eMax 6) and eMax 10 steps
+    batch = None if a.no_batch else batch_bench(64, 6, 8, a.batch_cpu_check)
+    e10 = None if a.no_emax10 else emax10_bench(3, 1)
+
     # ---- roofline of the dominant kernel
     cand = {
         "PPPP": ("mfma", wl.pppp_flops(b) / 1e12, "TFLOP/s", FP64_MFMA_PEAK_TFLOPS, wl.pppp_bytes(b)),
@@ -166,6 +174,8 @@ def main() -> int:
         "kernel_ms": {k: round(v, 4) for k, v in kms.items()},
         "grad_over_fwd": grad["grad_over_fwd"] if grad else None,
         "gradient": grad,
+        "batch": batch,
+        "emax10": e10,
         "roofline": roof,
         "cpu_baseline": cpu,
     }
@@ -243,6 +253,119 @@ def gradient_bench(b, Hv, ncheck: int):
                            "h_MeV": h, "max_rel_err": worst_rel, "max_abs_err": worst_abs,
                            "criterion": "|fd - grad| <= 1e-6 |grad| + 2e-7", "pass": bool(ok)}
     return out
+
+
+def batch_bench(ninst: int, emax: int, nseq: int, ncpu: int):
+    """ninst independent flows at eMax `emax` (instance k: seed k, two-body
+    scale U(0.8, 1.2), hw in {16, 20, 24}) run to |mbpt2| < 1e-6 as one
+    batch (grouped + hipGraph), against `nseq` of them flowed one after
+    another by single-instance flows (scaled to ninst); the first `ncpu`
+    instances' E0 checked against the oracle's flow on the host"""
+    import numpy as np
+    import torch
+    from eom_imsrg import Basis, Batch, Flow, SqOp, synth, commutators as cm
+
+    b = Basis.oxygen16(emax)
+    H = torch.empty(ninst, b.oplen, dtype=torch.float64, device="cuda")
+    Hv = []
+    for k in range(ninst):
+        seed, scale, hw = synth.instance_parameters(k)
+        v = synth.hamiltonian_vacuum(b, seed=seed, hw=hw, v_scale=scale)
+        Hv.append(v)
+        op = SqOp.from_numpy(b, v, 1)
+        cm.normal_order(op)
+        H[k].copy_(op.data)
+    bt = Batch(b, ninst, graphs=True)
+
+    def run_batch():
+        HS, G = H.clone(), bt.zeros()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        e0, st, nmax = bt.magnus_decouple(HS, G)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0, e0, st, nmax
+
+    run_batch()                                   # graph capture
+    t_b, e0, st, nmax = run_batch()
+    f = Flow(b, graphs=True)
+
+    def run_one(k):
+        HS, G = SqOp(b, H[k].clone(), 1), SqOp.zeros(b, -1)
+        log = f.magnus_decouple(HS, G)
+        return log[-1, 1], len(log)
+
+    run_one(0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    single = [run_one(k) for k in range(nseq)]
+    torch.cuda.synchronize()
+    t_s = (time.perf_counter() - t0) / nseq * ninst
+    dE = max(abs(single[k][0] - e0[k]) for k in range(nseq))
+    same_steps = all(single[k][1] == st[k] for k in range(nseq))
+    out = {"workload": f"magnus_decouple_emax{emax}_x{ninst}", "instances": ninst, "emax": emax,
+           "batch_s": round(t_b, 4), "instance_steps_total": int(st.sum()), "batched_steps": int(nmax),
+           "steps_per_s": round(float(st.sum()) / t_b, 2),
+           "sequential_single_s_est": round(t_s, 4), "sequential_sample": nseq,
+           "speedup_vs_sequential": round(t_s / t_b, 3),
+           "E0_batch_vs_single_max_MeV": dE, "steps_match_single": bool(same_steps),
+           "E0": [float(x) for x in e0], "steps": [int(x) for x in st]}
+    if ncpu > 0:
+        sys.path.insert(0, ROOT)
+        from oracle.oracle import OracleBasis, lib
+        lib().orc_set_threads(max(1, min(16, len(os.sched_getaffinity(0)))))
+        o = OracleBasis(b.n, b.l, b.jj, b.tz, b.hole)
+        tol = float(np.float32(1e-6))
+        errs = []
+        for k in range(ncpu):
+            HSr, _, _, stc = o.magnus_decouple(o.normal_order(Hv[k]), 0.5, tol, tol, 0, tol)
+            errs.append({"instance": k, "E0_cpu": float(HSr[0]), "E0_gpu": float(e0[k]),
+                         "dE0_MeV": abs(float(HSr[0]) - float(e0[k])), "steps_cpu": int(stc), "steps_gpu": int(st[k])})
+        out["cpu_check"] = errs
+    return out
+
+
+def emax10_bench(nsteps: int, warm: int):
+    """Magnus steps 1..nsteps of the eMax 10 flow (132 orbits, seed 0, hw 20)
+    on the grouped + hipGraph path, BCH terms to 1e-10"""
+    import numpy as np
+    import torch
+    from eom_imsrg import Basis, Flow, SqOp, synth, commutators as cm, _lib, workload as wl
+    import ctypes as C
+
+    b = Basis.oxygen16(10)
+    H = SqOp.from_numpy(b, synth.hamiltonian_vacuum(b, seed=0, hw=20.0), 1)
+    cm.normal_order(H)
+    f = Flow(b, graphs=True)
+
+    def fresh():
+        HS, G, DG = H.clone(), SqOp.zeros(b, -1), SqOp.zeros(b, -1)
+        cm.build_gs_white(HS, DG)
+        return HS, G, DG
+
+    HS, G, DG = fresh()
+    for _ in range(warm):
+        f.magnus_step(HS, G, DG, H, bch_conv=1e-10)
+    HS, G, DG = fresh()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    log = [f.magnus_step(HS, G, DG, H, bch_conv=1e-10) for _ in range(nsteps)]
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) / nsteps * 1e3
+    # one commutator, hipEvent-timed on the launch stream
+    Om = SqOp.from_numpy(b, synth.antihermitian(b, seed=1, sigma=0.05), -1)
+    RES, w1, w2 = SqOp.zeros(b, 1), SqOp.zeros(b), SqOp.zeros(b)
+    LCC = torch.zeros(max(b.cclen, 1), dtype=torch.float64, device="cuda")
+    RCC = torch.zeros_like(LCC)
+    WCC = torch.zeros(max(b.wcclen, 1), dtype=torch.float64, device="cuda")
+    cms = C.c_double()
+    _lib.check(_lib.lib.imsrg_profile_kernel(b.handle, _lib.KERNEL["COMMUTATOR"], Om.ptr, -1, H.ptr, 1, RES.ptr, 1,
+                                             w1.ptr, w2.ptr, LCC.data_ptr(), RCC.data_ptr(), WCC.data_ptr(), 3,
+                                             C.byref(cms), C.c_void_p(torch.cuda.current_stream().cuda_stream)))
+    F = wl.commutator_flops(b)
+    return {"workload": "magnus_step_emax10", "orbits": b.norb, "E2": b.e2_count(), "ms_per_step": round(ms, 3),
+            "steps": nsteps, "bch_conv": 1e-10, "commutators_per_step": [int(x[2] + x[3] - 1) for x in log],
+            "E0_after_steps": log[-1][0], "commutator_ms": round(cms.value, 3),
+            "commutator_tflops": round(F / (cms.value / 1e3) / 1e12, 3)}
 
 
 def mirror_index(b):
