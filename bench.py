@@ -42,6 +42,7 @@ def _parse():
     ap.add_argument("--batch-cpu-check", type=int, default=1,
                     help="instances of the batch whose flowed E0 is checked against the oracle on the host")
     ap.add_argument("--no-emax10", action="store_true", help="skip the eMax 10 grouped + hipGraph steps")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default GEMM tiles (no autotuner)")
     ap.add_argument("--kernel-only", default=None,
                     help="launch one kernel class repeatedly and exit (for rocprofv3 counter runs)")
     return ap.parse_args()
@@ -63,6 +64,8 @@ def main() -> int:
         return 1
     dev = torch.device("cuda:0")
     b = Basis.oxygen16(a.emax)
+    from eom_imsrg import tune
+    tuned = None if a.no_tune else tune.tune(b)
     Hv = synth.hamiltonian_vacuum(b, seed=0, hw=20.0)
     H = SqOp.from_numpy(b, Hv, 1)
     cm.normal_order(H)
@@ -136,7 +139,7 @@ def main() -> int:
 
     # ---- many-instance batch (64 x eMax 6) and eMax 10 steps
     batch = None if a.no_batch else batch_bench(64, 6, 8, a.batch_cpu_check)
-    e10 = None if a.no_emax10 else emax10_bench(3, 1)
+    e10 = None if a.no_emax10 else emax10_bench(3, 1, not a.no_tune)
 
     # ---- roofline of the dominant kernel
     cand = {
@@ -165,7 +168,9 @@ def main() -> int:
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"magnus_step_emax{a.emax}", "emax": a.emax, "orbits": b.norb, "holes": "16O",
                    "hw_MeV": 20.0, "seed": 0, "generator": "white", "path": "grouped+hipgraph",
-                   "channels": b.nblocks, "E2": b.e2_count()},
+                   "channels": b.nblocks, "E2": b.e2_count(),
+                   "gemm_tiles": tuned["choice"] if tuned else "default", "gemm_tiles_source": tuned["source"] if tuned else None},
+        "gemm_tune": tuned,
         "commutators_per_step": [int(x) for x in ncomm],
         "E0_after_steps": g_log[-1][0],
         **out_plain,
@@ -324,15 +329,17 @@ def batch_bench(ninst: int, emax: int, nseq: int, ncpu: int):
     return out
 
 
-def emax10_bench(nsteps: int, warm: int):
+def emax10_bench(nsteps: int, warm: int, autotune: bool):
     """Magnus steps 1..nsteps of the eMax 10 flow (132 orbits, seed 0, hw 20)
-    on the grouped + hipGraph path, BCH terms to 1e-10"""
+    on the grouped + hipGraph path with autotuned GEMM tiles, BCH terms to
+    1e-10"""
     import numpy as np
     import torch
-    from eom_imsrg import Basis, Flow, SqOp, synth, commutators as cm, _lib, workload as wl
+    from eom_imsrg import Basis, Flow, SqOp, synth, commutators as cm, _lib, workload as wl, tune
     import ctypes as C
 
     b = Basis.oxygen16(10)
+    tuned = tune.tune(b) if autotune else None
     H = SqOp.from_numpy(b, synth.hamiltonian_vacuum(b, seed=0, hw=20.0), 1)
     cm.normal_order(H)
     f = Flow(b, graphs=True)
@@ -365,7 +372,7 @@ def emax10_bench(nsteps: int, warm: int):
     return {"workload": "magnus_step_emax10", "orbits": b.norb, "E2": b.e2_count(), "ms_per_step": round(ms, 3),
             "steps": nsteps, "bch_conv": 1e-10, "commutators_per_step": [int(x[2] + x[3] - 1) for x in log],
             "E0_after_steps": log[-1][0], "commutator_ms": round(cms.value, 3),
-            "commutator_tflops": round(F / (cms.value / 1e3) / 1e12, 3)}
+            "commutator_tflops": round(F / (cms.value / 1e3) / 1e12, 3), "gemm_tune": tuned}
 
 
 def mirror_index(b):

@@ -52,6 +52,7 @@ struct imsrg_basis_s {
   int path = IMSRG_PATH_GROUPED;
   bool dev_ready = false;
   int tile_req[4] = {0, 0, 0, 0};   // GEMM tile shapes for pp1, k2, k3, ph (64x64, measured best)
+  int plan_ver = 0;                 // bumped whenever a task list is rebuilt (captured graphs go stale)
   DevBasis D{};
   std::vector<void*> allocs;
   // task lists (grouped by channel / cross-coupled block)
@@ -185,6 +186,7 @@ int tiles(int n) { return (n + 63) / 64; }
 // path and the grouped launch keeps each group inside one L2.
 int family_tasks(imsrg_basis_s* B, GemmFamily& f, int tile) {
   f.tile = tile;
+  ++B->plan_ver;
   const int bm = tile_rows(tile), bn = tile_cols(tile);
   const int ng = (int)f.group_end.size();
   std::vector<std::vector<GemmTask>> gt(ng);
@@ -966,6 +968,7 @@ struct imsrg_flow_s {
     }
   };
   std::map<GraphKey, hipGraphExec_t> execs;
+  int plan_ver = -1;   // the basis plan the cached graphs were captured from
   std::vector<void*> allocs;
   // reverse mode: nested-commutator terms of one BCH / Magnus expansion,
   // cotangent ping-pong, Omega's cross-coupled cotangent, and the flow
@@ -1015,6 +1018,11 @@ int run_term(imsrg_flow_s* F, const imsrg_flow_s::GraphKey& key, hipStream_t s, 
     body(s);
     HIPCHK(hipGetLastError());
     return 0;
+  }
+  if (F->plan_ver != F->B->plan_ver) {   // task lists were rebuilt: drop graphs that point at the old ones
+    for (auto& kv : F->execs) (void)hipGraphExecDestroy(kv.second);
+    F->execs.clear();
+    F->plan_ver = F->B->plan_ver;
   }
   auto it = F->execs.find(key);
   if (it == F->execs.end()) {

@@ -1,0 +1,63 @@
+"""GEMM tile autotuner: the stored choice is keyed by device and basis,
+read back from the cache file (or the shipped MI355X table), and applied
+through imsrg_set_gemm_tile; on the GPU the measured choice must leave the
+commutator unchanged (tile shape and XCD packing reorder tiles, not sums)."""
+import json
+
+import numpy as np
+import pytest
+import torch
+
+from conftest import assert_rel, basis
+
+
+def test_choice_round_trip(tmp_path, monkeypatch):
+    from eom_imsrg import tune
+    monkeypatch.setenv("EOM_IMSRG_TUNE_FILE", str(tmp_path / "tiles.json"))
+    b = basis(2)
+    before = tune.current(b)
+    assert tune.lookup(b) is None or isinstance(tune.lookup(b), dict)
+    choice = {"PPPP": 1, "PPX": 8, "XX": 2, "PH": 10}
+    (tmp_path / "tiles.json").write_text(json.dumps({tune.basis_key(b): choice}))
+    assert tune.lookup(b) == choice
+    got = tune.tune(b)                 # from the cache: applied, not measured
+    assert got == {"choice": choice, "source": "cache", "key": tune.basis_key(b)}
+    assert tune.current(b) == choice
+    tune.apply(b, before)
+    assert tune.current(b) == before
+
+
+def test_basis_key_separates_bases():
+    from eom_imsrg import tune
+    assert tune.basis_key(basis(2)) != tune.basis_key(basis(4))
+    assert tune.basis_key(basis(2)) == tune.basis_key(basis(2))
+
+
+def test_shipped_table_is_valid_json():
+    from eom_imsrg import tune
+    d = json.load(open(tune.SHIPPED))
+    for k, c in d.items():
+        assert set(c) == set(tune.FAMILIES), k
+        assert all(v in tune.CANDIDATES for v in c.values()), k
+
+
+@pytest.mark.gpu
+def test_measured_choice_keeps_results(gpu, tmp_path, monkeypatch):
+    from eom_imsrg import SqOp, commutators as cm, synth, tune
+    monkeypatch.setenv("EOM_IMSRG_TUNE_FILE", str(tmp_path / "tiles.json"))
+    b = basis(4)
+    print("[tune] emax=4", flush=True)
+    before = tune.current(b)
+    L = SqOp.from_numpy(b, synth.antihermitian(b, 3, 0.1), -1)
+    R = SqOp.from_numpy(b, synth.hermitian_random(b, 4, 1.0), 1)
+    ref = SqOp.zeros(b, 1)
+    cm.commutator(L, R, ref)
+    torch.cuda.synchronize()
+    out = tune.tune(b, iters=2, force=True)
+    assert out["source"] == "measured" and set(out["choice"]) == set(tune.FAMILIES)
+    assert json.load(open(tmp_path / "tiles.json"))[tune.basis_key(b)] == out["choice"]
+    got = SqOp.zeros(b, 1)
+    cm.commutator(L, R, got)
+    torch.cuda.synchronize()
+    assert_rel(got.numpy(), ref.numpy(), rtol=1e-14, what="commutator after tuning")
+    tune.apply(b, before)
