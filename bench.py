@@ -43,6 +43,8 @@ def _parse():
                     help="instances of the batch whose flowed E0 is checked against the oracle on the host")
     ap.add_argument("--no-emax10", action="store_true", help="skip the eMax 10 grouped + hipGraph steps")
     ap.add_argument("--no-tune", action="store_true", help="keep the default GEMM tiles (no autotuner)")
+    ap.add_argument("--leg", choices=["gradient", "batch", "emax10"], default=None,
+                    help="run only this part and print its JSON object (for profiler runs)")
     ap.add_argument("--kernel-only", default=None,
                     help="launch one kernel class repeatedly and exit (for rocprofv3 counter runs)")
     return ap.parse_args()
@@ -63,9 +65,18 @@ def main() -> int:
         print("bench.py needs a GPU", file=sys.stderr)
         return 1
     dev = torch.device("cuda:0")
-    b = Basis.oxygen16(a.emax)
     from eom_imsrg import tune
+    if a.leg == "batch":
+        print(json.dumps(batch_bench(64, 6, 8, a.batch_cpu_check)))
+        return 0
+    if a.leg == "emax10":
+        print(json.dumps(emax10_bench(3, 1, not a.no_tune)))
+        return 0
+    b = Basis.oxygen16(a.emax)
     tuned = None if a.no_tune else tune.tune(b)
+    if a.leg == "gradient":
+        print(json.dumps(gradient_bench(b, synth.hamiltonian_vacuum(b, seed=0, hw=20.0), a.grad_check)))
+        return 0
     Hv = synth.hamiltonian_vacuum(b, seed=0, hw=20.0)
     H = SqOp.from_numpy(b, Hv, 1)
     cm.normal_order(H)

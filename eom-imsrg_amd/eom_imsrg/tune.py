@@ -37,13 +37,21 @@ def cache_path() -> str:
                           os.path.join(os.path.expanduser("~"), ".cache", "eom_imsrg", "gemm_tiles.json"))
 
 
+def device_tag() -> str:
+    """the GPU's architecture (e.g. gfx950) and compute-unit count"""
+    if not torch.cuda.is_available():
+        return "cpu"
+    p = torch.cuda.get_device_properties(0)
+    arch = (getattr(p, "gcnArchName", "") or p.name or "gpu").split(":")[0]
+    return f"{arch}x{p.multi_processor_count}"
+
+
 def basis_key(b: Basis) -> str:
-    """device name + a digest of the orbit table and hole set"""
+    """device tag + a digest of the orbit table and hole set"""
     h = hashlib.sha1()
     for a in (b.n, b.l, b.jj, b.tz, b.hole):
         h.update(np.ascontiguousarray(np.asarray(a, np.int64)).tobytes())
-    dev = torch.cuda.get_device_name(0) if torch.cuda.is_available() else "cpu"
-    return f"{dev}|norb={b.norb}|{h.hexdigest()[:16]}"
+    return f"{device_tag()}|norb={b.norb}|{h.hexdigest()[:16]}"
 
 
 def _load(path: str) -> dict:
