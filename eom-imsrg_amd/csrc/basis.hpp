@@ -63,6 +63,9 @@ struct DevBasis {
   // many-instance batches: blockIdx.y is the instance of a launch, and each
   // array of an instance follows the previous instance's at a stride of one
   // instance's length (oplen, cclen, wclen, N*N, and the lengths below)
+  const int* php;                  // [2*nphp] (particle index, hole index) pairs in one (l, j, tz) block
+  int nphp;
+  int jsmax;                       // 1 + max 2j: bound on the J count of any orbit pair
   int ninst;                       // grid.y of batched launches (1: one instance)
   const int* act;                  // act[inst] == 0 skips the instance; null: all run
   i64 spart, spscr, sk2scr;        // partial sums, 222_pp_hh split-K scratch (P, K2)

@@ -516,6 +516,23 @@ int build_plan(imsrg_basis_s* B) {
     if ((rc = family_tasks(B, *f, f->tile))) return rc;
   }
   if ((rc = upload(B, B->one_h, &B->one))) return rc;
+  // particle-hole pairs inside one (l, j, tz) block (121's intermediate sums)
+  {
+    std::vector<int> php;
+    int jmx = 0;
+    for (int x = 0; x < b.N; ++x) jmx = std::max(jmx, b.jj[x]);
+    for (int i = 0; i < b.A; ++i)
+      for (int a = 0; a < b.P; ++a) {
+        const int ak = b.parts[a], ik = b.holes[i];
+        if (b.jj[ak] == b.jj[ik] && b.ll[ak] == b.ll[ik] && b.tz[ak] == b.tz[ik]) {
+          php.push_back(a);
+          php.push_back(i);
+        }
+      }
+    B->D.nphp = (int)php.size() / 2;
+    B->D.jsmax = jmx + 1;
+    if ((rc = upload(B, php, (int**)&B->D.php))) return rc;
+  }
   if ((rc = upload(B, B->quad_h, &B->quad))) return rc;
   if ((rc = upload(B, B->part_h, &B->partners))) return rc;
   if ((rc = upload(B, B->rowp_h, &B->rowp))) return rc;

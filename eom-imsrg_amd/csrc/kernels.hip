@@ -330,7 +330,14 @@ void launch_pandya(const DevBasis& B, const CCTask* t, int nt, const double* op,
 }
 
 // ============================================================ commutator_111
-// commutators.f90:36-80.  One thread per one-body output; assigns.
+// commutators.f90:36-80.  One wave per one-body output (assigns); the lanes
+// split the intermediate orbit k and the partial dot products are reduced
+// across the wave.
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
 __global__ __launch_bounds__(256) void k_comm111(DevBasis B, const double* L, int Lh, const double* R, int Rh,
                                                  double* RES) {
   int inst;
@@ -339,51 +346,53 @@ __global__ __launch_bounds__(256) void k_comm111(DevBasis B, const double* L, in
   R += (i64)inst * B.oplen;
   RES += (i64)inst * B.oplen;
   const int A = B.A, P = B.P;
-  const i64 idx = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  const i64 idx = (i64)blockIdx.x * 4 + (threadIdx.x >> 6);   // wave-uniform
+  const int lane = threadIdx.x & 63;
   const double *Lhh = L + B.off_fhh, *Lph = L + B.off_fph, *Lpp = L + B.off_fpp;
   const double *Rhh = R + B.off_fhh, *Rph = R + B.off_fph, *Rpp = R + B.off_fpp;
+  double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
   if (idx < (i64)A * A) {
     const int i = (int)(idx % A), j = (int)(idx / A);
-    double th1ij = 0, th1ji = 0, th2ij = 0, th2ji = 0;
-    for (int k = 0; k < A; ++k) {
-      th1ij += Lhh[i + k * A] * Rhh[k + j * A];
-      th1ji += Lhh[j + k * A] * Rhh[k + i * A];
+    for (int k = lane; k < A; k += 64) {
+      s0 += Lhh[i + k * A] * Rhh[k + j * A];
+      s1 += Lhh[j + k * A] * Rhh[k + i * A];
     }
-    for (int k = 0; k < P; ++k) {
-      th2ij += Lph[k + (i64)i * P] * Rph[k + (i64)j * P];
-      th2ji += Lph[k + (i64)j * P] * Rph[k + (i64)i * P];
+    for (int k = lane; k < P; k += 64) {
+      s2 += Lph[k + (i64)i * P] * Rph[k + (i64)j * P];
+      s3 += Lph[k + (i64)j * P] * Rph[k + (i64)i * P];
     }
-    RES[B.off_fhh + idx] = th1ij + th2ij * Lh - (th1ji * Lh + th2ji) * Rh;
+    s0 = wave_sum(s0); s1 = wave_sum(s1); s2 = wave_sum(s2); s3 = wave_sum(s3);
+    if (lane == 0) RES[B.off_fhh + idx] = s0 + s2 * Lh - (s1 * Lh + s3) * Rh;
     return;
   }
   if (idx < (i64)A * A + (i64)P * A) {
     const i64 x = idx - (i64)A * A;
     const int a = (int)(x % P), i = (int)(x / P);
-    double tb1 = 0, tb2 = 0, tc1 = 0, tc2 = 0;
-    for (int k = 0; k < A; ++k) {
-      tb1 += Lph[a + (i64)k * P] * Rhh[k + i * A];
-      tc1 += Rph[a + (i64)k * P] * Lhh[k + i * A];
+    for (int k = lane; k < A; k += 64) {
+      s0 += Lph[a + (i64)k * P] * Rhh[k + i * A];
+      s1 += Rph[a + (i64)k * P] * Lhh[k + i * A];
     }
-    for (int k = 0; k < P; ++k) {
-      tb2 += Lpp[a + (i64)k * P] * Rph[k + (i64)i * P];
-      tc2 += Rpp[a + (i64)k * P] * Lph[k + (i64)i * P];
+    for (int k = lane; k < P; k += 64) {
+      s2 += Lpp[a + (i64)k * P] * Rph[k + (i64)i * P];
+      s3 += Rpp[a + (i64)k * P] * Lph[k + (i64)i * P];
     }
-    RES[B.off_fph + x] = (tb1 + tb2) - tc1 - tc2;
+    s0 = wave_sum(s0); s1 = wave_sum(s1); s2 = wave_sum(s2); s3 = wave_sum(s3);
+    if (lane == 0) RES[B.off_fph + x] = (s0 + s2) - s1 - s3;
     return;
   }
   if (idx < (i64)A * A + (i64)P * A + (i64)P * P) {
     const i64 x = idx - (i64)A * A - (i64)P * A;
     const int a = (int)(x % P), b = (int)(x / P);
-    double tp1ab = 0, tp1ba = 0, tp2ab = 0, tp2ba = 0;
-    for (int k = 0; k < A; ++k) {
-      tp1ab += Lph[a + (i64)k * P] * Rph[b + (i64)k * P];
-      tp1ba += Lph[b + (i64)k * P] * Rph[a + (i64)k * P];
+    for (int k = lane; k < A; k += 64) {
+      s0 += Lph[a + (i64)k * P] * Rph[b + (i64)k * P];
+      s1 += Lph[b + (i64)k * P] * Rph[a + (i64)k * P];
     }
-    for (int k = 0; k < P; ++k) {
-      tp2ab += Lpp[a + (i64)k * P] * Rpp[k + (i64)b * P];
-      tp2ba += Lpp[b + (i64)k * P] * Rpp[k + (i64)a * P];
+    for (int k = lane; k < P; k += 64) {
+      s2 += Lpp[a + (i64)k * P] * Rpp[k + (i64)b * P];
+      s3 += Lpp[b + (i64)k * P] * Rpp[k + (i64)a * P];
     }
-    RES[B.off_fpp + x] = tp1ab * Rh + tp2ab - (tp1ba + tp2ba * Rh) * Lh;
+    s0 = wave_sum(s0); s1 = wave_sum(s1); s2 = wave_sum(s2); s3 = wave_sum(s3);
+    if (lane == 0) RES[B.off_fpp + x] = s0 * Rh + s2 - (s1 + s3 * Rh) * Lh;
   }
 }
 
@@ -391,7 +400,7 @@ void launch_comm111(const DevBasis& B, const double* L, int Lh, const double* R,
                     hipStream_t s) {
   const i64 n = (i64)B.A * B.A + (i64)B.P * B.A + (i64)B.P * B.P;
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_comm111, dim3((unsigned)((n + 255) / 256), B.ninst), dim3(256), 0, s, B, L, Lh, R, Rh, RES);
+  hipLaunchKernelGGL(k_comm111, dim3((unsigned)((n + 3) / 4), B.ninst), dim3(256), 0, s, B, L, Lh, R, Rh, RES);
 }
 
 __device__ double block_sum(double v, double* red);
@@ -417,36 +426,35 @@ __device__ __forceinline__ void onebody_update(const DevBasis& B, const OneTask&
 }
 
 // ============================================================ commutator_121
-// commutators.f90:83-280 (accumulates).  One workgroup per one-body output;
-// its threads split the (particle a, hole i) intermediates, tree-reduced.
-__global__ __launch_bounds__(128) void k_comm121(DevBasis B, const OneTask* tasks, const double* L, int Lh,
+// commutators.f90:83-280 (accumulates).  One workgroup per one-body output
+// (p, q); its threads take the (particle a, hole i, J) terms, with (a, i)
+// from the list of particle-hole pairs inside one (l, j, tz) block (the
+// only pairs the reference's loop keeps), tree-reduced in LDS.
+__global__ __launch_bounds__(256) void k_comm121(DevBasis B, const OneTask* tasks, const double* L, int Lh,
                                                  const double* R, int Rh, double* RES, int RESh) {
+  __shared__ double red[256];
   int inst;
   if (!batch_inst(B, inst)) return;
   L += (i64)inst * B.oplen;
   R += (i64)inst * B.oplen;
   RES += (i64)inst * B.oplen;
-  __shared__ double red[128];
   const OneTask t = tasks[blockIdx.x];
   const int pk = t.pk, qk = t.qk, jq = B.jj[qk];
   const double* Lph = L + B.off_fph;
   const double* Rph = R + B.off_fph;
+  const int JS = B.jsmax;
   double v = 0.0;
-  for (int x = threadIdx.x; x < B.P * B.A; x += blockDim.x) {
-    const int a = x / B.A, i = x % B.A;
+  for (int x = threadIdx.x; x < B.nphp * JS; x += blockDim.x) {
+    const int pr = x / JS, js = x % JS;
+    const int a = B.php[2 * pr], i = B.php[2 * pr + 1];
     const int ak = B.parts[a], ik = B.holes[i];
-    if (B.jj[ik] != B.jj[ak] || B.ll[ik] != B.ll[ak] || B.tz[ik] != B.tz[ak]) continue;
     const int ji = B.jj[ik];
-    double smx = 0, smy = 0, smx2 = 0, smy2 = 0;
-    for (int JT = abs(ji - jq); JT <= ji + jq; JT += 2) {
-      const VPos p1 = vpos(B, ak, pk, ik, qk, JT);
-      smx = smx + vat(p1, R, Rh) * (JT + 1);
-      smx2 = smx2 + vat(p1, L, Lh) * (JT + 1);
-      const VPos p2 = vpos(B, ik, pk, ak, qk, JT);
-      smy = smy + vat(p2, R, Rh) * (JT + 1);
-      smy2 = smy2 + vat(p2, L, Lh) * (JT + 1);
-    }
-    v = v + Lph[a + (i64)i * B.P] * (Lh * smx - smy) - Rph[a + (i64)i * B.P] * (Rh * smx2 - smy2);
+    const int JT = abs(ji - jq) + 2 * js;
+    if (JT > ji + jq) continue;
+    const VPos p1 = vpos(B, ak, pk, ik, qk, JT);
+    const VPos p2 = vpos(B, ik, pk, ak, qk, JT);
+    const double la = Lph[a + (i64)i * B.P], ra = Rph[a + (i64)i * B.P];
+    v = v + (JT + 1) * (la * (Lh * vat(p1, R, Rh) - vat(p2, R, Rh)) - ra * (Rh * vat(p1, L, Lh) - vat(p2, L, Lh)));
   }
   const double sm = block_sum(v, red);
   if (threadIdx.x == 0) onebody_update(B, t, sm / (B.jj[pk] + 1.0), RES, RESh);
@@ -455,15 +463,15 @@ __global__ __launch_bounds__(128) void k_comm121(DevBasis B, const OneTask* task
 void launch_comm121(const DevBasis& B, const OneTask* t, int nt, const double* L, int Lh, const double* R,
                     int Rh, double* RES, int RESh, hipStream_t s) {
   if (nt <= 0) return;
-  hipLaunchKernelGGL(k_comm121, dim3(nt, B.ninst), dim3(128), 0, s, B, t, L, Lh, R, Rh, RES, RESh);
+  hipLaunchKernelGGL(k_comm121, dim3(nt, B.ninst), dim3(256), 0, s, B, t, L, Lh, R, Rh, RES, RESh);
 }
 
 // ============================================================ commutator_221
 // commutators.f90:435-578 (accumulates; reads w1/w2 of 222_pp_hh, herm 1).
-// One workgroup per one-body output; thread x takes intermediate orbit c
-// (holes first with w1, then particles with w2) and its J sum.  Elements
-// of w2's pppp blocks (W2 = Rh L3 R3^T, rank nh) are formed from L and R
-// here, so 222_pp_hh need not store that block.
+// One workgroup per one-body output; its threads take the (intermediate
+// orbit c, 2J) terms (holes c with w1, particles c with w2), tree-reduced.
+// Elements of w2's pppp blocks (W2 = Rh L3 R3^T, rank nh) are formed from L
+// and R here, so 222_pp_hh need not store that block.
 __device__ __forceinline__ double w2_elem(const DevBasis& B, const double* w2, const double* L, const double* R,
                                           int Rh, int a, int b, int c, int d, int J) {
   const VPos p = vpos(B, a, b, c, d, J);
@@ -480,9 +488,10 @@ __device__ __forceinline__ double w2_elem(const DevBasis& B, const double* w2, c
   return (s * Rh) * p.pre;
 }
 
-__global__ __launch_bounds__(128) void k_comm221(DevBasis B, const OneTask* tasks, const double* L, int Lh,
+__global__ __launch_bounds__(256) void k_comm221(DevBasis B, const OneTask* tasks, const double* L, int Lh,
                                                  const double* R, int Rh, const double* w1, const double* w2,
                                                  double* RES, int RESh) {
+  __shared__ double red[256];
   int inst;
   if (!batch_inst(B, inst)) return;
   L += (i64)inst * B.oplen;
@@ -490,29 +499,32 @@ __global__ __launch_bounds__(128) void k_comm221(DevBasis B, const OneTask* task
   w1 += (i64)inst * B.oplen;
   w2 += (i64)inst * B.oplen;
   RES += (i64)inst * B.oplen;
-  __shared__ double red[128];
   const OneTask t = tasks[blockIdx.x];
   const int ik = t.pk, jk = t.qk, ji = B.jj[ik];
   const int pm = Lh * Rh;
   const bool both = (t.kind != 2);
+  const int JS = B.jsmax;
   double v = 0.0;
-  for (int x = threadIdx.x; x < B.N; x += blockDim.x) {
+  // thread item (c, J): intermediate orbit c (holes first, with w1; then
+  // particles, with w2) and one 2J of its coupling to i
+  for (int y = threadIdx.x; y < B.N * JS; y += blockDim.x) {
+    const int x = y / JS;
     const bool hole = x < B.A;
     const int ck = hole ? B.holes[x] : B.parts[x - B.A];
     const int jc = B.jj[ck];
-    for (int JT = abs(jc - ji); JT <= jc + ji; JT += 2) {
-      if (hole) {
-        if (both)
-          v = v + (v_elem(B, w1, 1, ck, ik, ck, jk, JT) - pm * v_elem(B, w1, 1, ck, jk, ck, ik, JT)) * (JT + 1);
-        else
-          v = v + v_elem(B, w1, 1, ck, ik, ck, jk, JT) * (JT + 1);
-      } else {
-        if (both)
-          v = v + (w2_elem(B, w2, L, R, Rh, ck, ik, ck, jk, JT) - pm * w2_elem(B, w2, L, R, Rh, ck, jk, ck, ik, JT)) *
-                      (JT + 1);
-        else
-          v = v + w2_elem(B, w2, L, R, Rh, ck, ik, ck, jk, JT) * (JT + 1);
-      }
+    const int JT = abs(jc - ji) + 2 * (y % JS);
+    if (JT > jc + ji) continue;
+    if (hole) {
+      if (both)
+        v = v + (v_elem(B, w1, 1, ck, ik, ck, jk, JT) - pm * v_elem(B, w1, 1, ck, jk, ck, ik, JT)) * (JT + 1);
+      else
+        v = v + v_elem(B, w1, 1, ck, ik, ck, jk, JT) * (JT + 1);
+    } else {
+      if (both)
+        v = v + (w2_elem(B, w2, L, R, Rh, ck, ik, ck, jk, JT) - pm * w2_elem(B, w2, L, R, Rh, ck, jk, ck, ik, JT)) *
+                    (JT + 1);
+      else
+        v = v + w2_elem(B, w2, L, R, Rh, ck, ik, ck, jk, JT) * (JT + 1);
     }
   }
   const double sm = block_sum(v, red);
@@ -522,7 +534,7 @@ __global__ __launch_bounds__(128) void k_comm221(DevBasis B, const OneTask* task
 void launch_comm221(const DevBasis& B, const OneTask* t, int nt, const double* L, int Lh, const double* R, int Rh,
                     const double* w1, const double* w2, double* RES, int RESh, hipStream_t s) {
   if (nt <= 0) return;
-  hipLaunchKernelGGL(k_comm221, dim3(nt, B.ninst), dim3(128), 0, s, B, t, L, Lh, R, Rh, w1, w2, RES, RESh);
+  hipLaunchKernelGGL(k_comm221, dim3(nt, B.ninst), dim3(256), 0, s, B, t, L, Lh, R, Rh, w1, w2, RES, RESh);
 }
 
 // ================================================= commutator_110 + 220 (E0)
