@@ -94,6 +94,12 @@ __device__ __forceinline__ double sixj(const DevBasis& B, int j1, int j2, int j3
 
 // (row, col) of element t of the upper triangle (row <= col) of an n x n
 // block, enumerated column by column.
+// sum over the 64 lanes of a wave (every lane receives the total)
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
 // instance of a batched launch; false when it is masked off (the whole
 // workgroup returns together)
 __device__ __forceinline__ bool batch_inst(const DevBasis& B, int& inst) {

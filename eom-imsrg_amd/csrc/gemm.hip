@@ -25,48 +25,48 @@ struct Regs {
   double b[TBN * GBK / 256];
 };
 
-template <int TBM, int TBN>
-__device__ __forceinline__ void fetch(Regs<TBM, TBN>& t, const double* A, int lda, int tA, const double* Bm, int ldb,
-                                      int tB, int M, int N, int K, int m0, int n0, int k0, double scale) {
+// The transposes TA / TB are template parameters (four specialised bodies
+// chosen once per workgroup), and out-of-range elements load a valid
+// address and select zero, so the fetch has no per-element branches.
+template <int TBM, int TBN, bool TA, bool TB>
+__device__ __forceinline__ void fetch(Regs<TBM, TBN>& t, const double* A, int lda, const double* Bm, int ldb, int M,
+                                      int N, int K, int m0, int n0, int k0, double scale) {
   const int tid = threadIdx.x;
 #pragma unroll
   for (int r = 0; r < TBM * GBK / 256; ++r) {
     const int e = tid + 256 * r;
-    int i, kk;
-    if (!tA) { i = e % TBM; kk = e / TBM; }
-    else { kk = e % GBK; i = e / GBK; }
+    const int i = TA ? e / GBK : e % TBM, kk = TA ? e % GBK : e / TBM;
     const int gi = m0 + i, gk = k0 + kk;
-    double v = 0.0;
-    if (gi < M && gk < K) v = tA ? A[gk + (i64)gi * lda] : A[gi + (i64)gk * lda];
-    t.a[r] = v * scale;
+    const bool in = gi < M && gk < K;
+    const int ci = in ? gi : 0, ck = in ? gk : 0;
+    const double v = TA ? A[ck + (i64)ci * lda] : A[ci + (i64)ck * lda];
+    t.a[r] = in ? v * scale : 0.0;
   }
 #pragma unroll
   for (int r = 0; r < TBN * GBK / 256; ++r) {
     const int e = tid + 256 * r;
-    int j, kk;
-    if (!tB) { kk = e % GBK; j = e / GBK; }
-    else { j = e % TBN; kk = e / TBN; }
+    const int j = TB ? e % TBN : e / GBK, kk = TB ? e / TBN : e % GBK;
     const int gj = n0 + j, gk = k0 + kk;
-    double w = 0.0;
-    if (gj < N && gk < K) w = tB ? Bm[gj + (i64)gk * ldb] : Bm[gk + (i64)gj * ldb];
-    t.b[r] = w;
+    const bool in = gj < N && gk < K;
+    const int cj = in ? gj : 0, ck = in ? gk : 0;
+    const double w = TB ? Bm[cj + (i64)ck * ldb] : Bm[ck + (i64)cj * ldb];
+    t.b[r] = in ? w : 0.0;
   }
 }
 
-template <int TBM, int TBN>
-__device__ __forceinline__ void stash(const Regs<TBM, TBN>& t, double (*As)[TBM + 1], double (*Bs)[TBN + 1], int tA,
-                                      int tB) {
+template <int TBM, int TBN, bool TA, bool TB>
+__device__ __forceinline__ void stash(const Regs<TBM, TBN>& t, double (*As)[TBM + 1], double (*Bs)[TBN + 1]) {
   const int tid = threadIdx.x;
 #pragma unroll
   for (int r = 0; r < TBM * GBK / 256; ++r) {
     const int e = tid + 256 * r;
-    if (!tA) As[e / TBM][e % TBM] = t.a[r];
+    if (!TA) As[e / TBM][e % TBM] = t.a[r];
     else As[e % GBK][e / GBK] = t.a[r];
   }
 #pragma unroll
   for (int r = 0; r < TBN * GBK / 256; ++r) {
     const int e = tid + 256 * r;
-    if (!tB) Bs[e % GBK][e / GBK] = t.b[r];
+    if (!TB) Bs[e % GBK][e / GBK] = t.b[r];
     else Bs[e / TBN][e % TBN] = t.b[r];
   }
 }
@@ -76,23 +76,23 @@ __device__ __forceinline__ void stash(const Regs<TBM, TBN>& t, double (*As)[TBM 
 // l of tile (mi, ni) holds C row wm*TBM/2 + mi*16 + l%16, column
 // wn*TBN/2 + ni*16 + l/16 + 4r (16 consecutive rows per row group, so the
 // column-major stores are contiguous).
-template <int TBM, int TBN>
-__device__ __forceinline__ void mma(v4d* acc, double (*As)[GBK][TBM + 1], double (*Bs)[GBK][TBN + 1], const double* A,
-                                    int lda, int tA, const double* Bm, int ldb, int tB, int M, int N, int k0s, int k1,
-                                    int m0, int n0, double scale) {
+template <int TBM, int TBN, bool TA, bool TB>
+__device__ __forceinline__ void mma_t(v4d* acc, double (*As)[GBK][TBM + 1], double (*Bs)[GBK][TBN + 1],
+                                      const double* A, int lda, const double* Bm, int ldb, int M, int N, int k0s,
+                                      int k1, int m0, int n0, double scale) {
   constexpr int MI = TBM / 32, NI = TBN / 32;   // acc[mi * NI + ni]
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wm = wave >> 1, wn = wave & 1;
   const int lr = lane & 15, lk = lane >> 4;
   if (k0s >= k1) return;
   Regs<TBM, TBN> t;
-  fetch<TBM, TBN>(t, A, lda, tA, Bm, ldb, tB, M, N, k1, m0, n0, k0s, scale);
-  stash<TBM, TBN>(t, As[0], Bs[0], tA, tB);
+  fetch<TBM, TBN, TA, TB>(t, A, lda, Bm, ldb, M, N, k1, m0, n0, k0s, scale);
+  stash<TBM, TBN, TA, TB>(t, As[0], Bs[0]);
   __syncthreads();
   int cur = 0;
   for (int k0 = k0s; k0 < k1; k0 += GBK) {
     const bool more = (k0 + GBK < k1);
-    if (more) fetch<TBM, TBN>(t, A, lda, tA, Bm, ldb, tB, M, N, k1, m0, n0, k0 + GBK, scale);
+    if (more) fetch<TBM, TBN, TA, TB>(t, A, lda, Bm, ldb, M, N, k1, m0, n0, k0 + GBK, scale);
 #pragma unroll
     for (int kk = 0; kk < GBK; kk += 4) {
       double a[MI], b[NI];
@@ -106,14 +106,35 @@ __device__ __forceinline__ void mma(v4d* acc, double (*As)[GBK][TBM + 1], double
         for (int ni = 0; ni < NI; ++ni)
           acc[mi * NI + ni] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[ni], a[mi], acc[mi * NI + ni], 0, 0, 0);
     }
-    if (more) stash<TBM, TBN>(t, As[cur ^ 1], Bs[cur ^ 1], tA, tB);
+    if (more) stash<TBM, TBN, TA, TB>(t, As[cur ^ 1], Bs[cur ^ 1]);
     __syncthreads();
     cur ^= 1;
   }
 }
 
+// operand slot k (uniform): a select chain, so the slot table stays in
+// scalar registers instead of an indexed copy in private memory
+__device__ __forceinline__ const double* slot(const Slots& sl, int k, int inst) {
+  const double* r = sl.p[0];
+  i64 st = sl.st[0];
+#pragma unroll
+  for (int x = 1; x < 6; ++x)
+    if (k == x) { r = sl.p[x]; st = sl.st[x]; }
+  return r + (i64)inst * st;
+}
+
 template <int TBM, int TBN>
-__global__ __launch_bounds__(256) void k_gemm_t(Slots sl, const GemmProb* probs, const GemmTask* tasks, XcdMap xm,
+__device__ __forceinline__ void mma(v4d* acc, double (*As)[GBK][TBM + 1], double (*Bs)[GBK][TBN + 1], const double* A,
+                                    int lda, int tA, const double* Bm, int ldb, int tB, int M, int N, int k0s, int k1,
+                                    int m0, int n0, double scale) {
+  if (!tA && !tB) mma_t<TBM, TBN, false, false>(acc, As, Bs, A, lda, Bm, ldb, M, N, k0s, k1, m0, n0, scale);
+  else if (!tA) mma_t<TBM, TBN, false, true>(acc, As, Bs, A, lda, Bm, ldb, M, N, k0s, k1, m0, n0, scale);
+  else if (!tB) mma_t<TBM, TBN, true, false>(acc, As, Bs, A, lda, Bm, ldb, M, N, k0s, k1, m0, n0, scale);
+  else mma_t<TBM, TBN, true, true>(acc, As, Bs, A, lda, Bm, ldb, M, N, k0s, k1, m0, n0, scale);
+}
+
+template <int TBM, int TBN>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TBM == 64 ? 4 : 1))) void k_gemm_t(Slots sl, const GemmProb* probs, const GemmTask* tasks, XcdMap xm,
                                                 double ascale, double s2scale) {
   constexpr int MI = TBM / 32, NI = TBN / 32;
   __shared__ double As[2][GBK][TBM + 1];
@@ -122,8 +143,6 @@ __global__ __launch_bounds__(256) void k_gemm_t(Slots sl, const GemmProb* probs,
   if (ti < 0) return;
   int inst;
   if (!batch_on(sl.act, inst)) return;
-  if (inst > 0)
-    for (int x = 0; x < 6; ++x) sl.p[x] += (i64)inst * sl.st[x];
   const GemmTask t = tasks[ti];
   const GemmProb pr = probs[t.prob];
   const int m0 = t.tm * TBM, n0 = t.tn * TBN;
@@ -135,20 +154,20 @@ __global__ __launch_bounds__(256) void k_gemm_t(Slots sl, const GemmProb* probs,
   // mode bit3: A is upper triangular (as stored); rows m0.. of op(A) reach
   // only k >= m0 (A as stored) or k < m0 + TBM (A transposed)
   const bool tri = pr.mode & 8;
-  mma<TBM, TBN>(acc, As, Bs, sl.p[pr.sa1] + pr.a1, pr.lda, pr.ta1, sl.p[pr.sb1] + pr.b1, pr.ldb, pr.tb1, pr.m, pr.n,
+  mma<TBM, TBN>(acc, As, Bs, slot(sl, pr.sa1, inst) + pr.a1, pr.lda, pr.ta1, slot(sl, pr.sb1, inst) + pr.b1, pr.ldb, pr.tb1, pr.m, pr.n,
                 (tri && !pr.ta1) ? max(kb, m0) : kb, (tri && pr.ta1) ? min(ke, m0 + TBM) : ke, m0, n0, 1.0);
   if (pr.nprod == 2) {
     // second product: same K range, or its own K (k2 > 0) with its own
     // leading dimensions packed in a2/b2's problem (lda2 = ldb2 = m here)
     if (pr.k2 > 0)
-      mma<TBM, TBN>(acc, As, Bs, sl.p[pr.sa2] + pr.a2, pr.m, pr.ta2, sl.p[pr.sb2] + pr.b2, pr.n, pr.tb2, pr.m, pr.n,
+      mma<TBM, TBN>(acc, As, Bs, slot(sl, pr.sa2, inst) + pr.a2, pr.m, pr.ta2, slot(sl, pr.sb2, inst) + pr.b2, pr.n, pr.tb2, pr.m, pr.n,
                     0, pr.k2, m0, n0, pr.s2 * s2scale);
     else
-      mma<TBM, TBN>(acc, As, Bs, sl.p[pr.sa2] + pr.a2, pr.lda, pr.ta2, sl.p[pr.sb2] + pr.b2, pr.ldb, pr.tb2, pr.m,
+      mma<TBM, TBN>(acc, As, Bs, slot(sl, pr.sa2, inst) + pr.a2, pr.lda, pr.ta2, slot(sl, pr.sb2, inst) + pr.b2, pr.ldb, pr.tb2, pr.m,
                     pr.n, (tri && !pr.ta2) ? max(kb, m0) : kb, (tri && pr.ta2) ? min(ke, m0 + TBM) : ke, m0, n0,
                     pr.s2 * ((pr.mode == 0 || (pr.mode & 16)) ? s2scale : 1.0));
   }
-  double* C = const_cast<double*>(sl.p[pr.sc]) + pr.c + (i64)t.ks * pr.cstride;
+  double* C = const_cast<double*>(slot(sl, pr.sc, inst)) + pr.c + (i64)t.ks * pr.cstride;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wm = wave >> 1, wn = wave & 1;
   const double al = pr.mode == 0 ? pr.alpha * ascale

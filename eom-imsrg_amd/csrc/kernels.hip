@@ -333,11 +333,6 @@ void launch_pandya(const DevBasis& B, const CCTask* t, int nt, const double* op,
 // commutators.f90:36-80.  One wave per one-body output (assigns); the lanes
 // split the intermediate orbit k and the partial dot products are reduced
 // across the wave.
-__device__ __forceinline__ double wave_sum(double v) {
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
-}
-
 __global__ __launch_bounds__(256) void k_comm111(DevBasis B, const double* L, int Lh, const double* R, int Rh,
                                                  double* RES) {
   int inst;

@@ -109,11 +109,14 @@ void launch_e0_vjp(const DevBasis& B, const SegTask* g3, int ng3, const double* 
 
 // ------------------------------------------------------- 111 transpose
 // One thread per one-body element of Lb (first E1 threads) or Rb.
+// One wave per one-body cotangent element (Lb block elements first, then
+// Rb); the lanes split the intermediate orbit and reduce across the wave.
 __global__ __launch_bounds__(256) void k_111_vjp(DevBasis B, const double* L, int Lh, const double* R, int Rh,
                                                  const double* RESb, double* Lb, double* Rb) {
   const int A = B.A, P = B.P;
   const i64 E1 = (i64)A * A + (i64)P * A + (i64)P * P;
-  const i64 idx = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  const i64 idx = (i64)blockIdx.x * 4 + (threadIdx.x >> 6);   // wave-uniform
+  const int lane = threadIdx.x & 63;
   if (idx >= 2 * E1) return;
   const bool side = idx >= E1;   // false: Lb, true: Rb
   const i64 x = side ? idx - E1 : idx;
@@ -125,53 +128,60 @@ __global__ __launch_bounds__(256) void k_111_vjp(DevBasis B, const double* L, in
   auto tp1b = [&](int p, int r) { return Rh * Bpp[p + (i64)r * P] - Lh * Bpp[r + (i64)p * P]; };
   auto tp2b = [&](int p, int r) { return Bpp[p + (i64)r * P] - Rh * Lh * Bpp[r + (i64)p * P]; };
   double s = 0.0;
+  double* dst;
   if (x < (i64)A * A) {
     const int i = (int)(x % A), j = (int)(x / A);
     if (!side) {
-      for (int k = 0; k < A; ++k) s += th1b(i, k) * Rhh[j + k * A];
-      for (int p = 0; p < P; ++p) s -= Rph[p + (i64)i * P] * Bph[p + (i64)j * P];
-      Lb[B.off_fhh + x] += s;
+      for (int k = lane; k < A; k += 64) s += th1b(i, k) * Rhh[j + k * A];
+      for (int p = lane; p < P; p += 64) s -= Rph[p + (i64)i * P] * Bph[p + (i64)j * P];
+      dst = Lb + B.off_fhh + x;
     } else {
-      for (int k = 0; k < A; ++k) s += Lhh[k + i * A] * th1b(k, j);
-      for (int p = 0; p < P; ++p) s += Lph[p + (i64)i * P] * Bph[p + (i64)j * P];
-      Rb[B.off_fhh + x] += s;
+      for (int k = lane; k < A; k += 64) s += Lhh[k + i * A] * th1b(k, j);
+      for (int p = lane; p < P; p += 64) s += Lph[p + (i64)i * P] * Bph[p + (i64)j * P];
+      dst = Rb + B.off_fhh + x;
     }
   } else if (x < (i64)A * A + (i64)P * A) {
     const i64 y = x - (i64)A * A;
     const int p = (int)(y % P), i = (int)(y / P);
     if (!side) {
-      for (int j = 0; j < A; ++j) s += Rph[p + (i64)j * P] * th2b(i, j) + Bph[p + (i64)j * P] * Rhh[i + j * A];
-      for (int r = 0; r < P; ++r) s += -Rpp[r + (i64)p * P] * Bph[r + (i64)i * P] + tp1b(p, r) * Rph[r + (i64)i * P];
-      Lb[B.off_fph + y] += s;
+      for (int j = lane; j < A; j += 64) s += Rph[p + (i64)j * P] * th2b(i, j) + Bph[p + (i64)j * P] * Rhh[i + j * A];
+      for (int r = lane; r < P; r += 64)
+        s += -Rpp[r + (i64)p * P] * Bph[r + (i64)i * P] + tp1b(p, r) * Rph[r + (i64)i * P];
+      dst = Lb + B.off_fph + y;
     } else {
-      for (int k = 0; k < A; ++k) s += Lph[p + (i64)k * P] * th2b(k, i) - Bph[p + (i64)k * P] * Lhh[i + k * A];
-      for (int r = 0; r < P; ++r) s += Lpp[r + (i64)p * P] * Bph[r + (i64)i * P] + tp1b(r, p) * Lph[r + (i64)i * P];
-      Rb[B.off_fph + y] += s;
+      for (int k = lane; k < A; k += 64) s += Lph[p + (i64)k * P] * th2b(k, i) - Bph[p + (i64)k * P] * Lhh[i + k * A];
+      for (int r = lane; r < P; r += 64)
+        s += Lpp[r + (i64)p * P] * Bph[r + (i64)i * P] + tp1b(r, p) * Lph[r + (i64)i * P];
+      dst = Rb + B.off_fph + y;
     }
   } else {
     const i64 y = x - (i64)A * A - (i64)P * A;
     const int p = (int)(y % P), r = (int)(y / P);
     if (!side) {
-      for (int i = 0; i < A; ++i) s += Bph[p + (i64)i * P] * Rph[r + (i64)i * P];
-      for (int k = 0; k < P; ++k) s += tp2b(p, k) * Rpp[r + (i64)k * P];
-      Lb[B.off_fpp + y] += s;
+      for (int i = lane; i < A; i += 64) s += Bph[p + (i64)i * P] * Rph[r + (i64)i * P];
+      for (int k = lane; k < P; k += 64) s += tp2b(p, k) * Rpp[r + (i64)k * P];
+      dst = Lb + B.off_fpp + y;
     } else {
-      for (int i = 0; i < A; ++i) s -= Bph[p + (i64)i * P] * Lph[r + (i64)i * P];
-      for (int k = 0; k < P; ++k) s += Lpp[k + (i64)p * P] * tp2b(k, r);
-      Rb[B.off_fpp + y] += s;
+      for (int i = lane; i < A; i += 64) s -= Bph[p + (i64)i * P] * Lph[r + (i64)i * P];
+      for (int k = lane; k < P; k += 64) s += Lpp[k + (i64)p * P] * tp2b(k, r);
+      dst = Rb + B.off_fpp + y;
     }
   }
+  s = wave_sum(s);
+  if (lane == 0) *dst += s;
 }
 
 void launch_111_vjp(const DevBasis& B, const double* L, int Lh, const double* R, int Rh, const double* RESb,
                     double* Lb, double* Rb, hipStream_t s) {
   const i64 n = 2 * ((i64)B.A * B.A + (i64)B.P * B.A + (i64)B.P * B.P);
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_111_vjp, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, B, L, Lh, R, Rh, RESb, Lb, Rb);
+  hipLaunchKernelGGL(k_111_vjp, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, s, B, L, Lh, R, Rh, RESb, Lb, Rb);
 }
 
 // ------------------------------------------------------- 121 transpose
-__global__ __launch_bounds__(128) void k_121_vjp(DevBasis B, const OneTask* tasks, const double* L, int Lh,
+// one workgroup per one-body output, threads over (same-block
+// particle-hole pair, 2J) as in the forward kernel
+__global__ __launch_bounds__(256) void k_121_vjp(DevBasis B, const OneTask* tasks, const double* L, int Lh,
                                                  const double* R, int Rh, const double* RESb, int RESh, double* Lb,
                                                  double* Rb) {
   const OneTask t = tasks[blockIdx.x];
@@ -180,41 +190,39 @@ __global__ __launch_bounds__(128) void k_121_vjp(DevBasis B, const OneTask* task
   if (sb == 0.0) return;
   const double* Lph = L + B.off_fph;
   const double* Rph = R + B.off_fph;
-  for (int x = threadIdx.x; x < B.P * B.A; x += blockDim.x) {
-    const int a = x / B.A, i = x % B.A;
+  const int JS = B.jsmax;
+  for (int x = threadIdx.x; x < B.nphp * JS; x += blockDim.x) {
+    const int pr = x / JS, js = x % JS;
+    const int a = B.php[2 * pr], i = B.php[2 * pr + 1];
     const int ak = B.parts[a], ik = B.holes[i];
-    if (B.jj[ik] != B.jj[ak] || B.ll[ik] != B.ll[ak] || B.tz[ik] != B.tz[ak]) continue;
     const int ji = B.jj[ik];
+    const int JT = abs(ji - jq) + 2 * js;
+    if (JT > ji + jq) continue;
     const i64 o = a + (i64)i * B.P;
     const double lf = Lph[o], rf = Rph[o];
-    double smx = 0, smy = 0, smx2 = 0, smy2 = 0;
-    for (int JT = abs(ji - jq); JT <= ji + jq; JT += 2) {
-      const VPos p1 = vpos(B, ak, pk, ik, qk, JT);
-      const VPos p2 = vpos(B, ik, pk, ak, qk, JT);
-      smx += vat(p1, R, Rh) * (JT + 1);
-      smx2 += vat(p1, L, Lh) * (JT + 1);
-      smy += vat(p2, R, Rh) * (JT + 1);
-      smy2 += vat(p2, L, Lh) * (JT + 1);
-      vadj(p1, Rb, Rh, sb * lf * Lh * (JT + 1));
-      vadj(p2, Rb, Rh, -sb * lf * (JT + 1));
-      vadj(p1, Lb, Lh, -sb * rf * Rh * (JT + 1));
-      vadj(p2, Lb, Lh, sb * rf * (JT + 1));
-    }
-    aadd(Lb + B.off_fph + o, sb * (Lh * smx - smy));
-    aadd(Rb + B.off_fph + o, -sb * (Rh * smx2 - smy2));
+    const VPos p1 = vpos(B, ak, pk, ik, qk, JT);
+    const VPos p2 = vpos(B, ik, pk, ak, qk, JT);
+    const double w = sb * (JT + 1);
+    vadj(p1, Rb, Rh, w * lf * Lh);
+    vadj(p2, Rb, Rh, -w * lf);
+    vadj(p1, Lb, Lh, -w * rf * Rh);
+    vadj(p2, Lb, Lh, w * rf);
+    aadd(Lb + B.off_fph + o, w * (Lh * vat(p1, R, Rh) - vat(p2, R, Rh)));
+    aadd(Rb + B.off_fph + o, -w * (Rh * vat(p1, L, Lh) - vat(p2, L, Lh)));
   }
 }
 
 void launch_121_vjp(const DevBasis& B, const OneTask* t, int nt, const double* L, int Lh, const double* R, int Rh,
                     const double* RESb, int RESh, double* Lb, double* Rb, hipStream_t s) {
   if (nt <= 0) return;
-  hipLaunchKernelGGL(k_121_vjp, dim3(nt), dim3(128), 0, s, B, t, L, Lh, R, Rh, RESb, RESh, Lb, Rb);
+  hipLaunchKernelGGL(k_121_vjp, dim3(nt), dim3(256), 0, s, B, t, L, Lh, R, Rh, RESb, RESh, Lb, Rb);
 }
 
 // ------------------------------------------------------- 221 transpose
 // RESb (one-body) -> cotangents of the w1 / w2 elements 221 reads, added
-// into Wb1 / Wb2 (which already hold the 222_pp_hh part)
-__global__ __launch_bounds__(128) void k_221_vjp(DevBasis B, const OneTask* tasks, int Lh, int Rh,
+// into Wb1 / Wb2 (which already hold the 222_pp_hh part); threads over
+// (intermediate orbit c, 2J)
+__global__ __launch_bounds__(256) void k_221_vjp(DevBasis B, const OneTask* tasks, int Lh, int Rh,
                                                  const double* RESb, int RESh, double* Wb1, double* Wb2) {
   const OneTask t = tasks[blockIdx.x];
   const int ik = t.pk, jk = t.qk, ji = B.jj[ik];
@@ -222,22 +230,24 @@ __global__ __launch_bounds__(128) void k_221_vjp(DevBasis B, const OneTask* task
   const bool both = (t.kind != 2);
   const double sb = fold1(B, t, RESb, RESh) / (ji + 1.0);
   if (sb == 0.0) return;
-  for (int x = threadIdx.x; x < B.N; x += blockDim.x) {
+  const int JS = B.jsmax;
+  for (int y = threadIdx.x; y < B.N * JS; y += blockDim.x) {
+    const int x = y / JS;
     const bool hole = x < B.A;
     const int ck = hole ? B.holes[x] : B.parts[x - B.A];
     double* W = hole ? Wb1 : Wb2;
     const int jc = B.jj[ck];
-    for (int JT = abs(jc - ji); JT <= jc + ji; JT += 2) {
-      vadj(vpos(B, ck, ik, ck, jk, JT), W, 1, sb * (JT + 1));
-      if (both) vadj(vpos(B, ck, jk, ck, ik, JT), W, 1, -pm * sb * (JT + 1));
-    }
+    const int JT = abs(jc - ji) + 2 * (y % JS);
+    if (JT > jc + ji) continue;
+    vadj(vpos(B, ck, ik, ck, jk, JT), W, 1, sb * (JT + 1));
+    if (both) vadj(vpos(B, ck, jk, ck, ik, JT), W, 1, -pm * sb * (JT + 1));
   }
 }
 
 void launch_221_vjp(const DevBasis& B, const OneTask* t, int nt, int Lh, int Rh, const double* RESb, int RESh,
                     double* Wb1, double* Wb2, hipStream_t s) {
   if (nt <= 0) return;
-  hipLaunchKernelGGL(k_221_vjp, dim3(nt), dim3(128), 0, s, B, t, Lh, Rh, RESb, RESh, Wb1, Wb2);
+  hipLaunchKernelGGL(k_221_vjp, dim3(nt), dim3(256), 0, s, B, t, Lh, Rh, RESb, RESh, Wb1, Wb2);
 }
 
 // ------------------------------------------- 222_pp_hh cotangent blocks
