@@ -134,7 +134,7 @@ __device__ __forceinline__ void mma(v4d* acc, double (*As)[GBK][TBM + 1], double
 }
 
 template <int TBM, int TBN>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TBM == 64 ? 4 : 1))) void k_gemm_t(Slots sl, const GemmProb* probs, const GemmTask* tasks, XcdMap xm,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TBM == 64 ? 4 : (TBN == 64 ? 2 : 1)))) void k_gemm_t(Slots sl, const GemmProb* probs, const GemmTask* tasks, XcdMap xm,
                                                 double ascale, double s2scale) {
   constexpr int MI = TBM / 32, NI = TBN / 32;
   __shared__ double As[2][GBK][TBM + 1];
