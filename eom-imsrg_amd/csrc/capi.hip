@@ -59,6 +59,7 @@ struct imsrg_basis_s {
   TaskList<SegTask> seg_tri, seg_full, seg_g3, seg_epi;
   TaskList<CCTask> cc;
   TaskList<PairTask> pairs;
+  TaskList<PairTask> symt;          // 64 x 64 tile pairs of every square block (pad = g)
   GemmFamily pp1, k2, k3, ph;       // pppp L1 R1, pp x (ph|hh), (ph|hh)^2, cross-coupled W'
   std::vector<Partner> part_h;      // 122 partner lists (CSR by global pair id)
   std::vector<int> rowp_h, colp_h;
@@ -516,6 +517,16 @@ int build_plan(imsrg_basis_s* B) {
     if ((rc = family_tasks(B, *f, f->tile))) return rc;
   }
   if ((rc = upload(B, B->one_h, &B->one))) return rc;
+  B->symt.host.clear();
+  for (int q = 0; q < b.nblocks; ++q)
+    for (int g : {0, 3, 4}) {
+      const int n = b.dims[q * 3 + (g == 0 ? 0 : (g == 3 ? 1 : 2))];
+      const int nt = (n + 63) / 64;
+      for (int tj = 0; tj < nt; ++tj)
+        for (int ti = 0; ti <= tj; ++ti) B->symt.host.push_back(PairTask{q, ti, tj, g});
+    }
+  B->symt.n = (int)B->symt.host.size();
+  if ((rc = upload(B, B->symt.host, &B->symt.dev))) return rc;
   // particle-hole pairs inside one (l, j, tz) block (121's intermediate sums)
   {
     std::vector<int> php;
@@ -657,10 +668,10 @@ void t_121(const Exec& X, const double* L, int Lh, const double* R, int Rh, doub
   launch_comm121(X.D, X.B->one, (int)X.B->one_h.size(), L, Lh, R, Rh, RES, RESh, s);
 }
 void t_122(const Exec& X, const double* L, int Lh, const double* R, int Rh, double* RES, int RESh,
-           hipStream_t s) {
+           hipStream_t s, bool mirror = true) {
   imsrg_basis_s* B = X.B;
   run(B, B->seg_tri, [&](const SegTask* t, int n) {
-    launch_comm122p(X.D, t, n, B->partners, B->rowp, B->colp, L, Lh, R, Rh, RES, RESh, s);
+    launch_comm122p(X.D, t, n, B->partners, B->rowp, B->colp, L, Lh, R, Rh, RES, RESh, s, mirror ? 1 : 0);
   });
 }
 void t_pphh(const Exec& X, const double* L, int Lh, const double* R, int Rh, double* RES, int RESh,
@@ -695,14 +706,14 @@ void t_pandya(const Exec& X, const double* op, int herm, double* CC, hipStream_t
   run(X.B, X.B->cc, [&](const CCTask* t, int n) { launch_pandya(X.D, t, n, op, herm, CC, s); });
 }
 void t_ph(const Exec& X, const double* LCC, int Lh, const double* RCC, int Rh, double* RES, int RESh,
-          double* WCC, hipStream_t s) {
+          double* WCC, hipStream_t s, bool mirror = true) {
   imsrg_basis_s* B = X.B;
   Slots sl{};
   sl.p[0] = LCC; sl.p[1] = RCC; sl.p[2] = WCC;
   sl.st[0] = B->b.cclen; sl.st[1] = B->b.cclen; sl.st[2] = B->b.wclen;
   run_gemm_x(X, B->ph, sl, s, (double)Lh, -(double)(Rh * Lh));
   if (B->path == IMSRG_PATH_GROUPED)
-    launch_comm222ph_quad(X.D, B->quad, (int)B->quad_h.size(), WCC, Lh, Rh, RES, RESh, s);
+    launch_comm222ph_quad(X.D, B->quad, (int)B->quad_h.size(), WCC, Lh, Rh, RES, RESh, s, mirror ? 1 : 0);
   else
     run(B, B->seg_tri,
         [&](const SegTask* t, int n) { launch_comm222ph_scatter(X.D, t, n, WCC, Lh, Rh, RES, RESh, s); });
@@ -720,14 +731,20 @@ void t_norm(const Exec& X, const double* op, double* dst, hipStream_t s) {
 // cross-coupled L; RCC is computed here.
 void t_full(const Exec& X, const double* L, int Lh, const double* R, int Rh, double* RES, int RESh, bool e0,
             double* w1, double* w2, const double* LCC, double* RCC, double* WCC, hipStream_t s) {
+  // grouped: 122 and 222_ph write the upper triangles of the square blocks
+  // and one symmetrising pass fills the lower ones (every term's
+  // contribution is herm-symmetric, so the lower triangles need not be
+  // accumulated on the way)
+  const bool mirror = X.B->path != IMSRG_PATH_GROUPED;
   t_pandya(X, R, Rh, RCC, s);
   if (e0) t_e0(X, L, Lh, R, Rh, RES, s);
   t_111(X, L, Lh, R, Rh, RES, s);
   t_121(X, L, Lh, R, Rh, RES, RESh, s);
-  t_122(X, L, Lh, R, Rh, RES, RESh, s);
+  t_122(X, L, Lh, R, Rh, RES, RESh, s, mirror);
   t_pphh(X, L, Lh, R, Rh, RES, RESh, w1, w2, s, 0);
   t_221(X, L, Lh, R, Rh, RES, RESh, w1, w2, s);
-  t_ph(X, LCC, Lh, RCC, Rh, RES, RESh, WCC, s);
+  t_ph(X, LCC, Lh, RCC, Rh, RES, RESh, WCC, s, mirror);
+  if (!mirror) launch_sym_lower(X.D, X.B->symt.dev, X.B->symt.n, RES, RESh, s);
 }
 
 
@@ -2268,14 +2285,14 @@ int imsrg_profile_kernel(imsrg_basis B, int which, const double* L, int Lh, cons
         run_gemm(B, B->ph, sl, s, (double)Lh, -(double)(Rh * Lh));
         break;
       }
-      case IMSRG_K_PH_SCATTER:
+      case IMSRG_K_PH_SCATTER:   // as the grouped commutator runs it: upper triangles only
         if (B->path == IMSRG_PATH_GROUPED)
-          launch_comm222ph_quad(B->D, B->quad, (int)B->quad_h.size(), WCC, Lh, Rh, RES, RESh, s);
+          launch_comm222ph_quad(B->D, B->quad, (int)B->quad_h.size(), WCC, Lh, Rh, RES, RESh, s, 0);
         else
           run(B, B->seg_tri,
               [&](const SegTask* t, int n) { launch_comm222ph_scatter(B->D, t, n, WCC, Lh, Rh, RES, RESh, s); });
         break;
-      case IMSRG_K_122: t_122(basis_exec(B), L, Lh, R, Rh, RES, RESh, s); break;
+      case IMSRG_K_122: t_122(basis_exec(B), L, Lh, R, Rh, RES, RESh, s, B->path != IMSRG_PATH_GROUPED); break;
       case IMSRG_K_PANDYA: t_pandya(basis_exec(B), R, Rh, RCC, s); break;
       case IMSRG_K_COPY:   // first half of WCC into its second half: wclen/2 doubles each way
         launch_copy(WCC + B->b.wclen / 2, WCC, B->b.wclen / 2, s);

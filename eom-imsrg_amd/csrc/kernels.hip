@@ -86,7 +86,7 @@ __device__ __forceinline__ int blk_of(int lo, int hi) { return lo == 0 ? hi : (l
 
 __global__ __launch_bounds__(256) void k_comm122p(DevBasis B, const SegTask* tasks, const Partner* part,
                                                   const int* rowp, const int* colp, const double* L, int Lh,
-                                                  const double* R, int Rh, double* RES, int RESh) {
+                                                  const double* R, int Rh, double* RES, int RESh, int mirror) {
   int inst;
   if (!batch_inst(B, inst)) return;
   L += (i64)inst * B.oplen;
@@ -150,14 +150,15 @@ __global__ __launch_bounds__(256) void k_comm122p(DevBasis B, const SegTask* tas
   sm = sm / sqrt(1.0 + (a == b ? 1.0 : 0.0)) / sqrt(1.0 + (c == d ? 1.0 : 0.0));
   const i64 base = go[g];
   RES[base + IX + (i64)JX * n1] = sm;
-  if (sq) RES[base + JX + (i64)IX * n1] = sm * RESh;
+  if (sq && mirror) RES[base + JX + (i64)IX * n1] = sm * RESh;
 }
 
 void launch_comm122p(const DevBasis& B, const SegTask* t, int nt, const Partner* part, const int* rowp,
                      const int* colp, const double* L, int Lh, const double* R, int Rh, double* RES, int RESh,
-                     hipStream_t s) {
+                     hipStream_t s, int mirror) {
   if (nt <= 0) return;
-  hipLaunchKernelGGL(k_comm122p, dim3(nt, B.ninst), dim3(kTile), 0, s, B, t, part, rowp, colp, L, Lh, R, Rh, RES, RESh);
+  hipLaunchKernelGGL(k_comm122p, dim3(nt, B.ninst), dim3(kTile), 0, s, B, t, part, rowp, colp, L, Lh, R, Rh, RES, RESh,
+                     mirror);
 }
 
 // ================================================== commutator_222_ph scatter
@@ -174,7 +175,8 @@ __device__ __forceinline__ double wprime(const double* W, i64 rr, int x, int y, 
 }
 
 __device__ __forceinline__ void ph_element(const DevBasis& B, const double* WCC, int Lh, int Rh, double* RES,
-                                           int RESh, int q, int g, int IX, int JX, int i, int j, int k, int l) {
+                                           int RESh, int q, int g, int IX, int JX, int i, int j, int k, int l,
+                                           bool mirror = true) {
   const int n1 = brows(B, q, g);
   const bool sq = is_square(g);
   const double pre = (i == j) ? IMSRG_INV_SQRT2_REF : 1.0;
@@ -229,7 +231,7 @@ __device__ __forceinline__ void ph_element(const DevBasis& B, const double* WCC,
   double* X = RES + B.goff[q * 6 + g];
   const i64 o = IX + (i64)JX * n1;
   X[o] = X[o] + sm * pre * pre2;
-  if (sq) X[JX + (i64)IX * n1] = X[o] * RESh;
+  if (sq && mirror) X[JX + (i64)IX * n1] = X[o] * RESh;
 }
 
 // per-channel form: segment tiles of one channel (plain path)
@@ -263,7 +265,7 @@ void launch_comm222ph_scatter(const DevBasis& B, const SegTask* t, int nt, const
 // J fastest, so the lanes of one quadruple read identical WCC addresses
 // (one request per wave load) and neighbouring 6j symbols.
 __global__ __launch_bounds__(256) void k_comm222ph_quad(DevBasis B, const QuadItem* items, int n, const double* WCC,
-                                                        int Lh, int Rh, double* RES, int RESh) {
+                                                        int Lh, int Rh, double* RES, int RESh, int mirror) {
   int inst;
   if (!batch_inst(B, inst)) return;
   WCC += (i64)inst * B.wclen;
@@ -279,13 +281,46 @@ __global__ __launch_bounds__(256) void k_comm222ph_quad(DevBasis B, const QuadIt
   const int g = (rc == 0) ? cc : (rc == 1 ? (cc == 1 ? 3 : 5) : 4);
   const int IX = B.xmap[((i64)i * B.N + j) * B.nJ + (J >> 1)];
   const int JX = B.xmap[((i64)k * B.N + l) * B.nJ + (J >> 1)];
-  ph_element(B, WCC, Lh, Rh, RES, RESh, q, g, IX, JX, i, j, k, l);
+  ph_element(B, WCC, Lh, Rh, RES, RESh, q, g, IX, JX, i, j, k, l, mirror != 0);
 }
 
 void launch_comm222ph_quad(const DevBasis& B, const QuadItem* items, int n, const double* W, int Lh, int Rh,
-                           double* RES, int RESh, hipStream_t s) {
+                           double* RES, int RESh, hipStream_t s, int mirror) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_comm222ph_quad, dim3((n + 255) / 256, B.ninst), dim3(256), 0, s, B, items, n, W, Lh, Rh, RES, RESh);
+  hipLaunchKernelGGL(k_comm222ph_quad, dim3((n + 255) / 256, B.ninst), dim3(256), 0, s, B, items, n, W, Lh, Rh, RES,
+                     RESh, mirror);
+}
+
+// lower triangle of every square block (pppp, phph, hhhh) from its upper
+// triangle: X(y, x) = herm X(x, y) for x < y, by 64 x 64 tile pairs (tile
+// rows ti <= tile columns tj; task.pad = block g) through LDS, coalesced
+// both ways.  Lets the fused commutator's 122 and 222_ph write only the
+// upper triangles.
+__global__ __launch_bounds__(256) void k_sym_lower(DevBasis B, const PairTask* tasks, double* RES, int herm) {
+  __shared__ double T[64][65];
+  int inst;
+  if (!batch_inst(B, inst)) return;
+  RES += (i64)inst * B.oplen;
+  const PairTask t = tasks[blockIdx.x];
+  const int q = t.q, g = t.pad;
+  const int n = brows(B, q, g);
+  double* X = RES + B.goff[q * 6 + g];
+  const int x0 = t.ti * 64, y0 = t.tj * 64;
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  for (int c = ty; c < 64; c += 4) {   // upper tile, column y0 + c
+    const int x = x0 + tx, y = y0 + c;
+    T[c][tx] = (x < n && y < n && x < y) ? X[x + (i64)y * n] : 0.0;
+  }
+  __syncthreads();
+  for (int c = ty; c < 64; c += 4) {   // lower tile, column x0 + c
+    const int yy = y0 + tx, xx = x0 + c;
+    if (yy < n && xx < n && xx < yy) X[yy + (i64)xx * n] = herm * T[tx][c];
+  }
+}
+
+void launch_sym_lower(const DevBasis& B, const PairTask* t, int nt, double* RES, int herm, hipStream_t s) {
+  if (nt <= 0) return;
+  hipLaunchKernelGGL(k_sym_lower, dim3(nt, B.ninst), dim3(256), 0, s, B, t, RES, herm);
 }
 
 // =========================================================== Pandya forward

@@ -94,10 +94,12 @@ void launch_comm122(const DevBasis& B, const SegTask* t, int nt, const double* L
 void launch_comm222ph_scatter(const DevBasis& B, const SegTask* t, int nt, const double* W, int Lh, int Rh,
                               double* RES, int RESh, hipStream_t s);
 void launch_comm222ph_quad(const DevBasis& B, const QuadItem* items, int n, const double* W, int Lh, int Rh,
-                           double* RES, int RESh, hipStream_t s);
+                           double* RES, int RESh, hipStream_t s, int mirror = 1);
+// square blocks' lower triangles from their upper triangles (herm)
+void launch_sym_lower(const DevBasis& B, const PairTask* t, int nt, double* RES, int herm, hipStream_t s);
 void launch_comm122p(const DevBasis& B, const SegTask* t, int nt, const Partner* part, const int* rowp,
                      const int* colp, const double* L, int Lh, const double* R, int Rh, double* RES, int RESh,
-                     hipStream_t s);
+                     hipStream_t s, int mirror = 1);
 void launch_pandya(const DevBasis& B, const CCTask* t, int nt, const double* op, int herm, double* CC,
                    hipStream_t s);
 void launch_comm111(const DevBasis& B, const double* L, int Lh, const double* R, int Rh, double* RES,

@@ -266,8 +266,8 @@ int imsrg_get_gemm_tile(imsrg_basis b, int family);
 #define IMSRG_K_PPPP 0      /* 222_pp_hh pppp block: L1 R1 MFMA GEMM + symmetric epilogue */
 #define IMSRG_K_PPHH_SMALL 1 /* 222_pp_hh pp x (ph|hh) and (ph|hh)^2 GEMMs + epilogue */
 #define IMSRG_K_PH_GEMM 2   /* 222_ph WCC = LCC RCC^T (MFMA) */
-#define IMSRG_K_PH_SCATTER 3 /* 222_ph inverse Pandya */
-#define IMSRG_K_122 4
+#define IMSRG_K_PH_SCATTER 3 /* 222_ph inverse Pandya (grouped: upper triangles, as in the commutator) */
+#define IMSRG_K_122 4        /* commutator_122 (grouped: upper triangles, as in the commutator) */
 #define IMSRG_K_PANDYA 5
 #define IMSRG_K_ONEBODY 6   /* 111 + 121 + 221 */
 #define IMSRG_K_COMMUTATOR 7 /* the whole [L, R] */
