@@ -200,10 +200,10 @@ def main() -> int:
 
 
 def gradient_bench(b, Hv, ncheck: int):
-    """dE0/dV of the full flow (|mbpt2| < 1e-6, Omega checkpointed every 4
-    steps): time of a plain forward flow, of the checkpointed forward plus the
-    reverse sweep, and a central-difference check (h = 1e-5 MeV) on `ncheck`
-    seeded hermitian two-body elements of the vacuum interaction"""
+    """dE0/dV of the full flow (|mbpt2| < 1e-6, Omega and H(s) checkpointed
+    every 4 steps): time of a plain forward flow, of the checkpointed forward
+    plus the reverse sweep, and a central-difference check (h = 1e-5 MeV) on
+    `ncheck` seeded hermitian two-body elements of the vacuum interaction"""
     import numpy as np
     import torch
     from eom_imsrg import Flow, SqOp, adjoint, commutators as cm

@@ -984,6 +984,14 @@ void comm_vjp(imsrg_basis_s* B, const double* L, int Lh, const double* R, int Rh
 
 }  // namespace
 
+// Nested-commutator terms of one BCH / Magnus expansion kept for its
+// transpose: A[0] is the expanded operator, A[k] = [Omega, A[k-1]], c[k] the
+// coefficient of A[k] in the sum.
+struct TermRec {
+  std::vector<const double*> A;
+  std::vector<double> c;
+};
+
 // =================================================================== flow
 struct imsrg_flow_s {
   imsrg_basis_s* B = nullptr;
@@ -1009,7 +1017,9 @@ struct imsrg_flow_s {
   // gradient's checkpoints and segment states
   std::vector<double*> pool;   // recorded nested-commutator terms
   double *Ab = nullptr, *Prev = nullptr, *GCCb = nullptr, *mDG = nullptr;
-  std::vector<double*> ckpts, segG, segH, segE;
+  std::vector<double*> ckpts, hckpts, segG, segH, segE;   // Omega and H(s) every ckpt steps; one segment's states
+  std::vector<TermRec> lmrec, lbrec;   // terms of the last segment's expansions, kept by the forward sweep
+  int last_n0 = -1;                    // first step of that segment (-1: none kept)
   double* decDG = nullptr;   // generator buffer of imsrg_magnus_decouple
   bool fw_ok = false;        // a checkpointed forward sweep is held
   int fw_steps = 0, fw_ckpt = 1;
@@ -1076,6 +1086,16 @@ int run_term(imsrg_flow_s* F, const imsrg_flow_s::GraphKey& key, hipStream_t s, 
   return 0;
 }
 
+int pool_get(imsrg_flow_s* F, size_t& idx, double** out) {
+  while (F->pool.size() <= idx) {
+    double* p = nullptr;
+    if (int rc = falloc(F, F->B->b.oplen, &p)) return rc;
+    F->pool.push_back(p);
+  }
+  *out = F->pool[idx++];
+  return 0;
+}
+
 int read_scalar(imsrg_basis_s* B, const double* dev, double* out, hipStream_t s) {
   HIPCHK(hipMemcpyAsync(B->hval, dev, sizeof(double), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
@@ -1085,9 +1105,13 @@ int read_scalar(imsrg_basis_s* B, const double* dev, double* out, hipStream_t s)
 
 // BCH_EXPAND, IMSRG_MAGNUS.f90:149-234
 int bch(imsrg_flow_s* F, double* HS, const double* G, int Gh, const double* H, int Hh, double conv, int maxit,
-        int* nterms, double* advals, hipStream_t s) {
+        int* nterms, double* advals, hipStream_t s, TermRec* rec = nullptr, size_t* pidx = nullptr) {
   imsrg_basis_s* B = F->B;
   const i64 n = B->b.oplen;
+  if (rec) {   // keep every term (for a transpose of this expansion)
+    rec->A.assign(1, H);
+    rec->c.assign(1, 1.0);
+  }
   HIPCHK(hipMemcpyAsync(HS, H, n * sizeof(double), hipMemcpyDeviceToDevice, s));
   HIPCHK(hipMemcpyAsync(F->buf[0], H, n * sizeof(double), hipMemcpyDeviceToDevice, s));
   // G is constant over the expansion: its cross-coupled form once
@@ -1106,6 +1130,13 @@ int bch(imsrg_flow_s* F, double* HS, const double* G, int Gh, const double* H, i
       t_norm(F->X, INT2, B->dval, cs);
     });
     if (rc) return rc;
+    if (rec) {
+      double* keep = nullptr;
+      if ((rc = pool_get(F, *pidx, &keep))) return rc;
+      HIPCHK(hipMemcpyAsync(keep, INT2, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+      rec->A.push_back(keep);
+      rec->c.push_back(coef);
+    }
     double nrm = 0.0;
     if ((rc = read_scalar(B, B->dval, &nrm, s))) return rc;
     const double adv = nrm * coef;
@@ -1119,7 +1150,7 @@ int bch(imsrg_flow_s* F, double* HS, const double* G, int Gh, const double* H, i
 
 // MAGNUS_EXPAND, IMSRG_MAGNUS.f90:620-691
 int magnus(imsrg_flow_s* F, double* DG, int DGh, const double* G, int Gh, double conv, int* ncomm,
-           hipStream_t s) {
+           hipStream_t s, TermRec* rec = nullptr, size_t* pidx = nullptr, const double* eta = nullptr) {
   imsrg_basis_s* B = F->B;
   const i64 n = B->b.oplen;
   const double cof[7] = {1.0, -0.5, .0833333333, 0.0, -0.00138888888, 0.0, 3.306878e-5};
@@ -1135,6 +1166,10 @@ int magnus(imsrg_flow_s* F, double* DG, int DGh, const double* G, int Gh, double
   advals[0] = B->hval[0];
   const double fullnorm = B->hval[1];
   if (ncomm) *ncomm = 0;
+  if (rec) {   // keep every term; eta = a copy of DG on entry
+    rec->A.assign(1, eta);
+    rec->c.assign(1, 1.0);
+  }
   if (fullnorm < (double)1e-9f) return 0;
   t_pandya(F->X, G, Gh, F->GCC, s);
   int q = 1, cnt = 0;
@@ -1153,6 +1188,13 @@ int magnus(imsrg_flow_s* F, double* DG, int DGh, const double* G, int Gh, double
     });
     if (rc) return rc;
     ++cnt;
+    if (rec) {
+      double* keep = nullptr;
+      if ((rc = pool_get(F, *pidx, &keep))) return rc;
+      HIPCHK(hipMemcpyAsync(keep, INT2, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+      rec->A.push_back(keep);
+      rec->c.push_back(c);
+    }
     double nrm = 0.0;
     if ((rc = read_scalar(B, B->dval, &nrm, s))) return rc;
     advals[i - 1] = nrm * std::fabs(cof[i - 1]);
@@ -1362,24 +1404,6 @@ int grow(imsrg_flow_s* F, std::vector<double*>& v, size_t k) {
   return 0;
 }
 
-// Nested-commutator terms of one BCH / Magnus expansion kept for its
-// transpose: A[0] is the expanded operator, A[k] = [Omega, A[k-1]], c[k] the
-// coefficient of A[k] in the sum.
-struct TermRec {
-  std::vector<const double*> A;
-  std::vector<double> c;
-};
-
-int pool_get(imsrg_flow_s* F, size_t& idx, double** out) {
-  while (F->pool.size() <= idx) {
-    double* p = nullptr;
-    if (int rc = falloc(F, F->B->b.oplen, &p)) return rc;
-    F->pool.push_back(p);
-  }
-  *out = F->pool[idx++];
-  return 0;
-}
-
 // BCH_EXPAND (IMSRG_MAGNUS.f90:149-234) keeping its terms; HS (optional)
 // receives the sum.  Same kernels and order as bch(), so HS is identical.
 int bch_record(imsrg_flow_s* F, const double* G, const double* H, int Hh, double conv, int maxit, TermRec& rec,
@@ -1561,15 +1585,22 @@ int flow_forward(imsrg_flow_s* F, const double* T, double ds, double bch_conv, d
                  int maxsteps, int ckpt, double* E0, int* nsteps, hipStream_t s) {
   imsrg_basis_s* B = F->B;
   const i64 n = B->b.oplen;
+  const size_t nb = (size_t)n * sizeof(double);
   int rc, nt = 0, nc = 0;
   if (ckpt < 1) return fail(IMSRG_EINVAL, "checkpoint interval must be >= 1");
   if ((rc = ensure_adj(B))) return rc;
   for (double** p : {&F->gH, &F->gHS, &F->gG, &F->gDG, &F->gHSb, &F->gGb, &F->gHb, &F->gdOb, &F->getab})
     if ((rc = fbuf(F, p, n))) return rc;
+  if ((rc = grow(F, F->segG, (size_t)ckpt)) || (rc = grow(F, F->segH, (size_t)ckpt)) ||
+      (rc = grow(F, F->segE, (size_t)ckpt)))
+    return rc;
   F->fw_ok = false;
-  HIPCHK(hipMemcpyAsync(F->gH, T, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+  F->last_n0 = -1;
+  F->lmrec.assign(ckpt, TermRec());
+  F->lbrec.assign(ckpt, TermRec());
+  HIPCHK(hipMemcpyAsync(F->gH, T, nb, hipMemcpyDeviceToDevice, s));
   launch_normal_order(B->D, F->gH, B->dval + 8, s);
-  HIPCHK(hipMemcpyAsync(F->gHS, F->gH, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+  HIPCHK(hipMemcpyAsync(F->gHS, F->gH, nb, hipMemcpyDeviceToDevice, s));
   launch_fill(F->gG, n, 0.0, s);
   launch_fill(F->gDG, n, 0.0, s);
   if ((rc = white(F->X, F->gHS, F->gDG, s)) || (rc = mbpt2_dev(F->X, F->gHS, B->dval + 2, s))) return rc;
@@ -1577,19 +1608,35 @@ int flow_forward(imsrg_flow_s* F, const double* T, double ds, double bch_conv, d
   if ((rc = read_scalar(B, B->dval + 2, &crit, s))) return rc;
   crit = std::fabs(crit);
   int steps = 0;
+  size_t pidx = 0;
+  // Omega and H(s) are checkpointed every ckpt steps; the states and the
+  // nested-commutator terms of the current segment are kept as the reverse
+  // sweep would recompute them, so the last segment is not recomputed
   while (crit > crit_tol) {
     if (steps >= maxsteps) return fail(IMSRG_ESTEPS, "flow gradient: step limit reached");
-    if (steps % ckpt == 0) {
-      if ((rc = grow(F, F->ckpts, (size_t)(steps / ckpt)))) return rc;
-      HIPCHK(hipMemcpyAsync(F->ckpts[steps / ckpt], F->gG, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    const int k = steps % ckpt;
+    if (k == 0) {
+      const size_t c = (size_t)(steps / ckpt);
+      if ((rc = grow(F, F->ckpts, c)) || (rc = grow(F, F->hckpts, c))) return rc;
+      HIPCHK(hipMemcpyAsync(F->ckpts[c], F->gG, nb, hipMemcpyDeviceToDevice, s));
+      HIPCHK(hipMemcpyAsync(F->hckpts[c], F->gHS, nb, hipMemcpyDeviceToDevice, s));
+      pidx = 0;
     }
-    if ((rc = magnus(F, F->gDG, -1, F->gG, -1, mag_conv, &nc, s))) return rc;
+    HIPCHK(hipMemcpyAsync(F->segG[k], F->gG, nb, hipMemcpyDeviceToDevice, s));
+    HIPCHK(hipMemcpyAsync(F->segH[k], F->gHS, nb, hipMemcpyDeviceToDevice, s));
+    HIPCHK(hipMemcpyAsync(F->segE[k], F->gDG, nb, hipMemcpyDeviceToDevice, s));
+    if ((rc = magnus(F, F->gDG, -1, F->gG, -1, mag_conv, &nc, s, &F->lmrec[k], &pidx, F->segE[k]))) return rc;
     launch_axpy(n, F->gG, 1.0, F->gDG, ds, F->gG, s);
-    if ((rc = bch(F, F->gHS, F->gG, -1, F->gH, 1, bch_conv, 40, &nt, nullptr, s))) return rc;
+    if ((rc = bch(F, F->gHS, F->gG, -1, F->gH, 1, bch_conv, 40, &nt, nullptr, s, &F->lbrec[k], &pidx))) return rc;
     if ((rc = white(F->X, F->gHS, F->gDG, s)) || (rc = mbpt2_dev(F->X, F->gHS, B->dval + 2, s))) return rc;
     if ((rc = read_scalar(B, B->dval + 2, &crit, s))) return rc;
     crit = std::fabs(crit);
     ++steps;
+  }
+  if (steps > 0) {
+    const int n0 = ((steps - 1) / ckpt) * ckpt;
+    HIPCHK(hipMemcpyAsync(F->segG[steps - n0], F->gG, nb, hipMemcpyDeviceToDevice, s));
+    F->last_n0 = n0;
   }
   double e0 = 0.0;
   if ((rc = read_scalar(B, F->gHS, &e0, s))) return rc;
@@ -1600,9 +1647,10 @@ int flow_forward(imsrg_flow_s* F, const double* T, double ds, double bch_conv, d
   return 0;
 }
 
-// Reverse sweep: each checkpoint segment is recomputed (Omega, H(s), eta)
-// and swept backwards through BCH_EXPAND, euler_step, MAGNUS_EXPAND and the
-// generator; Tb = dE0/dT.
+// Reverse sweep: each checkpoint segment (but the last one, which the
+// forward sweep kept) is recomputed from its Omega and H(s) checkpoints,
+// keeping every nested-commutator term, and swept backwards through
+// BCH_EXPAND, euler_step, MAGNUS_EXPAND and the generator; Tb = dE0/dT.
 int flow_backward(imsrg_flow_s* F, double* Tb, hipStream_t s) {
   imsrg_basis_s* B = F->B;
   const i64 n = B->b.oplen;
@@ -1614,30 +1662,30 @@ int flow_backward(imsrg_flow_s* F, double* Tb, hipStream_t s) {
   launch_fill(F->gHSb, 1, 1.0, s);
   launch_fill(F->gGb, n, 0.0, s);
   launch_fill(F->gHb, n, 0.0, s);
-  if ((rc = grow(F, F->segG, (size_t)ckpt)) || (rc = grow(F, F->segH, (size_t)ckpt)) ||
-      (rc = grow(F, F->segE, (size_t)ckpt)))
-    return rc;
   const int nck = (steps + ckpt - 1) / ckpt;
+  std::vector<TermRec> mrec, brec;
   for (int seg = nck - 1; seg >= 0; --seg) {
     const int n0 = seg * ckpt, n1 = std::min(steps, n0 + ckpt);
-    HIPCHK(hipMemcpyAsync(F->segG[0], F->ckpts[seg], n * sizeof(double), hipMemcpyDeviceToDevice, s));
-    if (n0 == 0) {
-      HIPCHK(hipMemcpyAsync(F->segH[0], F->gH, n * sizeof(double), hipMemcpyDeviceToDevice, s));
-    } else if ((rc = bch(F, F->segH[0], F->segG[0], -1, F->gH, 1, bch_conv, 40, &nt, nullptr, s))) {
-      return rc;
-    }
-    launch_fill(F->segE[0], n, 0.0, s);
-    if ((rc = white(F->X, F->segH[0], F->segE[0], s))) return rc;
-    std::vector<TermRec> mrec(n1 - n0), brec(n1 - n0);
-    size_t pidx = 0;
-    for (int m = n0; m < n1; ++m) {
-      const int k = m - n0;
-      if ((rc = magnus_record(F, F->segE[k], F->gDG, F->segG[k], mag_conv, mrec[k], pidx, &nc, s))) return rc;
-      launch_axpy(n, F->segG[k], 1.0, F->gDG, ds, F->segG[k + 1], s);
-      if ((rc = bch_record(F, F->segG[k + 1], F->gH, 1, bch_conv, 40, brec[k], pidx, F->segH[k + 1], &nt, s)))
-        return rc;
-      launch_fill(F->segE[k + 1], n, 0.0, s);
-      if ((rc = white(F->X, F->segH[k + 1], F->segE[k + 1], s))) return rc;
+    if (seg == nck - 1 && F->last_n0 == n0) {   // kept by the forward sweep
+      mrec.assign(F->lmrec.begin(), F->lmrec.begin() + (n1 - n0));
+      brec.assign(F->lbrec.begin(), F->lbrec.begin() + (n1 - n0));
+    } else {
+      mrec.assign(n1 - n0, TermRec());
+      brec.assign(n1 - n0, TermRec());
+      HIPCHK(hipMemcpyAsync(F->segG[0], F->ckpts[seg], n * sizeof(double), hipMemcpyDeviceToDevice, s));
+      HIPCHK(hipMemcpyAsync(F->segH[0], F->hckpts[seg], n * sizeof(double), hipMemcpyDeviceToDevice, s));
+      launch_fill(F->segE[0], n, 0.0, s);
+      if ((rc = white(F->X, F->segH[0], F->segE[0], s))) return rc;
+      size_t pidx = 0;
+      for (int m = n0; m < n1; ++m) {
+        const int k = m - n0;
+        if ((rc = magnus_record(F, F->segE[k], F->gDG, F->segG[k], mag_conv, mrec[k], pidx, &nc, s))) return rc;
+        launch_axpy(n, F->segG[k], 1.0, F->gDG, ds, F->segG[k + 1], s);
+        if ((rc = bch_record(F, F->segG[k + 1], F->gH, 1, bch_conv, 40, brec[k], pidx, F->segH[k + 1], &nt, s)))
+          return rc;
+        launch_fill(F->segE[k + 1], n, 0.0, s);
+        if ((rc = white(F->X, F->segH[k + 1], F->segE[k + 1], s))) return rc;
+      }
     }
     for (int m = n1 - 1; m >= n0; --m) {
       const int k = m - n0;
@@ -1653,6 +1701,7 @@ int flow_backward(imsrg_flow_s* F, double* Tb, hipStream_t s) {
       }
     }
   }
+  F->last_n0 = -1;   // the kept segment's buffers were reused by the recomputation
   if ((rc = normal_order_vjp(B, F->gHb, Tb, s))) return rc;
   HIPCHK(hipStreamSynchronize(s));
   return 0;
