@@ -152,3 +152,25 @@ def test_flow_energy_autograd_and_central_differences(gpu):
         fd = (ep - em) / (2 * h)
         an = float(g @ d.cpu().numpy())
         assert abs(fd - an) < 1e-6 * abs(fd) + 5e-8, (fd, an)
+
+
+def test_flow_grad_emax8_vs_cpu_reference_fixture(gpu):
+    """dE0/dT of the whole eMax 8 flow (Omega checkpointed every 4 steps)
+    against the oracle's reverse sweep run on the CPU
+    (tests/golden/grad_emax8.json, written by make_grad_emax8.py): the 200
+    largest and 2000 seeded further elements of the folded gradient, 1e-6
+    relative; E0 and the step count of the forward sweep exactly as there."""
+    import json
+    import os
+    from eom_imsrg import Flow, adjoint, synth
+    fx = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "grad_emax8.json")))
+    b = basis(8)
+    print("[flow_grad] emax=8 vs CPU fixture", flush=True)
+    T = synth.hamiltonian_vacuum(b, seed=0, hw=20.0)
+    e0, Tb, st = adjoint.flow_grad(Flow(b, graphs=True), _t(T), ckpt=4)
+    assert st == fx["steps"] and abs(e0 - fx["E0"]) < 1e-8, (st, e0, fx["E0"])
+    g = _sym(b, Tb.cpu().numpy(), 1)
+    idx = np.asarray(fx["index"], np.int64)
+    ref = np.asarray(fx["grad"])
+    assert_rel(g[idx], ref, rtol=1e-6, what="dE0/dT vs CPU reference gradient")
+    assert abs(np.linalg.norm(g) - fx["norm"]) <= 1e-6 * fx["norm"]
