@@ -1001,6 +1001,7 @@ struct imsrg_flow_s {
   double *int1 = nullptr, *Hbare = nullptr;
   double* coefs = nullptr;   // device coefficient table for graph terms
   bool graphs = false;
+  int generator = 0;   // IMSRG_GEN_WHITE / _ATAN / _W2 for magnus_decouple and magnus_step
   struct GraphKey {
     int kind, iw;
     const void *a, *b, *c;
@@ -1203,19 +1204,52 @@ int magnus(imsrg_flow_s* F, double* DG, int DGh, const double* G, int Gh, double
   return 0;
 }
 
-int white(const Exec& X, const double* H, double* ETA, hipStream_t s) {
-  launch_monopole(X.D, H, X.M, s);
-  launch_white_1b(X.D, H, ETA, s);
-  launch_white_2b(X.D, X.B->seg_full.dev, X.B->seg_full.n, H, X.M, ETA, s);
+// generator with the denominators of Hd and the numerators of Hn (mode 0
+// white, 1 atan / atanxx)
+int gen(const Exec& X, const double* Hd, const double* Hn, double* ETA, int mode, hipStream_t s) {
+  launch_monopole(X.D, Hd, X.M, s);
+  launch_gen_1b(X.D, Hd, Hn, ETA, mode, s);
+  launch_gen_2b(X.D, X.B->seg_full.dev, X.B->seg_full.n, Hd, X.M, Hn, ETA, mode, s);
   HIPCHK(hipGetLastError());
   return 0;
 }
+
+// build_gs_white, generators.f90:4-106
+int white(const Exec& X, const double* H, double* ETA, hipStream_t s) { return gen(X, H, H, ETA, 0, s); }
 
 int mbpt2_dev(const Exec& X, const double* H, double* dst, hipStream_t s) {
   launch_mbpt2_partials(X.D, X.B->seg_g3.dev, X.B->seg_g3.n, H, X.part, s);
   launch_sum_sqrt_b(X.D, X.part, X.B->seg_g3.n, dst, kDStride, 0, s);
   HIPCHK(hipGetLastError());
   return 0;
+}
+
+// build_gs_w2, generators.f90:109-160: eta1 = atan(H), H2 = [eta1, H]
+// (E0 included), eta = 1.5 eta1 + 0.5 atanxx(H, H2).  Uses the flow's
+// ping-pong operators and commutator workspace (free between expansions).
+int gen_w2(imsrg_flow_s* F, const Exec& X, const double* H, double* ETA, hipStream_t s) {
+  const i64 n = F->B->b.oplen;
+  double* eta1 = F->buf[0];
+  double* H2 = F->buf[1];
+  int rc;
+  if ((rc = gen(X, H, H, eta1, 1, s))) return rc;
+  launch_fill_e0(X.D, eta1, 0.0, s);
+  t_pandya(X, eta1, -1, F->GCC, s);
+  t_full(X, eta1, -1, H, 1, H2, 1, true, F->w1, F->w2, F->GCC, F->ADCC, F->WCC, s);
+  if ((rc = gen(X, H, H2, ETA, 1, s))) return rc;
+  launch_fill_e0(X.D, ETA, 0.0, s);
+  launch_axpy_b(n, eta1, 1.5, ETA, 0.5, ETA, X.D.ninst, X.D.act, s);   // add_sq_op(eta1, 1.5, eta2, 0.5, eta)
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// the flow's generator of H into ETA
+int generator(imsrg_flow_s* F, const Exec& X, const double* H, double* ETA, hipStream_t s) {
+  switch (F->generator) {
+    case IMSRG_GEN_ATAN: return gen(X, H, H, ETA, 1, s);
+    case IMSRG_GEN_W2: return gen_w2(F, X, H, ETA, s);
+    default: return white(X, H, ETA, s);
+  }
 }
 
 int magnus_step(imsrg_flow_s* F, double* HS, double* G, double* DG, const double* H, double ds, double bch_conv,
@@ -1225,7 +1259,7 @@ int magnus_step(imsrg_flow_s* F, double* HS, double* G, double* DG, const double
   if ((rc = magnus(F, DG, -1, G, -1, mag_conv, &nm, s))) return rc;
   launch_axpy(B->b.oplen, G, 1.0, DG, ds, G, s);   // euler_step
   if ((rc = bch(F, HS, G, -1, H, 1, bch_conv, 40, &nb, nullptr, s))) return rc;
-  if ((rc = white(F->X, HS, DG, s))) return rc;
+  if ((rc = generator(F, F->X, HS, DG, s))) return rc;
   if ((rc = mbpt2_dev(F->X, HS, B->dval + 2, s))) return rc;
   HIPCHK(hipMemcpyAsync(B->dval + 3, HS, sizeof(double), hipMemcpyDeviceToDevice, s));
   HIPCHK(hipMemcpyAsync(B->hval, B->dval + 2, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -1385,7 +1419,7 @@ int magnus_step_b(imsrg_flow_s* F, double* HS, double* G, double* DG, const doub
   if ((rc = magnus_b(F, DG, G, mag_conv, nm, s))) return rc;
   launch_axpy_b(B->b.oplen, G, 1.0, DG, ds, G, F->ninst, F->actf, s);   // euler_step
   if ((rc = bch_b(F, HS, G, H, bch_conv, 40, nb, s))) return rc;
-  if ((rc = white(F->X, HS, DG, s))) return rc;
+  if ((rc = generator(F, F->X, HS, DG, s))) return rc;
   if ((rc = mbpt2_dev(F->X, HS, F->X.dval + 2, s))) return rc;
   return read_inst(F, 3, HS, s);
 }
@@ -1588,6 +1622,7 @@ int flow_forward(imsrg_flow_s* F, const double* T, double ds, double bch_conv, d
   const size_t nb = (size_t)n * sizeof(double);
   int rc, nt = 0, nc = 0;
   if (ckpt < 1) return fail(IMSRG_EINVAL, "checkpoint interval must be >= 1");
+  if (F->generator != IMSRG_GEN_WHITE) return fail(IMSRG_EINVAL, "the flow gradient is built for the white generator");
   if ((rc = ensure_adj(B))) return rc;
   for (double** p : {&F->gH, &F->gHS, &F->gG, &F->gDG, &F->gHSb, &F->gGb, &F->gHb, &F->gdOb, &F->getab})
     if ((rc = fbuf(F, p, n))) return rc;
@@ -1951,6 +1986,18 @@ int imsrg_build_gs_white(imsrg_basis B, const double* H, double* ETA, imsrg_stre
   return white(basis_exec(B), H, ETA, s);
 }
 
+int imsrg_build_gs_atan(imsrg_basis B, const double* H, double* ETA, imsrg_stream stream) {
+  PREP();
+  if (!H || !ETA) return fail(IMSRG_EINVAL, "null argument");
+  return gen(basis_exec(B), H, H, ETA, 1, s);
+}
+
+int imsrg_build_gs_atanxx(imsrg_basis B, const double* Hold, const double* H, double* ETA, imsrg_stream stream) {
+  PREP();
+  if (!Hold || !H || !ETA) return fail(IMSRG_EINVAL, "null argument");
+  return gen(basis_exec(B), Hold, H, ETA, 1, s);
+}
+
 int imsrg_mbpt2(imsrg_basis B, const double* H, double* value, imsrg_stream stream) {
   PREP();
   int rc = mbpt2_dev(basis_exec(B), H, B->dval, s);
@@ -1995,6 +2042,20 @@ int imsrg_flow_set_graphs(imsrg_flow F, int on) {
   return 0;
 }
 
+int imsrg_flow_set_generator(imsrg_flow F, int generator) {
+  if (!F) return fail(IMSRG_EINVAL, "null flow");
+  if (generator != IMSRG_GEN_WHITE && generator != IMSRG_GEN_ATAN && generator != IMSRG_GEN_W2)
+    return fail(IMSRG_EINVAL, "unknown generator");
+  F->generator = generator;
+  return 0;
+}
+
+int imsrg_build_gs_w2(imsrg_flow F, const double* H, double* ETA, imsrg_stream stream) {
+  if (!F || !H || !ETA) return fail(IMSRG_EINVAL, "null argument");
+  if (F->batch) return fail(IMSRG_EINVAL, "batched flow: use the imsrg_batch_* calls");
+  return gen_w2(F, F->X, H, ETA, S(stream));
+}
+
 int imsrg_bch_expand(imsrg_flow F, double* HS, const double* G, int Gh, const double* H, int Hh, double conv,
                      int maxit, int* nterms, double* advals, imsrg_stream stream) {
   if (!F) return fail(IMSRG_EINVAL, "null flow");
@@ -2036,7 +2097,7 @@ int imsrg_magnus_decouple(imsrg_flow F, double* HS, double* G, double ds, double
   double* DG = F->decDG;
   launch_fill(DG, n, 0.0, s);
   HIPCHK(hipMemcpyAsync(F->Hbare, HS, n * sizeof(double), hipMemcpyDeviceToDevice, s));
-  if ((rc = white(F->X, HS, DG, s))) return rc;
+  if ((rc = generator(F, F->X, HS, DG, s))) return rc;
   if ((rc = mbpt2_dev(F->X, HS, B->dval + 2, s))) return rc;
   HIPCHK(hipMemcpyAsync(B->dval + 3, HS, sizeof(double), hipMemcpyDeviceToDevice, s));
   HIPCHK(hipMemcpyAsync(B->hval, B->dval + 2, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -2181,7 +2242,7 @@ int imsrg_batch_magnus_decouple(imsrg_flow F, double* HS, double* G, double ds, 
   double* DG = F->decDG;
   launch_fill_b(DG, n, 0.0, ni, F->actf, s);
   launch_copy_b(F->Hbare, HS, n, ni, F->actf, s);
-  if ((rc = white(F->X, HS, DG, s))) return rc;
+  if ((rc = generator(F, F->X, HS, DG, s))) return rc;
   if ((rc = mbpt2_dev(F->X, HS, F->X.dval + 2, s))) return rc;
   if ((rc = read_inst(F, 3, HS, s))) return rc;
   bool any = false;

@@ -944,11 +944,23 @@ void launch_monopole(const DevBasis& B, const double* H, double* M, hipStream_t 
   hipLaunchKernelGGL(k_monopole, dim3((B.N * B.N + 255) / 256, B.ninst), dim3(256), 0, s, B, H, M);
 }
 
-// build_gs_white one-body part, generators.f90:19-55 (also zeroes fhh, fpp)
-__global__ __launch_bounds__(256) void k_white_1b(DevBasis B, const double* H, double* ETA) {
+// Ground-state generators: the energy denominators of build_gs_white
+// (generators.f90:4-106) from Hd, the numerators from Hn:
+//   mode 0 (white, generators.f90:54,100):  eta = Hn / Eden
+//   mode 1 (atan / atanxx, generators.f90:213,263 and 318,366):
+//                                            eta = 0.5 atan(2 Hn / Eden)
+// (build_gs_atan is mode 1 with Hd = Hn; build_gs_atanxx passes Hold as Hd)
+__device__ __forceinline__ double gen_value(double x, double Eden, int mode) {
+  return mode == 0 ? x / Eden : atan(2 * x / Eden) * 0.5;
+}
+
+// one-body part (also zeroes fhh, fpp)
+__global__ __launch_bounds__(256) void k_gen_1b(DevBasis B, const double* Hd, const double* Hn, double* ETA,
+                                                int mode) {
   int inst;
   if (!batch_inst(B, inst)) return;
-  H += (i64)inst * B.oplen;
+  Hd += (i64)inst * B.oplen;
+  Hn += (i64)inst * B.oplen;
   ETA += (i64)inst * B.oplen;
   const i64 idx = (i64)blockIdx.x * blockDim.x + threadIdx.x;
   const i64 nA = (i64)B.A * B.A, nPA = (i64)B.P * B.A, nP = (i64)B.P * B.P;
@@ -961,22 +973,27 @@ __global__ __launch_bounds__(256) void k_white_1b(DevBasis B, const double* H, d
   if (B.jj[ik] != B.jj[ak] || B.ll[ik] != B.ll[ak] || B.tz[ik] != B.tz[ak]) return;
   const int ji = B.jj[ik];
   double Eden = 0.0;
-  for (int JT = 0; JT <= 2 * ji; JT += 2) Eden = Eden - (JT + 1) * v_elem(B, H, 1, ak, ik, ak, ik, JT);
+  for (int JT = 0; JT <= 2 * ji; JT += 2) Eden = Eden - (JT + 1) * v_elem(B, Hd, 1, ak, ik, ak, ik, JT);
   Eden = Eden / (ji + 1.0) / (ji + 1.0);
-  Eden = Eden + H[B.off_fpp + a + (i64)a * B.P] - H[B.off_fhh + i + (i64)i * B.A];
-  ETA[B.off_fph + x] = H[B.off_fph + x] / Eden;
+  Eden = Eden + Hd[B.off_fpp + a + (i64)a * B.P] - Hd[B.off_fhh + i + (i64)i * B.A];
+  ETA[B.off_fph + x] = gen_value(Hn[B.off_fph + x], Eden, mode);
+}
+void launch_gen_1b(const DevBasis& B, const double* Hd, const double* Hn, double* ETA, int mode, hipStream_t s) {
+  const i64 n = (i64)B.A * B.A + (i64)B.P * B.P + (i64)B.P * B.A;
+  hipLaunchKernelGGL(k_gen_1b, dim3((unsigned)((n + 255) / 256), B.ninst), dim3(256), 0, s, B, Hd, Hn, ETA, mode);
 }
 void launch_white_1b(const DevBasis& B, const double* H, double* ETA, hipStream_t s) {
-  const i64 n = (i64)B.A * B.A + (i64)B.P * B.P + (i64)B.P * B.A;
-  hipLaunchKernelGGL(k_white_1b, dim3((unsigned)((n + 255) / 256), B.ninst), dim3(256), 0, s, B, H, ETA);
+  launch_gen_1b(B, H, H, ETA, 0, s);
 }
 
-// build_gs_white two-body part, generators.f90:57-104: full-segment tasks
-__global__ __launch_bounds__(256) void k_white_2b(DevBasis B, const SegTask* tasks, const double* H,
-                                                  const double* M, double* ETA) {
+// two-body part, full-segment tasks (gam3 from the formula, the rest zero);
+// M = the monopoles of Hd
+__global__ __launch_bounds__(256) void k_gen_2b(DevBasis B, const SegTask* tasks, const double* Hd, const double* M,
+                                                const double* Hn, double* ETA, int mode) {
   int inst;
   if (!batch_inst(B, inst)) return;
-  H += (i64)inst * B.oplen;
+  Hd += (i64)inst * B.oplen;
+  Hn += (i64)inst * B.oplen;
   ETA += (i64)inst * B.oplen;
   M += (i64)inst * B.N * B.N;
   const SegTask t = tasks[blockIdx.x];
@@ -998,13 +1015,17 @@ __global__ __launch_bounds__(256) void k_white_2b(DevBasis B, const SegTask* tas
   Eden = Eden - M[a + j * N];
   Eden = Eden - M[i + b * N];
   Eden = Eden - M[j + b * N];
-  Eden = Eden + f_elem(B, H, 1, a, a) + f_elem(B, H, 1, b, b) - f_elem(B, H, 1, i, i) - f_elem(B, H, 1, j, j);
-  ETA[o] = H[o] / Eden;
+  Eden = Eden + f_elem(B, Hd, 1, a, a) + f_elem(B, Hd, 1, b, b) - f_elem(B, Hd, 1, i, i) - f_elem(B, Hd, 1, j, j);
+  ETA[o] = gen_value(Hn[o], Eden, mode);
+}
+void launch_gen_2b(const DevBasis& B, const SegTask* t, int nt, const double* Hd, const double* M, const double* Hn,
+                   double* ETA, int mode, hipStream_t s) {
+  if (nt <= 0) return;
+  hipLaunchKernelGGL(k_gen_2b, dim3(nt, B.ninst), dim3(256), 0, s, B, t, Hd, M, Hn, ETA, mode);
 }
 void launch_white_2b(const DevBasis& B, const SegTask* t, int nt, const double* H, const double* M, double* ETA,
                      hipStream_t s) {
-  if (nt <= 0) return;
-  hipLaunchKernelGGL(k_white_2b, dim3(nt, B.ninst), dim3(256), 0, s, B, t, H, M, ETA);
+  launch_gen_2b(B, t, nt, H, M, H, ETA, 0, s);
 }
 
 // mbpt2, basic_IMSRG.f90:4165-4199: one partial per gam3 tile
@@ -1092,6 +1113,16 @@ __global__ __launch_bounds__(256) void k_no_1b(DevBasis B, double* H) {
 }
 
 __global__ void k_set_e0(double* H, const double* E0) { H[0] = E0[0]; }
+
+// E0 = v for every instance of an operator array
+__global__ void k_fill_e0(DevBasis B, double* op, double v) {
+  int inst;
+  if (!batch_inst(B, inst)) return;
+  if (threadIdx.x == 0) op[(i64)inst * B.oplen] = v;
+}
+void launch_fill_e0(const DevBasis& B, double* op, double v, hipStream_t s) {
+  hipLaunchKernelGGL(k_fill_e0, dim3(1, B.ninst), dim3(64), 0, s, B, op, v);
+}
 
 void launch_normal_order(const DevBasis& B, double* H, double* scratch, hipStream_t s) {
   hipLaunchKernelGGL(k_no_e0, dim3(1), dim3(256), 0, s, B, H, scratch);

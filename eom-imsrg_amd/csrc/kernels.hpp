@@ -144,6 +144,12 @@ void launch_sum_sqrt_b(const DevBasis& B, const double* part, int n, double* dst
 void launch_e0_final_b(const DevBasis& B, const double* part, int use220, int use110, double* dst, int accumulate,
                        hipStream_t s);
 void launch_monopole(const DevBasis& B, const double* H, double* M, hipStream_t s);
+// generators: denominators of build_gs_white from Hd, numerators from Hn;
+// mode 0 Hn / Eden (white), 1 0.5 atan(2 Hn / Eden) (atan, atanxx)
+void launch_gen_1b(const DevBasis& B, const double* Hd, const double* Hn, double* ETA, int mode, hipStream_t s);
+void launch_gen_2b(const DevBasis& B, const SegTask* t, int nt, const double* Hd, const double* M, const double* Hn,
+                   double* ETA, int mode, hipStream_t s);
+void launch_fill_e0(const DevBasis& B, double* op, double v, hipStream_t s);
 void launch_white_1b(const DevBasis& B, const double* H, double* ETA, hipStream_t s);
 void launch_white_2b(const DevBasis& B, const SegTask* t, int nt, const double* H, const double* M, double* ETA,
                      hipStream_t s);

@@ -88,6 +88,10 @@ SIGNATURES = {
     "imsrg_batch_magnus_expand": (_i, [_p, _dp, _dp, C.c_double, _ip, _p]),
     "imsrg_batch_magnus_decouple": (_i, [_p, _dp, _dp, C.c_double, C.c_double, C.c_double, C.c_double, _i, _d, _ip,
                                          _ip, _p]),
+    "imsrg_build_gs_atan": (_i, [_p, _dp, _dp, _p]),
+    "imsrg_build_gs_atanxx": (_i, [_p, _dp, _dp, _dp, _p]),
+    "imsrg_build_gs_w2": (_i, [_p, _dp, _dp, _p]),
+    "imsrg_flow_set_generator": (_i, [_p, _i]),
     "imsrg_commutator_vjp": (_i, [_p, _dp, _i, _dp, _i, _dp, _i, _dp, _dp, _p]),
     "imsrg_bch_vjp": (_i, [_p, _dp, _dp, _i, C.c_double, _i, _dp, _dp, _dp, _ip, _p]),
     "imsrg_magnus_vjp": (_i, [_p, _dp, _dp, C.c_double, _dp, _dp, _dp, _ip, _p]),
@@ -102,6 +106,7 @@ SIGNATURES = {
 }
 
 GEMM = dict(PPPP=0, PPX=1, XX=2, PH=3)
+GENERATOR = dict(white=0, atan=1, w2=2)
 KERNEL = dict(PPPP=0, PPHH_SMALL=1, PH_GEMM=2, PH_SCATTER=3, K122=4, PANDYA=5, ONEBODY=6, COMMUTATOR=7,
               PPPP_GEMM=8, COPY=9)
 

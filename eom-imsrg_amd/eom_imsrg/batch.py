@@ -21,7 +21,7 @@ from .operators import _stream
 
 
 class Batch:
-    def __init__(self, basis: Basis, ninst: int, graphs: bool = True):
+    def __init__(self, basis: Basis, ninst: int, graphs: bool = True, generator: str = "white"):
         self.basis = basis
         self.ninst = int(ninst)
         h = C.c_void_p()
@@ -29,6 +29,8 @@ class Batch:
         self.handle = h
         _lib.check(_lib.lib.imsrg_flow_set_graphs(h, 1 if graphs else 0))
         self.graphs = bool(graphs)
+        _lib.check(_lib.lib.imsrg_flow_set_generator(h, _lib.GENERATOR[generator]))
+        self.generator = generator
 
     def _check(self, *ts: torch.Tensor) -> None:
         for t in ts:

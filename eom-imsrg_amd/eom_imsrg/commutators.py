@@ -131,6 +131,18 @@ def build_gs_white(H: SqOp, ETA: SqOp) -> None:
     _lib.check(L_.imsrg_build_gs_white(_b(H).handle, H.ptr, ETA.ptr, _stream()))
 
 
+def build_gs_atan(H: SqOp, ETA: SqOp) -> None:
+    """build_gs_atan, generators.f90:163-269"""
+    ETA.herm = -1
+    _lib.check(L_.imsrg_build_gs_atan(_b(H).handle, H.ptr, ETA.ptr, _stream()))
+
+
+def build_gs_atanxx(Hold: SqOp, H: SqOp, ETA: SqOp) -> None:
+    """build_gs_atanxx, generators.f90:271-374 (denominators of Hold)"""
+    ETA.herm = -1
+    _lib.check(L_.imsrg_build_gs_atanxx(_b(H).handle, Hold.ptr, H.ptr, ETA.ptr, _stream()))
+
+
 def mbpt2(H: SqOp) -> float:
     """mbpt2, basic_IMSRG.f90:4165-4199"""
     v = C.c_double()

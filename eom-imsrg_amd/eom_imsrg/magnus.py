@@ -26,12 +26,24 @@ REF_TOL = float(np.float32(1e-6))   # the reference's 1e-6 default-real literal
 
 
 class Flow:
-    def __init__(self, basis: Basis, graphs: bool = False):
+    def __init__(self, basis: Basis, graphs: bool = False, generator: str = "white"):
         self.basis = basis
         h = C.c_void_p()
         _lib.check(_lib.lib.imsrg_flow_create(basis.handle, C.byref(h)))
         self.handle = h
         self.set_graphs(graphs)
+        self.set_generator(generator)
+
+    def set_generator(self, name: str) -> None:
+        """the generator magnus_decouple builds: "white" (build_gs_white),
+        "atan" (build_gs_atan) or "w2" (build_gs_w2, the reference main's)"""
+        _lib.check(_lib.lib.imsrg_flow_set_generator(self.handle, _lib.GENERATOR[name]))
+        self.generator = name
+
+    def build_gs_w2(self, H: SqOp, ETA: SqOp) -> None:
+        """build_gs_w2, generators.f90:109-160 (uses this flow's workspace)"""
+        ETA.herm = -1
+        _lib.check(_lib.lib.imsrg_build_gs_w2(self.handle, H.ptr, ETA.ptr, _stream()))
 
     def set_graphs(self, on: bool) -> None:
         _lib.check(_lib.lib.imsrg_flow_set_graphs(self.handle, 1 if on else 0))

@@ -137,6 +137,12 @@ int imsrg_add_sq_op(imsrg_basis b, const double* A, double ax, const double* B, 
 int imsrg_mat_frob_norm(imsrg_basis b, const double* op, double* value, imsrg_stream stream);
 /* build_gs_white, generators.f90:4-106 (ETA herm = -1) */
 int imsrg_build_gs_white(imsrg_basis b, const double* H, double* ETA, imsrg_stream stream);
+/* build_gs_atan, generators.f90:163-269: eta = 0.5 atan(2 H / Eden) on fph
+ * and the pphh blocks, Eden as in build_gs_white (ETA herm = -1; E0 kept) */
+int imsrg_build_gs_atan(imsrg_basis b, const double* H, double* ETA, imsrg_stream stream);
+/* build_gs_atanxx, generators.f90:271-374: the same with the denominators
+ * of Hold and the numerators of H */
+int imsrg_build_gs_atanxx(imsrg_basis b, const double* Hold, const double* H, double* ETA, imsrg_stream stream);
 /* mbpt2, basic_IMSRG.f90:4165-4199 (returns the value; synchronises) */
 int imsrg_mbpt2(imsrg_basis b, const double* H, double* value, imsrg_stream stream);
 /* normal_order, basic_IMSRG.f90:2419-2550 (in place) */
@@ -149,6 +155,18 @@ int imsrg_flow_create(imsrg_basis b, imsrg_flow* out);
 int imsrg_flow_destroy(imsrg_flow f);
 /* capture each nested commutator term as a hipGraph (grouped path only) */
 int imsrg_flow_set_graphs(imsrg_flow f, int on);
+/* the generator magnus_decouple / magnus_step build after each step (the
+ * reference passes it as build_generator, IMSRG_MAGNUS.f90:15;
+ * main_IMSRG.f90:230 passes build_gs_w2).  Default white.  The flow
+ * gradient (imsrg_flow_forward / _backward / _grad) requires white. */
+#define IMSRG_GEN_WHITE 0
+#define IMSRG_GEN_ATAN 1
+#define IMSRG_GEN_W2 2
+int imsrg_flow_set_generator(imsrg_flow f, int generator);
+/* build_gs_w2, generators.f90:109-160: eta1 = build_gs_atan(H), H2 =
+ * [eta1, H] (E0 = 110 + 220 included), eta = 1.5 eta1 + 0.5
+ * build_gs_atanxx(H, H2); uses the flow's workspace (ETA E0 = 0) */
+int imsrg_build_gs_w2(imsrg_flow f, const double* H, double* ETA, imsrg_stream stream);
 /* BCH_EXPAND, IMSRG_MAGNUS.f90:149-234: HS = exp(G) H exp(-G), terms until
  * ||term|| < conv (reference: 1e-6 as a single-precision literal), at most
  * maxit-1 nested commutators.  nterms = the loop index iw of the exit.

@@ -803,9 +803,13 @@ double monopole(int a, int bb, const Op& H) {
   return sm / (ja + 1.0) / (jb + 1.0);
 }
 
-// build_gs_white, generators.f90:4-106 (ETA herm = -1)
-void white(const Op& H, Op& ETA) {
-  const Basis& b = *H.b;
+// Ground-state generators with the denominators of Hd and the numerators
+// of Hn: build_gs_white (generators.f90:4-106; mode 0, eta = Hn / Eden),
+// build_gs_atan (163-269; mode 1, eta = 0.5 atan(2 Hn / Eden), Hd = Hn),
+// build_gs_atanxx (271-374; mode 1, Hd = Hold).  ETA herm = -1; E0 kept.
+void gen_core(const Op& Hd, const Op& Hn, Op& ETA, int mode) {
+  const Basis& b = *Hd.b;
+  auto val = [mode](double x, double Eden) { return mode == 0 ? x / Eden : std::atan(2 * x / Eden) * 0.5; };
   std::fill(ETA.fpp(), ETA.fpp() + (i64)b.P * b.P, 0.0);
   std::fill(ETA.fhh(), ETA.fhh() + (i64)b.A * b.A, 0.0);
   for (int a = 0; a < b.P; ++a) {
@@ -815,10 +819,10 @@ void white(const Op& H, Op& ETA) {
       if (b.jj[ik] != b.jj[ak] || b.ll[ik] != b.ll[ak] || b.tz[ik] != b.tz[ak]) continue;
       int ji = b.jj[ik];
       double Eden = 0.0;
-      for (int JT = 0; JT <= 2 * ji; JT += 2) Eden = Eden - (JT + 1) * v_elem(ak, ik, ak, ik, JT, H);
+      for (int JT = 0; JT <= 2 * ji; JT += 2) Eden = Eden - (JT + 1) * v_elem(ak, ik, ak, ik, JT, Hd);
       Eden = Eden / (ji + 1.0) / (ji + 1.0);
-      Eden = Eden + H.fpp()[a + (i64)a * b.P] - H.fhh()[i + (i64)i * b.A];
-      ETA.fph()[a + (i64)i * b.P] = H.fph()[a + (i64)i * b.P] / Eden;
+      Eden = Eden + Hd.fpp()[a + (i64)a * b.P] - Hd.fhh()[i + (i64)i * b.A];
+      ETA.fph()[a + (i64)i * b.P] = val(Hn.fph()[a + (i64)i * b.P], Eden);
     }
   }
   for (int q = 0; q < b.nblocks; ++q) {
@@ -832,18 +836,20 @@ void white(const Op& H, Op& ETA) {
       for (int JX = 0; JX < nh; ++JX) {
         int i = Ph[2 * JX], j = Ph[2 * JX + 1];
         double Eden = 0.0;
-        Eden = Eden + monopole(a, bb, H);
-        Eden = Eden + monopole(i, j, H);
-        Eden = Eden - monopole(a, i, H);
-        Eden = Eden - monopole(a, j, H);
-        Eden = Eden - monopole(i, bb, H);
-        Eden = Eden - monopole(j, bb, H);
-        Eden = Eden + f_elem(a, a, H) + f_elem(bb, bb, H) - f_elem(i, i, H) - f_elem(j, j, H);
-        ETA.gam(q, 2)[IX + (i64)JX * np] = H.gam(q, 2)[IX + (i64)JX * np] / Eden;
+        Eden = Eden + monopole(a, bb, Hd);
+        Eden = Eden + monopole(i, j, Hd);
+        Eden = Eden - monopole(a, i, Hd);
+        Eden = Eden - monopole(a, j, Hd);
+        Eden = Eden - monopole(i, bb, Hd);
+        Eden = Eden - monopole(j, bb, Hd);
+        Eden = Eden + f_elem(a, a, Hd) + f_elem(bb, bb, Hd) - f_elem(i, i, Hd) - f_elem(j, j, Hd);
+        ETA.gam(q, 2)[IX + (i64)JX * np] = val(Hn.gam(q, 2)[IX + (i64)JX * np], Eden);
       }
     }
   }
 }
+
+void white(const Op& H, Op& ETA) { gen_core(H, H, ETA, 0); }
 
 // mbpt2, basic_IMSRG.f90:4165-4199
 double mbpt2(const Op& H) {
@@ -943,6 +949,27 @@ void full_commutator(const Op& G, const Op& AD, Op& INT2, Work& w) {
   comm222pphh(G, AD, INT2, w1, w2);
   comm221(G, AD, INT2, w1, w2);
   comm222ph(w.LCC.data(), G.herm, w.RCC.data(), AD.herm, INT2, w.WCC.data());
+}
+
+// build_gs_w2, generators.f90:109-160: eta1 = build_gs_atan(H); H2 =
+// [eta1, H] with E0 = 110 + 220; eta = 1.5 eta1 + 0.5 build_gs_atanxx(H, H2)
+// (add_sq_op, basic_IMSRG.f90:3551-3579; eta1 and eta2 start from zero)
+void gs_w2(const Op& H, Op& ETA, Work& w) {
+  const Basis& b = *H.b;
+  std::vector<double> e1(b.oplen, 0.0), h2(b.oplen, 0.0), e2(b.oplen, 0.0);
+  Op eta1{&b, e1.data(), -1}, H2{&b, h2.data(), 1}, eta2{&b, e2.data(), -1};
+  gen_core(H, H, eta1, 1);
+  h2[0] = comm110(eta1, H) + comm220(eta1, H);
+  full_commutator(eta1, H, H2, w);
+  gen_core(H, H2, eta2, 1);
+  for (i64 x = 0; x < b.oplen; ++x) ETA.d[x] = e1[x] * 1.5 + e2[x] * 0.5;
+}
+
+// the generator of a flow: 0 white, 1 atan, 2 w2
+void build_generator(int gen, const Op& H, Op& ETA, Work& w) {
+  if (gen == 1) gen_core(H, H, ETA, 1);
+  else if (gen == 2) gs_w2(H, ETA, w);
+  else white(H, ETA);
 }
 
 // BCH_EXPAND, IMSRG_MAGNUS.f90:149-234.  HS = exp(G) H exp(-G); returns
@@ -1177,13 +1204,13 @@ int orc_magnus(void* h, double* DG, const double* G, double conv) {
 // crit_mode 0: loop while |mbpt2| > crit_tol (reference);
 //           1: loop until |E0(s) - E0(s-ds)| < crit_tol.
 // log[step*4 + {s, E0, E0+mbpt2, n_comm}] for up to maxsteps steps.
-int orc_magnus_decouple(void* h, double* HS, double* G, double ds, double bch_conv, double mag_conv,
-                        int crit_mode, double crit_tol, int maxsteps, double* log) {
+int orc_magnus_decouple_gen(void* h, double* HS, double* G, double ds, double bch_conv, double mag_conv,
+                            int crit_mode, double crit_tol, int maxsteps, double* log, int gen) {
   OPS(h)
   Work w;
   std::vector<double> Hb(HS, HS + b.oplen), dg(b.oplen, 0.0);
   Op hs{&b, HS, 1}, g{&b, G, -1}, DG{&b, dg.data(), -1}, H{&b, Hb.data(), 1};
-  white(hs, DG);
+  build_generator(gen, hs, DG, w);
   double s = 0.0;
   double E_mbpt2 = mbpt2(hs);
   double crit = std::fabs(E_mbpt2);
@@ -1198,7 +1225,7 @@ int orc_magnus_decouple(void* h, double* HS, double* G, double ds, double bch_co
     s += ds;
     int nb = bch(hs, g, H, bch_conv, 40, w, nullptr);
     if (nb < 0) return -100000;
-    white(hs, DG);
+    build_generator(gen, hs, DG, w);
     E_mbpt2 = mbpt2(hs);
     crit = std::fabs(E_mbpt2);
     if (log) {
@@ -1213,6 +1240,20 @@ int orc_magnus_decouple(void* h, double* HS, double* G, double ds, double bch_co
     }
   }
   return steps;
+}
+
+int orc_magnus_decouple(void* h, double* HS, double* G, double ds, double bch_conv, double mag_conv,
+                        int crit_mode, double crit_tol, int maxsteps, double* log) {
+  return orc_magnus_decouple_gen(h, HS, G, ds, bch_conv, mag_conv, crit_mode, crit_tol, maxsteps, log, 0);
+}
+void orc_gs_atan(void* h, const double* H, double* ETA) {
+  OPS(h) Op eta{&b, ETA, -1}; gen_core(Op{&b, (double*)H, 1}, Op{&b, (double*)H, 1}, eta, 1);
+}
+void orc_gs_atanxx(void* h, const double* Hold, const double* H, double* ETA) {
+  OPS(h) Op eta{&b, ETA, -1}; gen_core(Op{&b, (double*)Hold, 1}, Op{&b, (double*)H, 1}, eta, 1);
+}
+void orc_gs_w2(void* h, const double* H, double* ETA) {
+  OPS(h) Op eta{&b, ETA, -1}; Work w; gs_w2(Op{&b, (double*)H, 1}, eta, w);
 }
 
 // --------------------------------------------- brute-force formulas (pins)

@@ -47,6 +47,9 @@ def lib():
             "orc_commutator": (None, [vp, dp, i, dp, i, dp, i]),
             "orc_bch": (i, [vp, dp, dp, dp, d, i, dp]), "orc_magnus": (i, [vp, dp, dp, d]),
             "orc_magnus_decouple": (i, [vp, dp, dp, d, d, d, i, d, i, dp]),
+            "orc_magnus_decouple_gen": (i, [vp, dp, dp, d, d, d, i, d, i, dp, i]),
+            "orc_gs_atan": (None, [vp, dp, dp]), "orc_gs_atanxx": (None, [vp, dp, dp, dp]),
+            "orc_gs_w2": (None, [vp, dp, dp]),
             "orc_commutator_vjp": (None, [vp, dp, i, dp, i, dp, i, dp, dp]),
             "orc_bch_vjp": (i, [vp, dp, dp, d, i, dp, dp, dp]),
             "orc_magnus_vjp": (i, [vp, dp, dp, d, dp, dp, dp]),
@@ -165,6 +168,21 @@ class OracleBasis:
         lib().orc_white(self.h, _dp(H), _dp(eta))
         return eta
 
+    def gs_atan(self, H):
+        eta = np.zeros(self.oplen)
+        lib().orc_gs_atan(self.h, _dp(H), _dp(eta))
+        return eta
+
+    def gs_atanxx(self, Hold, H):
+        eta = np.zeros(self.oplen)
+        lib().orc_gs_atanxx(self.h, _dp(Hold), _dp(H), _dp(eta))
+        return eta
+
+    def gs_w2(self, H):
+        eta = np.zeros(self.oplen)
+        lib().orc_gs_w2(self.h, _dp(H), _dp(eta))
+        return eta
+
     def mbpt2(self, H):
         return lib().orc_mbpt2(self.h, _dp(H))
 
@@ -192,12 +210,13 @@ class OracleBasis:
         n = lib().orc_magnus(self.h, _dp(out), _dp(G), conv)
         return out, n
 
-    def magnus_decouple(self, H, ds, bch_conv, mag_conv, crit_mode, crit_tol, maxsteps=200):
+    def magnus_decouple(self, H, ds, bch_conv, mag_conv, crit_mode, crit_tol, maxsteps=200, generator=0):
+        """generator 0 white, 1 atan, 2 w2"""
         HS = H.copy()
         G = np.zeros(self.oplen)
         log = np.zeros(4 * maxsteps)
-        st = lib().orc_magnus_decouple(self.h, _dp(HS), _dp(G), ds, bch_conv, mag_conv, crit_mode, crit_tol,
-                                       maxsteps, _dp(log))
+        st = lib().orc_magnus_decouple_gen(self.h, _dp(HS), _dp(G), ds, bch_conv, mag_conv, crit_mode, crit_tol,
+                                           maxsteps, _dp(log), int(generator))
         return HS, G, log[: 4 * max(st, 0)].reshape(-1, 4), st
 
     # ---- reverse-mode derivatives (flat-array gradients)

@@ -395,6 +395,106 @@ def test_white_generator_closed_form():
     assert n > 100
 
 
+def _closed_form_hamiltonian(b, seed):
+    """H of the closed-form white test: returns (H, F, denominator(q, g, ix, jx) list)"""
+    rng = np.random.default_rng(seed)
+    H = np.zeros(b.oplen)
+    N = b.norb
+    F = np.diag([(-30.0 if b.hole[a] else 20.0) + rng.normal() for a in range(N)])
+    for blk in ljt_blocks(b):
+        for a in blk:
+            for i in blk:
+                if not b.hole[a] and b.hole[i]:
+                    F[a, i] = F[i, a] = 3.0 * rng.normal()
+    set_one_body(b, H, F)
+    _identity_two_body(b, H)
+    for q in range(b.nblocks):
+        n1, n2 = b.block_shape(q, 2)
+        o2 = int(b.goff[q, 2])
+        H[o2: o2 + n1 * n2] = 5.0 * rng.normal(size=n1 * n2)
+    return H, F
+
+
+def _check_atan_form(b, o, eta, Hn, F, what):
+    """eta = 0.5 atan(2 Hn / Eden) with the closed-form denominators of the
+    V = 1 + X Hamiltonian (F its one-body matrix), zero outside fph / gam3"""
+    m = lambda x, y: 1.0 if x != y else b.jj[x] / (b.jj[x] + 1.0)
+    E = one_body(o, b, eta, -1)
+    Fn = one_body(o, b, Hn, 1)
+    for a in b.parts:
+        for i in b.holes:
+            ref = 0.5 * np.arctan(2 * Fn[a, i] / (F[a, a] - F[i, i] - 1.0)) if Fn[a, i] != 0 else 0.0
+            assert abs(E[a, i] - ref) < 1e-13 * (1 + abs(ref)), what
+    assert np.abs(E[np.ix_(b.parts, b.parts)]).max() == 0 and np.abs(E[np.ix_(b.holes, b.holes)]).max() == 0
+    n = 0
+    for q, g, ix, jx, (a, bb), (i, j), J in each_element(b):
+        n1, _ = b.block_shape(q, g)
+        k = int(b.goff[q, g]) + ix + jx * n1
+        if g != 2:
+            assert eta[k] == 0.0, what
+            continue
+        den = F[a, a] + F[bb, bb] - F[i, i] - F[j, j] + m(a, bb) + m(i, j) - 4.0
+        ref = 0.5 * np.arctan(2 * Hn[k] / den)
+        assert abs(eta[k] - ref) < 1e-13 * (1 + abs(ref)), what
+        n += 1
+    assert n > 100
+    return n
+
+
+def test_atan_generator_closed_form():
+    """build_gs_atan: 0.5 atan(2 H / Eden) on fph and the pphh blocks with
+    the closed-form Epstein-Nesbet denominators; couplings up to several
+    times the gap so atan is far from linear"""
+    b, o = basis(4), oracle_basis(4)
+    H, F = _closed_form_hamiltonian(b, 31)
+    eta = o.gs_atan(H)
+    _check_atan_form(b, o, eta, H, F, "atan")
+    big = np.abs(2 * H[1:] / 10.0).max()
+    assert big > 1.0
+
+
+def test_atanxx_takes_denominators_from_hold():
+    """build_gs_atanxx(Hold, H): denominators of Hold (closed form), numerators
+    of an unrelated H; atanxx(H, H) = atan(H) to the bit"""
+    b, o = basis(4), oracle_basis(4)
+    Hold, F = _closed_form_hamiltonian(b, 32)
+    from eom_imsrg import synth
+    Hn = o.normal_order(synth.hamiltonian_vacuum(b, 3))
+    eta = o.gs_atanxx(Hold, Hn)
+    _check_atan_form(b, o, eta, Hn, F, "atanxx")
+    assert np.array_equal(o.gs_atanxx(Hn, Hn), o.gs_atan(Hn))
+
+
+def test_w2_generator_composition_and_small_coupling():
+    """build_gs_w2 = 1.5 atan(H) + 0.5 atanxx(H, [atan(H), H]) from the
+    separately pinned generators and commutator, E0 = 0.  With Moller-Plesset
+    denominators (diagonal f, V only in pphh) and V scaled by t, [eta, f]
+    = -V exactly and [eta, V] has no pphh part, so w2 = 1.5 white - 0.5
+    white + O(t^3) (atan's cubic term): the relative deviation falls as t^2"""
+    from eom_imsrg import synth
+    b, o = basis(2), oracle_basis(2)
+    H = o.normal_order(synth.hamiltonian_vacuum(b, 5))
+    e1 = o.gs_atan(H)
+    H2 = o.commutator(e1, -1, H, 1, 1)
+    ref = 1.5 * e1 + 0.5 * o.gs_atanxx(H, H2)
+    ref[0] = 0.0
+    w = o.gs_w2(H)
+    assert w[0] == 0.0
+    assert_rel(w, ref, rtol=1e-13, what="w2 composition")
+    Hmp = _mp_hamiltonian(b, 14)
+    r = []
+    for t in (2e-3, 1e-3):
+        Ht = Hmp.copy()
+        for q in range(b.nblocks):
+            n1, n2 = b.block_shape(q, 2)
+            o2 = int(b.goff[q, 2])
+            Ht[o2: o2 + n1 * n2] *= t
+        wt = o.white(Ht)
+        d = o.gs_w2(Ht)[1:] - wt[1:]
+        r.append(np.abs(d).max() / np.abs(wt[1:]).max())
+    assert r[0] < 1e-7 and 0.2 < r[1] / r[0] < 0.3, r
+
+
 def _mp_hamiltonian(b, seed):
     """diagonal f, zero ph one-body, V only in the pphh blocks: Epstein-Nesbet
     = Moller-Plesset denominators, no 110 term"""
@@ -460,3 +560,20 @@ def test_oracle_flow_matches_fixture():
     assert [int(x) for x in log[:, 3]] == g["flow"]["commutators"]
     assert np.abs(log[:, 1] - np.array(g["flow"]["E0"])).max() < 1e-8
     assert abs(HS[0] - g["flow"]["final_E0"]) < 1e-8
+
+
+def test_generators_flow_to_the_same_ground_state():
+    """white, atan and w2 flows of one Hamiltonian reach |mbpt2| < 1e-6 and
+    agree on E0 to the size the IMSRG(2) truncation and that tolerance leave
+    (1e-4 MeV here), while a different Hamiltonian differs by MeV"""
+    from eom_imsrg import synth
+    b, o = basis(2), oracle_basis(2)
+    tol = float(np.float32(1e-6))
+    H = o.normal_order(synth.hamiltonian_vacuum(b, seed=0, hw=20.0))
+    E = []
+    for gen in (0, 1, 2):
+        HS, _, log, st = o.magnus_decouple(H, 0.5, tol, tol, 0, tol, generator=gen)
+        assert 0 < st < 200 and abs(log[-1, 2] - log[-1, 1]) < tol
+        E.append(HS[0])
+    assert max(E) - min(E) < 1e-4, E
+    assert E[0] != E[1] and E[0] != E[2]
