@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <map>
 #include <string>
@@ -109,6 +110,11 @@ struct imsrg_basis_s {
   double *SB = nullptr, *CV = nullptr;
   double *Wb1 = nullptr, *Wb2 = nullptr, *cbar = nullptr, *Y = nullptr, *U = nullptr;
   double *aLCC = nullptr, *aRCC = nullptr, *aLCCb = nullptr, *aRCCb = nullptr, *Mb = nullptr;
+  // vectorize stream: runs, length, device staging vector (built on first use)
+  std::vector<VecTask> vec_h;
+  VecTask* vec = nullptr;
+  double* vecbuf = nullptr;
+  i64 neq = 0;
 
   ~imsrg_basis_s() {
     for (void* p : allocs) (void)hipFree(p);
@@ -1022,6 +1028,7 @@ struct imsrg_flow_s {
   std::vector<TermRec> lmrec, lbrec;   // terms of the last segment's expansions, kept by the forward sweep
   int last_n0 = -1;                    // first step of that segment (-1: none kept)
   double* decDG = nullptr;   // generator buffer of imsrg_magnus_decouple
+  double* tOp = nullptr;     // result buffer of imsrg_transform_sq_op_bch, generator of imsrg_dhds
   bool fw_ok = false;        // a checkpointed forward sweep is held
   int fw_steps = 0, fw_ckpt = 1;
   double fw_ds = 0.5, fw_bch = 1e-6, fw_mag = 1e-6;
@@ -1742,6 +1749,60 @@ int flow_backward(imsrg_flow_s* F, double* Tb, hipStream_t s) {
   return 0;
 }
 
+// The vectorize order (basic_IMSRG.f90:3929-3981): E0, fhh, fpp, fph (each
+// column-major, whole), then per channel gam(1..6): rectangular blocks whole
+// (column-major), square blocks column by column from the diagonal down.
+// neq as read_omega_checkpoint counts it (basic_IMSRG.f90:3338-3348).
+void vec_plan(const Basis& b, std::vector<VecTask>* out, i64* neq) {
+  std::vector<VecTask> v;
+  i64 k = 0;
+  auto run = [&](i64 off, i64 len, int mirror) {
+    // split long whole-block runs so no workgroup streams more than 64 Ki
+    // elements (a square-block column, len < 2^16, stays one run: its
+    // mirrored partners are addressed from the run's start)
+    if (mirror) {
+      v.push_back(VecTask{off, k, (int)len, mirror});
+      k += len;
+      return;
+    }
+    for (i64 x = 0; x < len; x += 65536) {
+      const int l = (int)std::min<i64>(65536, len - x);
+      v.push_back(VecTask{off + x, k, l, mirror});
+      k += l;
+    }
+  };
+  run(0, 1, 0);
+  run(b.off_fhh, (i64)b.A * b.A, 0);
+  run(b.off_fpp, (i64)b.P * b.P, 0);
+  run(b.off_fph, (i64)b.P * b.A, 0);
+  for (int q = 0; q < b.nblocks; ++q) {
+    const int np = b.dims[q * 3], nb = b.dims[q * 3 + 1], nh = b.dims[q * 3 + 2];
+    const int rows[6] = {np, np, np, nb, nh, nb}, cols[6] = {np, nb, nh, nb, nh, nh};
+    for (int g = 0; g < 6; ++g) {
+      const i64 off = b.goff[q * 6 + g];
+      const int nr = rows[g], nc = cols[g];
+      if (g == 0 || g == 3 || g == 4) {
+        for (int i = 0; i < nc; ++i) run(off + i + (i64)i * nr, nr - i, nr);
+      } else {
+        run(off, (i64)nr * nc, 0);
+      }
+    }
+  }
+  if (out) *out = std::move(v);
+  if (neq) *neq = k;
+}
+
+int ensure_vec(imsrg_basis_s* B) {
+  if (B->vec) return 0;
+  int rc;
+  vec_plan(B->b, &B->vec_h, &B->neq);
+  if ((rc = upload(B, B->vec_h, &B->vec))) return rc;
+  void* p = nullptr;
+  if ((rc = dalloc(B, (size_t)B->neq * sizeof(double), &p))) return rc;
+  B->vecbuf = (double*)p;
+  return 0;
+}
+
 }  // namespace
 
 // ================================================================== C-ABI
@@ -1780,6 +1841,11 @@ int64_t imsrg_basis_size(imsrg_basis B, int what) {
     case IMSRG_SIZE_NCC: return b.ncc;
     case IMSRG_SIZE_CC: return b.cclen;
     case IMSRG_SIZE_WCC: return b.wclen;
+    case IMSRG_SIZE_NEQ: {
+      i64 n = 0;
+      vec_plan(b, nullptr, &n);
+      return n;
+    }
   }
   return -1;
 }
@@ -1998,6 +2064,72 @@ int imsrg_build_gs_atanxx(imsrg_basis B, const double* Hold, const double* H, do
   return gen(basis_exec(B), Hold, H, ETA, 1, s);
 }
 
+int imsrg_vectorize(imsrg_basis B, const double* op, double* vec, imsrg_stream stream) {
+  PREP();
+  if (!op || !vec) return fail(IMSRG_EINVAL, "null argument");
+  int rc;
+  if ((rc = ensure_vec(B))) return rc;
+  launch_vectorize(B->vec, (int)B->vec_h.size(), (double*)op, vec, 0, 1, s);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int imsrg_repackage(imsrg_basis B, const double* vec, int herm, double* op, imsrg_stream stream) {
+  PREP();
+  if (!op || !vec) return fail(IMSRG_EINVAL, "null argument");
+  if (herm != 1 && herm != -1) return fail(IMSRG_EINVAL, "herm must be +1 or -1");
+  int rc;
+  if ((rc = ensure_vec(B))) return rc;
+  launch_vectorize(B->vec, (int)B->vec_h.size(), op, (double*)vec, 1, herm, s);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int imsrg_write_op_bin(imsrg_basis B, const double* op, const char* path, imsrg_stream stream) {
+  PREP();
+  if (!op || !path) return fail(IMSRG_EINVAL, "null argument");
+  int rc;
+  if ((rc = ensure_vec(B))) return rc;
+  std::vector<double> h((size_t)B->neq);
+  launch_vectorize(B->vec, (int)B->vec_h.size(), (double*)op, B->vecbuf, 0, 1, s);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(h.data(), B->vecbuf, h.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  FILE* f = std::fopen(path, "wb");
+  if (!f) return fail(IMSRG_EINVAL, std::string("cannot open ") + path + " for writing");
+  const size_t w = std::fwrite(h.data(), sizeof(double), h.size(), f);
+  const int cl = std::fclose(f);
+  if (w != h.size() || cl != 0) return fail(IMSRG_EINVAL, std::string("short write to ") + path);
+  return 0;
+}
+
+int imsrg_read_op_bin(imsrg_basis B, const char* path, int herm, double* op, imsrg_stream stream) {
+  PREP();
+  if (!op || !path) return fail(IMSRG_EINVAL, "null argument");
+  if (herm != 1 && herm != -1) return fail(IMSRG_EINVAL, "herm must be +1 or -1");
+  int rc;
+  if ((rc = ensure_vec(B))) return rc;
+  FILE* f = std::fopen(path, "rb");
+  if (!f) return fail(IMSRG_EINVAL, std::string("cannot open ") + path);
+  std::fseek(f, 0, SEEK_END);
+  const long bytes = std::ftell(f);
+  std::fseek(f, 0, SEEK_SET);
+  if (bytes != (long)(B->neq * (i64)sizeof(double))) {
+    std::fclose(f);
+    return fail(IMSRG_EINVAL, std::string(path) + ": " + std::to_string(bytes) + " bytes, the basis needs " +
+                                  std::to_string(B->neq * 8));
+  }
+  std::vector<double> h((size_t)B->neq);
+  const size_t r = std::fread(h.data(), sizeof(double), h.size(), f);
+  std::fclose(f);
+  if (r != h.size()) return fail(IMSRG_EINVAL, std::string("short read from ") + path);
+  HIPCHK(hipMemcpyAsync(B->vecbuf, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, s));
+  launch_vectorize(B->vec, (int)B->vec_h.size(), op, B->vecbuf, 1, herm, s);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(s));
+  return 0;
+}
+
 int imsrg_mbpt2(imsrg_basis B, const double* H, double* value, imsrg_stream stream) {
   PREP();
   int rc = mbpt2_dev(basis_exec(B), H, B->dval, s);
@@ -2061,6 +2193,35 @@ int imsrg_bch_expand(imsrg_flow F, double* HS, const double* G, int Gh, const do
   if (!F) return fail(IMSRG_EINVAL, "null flow");
   if (F->batch) return fail(IMSRG_EINVAL, "batched flow: use the imsrg_batch_* calls");
   return bch(F, HS, G, Gh, H, Hh, conv, maxit, nterms, advals, S(stream));
+}
+
+int imsrg_transform_sq_op_bch(imsrg_flow F, double* Op, int Oph, const double* G, double conv, int maxit,
+                              int* nterms, imsrg_stream stream) {
+  if (!F || !Op || !G) return fail(IMSRG_EINVAL, "null argument");
+  if (F->batch) return fail(IMSRG_EINVAL, "batched flow: use the imsrg_batch_* calls");
+  hipStream_t s = S(stream);
+  const i64 n = F->B->b.oplen;
+  int rc;
+  if ((rc = fbuf(F, &F->tOp, n))) return rc;
+  // Oevolved = BCH(G, Op); copy_sq_op(Oevolved, Op)
+  if ((rc = bch(F, F->tOp, G, -1, Op, Oph, conv, maxit, nterms, nullptr, s))) return rc;
+  HIPCHK(hipMemcpyAsync(Op, F->tOp, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+  return 0;
+}
+
+int imsrg_dhds(imsrg_flow F, const double* HS, double* DH, imsrg_stream stream) {
+  if (!F || !HS || !DH) return fail(IMSRG_EINVAL, "null argument");
+  if (F->batch) return fail(IMSRG_EINVAL, "batched flow: use the imsrg_batch_* calls");
+  if (HS == DH) return fail(IMSRG_EINVAL, "DH must not alias HS");
+  hipStream_t s = S(stream);
+  int rc;
+  if ((rc = fbuf(F, &F->tOp, F->B->b.oplen))) return rc;
+  double* eta = F->tOp;
+  if ((rc = generator(F, F->X, HS, eta, s))) return rc;
+  t_pandya(F->X, eta, -1, F->GCC, s);
+  t_full(F->X, eta, -1, HS, 1, DH, 1, true, F->w1, F->w2, F->GCC, F->ADCC, F->WCC, s);
+  HIPCHK(hipGetLastError());
+  return 0;
 }
 
 int imsrg_magnus_expand(imsrg_flow F, double* DG, int DGh, const double* G, int Gh, double conv, int* ncomm,

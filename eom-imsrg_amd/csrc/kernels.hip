@@ -1114,6 +1114,28 @@ __global__ __launch_bounds__(256) void k_no_1b(DevBasis B, double* H) {
 
 __global__ void k_set_e0(double* H, const double* E0) { H[0] = E0[0]; }
 
+// vectorize / repackage, basic_IMSRG.f90:3929-4053: one workgroup per run.
+// In repackage the lower element X(j,i) is stored before its partner
+// X(i,j) = herm * v by the same thread, so a diagonal element ends as
+// herm * v, as in the reference's loop.
+__global__ __launch_bounds__(256) void k_vectorize(const VecTask* tasks, double* op, double* vec, int dir,
+                                                   double herm) {
+  const VecTask t = tasks[blockIdx.x];
+  for (int x = threadIdx.x; x < t.len; x += blockDim.x) {
+    if (dir == 0) {
+      vec[t.vec_off + x] = op[t.op_off + x];
+    } else {
+      const double v = vec[t.vec_off + x];
+      op[t.op_off + x] = v;
+      if (t.mirror) op[t.op_off + (i64)x * t.mirror] = v * herm;
+    }
+  }
+}
+void launch_vectorize(const VecTask* t, int nt, double* op, double* vec, int dir, int herm, hipStream_t s) {
+  if (nt <= 0) return;
+  hipLaunchKernelGGL(k_vectorize, dim3(nt), dim3(256), 0, s, t, op, vec, dir, (double)herm);
+}
+
 // E0 = v for every instance of an operator array
 __global__ void k_fill_e0(DevBasis B, double* op, double v) {
   int inst;

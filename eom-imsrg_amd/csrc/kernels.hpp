@@ -63,6 +63,12 @@ struct GemmTask { int prob; int tm; int tn; int ks; };
 // Tile pair (I,J), I <= J, of the pppp block of channel q
 struct PairTask { int q; int ti; int tj; int pad; };
 
+// One contiguous run of the vectorize stream (basic_IMSRG.f90:3929-3981):
+// len operator elements from op_off map to vector positions vec_off...; for
+// a column of a square block (mirror = its dimension n) repackage also
+// writes the transposed partners op_off + x * n, times herm.
+struct VecTask { i64 op_off; i64 vec_off; int len; int mirror; };
+
 // Workgroups are dispatched to the 8 XCDs round-robin (blockIdx mod 8), each
 // XCD with its own L2.  An XcdMap gives every XCD its own contiguous task
 // range so the tiles of one channel share an L2: block b runs task
@@ -97,6 +103,8 @@ void launch_comm222ph_quad(const DevBasis& B, const QuadItem* items, int n, cons
                            double* RES, int RESh, hipStream_t s, int mirror = 1);
 // square blocks' lower triangles from their upper triangles (herm)
 void launch_sym_lower(const DevBasis& B, const PairTask* t, int nt, double* RES, int herm, hipStream_t s);
+// dir 0: vec <- op (vectorize); dir 1: op <- vec (repackage, herm fill)
+void launch_vectorize(const VecTask* t, int nt, double* op, double* vec, int dir, int herm, hipStream_t s);
 void launch_comm122p(const DevBasis& B, const SegTask* t, int nt, const Partner* part, const int* rowp,
                      const int* colp, const double* L, int Lh, const double* R, int Rh, double* RES, int RESh,
                      hipStream_t s, int mirror = 1);

@@ -19,7 +19,7 @@ IMSRG_ECLIP = 3
 IMSRG_ESTEPS = 4
 PATH_GROUPED = 0
 PATH_PLAIN = 1
-SIZE = dict(NORB=0, NHOLE=1, NPART=2, JTOTAL_MAX=3, NBLOCKS=4, OP=5, NCC=6, CC=7, WCC=8)
+SIZE = dict(NORB=0, NHOLE=1, NPART=2, JTOTAL_MAX=3, NBLOCKS=4, OP=5, NCC=6, CC=7, WCC=8, NEQ=9)
 
 
 class ImsrgError(RuntimeError):
@@ -57,6 +57,12 @@ SIGNATURES = {
     "imsrg_free": (_i, [_dp]),
     "imsrg_op_put": (_i, [_p, _dp, _i, _i, _d]),
     "imsrg_op_get": (_i, [_p, _dp, _i, _i, _d]),
+    "imsrg_vectorize": (_i, [_p, _dp, _dp, _p]),
+    "imsrg_repackage": (_i, [_p, _dp, _i, _dp, _p]),
+    "imsrg_write_op_bin": (_i, [_p, _dp, C.c_char_p, _p]),
+    "imsrg_read_op_bin": (_i, [_p, C.c_char_p, _i, _dp, _p]),
+    "imsrg_transform_sq_op_bch": (_i, [_p, _dp, _i, _dp, C.c_double, _i, _ip, _p]),
+    "imsrg_dhds": (_i, [_p, _dp, _dp, _p]),
     "imsrg_commutator_110": (_i, [_p, _dp, _i, _dp, _i, _d, _p]),
     "imsrg_commutator_220": (_i, [_p, _dp, _i, _dp, _i, _d, _p]),
     "imsrg_commutator_e0": (_i, [_p, _dp, _i, _dp, _i, _dp, _p]),

@@ -44,6 +44,7 @@ class Basis:
         self.ncc = size("NCC")
         self.cclen = size("CC")
         self.wcclen = size("WCC")
+        self.neq = size("NEQ")
         nb = self.nblocks
         self.dims = np.zeros(nb * 3, np.int32)
         self.goff = np.zeros(nb * 6, np.int64)

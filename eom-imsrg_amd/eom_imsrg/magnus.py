@@ -45,6 +45,20 @@ class Flow:
         ETA.herm = -1
         _lib.check(_lib.lib.imsrg_build_gs_w2(self.handle, H.ptr, ETA.ptr, _stream()))
 
+    def transform_sq_op_BCH(self, Op: SqOp, G: SqOp, conv: float = REF_TOL, maxit: int = 40) -> int:
+        """transform_sq_op_BCH, IMSRG_MAGNUS.f90:131-146 (scalar operators):
+        Op <- exp(G) Op exp(-G) in place; returns BCH_EXPAND's exit index"""
+        nt = C.c_int()
+        _lib.check(_lib.lib.imsrg_transform_sq_op_bch(self.handle, Op.ptr, Op.herm, G.ptr, float(conv), int(maxit),
+                                                      C.byref(nt), _stream()))
+        return nt.value
+
+    def dHds(self, HS: SqOp, DH: SqOp) -> None:
+        """dHds, IMSRG_ODE.f90:416-451: DH = [eta(HS), HS] with this flow's
+        generator (E0 included)"""
+        DH.herm = 1
+        _lib.check(_lib.lib.imsrg_dhds(self.handle, HS.ptr, DH.ptr, _stream()))
+
     def set_graphs(self, on: bool) -> None:
         _lib.check(_lib.lib.imsrg_flow_set_graphs(self.handle, 1 if on else 0))
         self.graphs = bool(on)

@@ -55,6 +55,37 @@ class SqOp:
     def clone(self) -> "SqOp":
         return SqOp(self.basis, self.data.clone(), self.herm)
 
+    # vectorize / repackage, basic_IMSRG.f90:3929-4053
+    def vectorize(self) -> torch.Tensor:
+        """the reference's vectorize stream (basis.neq doubles, device)"""
+        from . import _lib
+        v = torch.empty(self.basis.neq, dtype=torch.float64, device=self.data.device)
+        _lib.check(_lib.lib.imsrg_vectorize(self.basis.handle, self.ptr, v.data_ptr(), _stream()))
+        return v
+
+    @classmethod
+    def repackage(cls, basis: Basis, vec: torch.Tensor, herm: int = 1) -> "SqOp":
+        from . import _lib
+        if vec.dtype != torch.float64 or not vec.is_cuda or vec.numel() != basis.neq:
+            raise ValueError(f"repackage needs a CUDA float64 vector of {basis.neq} elements")
+        op = cls.zeros(basis, herm, vec.device)
+        _lib.check(_lib.lib.imsrg_repackage(basis.handle, vec.contiguous().data_ptr(), int(herm), op.ptr,
+                                            _stream()))
+        return op
+
+    def write_bin(self, path: str) -> None:
+        """the operator file of write_omega_checkpoint (basic_IMSRG.f90:3215-3301)"""
+        from . import _lib
+        _lib.check(_lib.lib.imsrg_write_op_bin(self.basis.handle, self.ptr, str(path).encode(), _stream()))
+
+    @classmethod
+    def read_bin(cls, basis: Basis, path: str, herm: int = 1) -> "SqOp":
+        """read_omega_checkpoint's read + repackage (basic_IMSRG.f90:3304-3407)"""
+        from . import _lib
+        op = cls.zeros(basis, herm)
+        _lib.check(_lib.lib.imsrg_read_op_bin(basis.handle, str(path).encode(), int(herm), op.ptr, _stream()))
+        return op
+
     def gam(self, q: int, g: int) -> torch.Tensor:
         """view of gam(g+1) of channel q as an (rows, cols) matrix"""
         n1, n2 = self.basis.block_shape(q, g)

@@ -56,6 +56,7 @@ typedef void* imsrg_stream;
 #define IMSRG_SIZE_NCC 6  /* cross-coupled blocks */
 #define IMSRG_SIZE_CC 7   /* doubles per cross-coupled matrix set (cc_mat) */
 #define IMSRG_SIZE_WCC 8  /* doubles of the WCC workspace */
+#define IMSRG_SIZE_NEQ 9  /* doubles of the vectorize stream (H%neq) */
 
 const char* imsrg_last_error(void);
 
@@ -93,6 +94,22 @@ int imsrg_free(double* p);
  * Column-major, exactly the Fortran array.  Synchronous. */
 int imsrg_op_put(imsrg_basis b, double* op, int q, int g, const double* host);
 int imsrg_op_get(imsrg_basis b, const double* op, int q, int g, double* host);
+
+/* vectorize, basic_IMSRG.f90:3929-3981: vec (device, IMSRG_SIZE_NEQ
+ * doubles) <- E0, fhh, fpp, fph, then per channel gam(1..6) with the square
+ * blocks' lower triangles only. */
+int imsrg_vectorize(imsrg_basis b, const double* op, double* vec, imsrg_stream stream);
+/* repackage, basic_IMSRG.f90:3984-4053: op <- vec; square blocks get
+ * X(i,j) = herm * X(j,i) (a diagonal element ends as herm * v, as there). */
+int imsrg_repackage(imsrg_basis b, const double* vec, int herm, double* op, imsrg_stream stream);
+/* the .bin operator file of write_omega_checkpoint / read_omega_checkpoint
+ * (basic_IMSRG.f90:3215-3407): the vectorize stream as raw native doubles
+ * (Fortran unformatted stream access, no record markers).  The file name
+ * (prefix, _s<s>.bin) is the caller's.  Reading requires exactly
+ * IMSRG_SIZE_NEQ doubles (the reference reads the first neq without a
+ * check).  Synchronous. */
+int imsrg_write_op_bin(imsrg_basis b, const double* op, const char* path, imsrg_stream stream);
+int imsrg_read_op_bin(imsrg_basis b, const char* path, int herm, double* op, imsrg_stream stream);
 
 /* ------------------------------------------------ commutator terms */
 /* commutator_110, commutators.f90:10-33 (returns the value; synchronises) */
@@ -174,6 +191,15 @@ int imsrg_build_gs_w2(imsrg_flow f, const double* H, double* ETA, imsrg_stream s
  * advals (host, optional, length maxit) receives ||term||*coef. */
 int imsrg_bch_expand(imsrg_flow f, double* HS, const double* G, int Gherm, const double* H, int Hherm,
                      double conv, int maxit, int* nterms, double* advals, imsrg_stream stream);
+/* transform_sq_op_BCH, IMSRG_MAGNUS.f90:131-146, scalar operators: Op <-
+ * exp(G) Op exp(-G) in place (G antihermitian, Op of hermiticity Opherm),
+ * with BCH_EXPAND's terms and exit (nterms as imsrg_bch_expand). */
+int imsrg_transform_sq_op_bch(imsrg_flow f, double* Op, int Opherm, const double* G, double conv, int maxit,
+                              int* nterms, imsrg_stream stream);
+/* dHds, IMSRG_ODE.f90:416-451: DH = [eta(HS), HS] with E0 = 110 + 220, eta
+ * the flow's generator (imsrg_flow_set_generator); DH herm = +1.  The
+ * ODE solver's vectorize of DH is imsrg_vectorize. */
+int imsrg_dhds(imsrg_flow f, const double* HS, double* DH, imsrg_stream stream);
 /* MAGNUS_EXPAND, IMSRG_MAGNUS.f90:620-691: DG <- dOmega/ds from eta (in DG)
  * and Omega G.  ncomm = nested commutators evaluated. */
 int imsrg_magnus_expand(imsrg_flow f, double* DG, int DGherm, const double* G, int Gherm, double conv,

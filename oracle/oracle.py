@@ -117,6 +117,52 @@ class OracleBasis:
                            jtp.ctypes.data_as(C.POINTER(C.c_int)))
         return rl, nc, co, wo, jtp.reshape(m, 3)
 
+    # --- vectorize / repackage, basic_IMSRG.f90:3929-4053, on the flat layout
+    # [E0 | fhh A x A | fph P x A | fpp P x P | channel blocks]
+    def _one_body_views(self, op):
+        A, P = self.A, self.P
+        o1 = 1 + A * A
+        o2 = o1 + P * A
+        return op[1:o1], op[o2: o2 + P * P], op[o1:o2]   # fhh, fpp, fph (column-major)
+
+    def _blocks(self):
+        for q in range(self.nblocks):
+            npp, nph, nhh = (int(x) for x in self.dims[q])
+            shapes = [(npp, npp), (npp, nph), (npp, nhh), (nph, nph), (nhh, nhh), (nph, nhh)]
+            for g, (nr, nc) in enumerate(shapes):
+                yield int(self.goff[q, g]), nr, nc, g in (0, 3, 4)
+
+    def vectorize(self, op):
+        out = [op[:1], *self._one_body_views(op)]
+        for o, nr, nc, square in self._blocks():
+            X = op[o: o + nr * nc].reshape(nc, nr)        # X[i] = column i
+            if square:
+                out.extend(X[i, i:] for i in range(nc))   # rows j = i .. nr-1
+            else:
+                out.append(X.reshape(-1))
+        return np.concatenate(out)
+
+    def repackage(self, vec, herm):
+        op = np.zeros(self.oplen)
+        fhh, fpp, fph = self._one_body_views(op)
+        A, P = self.A, self.P
+        op[0] = vec[0]
+        k = 1
+        for part, n in ((fhh, A * A), (fpp, P * P), (fph, P * A)):
+            part[:] = vec[k: k + n]
+            k += n
+        for o, nr, nc, square in self._blocks():
+            X = op[o: o + nr * nc].reshape(nc, nr)        # view: X[i, j] = element (j, i)
+            for i in range(nc):
+                j0 = i if square else 0
+                for j in range(j0, nr):
+                    X[i, j] = vec[k]
+                    if square:
+                        X[j, i] = vec[k] * herm
+                    k += 1
+        assert k == len(vec)
+        return op
+
     # --- reference routines on host arrays
     def v_elem(self, op, herm, a, b, c, d, J):
         return lib().orc_v_elem(self.h, _dp(op), herm, a, b, c, d, J)

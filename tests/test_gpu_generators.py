@@ -107,3 +107,50 @@ def test_flow_gradient_requires_white(gpu):
         adjoint.flow_forward(f, T)
     with pytest.raises(KeyError):
         f.set_generator("imtime")
+
+
+def test_transform_sq_op_bch(gpu):
+    """transform_sq_op_BCH: the bare H transformed by a flow's final Omega is
+    that flow's decoupled H (the flow's last BCH), and an observable against
+    the oracle's BCH with equal term counts"""
+    from eom_imsrg import Flow, SqOp, synth
+    emax = 4
+    b, o = basis(emax), oracle_basis(emax)
+    print("[transform_sq_op_BCH] emax=4", flush=True)
+    H = _hamiltonian(emax)
+    f = Flow(b, graphs=True)
+    HS = SqOp.from_numpy(b, H, 1)
+    G = SqOp.zeros(b, -1)
+    f.magnus_decouple(HS, G)
+    Op = SqOp.from_numpy(b, H, 1)
+    f.transform_sq_op_BCH(Op, G)
+    torch.cuda.synchronize()
+    assert_rel(Op.numpy(), HS.numpy(), rtol=1e-12, what="transformed bare H")
+    # an observable (another hermitian operator) under the flow's Omega
+    Obs = _hamiltonian(emax, seed=9, hw=16.0, scale=0.5)
+    Gn = G.numpy()
+    ref, it, _ = o.bch(Gn, Obs, REF)
+    O = SqOp.from_numpy(b, Obs, 1)
+    nt = f.transform_sq_op_BCH(O, G)
+    torch.cuda.synchronize()
+    assert nt == it
+    assert_rel(O.numpy(), ref, what="transformed observable")
+
+
+@pytest.mark.parametrize("gen", ["white", "w2"])
+def test_dhds(gpu, gen):
+    """dHds: DH = [eta(H), H] against the oracle's generator and commutator"""
+    from eom_imsrg import Flow, SqOp
+    emax = 4
+    b, o = basis(emax), oracle_basis(emax)
+    print(f"[dHds] emax=4 generator={gen}", flush=True)
+    H = _hamiltonian(emax)
+    eta = o.white(H) if gen == "white" else o.gs_w2(H)
+    ref = o.commutator(eta, -1, H, 1, 1)
+    f = Flow(b, generator=gen)
+    DH = SqOp.zeros(b)
+    Hd = SqOp.from_numpy(b, H, 1)
+    f.dHds(Hd, DH)
+    torch.cuda.synchronize()
+    assert_rel(DH.numpy(), ref, what=f"dHds {gen}")
+    assert np.array_equal(Hd.numpy(), H)
