@@ -68,6 +68,11 @@ struct imsrg_basis_s {
   int *rowp = nullptr, *colp = nullptr;
   std::vector<QuadItem> quad_h;   // 222_ph outputs, quadruple-major (grouped path)
   QuadItem* quad = nullptr;
+  // the same outputs in chunks of quadruples (k_comm222ph_chunk)
+  std::vector<PhChunk> phc_h;
+  std::vector<int> fq_h, fql_h, fqi_h;
+  PhChunk* phc = nullptr;
+  int *fq = nullptr, *fql = nullptr, *fqi = nullptr;
   std::vector<OneTask> one_h;
   OneTask* one = nullptr;
   // norm chunks
@@ -494,6 +499,46 @@ int build_plan(imsrg_basis_s* B) {
       for (int g = 0; g < 6; ++g)
         tri += kSquare[g] ? (i64)b.rows(q, g) * (b.rows(q, g) + 1) / 2 : b.blocksize(q, g);
     if ((i64)B->quad_h.size() != tri) return fail(IMSRG_EINVAL, "quadruple enumeration does not cover the channels");
+    if (b.wclen >= ((i64)1 << 32)) return fail(IMSRG_EINVAL, "WCC exceeds 2^32 doubles");
+    // chunks: consecutive quadruples with at most kPhChunkItems items and
+    // kPhChunkLanes J' terms (the two recouplings' ranges, none when an
+    // isospin / parity check fails: the rules of ph_ranges)
+    B->phc_h.clear(); B->fq_h.clear(); B->fql_h.clear(); B->fqi_h.clear();
+    auto lanes_of = [&](int u) {
+      const int i = u & 255, j = (u >> 8) & 255, k = (u >> 16) & 255, l = (u >> 24) & 255;
+      const int ji = b.jj[i], jj = b.jj[j], jk = b.jj[k], jl = b.jj[l];
+      if (std::abs(b.tz[l] - b.tz[j]) != std::abs(b.tz[i] - b.tz[k]) ||
+          (b.ll[l] + b.ll[j]) % 2 != (b.ll[i] + b.ll[k]) % 2)
+        return 0;
+      if (std::abs(b.tz[k] - b.tz[j]) != std::abs(b.tz[i] - b.tz[l]) ||
+          (b.ll[k] + b.ll[j]) % 2 != (b.ll[i] + b.ll[l]) % 2)
+        return 0;
+      const int jin = std::max(std::abs(jj - jl), std::abs(ji - jk));
+      const int nin = std::max(0, (std::min(jj + jl, ji + jk) - jin) / 2 + 1);
+      const int jout = std::max(std::abs(ji - jl), std::abs(jj - jk));
+      const int nout = std::max(0, (std::min(ji + jl, jj + jk) - jout) / 2 + 1);
+      return nin + nout;
+    };
+    PhChunk cur{0, 0, 0, 0, 0, 0, 0, 0};
+    const size_t nit = B->quad_h.size();
+    for (size_t x = 0; x < nit;) {
+      size_t e = x;
+      while (e < nit && B->quad_h[e].ijkl == B->quad_h[x].ijkl) ++e;
+      const int items = (int)(e - x), lanes = lanes_of(B->quad_h[x].ijkl);
+      if (items > kPhChunkItems || lanes > kPhChunkLanes) return fail(IMSRG_EINVAL, "quadruple exceeds a 222_ph chunk");
+      if (cur.nq > 0 && (cur.nitems + items > kPhChunkItems || cur.nlanes + lanes > kPhChunkLanes)) {
+        B->phc_h.push_back(cur);
+        cur = PhChunk{(int)B->fq_h.size(), 0, (int)x, 0, 0, 0, 0, 0};
+      }
+      B->fq_h.push_back(B->quad_h[x].ijkl);
+      B->fql_h.push_back(cur.nlanes);
+      B->fqi_h.push_back((int)x);
+      cur.nq += 1;
+      cur.nitems += items;
+      cur.nlanes += lanes;
+      x = e;
+    }
+    if (cur.nq > 0) B->phc_h.push_back(cur);
   }
   // norm chunks: fhh (w 1), fph (w 2), fpp (w 1), gam blocks (w 1 square, 2 rectangular)
   B->ch_off_h.clear(); B->ch_len_h.clear(); B->ch_w_h.clear();
@@ -551,6 +596,9 @@ int build_plan(imsrg_basis_s* B) {
     if ((rc = upload(B, php, (int**)&B->D.php))) return rc;
   }
   if ((rc = upload(B, B->quad_h, &B->quad))) return rc;
+  if ((rc = upload(B, B->phc_h, &B->phc)) || (rc = upload(B, B->fq_h, &B->fq)) ||
+      (rc = upload(B, B->fql_h, &B->fql)) || (rc = upload(B, B->fqi_h, &B->fqi)))
+    return rc;
   if ((rc = upload(B, B->part_h, &B->partners))) return rc;
   if ((rc = upload(B, B->rowp_h, &B->rowp))) return rc;
   if ((rc = upload(B, B->colp_h, &B->colp))) return rc;
@@ -719,7 +767,8 @@ void t_ph(const Exec& X, const double* LCC, int Lh, const double* RCC, int Rh, d
   sl.st[0] = B->b.cclen; sl.st[1] = B->b.cclen; sl.st[2] = B->b.wclen;
   run_gemm_x(X, B->ph, sl, s, (double)Lh, -(double)(Rh * Lh));
   if (B->path == IMSRG_PATH_GROUPED)
-    launch_comm222ph_quad(X.D, B->quad, (int)B->quad_h.size(), WCC, Lh, Rh, RES, RESh, s, mirror ? 1 : 0);
+    launch_comm222ph_chunk(X.D, B->phc, (int)B->phc_h.size(), B->fq, B->fql, B->fqi, B->quad, WCC, Lh, Rh, RES,
+                           RESh, s, mirror ? 1 : 0);
   else
     run(B, B->seg_tri,
         [&](const SegTask* t, int n) { launch_comm222ph_scatter(X.D, t, n, WCC, Lh, Rh, RES, RESh, s); });
@@ -1239,6 +1288,9 @@ int gen_w2(imsrg_flow_s* F, const Exec& X, const double* H, double* ETA, hipStre
   double* eta1 = F->buf[0];
   double* H2 = F->buf[1];
   int rc;
+  // eta1 starts from zero as duplicate_sq_op leaves it: the generator
+  // writes fph only where a and i share (l, j, tz)
+  launch_fill_b(eta1, n, 0.0, X.D.ninst, X.D.act, s);
   if ((rc = gen(X, H, H, eta1, 1, s))) return rc;
   launch_fill_e0(X.D, eta1, 0.0, s);
   t_pandya(X, eta1, -1, F->GCC, s);
@@ -2217,6 +2269,7 @@ int imsrg_dhds(imsrg_flow F, const double* HS, double* DH, imsrg_stream stream) 
   int rc;
   if ((rc = fbuf(F, &F->tOp, F->B->b.oplen))) return rc;
   double* eta = F->tOp;
+  launch_fill(eta, F->B->b.oplen, 0.0, s);   // duplicate_sq_op(HS, ETA): a zero operator
   if ((rc = generator(F, F->X, HS, eta, s))) return rc;
   t_pandya(F->X, eta, -1, F->GCC, s);
   t_full(F->X, eta, -1, HS, 1, DH, 1, true, F->w1, F->w2, F->GCC, F->ADCC, F->WCC, s);
@@ -2558,7 +2611,8 @@ int imsrg_profile_kernel(imsrg_basis B, int which, const double* L, int Lh, cons
       }
       case IMSRG_K_PH_SCATTER:   // as the grouped commutator runs it: upper triangles only
         if (B->path == IMSRG_PATH_GROUPED)
-          launch_comm222ph_quad(B->D, B->quad, (int)B->quad_h.size(), WCC, Lh, Rh, RES, RESh, s, 0);
+          launch_comm222ph_chunk(B->D, B->phc, (int)B->phc_h.size(), B->fq, B->fql, B->fqi, B->quad, WCC, Lh, Rh,
+                                 RES, RESh, s, 0);
         else
           run(B, B->seg_tri,
               [&](const SegTask* t, int n) { launch_comm222ph_scatter(B->D, t, n, WCC, Lh, Rh, RES, RESh, s); });

@@ -169,9 +169,54 @@ void launch_comm122p(const DevBasis& B, const SegTask* t, int nt, const Partner*
 //        =  W'(gik,rjl) + ph W'(gjl,rik)
 // outside: W'(gil,rjk) + ph W'(gjk,ril).
 // W' satisfies W'(y,x) = -Lh Rh W'(x,y), so the GEMM stores its upper
-// triangle only and the scatter reads the element (min, max).
-__device__ __forceinline__ double wprime(const double* W, i64 rr, int x, int y, double mpm) {
-  return x <= y ? W[x + (i64)y * rr] : mpm * W[y + (i64)x * rr];
+// triangle only and the scatter reads the element (min, max).  (Storing
+// both triangles, for reads along columns only, costs the GEMM more in
+// transposed stores than it saves here.)
+
+// sixj for arguments whose four triangle conditions hold by construction
+// (the J' ranges below), so the table read is unconditional
+__device__ __forceinline__ double sixj_tri(const DevBasis& B, int j1, int j2, int j3, int j4, int j5, int j6) {
+  const int nh = B.nh6;
+  const i64 idx = (((((i64)(j1 >> 1) * nh + (j2 >> 1)) * nh + (j4 >> 1)) * nh + (j5 >> 1)) * B.nJ6 + (j6 >> 1)) *
+                      B.nJ6 + (j3 >> 1);
+  return B.six[idx];
+}
+
+// One recoupling's sum over J' (jmin..jmax step 2), added onto sm:
+//   inside:  - (-1)^((jl+jk+J)/2) (W'(gik,rjl) + phase W'(gjl,rik)) {ji jj J; jl jk J'}
+//   outside: + (W'(gil,rjk) + phase W'(gjk,ril)) {ji jj J; jk jl J'}
+// ph_pair gives the W' combination of one J' term.
+template <bool INSIDE>
+__device__ __forceinline__ double ph_pair(const DevBasis& B, const double* WCC, double mpm, int phase, int i, int j,
+                                          int k, int l, int JP, int Tz, int PAR) {
+  // inside reads rows (k,i) / cols (j,l); outside rows (l,i) / cols (j,k).
+  // 32-bit element offsets from uniform bases (WCC holds < 2^32 doubles,
+  // checked at plan time)
+  const int N = B.N, JTM = B.JTM;
+  const int a = INSIDE ? k : l, c = INSIDE ? l : k;
+  const int qx = JP / 2 + Tz * (JTM + 1) + 2 * PAR * (JTM + 1);
+  const unsigned t = (unsigned)(JP >> 1) * (unsigned)(N * N);
+  const unsigned wo = (unsigned)B.wcoff[qx], rr = (unsigned)B.rlen[qx];
+  const unsigned x1 = B.rmapJ[t + a * N + i];    // gik | gil
+  const unsigned y1 = B.rmapJ[t + j * N + c];    // rjl | rjk
+  const unsigned x2 = B.rmapJT[t + a * N + i];   // rik | ril
+  const unsigned y2 = B.rmapJ[t + c * N + j];    // gjl | gjk
+  // W'(x1, y1) + phase W'(y2, x2), each read from the upper triangle
+  const double A1 = x1 <= y1 ? WCC[wo + x1 + y1 * rr] : mpm * WCC[wo + y1 + x1 * rr];
+  const double A2 = y2 <= x2 ? WCC[wo + y2 + x2 * rr] : mpm * WCC[wo + x2 + y2 * rr];
+  return A1 + phase * A2;
+}
+
+template <bool INSIDE>
+__device__ __forceinline__ double ph_jsum(const DevBasis& B, const double* WCC, double mpm, int i, int j, int k, int l,
+                                          int Jtot, int jmin, int jmax, int phase, int Tz, int PAR, double sm) {
+  const int ji = B.jj[i], jj = B.jj[j], jk = B.jj[k], jl = B.jj[l];
+  const double sg = INSIDE ? -(double)m1pow((jl + jk + Jtot) / 2) : 1.0;
+  for (int JP = jmin; JP <= jmax; JP += 2) {
+    const double v = ph_pair<INSIDE>(B, WCC, mpm, phase, i, j, k, l, JP, Tz, PAR);
+    sm = sm + sg * v * (INSIDE ? sixj_tri(B, ji, jj, Jtot, jl, jk, JP) : sixj_tri(B, ji, jj, Jtot, jk, jl, JP));
+  }
+  return sm;
 }
 
 __device__ __forceinline__ void ph_element(const DevBasis& B, const double* WCC, int Lh, int Rh, double* RES,
@@ -186,48 +231,22 @@ __device__ __forceinline__ void ph_element(const DevBasis& B, const double* WCC,
   const int ji = B.jj[i], jj = B.jj[j], jk = B.jj[k], jl = B.jj[l];
   const int li = B.ll[i], lj = B.ll[j], lk = B.ll[k], ll = B.ll[l];
   const int ti = B.tz[i], tj = B.tz[j], tk = B.tz[k], tl = B.tz[l];
-  const int N = B.N, JTM = B.JTM;
+  const int phase = m1pow((ji + jj + jk + jl) / 2);
   double sm = 0.0;
   // "inside" recoupling
   int Tz = abs(ti - tk) / 2;
   if (abs(tl - tj) != Tz * 2) return;
   int PAR = (li + lk) % 2;
   if ((ll + lj) % 2 != PAR) return;
-  int jmin = max(abs(jj - jl), abs(ji - jk));
-  int jmax = min(jj + jl, ji + jk);
-  int phase = m1pow((ji + jj + jk + jl) / 2);
-  for (int JP = jmin; JP <= jmax; JP += 2) {
-    const int qx = JP / 2 + Tz * (JTM + 1) + 2 * PAR * (JTM + 1);
-    const double* W = WCC + B.wcoff[qx];
-    const i64 rr = B.rlen[qx];
-    const int jo = JP >> 1;
-    const int* rJ = B.rmapJ + (i64)jo * N * N;
-    const int* rT = B.rmapJT + (i64)jo * N * N;
-    const int rjl = rJ[j * N + l], rik = rT[k * N + i];
-    const int gjl = rJ[l * N + j], gik = rJ[k * N + i];
-    sm = sm - m1pow((jl + jk + Jtot) / 2) * (wprime(W, rr, gik, rjl, mpm) + phase * wprime(W, rr, gjl, rik, mpm)) *
-                  sixj(B, ji, jj, Jtot, jl, jk, JP);
-  }
+  sm = ph_jsum<true>(B, WCC, mpm, i, j, k, l, Jtot, max(abs(jj - jl), abs(ji - jk)),
+                                min(jj + jl, ji + jk), phase, Tz, PAR, sm);
   // "outside" recoupling
   Tz = abs(ti - tl) / 2;
   if (abs(tk - tj) != Tz * 2) return;
   PAR = (li + ll) % 2;
   if ((lk + lj) % 2 != PAR) return;
-  jmin = max(abs(ji - jl), abs(jj - jk));
-  jmax = min(ji + jl, jj + jk);
-  phase = m1pow((ji + jj + jk + jl) / 2);
-  for (int JP = jmin; JP <= jmax; JP += 2) {
-    const int qx = JP / 2 + Tz * (JTM + 1) + 2 * PAR * (JTM + 1);
-    const double* W = WCC + B.wcoff[qx];
-    const i64 rr = B.rlen[qx];
-    const int jo = JP >> 1;
-    const int* rJ = B.rmapJ + (i64)jo * N * N;
-    const int* rT = B.rmapJT + (i64)jo * N * N;
-    const int ril = rT[l * N + i], rjk = rJ[j * N + k];
-    const int gil = rJ[l * N + i], gjk = rJ[k * N + j];
-    sm = sm + (wprime(W, rr, gil, rjk, mpm) + phase * wprime(W, rr, gjk, ril, mpm)) *
-                  sixj(B, ji, jj, Jtot, jk, jl, JP);
-  }
+  sm = ph_jsum<false>(B, WCC, mpm, i, j, k, l, Jtot, max(abs(ji - jl), abs(jj - jk)),
+                                 min(ji + jl, jj + jk), phase, Tz, PAR, sm);
   double* X = RES + B.goff[q * 6 + g];
   const i64 o = IX + (i64)JX * n1;
   X[o] = X[o] + sm * pre * pre2;
@@ -261,34 +280,121 @@ void launch_comm222ph_scatter(const DevBasis& B, const SegTask* t, int nt, const
   hipLaunchKernelGGL(k_comm222ph_scatter, dim3(nt, B.ninst), dim3(kTile), 0, s, B, t, W, Lh, Rh, RES, RESh);
 }
 
-// grouped form: one thread per (orbit quadruple, J), quadruple-major with
-// J fastest, so the lanes of one quadruple read identical WCC addresses
-// (one request per wave load) and neighbouring 6j symbols.
-__global__ __launch_bounds__(256) void k_comm222ph_quad(DevBasis B, const QuadItem* items, int n, const double* WCC,
-                                                        int Lh, int Rh, double* RES, int RESh, int mirror) {
+// J' ranges of the two recouplings of quadruple (i,j,k,l) (empty when
+// jmin > jmax); ok = false when an isospin / parity check fails, in which
+// case the element receives nothing (as in ph_element)
+struct PhRanges { int jin, nin, jout, nout; bool ok; };
+__device__ __forceinline__ PhRanges ph_ranges(const DevBasis& B, int i, int j, int k, int l) {
+  PhRanges r{0, 0, 0, 0, false};
+  const int ji = B.jj[i], jj = B.jj[j], jk = B.jj[k], jl = B.jj[l];
+  const int li = B.ll[i], lj = B.ll[j], lk = B.ll[k], ll = B.ll[l];
+  const int ti = B.tz[i], tj = B.tz[j], tk = B.tz[k], tl = B.tz[l];
+  if (abs(tl - tj) != abs(ti - tk) || (ll + lj) % 2 != (li + lk) % 2) return r;
+  if (abs(tk - tj) != abs(ti - tl) || (lk + lj) % 2 != (li + ll) % 2) return r;
+  r.ok = true;
+  r.jin = max(abs(jj - jl), abs(ji - jk));
+  r.nin = max(0, (min(jj + jl, ji + jk) - r.jin) / 2 + 1);
+  r.jout = max(abs(ji - jl), abs(jj - jk));
+  r.nout = max(0, (min(ji + jl, jj + jk) - r.jout) / 2 + 1);
+  return r;
+}
+
+// largest x in [0, n) with a[x] <= t (a ascending, a[0] <= t)
+__device__ __forceinline__ int lds_floor(const int* a, int n, int t) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (a[mid] <= t) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// grouped form by quadruple chunks.  Phase 1: one thread per J' term of
+// the chunk's quadruples loads its two W' values (index tables, then W')
+// into LDS; phase 2: one thread per output (quadruple, J) sums its
+// quadruple's terms from LDS with the 6j weights, in the order ph_element
+// adds them (same result bits).  Each W' term is fetched once per
+// quadruple instead of once per (quadruple, J).
+__global__ __launch_bounds__(256) void k_comm222ph_chunk(DevBasis B, const PhChunk* chunks, const int* fq,
+                                                         const int* fql, const int* fqi, const QuadItem* items,
+                                                         const double* WCC, int Lh, int Rh, double* RES, int RESh,
+                                                         int mirror) {
   int inst;
   if (!batch_inst(B, inst)) return;
   WCC += (i64)inst * B.wclen;
   RES += (i64)inst * B.oplen;
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n) return;
-  const QuadItem it = items[t];
-  const int i = it.ijkl & 255, j = (it.ijkl >> 8) & 255, k = (it.ijkl >> 16) & 255, l = (it.ijkl >> 24) & 255;
+  __shared__ int s_ijkl[kPhChunkItems], s_loff[kPhChunkItems], s_ioff[kPhChunkItems];
+  __shared__ int s_jin[kPhChunkItems], s_nin[kPhChunkItems], s_jout[kPhChunkItems];
+  __shared__ double s_val[kPhChunkLanes];
+  const PhChunk c = chunks[blockIdx.x];
+  const int tid = threadIdx.x;
+  const double mpm = -(double)(Lh * Rh);
+  if (tid < c.nq) {
+    const int u = fq[c.q0 + tid];
+    const int i = u & 255, j = (u >> 8) & 255, k = (u >> 16) & 255, l = (u >> 24) & 255;
+    const PhRanges r = ph_ranges(B, i, j, k, l);
+    s_ijkl[tid] = u;
+    s_loff[tid] = fql[c.q0 + tid];
+    s_ioff[tid] = fqi[c.q0 + tid] - c.item0;
+    s_jin[tid] = r.jin;
+    s_nin[tid] = r.ok ? r.nin : 0;
+    s_jout[tid] = r.ok ? r.jout : -1;   // -1: no contribution at all
+  }
+  __syncthreads();
+  for (int t = tid; t < c.nlanes; t += blockDim.x) {
+    const int q = lds_floor(s_loff, c.nq, t);
+    const int u = s_ijkl[q];
+    const int i = u & 255, j = (u >> 8) & 255, k = (u >> 16) & 255, l = (u >> 24) & 255;
+    const int r = t - s_loff[q];
+    const int phase = m1pow((B.jj[i] + B.jj[j] + B.jj[k] + B.jj[l]) / 2);
+    double v;
+    if (r < s_nin[q])
+      v = ph_pair<true>(B, WCC, mpm, phase, i, j, k, l, s_jin[q] + 2 * r, abs(B.tz[i] - B.tz[k]) / 2,
+                        (B.ll[i] + B.ll[k]) % 2);
+    else
+      v = ph_pair<false>(B, WCC, mpm, phase, i, j, k, l, s_jout[q] + 2 * (r - s_nin[q]),
+                         abs(B.tz[i] - B.tz[l]) / 2, (B.ll[i] + B.ll[l]) % 2);
+    s_val[t] = v;
+  }
+  __syncthreads();
+  if (tid >= c.nitems) return;
+  const int q = lds_floor(s_ioff, c.nq, tid);
+  if (s_jout[q] < 0) return;
+  const QuadItem it = items[c.item0 + tid];
+  const int u = it.ijkl;
+  const int i = u & 255, j = (u >> 8) & 255, k = (u >> 16) & 255, l = (u >> 24) & 255;
   const int J = it.J;
+  const int ji = B.jj[i], jj = B.jj[j], jk = B.jj[k], jl = B.jj[l];
+  const double* v = s_val + s_loff[q];
+  const int nin = s_nin[q], jin = s_jin[q], jout = s_jout[q];
+  const int nout = (q + 1 < c.nq ? s_loff[q + 1] : c.nlanes) - s_loff[q] - nin;
+  double sm = 0.0;
+  const double sg = -(double)m1pow((jl + jk + J) / 2);
+  for (int r = 0; r < nin; ++r) sm = sm + sg * v[r] * sixj_tri(B, ji, jj, J, jl, jk, jin + 2 * r);
+  for (int r = 0; r < nout; ++r) sm = sm + 1.0 * v[nin + r] * sixj_tri(B, ji, jj, J, jk, jl, jout + 2 * r);
+  // output position: channel from (J, Tz, parity), block from the pair classes
   const int P = (B.ll[i] + B.ll[j]) & 1;
-  const int q = 3 * J + 2 * ((B.tz[i] + B.tz[j]) / 2 + 1) + P;
+  const int qc = 3 * J + 2 * ((B.tz[i] + B.tz[j]) / 2 + 1) + P;
   const int rc = B.con[i] + B.con[j], cc = B.con[k] + B.con[l];
   const int g = (rc == 0) ? cc : (rc == 1 ? (cc == 1 ? 3 : 5) : 4);
   const int IX = B.xmap[((i64)i * B.N + j) * B.nJ + (J >> 1)];
   const int JX = B.xmap[((i64)k * B.N + l) * B.nJ + (J >> 1)];
-  ph_element(B, WCC, Lh, Rh, RES, RESh, q, g, IX, JX, i, j, k, l, mirror != 0);
+  const int n1 = brows(B, qc, g);
+  const double pre = (i == j) ? IMSRG_INV_SQRT2_REF : 1.0;
+  const double pre2 = (k == l) ? IMSRG_INV_SQRT2_REF : 1.0;
+  double* X = RES + B.goff[qc * 6 + g];
+  const i64 o = IX + (i64)JX * n1;
+  X[o] = X[o] + sm * pre * pre2;
+  if (is_square(g) && mirror) X[JX + (i64)IX * n1] = X[o] * RESh;
 }
 
-void launch_comm222ph_quad(const DevBasis& B, const QuadItem* items, int n, const double* W, int Lh, int Rh,
-                           double* RES, int RESh, hipStream_t s, int mirror) {
-  if (n <= 0) return;
-  hipLaunchKernelGGL(k_comm222ph_quad, dim3((n + 255) / 256, B.ninst), dim3(256), 0, s, B, items, n, W, Lh, Rh, RES,
-                     RESh, mirror);
+void launch_comm222ph_chunk(const DevBasis& B, const PhChunk* ch, int nch, const int* fq, const int* fql,
+                            const int* fqi, const QuadItem* items, const double* W, int Lh, int Rh, double* RES,
+                            int RESh, hipStream_t s, int mirror) {
+  if (nch <= 0) return;
+  hipLaunchKernelGGL(k_comm222ph_chunk, dim3(nch, B.ninst), dim3(256), 0, s, B, ch, fq, fql, fqi, items, W, Lh, Rh,
+                     RES, RESh, mirror);
 }
 
 // lower triangle of every square block (pppp, phph, hhhh) from its upper

@@ -63,6 +63,13 @@ struct GemmTask { int prob; int tm; int tn; int ks; };
 // Tile pair (I,J), I <= J, of the pppp block of channel q
 struct PairTask { int q; int ti; int tj; int pad; };
 
+// 222_ph on the grouped path: a chunk of consecutive orbit quadruples of
+// the output items (items item0 .. item0 + nitems, at most kPhChunkItems;
+// their J' terms, at most kPhChunkLanes, are gathered once into LDS)
+constexpr int kPhChunkItems = 256;
+constexpr int kPhChunkLanes = 1024;
+struct PhChunk { int q0, nq, item0, nitems, nlanes, pad0, pad1, pad2; };
+
 // One contiguous run of the vectorize stream (basic_IMSRG.f90:3929-3981):
 // len operator elements from op_off map to vector positions vec_off...; for
 // a column of a square block (mirror = its dimension n) repackage also
@@ -99,10 +106,13 @@ void launch_comm122(const DevBasis& B, const SegTask* t, int nt, const double* L
                     int Rh, double* RES, int RESh, hipStream_t s);
 void launch_comm222ph_scatter(const DevBasis& B, const SegTask* t, int nt, const double* W, int Lh, int Rh,
                               double* RES, int RESh, hipStream_t s);
-void launch_comm222ph_quad(const DevBasis& B, const QuadItem* items, int n, const double* W, int Lh, int Rh,
-                           double* RES, int RESh, hipStream_t s, int mirror = 1);
 // square blocks' lower triangles from their upper triangles (herm)
 void launch_sym_lower(const DevBasis& B, const PairTask* t, int nt, double* RES, int herm, hipStream_t s);
+// 222_ph scatter by quadruple chunks: fq packed ijkl per quadruple, fql its
+// first J' term within its chunk, fqi its first item
+void launch_comm222ph_chunk(const DevBasis& B, const PhChunk* ch, int nch, const int* fq, const int* fql,
+                            const int* fqi, const QuadItem* items, const double* W, int Lh, int Rh, double* RES,
+                            int RESh, hipStream_t s, int mirror);
 // dir 0: vec <- op (vectorize); dir 1: op <- vec (repackage, herm fill)
 void launch_vectorize(const VecTask* t, int nt, double* op, double* vec, int dir, int herm, hipStream_t s);
 void launch_comm122p(const DevBasis& B, const SegTask* t, int nt, const Partner* part, const int* rowp,
