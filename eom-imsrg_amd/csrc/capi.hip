@@ -100,8 +100,8 @@ struct imsrg_basis_s {
   // starts, the compact one-body map of 122
   bool adj_ready = false;
   GemmFamily pb[6], phb;
-  std::vector<int> f1map_h, f1off_h;
-  int *f1map = nullptr, *f1off = nullptr;
+  std::vector<int> f1map_h, f1off_h;   // 122: one-body offset -> compact index, and back
+  int* f1off = nullptr;
   int nf1 = 0;
   // 222_ph transpose: quadruples (packed ijkl) with the start of their J
   // items, one lane per (quadruple, recoupling, J'), the upper 64 x 64 tiles
@@ -457,6 +457,16 @@ int build_plan(imsrg_basis_s* B) {
       B->part_h.insert(B->part_h.end(), coll[gi].begin(), coll[gi].end());
     }
     B->colp_h[npairs] = (int)B->part_h.size();
+    // compact index of the one-body elements the partner terms read (the
+    // kernels stage them in LDS); the lists above carried operator offsets
+    B->f1map_h.assign((size_t)(b.off_fpp + (i64)b.P * b.P), -1);
+    B->f1off_h.clear();
+    for (Partner& p : B->part_h) {
+      if (B->f1map_h[p.fk] < 0) { B->f1map_h[p.fk] = (int)B->f1off_h.size(); B->f1off_h.push_back(p.fk); }
+      p.fk = B->f1map_h[p.fk];
+    }
+    B->nf1 = (int)B->f1off_h.size();
+    if (B->nf1 > kF1Max) return fail(IMSRG_EINVAL, "one-body partner set exceeds the LDS table of 122");
   }
   // 222_ph outputs as (orbit quadruple, 2J) items, J fastest: every upper
   // element of every channel exactly once (checked against seg_tri)
@@ -600,6 +610,7 @@ int build_plan(imsrg_basis_s* B) {
       (rc = upload(B, B->fql_h, &B->fql)) || (rc = upload(B, B->fqi_h, &B->fqi)))
     return rc;
   if ((rc = upload(B, B->part_h, &B->partners))) return rc;
+  if ((rc = upload(B, B->f1off_h, &B->f1off))) return rc;
   if ((rc = upload(B, B->rowp_h, &B->rowp))) return rc;
   if ((rc = upload(B, B->colp_h, &B->colp))) return rc;
   if ((rc = upload(B, B->ch_off_h, &B->ch_off))) return rc;
@@ -725,7 +736,8 @@ void t_122(const Exec& X, const double* L, int Lh, const double* R, int Rh, doub
            hipStream_t s, bool mirror = true) {
   imsrg_basis_s* B = X.B;
   run(B, B->seg_tri, [&](const SegTask* t, int n) {
-    launch_comm122p(X.D, t, n, B->partners, B->rowp, B->colp, L, Lh, R, Rh, RES, RESh, s, mirror ? 1 : 0);
+    launch_comm122p(X.D, t, n, B->partners, B->rowp, B->colp, B->f1off, B->nf1, L, Lh, R, Rh, RES, RESh, s,
+                    mirror ? 1 : 0);
   });
 }
 void t_pphh(const Exec& X, const double* L, int Lh, const double* R, int Rh, double* RES, int RESh,
@@ -951,13 +963,6 @@ int ensure_adj(imsrg_basis_s* B) {
     std::vector<int> fillp(B->pht0_h.begin(), B->pht0_h.end() - 1);
     for (size_t e = 0; e < term.size(); ++e) B->phterm_h[fillp[tid[e]]++] = term[e];
   }
-  // compact index of the one-body elements the 122 partner sums read
-  B->f1map_h.assign((size_t)(b.off_fpp + (i64)b.P * b.P), -1);
-  B->f1off_h.clear();
-  for (const Partner& p : B->part_h)
-    if (B->f1map_h[p.foff] < 0) { B->f1map_h[p.foff] = (int)B->f1off_h.size(); B->f1off_h.push_back(p.foff); }
-  B->nf1 = (int)B->f1off_h.size();
-  if (B->nf1 > 4000) return fail(IMSRG_EINVAL, "one-body partner set exceeds the LDS of the 122 transpose");
   for (GemmFamily* f : {&B->pb[0], &B->pb[1], &B->pb[2], &B->pb[3], &B->pb[4], &B->pb[5], &B->phb}) {
     if ((rc = upload(B, f->probs_h, &f->probs))) return rc;
     if ((rc = family_tasks(B, *f, 0))) return rc;
@@ -966,8 +971,6 @@ int ensure_adj(imsrg_basis_s* B) {
       (rc = upload(B, B->phl_h, &B->phl)) || (rc = upload(B, B->phtile_h, &B->phtile)) ||
       (rc = upload(B, B->pht0_h, &B->pht0)) || (rc = upload(B, B->phterm_h, &B->phterm)))
     return rc;
-  if ((rc = upload(B, B->f1map_h, &B->f1map))) return rc;
-  if ((rc = upload(B, B->f1off_h, &B->f1off))) return rc;
   const size_t op = (size_t)std::max<i64>(b.oplen, 1) * sizeof(double);
   const size_t cc = (size_t)std::max<i64>(b.cclen, 1) * sizeof(double);
   if ((rc = dalloc(B, op, (void**)&B->Wb1)) || (rc = dalloc(B, op, (void**)&B->Wb2)) ||
@@ -1011,9 +1014,9 @@ void comm_vjp(imsrg_basis_s* B, const double* L, int Lh, const double* R, int Rh
   // the partner sums
   launch_fill(B->cbar, b.off_fpp + (i64)b.P * b.P, 0.0, s);
   launch_to_dense(D, B->seg_full.dev, B->seg_full.n, RESb, RESh, B->cbar, s);
-  launch_122_vjp_2b(D, B->seg_tri.dev, B->seg_tri.n, B->partners, B->rowp, B->colp, L, Lh, R, Rh, B->cbar, RESh, Lb,
-                    Rb, s);
-  launch_122_vjp_1b(D, B->seg_tri.dev, B->seg_tri.n, B->partners, B->rowp, B->colp, B->f1map, B->f1off, B->nf1, L,
+  launch_122_vjp_2b(D, B->seg_tri.dev, B->seg_tri.n, B->partners, B->rowp, B->colp, B->f1off, B->nf1, L, Lh, R, Rh,
+                    B->cbar, RESh, Lb, Rb, s);
+  launch_122_vjp_1b(D, B->seg_tri.dev, B->seg_tri.n, B->partners, B->rowp, B->colp, B->f1off, B->nf1, L,
                     Lh, R, Rh, RESb, RESh, Lb, Rb, s);
   // 222_ph: scatter transpose into the W' cotangent U, cross-coupled
   // cotangents (triangular-U GEMMs), Pandya transposes

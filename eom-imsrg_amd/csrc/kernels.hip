@@ -85,13 +85,22 @@ void launch_comm122(const DevBasis& B, const SegTask* t, int nt, const double* L
 __device__ __forceinline__ int blk_of(int lo, int hi) { return lo == 0 ? hi : (lo == 1 ? (hi == 1 ? 3 : 5) : 4); }
 
 __global__ __launch_bounds__(256) void k_comm122p(DevBasis B, const SegTask* tasks, const Partner* part,
-                                                  const int* rowp, const int* colp, const double* L, int Lh,
-                                                  const double* R, int Rh, double* RES, int RESh, int mirror) {
+                                                  const int* rowp, const int* colp, const int* f1off, int nf1,
+                                                  const double* L, int Lh, const double* R, int Rh, double* RES,
+                                                  int RESh, int mirror) {
   int inst;
   if (!batch_inst(B, inst)) return;
   L += (i64)inst * B.oplen;
   R += (i64)inst * B.oplen;
   RES += (i64)inst * B.oplen;
+  // the one-body elements the partner terms read, L then R, staged once
+  extern __shared__ double s_f[];
+  for (int k = threadIdx.x; k < nf1; k += blockDim.x) {
+    const int o = f1off[k];
+    s_f[k] = L[o];
+    s_f[nf1 + k] = R[o];
+  }
+  __syncthreads();
   const SegTask t = tasks[blockIdx.x];
   const i64 e = (i64)t.start + threadIdx.x;
   if (e >= t.cnt) return;
@@ -125,7 +134,7 @@ __global__ __launch_bounds__(256) void k_comm122p(DevBasis B, const SegTask* tas
       const i64 o = go[blk_of(cc, cl)] + JX + (i64)idx * dm[cc];
       vr = R[o] * Rh * pre; vl = L[o] * Lh * pre;
     }
-    double fl = L[p.foff], fr = R[p.foff];
+    double fl = s_f[p.fk], fr = s_f[nf1 + p.fk];
     if ((p.code >> 24) & 1) { fl = fl * Lh; fr = fr * Rh; }
     sm = sm + fl * vr - fr * vl;
   }
@@ -143,7 +152,7 @@ __global__ __launch_bounds__(256) void k_comm122p(DevBasis B, const SegTask* tas
       const i64 o = go[blk_of(cl, rc)] + idx + (i64)IX * dm[cl];
       vr = R[o] * Rh * pre; vl = L[o] * Lh * pre;
     }
-    double fl = L[p.foff], fr = R[p.foff];
+    double fl = s_f[p.fk], fr = s_f[nf1 + p.fk];
     if ((p.code >> 24) & 1) { fl = fl * Lh; fr = fr * Rh; }
     sm = sm - fl * vr + fr * vl;
   }
@@ -154,11 +163,11 @@ __global__ __launch_bounds__(256) void k_comm122p(DevBasis B, const SegTask* tas
 }
 
 void launch_comm122p(const DevBasis& B, const SegTask* t, int nt, const Partner* part, const int* rowp,
-                     const int* colp, const double* L, int Lh, const double* R, int Rh, double* RES, int RESh,
-                     hipStream_t s, int mirror) {
+                     const int* colp, const int* f1off, int nf1, const double* L, int Lh, const double* R, int Rh,
+                     double* RES, int RESh, hipStream_t s, int mirror) {
   if (nt <= 0) return;
-  hipLaunchKernelGGL(k_comm122p, dim3(nt, B.ninst), dim3(kTile), 0, s, B, t, part, rowp, colp, L, Lh, R, Rh, RES, RESh,
-                     mirror);
+  hipLaunchKernelGGL(k_comm122p, dim3(nt, B.ninst), dim3(kTile), 2 * nf1 * sizeof(double), s, B, t, part, rowp, colp,
+                     f1off, nf1, L, Lh, R, Rh, RES, RESh, mirror);
 }
 
 // ================================================== commutator_222_ph scatter

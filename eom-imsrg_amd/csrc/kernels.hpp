@@ -27,7 +27,10 @@ struct QuadItem { int ijkl; int J; };
 // its class list (bits 0-21) | class << 22 | f stored transposed << 24 |
 // negative phase << 25 | sqrt(2) factor << 26.  foff: offset of the
 // one-body element in the operator.
-struct Partner { int code; int foff; };
+// fk: compact index of the one-body element f(x, i) the term reads
+// (operator offset f1off[fk]); the kernels stage those few elements in LDS
+struct Partner { int code; int fk; };
+constexpr int kF1Max = 2048;   // compact one-body elements of the 122 partner sums
 
 // Tile of up to kTile CCX elements of cross-coupled block q1
 struct CCTask { int q1; int start; };
@@ -116,8 +119,8 @@ void launch_comm222ph_chunk(const DevBasis& B, const PhChunk* ch, int nch, const
 // dir 0: vec <- op (vectorize); dir 1: op <- vec (repackage, herm fill)
 void launch_vectorize(const VecTask* t, int nt, double* op, double* vec, int dir, int herm, hipStream_t s);
 void launch_comm122p(const DevBasis& B, const SegTask* t, int nt, const Partner* part, const int* rowp,
-                     const int* colp, const double* L, int Lh, const double* R, int Rh, double* RES, int RESh,
-                     hipStream_t s, int mirror = 1);
+                     const int* colp, const int* f1off, int nf1, const double* L, int Lh, const double* R, int Rh,
+                     double* RES, int RESh, hipStream_t s, int mirror = 1);
 void launch_pandya(const DevBasis& B, const CCTask* t, int nt, const double* op, int herm, double* CC,
                    hipStream_t s);
 void launch_comm111(const DevBasis& B, const double* L, int Lh, const double* R, int Rh, double* RES,
@@ -193,14 +196,14 @@ void launch_to_dense(const DevBasis& B, const SegTask* t, int nt, const double* 
 void launch_from_dense(const DevBasis& B, const SegTask* t, int nt, const double* Y, double alpha, double* dst,
                        hipStream_t s);
 void launch_122_vjp_1b(const DevBasis& B, const SegTask* t, int nt, const Partner* part, const int* rowp,
-                       const int* colp, const int* f1map, const int* f1off, int nf1, const double* L, int Lh,
-                       const double* R, int Rh, const double* RESb, int RESh, double* Lb, double* Rb, hipStream_t s);
+                       const int* colp, const int* f1off, int nf1, const double* L, int Lh, const double* R, int Rh,
+                       const double* RESb, int RESh, double* Lb, double* Rb, hipStream_t s);
 void launch_ph_vjp(const DevBasis& B, const QuadItem* items, int nitems, const int* lanes, int nl, const int* quads,
                    const int* qstart, const PairTask* tiles, int ntiles, const int* tstart, const int2* terms,
                    const double* RESb, int RESh, int Lh, int Rh, double* SB, double* CV, double* U, hipStream_t s);
 void launch_122_vjp_2b(const DevBasis& B, const SegTask* t, int nt, const Partner* part, const int* rowp,
-                       const int* colp, const double* L, int Lh, const double* R, int Rh, const double* cbar, int ch,
-                       double* Lb, double* Rb, hipStream_t s);
+                       const int* colp, const int* f1off, int nf1, const double* L, int Lh, const double* R, int Rh,
+                       const double* cbar, int ch, double* Lb, double* Rb, hipStream_t s);
 void launch_pandya_vjp(const DevBasis& B, const CCTask* t, int nt, const double* CCb, int herm, double* opb,
                        hipStream_t s);
 void launch_white_vjp(const DevBasis& B, const SegTask* g3, int ng3, const double* H, const double* M,

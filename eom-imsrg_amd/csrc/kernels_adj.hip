@@ -331,9 +331,17 @@ void launch_from_dense(const DevBasis& B, const SegTask* t, int nt, const double
 // square blocks through their herm (Lh for y1, Rh for y2), rectangular
 // blocks twice (their transposed partners are not stored)
 __global__ __launch_bounds__(256) void k_122_vjp_2b(DevBasis B, const SegTask* tasks, const Partner* part,
-                                                    const int* rowp, const int* colp, const double* L, int Lh,
-                                                    const double* R, int Rh, const double* cbar, int ch, double* Lb,
-                                                    double* Rb) {
+                                                    const int* rowp, const int* colp, const int* f1off, int nf1,
+                                                    const double* L, int Lh, const double* R, int Rh,
+                                                    const double* cbar, int ch, double* Lb, double* Rb) {
+  // the one-body elements the partner terms read, L then R
+  extern __shared__ double s_f[];
+  for (int k = threadIdx.x; k < nf1; k += blockDim.x) {
+    const int o = f1off[k];
+    s_f[k] = L[o];
+    s_f[nf1 + k] = R[o];
+  }
+  __syncthreads();
   const SegTask t = tasks[blockIdx.x];
   const i64 e = (i64)t.start + threadIdx.x;
   if (e >= t.cnt) return;
@@ -363,8 +371,8 @@ __global__ __launch_bounds__(256) void k_122_vjp_2b(DevBasis B, const SegTask* t
     if (cl <= cc) v = cbar[go[cl == 0 ? cc : (cl == 1 ? (cc == 1 ? 3 : 5) : 4)] + idx + (i64)JX * dm[cl]] * pre;
     else v = cbar[go[cc == 0 ? cl : (cc == 1 ? (cl == 1 ? 3 : 5) : 4)] + JX + (i64)idx * dm[cc]] * ch * pre;
     const bool fs = (p.code >> 24) & 1;
-    y1 += R[p.foff] * (fs ? Rh : 1) * v;
-    y2 += L[p.foff] * (fs ? Lh : 1) * v;
+    y1 += s_f[nf1 + p.fk] * (fs ? Rh : 1) * v;
+    y2 += s_f[p.fk] * (fs ? Lh : 1) * v;
   }
   for (int x = colp[gc]; x < colp[gc + 1]; ++x) {
     const Partner p = part[x];
@@ -376,8 +384,8 @@ __global__ __launch_bounds__(256) void k_122_vjp_2b(DevBasis B, const SegTask* t
     if (rc <= cl) v = cbar[go[rc == 0 ? cl : (rc == 1 ? (cl == 1 ? 3 : 5) : 4)] + IX + (i64)idx * dm[rc]] * pre;
     else v = cbar[go[cl == 0 ? rc : (cl == 1 ? (rc == 1 ? 3 : 5) : 4)] + idx + (i64)IX * dm[cl]] * ch * pre;
     const bool fs = (p.code >> 24) & 1;
-    y1 -= R[p.foff] * (fs ? Rh : 1) * v;
-    y2 -= L[p.foff] * (fs ? Lh : 1) * v;
+    y1 -= s_f[nf1 + p.fk] * (fs ? Rh : 1) * v;
+    y2 -= s_f[p.fk] * (fs ? Lh : 1) * v;
   }
   const double nrm = 1.0 / (sqrt(1.0 + (a == b ? 1.0 : 0.0)) * sqrt(1.0 + (c == d ? 1.0 : 0.0)));
   y1 *= nrm;
@@ -399,20 +407,20 @@ __global__ __launch_bounds__(256) void k_122_vjp_2b(DevBasis B, const SegTask* t
 }
 
 void launch_122_vjp_2b(const DevBasis& B, const SegTask* t, int nt, const Partner* part, const int* rowp,
-                       const int* colp, const double* L, int Lh, const double* R, int Rh, const double* cbar, int ch,
-                       double* Lb, double* Rb, hipStream_t s) {
+                       const int* colp, const int* f1off, int nf1, const double* L, int Lh, const double* R, int Rh,
+                       const double* cbar, int ch, double* Lb, double* Rb, hipStream_t s) {
   if (nt <= 0) return;
-  hipLaunchKernelGGL(k_122_vjp_2b, dim3(nt), dim3(kTile), 0, s, B, t, part, rowp, colp, L, Lh, R, Rh, cbar, ch, Lb,
-                     Rb);
+  hipLaunchKernelGGL(k_122_vjp_2b, dim3(nt), dim3(kTile), 2 * nf1 * sizeof(double), s, B, t, part, rowp, colp, f1off,
+                     nf1, L, Lh, R, Rh, cbar, ch, Lb, Rb);
 }
 
 // one-body part of the 122 transpose: the partner sums of k_comm122p with
 // the cotangent of each upper element; contributions to the few one-body
-// elements are summed per workgroup in LDS (compact index f1map[foff])
+// elements are summed per workgroup in LDS (compact index fk)
 __global__ __launch_bounds__(256) void k_122_vjp_1b(DevBasis B, const SegTask* tasks, int ntasks,
                                                     const Partner* part, const int* rowp, const int* colp,
-                                                    const int* f1map, const int* f1off, int nf1, const double* L,
-                                                    int Lh, const double* R, int Rh, const double* RESb, int RESh,
+                                                    const int* f1off, int nf1, const double* L, int Lh,
+                                                    const double* R, int Rh, const double* RESb, int RESh,
                                                     double* Lb, double* Rb) {
   extern __shared__ double sh[];
   double* lbar = sh;
@@ -455,7 +463,7 @@ __global__ __launch_bounds__(256) void k_122_vjp_1b(DevBasis B, const SegTask* t
         vr = R[o] * Rh * pre; vl = L[o] * Lh * pre;
       }
       const bool fs = (p.code >> 24) & 1;
-      const int k = f1map[p.foff];
+      const int k = p.fk;
       unsafeAtomicAdd(lbar + k, sb * (fs ? Lh : 1) * vr);
       unsafeAtomicAdd(rbar + k, -sb * (fs ? Rh : 1) * vl);
     }
@@ -474,7 +482,7 @@ __global__ __launch_bounds__(256) void k_122_vjp_1b(DevBasis B, const SegTask* t
         vr = R[o] * Rh * pre; vl = L[o] * Lh * pre;
       }
       const bool fs = (p.code >> 24) & 1;
-      const int k = f1map[p.foff];
+      const int k = p.fk;
       unsafeAtomicAdd(lbar + k, -sb * (fs ? Lh : 1) * vr);
       unsafeAtomicAdd(rbar + k, sb * (fs ? Rh : 1) * vl);
     }
@@ -487,12 +495,12 @@ __global__ __launch_bounds__(256) void k_122_vjp_1b(DevBasis B, const SegTask* t
 }
 
 void launch_122_vjp_1b(const DevBasis& B, const SegTask* t, int nt, const Partner* part, const int* rowp,
-                       const int* colp, const int* f1map, const int* f1off, int nf1, const double* L, int Lh,
-                       const double* R, int Rh, const double* RESb, int RESh, double* Lb, double* Rb, hipStream_t s) {
+                       const int* colp, const int* f1off, int nf1, const double* L, int Lh, const double* R, int Rh,
+                       const double* RESb, int RESh, double* Lb, double* Rb, hipStream_t s) {
   if (nt <= 0 || nf1 <= 0) return;
   const int grid = nt < 2048 ? nt : 2048;
   hipLaunchKernelGGL(k_122_vjp_1b, dim3(grid), dim3(256), 2 * nf1 * sizeof(double), s, B, t, nt, part, rowp, colp,
-                     f1map, f1off, nf1, L, Lh, R, Rh, RESb, RESh, Lb, Rb);
+                     f1off, nf1, L, Lh, R, Rh, RESb, RESh, Lb, Rb);
 }
 
 // -------------------------------------------------- 222_ph transpose

@@ -9,6 +9,11 @@ from __future__ import annotations
 import ctypes as C
 import os
 
+# torch first: its HIP runtime is then the one libimsrg_hip.so binds to
+# (same SONAME), so the library and torch's tensors and streams share one
+# runtime whatever the caller imports first
+import torch  # noqa: F401
+
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libimsrg_hip.so")
 
