@@ -510,6 +510,7 @@ int build_plan(imsrg_basis_s* B) {
         tri += kSquare[g] ? (i64)b.rows(q, g) * (b.rows(q, g) + 1) / 2 : b.blocksize(q, g);
     if ((i64)B->quad_h.size() != tri) return fail(IMSRG_EINVAL, "quadruple enumeration does not cover the channels");
     if (b.wclen >= ((i64)1 << 32)) return fail(IMSRG_EINVAL, "WCC exceeds 2^32 doubles");
+    if (b.ncc > kPhMaxCC) return fail(IMSRG_EINVAL, "more cross-coupled blocks than the 222_ph chunk kernel holds");
     // chunks: consecutive quadruples with at most kPhChunkItems items and
     // kPhChunkLanes J' terms (the two recouplings' ranges, none when an
     // isospin / parity check fails: the rules of ph_ranges)

@@ -196,8 +196,8 @@ __device__ __forceinline__ double sixj_tri(const DevBasis& B, int j1, int j2, in
 //   outside: + (W'(gil,rjk) + phase W'(gjk,ril)) {ji jj J; jk jl J'}
 // ph_pair gives the W' combination of one J' term.
 template <bool INSIDE>
-__device__ __forceinline__ double ph_pair(const DevBasis& B, const double* WCC, double mpm, int phase, int i, int j,
-                                          int k, int l, int JP, int Tz, int PAR) {
+__device__ __forceinline__ double ph_pair(const DevBasis& B, const i64* wcoff, const int* rlen, const double* WCC,
+                                          double mpm, int phase, int i, int j, int k, int l, int JP, int Tz, int PAR) {
   // inside reads rows (k,i) / cols (j,l); outside rows (l,i) / cols (j,k).
   // 32-bit element offsets from uniform bases (WCC holds < 2^32 doubles,
   // checked at plan time)
@@ -205,7 +205,7 @@ __device__ __forceinline__ double ph_pair(const DevBasis& B, const double* WCC, 
   const int a = INSIDE ? k : l, c = INSIDE ? l : k;
   const int qx = JP / 2 + Tz * (JTM + 1) + 2 * PAR * (JTM + 1);
   const unsigned t = (unsigned)(JP >> 1) * (unsigned)(N * N);
-  const unsigned wo = (unsigned)B.wcoff[qx], rr = (unsigned)B.rlen[qx];
+  const unsigned wo = (unsigned)wcoff[qx], rr = (unsigned)rlen[qx];
   const unsigned x1 = B.rmapJ[t + a * N + i];    // gik | gil
   const unsigned y1 = B.rmapJ[t + j * N + c];    // rjl | rjk
   const unsigned x2 = B.rmapJT[t + a * N + i];   // rik | ril
@@ -222,7 +222,7 @@ __device__ __forceinline__ double ph_jsum(const DevBasis& B, const double* WCC, 
   const int ji = B.jj[i], jj = B.jj[j], jk = B.jj[k], jl = B.jj[l];
   const double sg = INSIDE ? -(double)m1pow((jl + jk + Jtot) / 2) : 1.0;
   for (int JP = jmin; JP <= jmax; JP += 2) {
-    const double v = ph_pair<INSIDE>(B, WCC, mpm, phase, i, j, k, l, JP, Tz, PAR);
+    const double v = ph_pair<INSIDE>(B, B.wcoff, B.rlen, WCC, mpm, phase, i, j, k, l, JP, Tz, PAR);
     sm = sm + sg * v * (INSIDE ? sixj_tri(B, ji, jj, Jtot, jl, jk, JP) : sixj_tri(B, ji, jj, Jtot, jk, jl, JP));
   }
   return sm;
@@ -336,9 +336,15 @@ __global__ __launch_bounds__(256) void k_comm222ph_chunk(DevBasis B, const PhChu
   __shared__ int s_ijkl[kPhChunkItems], s_loff[kPhChunkItems], s_ioff[kPhChunkItems];
   __shared__ int s_jin[kPhChunkItems], s_nin[kPhChunkItems], s_jout[kPhChunkItems];
   __shared__ double s_val[kPhChunkLanes];
+  __shared__ i64 s_wcoff[kPhMaxCC];
+  __shared__ int s_rlen[kPhMaxCC];
   const PhChunk c = chunks[blockIdx.x];
   const int tid = threadIdx.x;
   const double mpm = -(double)(Lh * Rh);
+  for (int x = tid; x < B.ncc; x += blockDim.x) {   // cross-coupled block offsets and row counts
+    s_wcoff[x] = B.wcoff[x];
+    s_rlen[x] = B.rlen[x];
+  }
   if (tid < c.nq) {
     const int u = fq[c.q0 + tid];
     const int i = u & 255, j = (u >> 8) & 255, k = (u >> 16) & 255, l = (u >> 24) & 255;
@@ -359,10 +365,10 @@ __global__ __launch_bounds__(256) void k_comm222ph_chunk(DevBasis B, const PhChu
     const int phase = m1pow((B.jj[i] + B.jj[j] + B.jj[k] + B.jj[l]) / 2);
     double v;
     if (r < s_nin[q])
-      v = ph_pair<true>(B, WCC, mpm, phase, i, j, k, l, s_jin[q] + 2 * r, abs(B.tz[i] - B.tz[k]) / 2,
-                        (B.ll[i] + B.ll[k]) % 2);
+      v = ph_pair<true>(B, s_wcoff, s_rlen, WCC, mpm, phase, i, j, k, l, s_jin[q] + 2 * r,
+                        abs(B.tz[i] - B.tz[k]) / 2, (B.ll[i] + B.ll[k]) % 2);
     else
-      v = ph_pair<false>(B, WCC, mpm, phase, i, j, k, l, s_jout[q] + 2 * (r - s_nin[q]),
+      v = ph_pair<false>(B, s_wcoff, s_rlen, WCC, mpm, phase, i, j, k, l, s_jout[q] + 2 * (r - s_nin[q]),
                          abs(B.tz[i] - B.tz[l]) / 2, (B.ll[i] + B.ll[l]) % 2);
     s_val[t] = v;
   }

@@ -71,6 +71,7 @@ struct PairTask { int q; int ti; int tj; int pad; };
 // their J' terms, at most kPhChunkLanes, are gathered once into LDS)
 constexpr int kPhChunkItems = 256;
 constexpr int kPhChunkLanes = 1024;
+constexpr int kPhMaxCC = 256;   // cross-coupled blocks whose offsets the chunk kernel keeps in LDS
 struct PhChunk { int q0, nq, item0, nitems, nlanes, pad0, pad1, pad2; };
 
 // One contiguous run of the vectorize stream (basic_IMSRG.f90:3929-3981):
