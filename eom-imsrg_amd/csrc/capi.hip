@@ -46,7 +46,31 @@ struct GemmFamily {
   TaskList<GemmTask> tasks;
   GemmTask* tasks_alloc = nullptr;
   bool xcd_aware = false;         // pack groups onto XCDs (else one largest-first list)
+  int sig = -1;                   // shared transposes of the problems (launch_gemm_tiled), -1 mixed
 };
+
+// the transposes every problem of a family shares, as launch_gemm_tiled's
+// sig, or -1 when they differ or no kernel is specialised for them
+int family_sig(const std::vector<GemmProb>& probs) {
+  int sig = -2;
+  for (const GemmProb& p : probs) {
+    const int s = (p.ta1 ? 1 : 0) | (p.tb1 ? 2 : 0) | (p.nprod == 2 ? ((p.ta2 ? 4 : 0) | (p.tb2 ? 8 : 0)) : -1);
+    if (s < 0) {   // one product: any second-product bits are compatible
+      const int s1 = (p.ta1 ? 1 : 0) | (p.tb1 ? 2 : 0);
+      if (sig == -2) sig = s1 | 256;   // 256: second product not yet fixed
+      else if ((sig & 3) != s1) return -1;
+      continue;
+    }
+    if (sig == -2) sig = s;
+    else if (sig & 256) {
+      if ((sig & 3) != (s & 3)) return -1;
+      sig = s;
+    } else if (sig != s) return -1;
+  }
+  if (sig == -2) return -1;
+  sig &= 255;
+  return gemm_sig_supported(sig) ? sig : -1;
+}
 
 struct imsrg_basis_s {
   Basis b;
@@ -198,6 +222,7 @@ int tiles(int n) { return (n + 63) / 64; }
 // path and the grouped launch keeps each group inside one L2.
 int family_tasks(imsrg_basis_s* B, GemmFamily& f, int tile) {
   f.tile = tile;
+  f.sig = family_sig(f.probs_h);
   ++B->plan_ver;
   const int bm = tile_rows(tile), bn = tile_cols(tile);
   const int ng = (int)f.group_end.size();
@@ -680,13 +705,13 @@ void run_gemm(imsrg_basis_s* B, GemmFamily& f, const Slots& sl, hipStream_t s, d
               double s2scale = 1.0) {
   if (f.tasks.n == 0) return;
   if (B->path == IMSRG_PATH_GROUPED) {
-    launch_gemm_tiled(f.tile, sl, f.probs, f.tasks.dev, f.tasks.grid, f.tasks.xmap, s, ascale, s2scale);
+    launch_gemm_tiled(f.tile, sl, f.probs, f.tasks.dev, f.tasks.grid, f.tasks.xmap, s, ascale, s2scale, 1, f.sig);
     return;
   }
   const XcdMap ident{{0}, {-1}};
   for (size_t g = 0; g + 1 < f.tasks.chan_begin.size(); ++g) {
     const int b0 = f.tasks.chan_begin[g], e0 = f.tasks.chan_begin[g + 1];
-    if (e0 > b0) launch_gemm_tiled(f.tile, sl, f.probs, f.tasks.dev + b0, e0 - b0, ident, s, ascale, s2scale);
+    if (e0 > b0) launch_gemm_tiled(f.tile, sl, f.probs, f.tasks.dev + b0, e0 - b0, ident, s, ascale, s2scale, 1, f.sig);
   }
 }
 
@@ -716,7 +741,8 @@ void run_gemm_x(const Exec& X, GemmFamily& f, Slots sl, hipStream_t s, double as
   sl.act = X.D.act;
   if (X.D.ninst > 1 && X.B->path == IMSRG_PATH_GROUPED) {
     if (f.tasks.n)
-      launch_gemm_tiled(f.tile, sl, f.probs, f.tasks.dev, f.tasks.grid, f.tasks.xmap, s, ascale, s2scale, X.D.ninst);
+      launch_gemm_tiled(f.tile, sl, f.probs, f.tasks.dev, f.tasks.grid, f.tasks.xmap, s, ascale, s2scale, X.D.ninst,
+                        f.sig);
     return;
   }
   run_gemm(X.B, f, sl, s, ascale, s2scale);
@@ -1010,7 +1036,7 @@ void comm_vjp(imsrg_basis_s* B, const double* L, int Lh, const double* R, int Rh
   sp.p[0] = L; sp.p[1] = R; sp.p[2] = Lb; sp.p[3] = Rb; sp.p[4] = B->Wb1; sp.p[5] = B->Wb2;
   for (GemmFamily& f : B->pb)
     if (f.tasks.n) launch_gemm_tiled(f.tile, sp, f.probs, f.tasks.dev, f.tasks.grid, f.tasks.xmap, s, (double)Lh,
-                                     (double)Rh);
+                                     (double)Rh, 1, f.sig);
   // 122: two-body parts from the symmetrised cotangent, one-body part by
   // the partner sums
   launch_fill(B->cbar, b.off_fpp + (i64)b.P * b.P, 0.0, s);
@@ -1036,7 +1062,7 @@ void comm_vjp(imsrg_basis_s* B, const double* L, int Lh, const double* R, int Rh
   sc.p[0] = LCC; sc.p[1] = B->aRCC; sc.p[2] = LCCb; sc.p[3] = B->aRCCb; sc.p[4] = B->U;
   if (B->phb.tasks.n)
     launch_gemm_tiled(B->phb.tile, sc, B->phb.probs, B->phb.tasks.dev, B->phb.tasks.grid, B->phb.tasks.xmap, s,
-                      (double)Lh, -(double)(Rh * Lh));
+                      (double)Lh, -(double)(Rh * Lh), 1, B->phb.sig);
   launch_pandya_vjp(D, B->cc.dev, B->cc.n, B->aRCCb, Rh, Rb, s);
   if (!LCCb_acc) launch_pandya_vjp(D, B->cc.dev, B->cc.n, B->aLCCb, Lh, Lb, s);
 }

@@ -133,7 +133,23 @@ __device__ __forceinline__ void mma(v4d* acc, double (*As)[GBK][TBM + 1], double
   else mma_t<TBM, TBN, true, true>(acc, As, Bs, A, lda, Bm, ldb, M, N, k0s, k1, m0, n0, scale);
 }
 
-template <int TBM, int TBN>
+// product P (1 or 2) of a kernel specialised on the transposes of its
+// family (SIG bit0 TA1, bit1 TB1, bit2 TA2, bit3 TB2), or with the
+// problem's own transposes chosen at run time (SIG < 0).  A specialised
+// kernel inlines one main loop per product instead of four.
+template <int TBM, int TBN, int SIG, int P>
+__device__ __forceinline__ void mma_p(v4d* acc, double (*As)[GBK][TBM + 1], double (*Bs)[GBK][TBN + 1],
+                                      const double* A, int lda, int tA, const double* Bm, int ldb, int tB, int M, int N,
+                                      int k0s, int k1, int m0, int n0, double scale) {
+  if constexpr (SIG < 0) {
+    mma<TBM, TBN>(acc, As, Bs, A, lda, tA, Bm, ldb, tB, M, N, k0s, k1, m0, n0, scale);
+  } else {
+    constexpr bool TA = (SIG >> (P == 1 ? 0 : 2)) & 1, TB = (SIG >> (P == 1 ? 1 : 3)) & 1;
+    mma_t<TBM, TBN, TA, TB>(acc, As, Bs, A, lda, Bm, ldb, M, N, k0s, k1, m0, n0, scale);
+  }
+}
+
+template <int TBM, int TBN, int SIG>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TBM == 64 ? 4 : (TBN == 64 ? 2 : 1)))) void k_gemm_t(Slots sl, const GemmProb* probs, const GemmTask* tasks, XcdMap xm,
                                                 double ascale, double s2scale) {
   constexpr int MI = TBM / 32, NI = TBN / 32;
@@ -154,18 +170,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TBM == 64 ?
   // mode bit3: A is upper triangular (as stored); rows m0.. of op(A) reach
   // only k >= m0 (A as stored) or k < m0 + TBM (A transposed)
   const bool tri = pr.mode & 8;
-  mma<TBM, TBN>(acc, As, Bs, slot(sl, pr.sa1, inst) + pr.a1, pr.lda, pr.ta1, slot(sl, pr.sb1, inst) + pr.b1, pr.ldb, pr.tb1, pr.m, pr.n,
-                (tri && !pr.ta1) ? max(kb, m0) : kb, (tri && pr.ta1) ? min(ke, m0 + TBM) : ke, m0, n0, 1.0);
+  mma_p<TBM, TBN, SIG, 1>(acc, As, Bs, slot(sl, pr.sa1, inst) + pr.a1, pr.lda, pr.ta1, slot(sl, pr.sb1, inst) + pr.b1,
+                          pr.ldb, pr.tb1, pr.m, pr.n, (tri && !pr.ta1) ? max(kb, m0) : kb,
+                          (tri && pr.ta1) ? min(ke, m0 + TBM) : ke, m0, n0, 1.0);
   if (pr.nprod == 2) {
     // second product: same K range, or its own K (k2 > 0) with its own
     // leading dimensions packed in a2/b2's problem (lda2 = ldb2 = m here)
     if (pr.k2 > 0)
-      mma<TBM, TBN>(acc, As, Bs, slot(sl, pr.sa2, inst) + pr.a2, pr.m, pr.ta2, slot(sl, pr.sb2, inst) + pr.b2, pr.n, pr.tb2, pr.m, pr.n,
-                    0, pr.k2, m0, n0, pr.s2 * s2scale);
+      mma_p<TBM, TBN, SIG, 2>(acc, As, Bs, slot(sl, pr.sa2, inst) + pr.a2, pr.m, pr.ta2, slot(sl, pr.sb2, inst) + pr.b2,
+                              pr.n, pr.tb2, pr.m, pr.n, 0, pr.k2, m0, n0, pr.s2 * s2scale);
     else
-      mma<TBM, TBN>(acc, As, Bs, slot(sl, pr.sa2, inst) + pr.a2, pr.lda, pr.ta2, slot(sl, pr.sb2, inst) + pr.b2, pr.ldb, pr.tb2, pr.m,
-                    pr.n, (tri && !pr.ta2) ? max(kb, m0) : kb, (tri && pr.ta2) ? min(ke, m0 + TBM) : ke, m0, n0,
-                    pr.s2 * ((pr.mode == 0 || (pr.mode & 16)) ? s2scale : 1.0));
+      mma_p<TBM, TBN, SIG, 2>(acc, As, Bs, slot(sl, pr.sa2, inst) + pr.a2, pr.lda, pr.ta2, slot(sl, pr.sb2, inst) + pr.b2,
+                              pr.ldb, pr.tb2, pr.m, pr.n, (tri && !pr.ta2) ? max(kb, m0) : kb,
+                              (tri && pr.ta2) ? min(ke, m0 + TBM) : ke, m0, n0,
+                              pr.s2 * ((pr.mode == 0 || (pr.mode & 16)) ? s2scale : 1.0));
   }
   double* C = const_cast<double*>(slot(sl, pr.sc, inst)) + pr.c + (i64)t.ks * pr.cstride;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -193,19 +211,43 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TBM == 64 ?
 int tile_rows(int tile) { return tile == 0 ? 64 : 128; }
 int tile_cols(int tile) { return tile == 2 ? 128 : 64; }
 
-void launch_gemm_tiled(int tile, const Slots& sl, const GemmProb* probs, const GemmTask* tasks, int grid,
-                       const XcdMap& xm, hipStream_t s, double ascale, double s2scale, int ninst) {
-  if (grid <= 0) return;
-  switch (tile) {
+template <int TBM, int TBN>
+static void launch_sig(int sig, int grid, int ninst, const Slots& sl, const GemmProb* probs, const GemmTask* tasks,
+                const XcdMap& xm, hipStream_t s, double ascale, double s2scale) {
+  switch (sig) {
     case 0:
-      hipLaunchKernelGGL((k_gemm_t<64, 64>), dim3(grid, ninst), dim3(256), 0, s, sl, probs, tasks, xm, ascale, s2scale);
+      hipLaunchKernelGGL((k_gemm_t<TBM, TBN, 0>), dim3(grid, ninst), dim3(256), 0, s, sl, probs, tasks, xm, ascale,
+                         s2scale);
       break;
     case 1:
-      hipLaunchKernelGGL((k_gemm_t<128, 64>), dim3(grid, ninst), dim3(256), 0, s, sl, probs, tasks, xm, ascale, s2scale);
+      hipLaunchKernelGGL((k_gemm_t<TBM, TBN, 1>), dim3(grid, ninst), dim3(256), 0, s, sl, probs, tasks, xm, ascale,
+                         s2scale);
+      break;
+    case 8:
+      hipLaunchKernelGGL((k_gemm_t<TBM, TBN, 8>), dim3(grid, ninst), dim3(256), 0, s, sl, probs, tasks, xm, ascale,
+                         s2scale);
+      break;
+    case 10:
+      hipLaunchKernelGGL((k_gemm_t<TBM, TBN, 10>), dim3(grid, ninst), dim3(256), 0, s, sl, probs, tasks, xm, ascale,
+                         s2scale);
       break;
     default:
-      hipLaunchKernelGGL((k_gemm_t<128, 128>), dim3(grid, ninst), dim3(256), 0, s, sl, probs, tasks, xm, ascale, s2scale);
+      hipLaunchKernelGGL((k_gemm_t<TBM, TBN, -1>), dim3(grid, ninst), dim3(256), 0, s, sl, probs, tasks, xm, ascale,
+                         s2scale);
       break;
+  }
+}
+
+int gemm_sig_supported(int sig) { return sig == 0 || sig == 1 || sig == 8 || sig == 10; }
+
+void launch_gemm_tiled(int tile, const Slots& sl, const GemmProb* probs, const GemmTask* tasks, int grid,
+                       const XcdMap& xm, hipStream_t s, double ascale, double s2scale, int ninst, int sig) {
+  if (grid <= 0) return;
+  if (!gemm_sig_supported(sig)) sig = -1;
+  switch (tile) {
+    case 0: launch_sig<64, 64>(sig, grid, ninst, sl, probs, tasks, xm, s, ascale, s2scale); break;
+    case 1: launch_sig<128, 64>(sig, grid, ninst, sl, probs, tasks, xm, s, ascale, s2scale); break;
+    default: launch_sig<128, 128>(sig, grid, ninst, sl, probs, tasks, xm, s, ascale, s2scale); break;
   }
 }
 

@@ -138,8 +138,13 @@ void launch_e0_final(const double* part, int nblocks, int use220, int use110, do
 // alpha and s2 of every problem are multiplied by ascale / s2scale.
 int tile_rows(int tile);
 int tile_cols(int tile);
+// sig: the family's transposes (bit0 TA1, bit1 TB1, bit2 TA2, bit3 TB2) when
+// every problem shares them and a kernel is specialised for them
+// (gemm_sig_supported), else -1 (each problem's own, chosen at run time)
 void launch_gemm_tiled(int tile, const Slots& sl, const GemmProb* probs, const GemmTask* tasks, int grid,
-                       const XcdMap& xm, hipStream_t s, double ascale = 1.0, double s2scale = 1.0, int ninst = 1);
+                       const XcdMap& xm, hipStream_t s, double ascale = 1.0, double s2scale = 1.0, int ninst = 1,
+                       int sig = -1);
+int gemm_sig_supported(int sig);
 // RES1 += C - pm C^T with C = L1 R1 - Rh L3 R3^T held in w1's pppp blocks;
 // ref_outputs: also leave the reference's W1 = L1 R1 and W2 = Rh L3 R3^T
 // in w1 / w2 (otherwise w1's pppp blocks keep C and w2's are untouched)
