@@ -87,3 +87,17 @@ def test_flow_vs_oracle(gpu, name):
     assert np.array_equal(log[:, 3], logr[:, 3])
     assert np.max(np.abs(log[:, 1] - logr[:, 1])) < 1e-8
     assert abs(HS.E0 - HSr[0]) < 1e-8
+
+
+def test_flow_gradient_vs_oracle_c14(gpu):
+    """dE0/dT of the whole 14C flow (Omega checkpointed every 4 steps)
+    against the oracle's reverse sweep at eMax 4, 1e-8 relative"""
+    from eom_imsrg import Flow, adjoint, synth
+    b, o = _bases("C14", 4)
+    print("[nucleus flow_grad] C14 emax=4", flush=True)
+    T = synth.hamiltonian_vacuum(b, 0)
+    e0_o, Tb_o, st_o = o.flow_grad(T, 0.5, REF, REF, REF, ckpt=4)
+    e0, Tb, st = adjoint.flow_grad(Flow(b, graphs=True), torch.from_numpy(T).cuda(), ckpt=4)
+    assert st == st_o and abs(e0 - e0_o) < 1e-8
+    assert_rel(adjoint.symmetric_gradient(b, Tb.cpu().numpy(), 1), adjoint.symmetric_gradient(b, Tb_o, 1), rtol=1e-8,
+               what="dE0/dT 14C")
