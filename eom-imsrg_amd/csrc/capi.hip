@@ -1108,6 +1108,7 @@ struct imsrg_flow_s {
   int last_n0 = -1;                    // first step of that segment (-1: none kept)
   double* decDG = nullptr;   // generator buffer of imsrg_magnus_decouple
   double* tOp = nullptr;     // result buffer of imsrg_transform_sq_op_bch, generator of imsrg_dhds
+  double *vE1 = nullptr, *vH2 = nullptr, *vE1b = nullptr, *vE2b = nullptr, *vH2b = nullptr;   // w2 transpose
   bool fw_ok = false;        // a checkpointed forward sweep is held
   int fw_steps = 0, fw_ckpt = 1;
   double fw_ds = 0.5, fw_bch = 1e-6, fw_mag = 1e-6;
@@ -1682,15 +1683,56 @@ int magnus_vjp(imsrg_flow_s* F, const double* ETA, const double* G, double conv,
   return magnus_reverse(F, G, rec, DGb, ETAb, Gb, s);
 }
 
-// build_gs_white transpose (generators.f90:4-106): Hb += d<ETAb, white(H)>/dH
-int white_vjp(imsrg_basis_s* B, const double* H, const double* ETAb, double* Hb, hipStream_t s) {
+// generator transpose with the denominators of Hd and the numerators of Hn
+// (mode 0 white, 1 atan / atanxx): Hdb, Hnb += d<ETAb, gen(Hd, Hn)>/d(Hd, Hn)
+int gen_vjp(imsrg_basis_s* B, const double* Hd, const double* Hn, const double* ETAb, double* Hdb, double* Hnb,
+            int mode, hipStream_t s) {
   int rc;
   if ((rc = ensure_adj(B))) return rc;
-  launch_monopole(B->D, H, B->M, s);
+  launch_monopole(B->D, Hd, B->M, s);
   launch_fill(B->Mb, (i64)B->b.N * B->b.N, 0.0, s);
-  launch_white_vjp(B->D, B->seg_g3.dev, B->seg_g3.n, H, B->M, ETAb, Hb, B->Mb, s);
+  launch_gen_vjp(B->D, B->seg_g3.dev, B->seg_g3.n, Hd, Hn, B->M, ETAb, Hdb, Hnb, B->Mb, mode, s);
   HIPCHK(hipGetLastError());
   return 0;
+}
+
+// build_gs_white transpose (generators.f90:4-106): Hb += d<ETAb, white(H)>/dH
+int white_vjp(imsrg_basis_s* B, const double* H, const double* ETAb, double* Hb, hipStream_t s) {
+  return gen_vjp(B, H, H, ETAb, Hb, Hb, 0, s);
+}
+
+// build_gs_w2 transpose (generators.f90:109-160): ETA = 1.5 eta1 + 0.5 eta2
+// with eta1 = atan(H), H2 = [eta1, H], eta2 = atanxx(H, H2); eta1 and H2 are
+// recomputed (on the flow's buffers), then eta2 -> (H, H2), H2 -> (eta1, H)
+// through the commutator transpose, eta1 -> H.  Hb accumulates.
+int w2_vjp(imsrg_flow_s* F, const double* H, const double* ETAb, double* Hb, hipStream_t s) {
+  imsrg_basis_s* B = F->B;
+  const i64 n = B->b.oplen;
+  int rc;
+  for (double** p : {&F->vE1, &F->vH2, &F->vE1b, &F->vE2b, &F->vH2b})
+    if ((rc = fbuf(F, p, n))) return rc;
+  if ((rc = ensure_adj(B))) return rc;
+  launch_fill(F->vE1, n, 0.0, s);
+  if ((rc = gen(F->X, H, H, F->vE1, 1, s))) return rc;
+  t_pandya(F->X, F->vE1, -1, F->GCC, s);
+  t_full(F->X, F->vE1, -1, H, 1, F->vH2, 1, true, F->w1, F->w2, F->GCC, F->ADCC, F->WCC, s);
+  launch_axpy(n, ETAb, 1.5, ETAb, 0.0, F->vE1b, s);
+  launch_axpy(n, ETAb, 0.5, ETAb, 0.0, F->vE2b, s);
+  launch_fill(F->vH2b, n, 0.0, s);
+  if ((rc = gen_vjp(B, H, F->vH2, F->vE2b, Hb, F->vH2b, 1, s))) return rc;
+  comm_vjp(B, F->vE1, -1, H, 1, F->vH2b, 1, F->vE1b, Hb, true, nullptr, nullptr, s);
+  if ((rc = gen_vjp(B, H, H, F->vE1b, Hb, Hb, 1, s))) return rc;
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// transpose of the flow's generator
+int generator_vjp(imsrg_flow_s* F, const double* H, const double* ETAb, double* Hb, hipStream_t s) {
+  switch (F->generator) {
+    case IMSRG_GEN_ATAN: return gen_vjp(F->B, H, H, ETAb, Hb, Hb, 1, s);
+    case IMSRG_GEN_W2: return w2_vjp(F, H, ETAb, Hb, s);
+    default: return white_vjp(F->B, H, ETAb, Hb, s);
+  }
 }
 
 // normal_order transpose (basic_IMSRG.f90:2419-2550): Tb = d<Hb, NO(T)>/dT
@@ -1701,9 +1743,9 @@ int normal_order_vjp(imsrg_basis_s* B, const double* Hb, double* Tb, hipStream_t
   return 0;
 }
 
-// Forward sweep of the flow gradient: magnus_decouple (white generator,
-// |mbpt2| > crit_tol loop) from H = normal_order(T), keeping Omega every
-// `ckpt` steps in the flow object for flow_backward.
+// Forward sweep of the flow gradient: magnus_decouple (the flow's
+// generator, |mbpt2| > crit_tol loop) from H = normal_order(T), keeping
+// Omega every `ckpt` steps in the flow object for flow_backward.
 int flow_forward(imsrg_flow_s* F, const double* T, double ds, double bch_conv, double mag_conv, double crit_tol,
                  int maxsteps, int ckpt, double* E0, int* nsteps, hipStream_t s) {
   imsrg_basis_s* B = F->B;
@@ -1711,7 +1753,6 @@ int flow_forward(imsrg_flow_s* F, const double* T, double ds, double bch_conv, d
   const size_t nb = (size_t)n * sizeof(double);
   int rc, nt = 0, nc = 0;
   if (ckpt < 1) return fail(IMSRG_EINVAL, "checkpoint interval must be >= 1");
-  if (F->generator != IMSRG_GEN_WHITE) return fail(IMSRG_EINVAL, "the flow gradient is built for the white generator");
   if ((rc = ensure_adj(B))) return rc;
   for (double** p : {&F->gH, &F->gHS, &F->gG, &F->gDG, &F->gHSb, &F->gGb, &F->gHb, &F->gdOb, &F->getab})
     if ((rc = fbuf(F, p, n))) return rc;
@@ -1727,7 +1768,7 @@ int flow_forward(imsrg_flow_s* F, const double* T, double ds, double bch_conv, d
   HIPCHK(hipMemcpyAsync(F->gHS, F->gH, nb, hipMemcpyDeviceToDevice, s));
   launch_fill(F->gG, n, 0.0, s);
   launch_fill(F->gDG, n, 0.0, s);
-  if ((rc = white(F->X, F->gHS, F->gDG, s)) || (rc = mbpt2_dev(F->X, F->gHS, B->dval + 2, s))) return rc;
+  if ((rc = generator(F, F->X, F->gHS, F->gDG, s)) || (rc = mbpt2_dev(F->X, F->gHS, B->dval + 2, s))) return rc;
   double crit = 0.0;
   if ((rc = read_scalar(B, B->dval + 2, &crit, s))) return rc;
   crit = std::fabs(crit);
@@ -1752,7 +1793,7 @@ int flow_forward(imsrg_flow_s* F, const double* T, double ds, double bch_conv, d
     if ((rc = magnus(F, F->gDG, -1, F->gG, -1, mag_conv, &nc, s, &F->lmrec[k], &pidx, F->segE[k]))) return rc;
     launch_axpy(n, F->gG, 1.0, F->gDG, ds, F->gG, s);
     if ((rc = bch(F, F->gHS, F->gG, -1, F->gH, 1, bch_conv, 40, &nt, nullptr, s, &F->lbrec[k], &pidx))) return rc;
-    if ((rc = white(F->X, F->gHS, F->gDG, s)) || (rc = mbpt2_dev(F->X, F->gHS, B->dval + 2, s))) return rc;
+    if ((rc = generator(F, F->X, F->gHS, F->gDG, s)) || (rc = mbpt2_dev(F->X, F->gHS, B->dval + 2, s))) return rc;
     if ((rc = read_scalar(B, B->dval + 2, &crit, s))) return rc;
     crit = std::fabs(crit);
     ++steps;
@@ -1799,7 +1840,7 @@ int flow_backward(imsrg_flow_s* F, double* Tb, hipStream_t s) {
       HIPCHK(hipMemcpyAsync(F->segG[0], F->ckpts[seg], n * sizeof(double), hipMemcpyDeviceToDevice, s));
       HIPCHK(hipMemcpyAsync(F->segH[0], F->hckpts[seg], n * sizeof(double), hipMemcpyDeviceToDevice, s));
       launch_fill(F->segE[0], n, 0.0, s);
-      if ((rc = white(F->X, F->segH[0], F->segE[0], s))) return rc;
+      if ((rc = generator(F, F->X, F->segH[0], F->segE[0], s))) return rc;
       size_t pidx = 0;
       for (int m = n0; m < n1; ++m) {
         const int k = m - n0;
@@ -1808,7 +1849,7 @@ int flow_backward(imsrg_flow_s* F, double* Tb, hipStream_t s) {
         if ((rc = bch_record(F, F->segG[k + 1], F->gH, 1, bch_conv, 40, brec[k], pidx, F->segH[k + 1], &nt, s)))
           return rc;
         launch_fill(F->segE[k + 1], n, 0.0, s);
-        if ((rc = white(F->X, F->segH[k + 1], F->segE[k + 1], s))) return rc;
+        if ((rc = generator(F, F->X, F->segH[k + 1], F->segE[k + 1], s))) return rc;
       }
     }
     for (int m = n1 - 1; m >= n0; --m) {
@@ -1818,7 +1859,7 @@ int flow_backward(imsrg_flow_s* F, double* Tb, hipStream_t s) {
       launch_fill(F->getab, n, 0.0, s);
       if ((rc = magnus_reverse(F, F->segG[k], mrec[k], F->gdOb, F->getab, F->gGb, s))) return rc;
       launch_fill(F->gHSb, n, 0.0, s);
-      if ((rc = white_vjp(B, F->segH[k], F->getab, F->gHSb, s))) return rc;
+      if ((rc = generator_vjp(F, F->segH[k], F->getab, F->gHSb, s))) return rc;
       if (m == 0) {
         launch_axpy(n, F->gHb, 1.0, F->gHSb, 1.0, F->gHb, s);
         launch_fill(F->gHSb, n, 0.0, s);
@@ -2548,6 +2589,12 @@ int imsrg_magnus_vjp(imsrg_flow F, const double* ETA, const double* G, double co
   if (!F || !ETA || !G || !DGb || !ETAb || !Gb) return fail(IMSRG_EINVAL, "null argument");
   if (F->batch) return fail(IMSRG_EINVAL, "batched flow: use the imsrg_batch_* calls");
   return magnus_vjp(F, ETA, G, conv, DGb, ETAb, Gb, ncomm, S(stream));
+}
+
+int imsrg_generator_vjp(imsrg_flow F, const double* H, const double* ETAb, double* Hb, imsrg_stream stream) {
+  if (!F || !H || !ETAb || !Hb) return fail(IMSRG_EINVAL, "null argument");
+  if (F->batch) return fail(IMSRG_EINVAL, "batched flow: use the imsrg_batch_* calls");
+  return generator_vjp(F, H, ETAb, Hb, S(stream));
 }
 
 int imsrg_white_vjp(imsrg_basis B, const double* H, const double* ETAb, double* Hb, imsrg_stream stream) {

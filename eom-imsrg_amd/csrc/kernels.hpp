@@ -212,8 +212,9 @@ void launch_122_vjp_2b(const DevBasis& B, const SegTask* t, int nt, const Partne
                        const double* cbar, int ch, double* Lb, double* Rb, hipStream_t s);
 void launch_pandya_vjp(const DevBasis& B, const CCTask* t, int nt, const double* CCb, int herm, double* opb,
                        hipStream_t s);
-void launch_white_vjp(const DevBasis& B, const SegTask* g3, int ng3, const double* H, const double* M,
-                      const double* ETAb, double* Hb, double* Mb, hipStream_t s);
+void launch_gen_vjp(const DevBasis& B, const SegTask* g3, int ng3, const double* Hd, const double* Hn,
+                    const double* M, const double* ETAb, double* Hdb, double* Hnb, double* Mb, int mode,
+                    hipStream_t s);
 void launch_normal_order_vjp(const DevBasis& B, const double* Hb, double* Tb, hipStream_t s);
 void launch_add_e0(double* dst, const double* src, double factor, hipStream_t s);
 

@@ -636,7 +636,20 @@ void launch_pandya_vjp(const DevBasis& B, const CCTask* t, int nt, const double*
 }
 
 // ------------------------------------------------- white / normal order
-__global__ __launch_bounds__(256) void k_white_vjp_1b(DevBasis B, const double* H, const double* ETAb, double* Hb) {
+// gen_core transpose (build_gs_white, mode 0: eta = Hn / Eden;
+// build_gs_atan / atanxx, mode 1: eta = 0.5 atan(2 Hn / Eden), both with the
+// denominators of Hd): Hnb += deta/dHn ETAb, the cotangent of Eden goes to
+// Hdb (one-body) and Mb (monopoles of Hd).  Hdb and Hnb may be the same
+// operator.  Mode 1 scales both partial derivatives by 1 / (1 + u^2),
+// u = 2 Hn / Eden.
+__device__ __forceinline__ double gen_damp(double x, double Eden, int mode) {
+  if (mode == 0) return 1.0;
+  const double u = 2 * x / Eden;
+  return 1.0 / (1.0 + u * u);
+}
+
+__global__ __launch_bounds__(256) void k_gen_vjp_1b(DevBasis B, const double* Hd, const double* Hn, const double* ETAb,
+                                                    double* Hdb, double* Hnb, int mode) {
   const i64 x = (i64)blockIdx.x * blockDim.x + threadIdx.x;
   if (x >= (i64)B.P * B.A) return;
   const int a = (int)(x % B.P), i = (int)(x / B.P);
@@ -646,20 +659,22 @@ __global__ __launch_bounds__(256) void k_white_vjp_1b(DevBasis B, const double* 
   if (g == 0.0) return;
   const int ji = B.jj[ik];
   double Eden = 0.0;
-  for (int JT = 0; JT <= 2 * ji; JT += 2) Eden = Eden - (JT + 1) * v_elem(B, H, 1, ak, ik, ak, ik, JT);
+  for (int JT = 0; JT <= 2 * ji; JT += 2) Eden = Eden - (JT + 1) * v_elem(B, Hd, 1, ak, ik, ak, ik, JT);
   Eden = Eden / (ji + 1.0) / (ji + 1.0);
-  Eden = Eden + H[B.off_fpp + a + (i64)a * B.P] - H[B.off_fhh + i + (i64)i * B.A];
-  aadd(Hb + B.off_fph + x, g / Eden);
-  const double Eb = -g * H[B.off_fph + x] / (Eden * Eden);
-  aadd(Hb + B.off_fpp + a + (i64)a * B.P, Eb);
-  aadd(Hb + B.off_fhh + i + (i64)i * B.A, -Eb);
+  Eden = Eden + Hd[B.off_fpp + a + (i64)a * B.P] - Hd[B.off_fhh + i + (i64)i * B.A];
+  const double xv = Hn[B.off_fph + x], d = gen_damp(xv, Eden, mode);
+  aadd(Hnb + B.off_fph + x, g * d / Eden);
+  const double Eb = -g * xv * d / (Eden * Eden);
+  aadd(Hdb + B.off_fpp + a + (i64)a * B.P, Eb);
+  aadd(Hdb + B.off_fhh + i + (i64)i * B.A, -Eb);
   for (int JT = 0; JT <= 2 * ji; JT += 2)
-    vadj(vpos(B, ak, ik, ak, ik, JT), Hb, 1, -Eb * (JT + 1) / (ji + 1.0) / (ji + 1.0));
+    vadj(vpos(B, ak, ik, ak, ik, JT), Hdb, 1, -Eb * (JT + 1) / (ji + 1.0) / (ji + 1.0));
 }
 
 // two-body part; the cotangents of the monopoles go to Mb (N x N)
-__global__ __launch_bounds__(256) void k_white_vjp_2b(DevBasis B, const SegTask* tasks, const double* H,
-                                                      const double* M, const double* ETAb, double* Hb, double* Mb) {
+__global__ __launch_bounds__(256) void k_gen_vjp_2b(DevBasis B, const SegTask* tasks, const double* Hd,
+                                                    const double* Hn, const double* M, const double* ETAb,
+                                                    double* Hdb, double* Hnb, double* Mb, int mode) {
   const SegTask t = tasks[blockIdx.x];
   const i64 e = (i64)t.start + threadIdx.x;
   if (e >= t.cnt) return;
@@ -674,19 +689,20 @@ __global__ __launch_bounds__(256) void k_white_vjp_2b(DevBasis B, const SegTask*
   unpack(B.pairs[B.pairoff[q * 3 + 2] + JX], i, j);
   const int N = B.N;
   double Eden = M[a + b * N] + M[i + j * N] - M[a + i * N] - M[a + j * N] - M[i + b * N] - M[j + b * N];
-  Eden = Eden + f_elem(B, H, 1, a, a) + f_elem(B, H, 1, b, b) - f_elem(B, H, 1, i, i) - f_elem(B, H, 1, j, j);
-  aadd(Hb + o, g / Eden);
-  const double Eb = -g * H[o] / (Eden * Eden);
+  Eden = Eden + f_elem(B, Hd, 1, a, a) + f_elem(B, Hd, 1, b, b) - f_elem(B, Hd, 1, i, i) - f_elem(B, Hd, 1, j, j);
+  const double xv = Hn[o], d = gen_damp(xv, Eden, mode);
+  aadd(Hnb + o, g * d / Eden);
+  const double Eb = -g * xv * d / (Eden * Eden);
   aadd(Mb + a + b * N, Eb);
   aadd(Mb + i + j * N, Eb);
   aadd(Mb + a + i * N, -Eb);
   aadd(Mb + a + j * N, -Eb);
   aadd(Mb + i + b * N, -Eb);
   aadd(Mb + j + b * N, -Eb);
-  fadj(B, Hb, 1, a, a, Eb);
-  fadj(B, Hb, 1, b, b, Eb);
-  fadj(B, Hb, 1, i, i, -Eb);
-  fadj(B, Hb, 1, j, j, -Eb);
+  fadj(B, Hdb, 1, a, a, Eb);
+  fadj(B, Hdb, 1, b, b, Eb);
+  fadj(B, Hdb, 1, i, i, -Eb);
+  fadj(B, Hdb, 1, j, j, -Eb);
 }
 
 __global__ __launch_bounds__(256) void k_monopole_vjp(DevBasis B, const double* Mb, double* Hb) {
@@ -700,13 +716,15 @@ __global__ __launch_bounds__(256) void k_monopole_vjp(DevBasis B, const double* 
     vadj(vpos(B, a, b, a, b, JT), Hb, 1, m * (JT + 1) / (ja + 1.0) / (jb + 1.0));
 }
 
-void launch_white_vjp(const DevBasis& B, const SegTask* g3, int ng3, const double* H, const double* M,
-                      const double* ETAb, double* Hb, double* Mb, hipStream_t s) {
+void launch_gen_vjp(const DevBasis& B, const SegTask* g3, int ng3, const double* Hd, const double* Hn,
+                    const double* M, const double* ETAb, double* Hdb, double* Hnb, double* Mb, int mode,
+                    hipStream_t s) {
   const i64 n = (i64)B.P * B.A;
   if (n > 0)
-    hipLaunchKernelGGL(k_white_vjp_1b, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, B, H, ETAb, Hb);
-  if (ng3 > 0) hipLaunchKernelGGL(k_white_vjp_2b, dim3(ng3), dim3(256), 0, s, B, g3, H, M, ETAb, Hb, Mb);
-  hipLaunchKernelGGL(k_monopole_vjp, dim3((B.N * B.N + 255) / 256), dim3(256), 0, s, B, Mb, Hb);
+    hipLaunchKernelGGL(k_gen_vjp_1b, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, B, Hd, Hn, ETAb, Hdb, Hnb,
+                       mode);
+  if (ng3 > 0) hipLaunchKernelGGL(k_gen_vjp_2b, dim3(ng3), dim3(256), 0, s, B, g3, Hd, Hn, M, ETAb, Hdb, Hnb, Mb, mode);
+  hipLaunchKernelGGL(k_monopole_vjp, dim3((B.N * B.N + 255) / 256), dim3(256), 0, s, B, Mb, Hdb);
 }
 
 // normal_order transpose.  Tb must hold a copy of Hb on entry.  Pass 1: the

@@ -103,6 +103,7 @@ SIGNATURES = {
     "imsrg_build_gs_atanxx": (_i, [_p, _dp, _dp, _dp, _p]),
     "imsrg_build_gs_w2": (_i, [_p, _dp, _dp, _p]),
     "imsrg_flow_set_generator": (_i, [_p, _i]),
+    "imsrg_generator_vjp": (_i, [_p, _dp, _dp, _dp, _p]),
     "imsrg_commutator_vjp": (_i, [_p, _dp, _i, _dp, _i, _dp, _i, _dp, _dp, _p]),
     "imsrg_bch_vjp": (_i, [_p, _dp, _dp, _i, C.c_double, _i, _dp, _dp, _dp, _ip, _p]),
     "imsrg_magnus_vjp": (_i, [_p, _dp, _dp, C.c_double, _dp, _dp, _dp, _ip, _p]),

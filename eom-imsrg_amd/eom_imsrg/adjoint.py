@@ -51,6 +51,11 @@ def white_vjp(H: SqOp, ETAb: torch.Tensor, Hb: torch.Tensor) -> None:
     _lib.check(_lib.lib.imsrg_white_vjp(H.basis.handle, H.ptr, ETAb.data_ptr(), Hb.data_ptr(), _stream()))
 
 
+def generator_vjp(flow: Flow, H: SqOp, ETAb: torch.Tensor, Hb: torch.Tensor) -> None:
+    """transpose of the flow's generator (white, atan or w2): Hb += d<ETAb, eta(H)>/dH"""
+    _lib.check(_lib.lib.imsrg_generator_vjp(flow.handle, H.ptr, ETAb.data_ptr(), Hb.data_ptr(), _stream()))
+
+
 def normal_order_vjp(basis: Basis, Hb: torch.Tensor, Tb: torch.Tensor) -> None:
     _lib.check(_lib.lib.imsrg_normal_order_vjp(basis.handle, Hb.data_ptr(), Tb.data_ptr(), _stream()))
 

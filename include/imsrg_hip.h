@@ -207,11 +207,11 @@ int imsrg_magnus_expand(imsrg_flow f, double* DG, int DGherm, const double* G, i
 /* euler_step, IMSRG_MAGNUS.f90:754-764: G = G + ds*DG */
 int imsrg_euler_step(imsrg_basis b, double* G, const double* DG, double ds, imsrg_stream stream);
 /* One iteration of magnus_decouple's loop (IMSRG_MAGNUS.f90:100-125) with the
- * white generator: MAGNUS_EXPAND, euler_step, BCH_EXPAND, build_gs_white,
- * mbpt2.  out[0..3] = E0, mbpt2, Magnus commutators, BCH loop index. */
+ * flow's generator (imsrg_flow_set_generator; white by default):
+ * MAGNUS_EXPAND, euler_step, BCH_EXPAND, the generator, mbpt2.  out[0..3] = E0, mbpt2, Magnus commutators, BCH loop index. */
 int imsrg_magnus_step(imsrg_flow f, double* HS, double* G, double* DG, const double* H, double ds,
                       double bch_conv, double mag_conv, double* out, imsrg_stream stream);
-/* magnus_decouple, IMSRG_MAGNUS.f90:15-128 with build_gs_white: HS in = the
+/* magnus_decouple, IMSRG_MAGNUS.f90:15-128 with the flow's generator: HS in = the
  * bare normal-ordered H, out = decoupled H; G out = Omega.  crit_mode 0:
  * loop while |mbpt2| > crit_tol (reference); 1: until |dE0| < crit_tol.
  * log (host, optional) receives 4 doubles per step: s, E0, E0+mbpt2,
@@ -274,11 +274,14 @@ int imsrg_magnus_vjp(imsrg_flow f, const double* ETA, const double* G, double co
                      double* ETAb, double* Gb, int* ncomm, imsrg_stream stream);
 /* transpose of build_gs_white (generators.f90:4-106): Hb += d<ETAb,eta(H)>/dH */
 int imsrg_white_vjp(imsrg_basis b, const double* H, const double* ETAb, double* Hb, imsrg_stream stream);
+/* transpose of the flow's generator (build_gs_white, build_gs_atan or
+ * build_gs_w2, generators.f90:4-160): Hb += d<ETAb,eta(H)>/dH */
+int imsrg_generator_vjp(imsrg_flow f, const double* H, const double* ETAb, double* Hb, imsrg_stream stream);
 /* transpose of normal_order (basic_IMSRG.f90:2419-2550): Tb = d<Hb,H(T)>/dT
  * (overwritten) */
 int imsrg_normal_order_vjp(imsrg_basis b, const double* Hb, double* Tb, imsrg_stream stream);
-/* dE0/dT for magnus_decouple (IMSRG_MAGNUS.f90:15-128, white generator,
- * |mbpt2| > crit_tol loop) started from H = normal_order(T), in two sweeps:
+/* dE0/dT for magnus_decouple (IMSRG_MAGNUS.f90:15-128, the flow's
+ * generator, |mbpt2| > crit_tol loop) started from H = normal_order(T), in two sweeps:
  * imsrg_flow_forward runs the flow, returns E0 and the step count and keeps
  * Omega every `ckpt` steps in the flow object; imsrg_flow_backward then
  * writes dE0/dT into Tb (overwritten), recomputing the steps between

@@ -1177,7 +1177,20 @@ void orc_normal_order_vjp(void* h, const double* Hb, double* Tb) {
 }
 int orc_flow_grad(void* h, const double* T, double ds, double bch_conv, double mag_conv, double crit_tol,
                   int maxsteps, int ckpt, double* Tbar, double* E0) {
-  OPS(h) return flow_grad(b, T, ds, bch_conv, mag_conv, crit_tol, maxsteps, ckpt, Tbar, E0);
+  OPS(h) return flow_grad(b, T, ds, bch_conv, mag_conv, crit_tol, maxsteps, ckpt, Tbar, E0, 0);
+}
+int orc_flow_grad_gen(void* h, const double* T, double ds, double bch_conv, double mag_conv, double crit_tol,
+                      int maxsteps, int ckpt, double* Tbar, double* E0, int gen) {
+  OPS(h) return flow_grad(b, T, ds, bch_conv, mag_conv, crit_tol, maxsteps, ckpt, Tbar, E0, gen);
+}
+void orc_gen_vjp(void* h, const double* Hd, const double* Hn, const double* ETAb, double* Hdb, double* Hnb,
+                 int mode) {
+  OPS(h) Op hdb{&b, Hdb, 1}, hnb{&b, Hnb, 1};
+  adj_gen(Op{&b, (double*)Hd, 1}, Op{&b, (double*)Hn, 1}, Op{&b, (double*)ETAb, -1}, hdb, hnb, mode);
+}
+void orc_gs_w2_vjp(void* h, const double* H, const double* ETAb, double* Hb) {
+  OPS(h) Op hb{&b, Hb, 1}; Work w;
+  adj_gs_w2(Op{&b, (double*)H, 1}, Op{&b, (double*)ETAb, -1}, hb, w);
 }
 
 // One full scalar commutator RES = [L, R] in the BCH term order

@@ -56,6 +56,8 @@ def lib():
             "orc_white_vjp": (None, [vp, dp, dp, dp]),
             "orc_normal_order_vjp": (None, [vp, dp, dp]),
             "orc_flow_grad": (i, [vp, dp, d, d, d, d, i, i, dp, dp]),
+            "orc_flow_grad_gen": (i, [vp, dp, d, d, d, d, i, i, dp, dp, i]),
+            "orc_gen_vjp": (None, [vp, dp, dp, dp, dp, dp, i]), "orc_gs_w2_vjp": (None, [vp, dp, dp, dp]),
             "orc_bf_0body": (d, [vp, dp, i, dp, i]), "orc_bf_1body": (d, [vp, dp, i, dp, i, i, i]),
             "orc_bf_2body": (d, [vp, dp, i, dp, i, i, i, i, i, i, dp]),
         }
@@ -292,11 +294,24 @@ class OracleBasis:
         lib().orc_normal_order_vjp(self.h, _dp(np.ascontiguousarray(Hb)), _dp(Tb))
         return Tb
 
-    def flow_grad(self, T, ds, bch_conv, mag_conv, crit_tol, maxsteps=200, ckpt=4):
-        """(E0, dE0/dT, steps) for magnus_decouple started from normal_order(T)"""
+    def gen_vjp(self, Hd, Hn, ETAb, mode):
+        """(Hdb, Hnb) of gen_core (mode 0 white, 1 atan / atanxx)"""
+        Hdb, Hnb = np.zeros(self.oplen), np.zeros(self.oplen)
+        lib().orc_gen_vjp(self.h, _dp(Hd), _dp(Hn), _dp(np.ascontiguousarray(ETAb)), _dp(Hdb), _dp(Hnb), int(mode))
+        return Hdb, Hnb
+
+    def gs_w2_vjp(self, H, ETAb):
+        Hb = np.zeros(self.oplen)
+        lib().orc_gs_w2_vjp(self.h, _dp(H), _dp(np.ascontiguousarray(ETAb)), _dp(Hb))
+        return Hb
+
+    def flow_grad(self, T, ds, bch_conv, mag_conv, crit_tol, maxsteps=200, ckpt=4, generator=0):
+        """(E0, dE0/dT, steps) for magnus_decouple started from normal_order(T)
+        (generator 0 white, 1 atan, 2 w2)"""
         Tb = np.zeros(self.oplen)
         e0 = np.zeros(1)
-        st = lib().orc_flow_grad(self.h, _dp(T), ds, bch_conv, mag_conv, crit_tol, maxsteps, ckpt, _dp(Tb), _dp(e0))
+        st = lib().orc_flow_grad_gen(self.h, _dp(T), ds, bch_conv, mag_conv, crit_tol, maxsteps, ckpt, _dp(Tb),
+                                     _dp(e0), int(generator))
         return float(e0[0]), Tb, st
 
     def bf_0body(self, A, Ah, B, Bh):

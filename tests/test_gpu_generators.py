@@ -98,15 +98,49 @@ def test_batch_decouple_w2_rows(gpu):
     assert str_ == steps[0] and abs(HSr[0] - e0[0]) < 1e-8
 
 
-def test_flow_gradient_requires_white(gpu):
-    from eom_imsrg import Flow, _lib, adjoint, synth
-    b = basis(2)
-    f = Flow(b, generator="atan")
-    T = torch.from_numpy(synth.hamiltonian_vacuum(b, 0)).cuda()
-    with pytest.raises(_lib.ImsrgError):
-        adjoint.flow_forward(f, T)
+def test_unknown_generator_is_refused(gpu):
+    from eom_imsrg import Flow
+    f = Flow(basis(2))
     with pytest.raises(KeyError):
         f.set_generator("imtime")
+
+
+@pytest.mark.parametrize("gen", ["atan", "w2"])
+def test_generator_vjp_vs_oracle(gpu, gen):
+    """transpose of build_gs_atan / build_gs_w2 against the oracle's, eMax 4"""
+    from eom_imsrg import Flow, SqOp, adjoint
+    emax = 4
+    b, o = basis(emax), oracle_basis(emax)
+    print(f"[generator vjp] emax=4 {gen}", flush=True)
+    H = _hamiltonian(emax)
+    Eb = np.random.default_rng(6).normal(size=b.oplen)
+    if gen == "atan":
+        Hdb, Hnb = o.gen_vjp(H, H, Eb, 1)
+        ref = Hdb + Hnb
+    else:
+        ref = o.gs_w2_vjp(H, Eb)
+    f = Flow(b, generator=gen)
+    Hb = torch.zeros(b.oplen, dtype=torch.float64, device="cuda")
+    adjoint.generator_vjp(f, SqOp.from_numpy(b, H, 1), torch.from_numpy(Eb).cuda(), Hb)
+    torch.cuda.synchronize()
+    assert_rel(adjoint.symmetric_gradient(b, Hb.cpu().numpy(), 1), adjoint.symmetric_gradient(b, ref, 1),
+               rtol=1e-9, what=f"{gen} transpose")
+
+
+@pytest.mark.parametrize("gen", ["atan", "w2"])
+def test_flow_gradient_with_generator_vs_oracle(gpu, gen):
+    """dE0/dT of the whole flow with the atan / w2 generator (Omega and H(s)
+    checkpointed every 4 steps) against the oracle's reverse sweep, eMax 4"""
+    from eom_imsrg import Flow, adjoint, synth
+    b, o = basis(4), oracle_basis(4)
+    print(f"[flow_grad generator] emax=4 {gen}", flush=True)
+    T = synth.hamiltonian_vacuum(b, 0)
+    gid = {"atan": 1, "w2": 2}[gen]
+    e0_o, Tb_o, st_o = o.flow_grad(T, 0.5, REF, REF, REF, ckpt=4, generator=gid)
+    e0, Tb, st = adjoint.flow_grad(Flow(b, graphs=True, generator=gen), torch.from_numpy(T).cuda(), ckpt=4)
+    assert st == st_o and abs(e0 - e0_o) < 1e-8, (st, st_o, e0, e0_o)
+    assert_rel(adjoint.symmetric_gradient(b, Tb.cpu().numpy(), 1), adjoint.symmetric_gradient(b, Tb_o, 1), rtol=1e-8,
+               what=f"dE0/dT {gen}")
 
 
 def test_transform_sq_op_bch(gpu):

@@ -154,3 +154,68 @@ def test_flow_gradient_vs_central_differences(emax, ndir):
         em, _ = _flow_e0(o, T - h * d)
         fd = (ep - em) / (2 * h)
         assert abs(fd - Tb @ d) < 1e-6 * abs(fd) + 5e-8, (fd, Tb @ d)
+
+
+def test_atan_atanxx_vjp():
+    """gen_core transpose in atan mode against central differences: aliased
+    (build_gs_atan, Hd = Hn) and with separate directions for the
+    denominator operator and the numerator operator (build_gs_atanxx)"""
+    from eom_imsrg import synth
+    b, o = basis(4), oracle_basis(4)
+    H = o.normal_order(synth.hamiltonian_vacuum(b, 0))
+    H2 = o.commutator(synth.antihermitian(b, 7, 0.1), -1, H, 1, 1)
+    Eb = np.random.default_rng(4).normal(size=b.oplen)
+    Hdb, Hnb = o.gen_vjp(H, H, Eb, 1)
+    for s in range(2):
+        dH = synth.hermitian_random(b, 60 + s, 0.1)
+        fd = _central(lambda h: Eb @ o.gs_atan(h), H, dH, 1e-5)
+        assert abs(fd - (Hdb + Hnb) @ dH) < 1e-7 * (abs(fd) + 1e-3), (fd, (Hdb + Hnb) @ dH)
+    Hdb, Hnb = o.gen_vjp(H, H2, Eb, 1)
+    for s in range(2):
+        dH = synth.hermitian_random(b, 70 + s, 0.1)
+        fd = _central(lambda h: Eb @ o.gs_atanxx(h, H2), H, dH, 1e-5)
+        assert abs(fd - Hdb @ dH) < 1e-7 * (abs(fd) + 1e-3), ("Hold", fd, Hdb @ dH)
+        fd = _central(lambda h: Eb @ o.gs_atanxx(H, h), H2, dH, 1e-5)
+        assert abs(fd - Hnb @ dH) < 1e-7 * (abs(fd) + 1e-3), ("H", fd, Hnb @ dH)
+    # mode 0 with Hd = Hn is the white transpose
+    Hdb, Hnb = o.gen_vjp(H, H, Eb, 0)
+    assert np.allclose(Hdb + Hnb, o.white_vjp(H, Eb), rtol=1e-13, atol=1e-15)
+
+
+def test_w2_vjp():
+    from eom_imsrg import synth
+    b, o = basis(2), oracle_basis(2)
+    H = o.normal_order(synth.hamiltonian_vacuum(b, 0))
+    Eb = np.random.default_rng(5).normal(size=b.oplen)
+    Hb = o.gs_w2_vjp(H, Eb)
+    for s in range(3):
+        dH = synth.hermitian_random(b, 80 + s, 0.1)
+        fd = _central(lambda h: Eb @ o.gs_w2(h), H, dH, 1e-5)
+        assert abs(fd - Hb @ dH) < 1e-7 * (abs(fd) + 1e-3), (fd, Hb @ dH)
+
+
+def _flow_e0_gen(o, T, gen):
+    H = o.normal_order(T)
+    HS, _, log, st = o.magnus_decouple(H, 0.5, REF, REF, 0, REF, generator=gen)
+    return HS[0], st
+
+
+@pytest.mark.parametrize("gen", [1, 2], ids=["atan", "w2"])
+def test_flow_gradient_generators_vs_central_differences(gen):
+    """the flow gradient with the atan and w2 generators (reverse sweep
+    through their transposes) against central differences of the flow E0,
+    eMax 2"""
+    from eom_imsrg import synth
+    b, o = basis(2), oracle_basis(2)
+    T = synth.hamiltonian_vacuum(b, 0)
+    e0, Tb, st = o.flow_grad(T, 0.5, REF, REF, REF, ckpt=4, generator=gen)
+    e0_ref, st_ref = _flow_e0_gen(o, T, gen)
+    assert st == st_ref and abs(e0 - e0_ref) < 1e-10
+    h = 1e-5
+    for s in range(3):
+        d = _two_body_direction(b, 110 + s)
+        ep, sp = _flow_e0_gen(o, T + h * d, gen)
+        em, sm = _flow_e0_gen(o, T - h * d, gen)
+        assert sp == sm == st
+        fd = (ep - em) / (2 * h)
+        assert abs(fd - Tb @ d) < 1e-6 * abs(fd), (fd, Tb @ d)
