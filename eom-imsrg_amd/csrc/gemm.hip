@@ -28,9 +28,12 @@ struct Regs {
 // The transposes TA / TB are template parameters (four specialised bodies
 // chosen once per workgroup), and out-of-range elements load a valid
 // address and select zero, so the fetch has no per-element branches.
-template <int TBM, int TBN, bool TA, bool TB>
+template <int TBM, int TBN, bool TA0, bool TB0, bool RT>
 __device__ __forceinline__ void fetch(Regs<TBM, TBN>& t, const double* A, int lda, const double* Bm, int ldb, int M,
-                                      int N, int K, int m0, int n0, int k0, double scale) {
+                                      int N, int K, int m0, int n0, int k0, double scale, bool ta, bool tb) {
+  // RT: the transposes are the (workgroup-uniform) run-time ta / tb, one
+  // code path for all four; else the template constants
+  const bool TA = RT ? ta : TA0, TB = RT ? tb : TB0;
   const int tid = threadIdx.x;
 #pragma unroll
   for (int r = 0; r < TBM * GBK / 256; ++r) {
@@ -54,8 +57,10 @@ __device__ __forceinline__ void fetch(Regs<TBM, TBN>& t, const double* A, int ld
   }
 }
 
-template <int TBM, int TBN, bool TA, bool TB>
-__device__ __forceinline__ void stash(const Regs<TBM, TBN>& t, double (*As)[TBM + 1], double (*Bs)[TBN + 1]) {
+template <int TBM, int TBN, bool TA0, bool TB0, bool RT>
+__device__ __forceinline__ void stash(const Regs<TBM, TBN>& t, double (*As)[TBM + 1], double (*Bs)[TBN + 1], bool ta,
+                                      bool tb) {
+  const bool TA = RT ? ta : TA0, TB = RT ? tb : TB0;
   const int tid = threadIdx.x;
 #pragma unroll
   for (int r = 0; r < TBM * GBK / 256; ++r) {
@@ -76,23 +81,23 @@ __device__ __forceinline__ void stash(const Regs<TBM, TBN>& t, double (*As)[TBM 
 // l of tile (mi, ni) holds C row wm*TBM/2 + mi*16 + l%16, column
 // wn*TBN/2 + ni*16 + l/16 + 4r (16 consecutive rows per row group, so the
 // column-major stores are contiguous).
-template <int TBM, int TBN, bool TA, bool TB>
+template <int TBM, int TBN, bool TA, bool TB, bool RT = false>
 __device__ __forceinline__ void mma_t(v4d* acc, double (*As)[GBK][TBM + 1], double (*Bs)[GBK][TBN + 1],
                                       const double* A, int lda, const double* Bm, int ldb, int M, int N, int k0s,
-                                      int k1, int m0, int n0, double scale) {
+                                      int k1, int m0, int n0, double scale, bool ta = false, bool tb = false) {
   constexpr int MI = TBM / 32, NI = TBN / 32;   // acc[mi * NI + ni]
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wm = wave >> 1, wn = wave & 1;
   const int lr = lane & 15, lk = lane >> 4;
   if (k0s >= k1) return;
   Regs<TBM, TBN> t;
-  fetch<TBM, TBN, TA, TB>(t, A, lda, Bm, ldb, M, N, k1, m0, n0, k0s, scale);
-  stash<TBM, TBN, TA, TB>(t, As[0], Bs[0]);
+  fetch<TBM, TBN, TA, TB, RT>(t, A, lda, Bm, ldb, M, N, k1, m0, n0, k0s, scale, ta, tb);
+  stash<TBM, TBN, TA, TB, RT>(t, As[0], Bs[0], ta, tb);
   __syncthreads();
   int cur = 0;
   for (int k0 = k0s; k0 < k1; k0 += GBK) {
     const bool more = (k0 + GBK < k1);
-    if (more) fetch<TBM, TBN, TA, TB>(t, A, lda, Bm, ldb, M, N, k1, m0, n0, k0 + GBK, scale);
+    if (more) fetch<TBM, TBN, TA, TB, RT>(t, A, lda, Bm, ldb, M, N, k1, m0, n0, k0 + GBK, scale, ta, tb);
 #pragma unroll
     for (int kk = 0; kk < GBK; kk += 4) {
       double a[MI], b[NI];
@@ -106,7 +111,7 @@ __device__ __forceinline__ void mma_t(v4d* acc, double (*As)[GBK][TBM + 1], doub
         for (int ni = 0; ni < NI; ++ni)
           acc[mi * NI + ni] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[ni], a[mi], acc[mi * NI + ni], 0, 0, 0);
     }
-    if (more) stash<TBM, TBN, TA, TB>(t, As[cur ^ 1], Bs[cur ^ 1]);
+    if (more) stash<TBM, TBN, TA, TB, RT>(t, As[cur ^ 1], Bs[cur ^ 1], ta, tb);
     __syncthreads();
     cur ^= 1;
   }
@@ -123,26 +128,18 @@ __device__ __forceinline__ const double* slot(const Slots& sl, int k, int inst) 
   return r + (i64)inst * st;
 }
 
-template <int TBM, int TBN>
-__device__ __forceinline__ void mma(v4d* acc, double (*As)[GBK][TBM + 1], double (*Bs)[GBK][TBN + 1], const double* A,
-                                    int lda, int tA, const double* Bm, int ldb, int tB, int M, int N, int k0s, int k1,
-                                    int m0, int n0, double scale) {
-  if (!tA && !tB) mma_t<TBM, TBN, false, false>(acc, As, Bs, A, lda, Bm, ldb, M, N, k0s, k1, m0, n0, scale);
-  else if (!tA) mma_t<TBM, TBN, false, true>(acc, As, Bs, A, lda, Bm, ldb, M, N, k0s, k1, m0, n0, scale);
-  else if (!tB) mma_t<TBM, TBN, true, false>(acc, As, Bs, A, lda, Bm, ldb, M, N, k0s, k1, m0, n0, scale);
-  else mma_t<TBM, TBN, true, true>(acc, As, Bs, A, lda, Bm, ldb, M, N, k0s, k1, m0, n0, scale);
-}
 
 // product P (1 or 2) of a kernel specialised on the transposes of its
 // family (SIG bit0 TA1, bit1 TB1, bit2 TA2, bit3 TB2), or with the
-// problem's own transposes chosen at run time (SIG < 0).  A specialised
-// kernel inlines one main loop per product instead of four.
+// problem's own transposes read at run time (SIG < 0: one main loop whose
+// index maps select on the uniform ta / tb).
 template <int TBM, int TBN, int SIG, int P>
 __device__ __forceinline__ void mma_p(v4d* acc, double (*As)[GBK][TBM + 1], double (*Bs)[GBK][TBN + 1],
                                       const double* A, int lda, int tA, const double* Bm, int ldb, int tB, int M, int N,
                                       int k0s, int k1, int m0, int n0, double scale) {
   if constexpr (SIG < 0) {
-    mma<TBM, TBN>(acc, As, Bs, A, lda, tA, Bm, ldb, tB, M, N, k0s, k1, m0, n0, scale);
+    mma_t<TBM, TBN, false, false, true>(acc, As, Bs, A, lda, Bm, ldb, M, N, k0s, k1, m0, n0, scale, tA != 0,
+                                        tB != 0);
   } else {
     constexpr bool TA = (SIG >> (P == 1 ? 0 : 2)) & 1, TB = (SIG >> (P == 1 ? 1 : 3)) & 1;
     mma_t<TBM, TBN, TA, TB>(acc, As, Bs, A, lda, Bm, ldb, M, N, k0s, k1, m0, n0, scale);
