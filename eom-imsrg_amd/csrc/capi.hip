@@ -1041,10 +1041,8 @@ void comm_vjp(imsrg_basis_s* B, const double* L, int Lh, const double* R, int Rh
   // the partner sums
   launch_fill(B->cbar, b.off_fpp + (i64)b.P * b.P, 0.0, s);
   launch_to_dense(D, B->seg_full.dev, B->seg_full.n, RESb, RESh, B->cbar, s);
-  launch_122_vjp_2b(D, B->seg_tri.dev, B->seg_tri.n, B->partners, B->rowp, B->colp, B->f1off, B->nf1, L, Lh, R, Rh,
-                    B->cbar, RESh, Lb, Rb, s);
-  launch_122_vjp_1b(D, B->seg_tri.dev, B->seg_tri.n, B->partners, B->rowp, B->colp, B->f1off, B->nf1, L,
-                    Lh, R, Rh, RESb, RESh, Lb, Rb, s);
+  launch_122_vjp(D, B->seg_tri.dev, B->seg_tri.n, B->partners, B->rowp, B->colp, B->f1off, B->nf1, L, Lh, R, Rh,
+                 B->cbar, RESb, RESh, Lb, Rb, s);
   // 222_ph: scatter transpose into the W' cotangent U, cross-coupled
   // cotangents (triangular-U GEMMs), Pandya transposes
   launch_pandya(D, B->cc.dev, B->cc.n, R, Rh, B->aRCC, s);

@@ -201,15 +201,13 @@ void launch_to_dense(const DevBasis& B, const SegTask* t, int nt, const double* 
                      hipStream_t s);
 void launch_from_dense(const DevBasis& B, const SegTask* t, int nt, const double* Y, double alpha, double* dst,
                        hipStream_t s);
-void launch_122_vjp_1b(const DevBasis& B, const SegTask* t, int nt, const Partner* part, const int* rowp,
-                       const int* colp, const int* f1off, int nf1, const double* L, int Lh, const double* R, int Rh,
-                       const double* RESb, int RESh, double* Lb, double* Rb, hipStream_t s);
 void launch_ph_vjp(const DevBasis& B, const QuadItem* items, int nitems, const int* lanes, int nl, const int* quads,
                    const int* qstart, const PairTask* tiles, int ntiles, const int* tstart, const int2* terms,
                    const double* RESb, int RESh, int Lh, int Rh, double* SB, double* CV, double* U, hipStream_t s);
-void launch_122_vjp_2b(const DevBasis& B, const SegTask* t, int nt, const Partner* part, const int* rowp,
-                       const int* colp, const int* f1off, int nf1, const double* L, int Lh, const double* R, int Rh,
-                       const double* cbar, int ch, double* Lb, double* Rb, hipStream_t s);
+// both 122 transposes (two-body and one-body) in one pass
+void launch_122_vjp(const DevBasis& B, const SegTask* t, int nt, const Partner* part, const int* rowp, const int* colp,
+                    const int* f1off, int nf1, const double* L, int Lh, const double* R, int Rh, const double* cbar,
+                    const double* RESb, int RESh, double* Lb, double* Rb, hipStream_t s);
 void launch_pandya_vjp(const DevBasis& B, const CCTask* t, int nt, const double* CCb, int herm, double* opb,
                        hipStream_t s);
 void launch_gen_vjp(const DevBasis& B, const SegTask* g3, int ng3, const double* Hd, const double* Hn,

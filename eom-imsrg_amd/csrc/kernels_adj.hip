@@ -324,109 +324,40 @@ void launch_from_dense(const DevBasis& B, const SegTask* t, int nt, const double
   hipLaunchKernelGGL(k_from_dense, dim3(nt), dim3(256), 0, s, B, t, Y, alpha, dst);
 }
 
-// two-body part of the 122 transpose in one pass: with cbar the symmetrised
-// cotangent (one-body part zero), y1 = [R_1b, cbar] and y2 = [L_1b, cbar]
-// element by element (the partner sums of k_comm122p), then
-// Lb += -Rh y1 and Rb += Lh y2 on stored elements, the lower partners of
-// square blocks through their herm (Lh for y1, Rh for y2), rectangular
-// blocks twice (their transposed partners are not stored)
-__global__ __launch_bounds__(256) void k_122_vjp_2b(DevBasis B, const SegTask* tasks, const Partner* part,
-                                                    const int* rowp, const int* colp, const int* f1off, int nf1,
-                                                    const double* L, int Lh, const double* R, int Rh,
-                                                    const double* cbar, int ch, double* Lb, double* Rb) {
-  // the one-body elements the partner terms read, L then R
-  extern __shared__ double s_f[];
+
+
+// Both 122 transposes in one pass over the upper elements (the partner
+// lists, element decoding and partner positions are shared):
+// * two-body part: with cbar the symmetrised cotangent (one-body part
+//   zero), y1 = [R_1b, cbar] and y2 = [L_1b, cbar] element by element (the
+//   partner sums of k_comm122p), then Lb += -Rh y1 and Rb += Lh y2 on
+//   stored elements, the lower partners of square blocks through their herm
+//   (Lh for y1, Rh for y2), rectangular blocks twice (their transposed
+//   partners are not stored);
+// * one-body part: the partner sums of k_comm122p weighted by the cotangent
+//   of each upper element, accumulated per workgroup in LDS (compact index
+//   fk) and added to Lb / Rb once at the end.
+// Persistent grid; LDS holds the one-body values of L and R and the
+// one-body accumulators.
+__global__ __launch_bounds__(256) void k_122_vjp(DevBasis B, const SegTask* tasks, int ntasks, const Partner* part,
+                                                 const int* rowp, const int* colp, const int* f1off, int nf1,
+                                                 const double* L, int Lh, const double* R, int Rh,
+                                                 const double* cbar, const double* RESb, int RESh, double* Lb,
+                                                 double* Rb) {
+  extern __shared__ double sh[];
+  double* fL = sh;
+  double* fR = sh + nf1;
+  double* lbar = sh + 2 * nf1;
+  double* rbar = sh + 3 * nf1;
   for (int k = threadIdx.x; k < nf1; k += blockDim.x) {
     const int o = f1off[k];
-    s_f[k] = L[o];
-    s_f[nf1 + k] = R[o];
+    fL[k] = L[o];
+    fR[k] = R[o];
+    lbar[k] = 0.0;
+    rbar[k] = 0.0;
   }
   __syncthreads();
-  const SegTask t = tasks[blockIdx.x];
-  const i64 e = (i64)t.start + threadIdx.x;
-  if (e >= t.cnt) return;
-  const int q = t.q, g = t.g;
-  const int rc = row_cls(g), cc = col_cls(g);
-  const int n1 = B.dims[q * 3 + rc];
-  const bool sq = is_square(g);
-  int IX, JX;
-  if (sq) upper_decode(e, IX, JX);
-  else { IX = (int)(e % n1); JX = (int)(e / n1); }
-  const i64 gr = B.pairoff[q * 3 + rc] + IX, gc = B.pairoff[q * 3 + cc] + JX;
-  int a, b, c, d;
-  unpack(B.pairs[gr], a, b);
-  unpack(B.pairs[gc], c, d);
-  const double pre_ab = (a == b) ? 1.4142135623730951 : 1.0;
-  const double pre_cd = (c == d) ? 1.4142135623730951 : 1.0;
-  const i64* go = B.goff + q * 6;
-  const int* dm = B.dims + q * 3;
-  double y1 = 0.0, y2 = 0.0;
-  for (int x = rowp[gr]; x < rowp[gr + 1]; ++x) {
-    const Partner p = part[x];
-    const int idx = p.code & 0x3fffff, cl = (p.code >> 22) & 3;
-    double pre = ((p.code >> 26) & 1) ? 1.4142135623730951 : 1.0;
-    if ((p.code >> 25) & 1) pre = -pre;
-    pre = pre * pre_cd;
-    double v;
-    if (cl <= cc) v = cbar[go[cl == 0 ? cc : (cl == 1 ? (cc == 1 ? 3 : 5) : 4)] + idx + (i64)JX * dm[cl]] * pre;
-    else v = cbar[go[cc == 0 ? cl : (cc == 1 ? (cl == 1 ? 3 : 5) : 4)] + JX + (i64)idx * dm[cc]] * ch * pre;
-    const bool fs = (p.code >> 24) & 1;
-    y1 += s_f[nf1 + p.fk] * (fs ? Rh : 1) * v;
-    y2 += s_f[p.fk] * (fs ? Lh : 1) * v;
-  }
-  for (int x = colp[gc]; x < colp[gc + 1]; ++x) {
-    const Partner p = part[x];
-    const int idx = p.code & 0x3fffff, cl = (p.code >> 22) & 3;
-    double pre = ((p.code >> 26) & 1) ? 1.4142135623730951 : 1.0;
-    if ((p.code >> 25) & 1) pre = -pre;
-    pre = pre_ab * pre;
-    double v;
-    if (rc <= cl) v = cbar[go[rc == 0 ? cl : (rc == 1 ? (cl == 1 ? 3 : 5) : 4)] + IX + (i64)idx * dm[rc]] * pre;
-    else v = cbar[go[cl == 0 ? rc : (cl == 1 ? (rc == 1 ? 3 : 5) : 4)] + idx + (i64)IX * dm[cl]] * ch * pre;
-    const bool fs = (p.code >> 24) & 1;
-    y1 -= s_f[nf1 + p.fk] * (fs ? Rh : 1) * v;
-    y2 -= s_f[p.fk] * (fs ? Lh : 1) * v;
-  }
-  const double nrm = 1.0 / (sqrt(1.0 + (a == b ? 1.0 : 0.0)) * sqrt(1.0 + (c == d ? 1.0 : 0.0)));
-  y1 *= nrm;
-  y2 *= nrm;
-  const i64 o = go[g] + IX + (i64)JX * n1;
-  if (!sq) {
-    Lb[o] += -2.0 * Rh * y1;
-    Rb[o] += 2.0 * Lh * y2;
-  } else if (IX == JX) {
-    Lb[o] += -Rh * Lh * y1;
-    Rb[o] += Lh * Rh * y2;
-  } else {
-    const i64 ol = go[g] + JX + (i64)IX * n1;
-    Lb[o] += -Rh * y1;
-    Lb[ol] += -Rh * Lh * y1;
-    Rb[o] += Lh * y2;
-    Rb[ol] += Lh * Rh * y2;
-  }
-}
-
-void launch_122_vjp_2b(const DevBasis& B, const SegTask* t, int nt, const Partner* part, const int* rowp,
-                       const int* colp, const int* f1off, int nf1, const double* L, int Lh, const double* R, int Rh,
-                       const double* cbar, int ch, double* Lb, double* Rb, hipStream_t s) {
-  if (nt <= 0) return;
-  hipLaunchKernelGGL(k_122_vjp_2b, dim3(nt), dim3(kTile), 2 * nf1 * sizeof(double), s, B, t, part, rowp, colp, f1off,
-                     nf1, L, Lh, R, Rh, cbar, ch, Lb, Rb);
-}
-
-// one-body part of the 122 transpose: the partner sums of k_comm122p with
-// the cotangent of each upper element; contributions to the few one-body
-// elements are summed per workgroup in LDS (compact index fk)
-__global__ __launch_bounds__(256) void k_122_vjp_1b(DevBasis B, const SegTask* tasks, int ntasks,
-                                                    const Partner* part, const int* rowp, const int* colp,
-                                                    const int* f1off, int nf1, const double* L, int Lh,
-                                                    const double* R, int Rh, const double* RESb, int RESh,
-                                                    double* Lb, double* Rb) {
-  extern __shared__ double sh[];
-  double* lbar = sh;
-  double* rbar = sh + nf1;
-  for (int k = threadIdx.x; k < 2 * nf1; k += blockDim.x) sh[k] = 0.0;
-  __syncthreads();
+  const int ch = RESh;
   for (int ti = blockIdx.x; ti < ntasks; ti += gridDim.x) {
     const SegTask t = tasks[ti];
     const i64 e = (i64)t.start + threadIdx.x;
@@ -434,38 +365,40 @@ __global__ __launch_bounds__(256) void k_122_vjp_1b(DevBasis B, const SegTask* t
     const int q = t.q, g = t.g;
     const int rc = row_cls(g), cc = col_cls(g);
     const int n1 = B.dims[q * 3 + rc];
+    const bool sq = is_square(g);
     int IX, JX;
-    if (is_square(g)) upper_decode(e, IX, JX);
+    if (sq) upper_decode(e, IX, JX);
     else { IX = (int)(e % n1); JX = (int)(e / n1); }
     const i64 gr = B.pairoff[q * 3 + rc] + IX, gc = B.pairoff[q * 3 + cc] + JX;
     int a, b, c, d;
     unpack(B.pairs[gr], a, b);
     unpack(B.pairs[gc], c, d);
-    double sb = fold2(B, RESb, q, g, IX, JX, RESh);
-    sb = sb / sqrt(1.0 + (a == b ? 1.0 : 0.0)) / sqrt(1.0 + (c == d ? 1.0 : 0.0));
-    if (sb == 0.0) continue;
     const double pre_ab = (a == b) ? 1.4142135623730951 : 1.0;
     const double pre_cd = (c == d) ? 1.4142135623730951 : 1.0;
+    const double nrm = 1.0 / (sqrt(1.0 + (a == b ? 1.0 : 0.0)) * sqrt(1.0 + (c == d ? 1.0 : 0.0)));
+    double sb = fold2(B, RESb, q, g, IX, JX, RESh);
+    sb = sb / sqrt(1.0 + (a == b ? 1.0 : 0.0)) / sqrt(1.0 + (c == d ? 1.0 : 0.0));
     const i64* go = B.goff + q * 6;
     const int* dm = B.dims + q * 3;
+    double y1 = 0.0, y2 = 0.0;
     for (int x = rowp[gr]; x < rowp[gr + 1]; ++x) {
       const Partner p = part[x];
       const int idx = p.code & 0x3fffff, cl = (p.code >> 22) & 3;
       double pre = ((p.code >> 26) & 1) ? 1.4142135623730951 : 1.0;
       if ((p.code >> 25) & 1) pre = -pre;
       pre = pre * pre_cd;
-      double vr, vl;
-      if (cl <= cc) {
-        const i64 o = go[cl == 0 ? cc : (cl == 1 ? (cc == 1 ? 3 : 5) : 4)] + idx + (i64)JX * dm[cl];
-        vr = R[o] * pre; vl = L[o] * pre;
-      } else {
-        const i64 o = go[cc == 0 ? cl : (cc == 1 ? (cl == 1 ? 3 : 5) : 4)] + JX + (i64)idx * dm[cc];
-        vr = R[o] * Rh * pre; vl = L[o] * Lh * pre;
-      }
+      const bool lo = cl <= cc;
+      const i64 o = lo ? go[cl == 0 ? cc : (cl == 1 ? (cc == 1 ? 3 : 5) : 4)] + idx + (i64)JX * dm[cl]
+                       : go[cc == 0 ? cl : (cc == 1 ? (cl == 1 ? 3 : 5) : 4)] + JX + (i64)idx * dm[cc];
+      const double v = lo ? cbar[o] * pre : cbar[o] * ch * pre;
       const bool fs = (p.code >> 24) & 1;
-      const int k = p.fk;
-      unsafeAtomicAdd(lbar + k, sb * (fs ? Lh : 1) * vr);
-      unsafeAtomicAdd(rbar + k, -sb * (fs ? Rh : 1) * vl);
+      y1 += fR[p.fk] * (fs ? Rh : 1) * v;
+      y2 += fL[p.fk] * (fs ? Lh : 1) * v;
+      if (sb != 0.0) {
+        const double vr = lo ? R[o] * pre : R[o] * Rh * pre, vl = lo ? L[o] * pre : L[o] * Lh * pre;
+        unsafeAtomicAdd(lbar + p.fk, sb * (fs ? Lh : 1) * vr);
+        unsafeAtomicAdd(rbar + p.fk, -sb * (fs ? Rh : 1) * vl);
+      }
     }
     for (int x = colp[gc]; x < colp[gc + 1]; ++x) {
       const Partner p = part[x];
@@ -473,18 +406,34 @@ __global__ __launch_bounds__(256) void k_122_vjp_1b(DevBasis B, const SegTask* t
       double pre = ((p.code >> 26) & 1) ? 1.4142135623730951 : 1.0;
       if ((p.code >> 25) & 1) pre = -pre;
       pre = pre_ab * pre;
-      double vr, vl;
-      if (rc <= cl) {
-        const i64 o = go[rc == 0 ? cl : (rc == 1 ? (cl == 1 ? 3 : 5) : 4)] + IX + (i64)idx * dm[rc];
-        vr = R[o] * pre; vl = L[o] * pre;
-      } else {
-        const i64 o = go[cl == 0 ? rc : (cl == 1 ? (rc == 1 ? 3 : 5) : 4)] + idx + (i64)IX * dm[cl];
-        vr = R[o] * Rh * pre; vl = L[o] * Lh * pre;
-      }
+      const bool lo = rc <= cl;
+      const i64 o = lo ? go[rc == 0 ? cl : (rc == 1 ? (cl == 1 ? 3 : 5) : 4)] + IX + (i64)idx * dm[rc]
+                       : go[cl == 0 ? rc : (cl == 1 ? (rc == 1 ? 3 : 5) : 4)] + idx + (i64)IX * dm[cl];
+      const double v = lo ? cbar[o] * pre : cbar[o] * ch * pre;
       const bool fs = (p.code >> 24) & 1;
-      const int k = p.fk;
-      unsafeAtomicAdd(lbar + k, -sb * (fs ? Lh : 1) * vr);
-      unsafeAtomicAdd(rbar + k, sb * (fs ? Rh : 1) * vl);
+      y1 -= fR[p.fk] * (fs ? Rh : 1) * v;
+      y2 -= fL[p.fk] * (fs ? Lh : 1) * v;
+      if (sb != 0.0) {
+        const double vr = lo ? R[o] * pre : R[o] * Rh * pre, vl = lo ? L[o] * pre : L[o] * Lh * pre;
+        unsafeAtomicAdd(lbar + p.fk, -sb * (fs ? Lh : 1) * vr);
+        unsafeAtomicAdd(rbar + p.fk, sb * (fs ? Rh : 1) * vl);
+      }
+    }
+    y1 *= nrm;
+    y2 *= nrm;
+    const i64 o = go[g] + IX + (i64)JX * n1;
+    if (!sq) {
+      Lb[o] += -2.0 * Rh * y1;
+      Rb[o] += 2.0 * Lh * y2;
+    } else if (IX == JX) {
+      Lb[o] += -Rh * Lh * y1;
+      Rb[o] += Lh * Rh * y2;
+    } else {
+      const i64 ol = go[g] + JX + (i64)IX * n1;
+      Lb[o] += -Rh * y1;
+      Lb[ol] += -Rh * Lh * y1;
+      Rb[o] += Lh * y2;
+      Rb[ol] += Lh * Rh * y2;
     }
   }
   __syncthreads();
@@ -494,13 +443,13 @@ __global__ __launch_bounds__(256) void k_122_vjp_1b(DevBasis B, const SegTask* t
   }
 }
 
-void launch_122_vjp_1b(const DevBasis& B, const SegTask* t, int nt, const Partner* part, const int* rowp,
-                       const int* colp, const int* f1off, int nf1, const double* L, int Lh, const double* R, int Rh,
-                       const double* RESb, int RESh, double* Lb, double* Rb, hipStream_t s) {
-  if (nt <= 0 || nf1 <= 0) return;
+void launch_122_vjp(const DevBasis& B, const SegTask* t, int nt, const Partner* part, const int* rowp, const int* colp,
+                    const int* f1off, int nf1, const double* L, int Lh, const double* R, int Rh, const double* cbar,
+                    const double* RESb, int RESh, double* Lb, double* Rb, hipStream_t s) {
+  if (nt <= 0) return;
   const int grid = nt < 2048 ? nt : 2048;
-  hipLaunchKernelGGL(k_122_vjp_1b, dim3(grid), dim3(256), 2 * nf1 * sizeof(double), s, B, t, nt, part, rowp, colp,
-                     f1off, nf1, L, Lh, R, Rh, RESb, RESh, Lb, Rb);
+  hipLaunchKernelGGL(k_122_vjp, dim3(grid), dim3(256), 4 * nf1 * sizeof(double), s, B, t, nt, part, rowp, colp, f1off,
+                     nf1, L, Lh, R, Rh, cbar, RESb, RESh, Lb, Rb);
 }
 
 // -------------------------------------------------- 222_ph transpose
