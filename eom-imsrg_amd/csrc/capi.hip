@@ -82,6 +82,7 @@ struct imsrg_basis_s {
   std::vector<void*> allocs;
   // task lists (grouped by channel / cross-coupled block)
   TaskList<SegTask> seg_tri, seg_full, seg_g3, seg_epi;
+  TaskList<SegTask> seg_pv;         // reverse mode: the blocks a Pandya term reads (gam 2, 3, 4, 6)
   TaskList<CCTask> cc;
   TaskList<PairTask> pairs;
   TaskList<PairTask> symt;          // 64 x 64 tile pairs of every square block (pad = g)
@@ -928,6 +929,11 @@ int ensure_adj(imsrg_basis_s* B) {
     }
     B->phb.group_end.push_back((int)B->phb.probs_h.size());
   }
+  B->seg_pv.host.clear();
+  for (const SegTask& t : B->seg_full.host)
+    if (t.g != 0 && t.g != 4) B->seg_pv.host.push_back(t);
+  B->seg_pv.n = (int)B->seg_pv.host.size();
+  if ((rc = upload(B, B->seg_pv.host, &B->seg_pv.dev))) return rc;
   // 222_ph transpose: lanes, tiles and the terms of each tile
   {
     B->phq_h.clear(); B->phqs_h.clear(); B->phl_h.clear(); B->phtile_h.clear();
@@ -1061,8 +1067,8 @@ void comm_vjp(imsrg_basis_s* B, const double* L, int Lh, const double* R, int Rh
   if (B->phb.tasks.n)
     launch_gemm_tiled(B->phb.tile, sc, B->phb.probs, B->phb.tasks.dev, B->phb.tasks.grid, B->phb.tasks.xmap, s,
                       (double)Lh, -(double)(Rh * Lh), 1, B->phb.sig);
-  launch_pandya_vjp(D, B->cc.dev, B->cc.n, B->aRCCb, Rh, Rb, s);
-  if (!LCCb_acc) launch_pandya_vjp(D, B->cc.dev, B->cc.n, B->aLCCb, Lh, Lb, s);
+  launch_pandya_vjp_gather(D, B->seg_pv.dev, B->seg_pv.n, B->aRCCb, Rh, Rb, s);
+  if (!LCCb_acc) launch_pandya_vjp_gather(D, B->seg_pv.dev, B->seg_pv.n, B->aLCCb, Lh, Lb, s);
 }
 
 }  // namespace
@@ -1576,7 +1582,7 @@ int bch_reverse(imsrg_flow_s* F, const double* G, int Hh, const TermRec& rec, co
     launch_axpy(n, F->Prev, 1.0, HSb, rec.c[k - 1], F->Ab, s);
   }
   launch_axpy(n, Hb, 1.0, F->Ab, 1.0, Hb, s);
-  if (K > 0) launch_pandya_vjp(B->D, B->cc.dev, B->cc.n, F->GCCb, -1, Gb, s);
+  if (K > 0) launch_pandya_vjp_gather(B->D, B->seg_pv.dev, B->seg_pv.n, F->GCCb, -1, Gb, s);
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -1649,7 +1655,7 @@ int magnus_reverse(imsrg_flow_s* F, const double* G, const TermRec& rec, const d
   }
   launch_axpy(n, ETAb, 1.0, F->Ab, 1.0, ETAb, s);
   launch_add_e0(ETAb, DGb, e0sum, s);
-  if (K > 0) launch_pandya_vjp(B->D, B->cc.dev, B->cc.n, F->GCCb, -1, Gb, s);
+  if (K > 0) launch_pandya_vjp_gather(B->D, B->seg_pv.dev, B->seg_pv.n, F->GCCb, -1, Gb, s);
   HIPCHK(hipGetLastError());
   return 0;
 }

@@ -208,8 +208,10 @@ void launch_ph_vjp(const DevBasis& B, const QuadItem* items, int nitems, const i
 void launch_122_vjp(const DevBasis& B, const SegTask* t, int nt, const Partner* part, const int* rowp, const int* colp,
                     const int* f1off, int nf1, const double* L, int Lh, const double* R, int Rh, const double* cbar,
                     const double* RESb, int RESh, double* Lb, double* Rb, hipStream_t s);
-void launch_pandya_vjp(const DevBasis& B, const CCTask* t, int nt, const double* CCb, int herm, double* opb,
-                       hipStream_t s);
+// the same transpose gathered per stored element over tasks of the blocks
+// a Pandya term reads (no atomics)
+void launch_pandya_vjp_gather(const DevBasis& B, const SegTask* t, int nt, const double* CCb, int herm, double* opb,
+                              hipStream_t s);
 void launch_gen_vjp(const DevBasis& B, const SegTask* g3, int ng3, const double* Hd, const double* Hn,
                     const double* M, const double* ETAb, double* Hdb, double* Hnb, double* Mb, int mode,
                     hipStream_t s);

@@ -552,36 +552,77 @@ void launch_ph_vjp(const DevBasis& B, const QuadItem* items, int nitems, const i
   hipLaunchKernelGGL(k_ph_tile, dim3(ntiles), dim3(256), 0, s, B, tiles, tstart, terms, CV, -(double)(Lh * Rh), U);
 }
 
-// Pandya transpose: CCb -> opb (calculate_cross_coupled read positions)
-__global__ __launch_bounds__(256) void k_pandya_vjp(DevBasis B, const CCTask* tasks, const double* CCb, int herm,
-                                                    double* opb) {
-  const CCTask t = tasks[blockIdx.x];
-  const int q1 = t.q1;
-  const int r = B.rlen[q1], nc = B.ncol[q1];
-  const i64 e = (i64)t.start + threadIdx.x;
-  if (e >= (i64)r * nc) return;
-  const int Rr = (int)(e % r), NB = (int)(e / r);
-  const double cb = CCb[B.ccoff[q1] + Rr + (i64)NB * r];
-  if (cb == 0.0) return;
-  int a, b, p, h;
-  unpack(B.ccrows[B.ccrowoff[q1] + Rr], a, b);
-  unpack(B.cccols[B.cccoloff[q1] + NB], p, h);
-  const int JC = B.ccJ[q1];
+
+// calculate_cross_coupled transpose by gathering (no atomics, fixed order):
+// each stored two-body element that some Pandya term reads sums the
+// cotangents of the cross-coupled elements reading it.  A term of CC element
+// (JC; row (a,b), column (p,h)) at 2J' reads v_elem(h,a,p,b,J'), stored where
+// vpos puts it, so the readers of position o of channel J' are the orbit
+// assignments (h,a) / (p,b) from its row pair / column pair (either order,
+// either orientation) that vpos maps to o, with h a hole and p a particle;
+// blocks pppp and hhhh are never read.  Every reader adds, over JC,
+//   CCb * (-1)^((jh+jb+JC)/2) sqrt(JC+1) (-1)^(J'/2) (J'+1) {jp jh JC; ja jb J'}
+// times vpos's phase (and herm when stored transposed), as the forward adds
+// the element into the CC element.
+__device__ __forceinline__ double pandya_reader(const DevBasis& B, const double* CCb, int h, int a, int p, int b,
+                                                int Jp) {
+  const int N = B.N, nJ = B.nJ, JTM = B.JTM;
   const int ja = B.jj[a], jb = B.jj[b], jp = B.jj[p], jh = B.jj[h];
-  const int la = B.ll[a], lb = B.ll[b], lp = B.ll[p], lh = B.ll[h];
-  const int ta = B.tz[a], tb = B.tz[b], tp = B.tz[p], th = B.tz[h];
-  if (!((((la + lh) & 1) == ((lb + lp) & 1)) && (ta + th == tb + tp))) return;
-  const double c = cb * m1pow((jh + jb + JC) / 2) * sqrt(JC + 1.0);
-  const int Jmin = max(abs(jp - jb), abs(ja - jh));
-  const int Jmax = min(jp + jb, ja + jh);
-  for (int JT = Jmin; JT <= Jmax; JT += 2)
-    vadj(vpos(B, h, a, p, b, JT), opb, herm, c * (double)(m1pow(JT / 2) * (JT + 1)) * sixj(B, jp, jh, JC, ja, jb, JT));
+  const int TZ = abs(B.tz[a] - B.tz[b]) / 2, PAR = (B.ll[a] + B.ll[b]) & 1;
+  if (abs(B.tz[p] - B.tz[h]) / 2 != TZ || ((B.ll[p] + B.ll[h]) & 1) != PAR) return 0.0;
+  const int JCmin = max(abs(ja - jb), abs(jp - jh)), JCmax = min(ja + jb, jp + jh);
+  double sm = 0.0;
+  for (int JC = JCmin; JC <= JCmax; JC += 2) {
+    const int r = B.rmap[((i64)a * N + b) * nJ + (JC >> 1)];
+    const int c = B.nbmap[((i64)p * N + h) * nJ + (JC >> 1)];
+    if (r < 0 || c < 0) continue;
+    const int q1 = (JC >> 1) + TZ * (JTM + 1) + 2 * PAR * (JTM + 1);
+    const double cb = CCb[B.ccoff[q1] + r + (i64)c * B.rlen[q1]];
+    sm = sm + cb * m1pow((jh + jb + JC) / 2) * sqrt(JC + 1.0) * sixj(B, jp, jh, JC, ja, jb, Jp);
+  }
+  return sm * (double)(m1pow(Jp / 2) * (Jp + 1));
 }
 
-void launch_pandya_vjp(const DevBasis& B, const CCTask* t, int nt, const double* CCb, int herm, double* opb,
-                       hipStream_t s) {
+__global__ __launch_bounds__(256) void k_pandya_vjp_gather(DevBasis B, const SegTask* tasks, const double* CCb,
+                                                           int herm, double* opb) {
+  const SegTask t = tasks[blockIdx.x];
+  const i64 e = (i64)t.start + threadIdx.x;
+  if (e >= t.cnt) return;
+  const int q = t.q, g = t.g;
+  const int n1 = brows(B, q, g);
+  const int IX = (int)(e % n1), JX = (int)(e / n1);
+  int x1, x2, y1, y2;
+  unpack(B.pairs[B.pairoff[q * 3 + row_cls(g)] + IX], x1, x2);
+  unpack(B.pairs[B.pairoff[q * 3 + col_cls(g)] + JX], y1, y2);
+  const int Jp = B.chJ[q];
+  const i64 o = B.goff[q * 6 + g] + e;
+  // the eight assignments (h,a,p,b); repeats (x1 = x2, y1 = y2, or the row
+  // pair equal to the column pair) are counted once
+  const int H[8] = {x1, x2, x1, x2, y1, y2, y1, y2};
+  const int A[8] = {x2, x1, x2, x1, y2, y1, y2, y1};
+  const int Pp[8] = {y1, y1, y2, y2, x1, x1, x2, x2};
+  const int Bb[8] = {y2, y2, y1, y1, x2, x2, x1, x1};
+  double tot = 0.0;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const int h = H[k], a = A[k], p = Pp[k], b = Bb[k];
+    if (B.con[h] != 1 || B.con[p] != 0) continue;
+    bool dup = false;
+#pragma unroll
+    for (int m = 0; m < k; ++m) dup = dup || (H[m] == h && A[m] == a && Pp[m] == p && Bb[m] == b);
+    if (dup) continue;
+    const VPos vp = vpos(B, h, a, p, b, Jp);
+    if (vp.off != o) continue;
+    const double v = pandya_reader(B, CCb, h, a, p, b, Jp);
+    tot = tot + (vp.swp ? v * herm : v) * vp.pre;
+  }
+  if (tot != 0.0) opb[o] += tot;
+}
+
+void launch_pandya_vjp_gather(const DevBasis& B, const SegTask* t, int nt, const double* CCb, int herm, double* opb,
+                              hipStream_t s) {
   if (nt <= 0) return;
-  hipLaunchKernelGGL(k_pandya_vjp, dim3(nt), dim3(kTile), 0, s, B, t, CCb, herm, opb);
+  hipLaunchKernelGGL(k_pandya_vjp_gather, dim3(nt), dim3(256), 0, s, B, t, CCb, herm, opb);
 }
 
 // ------------------------------------------------- white / normal order
