@@ -25,7 +25,14 @@
 //     with the production-order routines below except f_elem / v_elem;
 //   * 6j / 9j closed forms, symmetries and orthogonality (tests);
 //   * BCH closed forms (Omega = 0; one-body Omega = orbital rotation), the
-//     Magnus identity dOmega = eta for Omega = 0, antisymmetry of [A,B].
+//     Magnus identity dOmega = eta for Omega = 0, antisymmetry of [A,B];
+//   * the generators (white, atan, atanxx) against closed-form Epstein-Nesbet
+//     denominators, w2 as the composition of pinned parts and its
+//     Moller-Plesset small-coupling limit, the three flows' common E0;
+//   * vectorize / repackage against the reference's own neq count, the
+//     visiting order read off an index-valued operator, and round trips;
+//   * every transpose (imsrg_adjoint.inc) against central differences of
+//     its forward routine, including the whole flow gradient.
 // The reference itself is unbuildable here (it needs an external BLAS/LAPACK
 // library that the image lacks), so there is no oracle/_ref.
 // ============================================================================
