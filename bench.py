@@ -36,7 +36,7 @@ def _parse():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-grad", action="store_true")
-    ap.add_argument("--grad-check", type=int, default=20,
+    ap.add_argument("--grad-check", type=int, default=200,
                     help="central-difference check of dE0/dV on this many seeded two-body elements")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-instance eMax 6 batch")
     ap.add_argument("--batch-cpu-check", type=int, default=1,
