@@ -1,8 +1,9 @@
 // HIP kernels of the IMSRG(2) scalar-commutator path for gfx950 (CDNA4).
 //
-//  * k_comm122, k_comm222ph_scatter, k_pandya: per-element gathers, one
+//  * k_comm122p, k_comm222ph_scatter, k_pandya: per-element gathers, one
 //    thread per output element, 256-element tiles from a task list that
-//    spans every channel (grouped) or one channel (plain path).
+//    spans every channel (grouped) or one channel (plain path);
+//    k_comm222ph_chunk: the grouped inverse Pandya by quadruple chunks.
 //  * the FP64 MFMA GEMMs live in gemm.hip; k_pppp_sym applies the
 //    reference's  W - pm W^T  epilogue of 222_pp_hh to the pppp product.
 //  * one-body terms, norms, axpy, generator, mbpt2 and normal ordering.
@@ -14,74 +15,12 @@
 namespace imsrg {
 
 // ============================================================ commutator_122
-// commutators.f90:283-396.  RES two-body = 1b x 2b part (assigned).
-__global__ __launch_bounds__(256) void k_comm122(DevBasis B, const SegTask* tasks, const double* L, int Lh,
-                                                 const double* R, int Rh, double* RES, int RESh) {
-  int inst;
-  if (!batch_inst(B, inst)) return;
-  L += (i64)inst * B.oplen;
-  R += (i64)inst * B.oplen;
-  RES += (i64)inst * B.oplen;
-  const SegTask t = tasks[blockIdx.x];
-  const i64 e = (i64)t.start + threadIdx.x;
-  if (e >= t.cnt) return;
-  const int q = t.q, g = t.g;
-  const int n1 = brows(B, q, g);
-  const bool sq = is_square(g);
-  int IX, JX;
-  if (sq) upper_decode(e, IX, JX);
-  else { IX = (int)(e % n1); JX = (int)(e / n1); }
-  int a, b, c, d;
-  unpack(B.pairs[B.pairoff[q * 3 + row_cls(g)] + IX], a, b);
-  unpack(B.pairs[B.pairoff[q * 3 + col_cls(g)] + JX], c, d);
-  const int JT = B.chJ[q];
-  double sm = 0.0;
-  {
-    const int k = B.spblk[a];
-    for (int x = B.spstart[k]; x < B.spstart[k + 1]; ++x) {
-      const int i = B.spmem[x];
-      const VPos p = vpos(B, i, b, c, d, JT);
-      sm = sm + f_elem(B, L, Lh, a, i) * vat(p, R, Rh) - f_elem(B, R, Rh, a, i) * vat(p, L, Lh);
-    }
-  }
-  {
-    const int k = B.spblk[b];
-    for (int x = B.spstart[k]; x < B.spstart[k + 1]; ++x) {
-      const int i = B.spmem[x];
-      const VPos p = vpos(B, a, i, c, d, JT);
-      sm = sm + f_elem(B, L, Lh, b, i) * vat(p, R, Rh) - f_elem(B, R, Rh, b, i) * vat(p, L, Lh);
-    }
-  }
-  {
-    const int k = B.spblk[c];
-    for (int x = B.spstart[k]; x < B.spstart[k + 1]; ++x) {
-      const int i = B.spmem[x];
-      const VPos p = vpos(B, a, b, i, d, JT);
-      sm = sm - f_elem(B, L, Lh, i, c) * vat(p, R, Rh) + f_elem(B, R, Rh, i, c) * vat(p, L, Lh);
-    }
-  }
-  {
-    const int k = B.spblk[d];
-    for (int x = B.spstart[k]; x < B.spstart[k + 1]; ++x) {
-      const int i = B.spmem[x];
-      const VPos p = vpos(B, a, b, c, i, JT);
-      sm = sm - f_elem(B, L, Lh, i, d) * vat(p, R, Rh) + f_elem(B, R, Rh, i, d) * vat(p, L, Lh);
-    }
-  }
-  sm = sm / sqrt(1.0 + (a == b ? 1.0 : 0.0)) / sqrt(1.0 + (c == d ? 1.0 : 0.0));
-  const i64 base = B.goff[q * 6 + g];
-  RES[base + IX + (i64)JX * n1] = sm;
-  if (sq) RES[base + JX + (i64)IX * n1] = sm * RESh;
-}
-
-void launch_comm122(const DevBasis& B, const SegTask* t, int nt, const double* L, int Lh, const double* R,
-                    int Rh, double* RES, int RESh, hipStream_t s) {
-  if (nt <= 0) return;
-  hipLaunchKernelGGL(k_comm122, dim3(nt, B.ninst), dim3(kTile), 0, s, B, t, L, Lh, R, Rh, RES, RESh);
-}
-
-// commutator_122 with precomputed partner lists: the same sums as k_comm122
-// (a-, b-, c-, d-replaced, in that order) without per-term index searches.
+// commutators.f90:283-396.  RES two-body = 1b x 2b part (assigned): for
+// every stored element (ab,cd) at J, the sums over the (l,j,tz) block
+// partners i of a, b, c, d of f(L)*v(R) - f(R)*v(L), then divided by
+// sqrt(1+delta_ab) sqrt(1+delta_cd).
+// The partner lists are precomputed per stored pair (a-, b-, c-, d-replaced,
+// in that order), so no term performs an index search.
 __device__ __forceinline__ int blk_of(int lo, int hi) { return lo == 0 ? hi : (lo == 1 ? (hi == 1 ? 3 : 5) : 4); }
 
 __global__ __launch_bounds__(256) void k_comm122p(DevBasis B, const SegTask* tasks, const Partner* part,
@@ -1025,16 +964,6 @@ __global__ __launch_bounds__(1024) void k_sum_sqrt(const double* part, int n, do
   if (threadIdx.x == 0) dst[0] = do_sqrt ? sqrt(s) : s;
 }
 
-void launch_norm_partials(const double* op, const i64* chunk_off, const int* chunk_len, const double* chunk_w,
-                          int nchunk, double* part, hipStream_t s) {
-  if (nchunk <= 0) return;
-  hipLaunchKernelGGL(k_norm_partials, dim3(nchunk), dim3(256), 0, s, op, chunk_off, chunk_len, chunk_w, part, (i64)0,
-                     (i64)0, (const int*)nullptr);
-}
-void launch_sum_sqrt(const double* part, int n, double* dst, int do_sqrt, hipStream_t s) {
-  hipLaunchKernelGGL(k_sum_sqrt, dim3(1), dim3(1024), 0, s, part, n, dst, do_sqrt, (i64)0, (i64)0,
-                     (const int*)nullptr);
-}
 void launch_norm_partials_b(const DevBasis& B, const double* op, const i64* chunk_off, const int* chunk_len,
                             const double* chunk_w, int nchunk, double* part, hipStream_t s) {
   if (nchunk <= 0) return;
@@ -1103,9 +1032,6 @@ void launch_gen_1b(const DevBasis& B, const double* Hd, const double* Hn, double
   const i64 n = (i64)B.A * B.A + (i64)B.P * B.P + (i64)B.P * B.A;
   hipLaunchKernelGGL(k_gen_1b, dim3((unsigned)((n + 255) / 256), B.ninst), dim3(256), 0, s, B, Hd, Hn, ETA, mode);
 }
-void launch_white_1b(const DevBasis& B, const double* H, double* ETA, hipStream_t s) {
-  launch_gen_1b(B, H, H, ETA, 0, s);
-}
 
 // two-body part, full-segment tasks (gam3 from the formula, the rest zero);
 // M = the monopoles of Hd
@@ -1143,10 +1069,6 @@ void launch_gen_2b(const DevBasis& B, const SegTask* t, int nt, const double* Hd
                    double* ETA, int mode, hipStream_t s) {
   if (nt <= 0) return;
   hipLaunchKernelGGL(k_gen_2b, dim3(nt, B.ninst), dim3(256), 0, s, B, t, Hd, M, Hn, ETA, mode);
-}
-void launch_white_2b(const DevBasis& B, const SegTask* t, int nt, const double* H, const double* M, double* ETA,
-                     hipStream_t s) {
-  launch_gen_2b(B, t, nt, H, M, H, ETA, 0, s);
 }
 
 // mbpt2, basic_IMSRG.f90:4165-4199: one partial per gam3 tile

@@ -106,8 +106,6 @@ struct TaskList {
 };
 
 // launchers (all asynchronous on `s`)
-void launch_comm122(const DevBasis& B, const SegTask* t, int nt, const double* L, int Lh, const double* R,
-                    int Rh, double* RES, int RESh, hipStream_t s);
 void launch_comm222ph_scatter(const DevBasis& B, const SegTask* t, int nt, const double* W, int Lh, int Rh,
                               double* RES, int RESh, hipStream_t s);
 // square blocks' lower triangles from their upper triangles (herm)
@@ -154,9 +152,6 @@ void launch_pphh_epilogue(const DevBasis& B, const SegTask* t, int nt, const dou
                           int Rh, double* RES, double* w1, double* w2, const double* Pscr, const i64* poff,
                           const int* k3ks, const double* K2scr, const i64* k2off, const int* k2ks, hipStream_t s);
 void launch_axpy(i64 n, const double* A, double ax, const double* B, double bx, double* C, hipStream_t s);
-void launch_norm_partials(const double* op, const i64* chunk_off, const int* chunk_len, const double* chunk_w,
-                          int nchunk, double* part, hipStream_t s);
-void launch_sum_sqrt(const double* part, int n, double* dst, int do_sqrt, hipStream_t s);
 // many-instance forms: B.ninst instances back to back, masked by B.act
 // (axpy / fill / copy: stride n; norms: operand stride oplen, partials
 // stride spart, results stride sdst; e0_final: partials spart, dst oplen)
@@ -177,9 +172,6 @@ void launch_gen_1b(const DevBasis& B, const double* Hd, const double* Hn, double
 void launch_gen_2b(const DevBasis& B, const SegTask* t, int nt, const double* Hd, const double* M, const double* Hn,
                    double* ETA, int mode, hipStream_t s);
 void launch_fill_e0(const DevBasis& B, double* op, double v, hipStream_t s);
-void launch_white_1b(const DevBasis& B, const double* H, double* ETA, hipStream_t s);
-void launch_white_2b(const DevBasis& B, const SegTask* t, int nt, const double* H, const double* M, double* ETA,
-                     hipStream_t s);
 void launch_mbpt2_partials(const DevBasis& B, const SegTask* t, int nt, const double* H, double* part,
                            hipStream_t s);
 void launch_normal_order(const DevBasis& B, double* H, double* scratch, hipStream_t s);
@@ -199,8 +191,6 @@ void launch_pphh_wbar(const DevBasis& B, const SegTask* t, int nt, const double*
                       double* Wb2, hipStream_t s);
 void launch_to_dense(const DevBasis& B, const SegTask* t, int nt, const double* X, int herm, double* cbar,
                      hipStream_t s);
-void launch_from_dense(const DevBasis& B, const SegTask* t, int nt, const double* Y, double alpha, double* dst,
-                       hipStream_t s);
 void launch_ph_vjp(const DevBasis& B, const QuadItem* items, int nitems, const int* lanes, int nl, const int* quads,
                    const int* qstart, const PairTask* tiles, int ntiles, const int* tstart, const int2* terms,
                    const double* RESb, int RESh, int Lh, int Rh, double* SB, double* CV, double* U, hipStream_t s);

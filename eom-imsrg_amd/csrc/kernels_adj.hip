@@ -303,26 +303,12 @@ __global__ __launch_bounds__(256) void k_to_dense(DevBasis B, const SegTask* tas
   cbar[base + e] = v;
 }
 
-// dst += alpha * Y on square blocks, 2 alpha * Y on rectangular blocks
-__global__ __launch_bounds__(256) void k_from_dense(DevBasis B, const SegTask* tasks, const double* Y, double alpha,
-                                                    double* dst) {
-  const SegTask t = tasks[blockIdx.x];
-  const i64 e = (i64)t.start + threadIdx.x;
-  if (e >= t.cnt) return;
-  const i64 o = B.goff[t.q * 6 + t.g] + e;
-  dst[o] += (is_square(t.g) ? alpha : 2.0 * alpha) * Y[o];
-}
-
 void launch_to_dense(const DevBasis& B, const SegTask* t, int nt, const double* X, int herm, double* cbar,
                      hipStream_t s) {
   if (nt <= 0) return;
   hipLaunchKernelGGL(k_to_dense, dim3(nt), dim3(256), 0, s, B, t, X, herm, cbar);
 }
-void launch_from_dense(const DevBasis& B, const SegTask* t, int nt, const double* Y, double alpha, double* dst,
-                       hipStream_t s) {
-  if (nt <= 0) return;
-  hipLaunchKernelGGL(k_from_dense, dim3(nt), dim3(256), 0, s, B, t, Y, alpha, dst);
-}
+
 
 
 
