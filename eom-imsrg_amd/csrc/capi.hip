@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <initializer_list>
 #include <map>
 #include <string>
 #include <tuple>
@@ -26,7 +27,17 @@ int fail(int code, const std::string& msg) {
   g_err = msg;
   return code;
 }
+// Argument check of the operator entry points: every pointer set, every
+// hermiticity flag +1 (hermitian) or -1 (anti-hermitian).
+int check_args(std::initializer_list<const void*> ptrs, std::initializer_list<int> herms) {
+  for (const void* p : ptrs)
+    if (!p) return fail(IMSRG_EINVAL, "null argument");
+  for (int h : herms)
+    if (h != 1 && h != -1) return fail(IMSRG_EINVAL, "herm must be +1 or -1");
+  return 0;
+}
 }  // namespace
+
 
 #define HIPCHK(x)                                                                        \
   do {                                                                                   \
@@ -2079,6 +2090,7 @@ int imsrg_set_path(imsrg_basis B, int path) {
 
 int imsrg_commutator_110(imsrg_basis B, const double* L, int Lh, const double* R, int Rh, double* value,
                          imsrg_stream stream) {
+  if (int a_ = check_args({L, R, value}, {Lh, Rh})) return a_;
   PREP();
   launch_e0_partials(B->D, L, Lh, R, Rh, B->part, s);
   launch_e0_final(B->part, B->b.nblocks, 0, 1, B->dval, 0, s);
@@ -2088,6 +2100,7 @@ int imsrg_commutator_110(imsrg_basis B, const double* L, int Lh, const double* R
 
 int imsrg_commutator_220(imsrg_basis B, const double* L, int Lh, const double* R, int Rh, double* value,
                          imsrg_stream stream) {
+  if (int a_ = check_args({L, R, value}, {Lh, Rh})) return a_;
   PREP();
   launch_e0_partials(B->D, L, Lh, R, Rh, B->part, s);
   launch_e0_final(B->part, B->b.nblocks, (Lh * Rh == -1) ? 1 : 0, 0, B->dval, 0, s);
@@ -2097,6 +2110,7 @@ int imsrg_commutator_220(imsrg_basis B, const double* L, int Lh, const double* R
 
 int imsrg_commutator_e0(imsrg_basis B, const double* L, int Lh, const double* R, int Rh, double* RES,
                         imsrg_stream stream) {
+  if (int a_ = check_args({L, R, RES}, {Lh, Rh})) return a_;
   PREP();
   t_e0(basis_exec(B), L, Lh, R, Rh, RES, s);
   HIPCHK(hipGetLastError());
@@ -2105,6 +2119,7 @@ int imsrg_commutator_e0(imsrg_basis B, const double* L, int Lh, const double* R,
 
 int imsrg_commutator_111(imsrg_basis B, const double* L, int Lh, const double* R, int Rh, double* RES, int RESh,
                          imsrg_stream stream) {
+  if (int a_ = check_args({L, R, RES}, {Lh, Rh})) return a_;
   PREP();
   (void)RESh;
   t_111(basis_exec(B), L, Lh, R, Rh, RES, s);
@@ -2114,6 +2129,7 @@ int imsrg_commutator_111(imsrg_basis B, const double* L, int Lh, const double* R
 
 int imsrg_commutator_121(imsrg_basis B, const double* L, int Lh, const double* R, int Rh, double* RES, int RESh,
                          imsrg_stream stream) {
+  if (int a_ = check_args({L, R, RES}, {Lh, Rh, RESh})) return a_;
   PREP();
   t_121(basis_exec(B), L, Lh, R, Rh, RES, RESh, s);
   HIPCHK(hipGetLastError());
@@ -2122,6 +2138,7 @@ int imsrg_commutator_121(imsrg_basis B, const double* L, int Lh, const double* R
 
 int imsrg_commutator_122(imsrg_basis B, const double* L, int Lh, const double* R, int Rh, double* RES, int RESh,
                          imsrg_stream stream) {
+  if (int a_ = check_args({L, R, RES}, {Lh, Rh, RESh})) return a_;
   PREP();
   t_122(basis_exec(B), L, Lh, R, Rh, RES, RESh, s);
   HIPCHK(hipGetLastError());
@@ -2130,6 +2147,7 @@ int imsrg_commutator_122(imsrg_basis B, const double* L, int Lh, const double* R
 
 int imsrg_commutator_222_pp_hh(imsrg_basis B, const double* L, int Lh, const double* R, int Rh, double* RES,
                                int RESh, double* w1, double* w2, imsrg_stream stream) {
+  if (int a_ = check_args({L, R, RES, w1, w2}, {Lh, Rh, RESh})) return a_;
   PREP();
   t_pphh(basis_exec(B), L, Lh, R, Rh, RES, RESh, w1, w2, s);
   HIPCHK(hipGetLastError());
@@ -2138,6 +2156,7 @@ int imsrg_commutator_222_pp_hh(imsrg_basis B, const double* L, int Lh, const dou
 
 int imsrg_commutator_221(imsrg_basis B, const double* L, int Lh, const double* R, int Rh, double* RES, int RESh,
                          const double* w1, const double* w2, imsrg_stream stream) {
+  if (int a_ = check_args({L, R, RES, w1, w2}, {Lh, Rh, RESh})) return a_;
   PREP();
   t_221(basis_exec(B), L, Lh, R, Rh, RES, RESh, w1, w2, s);
   HIPCHK(hipGetLastError());
@@ -2145,6 +2164,7 @@ int imsrg_commutator_221(imsrg_basis B, const double* L, int Lh, const double* R
 }
 
 int imsrg_calculate_cross_coupled(imsrg_basis B, const double* op, int herm, double* CC, imsrg_stream stream) {
+  if (int a_ = check_args({op, CC}, {herm})) return a_;
   PREP();
   t_pandya(basis_exec(B), op, herm, CC, s);
   HIPCHK(hipGetLastError());
@@ -2153,6 +2173,7 @@ int imsrg_calculate_cross_coupled(imsrg_basis B, const double* op, int herm, dou
 
 int imsrg_commutator_222_ph(imsrg_basis B, const double* LCC, int Lh, const double* RCC, int Rh, double* RES,
                             int RESh, double* WCC, imsrg_stream stream) {
+  if (int a_ = check_args({LCC, RCC, RES, WCC}, {Lh, Rh, RESh})) return a_;
   PREP();
   t_ph(basis_exec(B), LCC, Lh, RCC, Rh, RES, RESh, WCC, s);
   HIPCHK(hipGetLastError());
@@ -2161,6 +2182,7 @@ int imsrg_commutator_222_ph(imsrg_basis B, const double* LCC, int Lh, const doub
 
 int imsrg_add_sq_op(imsrg_basis B, const double* A, double ax, const double* Bv, double bx, double* C,
                     imsrg_stream stream) {
+  if (int a_ = check_args({A, Bv, C}, {})) return a_;
   PREP();
   launch_axpy(B->b.oplen, A, ax, Bv, bx, C, s);
   HIPCHK(hipGetLastError());
@@ -2168,6 +2190,7 @@ int imsrg_add_sq_op(imsrg_basis B, const double* A, double ax, const double* Bv,
 }
 
 int imsrg_mat_frob_norm(imsrg_basis B, const double* op, double* value, imsrg_stream stream) {
+  if (int a_ = check_args({op, value}, {})) return a_;
   PREP();
   t_norm(basis_exec(B), op, B->dval, s);
   HIPCHK(hipGetLastError());
@@ -2175,25 +2198,26 @@ int imsrg_mat_frob_norm(imsrg_basis B, const double* op, double* value, imsrg_st
 }
 
 int imsrg_build_gs_white(imsrg_basis B, const double* H, double* ETA, imsrg_stream stream) {
+  if (int a_ = check_args({H, ETA}, {})) return a_;
   PREP();
   return white(basis_exec(B), H, ETA, s);
 }
 
 int imsrg_build_gs_atan(imsrg_basis B, const double* H, double* ETA, imsrg_stream stream) {
-  PREP();
   if (!H || !ETA) return fail(IMSRG_EINVAL, "null argument");
+  PREP();
   return gen(basis_exec(B), H, H, ETA, 1, s);
 }
 
 int imsrg_build_gs_atanxx(imsrg_basis B, const double* Hold, const double* H, double* ETA, imsrg_stream stream) {
-  PREP();
   if (!Hold || !H || !ETA) return fail(IMSRG_EINVAL, "null argument");
+  PREP();
   return gen(basis_exec(B), Hold, H, ETA, 1, s);
 }
 
 int imsrg_vectorize(imsrg_basis B, const double* op, double* vec, imsrg_stream stream) {
-  PREP();
   if (!op || !vec) return fail(IMSRG_EINVAL, "null argument");
+  PREP();
   int rc;
   if ((rc = ensure_vec(B))) return rc;
   launch_vectorize(B->vec, (int)B->vec_h.size(), (double*)op, vec, 0, 1, s);
@@ -2202,9 +2226,9 @@ int imsrg_vectorize(imsrg_basis B, const double* op, double* vec, imsrg_stream s
 }
 
 int imsrg_repackage(imsrg_basis B, const double* vec, int herm, double* op, imsrg_stream stream) {
-  PREP();
   if (!op || !vec) return fail(IMSRG_EINVAL, "null argument");
   if (herm != 1 && herm != -1) return fail(IMSRG_EINVAL, "herm must be +1 or -1");
+  PREP();
   int rc;
   if ((rc = ensure_vec(B))) return rc;
   launch_vectorize(B->vec, (int)B->vec_h.size(), op, (double*)vec, 1, herm, s);
@@ -2213,8 +2237,8 @@ int imsrg_repackage(imsrg_basis B, const double* vec, int herm, double* op, imsr
 }
 
 int imsrg_write_op_bin(imsrg_basis B, const double* op, const char* path, imsrg_stream stream) {
-  PREP();
   if (!op || !path) return fail(IMSRG_EINVAL, "null argument");
+  PREP();
   int rc;
   if ((rc = ensure_vec(B))) return rc;
   std::vector<double> h((size_t)B->neq);
@@ -2231,9 +2255,9 @@ int imsrg_write_op_bin(imsrg_basis B, const double* op, const char* path, imsrg_
 }
 
 int imsrg_read_op_bin(imsrg_basis B, const char* path, int herm, double* op, imsrg_stream stream) {
-  PREP();
   if (!op || !path) return fail(IMSRG_EINVAL, "null argument");
   if (herm != 1 && herm != -1) return fail(IMSRG_EINVAL, "herm must be +1 or -1");
+  PREP();
   int rc;
   if ((rc = ensure_vec(B))) return rc;
   FILE* f = std::fopen(path, "rb");
@@ -2258,6 +2282,7 @@ int imsrg_read_op_bin(imsrg_basis B, const char* path, int herm, double* op, ims
 }
 
 int imsrg_mbpt2(imsrg_basis B, const double* H, double* value, imsrg_stream stream) {
+  if (int a_ = check_args({H, value}, {})) return a_;
   PREP();
   int rc = mbpt2_dev(basis_exec(B), H, B->dval, s);
   if (rc) return rc;
@@ -2265,6 +2290,7 @@ int imsrg_mbpt2(imsrg_basis B, const double* H, double* value, imsrg_stream stre
 }
 
 int imsrg_normal_order(imsrg_basis B, double* H, imsrg_stream stream) {
+  if (int a_ = check_args({H}, {})) return a_;
   PREP();
   launch_normal_order(B->D, H, B->dval, s);
   HIPCHK(hipGetLastError());
@@ -2360,6 +2386,7 @@ int imsrg_magnus_expand(imsrg_flow F, double* DG, int DGh, const double* G, int 
 }
 
 int imsrg_euler_step(imsrg_basis B, double* G, const double* DG, double ds, imsrg_stream stream) {
+  if (int a_ = check_args({G, DG}, {})) return a_;
   PREP();
   launch_axpy(B->b.oplen, G, 1.0, DG, ds, G, s);
   HIPCHK(hipGetLastError());

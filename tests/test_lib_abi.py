@@ -75,6 +75,21 @@ def test_error_paths_without_gpu():
     assert lib.imsrg_flow_destroy(None) == 0                       # like free(NULL)
     assert lib.imsrg_flow_set_graphs(None, 1) == _lib.IMSRG_EINVAL
     assert lib.imsrg_basis_size(None, 0) == -1
+    # operator entry points check their arguments before touching the device
+    x = C.c_void_p(16)                                             # never dereferenced: the checks fail first
+    d = C.c_double()
+    assert lib.imsrg_commutator_111(b.handle, x, 1, None, 1, x, -1, None) == _lib.IMSRG_EINVAL
+    assert b"null" in lib.imsrg_last_error()
+    assert lib.imsrg_commutator_122(b.handle, x, 1, x, 2, x, -1, None) == _lib.IMSRG_EINVAL
+    assert b"herm" in lib.imsrg_last_error()
+    assert lib.imsrg_commutator_222_pp_hh(b.handle, x, 1, x, -1, x, 0, x, x, None) == _lib.IMSRG_EINVAL
+    assert lib.imsrg_commutator_220(b.handle, x, 1, x, 1, None, None) == _lib.IMSRG_EINVAL
+    assert lib.imsrg_calculate_cross_coupled(b.handle, x, 3, x, None) == _lib.IMSRG_EINVAL
+    assert lib.imsrg_mat_frob_norm(b.handle, None, C.byref(d), None) == _lib.IMSRG_EINVAL
+    assert lib.imsrg_build_gs_atan(b.handle, x, None, None) == _lib.IMSRG_EINVAL
+    assert lib.imsrg_repackage(b.handle, x, 0, x, None) == _lib.IMSRG_EINVAL
+    assert lib.imsrg_read_op_bin(b.handle, None, 1, x, None) == _lib.IMSRG_EINVAL
+    assert lib.imsrg_commutator_111(None, x, 1, x, 1, x, -1, None) == _lib.IMSRG_EINVAL
     # the Python wrapper raises with the library's message
     try:
         _lib.check(lib.imsrg_set_path(b.handle, 7))
