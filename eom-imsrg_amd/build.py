@@ -29,7 +29,7 @@ HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 DEV_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-I", os.path.join(ROOT, "include"), "-I", CSRC]
 
-SOURCES = ["basis.cpp", "kernels.hip", "kernels_adj.hip", "gemm.hip", "capi.hip"]
+SOURCES = ["basis.cpp", "interaction.cpp", "kernels.hip", "kernels_adj.hip", "gemm.hip", "capi.hip"]
 
 
 def _compile(src: str) -> str:
@@ -49,7 +49,7 @@ def build_library(verbose: bool = False) -> str:
     with cf.ThreadPoolExecutor(max_workers=len(SOURCES)) as ex:
         objs = list(ex.map(_compile, SOURCES))
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp, *objs]
+    cmd = [HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp, *objs, "-lz"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")

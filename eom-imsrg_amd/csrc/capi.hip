@@ -17,6 +17,7 @@
 
 #include "basis.hpp"
 #include "imsrg_hip.h"
+#include "interaction.hpp"
 #include "kernels.hpp"
 
 using namespace imsrg;
@@ -2277,6 +2278,56 @@ int imsrg_read_op_bin(imsrg_basis B, const char* path, int herm, double* op, ims
   HIPCHK(hipMemcpyAsync(B->vecbuf, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, s));
   launch_vectorize(B->vec, (int)B->vec_h.size(), op, B->vecbuf, 1, herm, s);
   HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(s));
+  return 0;
+}
+
+int imsrg_h0_harm_osc(imsrg_basis B, double hw, int htype, double lawson_beta, double com_hw, int mass, double* H,
+                      imsrg_stream stream) {
+  if (!H) return fail(IMSRG_EINVAL, "null argument");
+  if (!B) return fail(IMSRG_EINVAL, "null basis");
+  const Basis& b = B->b;
+  HamParams p;
+  p.hw = hw; p.htype = htype; p.lawson_beta = lawson_beta; p.com_hw = com_hw; p.mass = mass;
+  // host copy of the operator up to the end of fpp
+  std::vector<double> h((size_t)(b.off_fpp + (i64)b.P * b.P), 0.0);
+  std::string err = h0_harm_osc(b, p, h.data());
+  if (!err.empty()) return fail(IMSRG_EINVAL, err);
+  PREP();
+  const size_t n1 = h.size() - (size_t)b.off_fhh;
+  HIPCHK(hipMemcpyAsync(H + b.off_fhh, h.data() + b.off_fhh, n1 * sizeof(double), hipMemcpyHostToDevice, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return 0;
+}
+
+int imsrg_read_me2j(imsrg_basis B, const char* path, int nfile, const int* n, const int* l, const int* jj,
+                    const int* tz, double hw, int htype, double lawson_beta, double com_hw, int mass, double* H,
+                    imsrg_stream stream) {
+  if (!path || !n || !l || !jj || !tz || !H) return fail(IMSRG_EINVAL, "null argument");
+  if (!B) return fail(IMSRG_EINVAL, "null basis");
+  if (htype < 1 || htype > 5) return fail(IMSRG_EINVAL, "htype must be 1..5");
+  if (mass < 1) return fail(IMSRG_EINVAL, "mass must be positive");
+  if (!(hw > 0.0)) return fail(IMSRG_EINVAL, "hw must be positive");
+  const Basis& b = B->b;
+  HamParams p;
+  p.hw = hw; p.htype = htype; p.lawson_beta = lawson_beta; p.com_hw = com_hw; p.mass = mass;
+  FileOrbits f;
+  std::string err = map_file_orbits(b, nfile, n, l, jj, tz, f);
+  if (!err.empty()) return fail(IMSRG_EINVAL, err);
+  std::vector<double> me;
+  err = me2j_read_values(path, me2j_count(f), me);
+  if (!err.empty()) return fail(IMSRG_EINVAL, err);
+  PREP();
+  // the two-body part round trips through the host: elements the file
+  // does not hold keep their values
+  std::vector<double> h((size_t)b.oplen, 0.0);
+  const i64 t0 = b.goff[0];
+  const size_t n2 = (size_t)(b.oplen - t0);
+  HIPCHK(hipMemcpyAsync(h.data() + t0, H + t0, n2 * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  err = me2j_to_op(b, f, me, p, h.data());
+  if (!err.empty()) return fail(IMSRG_EINVAL, err);
+  HIPCHK(hipMemcpyAsync(H + t0, h.data() + t0, n2 * sizeof(double), hipMemcpyHostToDevice, s));
   HIPCHK(hipStreamSynchronize(s));
   return 0;
 }

@@ -111,6 +111,34 @@ int imsrg_repackage(imsrg_basis b, const double* vec, int herm, double* op, imsr
 int imsrg_write_op_bin(imsrg_basis b, const double* op, const char* path, imsrg_stream stream);
 int imsrg_read_op_bin(imsrg_basis b, const char* path, int herm, double* op, imsrg_stream stream);
 
+/* ------------------------------------------------ Hamiltonian set-up */
+/* The scalars the reference keeps on the Hamiltonian (hospace, htype,
+ * lawson_beta, com_hw, Aprot + Aneut) are arguments here.  htype: 1 = T -
+ * Tcm + V, 2 = T + Uho + V, 3 = T + V, 4 = T / A, 5 = the Hcm one-body part
+ * without (com_hw/hw)^2. */
+#define IMSRG_HTYPE_T_MINUS_TCM 1
+#define IMSRG_HTYPE_T_PLUS_UHO 2
+#define IMSRG_HTYPE_T 3
+/* calculate_h0_harm_osc, basic_IMSRG.f90:2093-2191: fhh, fph and fpp of the
+ * device operator H set to the harmonic-oscillator one-body part (zero
+ * outside the (l, j, tz) blocks); E0 and the two-body part untouched.
+ * Synchronous. */
+int imsrg_h0_harm_osc(imsrg_basis b, double hw, int htype, double lawson_beta, double com_hw, int mass, double* H,
+                      imsrg_stream stream);
+/* read_me2j_interaction, interaction_IO.f90:282-557: the isospin-coupled
+ * two-body matrix elements of an me2j file (gzip or plain text: one header
+ * line, then ten fixed-width values a line) over the file's orbit table
+ * (nfile entries in .sps order: proton and neutron entry of each (n, l, j)
+ * adjacent, proton first), converted to proton-neutron form with the
+ * centre-of-mass (htype 1) and Lawson (lawson_beta, com_hw) two-body terms
+ * and written into the two-body blocks of the device operator H in the
+ * reference's loop order.  File orbits the basis lacks are skipped (the
+ * reference's eMax / lmax cut); elements the file does not hold keep their
+ * values.  Synchronous. */
+int imsrg_read_me2j(imsrg_basis b, const char* path, int nfile, const int* n, const int* l, const int* jj,
+                    const int* tz, double hw, int htype, double lawson_beta, double com_hw, int mass, double* H,
+                    imsrg_stream stream);
+
 /* ------------------------------------------------ commutator terms */
 /* commutator_110, commutators.f90:10-33 (returns the value; synchronises) */
 int imsrg_commutator_110(imsrg_basis b, const double* L, int Lherm, const double* R, int Rherm, double* value,

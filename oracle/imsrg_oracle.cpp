@@ -32,7 +32,11 @@
 //   * vectorize / repackage against the reference's own neq count, the
 //     visiting order read off an index-valued operator, and round trips;
 //   * every transpose (imsrg_adjoint.inc) against central differences of
-//     its forward routine, including the whole flow gradient.
+//     its forward routine, including the whole flow gradient;
+//   * the Hamiltonian set-up (imsrg_interaction.inc): HO one-body elements
+//     against quadrature of Laguerre radial functions, r1.r2 + p1.p2 through
+//     the exact two-nucleon centre-of-mass spectrum, the me2j isospin to
+//     proton-neutron conversion against the Clebsch-Gordan decomposition.
 // The reference itself is unbuildable here (it needs an external BLAS/LAPACK
 // library that the image lacks), so there is no oracle/_ref.
 // ============================================================================
@@ -1034,6 +1038,7 @@ int magnus(Op& DG, const Op& G, double conv, Work& w) {
 }
 
 #include "imsrg_adjoint.inc"
+#include "imsrg_interaction.inc"
 
 }  // namespace
 
@@ -1147,6 +1152,18 @@ void orc_comm222ph(void* h, const double* LCC, int Lh, const double* RCC, int Rh
                    double* WCC) {
   OPS(h) Op res{&b, RES, RESh}; comm222ph(LCC, Lh, RCC, Rh, res, WCC);
 }
+void orc_h0_harm_osc(void* h, double hw, int htype, double beta, double cmhw, int mass, double* op) {
+  OPS(h) Op o{&b, op, 1};
+  h0_harm_osc(o, hw, htype, beta, cmhw, mass);
+}
+i64 orc_me2j_count(int nfile, const int* l, const int* jj) { return me2j_values(nfile / 2, l, jj); }
+int orc_me2j_convert(void* h, const double* me, i64 nme, int nfile, const int* l, const int* jj, const int* con,
+                     double hw, int htype, double beta, double cmhw, int mass, double* op) {
+  OPS(h) Op o{&b, op, 1};
+  return me2j_convert(o, me, nme, nfile, l, jj, con, hw, htype, beta, cmhw, mass);
+}
+double orc_r1_r2(void* h, int a, int bb, int c, int d, int J) { OPS(h) return vec_dot(b, a, bb, c, d, J, false); }
+double orc_p1_p2(void* h, int a, int bb, int c, int d, int J) { OPS(h) return vec_dot(b, a, bb, c, d, J, true); }
 double orc_frob_norm(void* h, const double* op) { OPS(h) return frob_norm(Op{&b, (double*)op, 1}); }
 void orc_add(void* h, const double* A, double ax, const double* B, double bx, double* C) {
   OPS(h) add_op(b, A, ax, B, bx, C);

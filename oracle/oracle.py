@@ -60,6 +60,10 @@ def lib():
             "orc_gen_vjp": (None, [vp, dp, dp, dp, dp, dp, i]), "orc_gs_w2_vjp": (None, [vp, dp, dp, dp]),
             "orc_bf_0body": (d, [vp, dp, i, dp, i]), "orc_bf_1body": (d, [vp, dp, i, dp, i, i, i]),
             "orc_bf_2body": (d, [vp, dp, i, dp, i, i, i, i, i, i, dp]),
+            "orc_h0_harm_osc": (None, [vp, d, i, d, d, i, dp]),
+            "orc_me2j_count": (C.c_int64, [i, ip, ip]),
+            "orc_me2j_convert": (i, [vp, dp, C.c_int64, i, ip, ip, ip, d, i, d, d, i, dp]),
+            "orc_r1_r2": (d, [vp, i, i, i, i, i]), "orc_p1_p2": (d, [vp, i, i, i, i, i]),
         }
         for k, (r, a) in sig.items():
             f = getattr(L, k)
@@ -325,6 +329,41 @@ class OracleBasis:
         v = lib().orc_bf_2body(self.h, _dp(A), Ah, _dp(B), Bh, a, b, c, d, J, _dp(parts))
         return v, parts
 
+    # --- Hamiltonian set-up (oracle/imsrg_interaction.inc)
+    def h0_harm_osc(self, hw, htype, mass, beta=0.0, cmhw=None, op=None):
+        """calculate_h0_harm_osc, basic_IMSRG.f90:2093-2191"""
+        op = np.zeros(self.oplen) if op is None else op
+        lib().orc_h0_harm_osc(self.h, hw, htype, beta, hw if cmhw is None else cmhw, mass, _dp(op))
+        return op
+
+    def r1_r2(self, a, b, c, d, J):
+        return lib().orc_r1_r2(self.h, a, b, c, d, J)
+
+    def p1_p2(self, a, b, c, d, J):
+        return lib().orc_p1_p2(self.h, a, b, c, d, J)
+
+    def read_me2j(self, path, orbits, hw, htype, mass, beta=0.0, cmhw=None, op=None):
+        """read_me2j_interaction, interaction_IO.f90:282-557: the file parsed
+        here (gzip or text, fixed-width f12.7 / f13.7 fields), converted by
+        the oracle; `orbits` = the file's (n, l, jj, tz) table"""
+        fl = np.ascontiguousarray(orbits["l"], np.int32)
+        fj = np.ascontiguousarray(orbits["jj"], np.int32)
+        nfile = len(fl)
+        con = np.full(nfile, -1, np.int32)
+        mine = {(int(n), int(l), int(j), int(t)): k for k, (n, l, j, t) in
+                enumerate(zip(self._keep[0], self._keep[1], self._keep[2], self._keep[3]))}
+        for x in range(nfile):
+            con[x] = mine.get((int(orbits["n"][x]), int(fl[x]), int(fj[x]), int(orbits["tz"][x])), -1)
+        ipt = lambda a: a.ctypes.data_as(C.POINTER(C.c_int))
+        count = lib().orc_me2j_count(nfile, ipt(fl), ipt(fj))
+        me = read_me2j_values(path, count)
+        op = np.zeros(self.oplen) if op is None else op
+        rc = lib().orc_me2j_convert(self.h, _dp(me), count, nfile, ipt(fl), ipt(fj), ipt(con), hw, htype, beta,
+                                    hw if cmhw is None else cmhw, mass, _dp(op))
+        if rc != 0:
+            raise ValueError("value count does not match the file orbits")
+        return op
+
     def __del__(self):
         if getattr(self, "h", None):
             lib().orc_basis_free(self.h)
@@ -337,3 +376,25 @@ def sixj(*a):
 
 def ninej(*a):
     return lib().orc_ninej(*a)
+
+
+def read_me2j_values(path, count):
+    """the values of an me2j file (interaction_IO.f90:375-392): one header
+    line, then lines of fixed-width fields f12.7, 9 x f13.7"""
+    import gzip
+    opener = gzip.open if open(path, "rb").read(2) == b"\x1f\x8b" else open
+    out = np.zeros(count)
+    with opener(path, "rt") as f:
+        f.readline()
+        k = 0
+        while k < count:
+            line = f.readline()
+            if not line:
+                raise ValueError(f"{path}: ends after {k} of {count} values")
+            line = line.rstrip("\n")
+            for x in range(min(10, count - k)):
+                at, w = (0, 12) if x == 0 else (12 + 13 * (x - 1), 13)
+                field = line[at: at + w].strip()
+                out[k + x] = float(field) if field else 0.0
+            k += min(10, count - k)
+    return out
