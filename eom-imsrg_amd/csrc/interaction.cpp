@@ -261,6 +261,30 @@ std::string me2j_to_op(const Basis& b, const FileOrbits& f, const std::vector<do
   const int COM = p.htype == 1 ? 1 : 0;
   const double hw = p.hw, beta = p.lawson_beta, cmhw = p.com_hw;
   const int N = b.N, nJ = b.nJ;
+  // r1.r2 and p1.p2 from per-orbit-pair factors and the basis's 6j table
+  // (the values r1_r2 / p1_p2 compute, in the same operation order); each
+  // is evaluated only where its coefficient is nonzero
+  const bool need_r = beta != 0.0, need_p = beta != 0.0 || COM != 0;
+  std::vector<double> jr((size_t)N * N, 0.0), jp((size_t)N * N, 0.0);
+  for (int x = 0; x < N; ++x)
+    for (int y = 0; y < N; ++y) {
+      if (need_r) jr[(size_t)x * N + y] = jfac(b, x, y, reduced_r);
+      if (need_p) jp[(size_t)x * N + y] = jfac(b, x, y, reduced_p);
+    }
+  const int nh = b.nh6, nJ6 = b.nJ6;
+  auto six2 = [&](int j1, int j2, int J, int j4, int j5) {   // {j1 j2 J; j4 j5 1}
+    return b.six[(((((size_t)(j1 >> 1) * nh + (j2 >> 1)) * nh + (j4 >> 1)) * nh + (j5 >> 1)) * nJ6 + 1) * nJ6 +
+                 J / 2];
+  };
+  auto vdot_tab = [&](const std::vector<double>& t, int a, int bb, int c, int d, int J, int sgn) {
+    const int ja = b.jj[a], jb = b.jj[bb], jc = b.jj[c], jd = b.jj[d];
+    const double j_ac = t[(size_t)a * N + c], j_ad = t[(size_t)a * N + d];
+    const double j_bc = t[(size_t)bb * N + c], j_bd = t[(size_t)bb * N + d];
+    const int e1 = (jb + sgn * jc - J) / 2, e2 = (jc + jd - J) / 2, e3 = (jb + sgn * jd - J) / 2;
+    const double t1 = (double)m1pow(e1) * six2(ja, jb, J, jd, jc) * j_ac * j_bd;
+    const double t2 = (double)(m1pow(e2) * m1pow(e3)) * six2(ja, jb, J, jc, jd) * j_ad * j_bc;
+    return (t1 - t2) / std::sqrt((1.0 + kron(a, bb)) * (1.0 + kron(c, d)));
+  };
   long long i = 0;
   me2j_loop(f, [&](int k1, int k2, int kk1, int kk2, int JT) {
     const double* m = me.data() + i;
@@ -287,8 +311,8 @@ std::string me2j_to_op(const Basis& b, const FileOrbits& f, const std::vector<do
                        (double)(kron(ta + tb, -2) * kron(tc + td, -2)) * m[1] +
                        (double)(kron(ta + tb, 2) * kron(tc + td, 2)) * m[3] +
                        0.125 * std::abs((ta - tb) * (tc - td)) * m[2];
-            const double g2 = r1_r2(b, a, bb, c, d, JT);
-            const double g3 = p1_p2(b, a, bb, c, d, JT);
+            const double g2 = need_r ? vdot_tab(jr, a, bb, c, d, JT, 1) : 0.0;
+            const double g3 = need_p ? vdot_tab(jp, a, bb, c, d, JT, -1) : 0.0;
             const double g1 = (g3 + cmhw * cmhw / (hw * hw) * g2) * beta;
             V = V * pre2 + (g1 - g3 * COM) * hw / p.mass;
             const int C1 = b.con[a] + b.con[bb] + 1, C2 = b.con[c] + b.con[d] + 1;
