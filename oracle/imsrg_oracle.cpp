@@ -36,7 +36,10 @@
 //   * the Hamiltonian set-up (imsrg_interaction.inc): HO one-body elements
 //     against quadrature of Laguerre radial functions, r1.r2 + p1.p2 through
 //     the exact two-nucleon centre-of-mass spectrum, the me2j isospin to
-//     proton-neutron conversion against the Clebsch-Gordan decomposition.
+//     proton-neutron conversion against the Clebsch-Gordan decomposition;
+//   * Hartree-Fock (imsrg_hf.inc): the two-body basis transform against the
+//     four-index sum over v_elem, the HF Hamiltonian against normal_order of
+//     the rotated vacuum Hamiltonian, the Brillouin condition.
 // The reference itself is unbuildable here (it needs an external BLAS/LAPACK
 // library that the image lacks), so there is no oracle/_ref.
 // ============================================================================
@@ -1039,6 +1042,7 @@ int magnus(Op& DG, const Op& G, double conv, Work& w) {
 
 #include "imsrg_adjoint.inc"
 #include "imsrg_interaction.inc"
+#include "imsrg_hf.inc"
 
 }  // namespace
 
@@ -1164,6 +1168,22 @@ int orc_me2j_convert(void* h, const double* me, i64 nme, int nfile, const int* l
 }
 double orc_r1_r2(void* h, int a, int bb, int c, int d, int J) { OPS(h) return vec_dot(b, a, bb, c, d, J, false); }
 double orc_p1_p2(void* h, int a, int bb, int c, int d, int J) { OPS(h) return vec_dot(b, a, bb, c, d, J, true); }
+int orc_sp_layout(void* h, int* sizes) {
+  OPS(h) for (size_t q = 0; q < b.states.size(); ++q) if (sizes) sizes[q] = (int)b.states[q].size();
+  return (int)b.states.size();
+}
+void orc_hf_kin(void* h, const double* H, double* T) { OPS(h) hf_kin(Op{&b, (double*)H, 1}, T); }
+void orc_hf_density(void* h, const double* D, double* rho) { OPS(h) hf_density(b, D, rho); }
+void orc_hf_gamma(void* h, const double* H, const double* rho, double* gam) {
+  OPS(h) hf_gamma(Op{&b, (double*)H, 1}, rho, gam);
+}
+void orc_hf_finish(void* h, double* H, const double* D, double* F, double* T, double* V) {
+  OPS(h) Op o{&b, H, 1};
+  hf_transform_1b(o, D, F, T, V);
+  o.E0() = hf_energy(b, T, V);
+  hf_transform_2b(o, D);
+}
+void orc_hf_transform_2b(void* h, double* H, const double* D) { OPS(h) Op o{&b, H, 1}; hf_transform_2b(o, D); }
 double orc_frob_norm(void* h, const double* op) { OPS(h) return frob_norm(Op{&b, (double*)op, 1}); }
 void orc_add(void* h, const double* A, double ax, const double* B, double bx, double* C) {
   OPS(h) add_op(b, A, ax, B, bx, C);

@@ -64,6 +64,9 @@ def lib():
             "orc_me2j_count": (C.c_int64, [i, ip, ip]),
             "orc_me2j_convert": (i, [vp, dp, C.c_int64, i, ip, ip, ip, d, i, d, d, i, dp]),
             "orc_r1_r2": (d, [vp, i, i, i, i, i]), "orc_p1_p2": (d, [vp, i, i, i, i, i]),
+            "orc_sp_layout": (i, [vp, ip]), "orc_hf_kin": (None, [vp, dp, dp]),
+            "orc_hf_density": (None, [vp, dp, dp]), "orc_hf_gamma": (None, [vp, dp, dp, dp]),
+            "orc_hf_finish": (None, [vp, dp, dp, dp, dp, dp]), "orc_hf_transform_2b": (None, [vp, dp, dp]),
         }
         for k, (r, a) in sig.items():
             f = getattr(L, k)
@@ -363,6 +366,61 @@ class OracleBasis:
         if rc != 0:
             raise ValueError("value count does not match the file orbits")
         return op
+
+    # --- Hartree-Fock (oracle/imsrg_hf.inc; the SCF loop of calc_HF,
+    # HF_mod.f90:14-105, with numpy's LAPACK eigh as dsyev)
+    def sp_blocks(self):
+        """sizes of the (tz, l, j) blocks in allocate_sp_mat order and their
+        offsets in the flat block storage"""
+        nb = lib().orc_sp_layout(self.h, None)
+        sz = np.zeros(nb, np.int32)
+        lib().orc_sp_layout(self.h, sz.ctypes.data_as(C.POINTER(C.c_int)))
+        off = np.concatenate([[0], np.cumsum(sz.astype(np.int64) ** 2)])
+        return sz, off
+
+    def hf_transform_2b(self, H, D):
+        out = H.copy()
+        lib().orc_hf_transform_2b(self.h, _dp(out), _dp(np.ascontiguousarray(D, np.float64)))
+        return out
+
+    def hartree_fock(self, H):
+        """calc_HF (HF_mod.f90:14-105) without three-body forces: returns
+        (H in the normal-ordered HF basis, D, iterations); H = the vacuum
+        (t, V) in the HO basis"""
+        sz, off = self.sp_blocks()
+        tot = int(off[-1])
+        T = np.zeros(tot)
+        lib().orc_hf_kin(self.h, _dp(H), _dp(T))
+        D = np.zeros(tot)
+        ev = [np.zeros(n) for n in sz]
+        for q, n in enumerate(sz):
+            D[off[q]: off[q + 1]] = np.eye(n).reshape(-1)
+        crit, it = 10.0, 0
+        rho, Vg = np.zeros(tot), np.zeros(tot)
+        while crit > float(np.float32(1e-6)):
+            lib().orc_hf_density(self.h, _dp(D), _dp(rho))
+            lib().orc_hf_gamma(self.h, _dp(H), _dp(rho), _dp(Vg))
+            F = T + Vg + 0.0
+            crit = 0.0
+            for q, n in enumerate(sz):
+                if n == 0:
+                    crit += 0.0
+                    continue
+                w, v = np.linalg.eigh(F[off[q]: off[q + 1]].reshape(n, n).T)
+                D[off[q]: off[q + 1]] = v.T.reshape(-1)          # column-major
+                crit += np.sqrt(np.sum((ev[q] - w) ** 2))
+                ev[q] = w
+            it += 1
+        for q, n in enumerate(sz):
+            M = D[off[q]: off[q + 1]].reshape(n, n).T.copy()
+            for i in range(n):
+                if M[i, i] < 0.0:
+                    M[:, i] = M[:, i] * (-1)
+            D[off[q]: off[q + 1]] = M.T.reshape(-1)
+        F = T + Vg + 0.0
+        out = H.copy()
+        lib().orc_hf_finish(self.h, _dp(out), _dp(D), _dp(F), _dp(T), _dp(Vg))
+        return out, D, it
 
     def __del__(self):
         if getattr(self, "h", None):
