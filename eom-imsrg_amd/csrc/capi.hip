@@ -17,6 +17,7 @@
 
 #include "basis.hpp"
 #include "imsrg_hip.h"
+#include "hf.hpp"
 #include "interaction.hpp"
 #include "kernels.hpp"
 
@@ -99,6 +100,12 @@ struct imsrg_basis_s {
   TaskList<PairTask> pairs;
   TaskList<PairTask> symt;          // 64 x 64 tile pairs of every square block (pad = g)
   GemmFamily pp1, k2, k3, ph;       // pppp L1 R1, pp x (ph|hh), (ph|hh)^2, cross-coupled W'
+  // Hartree-Fock two-body transform: T = V M, then V = M^T T per channel
+  // (built on first use), with the per-channel square matrices V, T, M
+  GemmFamily hf1, hf2;
+  double* hf_buf = nullptr;        // [3 * hf_len]: V, T, M
+  i64* hf_off = nullptr;           // [nblocks + 1] channel offsets
+  i64 hf_len = 0;
   std::vector<Partner> part_h;      // 122 partner lists (CSR by global pair id)
   std::vector<int> rowp_h, colp_h;
   Partner* partners = nullptr;
@@ -1985,6 +1992,12 @@ int64_t imsrg_basis_size(imsrg_basis B, int what) {
       vec_plan(b, nullptr, &n);
       return n;
     }
+    case IMSRG_SIZE_SPMAT: {
+      i64 n = 0;
+      for (size_t q = 0; q + 1 < b.spstart.size(); ++q)
+        n += (i64)(b.spstart[q + 1] - b.spstart[q]) * (b.spstart[q + 1] - b.spstart[q]);
+      return n;
+    }
   }
   return -1;
 }
@@ -2329,6 +2342,82 @@ int imsrg_read_me2j(imsrg_basis B, const char* path, int nfile, const int* n, co
   if (!err.empty()) return fail(IMSRG_EINVAL, err);
   HIPCHK(hipMemcpyAsync(H + t0, h.data() + t0, n2 * sizeof(double), hipMemcpyHostToDevice, s));
   HIPCHK(hipStreamSynchronize(s));
+  return 0;
+}
+
+// the two GEMM families and buffers of the Hartree-Fock two-body transform
+int ensure_hf(imsrg_basis_s* B) {
+  if (B->hf_buf) return 0;
+  const Basis& b = B->b;
+  std::vector<i64> off(b.nblocks + 1, 0);
+  for (int q = 0; q < b.nblocks; ++q) {
+    const i64 nt = (i64)b.dims[q * 3] + b.dims[q * 3 + 1] + b.dims[q * 3 + 2];
+    off[q + 1] = off[q] + nt * nt;
+  }
+  B->hf_len = std::max<i64>(off[b.nblocks], 1);
+  int rc;
+  if ((rc = dalloc(B, (size_t)(3 * B->hf_len) * sizeof(double), (void**)&B->hf_buf))) return rc;
+  if ((rc = upload(B, off, &B->hf_off))) return rc;
+  for (int pass = 0; pass < 2; ++pass) {
+    GemmFamily& f = pass == 0 ? B->hf1 : B->hf2;
+    f.probs_h.clear(); f.nks.clear(); f.group_end.clear();
+    for (int q = 0; q < b.nblocks; ++q) {
+      const int nt = b.dims[q * 3] + b.dims[q * 3 + 1] + b.dims[q * 3 + 2];
+      if (nt == 0) continue;
+      GemmProb p{};
+      p.a1 = p.b1 = p.c = off[q];
+      p.m = p.n = p.k = nt;
+      p.lda = p.ldb = p.ldc = nt;
+      p.nprod = 1;
+      // slots: 0 V, 1 T, 2 M.  pass 0: T = V M; pass 1: V = M^T T
+      p.ta1 = pass == 0 ? 0 : 1;
+      p.sa1 = pass == 0 ? 0 : 2;
+      p.sb1 = pass == 0 ? 2 : 1;
+      p.sc = pass == 0 ? 1 : 0;
+      p.mode = 0;
+      p.alpha = 1.0;
+      p.beta = 0.0;
+      f.probs_h.push_back(p);
+      f.nks.push_back(1);
+      f.group_end.push_back((int)f.probs_h.size());
+    }
+    if ((rc = upload(B, f.probs_h, &f.probs))) return rc;
+    if ((rc = family_tasks(B, f, 0))) return rc;
+  }
+  return 0;
+}
+
+int imsrg_hartree_fock(imsrg_basis B, double* H, int maxit, double* ehf, int* iterations, double* coefs,
+                       imsrg_stream stream) {
+  if (!H) return fail(IMSRG_EINVAL, "null argument");
+  if (maxit < 1) return fail(IMSRG_EINVAL, "maxit must be positive");
+  PREP();
+  const Basis& b = B->b;
+  std::vector<double> h((size_t)b.oplen);
+  HIPCHK(hipMemcpyAsync(h.data(), H, h.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  HFResult r;
+  std::vector<double> M;
+  std::string err = hartree_fock_host(b, h.data(), maxit, r, M);
+  if (!err.empty()) return fail(IMSRG_ESTEPS, err);
+  int rc;
+  if ((rc = ensure_hf(B))) return rc;
+  double *V = B->hf_buf, *T = B->hf_buf + B->hf_len, *Md = B->hf_buf + 2 * B->hf_len;
+  // E0 and the one-body part (Fock matrix in the HF basis)
+  const size_t n1 = (size_t)(b.off_fpp + (i64)b.P * b.P);
+  HIPCHK(hipMemcpyAsync(H, h.data(), n1 * sizeof(double), hipMemcpyHostToDevice, s));
+  if (!M.empty()) HIPCHK(hipMemcpyAsync(Md, M.data(), M.size() * sizeof(double), hipMemcpyHostToDevice, s));
+  launch_hf_pack(B->D, B->seg_full.dev, B->seg_full.n, H, V, B->hf_off, s);
+  Slots sl{};
+  sl.p[0] = V; sl.p[1] = T; sl.p[2] = Md;
+  run_gemm(B, B->hf1, sl, s);
+  run_gemm(B, B->hf2, sl, s);
+  launch_hf_unpack(B->D, B->seg_full.dev, B->seg_full.n, V, B->hf_off, H, s);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(s));
+  if (ehf) *ehf = r.ehf;
+  if (iterations) *iterations = r.iterations;
+  if (coefs) std::copy(r.D.begin(), r.D.end(), coefs);
   return 0;
 }
 

@@ -1230,4 +1230,56 @@ void launch_copy_b(double* dst, const double* src, i64 n, int ninst, const int* 
   hipLaunchKernelGGL(k_copy, dim3(g, ninst), dim3(256), 0, s, dst, src, n, act);
 }
 
+// ====================================================== transform_2b_to_HF
+// HF_mod.f90:567-692: each channel's two-body blocks as one square matrix
+// over its pairs in the order (hh, ph, pp), nt x nt column-major at
+// V + voff[q] (square blocks as stored, rectangular blocks in both
+// orientations); unpack writes the six blocks back from it.  One thread per
+// stored element (the full-block segment tasks).
+__device__ __forceinline__ void hf_pos(const DevBasis& B, int q, int g, i64 e, int& n1, i64& row, i64& col, i64& nt) {
+  const int np = B.dims[q * 3], nb = B.dims[q * 3 + 1], nh = B.dims[q * 3 + 2];
+  const int r0[3] = {nh + nb, nh, 0};
+  n1 = brows(B, q, g);
+  row = r0[row_cls(g)] + e % n1;
+  col = r0[col_cls(g)] + e / n1;
+  nt = (i64)np + nb + nh;
+}
+
+__global__ __launch_bounds__(256) void k_hf_pack(DevBasis B, const SegTask* tasks, const double* op, double* V,
+                                                 const i64* voff) {
+  const SegTask t = tasks[blockIdx.x];
+  const i64 e = (i64)t.start + threadIdx.x;
+  if (e >= t.cnt) return;
+  int n1;
+  i64 row, col, nt;
+  hf_pos(B, t.q, t.g, e, n1, row, col, nt);
+  const double x = op[B.goff[t.q * 6 + t.g] + e];
+  double* Vq = V + voff[t.q];
+  Vq[row + col * nt] = x;
+  if (!is_square(t.g)) Vq[col + row * nt] = x;
+}
+
+__global__ __launch_bounds__(256) void k_hf_unpack(DevBasis B, const SegTask* tasks, const double* V,
+                                                   const i64* voff, double* op) {
+  const SegTask t = tasks[blockIdx.x];
+  const i64 e = (i64)t.start + threadIdx.x;
+  if (e >= t.cnt) return;
+  int n1;
+  i64 row, col, nt;
+  hf_pos(B, t.q, t.g, e, n1, row, col, nt);
+  op[B.goff[t.q * 6 + t.g] + e] = V[voff[t.q] + row + col * nt];
+}
+
+void launch_hf_pack(const DevBasis& B, const SegTask* t, int nt, const double* op, double* V, const i64* voff,
+                    hipStream_t s) {
+  if (nt <= 0) return;
+  hipLaunchKernelGGL(k_hf_pack, dim3(nt), dim3(kTile), 0, s, B, t, op, V, voff);
+}
+
+void launch_hf_unpack(const DevBasis& B, const SegTask* t, int nt, const double* V, const i64* voff, double* op,
+                      hipStream_t s) {
+  if (nt <= 0) return;
+  hipLaunchKernelGGL(k_hf_unpack, dim3(nt), dim3(kTile), 0, s, B, t, V, voff, op);
+}
+
 }  // namespace imsrg

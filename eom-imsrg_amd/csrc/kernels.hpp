@@ -117,6 +117,11 @@ void launch_comm222ph_chunk(const DevBasis& B, const PhChunk* ch, int nch, const
                             int RESh, hipStream_t s, int mirror);
 // dir 0: vec <- op (vectorize); dir 1: op <- vec (repackage, herm fill)
 void launch_vectorize(const VecTask* t, int nt, double* op, double* vec, int dir, int herm, hipStream_t s);
+// transform_2b_to_HF: gam blocks <-> per-channel square matrices (hh, ph, pp)
+void launch_hf_pack(const DevBasis& B, const SegTask* t, int nt, const double* op, double* V, const i64* voff,
+                    hipStream_t s);
+void launch_hf_unpack(const DevBasis& B, const SegTask* t, int nt, const double* V, const i64* voff, double* op,
+                      hipStream_t s);
 void launch_comm122p(const DevBasis& B, const SegTask* t, int nt, const Partner* part, const int* rowp,
                      const int* colp, const int* f1off, int nf1, const double* L, int Lh, const double* R, int Rh,
                      double* RES, int RESh, hipStream_t s, int mirror = 1);

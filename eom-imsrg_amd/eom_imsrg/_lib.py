@@ -24,7 +24,7 @@ IMSRG_ECLIP = 3
 IMSRG_ESTEPS = 4
 PATH_GROUPED = 0
 PATH_PLAIN = 1
-SIZE = dict(NORB=0, NHOLE=1, NPART=2, JTOTAL_MAX=3, NBLOCKS=4, OP=5, NCC=6, CC=7, WCC=8, NEQ=9)
+SIZE = dict(NORB=0, NHOLE=1, NPART=2, JTOTAL_MAX=3, NBLOCKS=4, OP=5, NCC=6, CC=7, WCC=8, NEQ=9, SPMAT=10)
 
 
 class ImsrgError(RuntimeError):
@@ -69,6 +69,7 @@ SIGNATURES = {
     "imsrg_h0_harm_osc": (_i, [_p, C.c_double, _i, C.c_double, C.c_double, _i, _dp, _p]),
     "imsrg_read_me2j": (_i, [_p, C.c_char_p, _i, _ip, _ip, _ip, _ip, C.c_double, _i, C.c_double, C.c_double, _i,
                              _dp, _p]),
+    "imsrg_hartree_fock": (_i, [_p, _dp, _i, _d, _ip, _d, _p]),
     "imsrg_transform_sq_op_bch": (_i, [_p, _dp, _i, _dp, C.c_double, _i, _ip, _p]),
     "imsrg_dhds": (_i, [_p, _dp, _dp, _p]),
     "imsrg_commutator_110": (_i, [_p, _dp, _i, _dp, _i, _d, _p]),

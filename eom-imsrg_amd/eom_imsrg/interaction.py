@@ -6,7 +6,10 @@ before normal ordering (main_IMSRG.f90:121-140).
 
 The scalars the reference keeps on the Hamiltonian (hospace, lawson_beta,
 com_hw, Aprot + Aneut) are arguments; ``htype`` is the reference's
-Hamiltonian type (1: T - Tcm + V, 2: T + Uho + V, 3: T + V).
+Hamiltonian type (1: T - Tcm + V, 2: T + Uho + V, 3: T + V).  The vacuum
+Hamiltonian is then normal ordered either in the HO basis
+(``commutators.normal_order``) or in the Hartree-Fock basis
+(``hartree_fock``, calc_HF of HF_mod.f90:14-105).
 """
 from __future__ import annotations
 
@@ -47,3 +50,16 @@ def read_me2j(H: SqOp, path: str, file_emax: int | None = None, hw: float = 20.0
                                         float(lawson_beta), float(hw if com_hw is None else com_hw), int(mass),
                                         H.ptr, _stream()))
     return H
+
+
+def hartree_fock(H: SqOp, maxit: int = 200):
+    """H (vacuum (t, V), HO basis) <- normal ordered in the Hartree-Fock basis;
+    returns (e_HF, iterations, HF orbits per (tz, l, j) block, flat
+    column-major)"""
+    e = C.c_double()
+    it = C.c_int()
+    n = int(_lib.lib.imsrg_basis_size(H.basis.handle, _lib.SIZE["SPMAT"]))
+    D = np.zeros(max(n, 1))
+    _lib.check(_lib.lib.imsrg_hartree_fock(H.basis.handle, H.ptr, int(maxit), C.byref(e), C.byref(it),
+                                           D.ctypes.data_as(C.POINTER(C.c_double)), _stream()))
+    return e.value, it.value, D[:n]

@@ -57,6 +57,7 @@ typedef void* imsrg_stream;
 #define IMSRG_SIZE_CC 7   /* doubles per cross-coupled matrix set (cc_mat) */
 #define IMSRG_SIZE_WCC 8  /* doubles of the WCC workspace */
 #define IMSRG_SIZE_NEQ 9  /* doubles of the vectorize stream (H%neq) */
+#define IMSRG_SIZE_SPMAT 10 /* doubles of a single-particle block matrix set (full_sp_block_mat) */
 
 const char* imsrg_last_error(void);
 
@@ -138,6 +139,17 @@ int imsrg_h0_harm_osc(imsrg_basis b, double hw, int htype, double lawson_beta, d
 int imsrg_read_me2j(imsrg_basis b, const char* path, int nfile, const int* n, const int* l, const int* jj,
                     const int* tz, double hw, int htype, double lawson_beta, double com_hw, int mass, double* H,
                     imsrg_stream stream);
+/* calc_HF, HF_mod.f90:14-105 (no three-body force): H in = the vacuum
+ * Hamiltonian (t, V) in the HO basis; H out = normal ordered in the
+ * Hartree-Fock basis (E0 = e_HF, one-body = the Fock matrix, two-body
+ * transformed by transform_2b_to_HF).  The self-consistent loop (criterion:
+ * summed eigenvalue change < 1e-6, at most maxit iterations, else
+ * IMSRG_ESTEPS) runs on the host; the two-body transform runs on the device.
+ * coefs (host, optional, IMSRG_SIZE_SPMAT doubles): the HF orbits per
+ * (tz, l, j) block, column i of each column-major block = HF orbit i.
+ * Synchronous. */
+int imsrg_hartree_fock(imsrg_basis b, double* H, int maxit, double* ehf, int* iterations, double* coefs,
+                       imsrg_stream stream);
 
 /* ------------------------------------------------ commutator terms */
 /* commutator_110, commutators.f90:10-33 (returns the value; synchronises) */

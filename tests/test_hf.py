@@ -123,3 +123,55 @@ def test_hf_brillouin_and_variational(emax):
     assert np.max(np.abs(Hho[fph])) > 0.1
     assert np.max(np.abs(Hhf[fph])) < 1e-4
     assert Hhf[0] <= Hho[0] + 1e-9
+
+
+def test_hf_errors_without_gpu():
+    from eom_imsrg import _lib
+    import ctypes as C
+    b = basis(2)
+    x = C.c_void_p(16)   # never dereferenced: the checks fail first
+    assert _lib.lib.imsrg_hartree_fock(b.handle, None, 50, None, None, None, None) == _lib.IMSRG_EINVAL
+    assert _lib.lib.imsrg_hartree_fock(b.handle, x, 0, None, None, None, None) == _lib.IMSRG_EINVAL
+    assert _lib.lib.imsrg_hartree_fock(None, x, 50, None, None, None, None) == _lib.IMSRG_EINVAL
+    n = _lib.lib.imsrg_basis_size(b.handle, _lib.SIZE["SPMAT"])
+    assert n == int(oracle_basis(2).sp_blocks()[1][-1])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("emax", [4, 6])
+def test_hf_matches_oracle(gpu, emax):
+    """library HF (host SCF with Jacobi eigenvectors, two-body transform as
+    two grouped GEMM families on the device) vs the oracle (LAPACK eigh, the
+    reference's eight products): same iterations, E0, Fock matrix, two-body
+    part and HF orbits to 1e-10"""
+    from conftest import assert_rel
+    from eom_imsrg import SqOp, interaction, synth
+    b, ob = basis(emax), oracle_basis(emax)
+    print(f"[hf] emax={emax}", flush=True)
+    H = synth.hamiltonian_vacuum(b, seed=4, hw=20.0)
+    ref, Dref, itref = ob.hartree_fock(H)
+    Hd = SqOp.from_numpy(b, H, 1)
+    e, it, D = interaction.hartree_fock(Hd)
+    assert it == itref
+    assert abs(e - ref[0]) < 1e-10 * abs(ref[0])
+    assert_rel(Hd.numpy(), ref, rtol=1e-10, what=f"HF H emax {emax}")
+    np.testing.assert_allclose(D, Dref, rtol=1e-10, atol=1e-12)
+
+
+@pytest.mark.gpu
+def test_flow_from_hf(gpu):
+    """magnus_decouple from the HF-normal-ordered Hamiltonian: equal steps and
+    E0 per step within 1e-8 MeV of the oracle's flow"""
+    from eom_imsrg import Flow, SqOp, interaction, synth
+    REF = float(np.float32(1e-6))
+    b, ob = basis(4), oracle_basis(4)
+    print("[hf flow] emax=4", flush=True)
+    H = synth.hamiltonian_vacuum(b, seed=4, hw=20.0)
+    ref, _, _ = ob.hartree_fock(H)
+    Hd = SqOp.from_numpy(b, H, 1)
+    interaction.hartree_fock(Hd)
+    HSr, _, logr, str_ = ob.magnus_decouple(ref, 0.5, REF, REF, 0, REF)
+    log = Flow(b, graphs=True).magnus_decouple(Hd, SqOp.zeros(b, -1))
+    assert len(log) == str_
+    assert np.array_equal(log[:, 3], logr[:, 3])
+    assert np.max(np.abs(log[:, 1] - logr[:, 1])) < 1e-8
