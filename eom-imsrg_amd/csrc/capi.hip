@@ -165,7 +165,18 @@ struct imsrg_basis_s {
   double* vecbuf = nullptr;
   i64 neq = 0;
 
+  // concurrent term schedule of the grouped commutator: three auxiliary
+  // streams (default priority) and the events that fork and join them
+  bool term_streams = true;
+  bool aux_ready = false;
+  hipStream_t aux[3] = {nullptr, nullptr, nullptr};
+  hipEvent_t tev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+
   ~imsrg_basis_s() {
+    for (hipEvent_t e : tev)
+      if (e) (void)hipEventDestroy(e);
+    for (hipStream_t x : aux)
+      if (x) (void)hipStreamDestroy(x);
     for (void* p : allocs) (void)hipFree(p);
     if (hval) (void)hipHostFree(hval);
   }
@@ -700,6 +711,9 @@ int ensure_dev(imsrg_basis_s* B) {
   UP(cccols, cccols) UP(cccoloff, cccoloff) UP(six, six)
 #undef UP
   if ((rc = build_plan(B))) return rc;
+  for (hipStream_t& x : B->aux) HIPCHK(hipStreamCreateWithFlags(&x, hipStreamNonBlocking));
+  for (hipEvent_t& e : B->tev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  B->aux_ready = true;
   B->dev_ready = true;
   return 0;
 }
@@ -850,6 +864,63 @@ void t_full(const Exec& X, const double* L, int Lh, const double* R, int Rh, dou
   // contribution is herm-symmetric, so the lower triangles need not be
   // accumulated on the way)
   const bool mirror = X.B->path != IMSRG_PATH_GROUPED;
+  imsrg_basis_s* B = X.B;
+  if (!mirror && B->term_streams && B->aux_ready) {
+    // Concurrent schedule, same results: RES's two-body part is still
+    // written in the reference order (122, pppp, the other 222_pp_hh
+    // blocks, 222_ph) on s, its one-body part in the order 111, 121, 221;
+    // the GEMM families, the Pandya transform of R and the zero- and
+    // one-body terms run beside them on the auxiliary streams.
+    hipStream_t sg = B->aux[0], sc = B->aux[1], so = B->aux[2];
+    hipEvent_t* ev = B->tev;
+    (void)hipEventRecord(ev[0], s);
+    for (hipStream_t x : B->aux) (void)hipStreamWaitEvent(x, ev[0], 0);
+    // cross-coupled R and W' = Lh LCC RCC^T - Rh RCC LCC^T
+    t_pandya(X, R, Rh, RCC, sc);
+    Slots sw{};
+    sw.p[0] = LCC; sw.p[1] = RCC; sw.p[2] = WCC;
+    sw.st[0] = B->b.cclen; sw.st[1] = B->b.cclen; sw.st[2] = B->b.wclen;
+    run_gemm_x(X, B->ph, sw, sc, (double)Lh, -(double)(Rh * Lh));
+    (void)hipEventRecord(ev[1], sc);
+    // the 222_pp_hh GEMM families
+    const i64 op = B->b.oplen;
+    Slots s1{};
+    s1.p[0] = L; s1.p[1] = R; s1.p[2] = w1;
+    s1.st[0] = op; s1.st[1] = op; s1.st[2] = op;
+    run_gemm_x(X, B->pp1, s1, sg, 1.0, -(double)Rh);
+    (void)hipEventRecord(ev[2], sg);
+    Slots s2{};
+    s2.p[0] = L; s2.p[1] = R; s2.p[2] = X.K2scr;
+    s2.st[0] = op; s2.st[1] = op; s2.st[2] = X.D.sk2scr;
+    run_gemm_x(X, B->k2, s2, sg);
+    Slots s3{};
+    s3.p[0] = L; s3.p[1] = R; s3.p[2] = X.Pscr;
+    s3.st[0] = op; s3.st[1] = op; s3.st[2] = X.D.spscr;
+    run_gemm_x(X, B->k3, s3, sg);
+    (void)hipEventRecord(ev[3], sg);
+    // E0 and the one-body part
+    if (e0) t_e0(X, L, Lh, R, Rh, RES, so);
+    t_111(X, L, Lh, R, Rh, RES, so);
+    t_121(X, L, Lh, R, Rh, RES, RESh, so);
+    (void)hipEventRecord(ev[4], so);
+    // RES two-body in the reference order
+    t_122(X, L, Lh, R, Rh, RES, RESh, s, false);
+    (void)hipStreamWaitEvent(s, ev[2], 0);
+    run(B, B->pairs,
+        [&](const PairTask* t, int n) { launch_pppp_sym(X.D, t, n, L, Lh, R, Rh, RES, w1, w2, 0, s); });
+    (void)hipStreamWaitEvent(s, ev[3], 0);
+    run(B, B->seg_epi, [&](const SegTask* t, int n) {
+      launch_pphh_epilogue(X.D, t, n, L, Lh, R, Rh, RES, w1, w2, X.Pscr, B->poff, B->k3ks, X.K2scr, B->k2off,
+                           B->k2ks, s);
+    });
+    (void)hipStreamWaitEvent(s, ev[4], 0);
+    t_221(X, L, Lh, R, Rh, RES, RESh, w1, w2, s);
+    (void)hipStreamWaitEvent(s, ev[1], 0);
+    launch_comm222ph_chunk(X.D, B->phc, (int)B->phc_h.size(), B->fq, B->fql, B->fqi, B->quad, WCC, Lh, Rh, RES,
+                           RESh, s, 0);
+    launch_sym_lower(X.D, B->symt.dev, B->symt.n, RES, RESh, s);
+    return;
+  }
   t_pandya(X, R, Rh, RCC, s);
   if (e0) t_e0(X, L, Lh, R, Rh, RES, s);
   t_111(X, L, Lh, R, Rh, RES, s);
@@ -2091,6 +2162,13 @@ int imsrg_op_get(imsrg_basis B, const double* op, int q, int g, double* host) {
 int imsrg_set_path(imsrg_basis B, int path) {
   if (!B || (path != IMSRG_PATH_GROUPED && path != IMSRG_PATH_PLAIN)) return fail(IMSRG_EINVAL, "bad path");
   B->path = path;
+  return 0;
+}
+
+int imsrg_set_term_streams(imsrg_basis B, int on) {
+  if (!B) return fail(IMSRG_EINVAL, "null basis");
+  B->term_streams = (on != 0);
+  ++B->plan_ver;   // captured commutator graphs hold the other schedule
   return 0;
 }
 

@@ -117,6 +117,7 @@ SIGNATURES = {
     "imsrg_flow_backward": (_i, [_p, _dp, _p]),
     "imsrg_flow_grad": (_i, [_p, _dp, C.c_double, C.c_double, C.c_double, C.c_double, _i, _i, _dp, _d, _ip, _p]),
     "imsrg_set_gemm_tile": (_i, [_p, _i, _i]),
+    "imsrg_set_term_streams": (_i, [_p, _i]),
     "imsrg_get_gemm_tile": (_i, [_p, _i]),
     "imsrg_profile_kernel": (_i, [_p, _i, _dp, _i, _dp, _i, _dp, _i, _dp, _dp, _dp, _dp, _dp, _i, _d, _p]),
 }

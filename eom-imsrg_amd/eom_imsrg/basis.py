@@ -91,6 +91,10 @@ class Basis:
     def set_path(self, path: int) -> None:
         _lib.check(_lib.lib.imsrg_set_path(self.handle, path))
 
+    def set_term_streams(self, on: bool) -> None:
+        """grouped commutator: independent terms on auxiliary streams (default on)"""
+        _lib.check(_lib.lib.imsrg_set_term_streams(self.handle, 1 if on else 0))
+
     def sixj(self, j1, j2, j3, j4, j5, j6) -> float:
         return float(_lib.lib.imsrg_basis_sixj(self.handle, j1, j2, j3, j4, j5, j6))
 

@@ -343,6 +343,13 @@ int imsrg_flow_grad(imsrg_flow f, const double* T, double ds, double bch_conv, d
 #define IMSRG_GEMM_PH 3     /* 222_ph W' = Lh LCC RCC^T - Rh RCC LCC^T */
 int imsrg_set_gemm_tile(imsrg_basis b, int family, int tile);
 int imsrg_get_gemm_tile(imsrg_basis b, int family);
+/* No counterpart in the reference: on the grouped path, run the full
+ * commutator's GEMM families, the Pandya transform of R and the zero- and
+ * one-body terms on three auxiliary streams beside the two-body
+ * accumulation (on = 1, the default), or all on the caller's stream (0).
+ * Results are identical either way: every output element is accumulated in
+ * the same order. */
+int imsrg_set_term_streams(imsrg_basis b, int on);
 
 /* ------------------------------------------------ instrumentation */
 /* No counterpart in the reference: times `iters` launches of one kernel
