@@ -40,7 +40,7 @@ __global__ __launch_bounds__(256) void k_comm122p(DevBasis B, const SegTask* tas
     s_f[nf1 + k] = R[o];
   }
   __syncthreads();
-  const SegTask t = tasks[blockIdx.x];
+  const SegTask t = tasks[xcd_contig(blockIdx.x, gridDim.x)];   // neighbouring tiles share partner rows
   const i64 e = (i64)t.start + threadIdx.x;
   if (e >= t.cnt) return;
   const int q = t.q, g = t.g;
@@ -277,7 +277,7 @@ __global__ __launch_bounds__(256) void k_comm222ph_chunk(DevBasis B, const PhChu
   __shared__ double s_val[kPhChunkLanes];
   __shared__ i64 s_wcoff[kPhMaxCC];
   __shared__ int s_rlen[kPhMaxCC];
-  const PhChunk c = chunks[blockIdx.x];
+  const PhChunk c = chunks[xcd_contig(blockIdx.x, gridDim.x)];   // neighbouring chunks share W' lines
   const int tid = threadIdx.x;
   const double mpm = -(double)(Lh * Rh);
   for (int x = tid; x < B.ncc; x += blockDim.x) {   // cross-coupled block offsets and row counts
@@ -392,7 +392,7 @@ __global__ __launch_bounds__(256) void k_pandya(DevBasis B, const CCTask* tasks,
   if (!batch_inst(B, inst)) return;
   op += (i64)inst * B.oplen;
   CC += (i64)inst * B.cclen;
-  const CCTask t = tasks[blockIdx.x];
+  const CCTask t = tasks[xcd_contig(blockIdx.x, gridDim.x)];
   const int q1 = t.q1;
   const int r = B.rlen[q1], nc = B.ncol[q1];
   const i64 e = (i64)t.start + threadIdx.x;

@@ -87,6 +87,15 @@ struct VecTask { i64 op_off; i64 vec_off; int len; int mirror; };
 constexpr int kXcds = 8;
 struct XcdMap { int start[kXcds]; int count[kXcds]; };
 
+// Workgroups are dispatched to the XCDs round robin (block b on XCD b % 8);
+// this maps them so that XCD x runs one contiguous range of the n tasks, and
+// neighbouring tasks (which read neighbouring data) share that XCD's L2.
+// A bijection of [0, n).
+__device__ __forceinline__ int xcd_contig(int b, int n) {
+  const int x = b & (kXcds - 1), k = b / kXcds, fl = n / kXcds, rem = n % kXcds;
+  return x * fl + min(x, rem) + k;
+}
+
 __device__ __forceinline__ int xcd_task(const XcdMap& m, int block) {
   if (m.count[0] < 0) return block;
   const int x = block & (kXcds - 1), loc = block / kXcds;

@@ -256,7 +256,7 @@ void launch_221_vjp(const DevBasis& B, const OneTask* t, int nt, int Lh, int Rh,
 // rectangular g: Wb1 = X, Wb2 = -X.
 __global__ __launch_bounds__(256) void k_pphh_wbar(DevBasis B, const SegTask* tasks, const double* RESb, int pm,
                                                    double* Wb1, double* Wb2) {
-  const SegTask t = tasks[blockIdx.x];
+  const SegTask t = tasks[xcd_contig(blockIdx.x, gridDim.x)];
   const i64 e = (i64)t.start + threadIdx.x;
   if (e >= t.cnt) return;
   const int q = t.q, g = t.g;
@@ -284,7 +284,7 @@ void launch_pphh_wbar(const DevBasis& B, const SegTask* t, int nt, const double*
 // X/2, rows and columns of Pauli-forbidden pairs (a,a) at odd J zeroed
 __global__ __launch_bounds__(256) void k_to_dense(DevBasis B, const SegTask* tasks, const double* X, int herm,
                                                   double* cbar) {
-  const SegTask t = tasks[blockIdx.x];
+  const SegTask t = tasks[xcd_contig(blockIdx.x, gridDim.x)];
   const i64 e = (i64)t.start + threadIdx.x;
   if (e >= t.cnt) return;
   const int q = t.q, g = t.g;
@@ -510,7 +510,8 @@ __global__ __launch_bounds__(256) void k_ph_tile(DevBasis B, const PairTask* til
   __shared__ double acc[64 * 64];
   for (int e = threadIdx.x; e < 64 * 64; e += 256) acc[e] = 0.0;
   __syncthreads();
-  const int e0 = tstart[blockIdx.x], e1 = tstart[blockIdx.x + 1];
+  const int wb = xcd_contig(blockIdx.x, gridDim.x);
+  const int e0 = tstart[wb], e1 = tstart[wb + 1];
   for (int e = e0 + threadIdx.x; e < e1; e += 256) {
     const int2 tm = terms[e];
     double v = CV[tm.x];
@@ -519,7 +520,7 @@ __global__ __launch_bounds__(256) void k_ph_tile(DevBasis B, const PairTask* til
     atomicAdd(&acc[tm.y & 4095], v);
   }
   __syncthreads();
-  const PairTask tk = tiles[blockIdx.x];
+  const PairTask tk = tiles[wb];
   const int r = B.rlen[tk.q];
   double* W = U + B.wcoff[tk.q];
   const int x0 = tk.ti * 64, y0 = tk.tj * 64;
@@ -571,7 +572,7 @@ __device__ __forceinline__ double pandya_reader(const DevBasis& B, const double*
 
 __global__ __launch_bounds__(256) void k_pandya_vjp_gather(DevBasis B, const SegTask* tasks, const double* CCb,
                                                            int herm, double* opb) {
-  const SegTask t = tasks[blockIdx.x];
+  const SegTask t = tasks[xcd_contig(blockIdx.x, gridDim.x)];
   const i64 e = (i64)t.start + threadIdx.x;
   if (e >= t.cnt) return;
   const int q = t.q, g = t.g;
