@@ -1121,6 +1121,39 @@ void comm_vjp(imsrg_basis_s* B, const double* L, int Lh, const double* R, int Rh
   const Basis& b = B->b;
   const DevBasis& D = B->D;
   const int pm = Lh * Rh;
+  // The 222_ph chain (Pandya forms, scatter transpose, W' transpose
+  // products) and the dense 122 cotangent write only their own buffers until
+  // the final gathers, so they run on auxiliary streams; every accumulation
+  // into Lb / Rb stays on s in the order below (identical results).
+  const bool fork = B->term_streams && B->aux_ready;
+  hipStream_t sph = fork ? B->aux[1] : s, s122 = fork ? B->aux[2] : s;
+  if (fork) {
+    (void)hipEventRecord(B->tev[0], s);
+    (void)hipStreamWaitEvent(sph, B->tev[0], 0);
+    (void)hipStreamWaitEvent(s122, B->tev[0], 0);
+  }
+  launch_pandya(D, B->cc.dev, B->cc.n, R, Rh, B->aRCC, sph);
+  const double* LCC = LCCin;
+  if (!LCC) {
+    launch_pandya(D, B->cc.dev, B->cc.n, L, Lh, B->aLCC, sph);
+    LCC = B->aLCC;
+  }
+  launch_ph_vjp(D, B->quad, (int)B->quad_h.size(), B->phl, (int)B->phl_h.size(), B->phq, B->phqs, B->phtile,
+                (int)B->phtile_h.size(), B->pht0, B->phterm, RESb, RESh, Lh, Rh, B->SB, B->CV, B->U, sph);
+  double* LCCb = LCCb_acc ? LCCb_acc : B->aLCCb;
+  if (!LCCb_acc) launch_fill(B->aLCCb, b.cclen, 0.0, sph);
+  launch_fill(B->aRCCb, b.cclen, 0.0, sph);
+  Slots sc{};
+  sc.p[0] = LCC; sc.p[1] = B->aRCC; sc.p[2] = LCCb; sc.p[3] = B->aRCCb; sc.p[4] = B->U;
+  if (B->phb.tasks.n)
+    launch_gemm_tiled(B->phb.tile, sc, B->phb.probs, B->phb.tasks.dev, B->phb.tasks.grid, B->phb.tasks.xmap, sph,
+                      (double)Lh, -(double)(Rh * Lh), 1, B->phb.sig);
+  launch_fill(B->cbar, b.off_fpp + (i64)b.P * b.P, 0.0, s122);
+  launch_to_dense(D, B->seg_full.dev, B->seg_full.n, RESb, RESh, B->cbar, s122);
+  if (fork) {
+    (void)hipEventRecord(B->tev[1], sph);
+    (void)hipEventRecord(B->tev[2], s122);
+  }
   if (e0) launch_e0_vjp(D, B->seg_g3.dev, B->seg_g3.n, L, Lh, R, Rh, RESb, Lb, Rb, s);
   launch_111_vjp(D, L, Lh, R, Rh, RESb, Lb, Rb, s);
   launch_121_vjp(D, B->one, (int)B->one_h.size(), L, Lh, R, Rh, RESb, RESh, Lb, Rb, s);
@@ -1133,30 +1166,15 @@ void comm_vjp(imsrg_basis_s* B, const double* L, int Lh, const double* R, int Rh
   for (GemmFamily& f : B->pb)
     if (f.tasks.n) launch_gemm_tiled(f.tile, sp, f.probs, f.tasks.dev, f.tasks.grid, f.tasks.xmap, s, (double)Lh,
                                      (double)Rh, 1, f.sig);
-  // 122: two-body parts from the symmetrised cotangent, one-body part by
-  // the partner sums
-  launch_fill(B->cbar, b.off_fpp + (i64)b.P * b.P, 0.0, s);
-  launch_to_dense(D, B->seg_full.dev, B->seg_full.n, RESb, RESh, B->cbar, s);
+  // 122: two-body parts from the symmetrised cotangent (densified on
+  // s122), one-body part by the partner sums
+  if (fork) (void)hipStreamWaitEvent(s, B->tev[2], 0);
   launch_122_vjp(D, B->seg_tri.dev, B->seg_tri.n, B->partners, B->rowp, B->colp, B->f1off, B->nf1, L, Lh, R, Rh,
                  B->cbar, RESb, RESh, Lb, Rb, s);
-  // 222_ph: scatter transpose into the W' cotangent U, cross-coupled
-  // cotangents (triangular-U GEMMs), Pandya transposes
-  launch_pandya(D, B->cc.dev, B->cc.n, R, Rh, B->aRCC, s);
-  const double* LCC = LCCin;
-  if (!LCC) {
-    launch_pandya(D, B->cc.dev, B->cc.n, L, Lh, B->aLCC, s);
-    LCC = B->aLCC;
-  }
-  launch_ph_vjp(D, B->quad, (int)B->quad_h.size(), B->phl, (int)B->phl_h.size(), B->phq, B->phqs, B->phtile,
-                (int)B->phtile_h.size(), B->pht0, B->phterm, RESb, RESh, Lh, Rh, B->SB, B->CV, B->U, s);
-  double* LCCb = LCCb_acc ? LCCb_acc : B->aLCCb;
-  if (!LCCb_acc) launch_fill(B->aLCCb, b.cclen, 0.0, s);
-  launch_fill(B->aRCCb, b.cclen, 0.0, s);
-  Slots sc{};
-  sc.p[0] = LCC; sc.p[1] = B->aRCC; sc.p[2] = LCCb; sc.p[3] = B->aRCCb; sc.p[4] = B->U;
-  if (B->phb.tasks.n)
-    launch_gemm_tiled(B->phb.tile, sc, B->phb.probs, B->phb.tasks.dev, B->phb.tasks.grid, B->phb.tasks.xmap, s,
-                      (double)Lh, -(double)(Rh * Lh), 1, B->phb.sig);
+  // 222_ph: the Pandya transposes of the cross-coupled cotangents (the
+  // scatter transpose into the W' cotangent U and the triangular-U GEMMs ran
+  // on sph)
+  if (fork) (void)hipStreamWaitEvent(s, B->tev[1], 0);
   launch_pandya_vjp_gather(D, B->seg_pv.dev, B->seg_pv.n, B->aRCCb, Rh, Rb, s);
   if (!LCCb_acc) launch_pandya_vjp_gather(D, B->seg_pv.dev, B->seg_pv.n, B->aLCCb, Lh, Lb, s);
 }

@@ -56,3 +56,27 @@ def test_flow_streams_bitwise(gpu, graphs):
         b.set_term_streams(True)
     assert np.array_equal(logs[0], logs[1])
     assert np.array_equal(HS[0], HS[1])
+
+
+def test_flow_gradient_streams(gpu):
+    """the reverse sweep with the 222_ph chain and the dense 122 cotangent of
+    each commutator transpose on auxiliary streams equals the single-stream
+    sweep to rounding (the 222_ph scatter transpose accumulates its 64 x 64
+    W' cotangent tiles with LDS atomics, so the sweep is reproducible only to
+    rounding in either schedule; a missing wait would show as O(1) errors)"""
+    from eom_imsrg import Flow, adjoint, synth
+    b = basis(4)
+    print("[streams] flow gradient", flush=True)
+    T = torch.from_numpy(synth.hamiltonian_vacuum(b, 0)).cuda()
+    out = []
+    try:
+        for on in (True, False):
+            b.set_term_streams(on)
+            e0, Tb, st = adjoint.flow_grad(Flow(b, graphs=True), T, ckpt=4)
+            torch.cuda.synchronize()
+            out.append((e0, st, Tb.cpu().numpy()))
+    finally:
+        b.set_term_streams(True)
+    assert out[0][0] == out[1][0] and out[0][1] == out[1][1]
+    from conftest import assert_rel
+    assert_rel(out[0][2], out[1][2], rtol=1e-11, what="dE0/dT streams on vs off")
