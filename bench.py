@@ -42,8 +42,10 @@ def _parse():
     ap.add_argument("--batch-cpu-check", type=int, default=1,
                     help="instances of the batch whose flowed E0 is checked against the oracle on the host")
     ap.add_argument("--no-emax10", action="store_true", help="skip the eMax 10 grouped + hipGraph steps")
+    ap.add_argument("--no-setup", action="store_true",
+                    help="skip the Hamiltonian set-up timings (me2j reader, HO one-body part, HO / HF normal ordering)")
     ap.add_argument("--no-tune", action="store_true", help="keep the default GEMM tiles (no autotuner)")
-    ap.add_argument("--leg", choices=["gradient", "batch", "emax10"], default=None,
+    ap.add_argument("--leg", choices=["gradient", "batch", "emax10", "setup"], default=None,
                     help="run only this part and print its JSON object (for profiler runs)")
     ap.add_argument("--kernel-only", default=None,
                     help="launch one kernel class repeatedly and exit (for rocprofv3 counter runs)")
@@ -71,6 +73,9 @@ def main() -> int:
         return 0
     if a.leg == "emax10":
         print(json.dumps(emax10_bench(3, 1, not a.no_tune)))
+        return 0
+    if a.leg == "setup":
+        print(json.dumps(setup_bench(a.emax)))
         return 0
     b = Basis.oxygen16(a.emax)
     tuned = None if a.no_tune else tune.tune(b)
@@ -151,6 +156,7 @@ def main() -> int:
     # ---- many-instance batch (64 x eMax 6) and eMax 10 steps
     batch = None if a.no_batch else batch_bench(64, 6, 8, a.batch_cpu_check)
     e10 = None if a.no_emax10 else emax10_bench(3, 1, not a.no_tune)
+    setup = None if a.no_setup else setup_bench(a.emax)
 
     # ---- roofline of the dominant kernel
     cand = {
@@ -192,6 +198,7 @@ def main() -> int:
         "gradient": grad,
         "batch": batch,
         "emax10": e10,
+        "setup": setup,
         "roofline": roof,
         "cpu_baseline": cpu,
     }
@@ -384,6 +391,53 @@ def emax10_bench(nsteps: int, warm: int, autotune: bool):
             "steps": nsteps, "bch_conv": 1e-10, "commutators_per_step": [int(x[2] + x[3] - 1) for x in log],
             "E0_after_steps": log[-1][0], "commutator_ms": round(cms.value, 3),
             "commutator_tflops": round(F / (cms.value / 1e3) / 1e12, 3), "gemm_tune": tuned}
+
+
+def setup_bench(emax: int):
+    """The Hamiltonian set-up of a run from an interaction file at eMax:
+    read_me2j_interaction (a synthetic eMax-8-style me2j.gz written first,
+    untimed), calculate_h0_harm_osc, then normal ordering in the HO basis
+    (normal_order) and in the Hartree-Fock basis (calc_HF).  Host-to-device
+    transfers are inside the timings (the readers are synchronous)."""
+    import tempfile
+    import torch
+    from eom_imsrg import Basis, SqOp, synth, interaction, hk_orbits, commutators as cm
+    b = Basis.oxygen16(emax)
+    orb = hk_orbits(emax)
+    vals = synth.me2j_values(orb, seed=0)
+    out = {"workload": f"hamiltonian_setup_emax{emax}", "me2j_values": int(len(vals))}
+    H = SqOp.zeros(b, 1)
+    with tempfile.TemporaryDirectory() as d:
+        path = interaction.write_me2j(os.path.join(d, f"synthetic_eMax{emax:02d}_hw20.me2j.gz"), vals)
+        out["me2j_gz_bytes"] = os.path.getsize(path)
+        interaction.read_me2j(H, path, emax, 20.0, 1, 16)        # first read: device tables, file cache
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        interaction.read_me2j(H, path, emax, 20.0, 1, 16)
+        torch.cuda.synchronize()
+        out["read_me2j_ms"] = round(1e3 * (time.perf_counter() - t), 2)
+    t = time.perf_counter()
+    interaction.h0_harm_osc(H, 20.0, 1, 16)
+    torch.cuda.synchronize()
+    out["h0_harm_osc_ms"] = round(1e3 * (time.perf_counter() - t), 3)
+    Hho = H.clone()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    cm.normal_order(Hho)
+    torch.cuda.synchronize()
+    out["normal_order_ms"] = round(1e3 * (time.perf_counter() - t), 3)
+    out["E0_ho"] = Hho.E0
+    Hhf = H.clone()
+    interaction.hartree_fock(Hhf)                                # first call builds the transform plan
+    Hhf = H.clone()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    e, it, _ = interaction.hartree_fock(Hhf)
+    torch.cuda.synchronize()
+    out["hartree_fock_ms"] = round(1e3 * (time.perf_counter() - t), 2)
+    out["hf_iterations"] = it
+    out["e_hf"] = e
+    return out
 
 
 def mirror_index(b):

@@ -14,6 +14,7 @@ Hamiltonian is then normal ordered either in the HO basis
 from __future__ import annotations
 
 import ctypes as C
+import gzip
 
 import numpy as np
 
@@ -63,3 +64,44 @@ def hartree_fock(H: SqOp, maxit: int = 200):
     _lib.check(_lib.lib.imsrg_hartree_fock(H.basis.handle, H.ptr, int(maxit), C.byref(e), C.byref(it),
                                            D.ctypes.data_as(C.POINTER(C.c_double)), _stream()))
     return e.value, it.value, D[:n]
+
+
+def me2j_quadruples(orbits: dict):
+    """(k1, k2, kk1, kk2, 2J) of an me2j file over the isospin orbits of the
+    pn table `orbits` (entries 2k proton, 2k+1 neutron), in file order
+    (interaction_IO.f90:318-350): k1 >= k2, kk1 <= k1, kk2 <= kk1 (<= k2 when
+    kk1 == k1), equal parity, every common 2J; four values each in the order
+    (T, Mt) = (0, 0), (1, -1), (1, 0), (1, 1)"""
+    l, jj = np.asarray(orbits["l"])[::2], np.asarray(orbits["jj"])[::2]
+    nt = len(l)
+    for k1 in range(nt):
+        for k2 in range(k1 + 1):
+            for kk1 in range(k1 + 1):
+                top = k2 if kk1 == k1 else kk1
+                for kk2 in range(top + 1):
+                    if (l[k1] + l[k2]) % 2 != (l[kk1] + l[kk2]) % 2:
+                        continue
+                    lo = max(abs(jj[k1] - jj[k2]), abs(jj[kk1] - jj[kk2]))
+                    hi = min(jj[k1] + jj[k2], jj[kk1] + jj[kk2])
+                    for J in range(lo, hi + 1, 2):
+                        yield k1, k2, kk1, kk2, J
+
+
+def write_me2j(path: str, values, compress: bool = True, header: str = "me2j interaction file",
+               level: int = 6) -> str:
+    """values (in me2j_quadruples order, four per entry) as an me2j file: one
+    header line, ten values a line as f12.7 then f13.7, gzip-compressed
+    unless told not to (the format read_me2j reads)"""
+    v = np.round(np.asarray(values, np.float64), 7)
+    lines = [header.replace("\n", " ") + "\n"]
+    for i in range(0, len(v), 10):
+        chunk = v[i: i + 10]
+        lines.append(f"{chunk[0]:12.7f}" + "".join(f"{x:13.7f}" for x in chunk[1:]) + "\n")
+    data = "".join(lines).encode()
+    if compress:
+        with gzip.open(path, "wb", compresslevel=level) as f:
+            f.write(data)
+    else:
+        with open(path, "wb") as f:
+            f.write(data)
+    return path

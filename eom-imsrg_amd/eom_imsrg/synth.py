@@ -183,3 +183,26 @@ def hermitian_random(b: Basis, seed: int = 2, sigma: float = 1.0) -> np.ndarray:
         _set_one_body(b, op, a, c, float(v), 1)
     _fill_two_body(b, op, rng, 1, sigma, False)
     return op
+
+
+def me2j_values(orbits: dict, seed: int = 0, scale: float = 2.0) -> np.ndarray:
+    """synthetic isospin-coupled two-body matrix elements for an me2j file over
+    the pn orbit table `orbits`: N(0,1) * scale * exp(-(e1+e2+e3+e4)/8) per
+    isospin component (numpy's default_rng(seed)), with the components a
+    pair of identical isospin orbits cannot carry set to zero (two nucleons
+    in one j-shell: 2J/2 even needs T = 1, odd needs T = 0) and
+    charge-independent T = 1 values (the same for every Mt)"""
+    from .interaction import me2j_quadruples
+    rng = np.random.default_rng(seed)
+    e = 2 * np.asarray(orbits["n"])[::2] + np.asarray(orbits["l"])[::2]
+    out = []
+    for k1, k2, kk1, kk2, J in me2j_quadruples(orbits):
+        w = scale * np.exp(-(e[k1] + e[k2] + e[kk1] + e[kk2]) / 8.0)
+        v0, v1 = rng.standard_normal(2) * w
+        ident = k1 == k2 or kk1 == kk2
+        if ident and (J // 2) % 2 == 0:
+            v0 = 0.0
+        if ident and (J // 2) % 2 == 1:
+            v1 = 0.0
+        out.extend((v0, v1, v1, v1))
+    return np.array(out)
