@@ -779,9 +779,52 @@ __global__ __launch_bounds__(256) void k_pppp_sym(DevBasis B, const PairTask* ta
   }
 }
 
+// The same epilogue when w1 is not rewritten (the flow's commutator): one
+// workgroup per output tile (two per off-diagonal pair, blockIdx.x = 2 task
+// + side), its own C tile read straight from memory and only the tile whose
+// transpose it needs staged in LDS (33 KB instead of 66 KB: twice the
+// workgroups per CU).  Same arithmetic as k_pppp_sym.
+__global__ __launch_bounds__(256) void k_pppp_sym1(DevBasis B, const PairTask* tasks, double* RES, const double* w1,
+                                                   int pm) {
+  int inst;
+  if (!batch_inst(B, inst)) return;
+  RES += (i64)inst * B.oplen;
+  w1 += (i64)inst * B.oplen;
+  __shared__ double S[64][65];   // S[x][y] = C(A0 + x, B0 + y), the transposed source tile
+  const PairTask t = tasks[blockIdx.x >> 1];
+  const int side = blockIdx.x & 1;
+  if (side == 1 && t.ti == t.tj) return;
+  const int q = t.q;
+  const int np = B.dims[q * 3];
+  const i64 o1 = B.goff[q * 6 + 0];
+  const double* Cm = w1 + o1;
+  double* X1 = RES + o1;
+  // output tile rows R0.., cols C0..; source tile rows C0.., cols R0..
+  const int R0 = (side == 0 ? t.ti : t.tj) * 64, C0 = (side == 0 ? t.tj : t.ti) * 64;
+  for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+    const int x = e & 63, y = e >> 6;
+    S[x][y] = (C0 + x < np && R0 + y < np) ? Cm[(C0 + x) + (i64)(R0 + y) * np] : 0.0;
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+    const int x = e & 63, y = e >> 6;
+    const int gx = R0 + x, gy = C0 + y;
+    if (gx < np && gy < np) {
+      const i64 o = gx + (i64)gy * np;
+      const double cxy = Cm[o];
+      const double cyx = S[y][x];
+      X1[o] = X1[o] + cxy - cyx * pm;
+    }
+  }
+}
+
 void launch_pppp_sym(const DevBasis& B, const PairTask* t, int nt, const double* L, int Lh, const double* R, int Rh,
                      double* RES, double* w1, double* w2, int ref_outputs, hipStream_t s) {
   if (nt <= 0) return;
+  if (!ref_outputs) {
+    hipLaunchKernelGGL(k_pppp_sym1, dim3(2 * nt, B.ninst), dim3(256), 0, s, B, t, RES, (const double*)w1, Lh * Rh);
+    return;
+  }
   hipLaunchKernelGGL(k_pppp_sym, dim3(nt, B.ninst), dim3(256), 0, s, B, t, L, Lh, R, Rh, RES, w1, w2, ref_outputs);
 }
 
