@@ -323,8 +323,9 @@ void launch_to_dense(const DevBasis& B, const SegTask* t, int nt, const double* 
 // * one-body part: the partner sums of k_comm122p weighted by the cotangent
 //   of each upper element, accumulated per workgroup in LDS (compact index
 //   fk) and added to Lb / Rb once at the end.
-// Persistent grid; LDS holds the one-body values of L and R and the
-// one-body accumulators.
+// One workgroup per task, in XCD-contiguous order (neighbouring tiles share
+// partner rows in one L2); LDS holds the one-body values of L and R and the
+// workgroup's one-body accumulators.
 __global__ __launch_bounds__(256) void k_122_vjp(DevBasis B, const SegTask* tasks, int ntasks, const Partner* part,
                                                  const int* rowp, const int* colp, const int* f1off, int nf1,
                                                  const double* L, int Lh, const double* R, int Rh,
@@ -344,7 +345,7 @@ __global__ __launch_bounds__(256) void k_122_vjp(DevBasis B, const SegTask* task
   }
   __syncthreads();
   const int ch = RESh;
-  for (int ti = blockIdx.x; ti < ntasks; ti += gridDim.x) {
+  for (int ti = xcd_contig(blockIdx.x, gridDim.x); ti < ntasks; ti += gridDim.x) {
     const SegTask t = tasks[ti];
     const i64 e = (i64)t.start + threadIdx.x;
     if (e >= t.cnt) continue;
@@ -433,7 +434,7 @@ void launch_122_vjp(const DevBasis& B, const SegTask* t, int nt, const Partner* 
                     const int* f1off, int nf1, const double* L, int Lh, const double* R, int Rh, const double* cbar,
                     const double* RESb, int RESh, double* Lb, double* Rb, hipStream_t s) {
   if (nt <= 0) return;
-  const int grid = nt < 2048 ? nt : 2048;
+  const int grid = nt;
   hipLaunchKernelGGL(k_122_vjp, dim3(grid), dim3(256), 4 * nf1 * sizeof(double), s, B, t, nt, part, rowp, colp, f1off,
                      nf1, L, Lh, R, Rh, cbar, RESb, RESh, Lb, Rb);
 }
