@@ -101,3 +101,23 @@ def test_flow_gradient_vs_oracle_c14(gpu):
     assert st == st_o and abs(e0 - e0_o) < 1e-8
     assert_rel(adjoint.symmetric_gradient(b, Tb.cpu().numpy(), 1), adjoint.symmetric_gradient(b, Tb_o, 1), rtol=1e-8,
                what="dE0/dT 14C")
+
+
+@pytest.mark.parametrize("name", ["He4", "O22", "C14"])
+def test_hartree_fock_vs_oracle(gpu, name):
+    """Hartree-Fock (calc_HF) for other reference states at eMax 4: the
+    occupied HF orbits of a block are its lowest eigenvectors, as many as
+    the block has hole orbits (22O fills the neutron 0d5/2 block's lower
+    orbit, 14C leaves the proton 0p1/2 block empty); same iterations, E0,
+    Fock matrix, two-body part and orbits as the oracle to 1e-10"""
+    from eom_imsrg import SqOp, interaction, synth
+    b, o = _bases(name, 4)
+    print(f"[nucleus hf] {name} emax=4", flush=True)
+    H = synth.hamiltonian_vacuum(b, seed=4, hw=20.0)
+    ref, Dref, itref = o.hartree_fock(H)
+    Hd = SqOp.from_numpy(b, H, 1)
+    e, it, D = interaction.hartree_fock(Hd)
+    assert it == itref
+    assert abs(e - ref[0]) < 1e-10 * abs(ref[0])
+    assert_rel(Hd.numpy(), ref, rtol=1e-10, what=f"HF {name}")
+    np.testing.assert_allclose(D, Dref, rtol=1e-10, atol=1e-12)
