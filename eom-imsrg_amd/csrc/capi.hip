@@ -2483,6 +2483,48 @@ int ensure_hf(imsrg_basis_s* B) {
   return 0;
 }
 
+// E0 and the one-body part of the host copy h uploaded to H, then H's
+// two-body part transformed by the per-channel matrices M on the device
+int hf_transform_dev(imsrg_basis_s* B, const double* h, const std::vector<double>& M, double* H, hipStream_t s) {
+  const Basis& b = B->b;
+  int rc;
+  if ((rc = ensure_hf(B))) return rc;
+  double *V = B->hf_buf, *T = B->hf_buf + B->hf_len, *Md = B->hf_buf + 2 * B->hf_len;
+  const size_t n1 = (size_t)(b.off_fpp + (i64)b.P * b.P);
+  HIPCHK(hipMemcpyAsync(H, h, n1 * sizeof(double), hipMemcpyHostToDevice, s));
+  if (!M.empty()) HIPCHK(hipMemcpyAsync(Md, M.data(), M.size() * sizeof(double), hipMemcpyHostToDevice, s));
+  launch_hf_pack(B->D, B->seg_full.dev, B->seg_full.n, H, V, B->hf_off, s);
+  Slots sl{};
+  sl.p[0] = V; sl.p[1] = T; sl.p[2] = Md;
+  run_gemm(B, B->hf1, sl, s);
+  run_gemm(B, B->hf2, sl, s);
+  launch_hf_unpack(B->D, B->seg_full.dev, B->seg_full.n, V, B->hf_off, H, s);
+  HIPCHK(hipGetLastError());
+  // the host copy must outlive the upload
+  HIPCHK(hipStreamSynchronize(s));
+  return 0;
+}
+
+int imsrg_sqop_to_hf(imsrg_basis B, const double* coefs, double* Op, int herm, imsrg_stream stream) {
+  if (!coefs || !Op) return fail(IMSRG_EINVAL, "null argument");
+  if (herm != 1) return fail(IMSRG_EINVAL, "sqop_to_HF transforms hermitian scalar operators (herm = 1)");
+  PREP();
+  const Basis& b = B->b;
+  std::vector<double> h((size_t)b.oplen);
+  HIPCHK(hipMemcpyAsync(h.data(), Op, h.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  std::vector<i64> moff;
+  std::vector<double> M;
+  std::string err = sqop_to_hf_host(b, coefs, h.data(), moff, M);
+  if (!err.empty()) return fail(IMSRG_EINVAL, err);
+  int rc;
+  if ((rc = hf_transform_dev(B, h.data(), M, Op, s))) return rc;
+  launch_normal_order(B->D, Op, B->dval, s);   // normal_order (HF_mod.f90:204)
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(s));
+  return 0;
+}
+
 int imsrg_hartree_fock(imsrg_basis B, double* H, int maxit, double* ehf, int* iterations, double* coefs,
                        imsrg_stream stream) {
   if (!H) return fail(IMSRG_EINVAL, "null argument");
@@ -2497,19 +2539,7 @@ int imsrg_hartree_fock(imsrg_basis B, double* H, int maxit, double* ehf, int* it
   std::string err = hartree_fock_host(b, h.data(), maxit, r, M);
   if (!err.empty()) return fail(IMSRG_ESTEPS, err);
   int rc;
-  if ((rc = ensure_hf(B))) return rc;
-  double *V = B->hf_buf, *T = B->hf_buf + B->hf_len, *Md = B->hf_buf + 2 * B->hf_len;
-  // E0 and the one-body part (Fock matrix in the HF basis)
-  const size_t n1 = (size_t)(b.off_fpp + (i64)b.P * b.P);
-  HIPCHK(hipMemcpyAsync(H, h.data(), n1 * sizeof(double), hipMemcpyHostToDevice, s));
-  if (!M.empty()) HIPCHK(hipMemcpyAsync(Md, M.data(), M.size() * sizeof(double), hipMemcpyHostToDevice, s));
-  launch_hf_pack(B->D, B->seg_full.dev, B->seg_full.n, H, V, B->hf_off, s);
-  Slots sl{};
-  sl.p[0] = V; sl.p[1] = T; sl.p[2] = Md;
-  run_gemm(B, B->hf1, sl, s);
-  run_gemm(B, B->hf2, sl, s);
-  launch_hf_unpack(B->D, B->seg_full.dev, B->seg_full.n, V, B->hf_off, H, s);
-  HIPCHK(hipGetLastError());
+  if ((rc = hf_transform_dev(B, h.data(), M, H, s))) return rc;
   HIPCHK(hipStreamSynchronize(s));
   if (ehf) *ehf = r.ehf;
   if (iterations) *iterations = r.iterations;

@@ -126,7 +126,73 @@ void rotate(int n, const double* D, double* X) {
   std::copy(y.begin(), y.end(), X);
 }
 
+
+// the Fock-matrix remap of transform_1b_to_HF (HF_mod.f90:531-560): block q's
+// rotated matrix F (n x n) into fpp / fph / fhh of op
+void write_one_body(const Basis& b, int q, const double* Fq, double* op) {
+  const int n = b.spstart[q + 1] - b.spstart[q];
+  auto st = [&](int i) { return b.spmem[b.spstart[q] + i]; };
+  for (int a = 0; a < n; ++a)
+    for (int c = a; c < n; ++c) {
+      const int i = st(a), j = st(c);
+      const int c1 = b.con[i], c2 = b.con[j];
+      if (c1 + c2 == 0) {
+        op[b.off_fpp + b.cidx[i] + (i64)b.cidx[j] * b.P] = Fq[a + c * n];
+        op[b.off_fpp + b.cidx[j] + (i64)b.cidx[i] * b.P] = Fq[a + c * n];
+      } else if (c1 + c2 == 1) {
+        if (c2 > c1) op[b.off_fph + b.cidx[i] + (i64)b.cidx[j] * b.P] = Fq[a + c * n];
+        else op[b.off_fph + b.cidx[j] + (i64)b.cidx[i] * b.P] = Fq[c + a * n];
+      } else {
+        op[b.off_fhh + b.cidx[i] + (i64)b.cidx[j] * b.A] = Fq[a + c * n];
+        op[b.off_fhh + b.cidx[j] + (i64)b.cidx[i] * b.A] = Fq[a + c * n];
+      }
+    }
+}
+
 }  // namespace
+
+// transform_2b_to_HF (HF_mod.f90:567-692): the four products V1 - s (V2 + V3)
+// + V4 are one two-sided product with M = C - s Cr, s = (-1)^J, per channel
+void transform_matrices(const Basis& b, const std::vector<i64>& boff, const std::vector<double>& D,
+                        std::vector<i64>& moff, std::vector<double>& M) {
+  const int nbk = (int)b.spstart.size() - 1;
+  auto bn = [&](int q) { return b.spstart[q + 1] - b.spstart[q]; };
+  auto st = [&](int q, int i) { return b.spmem[b.spstart[q] + i]; };
+  std::vector<double> Ds((size_t)b.N * b.N, 0.0);
+  for (int q = 0; q < nbk; ++q)
+    for (int i = 0; i < bn(q); ++i)
+      for (int j = 0; j < bn(q); ++j) Ds[st(q, i) + (size_t)st(q, j) * b.N] = D[boff[q] + i + (i64)j * bn(q)];
+  auto dsm = [&](int x, int y) { return Ds[x + (size_t)y * b.N]; };
+  moff.assign(b.nblocks + 1, 0);
+  for (int q = 0; q < b.nblocks; ++q) {
+    const i64 nt = (i64)b.dims[q * 3] + b.dims[q * 3 + 1] + b.dims[q * 3 + 2];
+    moff[q + 1] = moff[q] + nt * nt;
+  }
+  M.assign((size_t)moff[b.nblocks], 0.0);
+  for (int q = 0; q < b.nblocks; ++q) {
+    const int np = b.dims[q * 3], nb = b.dims[q * 3 + 1], nh = b.dims[q * 3 + 2], nt = np + nb + nh;
+    if (nt == 0) continue;
+    std::vector<int> qa(nt), qb(nt);
+    const int r0[3] = {nh + nb, nh, 0};   // rows of the pp, ph, hh pairs
+    for (int c = 0; c < 3; ++c)
+      for (int x = 0; x < b.dims[q * 3 + c]; ++x) {
+        qa[r0[c] + x] = b.pl[q * 3 + c][2 * x];
+        qb[r0[c] + x] = b.pl[q * 3 + c][2 * x + 1];
+      }
+    const double s = m1pow(b.chJ[q] / 2);
+    double* Mq = M.data() + moff[q];
+    for (int II = 0; II < nt; ++II) {
+      const int i = qa[II], j = qb[II];
+      for (int JJ = 0; JJ < nt; ++JJ) {
+        const int a = qa[JJ], bb = qb[JJ];
+        const double nrm = std::sqrt(1.0 + (a == bb)) / std::sqrt(1.0 + (i == j));
+        const double c = dsm(a, i) * dsm(bb, j) * nrm;
+        const double cr = a == bb ? 0.0 : dsm(bb, i) * dsm(a, j) * m1pow((b.jj[a] + b.jj[bb]) / 2) * nrm;
+        Mq[JJ + (i64)II * nt] = c - s * cr;
+      }
+    }
+  }
+}
 
 std::string hartree_fock_host(const Basis& b, double* op, int maxit, HFResult& r, std::vector<double>& M) {
   const int nbk = (int)b.spstart.size() - 1;
@@ -223,21 +289,7 @@ std::string hartree_fock_host(const Basis& b, double* op, int maxit, HFResult& r
     rotate(n, Dq, Fq);
     rotate(n, Dq, Tq);
     rotate(n, Dq, Vq);
-    for (int a = 0; a < n; ++a)
-      for (int c = a; c < n; ++c) {
-        const int i = st(q, a), j = st(q, c);
-        const int c1 = b.con[i], c2 = b.con[j];
-        if (c1 + c2 == 0) {
-          op[b.off_fpp + b.cidx[i] + (i64)b.cidx[j] * b.P] = Fq[a + c * n];
-          op[b.off_fpp + b.cidx[j] + (i64)b.cidx[i] * b.P] = Fq[a + c * n];
-        } else if (c1 + c2 == 1) {
-          if (c2 > c1) op[b.off_fph + b.cidx[i] + (i64)b.cidx[j] * b.P] = Fq[a + c * n];
-          else op[b.off_fph + b.cidx[j] + (i64)b.cidx[i] * b.P] = Fq[c + a * n];
-        } else {
-          op[b.off_fhh + b.cidx[i] + (i64)b.cidx[j] * b.A] = Fq[a + c * n];
-          op[b.off_fhh + b.cidx[j] + (i64)b.cidx[i] * b.A] = Fq[a + c * n];
-        }
-      }
+    write_one_body(b, q, Fq, op);
   }
   for (int q = 0; q < nbk; ++q) {
     const int n = bn(q);
@@ -247,42 +299,32 @@ std::string hartree_fock_host(const Basis& b, double* op, int maxit, HFResult& r
   }
   r.ehf = ehf;
   op[0] = ehf;
-  // transform_2b_to_HF (567-692): the four products V1 - s (V2 + V3) + V4
-  // are one two-sided product with M = C - s Cr, s = (-1)^J
-  std::vector<double> Ds((size_t)b.N * b.N, 0.0);
-  for (int q = 0; q < nbk; ++q)
-    for (int i = 0; i < bn(q); ++i)
-      for (int j = 0; j < bn(q); ++j) Ds[st(q, i) + (size_t)st(q, j) * b.N] = r.D[r.boff[q] + i + (i64)j * bn(q)];
-  auto dsm = [&](int x, int y) { return Ds[x + (size_t)y * b.N]; };
-  r.moff.assign(b.nblocks + 1, 0);
-  for (int q = 0; q < b.nblocks; ++q) {
-    const i64 nt = (i64)b.dims[q * 3] + b.dims[q * 3 + 1] + b.dims[q * 3 + 2];
-    r.moff[q + 1] = r.moff[q] + nt * nt;
+  transform_matrices(b, r.boff, r.D, r.moff, M);
+  return "";
+}
+
+std::string sqop_to_hf_host(const Basis& b, const double* D, double* op, std::vector<i64>& moff,
+                            std::vector<double>& M) {
+  const int nbk = (int)b.spstart.size() - 1;
+  std::vector<i64> boff(nbk + 1, 0);
+  for (int q = 0; q < nbk; ++q) {
+    const i64 n = b.spstart[q + 1] - b.spstart[q];
+    boff[q + 1] = boff[q] + n * n;
   }
-  M.assign((size_t)r.moff[b.nblocks], 0.0);
-  for (int q = 0; q < b.nblocks; ++q) {
-    const int np = b.dims[q * 3], nb = b.dims[q * 3 + 1], nh = b.dims[q * 3 + 2], nt = np + nb + nh;
-    if (nt == 0) continue;
-    std::vector<int> qa(nt), qb(nt);
-    const int r0[3] = {nh + nb, nh, 0};   // rows of the pp, ph, hh pairs
-    for (int c = 0; c < 3; ++c)
-      for (int x = 0; x < b.dims[q * 3 + c]; ++x) {
-        qa[r0[c] + x] = b.pl[q * 3 + c][2 * x];
-        qb[r0[c] + x] = b.pl[q * 3 + c][2 * x + 1];
+  // write_kin_matrix (HF_mod.f90:211-240), then the transform_1b_to_HF pair
+  for (int q = 0; q < nbk; ++q) {
+    const int n = b.spstart[q + 1] - b.spstart[q];
+    if (n == 0) continue;
+    std::vector<double> T((size_t)n * n);
+    for (int i = 0; i < n; ++i)
+      for (int j = i; j < n; ++j) {
+        T[i + (size_t)j * n] = f_host(b, op, b.spmem[b.spstart[q] + i], b.spmem[b.spstart[q] + j]);
+        T[j + (size_t)i * n] = T[i + (size_t)j * n];
       }
-    const double s = m1pow(b.chJ[q] / 2);
-    double* Mq = M.data() + r.moff[q];
-    for (int II = 0; II < nt; ++II) {
-      const int i = qa[II], j = qb[II];
-      for (int JJ = 0; JJ < nt; ++JJ) {
-        const int a = qa[JJ], bb = qb[JJ];
-        const double nrm = std::sqrt(1.0 + (a == bb)) / std::sqrt(1.0 + (i == j));
-        const double c = dsm(a, i) * dsm(bb, j) * nrm;
-        const double cr = a == bb ? 0.0 : dsm(bb, i) * dsm(a, j) * m1pow((b.jj[a] + b.jj[bb]) / 2) * nrm;
-        Mq[JJ + (i64)II * nt] = c - s * cr;
-      }
-    }
+    rotate(n, D + boff[q], T.data());
+    write_one_body(b, q, T.data(), op);
   }
+  transform_matrices(b, boff, std::vector<double>(D, D + boff[nbk]), moff, M);
   return "";
 }
 

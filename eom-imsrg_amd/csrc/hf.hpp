@@ -26,4 +26,16 @@ struct HFResult {
 // e_HF), and `M` the per-channel two-body transform matrices.
 std::string hartree_fock_host(const Basis& b, double* op, int maxit, HFResult& r, std::vector<double>& M);
 
+// The per-channel matrices M of the two-body transform by the HF orbits D
+// (blocks at boff), channel q at [moff[q], moff[q+1]).
+void transform_matrices(const Basis& b, const std::vector<i64>& boff, const std::vector<double>& D,
+                        std::vector<i64>& moff, std::vector<double>& M);
+
+// sqop_to_HF (HF_mod.f90:181-209) for a hermitian scalar operator, host
+// part: the one-body part of `op` (host copy, all oplen doubles) rotated by
+// the HF orbits D (IMSRG_SIZE_SPMAT doubles), and the two-body transform
+// matrices; the caller transforms the two-body part and normal orders.
+std::string sqop_to_hf_host(const Basis& b, const double* D, double* op, std::vector<i64>& moff,
+                            std::vector<double>& M);
+
 }  // namespace imsrg

@@ -70,6 +70,7 @@ SIGNATURES = {
     "imsrg_read_me2j": (_i, [_p, C.c_char_p, _i, _ip, _ip, _ip, _ip, C.c_double, _i, C.c_double, C.c_double, _i,
                              _dp, _p]),
     "imsrg_hartree_fock": (_i, [_p, _dp, _i, _d, _ip, _d, _p]),
+    "imsrg_sqop_to_hf": (_i, [_p, _d, _dp, _i, _p]),
     "imsrg_transform_sq_op_bch": (_i, [_p, _dp, _i, _dp, C.c_double, _i, _ip, _p]),
     "imsrg_dhds": (_i, [_p, _dp, _dp, _p]),
     "imsrg_commutator_110": (_i, [_p, _dp, _i, _dp, _i, _d, _p]),

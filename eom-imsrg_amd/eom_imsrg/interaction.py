@@ -105,3 +105,13 @@ def write_me2j(path: str, values, compress: bool = True, header: str = "me2j int
         with open(path, "wb") as f:
             f.write(data)
     return path
+
+
+def sqop_to_hf(Op: SqOp, coefs: np.ndarray) -> SqOp:
+    """sqop_to_HF (HF_mod.f90:181-209): the hermitian scalar operator Op
+    (vacuum form, HO basis) <- rotated to the HF orbits `coefs` (as
+    hartree_fock returns them) and normal ordered"""
+    D = np.ascontiguousarray(coefs, np.float64)
+    _lib.check(_lib.lib.imsrg_sqop_to_hf(Op.basis.handle, D.ctypes.data_as(C.POINTER(C.c_double)), Op.ptr,
+                                         int(Op.herm), _stream()))
+    return Op

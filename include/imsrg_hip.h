@@ -150,6 +150,12 @@ int imsrg_read_me2j(imsrg_basis b, const char* path, int nfile, const int* n, co
  * Synchronous. */
 int imsrg_hartree_fock(imsrg_basis b, double* H, int maxit, double* ehf, int* iterations, double* coefs,
                        imsrg_stream stream);
+/* sqop_to_HF, HF_mod.f90:181-209, for a hermitian scalar operator (herm =
+ * 1; the reference's tensor branch is not built): Op (device, the vacuum
+ * operator in the HO basis) <- its one-body part rotated to the HF orbits
+ * `coefs` (host, IMSRG_SIZE_SPMAT doubles, as imsrg_hartree_fock returns
+ * them), its two-body part transformed, then normal ordered.  Synchronous. */
+int imsrg_sqop_to_hf(imsrg_basis b, const double* coefs, double* Op, int herm, imsrg_stream stream);
 
 /* ------------------------------------------------ commutator terms */
 /* commutator_110, commutators.f90:10-33 (returns the value; synchronises) */

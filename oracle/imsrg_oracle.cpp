@@ -1184,6 +1184,7 @@ void orc_hf_finish(void* h, double* H, const double* D, double* F, double* T, do
   hf_transform_2b(o, D);
 }
 void orc_hf_transform_2b(void* h, double* H, const double* D) { OPS(h) Op o{&b, H, 1}; hf_transform_2b(o, D); }
+void orc_sqop_to_hf(void* h, double* O, const double* D) { OPS(h) Op o{&b, O, 1}; hf_sqop_to_hf(o, D); }
 double orc_frob_norm(void* h, const double* op) { OPS(h) return frob_norm(Op{&b, (double*)op, 1}); }
 void orc_add(void* h, const double* A, double ax, const double* B, double bx, double* C) {
   OPS(h) add_op(b, A, ax, B, bx, C);

@@ -67,6 +67,7 @@ def lib():
             "orc_sp_layout": (i, [vp, ip]), "orc_hf_kin": (None, [vp, dp, dp]),
             "orc_hf_density": (None, [vp, dp, dp]), "orc_hf_gamma": (None, [vp, dp, dp, dp]),
             "orc_hf_finish": (None, [vp, dp, dp, dp, dp, dp]), "orc_hf_transform_2b": (None, [vp, dp, dp]),
+            "orc_sqop_to_hf": (None, [vp, dp, dp]),
         }
         for k, (r, a) in sig.items():
             f = getattr(L, k)
@@ -381,6 +382,12 @@ class OracleBasis:
     def hf_transform_2b(self, H, D):
         out = H.copy()
         lib().orc_hf_transform_2b(self.h, _dp(out), _dp(np.ascontiguousarray(D, np.float64)))
+        return out
+
+    def sqop_to_hf(self, O, D):
+        """sqop_to_HF, HF_mod.f90:181-209 (hermitian scalar operator)"""
+        out = O.copy()
+        lib().orc_sqop_to_hf(self.h, _dp(out), _dp(np.ascontiguousarray(D, np.float64)))
         return out
 
     def hartree_fock(self, H):

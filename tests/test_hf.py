@@ -106,6 +106,10 @@ def test_hf_is_normal_ordered_rotated_vacuum(emax):
                 synth._set_one_body(b, Hrot, x, y, tr[x, y], 1)
     Hrot[0] = 0.0
     no = ob.normal_order(Hrot)
+    # sqop_to_HF of the vacuum Hamiltonian is exactly this rotated, normal
+    # ordered operator (its one-body rotation is the oracle's, not numpy's)
+    sq = ob.sqop_to_hf(H, D)
+    np.testing.assert_allclose(sq, no, rtol=1e-12, atol=1e-12)
     assert abs(no[0] - Hhf[0]) < 1e-5 * abs(Hhf[0])
     one = slice(1, b.off_fpp + b.npart * b.npart)
     assert np.max(np.abs(no[one] - Hhf[one])) < 1e-4
@@ -175,3 +179,22 @@ def test_flow_from_hf(gpu):
     assert len(log) == str_
     assert np.array_equal(log[:, 3], logr[:, 3])
     assert np.max(np.abs(log[:, 1] - logr[:, 1])) < 1e-8
+
+
+@pytest.mark.gpu
+def test_sqop_to_hf_matches_oracle(gpu):
+    """an observable (a second hermitian scalar operator) carried to the HF
+    basis of H: library vs oracle to 1e-10"""
+    from conftest import assert_rel
+    from eom_imsrg import SqOp, interaction, synth
+    b, ob = basis(4), oracle_basis(4)
+    print("[sqop_to_hf] emax=4", flush=True)
+    H = synth.hamiltonian_vacuum(b, seed=4, hw=20.0)
+    Ob = synth.hamiltonian_vacuum(b, seed=9, hw=7.0)
+    _, D, _ = ob.hartree_fock(H)
+    ref = ob.sqop_to_hf(Ob, D)
+    Hd = SqOp.from_numpy(b, H, 1)
+    _, _, Dg = interaction.hartree_fock(Hd)
+    Od = SqOp.from_numpy(b, Ob, 1)
+    interaction.sqop_to_hf(Od, Dg)
+    assert_rel(Od.numpy(), ref, rtol=1e-10, what="sqop_to_HF")
