@@ -10,6 +10,7 @@
 #include <cstring>
 #include <initializer_list>
 #include <map>
+#include <mutex>
 #include <string>
 #include <tuple>
 #include <type_traits>
@@ -2027,6 +2028,34 @@ void vec_plan(const Basis& b, std::vector<VecTask>* out, i64* neq) {
   if (neq) *neq = k;
 }
 
+}  // namespace
+
+// Fixed device workspace of imsrg_commutator_222_ph_noalloc: 64 MiB per GPU,
+// held by every GPU the library runs on until the process ends.
+constexpr i64 kNoAllocWccDoubles = (i64)(64 << 20) / (i64)sizeof(double);
+static __device__ __align__(256) double scratch[kNoAllocWccDoubles];
+
+namespace {
+
+// device address of `scratch` per device, resolved once per device with
+// that device current
+int noalloc_scratch(double** out) {
+  static std::mutex mu;
+  static std::map<int, double*> addr;
+  int dev = -1;
+  HIPCHK(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(mu);
+  auto it = addr.find(dev);
+  if (it == addr.end()) {
+    double* p = nullptr;
+    HIPCHK(hipGetSymbolAddress((void**)&p, HIP_SYMBOL(scratch)));
+    if (!p) return fail(IMSRG_EHIP, "hipGetSymbolAddress returned no address for the no-allocation workspace");
+    it = addr.emplace(dev, p).first;
+  }
+  *out = it->second;
+  return 0;
+}
+
 int ensure_vec(imsrg_basis_s* B) {
   if (B->vec) return 0;
   int rc;
@@ -2936,6 +2965,32 @@ int imsrg_set_gemm_tile(imsrg_basis B, int family, int tile) {
   B->tile_req[family] = tile;
   if (!B->dev_ready) return 0;   // applied when the plan is built
   return family_tasks(B, *f[family], shape);
+}
+
+// ------------------------------------------------ no-allocation variant
+// imsrg_commutator_222_ph with the WCC workspace in the library's fixed
+// device array instead of a caller buffer.  The array's device address is
+// looked up once per device (hipGetSymbolAddress, no allocation) on the
+// first call made with that device current; host code uses only that
+// address, never the array's name.
+int imsrg_commutator_222_ph_noalloc(imsrg_basis B, const double* LCC, int Lh, const double* RCC, int Rh,
+                                    double* RES, int RESh, imsrg_stream stream) {
+  if (int a_ = check_args({LCC, RCC, RES}, {Lh, Rh, RESh})) return a_;
+  if (!B) return fail(IMSRG_EINVAL, "null basis");
+  if (B->b.wclen > kNoAllocWccDoubles)
+    return fail(IMSRG_EINVAL, "imsrg_commutator_222_ph_noalloc: the basis needs " + std::to_string(B->b.wclen) +
+                                  " doubles of WCC workspace, more than the fixed " +
+                                  std::to_string(kNoAllocWccDoubles) + " (64 MiB)");
+  if (!B->dev_ready)
+    return fail(IMSRG_EINVAL,
+                "imsrg_commutator_222_ph_noalloc: the basis has no device tables yet (any allocating call on the "
+                "basis, e.g. imsrg_flow_create or imsrg_commutator_222_ph, uploads them)");
+  double* wcc = nullptr;
+  if (int rc = noalloc_scratch(&wcc)) return rc;
+  hipStream_t s = S(stream);
+  t_ph(basis_exec(B), LCC, Lh, RCC, Rh, RES, RESh, wcc, s);
+  HIPCHK(hipGetLastError());
+  return 0;
 }
 
 int imsrg_get_gemm_tile(imsrg_basis B, int family) {

@@ -83,6 +83,7 @@ SIGNATURES = {
     "imsrg_commutator_221": (_i, [_p, _dp, _i, _dp, _i, _dp, _i, _dp, _dp, _p]),
     "imsrg_calculate_cross_coupled": (_i, [_p, _dp, _i, _dp, _p]),
     "imsrg_commutator_222_ph": (_i, [_p, _dp, _i, _dp, _i, _dp, _i, _dp, _p]),
+    "imsrg_commutator_222_ph_noalloc": (_i, [_p, _dp, _i, _dp, _i, _dp, _i, _p]),
     "imsrg_add_sq_op": (_i, [_p, _dp, C.c_double, _dp, C.c_double, _dp, _p]),
     "imsrg_mat_frob_norm": (_i, [_p, _dp, _d, _p]),
     "imsrg_build_gs_white": (_i, [_p, _dp, _dp, _p]),

@@ -192,6 +192,20 @@ int imsrg_calculate_cross_coupled(imsrg_basis b, const double* op, int herm, dou
 int imsrg_commutator_222_ph(imsrg_basis b, const double* LCC, int Lherm, const double* RCC, int Rherm,
                             double* RES, int RESherm, double* WCC, imsrg_stream stream);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc): imsrg_commutator_222_ph with the WCC workspace taken from a
+ * fixed 64 MiB device array inside the library (8,388,608 doubles per GPU,
+ * held until the process ends) instead of a caller buffer.  Allocates
+ * nothing.  Requires IMSRG_SIZE_WCC <= 8,388,608 (the 16O bases up to eMax
+ * 6), else IMSRG_EINVAL; requires the basis's device tables to exist (any
+ * earlier allocating call on the basis uploads them), else IMSRG_EINVAL.
+ * Every call on one GPU shares that GPU's array: calls that overlap on one
+ * GPU (other streams, other threads) race.  Same results as
+ * imsrg_commutator_222_ph. */
+int imsrg_commutator_222_ph_noalloc(imsrg_basis b, const double* LCC, int Lherm, const double* RCC, int Rherm,
+                                    double* RES, int RESherm, imsrg_stream stream);
+
 /* ------------------------------------------------ operator algebra */
 /* add_sq_op, basic_IMSRG.f90:3551-3583: C = ax*A + bx*B (C may alias A or B) */
 int imsrg_add_sq_op(imsrg_basis b, const double* A, double ax, const double* B, double bx, double* C,
