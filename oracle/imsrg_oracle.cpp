@@ -26,6 +26,13 @@
 //   * 6j / 9j closed forms, symmetries and orthogonality (tests);
 //   * BCH closed forms (Omega = 0; one-body Omega = orbital rotation), the
 //     Magnus identity dOmega = eta for Omega = 0, antisymmetry of [A,B];
+//   * frob_norm against the Frobenius norm of the full (anti)hermitian
+//     matrices assembled independently, the all-ones closed form and one
+//     element per block class (weights 1 and 2);
+//   * the magnus_decouple loop: each step recomposed from the pinned
+//     primitives (Omega_1 = ds eta, Euler of the Magnus series, BCH from the
+//     bare H, mbpt2 stop), planted mistakes rejected, and the exact energy
+//     of a Hamiltonian without two-body part (sum of lowest eigenvalues);
 //   * the generators (white, atan, atanxx) against closed-form Epstein-Nesbet
 //     denominators, w2 as the composition of pinned parts and its
 //     Moller-Plesset small-coupling limit, the three flows' common E0;

@@ -43,3 +43,21 @@ def test_fortran_host_reaches_library(tmp_path):
     out = dict(line.split() for line in r.stdout.splitlines())
     assert int(out["OP"]) == b.oplen and int(out["NBLOCKS"]) == b.nblocks and int(out["WCC"]) == b.wcclen
     assert int(out["BADPATH"]) == _lib.IMSRG_EINVAL
+
+
+REF_SRC = "/root/reference/src"
+SHIM = os.path.join(ROOT, "eom-imsrg_amd", "fortran", "imsrg_hip_shim.f90")
+
+
+@pytest.mark.skipif(FC is None or not os.path.isdir(REF_SRC),
+                    reason="needs the Fortran compiler and a reference checkout (development container only)")
+def test_shim_compiles_against_the_reference_types(tmp_path):
+    """the drop-in shim (reference argument lists on type(sq_op), type(spd),
+    type(cc_mat)) compiles against the reference's own basic_IMSRG and
+    cross_coupled modules, compiled in place into a temporary directory"""
+    srcs = [os.path.join(REF_SRC, f) for f in ("gzipmod.f90", "basic_IMSRG.f90", "cross_coupled.f90")] + [FMOD, SHIM]
+    for k, src in enumerate(srcs):
+        r = subprocess.run([FC, "-c", "-J", str(tmp_path), src, "-o", str(tmp_path / f"m{k}.o")],
+                           capture_output=True, text=True, cwd=str(tmp_path))
+        assert r.returncode == 0, (src, r.stderr[-2000:])
+    assert (tmp_path / "imsrg_hip_shim.mod").exists()
