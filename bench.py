@@ -6,10 +6,14 @@ Workload (eMax 8): HO orbits 2n+l <= 8 (90 orbits), 16O holes, hbar*omega =
 20 MeV, synthetic seeded Hamiltonian (seed 0), normal ordered, white
 generator.  A step is one iteration of magnus_decouple's loop
 (IMSRG_MAGNUS.f90:100-125): MAGNUS_EXPAND, euler_step, BCH_EXPAND,
-build_gs_white, mbpt2.  The timed steps are steps 1..K of the flow, on the
-grouped + hipGraph path (`value`) and on the per-channel path; E0 after K
-steps must agree between the two within 1e-8 MeV.  Inputs are resident in
-HBM before timing starts.
+build_gs_white, mbpt2.  The timed steps are steps of flows run until
+|E0(s) - E0(s - ds)| < 1e-8 MeV: after a flow's last step the next step
+starts the flow again from Omega = 0 (the reset is inside the timing), so
+no timed step lies past convergence.  They run on the grouped + hipGraph
+path (`value`) and on the per-channel path; E0 after K steps must agree
+between the two within 1e-8 MeV.  Inputs are resident in HBM before timing
+starts.  Whole converged flows (|mbpt2| < 1e-6, the reference's criterion,
+and |dE0| < 1e-8 MeV) are timed as their own fields.
 """
 from __future__ import annotations
 
@@ -41,6 +45,8 @@ def _parse():
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-instance eMax 6 batch")
     ap.add_argument("--batch-cpu-check", type=int, default=1,
                     help="instances of the batch whose flowed E0 is checked against the oracle on the host")
+    ap.add_argument("--batch-seq", type=int, default=8,
+                    help="single-instance flows timed one after another for the batch's comparison (0: none)")
     ap.add_argument("--no-emax10", action="store_true", help="skip the eMax 10 grouped + hipGraph steps")
     ap.add_argument("--no-setup", action="store_true",
                     help="skip the Hamiltonian set-up timings (me2j reader, HO one-body part, HO / HF normal ordering)")
@@ -49,6 +55,8 @@ def _parse():
                     help="run only this part and print its JSON object (for profiler runs)")
     ap.add_argument("--kernel-only", default=None,
                     help="launch one kernel class repeatedly and exit (for rocprofv3 counter runs)")
+    ap.add_argument("--steps-only", action="store_true",
+                    help="only the warm-up and timed grouped steps (for rocprofv3 runs of the Magnus step)")
     return ap.parse_args()
 
 
@@ -69,7 +77,7 @@ def main() -> int:
     dev = torch.device("cuda:0")
     from eom_imsrg import tune
     if a.leg == "batch":
-        print(json.dumps(batch_bench(64, 6, 8, a.batch_cpu_check)))
+        print(json.dumps(batch_bench(64, 6, a.batch_seq, a.batch_cpu_check)))
         return 0
     if a.leg == "emax10":
         print(json.dumps(emax10_bench(3, 1, not a.no_tune)))
@@ -110,43 +118,84 @@ def main() -> int:
         print(json.dumps({"kernel": a.kernel_only, "ms": t}))
         return 0
 
-    kms = {k: kernel_ms(k) for k in ("PPPP", "PPHH_SMALL", "PH_GEMM", "PH_SCATTER", "K122", "PANDYA", "ONEBODY",
-                                     "COMMUTATOR")}
-    comm_plain_ms = kernel_ms("COMMUTATOR", 1, 3)
+    kms, comm_plain_ms = {}, None
+    if not a.steps_only:
+        kms = {k: kernel_ms(k) for k in ("PPPP", "PPHH_SMALL", "PH_GEMM", "PH_SCATTER", "K122", "PANDYA",
+                                         "ONEBODY", "COMMUTATOR")}
+        if not a.no_plain:
+            comm_plain_ms = kernel_ms("COMMUTATOR", 1, 3)
 
-    # ---- the flow steps
+    # ---- the flow steps: flows to |dE0| < 1e-8 MeV, restarted from Omega = 0 after convergence
+    DE0_CRIT = 1e-8
+
     def run_steps(path: int, graphs: bool, nsteps: int, warm: int):
         b.set_path(path)
         f = Flow(b, graphs=graphs)
+        HS, G, DG = SqOp.zeros(b, 1), SqOp.zeros(b, -1), SqOp.zeros(b, -1)
+        st = {"Eold": 0.0, "flow_steps": 0, "flows": 0}
 
-        def fresh():
-            HS = H.clone()
-            G = SqOp.zeros(b, -1)
-            DG = SqOp.zeros(b, -1)
+        def restart():
+            HS.data.copy_(H.data)
+            G.data.zero_()
             cm.build_gs_white(HS, DG)
-            return HS, G, DG
+            st["Eold"], st["flow_steps"] = E0_bare, 0
 
-        if warm:
-            HS, G, DG = fresh()
-            for _ in range(warm):
-                f.magnus_step(HS, G, DG, H)
-        HS, G, DG = fresh()
+        def step():
+            out = f.magnus_step(HS, G, DG, H)
+            st["flow_steps"] += 1
+            if abs(out[0] - st["Eold"]) < DE0_CRIT:
+                st["flows"] += 1
+                restart()
+            else:
+                st["Eold"] = out[0]
+            return out
+
+        restart()
+        for _ in range(warm):
+            step()
+        restart()
+        st["flows"] = 0
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        log = [f.magnus_step(HS, G, DG, H) for _ in range(nsteps)]
+        log = [step() for _ in range(nsteps)]
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
         b.set_path(0)
-        return dt / nsteps * 1e3, log
+        return dt / nsteps * 1e3, log, st["flows"]
 
-    g_ms, g_log = run_steps(0, True, a.steps, a.warmup)
+    def converged_flow(path: int, graphs: bool, crit_mode: int, crit_tol: float):
+        b.set_path(path)
+        f = Flow(b, graphs=graphs)
+        HS, G = H.clone(), SqOp.zeros(b, -1)
+        if graphs:
+            f.magnus_decouple(HS, G, crit_mode=crit_mode, crit_tol=crit_tol, maxsteps=100)   # graph capture
+            HS, G = H.clone(), SqOp.zeros(b, -1)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        log = f.magnus_decouple(HS, G, crit_mode=crit_mode, crit_tol=crit_tol, maxsteps=100)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        b.set_path(0)
+        return {"s": round(dt, 4), "steps": int(len(log)), "E0": float(log[-1, 1]),
+                "commutators": int(log[:, 3].sum()), "ms_per_step": round(dt / len(log) * 1e3, 3)}
+
+    E0_bare = H.E0
+    g_ms, g_log, g_flows = run_steps(0, True, a.steps, a.warmup)
     ncomm = [s[2] + s[3] - 1 for s in g_log]
+    if a.steps_only:
+        print(json.dumps({"ms_per_step": round(g_ms, 3), "steps": a.steps, "commutators_per_step": ncomm,
+                          "E0_after_steps": g_log[-1][0]}))
+        return 0
+    flows = {"grouped_mbpt2_1e-6": converged_flow(0, True, 0, float(np.float32(1e-6))),
+             "grouped_dE0_1e-8": converged_flow(0, True, 1, DE0_CRIT)}
     out_plain = {}
     if not a.no_plain:
-        p_ms, p_log = run_steps(1, False, a.steps, min(a.warmup, 1))
+        p_ms, p_log, _ = run_steps(1, False, a.steps, min(a.warmup, 1))
         dE0 = abs(p_log[-1][0] - g_log[-1][0])
         out_plain = {"plain_ms_per_step": round(p_ms, 3), "speedup_vs_plain": round(p_ms / g_ms, 3),
                      "dE0_grouped_vs_plain_MeV": dE0, "dE0_ok": bool(dE0 < 1e-8)}
+        flows["plain_dE0_1e-8"] = converged_flow(1, False, 1, DE0_CRIT)
+        flows["speedup_vs_plain_dE0_1e-8"] = round(flows["plain_dE0_1e-8"]["s"] / flows["grouped_dE0_1e-8"]["s"], 3)
 
     # ---- gradient of the flowed E0 with respect to the vacuum two-body elements
     grad = None
@@ -154,7 +203,7 @@ def main() -> int:
         grad = gradient_bench(b, Hv, a.grad_check)
 
     # ---- many-instance batch (64 x eMax 6) and eMax 10 steps
-    batch = None if a.no_batch else batch_bench(64, 6, 8, a.batch_cpu_check)
+    batch = None if a.no_batch else batch_bench(64, 6, a.batch_seq, a.batch_cpu_check)
     e10 = None if a.no_emax10 else emax10_bench(3, 1, not a.no_tune)
     setup = None if a.no_setup else setup_bench(a.emax)
 
@@ -171,12 +220,13 @@ def main() -> int:
             "frac": round(achieved / peak, 4), "traffic": measured_traffic(dom, a.emax),
             "launch_ms": round(kms[dom], 4),
             "work_per_launch": work * (1e12 if unit == "TFLOP/s" else 1e9),
-            "work_unit": "flop" if unit == "TFLOP/s" else "byte"}
+            "work_unit": "flop" if unit == "TFLOP/s" else "byte",
+            "mfma_busy": measured_mfma(dom, a.emax)}
 
     # ---- CPU baseline: the oracle (C++ restatement of the reference) on this host
     cpu = None
     if not a.no_cpu:
-        cpu = cpu_baseline(b, a.emax, H, Omega, float(np.mean(ncomm)))
+        cpu = cpu_baseline(b, a.emax, H, Omega)
 
     F = wl.commutator_flops(b)
     res = {
@@ -190,8 +240,11 @@ def main() -> int:
         "gemm_tune": tuned,
         "commutators_per_step": [int(x) for x in ncomm],
         "E0_after_steps": g_log[-1][0],
+        "flows_completed_in_timed_steps": g_flows,
+        "converged_flows": flows,
         **out_plain,
-        "commutator_ms": {"grouped": round(kms["COMMUTATOR"], 4), "plain": round(comm_plain_ms, 4)},
+        "commutator_ms": {"grouped": round(kms["COMMUTATOR"], 4),
+                          "plain": round(comm_plain_ms, 4) if comm_plain_ms else None},
         "commutator_tflops": round(F / (kms["COMMUTATOR"] / 1e3) / 1e12, 3),
         "kernel_ms": {k: round(v, 4) for k, v in kms.items()},
         "grad_over_fwd": grad["grad_over_fwd"] if grad else None,
@@ -317,21 +370,21 @@ def batch_bench(ninst: int, emax: int, nseq: int, ncpu: int):
         log = f.magnus_decouple(HS, G)
         return log[-1, 1], len(log)
 
-    run_one(0)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    single = [run_one(k) for k in range(nseq)]
-    torch.cuda.synchronize()
-    t_s = (time.perf_counter() - t0) / nseq * ninst
-    dE = max(abs(single[k][0] - e0[k]) for k in range(nseq))
-    same_steps = all(single[k][1] == st[k] for k in range(nseq))
     out = {"workload": f"magnus_decouple_emax{emax}_x{ninst}", "instances": ninst, "emax": emax,
            "batch_s": round(t_b, 4), "instance_steps_total": int(st.sum()), "batched_steps": int(nmax),
            "steps_per_s": round(float(st.sum()) / t_b, 2),
-           "sequential_single_s_est": round(t_s, 4), "sequential_sample": nseq,
-           "speedup_vs_sequential": round(t_s / t_b, 3),
-           "E0_batch_vs_single_max_MeV": dE, "steps_match_single": bool(same_steps),
            "E0": [float(x) for x in e0], "steps": [int(x) for x in st]}
+    if nseq > 0:
+        run_one(0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        single = [run_one(k) for k in range(nseq)]
+        torch.cuda.synchronize()
+        t_s = (time.perf_counter() - t0) / nseq * ninst
+        out.update({"sequential_single_s_est": round(t_s, 4), "sequential_sample": nseq,
+                    "speedup_vs_sequential": round(t_s / t_b, 3),
+                    "E0_batch_vs_single_max_MeV": max(abs(single[k][0] - e0[k]) for k in range(nseq)),
+                    "steps_match_single": bool(all(single[k][1] == st[k] for k in range(nseq)))})
     if ncpu > 0:
         sys.path.insert(0, ROOT)
         from oracle.oracle import OracleBasis, lib
@@ -473,24 +526,84 @@ def measured_traffic(kernel: str, emax: int):
     return None
 
 
-def cpu_baseline(b, emax, H, Omega, comm_per_step):
-    """the oracle's [Omega, H] at this eMax on the host cores, scaled to one
-    Magnus step by the measured nested commutators per step"""
+def measured_mfma(kernel: str, emax: int):
+    """FP64-MFMA figures of `kernel` from the committed rocprofv3 counter
+    summary (SQ_VALU_MFMA_BUSY_CYCLES, SQ_INSTS_VALU_MFMA_MOPS_F64,
+    GRBM_GUI_ACTIVE passes), or None when no summary covers it"""
+    p = os.path.join(ROOT, "profiles", f"mfma_util_emax{emax}.md")
+    if not os.path.exists(p):
+        return None
+    out = {}
+    for line in open(p):
+        f = line.split()
+        if len(f) == 4 and f[0] == "mfma" and f[1] == kernel:
+            out[f[2]] = float(f[3])
+    return out or None
+
+
+def _cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(b, emax, H, Omega):
+    """The oracle (the C++/OpenMP restatement of the reference path, with the
+    host BLAS dgemm at the reference's dgemm sites) on this host: two
+    measured Magnus steps (steps 1 and 2 of the flow, one iteration of
+    IMSRG_MAGNUS.f90:100-125 each), the median of 5 [Omega, H] commutators
+    after one warm-up, and one commutator on a single thread.  Threads: the
+    process's affinity cores, capped by OMP_NUM_THREADS when it is set (the
+    GPU boxes grant each GPU a 16-core share and set it to 16)."""
     import numpy as np
     sys.path.insert(0, ROOT)
     from oracle.oracle import OracleBasis, use_host_blas, lib
-    nthreads = max(1, min(16, len(os.sched_getaffinity(0))))
+    aff = len(os.sched_getaffinity(0))
+    cap = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    nthreads = max(1, min(aff, cap) if cap > 0 else aff)
     lib().orc_set_threads(nthreads)
     blas = use_host_blas()
     o = OracleBasis(b.n, b.l, b.jj, b.tz, b.hole)
     Hh, Oh = H.numpy(), Omega.numpy()
+    o.commutator(Oh, -1, Hh, 1, 1)                    # warm-up
+    ts = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        o.commutator(Oh, -1, Hh, 1, 1)
+        ts.append(time.perf_counter() - t0)
+    comm_ms = float(np.median(ts)) * 1e3
+    lib().orc_set_threads(1)
     t0 = time.perf_counter()
     o.commutator(Oh, -1, Hh, 1, 1)
-    dt = time.perf_counter() - t0
-    return {"value": round(dt * 1e3 * comm_per_step, 1), "unit": "ms", "cores": nthreads, "kind": "port",
-            "sample": f"one [Omega,H] commutator at eMax {emax} on the CPU oracle "
-                      f"({dt * 1e3:.0f} ms, OpenMP + {'OpenBLAS dgemm' if blas else 'own dgemm'}) x "
-                      f"{comm_per_step:.2f} nested commutators per Magnus step"}
+    comm1_ms = (time.perf_counter() - t0) * 1e3
+    lib().orc_set_threads(nthreads)
+    # Magnus steps 1 and 2: MAGNUS_EXPAND, euler_step, BCH_EXPAND (from the
+    # bare H), build_gs_white, mbpt2, as the reference loop runs them
+    tol = float(np.float32(1e-6))
+    G = np.zeros(b.oplen)
+    DG = o.white(Hh)
+    steps = []
+    for _ in range(2):
+        t0 = time.perf_counter()
+        dG, nm = o.magnus(DG, G, tol)
+        G = o.add(G, 1.0, dG, 0.5)
+        HS, nb, _ = o.bch(G, Hh, tol)
+        DG = o.white(HS)
+        o.mbpt2(HS)
+        steps.append({"ms": round((time.perf_counter() - t0) * 1e3, 1), "commutators": int(nm + nb - 1)})
+    step_ms = float(np.mean([x["ms"] for x in steps]))
+    return {"value": round(step_ms, 1), "unit": "ms", "cores": nthreads, "kind": "port",
+            "sample": f"Magnus steps 1 and 2 of the eMax {emax} flow on the CPU oracle (OpenMP, "
+                      f"{'host OpenBLAS dgemm' if blas else 'own dgemm'}), mean ms per step; "
+                      f"{steps[0]['commutators']} + {steps[1]['commutators']} nested commutators",
+            "steps": steps,
+            "commutator_ms_median_of_5": round(comm_ms, 1), "commutator_ms_1_thread": round(comm1_ms, 1),
+            "nproc": os.cpu_count(), "affinity_cores": aff, "omp_num_threads_env": cap or None,
+            "cpu_model": _cpu_model(), "blas": os.path.basename(blas) if blas else None}
 
 
 if __name__ == "__main__":

@@ -847,6 +847,13 @@ void t_ph(const Exec& X, const double* LCC, int Lh, const double* RCC, int Rh, d
     run(B, B->seg_tri,
         [&](const SegTask* t, int n) { launch_comm222ph_scatter(X.D, t, n, WCC, Lh, Rh, RES, RESh, s); });
 }
+// op += c * T and the norm of T into dst, one pass over T (instances as t_norm)
+void t_axpy_norm(const Exec& X, double* op, const double* T, double c, double* dst, hipStream_t s) {
+  imsrg_basis_s* B = X.B;
+  const int nch = (int)B->ch_off_h.size();
+  launch_axpy_norm_b(X.D, op, T, c, B->ch_off, B->ch_len, B->ch_w, nch, X.part, s);
+  launch_sum_sqrt_b(X.D, X.part, nch, dst, kDStride, 1, s);
+}
 // device norm of op into dst (device; instance k at dst + k * kDStride)
 void t_norm(const Exec& X, const double* op, double* dst, hipStream_t s) {
   imsrg_basis_s* B = X.B;
@@ -1327,8 +1334,7 @@ int bch(imsrg_flow_s* F, double* HS, const double* G, int Gh, const double* H, i
     imsrg_flow_s::GraphKey key{0, iw, HS, G, H, Gh, Hh};
     int rc = run_term(F, key, s, [&](hipStream_t cs) {
       t_full(F->X, G, Gh, AD, Hh, INT2, Hh, true, F->w1, F->w2, F->GCC, F->ADCC, F->WCC, cs);
-      launch_axpy(n, HS, 1.0, INT2, c, HS, cs);
-      t_norm(F->X, INT2, B->dval, cs);
+      t_axpy_norm(F->X, HS, INT2, c, B->dval, cs);
     });
     if (rc) return rc;
     if (rec) {
@@ -1384,8 +1390,7 @@ int magnus(imsrg_flow_s* F, double* DG, int DGh, const double* G, int Gh, double
     imsrg_flow_s::GraphKey key{1, i, DG, G, nullptr, Gh, DGh};
     rc = run_term(F, key, s, [&](hipStream_t cs) {
       t_full(F->X, G, Gh, AD, DGh, INT2, DGh, false, F->w1, F->w2, F->GCC, F->ADCC, F->WCC, cs);
-      launch_axpy(n, DG, 1.0, INT2, c, DG, cs);
-      t_norm(F->X, INT2, B->dval, cs);
+      t_axpy_norm(F->X, DG, INT2, c, B->dval, cs);
     });
     if (rc) return rc;
     ++cnt;
@@ -1526,8 +1531,7 @@ int bch_b(imsrg_flow_s* F, double* HS, const double* G, const double* H, double 
     imsrg_flow_s::GraphKey key{2, iw, HS, G, H, -1, 1};
     rc = run_term(F, key, s, [&](hipStream_t cs) {
       t_full(F->Xe, G, -1, AD, 1, INT2, 1, true, F->w1, F->w2, F->GCC, F->ADCC, F->WCC, cs);
-      launch_axpy_b(n, HS, 1.0, INT2, c, HS, ni, F->acte, cs);
-      t_norm(F->Xe, INT2, F->Xe.dval, cs);
+      t_axpy_norm(F->Xe, HS, INT2, c, F->Xe.dval, cs);
     });
     if (rc) return rc;
     if ((rc = read_inst(F, 1, nullptr, s))) return rc;
@@ -1599,8 +1603,7 @@ int magnus_b(imsrg_flow_s* F, double* DG, const double* G, double conv, int* nco
     imsrg_flow_s::GraphKey key{3, i, DG, G, nullptr, -1, -1};
     rc = run_term(F, key, s, [&](hipStream_t cs) {
       t_full(F->Xe, G, -1, AD, -1, INT2, -1, false, F->w1, F->w2, F->GCC, F->ADCC, F->WCC, cs);
-      launch_axpy_b(n, DG, 1.0, INT2, c, DG, ni, F->acte, cs);
-      t_norm(F->Xe, INT2, F->Xe.dval, cs);
+      t_axpy_norm(F->Xe, DG, INT2, c, F->Xe.dval, cs);
     });
     if (rc) return rc;
     if ((rc = read_inst(F, 1, nullptr, s))) return rc;
@@ -2026,6 +2029,120 @@ void vec_plan(const Basis& b, std::vector<VecTask>* out, i64* neq) {
   }
   if (out) *out = std::move(v);
   if (neq) *neq = k;
+}
+
+// Name of the Omega checkpoint at flow parameter s, as write_omega_checkpoint
+// and read_omega_checkpoint form it (basic_IMSRG.f90:3239-3249,3340-3347):
+// s written as Fortran F6.3 (asterisks when it does not fit), left
+// adjusted, with a '0' appended when s < 10, after "<prefix>_s".
+std::string ckpt_name(const std::string& prefix, double s) {
+  char buf[64];
+  std::snprintf(buf, sizeof buf, "%6.3f", s);
+  std::string f(buf);
+  if (f.size() > 6) f = "******";
+  if (s < 10.0) {
+    const size_t a = f.find_first_not_of(' ');
+    return prefix + "_s" + (a == std::string::npos ? std::string() : f.substr(a)) + "0.bin";
+  }
+  return prefix + "_s" + f + ".bin";
+}
+
+bool file_exists(const std::string& path) {
+  FILE* f = std::fopen(path.c_str(), "rb");
+  if (!f) return false;
+  std::fclose(f);
+  return true;
+}
+
+// magnus_decouple, IMSRG_MAGNUS.f90:15-128, with the flow's generator.
+// prefix != null switches the reference's checkpointing on: on entry the
+// newest Omega file s = 95.5, 95.0, ... (> 1.0) is looked for
+// (read_omega_checkpoint, basic_IMSRG.f90:3304-3407); when one exists the
+// flow resumes from it (HS = BCH(Omega, H), the generator rebuilt, s = sx,
+// steps = nint(sx/ds), no mbpt2 test before the first step), and every chk
+// steps Omega is written before the step (not at the first such step after
+// a restart) and the file 12 (eMax 14: 4) units of s older removed
+// (write_omega_checkpoint, 3215-3301).
+int decouple(imsrg_flow_s* F, double* HS, double* G, double ds, double bch_conv, double mag_conv, int crit_mode,
+             double crit_tol, int maxsteps, const char* prefix, int chk, double* log, int* steps, double* restart_s,
+             hipStream_t s) {
+  if (!F) return fail(IMSRG_EINVAL, "null flow");
+  if (F->batch) return fail(IMSRG_EINVAL, "batched flow: use the imsrg_batch_* calls");
+  if (!HS || !G) return fail(IMSRG_EINVAL, "null argument");
+  imsrg_basis_s* B = F->B;
+  const i64 n = B->b.oplen;
+  int emax = 0;
+  for (int a = 0; a < B->b.N; ++a) emax = std::max(emax, 2 * B->b.nn[a] + B->b.ll[a]);
+  if (chk <= 0) chk = emax == 14 ? 12 : (emax > 14 ? 4 : 24);
+  const double drop = emax == 14 ? 4.0 : 12.0;
+  int rc;
+  if ((rc = fbuf(F, &F->decDG, n))) return rc;
+  double* DG = F->decDG;
+  launch_fill(DG, n, 0.0, s);
+  HIPCHK(hipMemcpyAsync(F->Hbare, HS, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+  double sx = -1.0;
+  if (prefix) {
+    for (double t = 95.5; t > 1.0; t -= 0.5)
+      if (file_exists(ckpt_name(prefix, t))) { sx = t; break; }
+  }
+  if (restart_s) *restart_s = sx;
+  double crit, Eold, s_flow = 0.0;
+  int st = 0;
+  bool first = false;
+  if (sx < 0.0) {
+    if ((rc = generator(F, F->X, HS, DG, s))) return rc;
+    if ((rc = mbpt2_dev(F->X, HS, B->dval + 2, s))) return rc;
+    HIPCHK(hipMemcpyAsync(B->dval + 3, HS, sizeof(double), hipMemcpyDeviceToDevice, s));
+    HIPCHK(hipMemcpyAsync(B->hval, B->dval + 2, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    crit = std::fabs(B->hval[0]);
+    Eold = B->hval[1];
+  } else {
+    if ((rc = imsrg_read_op_bin(B, ckpt_name(prefix, sx).c_str(), -1, G, s))) return rc;
+    if ((rc = bch(F, HS, G, -1, F->Hbare, 1, bch_conv, 40, nullptr, nullptr, s))) return rc;
+    if ((rc = generator(F, F->X, HS, DG, s))) return rc;
+    HIPCHK(hipMemcpyAsync(B->hval, HS, sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    crit = 10.0;
+    Eold = B->hval[0];
+    s_flow = sx;
+    st = (int)std::lround(sx / ds);
+    first = true;
+  }
+  const int st0 = st;
+  while (true) {
+    if (crit_mode == 0 && !(crit > crit_tol)) break;
+    if (st - st0 >= maxsteps) {
+      if (steps) *steps = st;
+      return fail(IMSRG_ESTEPS, "magnus_decouple: step limit reached");
+    }
+    ++st;
+    if (prefix && st % chk == 0) {
+      if (first) {
+        first = false;
+      } else {
+        if ((rc = imsrg_write_op_bin(B, G, ckpt_name(prefix, s_flow).c_str(), s))) return rc;
+        std::remove(ckpt_name(prefix, s_flow - drop).c_str());
+      }
+    }
+    double out[4];
+    if ((rc = magnus_step(F, HS, G, DG, F->Hbare, ds, bch_conv, mag_conv, out, s))) return rc;
+    s_flow += ds;
+    crit = std::fabs(out[1]);
+    if (log) {
+      const int r = st - st0 - 1;
+      log[r * 4 + 0] = s_flow;
+      log[r * 4 + 1] = out[0];
+      log[r * 4 + 2] = out[0] + out[1];
+      log[r * 4 + 3] = out[2] + (out[3] - 1);
+    }
+    if (crit_mode == 1) {
+      if (std::fabs(out[0] - Eold) < crit_tol) break;
+      Eold = out[0];
+    }
+  }
+  if (steps) *steps = st;
+  return 0;
 }
 
 }  // namespace
@@ -2698,48 +2815,23 @@ int imsrg_magnus_step(imsrg_flow F, double* HS, double* G, double* DG, const dou
 int imsrg_magnus_decouple(imsrg_flow F, double* HS, double* G, double ds, double bch_conv, double mag_conv,
                           int crit_mode, double crit_tol, int maxsteps, double* log, int* steps,
                           imsrg_stream stream) {
-  if (!F) return fail(IMSRG_EINVAL, "null flow");
-  if (F->batch) return fail(IMSRG_EINVAL, "batched flow: use the imsrg_batch_* calls");
-  imsrg_basis_s* B = F->B;
-  hipStream_t s = S(stream);
-  const i64 n = B->b.oplen;
-  int rc;
-  if ((rc = fbuf(F, &F->decDG, n))) return rc;
-  double* DG = F->decDG;
-  launch_fill(DG, n, 0.0, s);
-  HIPCHK(hipMemcpyAsync(F->Hbare, HS, n * sizeof(double), hipMemcpyDeviceToDevice, s));
-  if ((rc = generator(F, F->X, HS, DG, s))) return rc;
-  if ((rc = mbpt2_dev(F->X, HS, B->dval + 2, s))) return rc;
-  HIPCHK(hipMemcpyAsync(B->dval + 3, HS, sizeof(double), hipMemcpyDeviceToDevice, s));
-  HIPCHK(hipMemcpyAsync(B->hval, B->dval + 2, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
-  HIPCHK(hipStreamSynchronize(s));
-  double crit = std::fabs(B->hval[0]);
-  double Eold = B->hval[1];
-  int st = 0;
-  double s_flow = 0.0;
-  while (true) {
-    if (crit_mode == 0 && !(crit > crit_tol)) break;
-    if (st >= maxsteps) {
-      if (steps) *steps = st;
-      return fail(IMSRG_ESTEPS, "magnus_decouple: step limit reached");
-    }
-    ++st;
-    double out[4];
-    if ((rc = magnus_step(F, HS, G, DG, F->Hbare, ds, bch_conv, mag_conv, out, s))) return rc;
-    s_flow += ds;
-    crit = std::fabs(out[1]);
-    if (log) {
-      log[(st - 1) * 4 + 0] = s_flow;
-      log[(st - 1) * 4 + 1] = out[0];
-      log[(st - 1) * 4 + 2] = out[0] + out[1];
-      log[(st - 1) * 4 + 3] = out[2] + (out[3] - 1);
-    }
-    if (crit_mode == 1) {
-      if (std::fabs(out[0] - Eold) < crit_tol) break;
-      Eold = out[0];
-    }
-  }
-  if (steps) *steps = st;
+  return decouple(F, HS, G, ds, bch_conv, mag_conv, crit_mode, crit_tol, maxsteps, nullptr, 0, log, steps, nullptr,
+                  S(stream));
+}
+
+int imsrg_magnus_decouple_chk(imsrg_flow F, double* HS, double* G, double ds, double bch_conv, double mag_conv,
+                              int crit_mode, double crit_tol, int maxsteps, const char* prefix, int chk,
+                              double* log, int* steps, double* restart_s, imsrg_stream stream) {
+  if (!prefix) return fail(IMSRG_EINVAL, "null checkpoint prefix");
+  return decouple(F, HS, G, ds, bch_conv, mag_conv, crit_mode, crit_tol, maxsteps, prefix, chk, log, steps,
+                  restart_s, S(stream));
+}
+
+int imsrg_omega_checkpoint_name(const char* prefix, double s, char* out, int outlen) {
+  if (!prefix || !out || outlen < 1) return fail(IMSRG_EINVAL, "null argument");
+  const std::string n = ckpt_name(prefix, s);
+  if ((int)n.size() + 1 > outlen) return fail(IMSRG_EINVAL, "name buffer too short");
+  std::memcpy(out, n.c_str(), n.size() + 1);
   return 0;
 }
 

@@ -111,6 +111,52 @@ __device__ __forceinline__ bool batch_on(const int* act, int& inst) {
   return !(act && !act[inst]);
 }
 
+// commutator_122 partner terms (commutators.f90:283-396): the stored block
+// (g index) of classes lo <= hi of one channel
+__device__ __forceinline__ int blk_of(int lo, int hi) { return lo == 0 ? hi : (lo == 1 ? (hi == 1 ? 3 : 5) : 4); }
+
+// Addressing of the partner terms of one output element (IX, JX) of block
+// (rc, cc): a row-partner term at class cl reads the full channel matrix at
+// (partner, JX), a column-partner term at (IX, partner).  The stored block
+// holding that element and the element's offset are linear in the partner
+// index: off = base[cl] + idx * stride[cl], with the herm factor sg[cl]
+// (1 in a stored block, Lh / Rh in the transposed one).  Base and stride
+// depend on the thread's IX / JX only, so they are formed once per thread
+// and each term costs its partner record and the two operand loads.
+struct PartnerAddr {
+  i64 b0, b1, b2;   // base per class
+  int s0, s1, s2;   // stride per class
+  int t0, t1, t2;   // 1: element in the transposed stored block
+};
+// per-class value by arithmetic masks (no selects over member addresses,
+// which would make the compiler keep the struct in private memory)
+__device__ __forceinline__ i64 pick3(int cl, i64 v0, i64 v1, i64 v2) {
+  const i64 m0 = -(i64)(cl == 0), m1 = -(i64)(cl == 1), m2 = -(i64)(cl == 2);
+  return (v0 & m0) | (v1 & m1) | (v2 & m2);
+}
+__device__ __forceinline__ i64 pa_off(const PartnerAddr A, int cl, int idx) {
+  return pick3(cl, A.b0, A.b1, A.b2) + (i64)idx * pick3(cl, A.s0, A.s1, A.s2);
+}
+__device__ __forceinline__ bool pa_trans(const PartnerAddr A, int cl) { return pick3(cl, A.t0, A.t1, A.t2) != 0; }
+
+// the row- and column-partner addressing of output (IX, JX) of the block of
+// classes (rc, cc), from the channel's block offsets go[6] and class
+// dimensions dm[3]
+__device__ __forceinline__ void partner_addr(const i64* go, const int* dm, int rc, int cc, int IX, int JX,
+                                             PartnerAddr& AR, PartnerAddr& AC) {
+  // row partner at class cl: column JX of class cc; column partner at class
+  // cl: row IX of class rc
+#define IMSRG_PA(cl, bR, sR, tR, bC, sC, tC)                                              \
+  if (cl <= cc) { bR = go[blk_of(cl, cc)] + (i64)JX * dm[cl]; sR = 1; tR = 0; }           \
+  else { bR = go[blk_of(cc, cl)] + JX; sR = dm[cc]; tR = 1; }                             \
+  if (rc <= cl) { bC = go[blk_of(rc, cl)] + IX; sC = dm[rc]; tC = 0; }                    \
+  else { bC = go[blk_of(cl, rc)] + (i64)IX * dm[cl]; sC = 1; tC = 1; }
+  IMSRG_PA(0, AR.b0, AR.s0, AR.t0, AC.b0, AC.s0, AC.t0)
+  IMSRG_PA(1, AR.b1, AR.s1, AR.t1, AC.b1, AC.s1, AC.t1)
+  IMSRG_PA(2, AR.b2, AR.s2, AR.t2, AC.b2, AC.s2, AC.t2)
+#undef IMSRG_PA
+}
+
 __device__ __forceinline__ void upper_decode(i64 t, int& row, int& col) {
   int c = (int)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
   while ((i64)c * (c + 1) / 2 > t) --c;

@@ -21,7 +21,56 @@ namespace imsrg {
 // sqrt(1+delta_ab) sqrt(1+delta_cd).
 // The partner lists are precomputed per stored pair (a-, b-, c-, d-replaced,
 // in that order), so no term performs an index search.
-__device__ __forceinline__ int blk_of(int lo, int hi) { return lo == 0 ? hi : (lo == 1 ? (hi == 1 ? 3 : 5) : 4); }
+
+// one partner term: f (from LDS) times the operand elements r = R[o],
+// l = L[o], accumulated as sm + fl*vr - fr*vl (row partners) or
+// sm - fl*vr + fr*vl (column partners)
+template <bool ROW>
+__device__ __forceinline__ double term122(double sm, const Partner p, double r, double l, const PartnerAddr A,
+                                          double pre_other, const double* s_f, int nf1, int Lh, int Rh) {
+  const int cl = (p.code >> 22) & 3;
+  double pre = ((p.code >> 26) & 1) ? 1.4142135623730951 : 1.0;
+  if ((p.code >> 25) & 1) pre = -pre;
+  pre = ROW ? pre * pre_other : pre_other * pre;
+  double vr, vl;
+  if (pa_trans(A, cl)) { vr = r * Rh * pre; vl = l * Lh * pre; }
+  else { vr = r * pre; vl = l * pre; }
+  double fl = s_f[p.fk], fr = s_f[nf1 + p.fk];
+  if ((p.code >> 24) & 1) { fl = fl * Lh; fr = fr * Rh; }
+  return ROW ? sm + fl * vr - fr * vl : sm - fl * vr + fr * vl;
+}
+
+// the partner terms [x, xe) of one list, IMSRG_122_UNROLL records and their
+// operand loads in flight at a time; the terms are added in list order
+#ifndef IMSRG_122_UNROLL
+#define IMSRG_122_UNROLL 2
+#endif
+template <bool ROW>
+__device__ __forceinline__ double sum122(double sm, const Partner* part, int x, int xe, const PartnerAddr A,
+                                         double pre_other, const double* s_f, int nf1, const double* L, int Lh,
+                                         const double* R, int Rh) {
+  constexpr int U = IMSRG_122_UNROLL;
+  for (; x + U - 1 < xe; x += U) {
+    Partner p[U];
+    double r[U], l[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) p[u] = part[x + u];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const i64 o = pa_off(A, (p[u].code >> 22) & 3, p[u].code & 0x3fffff);
+      r[u] = R[o];
+      l[u] = L[o];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) sm = term122<ROW>(sm, p[u], r[u], l[u], A, pre_other, s_f, nf1, Lh, Rh);
+  }
+  for (; x < xe; ++x) {
+    const Partner p = part[x];
+    const i64 o = pa_off(A, (p.code >> 22) & 3, p.code & 0x3fffff);
+    sm = term122<ROW>(sm, p, R[o], L[o], A, pre_other, s_f, nf1, Lh, Rh);
+  }
+  return sm;
+}
 
 __global__ __launch_bounds__(256) void k_comm122p(DevBasis B, const SegTask* tasks, const Partner* part,
                                                   const int* rowp, const int* colp, const int* f1off, int nf1,
@@ -56,47 +105,13 @@ __global__ __launch_bounds__(256) void k_comm122p(DevBasis B, const SegTask* tas
   unpack(B.pairs[gc], c, d);
   const double pre_ab = (a == b) ? 1.4142135623730951 : 1.0;
   const double pre_cd = (c == d) ? 1.4142135623730951 : 1.0;
-  const i64* go = B.goff + q * 6;
-  const int* dm = B.dims + q * 3;
+  PartnerAddr AR, AC;
+  partner_addr(B.goff + q * 6, B.dims + q * 3, rc, cc, IX, JX, AR, AC);   // uniform (scalar) loads
   double sm = 0.0;
-  for (int x = rowp[gr]; x < rowp[gr + 1]; ++x) {
-    const Partner p = part[x];
-    const int idx = p.code & 0x3fffff, cl = (p.code >> 22) & 3;
-    double pre = ((p.code >> 26) & 1) ? 1.4142135623730951 : 1.0;
-    if ((p.code >> 25) & 1) pre = -pre;
-    pre = pre * pre_cd;
-    double vr, vl;
-    if (cl <= cc) {
-      const i64 o = go[blk_of(cl, cc)] + idx + (i64)JX * dm[cl];
-      vr = R[o] * pre; vl = L[o] * pre;
-    } else {
-      const i64 o = go[blk_of(cc, cl)] + JX + (i64)idx * dm[cc];
-      vr = R[o] * Rh * pre; vl = L[o] * Lh * pre;
-    }
-    double fl = s_f[p.fk], fr = s_f[nf1 + p.fk];
-    if ((p.code >> 24) & 1) { fl = fl * Lh; fr = fr * Rh; }
-    sm = sm + fl * vr - fr * vl;
-  }
-  for (int x = colp[gc]; x < colp[gc + 1]; ++x) {
-    const Partner p = part[x];
-    const int idx = p.code & 0x3fffff, cl = (p.code >> 22) & 3;
-    double pre = ((p.code >> 26) & 1) ? 1.4142135623730951 : 1.0;
-    if ((p.code >> 25) & 1) pre = -pre;
-    pre = pre_ab * pre;
-    double vr, vl;
-    if (rc <= cl) {
-      const i64 o = go[blk_of(rc, cl)] + IX + (i64)idx * dm[rc];
-      vr = R[o] * pre; vl = L[o] * pre;
-    } else {
-      const i64 o = go[blk_of(cl, rc)] + idx + (i64)IX * dm[cl];
-      vr = R[o] * Rh * pre; vl = L[o] * Lh * pre;
-    }
-    double fl = s_f[p.fk], fr = s_f[nf1 + p.fk];
-    if ((p.code >> 24) & 1) { fl = fl * Lh; fr = fr * Rh; }
-    sm = sm - fl * vr + fr * vl;
-  }
+  sm = sum122<true>(sm, part, rowp[gr], rowp[gr + 1], AR, pre_cd, s_f, nf1, L, Lh, R, Rh);
+  sm = sum122<false>(sm, part, colp[gc], colp[gc + 1], AC, pre_ab, s_f, nf1, L, Lh, R, Rh);
   sm = sm / sqrt(1.0 + (a == b ? 1.0 : 0.0)) / sqrt(1.0 + (c == d ? 1.0 : 0.0));
-  const i64 base = go[g];
+  const i64 base = B.goff[q * 6 + g];
   RES[base + IX + (i64)JX * n1] = sm;
   if (sq && mirror) RES[base + JX + (i64)IX * n1] = sm * RESh;
 }
@@ -994,6 +1009,40 @@ __global__ __launch_bounds__(256) void k_norm_partials(const double* op, const i
   const double s = block_sum(v, red);
   if (threadIdx.x == 0) part[c] = s * cw[c];
 }
+// X += c * T (every element, E0 by chunk 0's first thread) and the norm
+// partials of T in the same pass: the axpy and norm of one nested
+// commutator (append_operator + mat_frob_norm, IMSRG_MAGNUS.f90:227-229),
+// each element's update and each chunk's partial sum exactly as k_axpy and
+// k_norm_partials form them
+__global__ __launch_bounds__(256) void k_axpy_norm(double* X, const double* T, double c, const i64* coff,
+                                                   const int* clen, const double* cw, double* part, i64 sop, i64 spart,
+                                                   const int* act) {
+  __shared__ double red[256];
+  int inst;
+  if (!batch_on(act, inst)) return;
+  X += (i64)inst * sop;
+  T += (i64)inst * sop;
+  part += (i64)inst * spart;
+  const int ch = blockIdx.x;
+  if (ch == 0 && threadIdx.x == 0) X[0] = X[0] * 1.0 + T[0] * c;
+  const i64 o = coff[ch];
+  const int n = clen[ch];
+  double v = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const double y = T[o + i];
+    v += y * y;
+    X[o + i] = X[o + i] * 1.0 + y * c;
+  }
+  const double s = block_sum(v, red);
+  if (threadIdx.x == 0) part[ch] = s * cw[ch];
+}
+void launch_axpy_norm_b(const DevBasis& B, double* X, const double* T, double c, const i64* chunk_off,
+                        const int* chunk_len, const double* chunk_w, int nchunk, double* part, hipStream_t s) {
+  if (nchunk <= 0) return;
+  hipLaunchKernelGGL(k_axpy_norm, dim3(nchunk, B.ninst), dim3(256), 0, s, X, T, c, chunk_off, chunk_len, chunk_w, part,
+                     B.oplen, B.spart, B.act);
+}
+
 __global__ __launch_bounds__(1024) void k_sum_sqrt(const double* part, int n, double* dst, int do_sqrt, i64 spart,
                                                    i64 sdst, const int* act) {
   __shared__ double red[1024];

@@ -173,6 +173,8 @@ void launch_axpy_b(i64 n, const double* A, double ax, const double* Bv, double b
                    const int* act, hipStream_t s);
 void launch_fill_b(double* p, i64 n, double v, int ninst, const int* act, hipStream_t s);
 void launch_copy_b(double* dst, const double* src, i64 n, int ninst, const int* act, hipStream_t s);
+void launch_axpy_norm_b(const DevBasis& B, double* X, const double* T, double c, const i64* chunk_off,
+                        const int* chunk_len, const double* chunk_w, int nchunk, double* part, hipStream_t s);
 void launch_norm_partials_b(const DevBasis& B, const double* op, const i64* chunk_off, const int* chunk_len,
                             const double* chunk_w, int nchunk, double* part, hipStream_t s);
 void launch_sum_sqrt_b(const DevBasis& B, const double* part, int n, double* dst, i64 sdst, int do_sqrt,

@@ -326,6 +326,61 @@ void launch_to_dense(const DevBasis& B, const SegTask* t, int nt, const double* 
 // One workgroup per task, in XCD-contiguous order (neighbouring tiles share
 // partner rows in one L2); LDS holds the one-body values of L and R and the
 // workgroup's one-body accumulators.
+// the transposed partner terms [x, xe) of one list (row partners: signs +,
+// column partners: -), IMSRG_122_UNROLL records and their loads in flight
+#ifndef IMSRG_122_UNROLL
+#define IMSRG_122_UNROLL 2
+#endif
+template <bool ROW>
+__device__ __forceinline__ void vjp122(double& y1, double& y2, const Partner* part, int x, int xe,
+                                       const PartnerAddr A, double pre_other, double sb, int ch, const double* fL,
+                                       const double* fR, double* lbar, double* rbar, const double* cbar,
+                                       const double* L, int Lh, const double* R, int Rh) {
+  constexpr int U = IMSRG_122_UNROLL;
+  const double sg = ROW ? 1.0 : -1.0;
+  while (x < xe) {
+    const int nu = min(U, xe - x);
+    Partner p[U];
+    i64 o[U];
+    double c[U], r[U], l[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (u < nu) p[u] = part[x + u];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (u < nu) {
+        o[u] = pa_off(A, (p[u].code >> 22) & 3, p[u].code & 0x3fffff);
+        c[u] = cbar[o[u]];
+        if (sb != 0.0) { r[u] = R[o[u]]; l[u] = L[o[u]]; }
+      }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (u < nu) {
+        const int cl = (p[u].code >> 22) & 3;
+        double pre = ((p[u].code >> 26) & 1) ? 1.4142135623730951 : 1.0;
+        if ((p[u].code >> 25) & 1) pre = -pre;
+        pre = ROW ? pre * pre_other : pre_other * pre;
+        const bool lo = !pa_trans(A, cl);
+        const double v = lo ? c[u] * pre : c[u] * ch * pre;
+        const bool fs = (p[u].code >> 24) & 1;
+        const int fk = p[u].fk;
+        if (ROW) {
+          y1 += fR[fk] * (fs ? Rh : 1) * v;
+          y2 += fL[fk] * (fs ? Lh : 1) * v;
+        } else {
+          y1 -= fR[fk] * (fs ? Rh : 1) * v;
+          y2 -= fL[fk] * (fs ? Lh : 1) * v;
+        }
+        if (sb != 0.0) {
+          const double vr = lo ? r[u] * pre : r[u] * Rh * pre, vl = lo ? l[u] * pre : l[u] * Lh * pre;
+          unsafeAtomicAdd(lbar + fk, sg * sb * (fs ? Lh : 1) * vr);
+          unsafeAtomicAdd(rbar + fk, -sg * sb * (fs ? Rh : 1) * vl);
+        }
+      }
+    x += nu;
+  }
+}
+
 __global__ __launch_bounds__(256) void k_122_vjp(DevBasis B, const SegTask* tasks, int ntasks, const Partner* part,
                                                  const int* rowp, const int* colp, const int* f1off, int nf1,
                                                  const double* L, int Lh, const double* R, int Rh,
@@ -365,50 +420,14 @@ __global__ __launch_bounds__(256) void k_122_vjp(DevBasis B, const SegTask* task
     const double nrm = 1.0 / (sqrt(1.0 + (a == b ? 1.0 : 0.0)) * sqrt(1.0 + (c == d ? 1.0 : 0.0)));
     double sb = fold2(B, RESb, q, g, IX, JX, RESh);
     sb = sb / sqrt(1.0 + (a == b ? 1.0 : 0.0)) / sqrt(1.0 + (c == d ? 1.0 : 0.0));
-    const i64* go = B.goff + q * 6;
-    const int* dm = B.dims + q * 3;
+    PartnerAddr AR, AC;
+    partner_addr(B.goff + q * 6, B.dims + q * 3, rc, cc, IX, JX, AR, AC);
     double y1 = 0.0, y2 = 0.0;
-    for (int x = rowp[gr]; x < rowp[gr + 1]; ++x) {
-      const Partner p = part[x];
-      const int idx = p.code & 0x3fffff, cl = (p.code >> 22) & 3;
-      double pre = ((p.code >> 26) & 1) ? 1.4142135623730951 : 1.0;
-      if ((p.code >> 25) & 1) pre = -pre;
-      pre = pre * pre_cd;
-      const bool lo = cl <= cc;
-      const i64 o = lo ? go[cl == 0 ? cc : (cl == 1 ? (cc == 1 ? 3 : 5) : 4)] + idx + (i64)JX * dm[cl]
-                       : go[cc == 0 ? cl : (cc == 1 ? (cl == 1 ? 3 : 5) : 4)] + JX + (i64)idx * dm[cc];
-      const double v = lo ? cbar[o] * pre : cbar[o] * ch * pre;
-      const bool fs = (p.code >> 24) & 1;
-      y1 += fR[p.fk] * (fs ? Rh : 1) * v;
-      y2 += fL[p.fk] * (fs ? Lh : 1) * v;
-      if (sb != 0.0) {
-        const double vr = lo ? R[o] * pre : R[o] * Rh * pre, vl = lo ? L[o] * pre : L[o] * Lh * pre;
-        unsafeAtomicAdd(lbar + p.fk, sb * (fs ? Lh : 1) * vr);
-        unsafeAtomicAdd(rbar + p.fk, -sb * (fs ? Rh : 1) * vl);
-      }
-    }
-    for (int x = colp[gc]; x < colp[gc + 1]; ++x) {
-      const Partner p = part[x];
-      const int idx = p.code & 0x3fffff, cl = (p.code >> 22) & 3;
-      double pre = ((p.code >> 26) & 1) ? 1.4142135623730951 : 1.0;
-      if ((p.code >> 25) & 1) pre = -pre;
-      pre = pre_ab * pre;
-      const bool lo = rc <= cl;
-      const i64 o = lo ? go[rc == 0 ? cl : (rc == 1 ? (cl == 1 ? 3 : 5) : 4)] + IX + (i64)idx * dm[rc]
-                       : go[cl == 0 ? rc : (cl == 1 ? (rc == 1 ? 3 : 5) : 4)] + idx + (i64)IX * dm[cl];
-      const double v = lo ? cbar[o] * pre : cbar[o] * ch * pre;
-      const bool fs = (p.code >> 24) & 1;
-      y1 -= fR[p.fk] * (fs ? Rh : 1) * v;
-      y2 -= fL[p.fk] * (fs ? Lh : 1) * v;
-      if (sb != 0.0) {
-        const double vr = lo ? R[o] * pre : R[o] * Rh * pre, vl = lo ? L[o] * pre : L[o] * Lh * pre;
-        unsafeAtomicAdd(lbar + p.fk, -sb * (fs ? Lh : 1) * vr);
-        unsafeAtomicAdd(rbar + p.fk, sb * (fs ? Rh : 1) * vl);
-      }
-    }
+    vjp122<true>(y1, y2, part, rowp[gr], rowp[gr + 1], AR, pre_cd, sb, ch, fL, fR, lbar, rbar, cbar, L, Lh, R, Rh);
+    vjp122<false>(y1, y2, part, colp[gc], colp[gc + 1], AC, pre_ab, sb, ch, fL, fR, lbar, rbar, cbar, L, Lh, R, Rh);
     y1 *= nrm;
     y2 *= nrm;
-    const i64 o = go[g] + IX + (i64)JX * n1;
+    const i64 o = B.goff[q * 6 + g] + IX + (i64)JX * n1;
     if (!sq) {
       Lb[o] += -2.0 * Rh * y1;
       Rb[o] += 2.0 * Lh * y2;
@@ -416,7 +435,7 @@ __global__ __launch_bounds__(256) void k_122_vjp(DevBasis B, const SegTask* task
       Lb[o] += -Rh * Lh * y1;
       Rb[o] += Lh * Rh * y2;
     } else {
-      const i64 ol = go[g] + JX + (i64)IX * n1;
+      const i64 ol = B.goff[q * 6 + g] + JX + (i64)IX * n1;
       Lb[o] += -Rh * y1;
       Lb[ol] += -Rh * Lh * y1;
       Rb[o] += Lh * y2;

@@ -98,6 +98,9 @@ SIGNATURES = {
     "imsrg_magnus_step": (_i, [_p, _dp, _dp, _dp, _dp, C.c_double, C.c_double, C.c_double, _d, _p]),
     "imsrg_magnus_decouple": (_i, [_p, _dp, _dp, C.c_double, C.c_double, C.c_double, _i, C.c_double, _i, _d,
                                    _ip, _p]),
+    "imsrg_magnus_decouple_chk": (_i, [_p, _dp, _dp, C.c_double, C.c_double, C.c_double, _i, C.c_double, _i,
+                                       C.c_char_p, _i, _d, _ip, _d, _p]),
+    "imsrg_omega_checkpoint_name": (_i, [C.c_char_p, C.c_double, C.c_char_p, _i]),
     "imsrg_batch_create": (_i, [_p, _i, C.POINTER(_p)]),
     "imsrg_batch_size": (_i, [_p]),
     "imsrg_batch_commutator": (_i, [_p, _dp, _i, _dp, _i, _dp, _p]),

@@ -104,6 +104,26 @@ class Flow:
         G.herm = -1
         return np.array(log[: 4 * st.value]).reshape(st.value, 4)
 
+    def magnus_decouple_chk(self, HS: SqOp, G: SqOp, prefix: str, chk: int = 0, ds: float = 0.5,
+                            bch_conv: float = REF_TOL, mag_conv: float = REF_TOL, crit_mode: int = 0,
+                            crit_tol: float = REF_TOL, maxsteps: int = 200):
+        """magnus_decouple with the reference's Omega checkpoints
+        (IMSRG_MAGNUS.f90:64-110): restarts from the newest
+        <prefix>_s<s>.bin if there is one, writes Omega every chk steps
+        (0: the reference's rule).  Returns (log rows of the steps run here,
+        total steps, restart s or -1)."""
+        log = (C.c_double * (4 * maxsteps))()
+        st = C.c_int()
+        sx = C.c_double()
+        _lib.check(_lib.lib.imsrg_magnus_decouple_chk(self.handle, HS.ptr, G.ptr, float(ds), float(bch_conv),
+                                                      float(mag_conv), int(crit_mode), float(crit_tol),
+                                                      int(maxsteps), str(prefix).encode(), int(chk), log,
+                                                      C.byref(st), C.byref(sx), _stream()))
+        G.herm = -1
+        n0 = int(round(sx.value / ds)) if sx.value >= 0 else 0
+        nrun = st.value - n0
+        return np.array(log[: 4 * nrun]).reshape(nrun, 4), st.value, sx.value
+
     def __del__(self):
         h = getattr(self, "handle", None)
         if h is not None and h.value and _lib is not None and getattr(_lib, "lib", None) is not None:

@@ -86,6 +86,16 @@ class SqOp:
         _lib.check(_lib.lib.imsrg_read_op_bin(basis.handle, str(path).encode(), int(herm), op.ptr, _stream()))
         return op
 
+    @staticmethod
+    def omega_checkpoint_name(prefix: str, s: float) -> str:
+        """write_omega_checkpoint's file name for Omega at flow parameter s
+        (basic_IMSRG.f90:3239-3249)"""
+        from . import _lib
+        import ctypes as C
+        buf = C.create_string_buffer(len(prefix) + 64)
+        _lib.check(_lib.lib.imsrg_omega_checkpoint_name(str(prefix).encode(), float(s), buf, len(buf)))
+        return buf.value.decode()
+
     def gam(self, q: int, g: int) -> torch.Tensor:
         """view of gam(g+1) of channel q as an (rows, cols) matrix"""
         n1, n2 = self.basis.block_shape(q, g)

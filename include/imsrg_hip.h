@@ -280,6 +280,29 @@ int imsrg_magnus_decouple(imsrg_flow f, double* HS, double* G, double ds, double
                           int crit_mode, double crit_tol, int maxsteps, double* log, int* steps,
                           imsrg_stream stream);
 
+/* magnus_decouple with the reference's Omega checkpoints (IMSRG_MAGNUS.f90:
+ * 64-110 with checkpointing on; write_omega_checkpoint /
+ * read_omega_checkpoint, basic_IMSRG.f90:3215-3407).  Files are
+ * <prefix>_s<s>.bin (the .bin operator file of imsrg_write_op_bin), s as
+ * the reference writes it (Fortran F6.3, a '0' appended below 10; see
+ * imsrg_omega_checkpoint_name).  On entry the newest file s = 95.5, 95.0,
+ * ... (> 1.0) is looked for; if one exists the flow restarts from it: G <-
+ * Omega read from the file, HS (in: the bare H) <- BCH_EXPAND(G, H), the
+ * generator rebuilt, s = sx, steps = nint(sx / ds), and the first step runs
+ * without the mbpt2 test.  Every chk steps (chk <= 0: the reference's 24,
+ * 12 at eMax 14, 4 above) Omega is written before the step, except at the
+ * first such step after a restart, and the file of s - 12 (eMax 14: s - 4)
+ * is removed.  *restart_s = sx, or -1 for a fresh start; *steps = the step
+ * count including the restart's nint(sx / ds); log rows (as
+ * imsrg_magnus_decouple) for the steps run by this call; maxsteps limits
+ * the steps run by this call. */
+int imsrg_magnus_decouple_chk(imsrg_flow f, double* HS, double* G, double ds, double bch_conv, double mag_conv,
+                              int crit_mode, double crit_tol, int maxsteps, const char* prefix, int chk,
+                              double* log, int* steps, double* restart_s, imsrg_stream stream);
+/* the file name write_omega_checkpoint gives Omega at flow parameter s
+ * (basic_IMSRG.f90:3239-3249) for the stem `prefix` */
+int imsrg_omega_checkpoint_name(const char* prefix, double s, char* out, int outlen);
+
 /* ------------------------------------------------ many-instance batches */
 /* Instance-array variants of the step-level calls (IMSRG_MAGNUS.f90:15-234,
  * 620-691) for ninst independent problems on one basis, run together: every

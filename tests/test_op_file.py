@@ -109,3 +109,17 @@ def test_gpu_vectorize_repackage_and_file(gpu, tmp_path):
     from eom_imsrg import _lib
     with pytest.raises(_lib.ImsrgError):
         SqOp.read_bin(basis(4), os.path.join(str(tmp_path), "op_e6_1.bin"))
+
+
+def test_omega_checkpoint_names():
+    """write_omega_checkpoint's names (basic_IMSRG.f90:3239-3249): s as F6.3,
+    left adjusted, a '0' appended below 10, asterisks when it does not fit"""
+    from eom_imsrg import SqOp
+    name = SqOp.omega_checkpoint_name
+    assert name("x/o16_hw20", 0.5) == "x/o16_hw20_s0.5000.bin"
+    assert name("p", 5.5) == "p_s5.5000.bin"
+    assert name("p", 9.5) == "p_s9.5000.bin"
+    assert name("p", 10.0) == "p_s10.000.bin"
+    assert name("p", 95.5) == "p_s95.500.bin"
+    assert name("p", -0.5) == "p_s-0.5000.bin"
+    assert name("p", 100.0) == "p_s******.bin"
