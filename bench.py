@@ -341,7 +341,9 @@ def batch_bench(ninst: int, emax: int, nseq: int, ncpu: int):
     import torch
     from eom_imsrg import Basis, Batch, Flow, SqOp, synth, commutators as cm
 
+    from eom_imsrg import tune
     b = Basis.oxygen16(emax)
+    tuned = tune.tune(b)
     H = torch.empty(ninst, b.oplen, dtype=torch.float64, device="cuda")
     Hv = []
     for k in range(ninst):
@@ -372,7 +374,7 @@ def batch_bench(ninst: int, emax: int, nseq: int, ncpu: int):
 
     out = {"workload": f"magnus_decouple_emax{emax}_x{ninst}", "instances": ninst, "emax": emax,
            "batch_s": round(t_b, 4), "instance_steps_total": int(st.sum()), "batched_steps": int(nmax),
-           "steps_per_s": round(float(st.sum()) / t_b, 2),
+           "steps_per_s": round(float(st.sum()) / t_b, 2), "gemm_tune": tuned,
            "E0": [float(x) for x in e0], "steps": [int(x) for x in st]}
     if nseq > 0:
         run_one(0)

@@ -3051,7 +3051,8 @@ int imsrg_flow_grad(imsrg_flow F, const double* T, double ds, double bch_conv, d
 
 int imsrg_set_gemm_tile(imsrg_basis B, int family, int tile) {
   const int shape = tile & 7;
-  if (!B || family < 0 || family > 3 || shape > 2 || tile < 0 || tile > 10) return fail(IMSRG_EINVAL, "bad gemm family / tile");
+  if (!B || family < 0 || family > 3 || shape > 5 || tile < 0 || tile > (IMSRG_TILE_XCD | 5))
+    return fail(IMSRG_EINVAL, "bad gemm family / tile");
   GemmFamily* f[4] = {&B->pp1, &B->k2, &B->k3, &B->ph};
   f[family]->xcd_aware = (tile & IMSRG_TILE_XCD) != 0;
   B->tile_req[family] = tile;

@@ -3,8 +3,9 @@
 // One launch runs a list of tile tasks over a list of problems
 //   C = alpha * (op(A1) op(B1) + s2 * op(A2) op(B2))  (+ beta C),
 // operands addressed as (slot, offset) so a plan serves every call.  Tile
-// shapes 64x64, 128x64, 128x128 (256 threads = 4 waves in 2 x 2, each wave
-// a (TBM/2) x (TBN/2) block of 16x16 MFMA tiles), K in steps of 16 through
+// shapes 64x64, 128x64, 128x128, 64x32, 32x64, 32x32 (256 threads = 4 waves
+// in 2 x 2, each wave a (TBM/2) x (TBN/2) block of 16x16 MFMA tiles; the
+// small shapes cut the edge padding of thin problems), K in steps of 16 through
 // two LDS buffers with the next K-tile prefetched into registers.  Split K:
 // task.ks selects K-chunk [ks*kchunk, (ks+1)*kchunk) and slab C + ks*cstride.
 #include <hip/hip_runtime.h>
@@ -147,7 +148,8 @@ __device__ __forceinline__ void mma_p(v4d* acc, double (*As)[GBK][TBM + 1], doub
 }
 
 template <int TBM, int TBN, int SIG>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TBM == 64 ? 4 : (TBN == 64 ? 2 : 1)))) void k_gemm_t(Slots sl, const GemmProb* probs, const GemmTask* tasks, XcdMap xm,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
+    TBM * TBN <= 32 * 32 ? 8 : (TBM * TBN <= 64 * 64 ? 4 : (TBM * TBN <= 128 * 64 ? 2 : 1))))) void k_gemm_t(Slots sl, const GemmProb* probs, const GemmTask* tasks, XcdMap xm,
                                                 double ascale, double s2scale) {
   constexpr int MI = TBM / 32, NI = TBN / 32;
   __shared__ double As[2][GBK][TBM + 1];
@@ -205,8 +207,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TBM == 64 ?
 
 }  // namespace
 
-int tile_rows(int tile) { return tile == 0 ? 64 : 128; }
-int tile_cols(int tile) { return tile == 2 ? 128 : 64; }
+// tile shapes: 0 64x64, 1 128x64, 2 128x128, 3 64x32, 4 32x64, 5 32x32
+int tile_rows(int tile) { return (tile == 1 || tile == 2) ? 128 : (tile >= 4 ? 32 : 64); }
+int tile_cols(int tile) { return tile == 2 ? 128 : ((tile == 3 || tile == 5) ? 32 : 64); }
 
 template <int TBM, int TBN>
 static void launch_sig(int sig, int grid, int ninst, const Slots& sl, const GemmProb* probs, const GemmTask* tasks,
@@ -244,6 +247,9 @@ void launch_gemm_tiled(int tile, const Slots& sl, const GemmProb* probs, const G
   switch (tile) {
     case 0: launch_sig<64, 64>(sig, grid, ninst, sl, probs, tasks, xm, s, ascale, s2scale); break;
     case 1: launch_sig<128, 64>(sig, grid, ninst, sl, probs, tasks, xm, s, ascale, s2scale); break;
+    case 3: launch_sig<64, 32>(sig, grid, ninst, sl, probs, tasks, xm, s, ascale, s2scale); break;
+    case 4: launch_sig<32, 64>(sig, grid, ninst, sl, probs, tasks, xm, s, ascale, s2scale); break;
+    case 5: launch_sig<32, 32>(sig, grid, ninst, sl, probs, tasks, xm, s, ascale, s2scale); break;
     default: launch_sig<128, 128>(sig, grid, ninst, sl, probs, tasks, xm, s, ascale, s2scale); break;
   }
 }

@@ -111,7 +111,10 @@ def sqop_to_hf(Op: SqOp, coefs: np.ndarray) -> SqOp:
     """sqop_to_HF (HF_mod.f90:181-209): the hermitian scalar operator Op
     (vacuum form, HO basis) <- rotated to the HF orbits `coefs` (as
     hartree_fock returns them) and normal ordered"""
-    D = np.ascontiguousarray(coefs, np.float64)
+    D = np.ascontiguousarray(coefs, np.float64).reshape(-1)
+    n = int(_lib.lib.imsrg_basis_size(Op.basis.handle, _lib.SIZE["SPMAT"]))
+    if D.size != n:
+        raise ValueError(f"coefs holds {D.size} doubles; the basis's HF orbit blocks need {n}")
     _lib.check(_lib.lib.imsrg_sqop_to_hf(Op.basis.handle, D.ctypes.data_as(C.POINTER(C.c_double)), Op.ptr,
                                          int(Op.herm), _stream()))
     return Op

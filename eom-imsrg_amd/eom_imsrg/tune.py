@@ -1,6 +1,6 @@
 """GEMM tile autotuner: per basis and device, time every tile shape of each
-grouped MFMA GEMM family (64x64, 128x64, 128x128, each with and without
-XCD packing) with hipEvents on the launch stream, keep the fastest, and
+grouped MFMA GEMM family (64x64, 128x64, 128x128, 64x32, 32x64, 32x32,
+each with and without XCD packing) with hipEvents on the launch stream, keep the fastest, and
 persist the choice as JSON so later runs apply it without timing.
 
 Families (include/imsrg_hip.h, IMSRG_GEMM_*): PPPP (222_pp_hh L1 R1), PPX
@@ -28,7 +28,7 @@ from .basis import Basis
 FAMILIES = {"PPPP": ("PPPP", "PPPP_GEMM"), "PPX": ("PPX", "PPHH_SMALL"), "XX": ("XX", "PPHH_SMALL"),
             "PH": ("PH", "PH_GEMM")}
 XCD = 8
-CANDIDATES = [0, 1, 2, XCD | 0, XCD | 1, XCD | 2]
+CANDIDATES = [t | x for x in (0, XCD) for t in range(6)]
 SHIPPED = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuned_mi355x.json")
 
 

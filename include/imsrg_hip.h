@@ -377,8 +377,9 @@ int imsrg_flow_grad(imsrg_flow f, const double* T, double ds, double bch_conv, d
 
 /* ------------------------------------------------ tuning */
 /* No counterpart in the reference: selects the MFMA tile shape of one GEMM
- * family (IMSRG_GEMM_*): tile 0 = 64x64, 1 = 128x64, 2 = 128x128, plus
- * IMSRG_TILE_XCD to pack each channel's tiles onto one XCD (one L2). */
+ * family (IMSRG_GEMM_*): tile 0 = 64x64, 1 = 128x64, 2 = 128x128, 3 = 64x32,
+ * 4 = 32x64, 5 = 32x32, plus IMSRG_TILE_XCD to pack each channel's tiles onto
+ * one XCD (one L2). */
 #define IMSRG_TILE_XCD 8
 #define IMSRG_GEMM_PPPP 0   /* 222_pp_hh L1 R1 */
 #define IMSRG_GEMM_PPX 1    /* 222_pp_hh pp x (ph|hh) */

@@ -198,3 +198,5 @@ def test_sqop_to_hf_matches_oracle(gpu):
     Od = SqOp.from_numpy(b, Ob, 1)
     interaction.sqop_to_hf(Od, Dg)
     assert_rel(Od.numpy(), ref, rtol=1e-10, what="sqop_to_HF")
+    with pytest.raises(ValueError):
+        interaction.sqop_to_hf(SqOp.from_numpy(b, Ob, 1), Dg[:-1])   # coefs of the wrong length

@@ -61,3 +61,27 @@ def test_measured_choice_keeps_results(gpu, tmp_path, monkeypatch):
     torch.cuda.synchronize()
     assert_rel(got.numpy(), ref.numpy(), rtol=1e-14, what="commutator after tuning")
     tune.apply(b, before)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("tile", [0, 1, 2, 3, 4, 5])
+def test_every_tile_shape_matches_oracle(gpu, tile):
+    """every GEMM family on one tile shape (64x64, 128x64, 128x128, 64x32,
+    32x64, 32x32, with XCD packing on odd shapes): the commutator equals the
+    oracle's, including eMax 6 blocks larger than the tiles"""
+    from eom_imsrg import SqOp, commutators as cm, synth, tune
+    from conftest import oracle_basis
+    b, o = basis(6), oracle_basis(6)
+    print(f"[tile] shape={tile}", flush=True)
+    before = tune.current(b)
+    L = synth.antihermitian(b, 3, 0.1)
+    R = synth.hermitian_random(b, 4, 1.0)
+    ref = o.commutator(L, -1, R, 1, 1)
+    tune.apply(b, {fam: tile | (8 if tile % 2 else 0) for fam in tune.FAMILIES})
+    try:
+        got = SqOp.zeros(b, 1)
+        cm.commutator(SqOp.from_numpy(b, L, -1), SqOp.from_numpy(b, R, 1), got)
+        torch.cuda.synchronize()
+        assert_rel(got.numpy(), ref, what=f"[L,R] with tile shape {tile}")
+    finally:
+        tune.apply(b, before)
