@@ -915,7 +915,7 @@ void t_full(const Exec& X, const double* L, int Lh, const double* R, int Rh, dou
     t_122(X, L, Lh, R, Rh, RES, RESh, s, false);
     (void)hipStreamWaitEvent(s, ev[2], 0);
     run(B, B->pairs,
-        [&](const PairTask* t, int n) { launch_pppp_sym(X.D, t, n, L, Lh, R, Rh, RES, w1, w2, 0, s); });
+        [&](const PairTask* t, int n) { launch_pppp_sym(X.D, t, n, L, Lh, R, Rh, RES, w1, w2, 2, s); });
     (void)hipStreamWaitEvent(s, ev[3], 0);
     run(B, B->seg_epi, [&](const SegTask* t, int n) {
       launch_pphh_epilogue(X.D, t, n, L, Lh, R, Rh, RES, w1, w2, X.Pscr, B->poff, B->k3ks, X.K2scr, B->k2off,
@@ -934,7 +934,7 @@ void t_full(const Exec& X, const double* L, int Lh, const double* R, int Rh, dou
   t_111(X, L, Lh, R, Rh, RES, s);
   t_121(X, L, Lh, R, Rh, RES, RESh, s);
   t_122(X, L, Lh, R, Rh, RES, RESh, s, mirror);
-  t_pphh(X, L, Lh, R, Rh, RES, RESh, w1, w2, s, 0);
+  t_pphh(X, L, Lh, R, Rh, RES, RESh, w1, w2, s, mirror ? 0 : 2);
   t_221(X, L, Lh, R, Rh, RES, RESh, w1, w2, s);
   t_ph(X, LCC, Lh, RCC, Rh, RES, RESh, WCC, s, mirror);
   if (!mirror) launch_sym_lower(X.D, X.B->symt.dev, X.B->symt.n, RES, RESh, s);
@@ -1094,9 +1094,17 @@ int ensure_adj(imsrg_basis_s* B) {
     std::vector<int> fillp(B->pht0_h.begin(), B->pht0_h.end() - 1);
     for (size_t e = 0; e < term.size(); ++e) B->phterm_h[fillp[tid[e]]++] = term[e];
   }
+  // tile shapes of the transposed products (tile codes of imsrg_set_gemm_tile;
+  // 64x32 measured fastest for the reverse sweep at eMax 8: 0.494 -> 0.474 s)
+#ifndef IMSRG_ADJ_TILE
+#define IMSRG_ADJ_TILE 3
+#endif
+#ifndef IMSRG_ADJ_PH_TILE
+#define IMSRG_ADJ_PH_TILE 3
+#endif
   for (GemmFamily* f : {&B->pb[0], &B->pb[1], &B->pb[2], &B->pb[3], &B->pb[4], &B->pb[5], &B->phb}) {
     if ((rc = upload(B, f->probs_h, &f->probs))) return rc;
-    if ((rc = family_tasks(B, *f, 0))) return rc;
+    if ((rc = family_tasks(B, *f, f == &B->phb ? IMSRG_ADJ_PH_TILE : IMSRG_ADJ_TILE))) return rc;
   }
   if ((rc = upload(B, B->phq_h, &B->phq)) || (rc = upload(B, B->phqs_h, &B->phqs)) ||
       (rc = upload(B, B->phl_h, &B->phl)) || (rc = upload(B, B->phtile_h, &B->phtile)) ||
@@ -3103,7 +3111,9 @@ int imsrg_profile_kernel(imsrg_basis B, int which, const double* L, int Lh, cons
         s1.p[0] = L; s1.p[1] = R; s1.p[2] = w1;
         run_gemm(B, B->pp1, s1, s, 1.0, -(double)Rh);
         run(B, B->pairs,
-            [&](const PairTask* t, int n) { launch_pppp_sym(B->D, t, n, L, Lh, R, Rh, RES, w1, w2, 0, s); });
+            [&](const PairTask* t, int n) {   // as the commutator runs it (grouped: upper tiles)
+              launch_pppp_sym(B->D, t, n, L, Lh, R, Rh, RES, w1, w2, B->path == IMSRG_PATH_GROUPED ? 2 : 0, s);
+            });
         break;
       }
       case IMSRG_K_PPPP_GEMM: {

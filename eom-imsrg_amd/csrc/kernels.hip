@@ -800,14 +800,17 @@ __global__ __launch_bounds__(256) void k_pppp_sym(DevBasis B, const PairTask* ta
 // transpose it needs staged in LDS (33 KB instead of 66 KB: twice the
 // workgroups per CU).  Same arithmetic as k_pppp_sym.
 __global__ __launch_bounds__(256) void k_pppp_sym1(DevBasis B, const PairTask* tasks, double* RES, const double* w1,
-                                                   int pm) {
+                                                   int pm, int upper_only) {
   int inst;
   if (!batch_inst(B, inst)) return;
   RES += (i64)inst * B.oplen;
   w1 += (i64)inst * B.oplen;
   __shared__ double S[64][65];   // S[x][y] = C(A0 + x, B0 + y), the transposed source tile
-  const PairTask t = tasks[blockIdx.x >> 1];
-  const int side = blockIdx.x & 1;
+  // upper_only: one workgroup per tile pair for its upper tile (the caller
+  // fills the lower triangles afterwards), else two, the second for the
+  // lower tile
+  const PairTask t = tasks[upper_only ? blockIdx.x : blockIdx.x >> 1];
+  const int side = upper_only ? 0 : (blockIdx.x & 1);
   if (side == 1 && t.ti == t.tj) return;
   const int q = t.q;
   const int np = B.dims[q * 3];
@@ -836,8 +839,10 @@ __global__ __launch_bounds__(256) void k_pppp_sym1(DevBasis B, const PairTask* t
 void launch_pppp_sym(const DevBasis& B, const PairTask* t, int nt, const double* L, int Lh, const double* R, int Rh,
                      double* RES, double* w1, double* w2, int ref_outputs, hipStream_t s) {
   if (nt <= 0) return;
-  if (!ref_outputs) {
-    hipLaunchKernelGGL(k_pppp_sym1, dim3(2 * nt, B.ninst), dim3(256), 0, s, B, t, RES, (const double*)w1, Lh * Rh);
+  if (ref_outputs != 1) {
+    const int upper_only = ref_outputs == 2 ? 1 : 0;
+    hipLaunchKernelGGL(k_pppp_sym1, dim3(upper_only ? nt : 2 * nt, B.ninst), dim3(256), 0, s, B, t, RES,
+                       (const double*)w1, Lh * Rh, upper_only);
     return;
   }
   hipLaunchKernelGGL(k_pppp_sym, dim3(nt, B.ninst), dim3(256), 0, s, B, t, L, Lh, R, Rh, RES, w1, w2, ref_outputs);

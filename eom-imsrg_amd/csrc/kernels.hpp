@@ -158,8 +158,9 @@ void launch_gemm_tiled(int tile, const Slots& sl, const GemmProb* probs, const G
                        int sig = -1);
 int gemm_sig_supported(int sig);
 // RES1 += C - pm C^T with C = L1 R1 - Rh L3 R3^T held in w1's pppp blocks;
-// ref_outputs: also leave the reference's W1 = L1 R1 and W2 = Rh L3 R3^T
-// in w1 / w2 (otherwise w1's pppp blocks keep C and w2's are untouched)
+// ref_outputs 1: also leave the reference's W1 = L1 R1 and W2 = Rh L3 R3^T
+// in w1 / w2 (otherwise w1's pppp blocks keep C and w2's are untouched);
+// 2: only the upper tiles of RES1 (the caller fills the lower triangles)
 void launch_pppp_sym(const DevBasis& B, const PairTask* t, int nt, const double* L, int Lh, const double* R, int Rh,
                      double* RES, double* w1, double* w2, int ref_outputs, hipStream_t s);
 void launch_pphh_epilogue(const DevBasis& B, const SegTask* t, int nt, const double* L, int Lh, const double* R,

@@ -401,7 +401,7 @@ int imsrg_set_term_streams(imsrg_basis b, int on);
  * mean milliseconds per launch.  Inputs as for the term routines; every
  * kernel class runs once untimed first (fills w1/w2, LCC/RCC, WCC).
  * which: IMSRG_K_* below. */
-#define IMSRG_K_PPPP 0      /* 222_pp_hh pppp block: L1 R1 MFMA GEMM + symmetric epilogue */
+#define IMSRG_K_PPPP 0      /* 222_pp_hh pppp block: L1 R1 MFMA GEMM + symmetric epilogue (grouped: upper tiles, as in the commutator) */
 #define IMSRG_K_PPHH_SMALL 1 /* 222_pp_hh pp x (ph|hh) and (ph|hh)^2 GEMMs + epilogue */
 #define IMSRG_K_PH_GEMM 2   /* 222_ph WCC = LCC RCC^T (MFMA) */
 #define IMSRG_K_PH_SCATTER 3 /* 222_ph inverse Pandya (grouped: upper triangles, as in the commutator) */
