@@ -120,6 +120,7 @@ def main() -> int:
 
     kms, comm_plain_ms = {}, None
     if not a.steps_only:
+        kernel_ms("COMMUTATOR", 0, 20)   # untimed: every class's first launches and the clocks settle
         kms = {k: kernel_ms(k) for k in ("PPPP", "PPHH_SMALL", "PH_GEMM", "PH_SCATTER", "K122", "PANDYA",
                                          "ONEBODY", "COMMUTATOR")}
         if not a.no_plain:

@@ -17,7 +17,10 @@ namespace imsrg {
 
 namespace {
 
-constexpr int GBK = 16;
+#ifndef IMSRG_GEMM_BK
+#define IMSRG_GEMM_BK 16
+#endif
+constexpr int GBK = IMSRG_GEMM_BK;   // K step through LDS
 
 // per thread and K-tile: TBM*16/256 elements of op(A), TBN*16/256 of op(B)
 template <int TBM, int TBN>

@@ -4,9 +4,11 @@ each with and without XCD packing) with hipEvents on the launch stream, keep the
 persist the choice as JSON so later runs apply it without timing.
 
 Families (include/imsrg_hip.h, IMSRG_GEMM_*): PPPP (222_pp_hh L1 R1), PPX
-(pp x (ph|hh)), XX ((ph|hh)^T (ph|hh)), PH (222_ph W').  PPX and XX are
-timed through the kernel class that runs both (IMSRG_K_PPHH_SMALL), one
-family at a time with the other held at its current choice.
+(pp x (ph|hh)), XX ((ph|hh)^T (ph|hh)), PH (222_ph W').  Each family is
+timed through the whole commutator (IMSRG_K_COMMUTATOR), one family at a
+time with the others held at their current choice: a tile shape changes
+how the family's output meets the kernels that read it (the pppp epilogue
+reads the GEMM's tiles), so the family alone is not the objective.
 
 The cache file is EOM_IMSRG_TUNE_FILE if set, else
 ~/.cache/eom_imsrg/gemm_tiles.json; the entries measured on MI355X ship in
@@ -25,8 +27,8 @@ import torch
 from . import _lib
 from .basis import Basis
 
-FAMILIES = {"PPPP": ("PPPP", "PPPP_GEMM"), "PPX": ("PPX", "PPHH_SMALL"), "XX": ("XX", "PPHH_SMALL"),
-            "PH": ("PH", "PH_GEMM")}
+FAMILIES = {"PPPP": ("PPPP", "COMMUTATOR"), "PPX": ("PPX", "COMMUTATOR"), "XX": ("XX", "COMMUTATOR"),
+            "PH": ("PH", "COMMUTATOR")}
 XCD = 8
 CANDIDATES = [t | x for x in (0, XCD) for t in range(6)]
 SHIPPED = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuned_mi355x.json")
@@ -81,7 +83,7 @@ def current(b: Basis) -> dict:
     return {fam: int(_lib.lib.imsrg_get_gemm_tile(b.handle, _lib.GEMM[FAMILIES[fam][0]])) for fam in FAMILIES}
 
 
-def measure(b: Basis, iters: int = 5) -> tuple[dict, dict]:
+def measure(b: Basis, iters: int = 10) -> tuple[dict, dict]:
     """time every candidate of every family; returns (best choice,
     {family: {tile: ms}}).  Leaves the best choice applied."""
     from . import synth
@@ -117,7 +119,7 @@ def measure(b: Basis, iters: int = 5) -> tuple[dict, dict]:
     return choice, table
 
 
-def tune(b: Basis, iters: int = 5, force: bool = False, persist: bool = True) -> dict:
+def tune(b: Basis, iters: int = 10, force: bool = False, persist: bool = True) -> dict:
     """apply the stored choice for this basis and device, or measure, apply
     and store one; returns {"choice": ..., "source": "cache" | "measured",
     "times_ms": ... (measured only)}"""
