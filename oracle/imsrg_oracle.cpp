@@ -178,6 +178,8 @@ struct Basis {
   std::vector<i64> ccoff, wcoff;
   i64 cclen = 0, wclen = 0;
   std::vector<int> rmap, nbmap;              // [(i*N+j)*(JTM+1) + JC/2]
+  std::vector<int> ccpair;                   // ordered pairs (i, j) of the rows, block by block
+  std::vector<i64> ccpoff;                   // [q] first row of block q in ccpair / 2
   // 6j cache over (j1,j2,j4,j5 half-integer; j3,j6 integer), 2j units
   int nh6 = 0, nJ6 = 0;
   std::vector<double> six;
@@ -264,13 +266,14 @@ void build_basis(Basis& b, int N, const int* n, const int* l, const int* jj,
   b.rlen.assign(b.ncc, 0); b.ncol.assign(b.ncc, 0);
   b.rmap.assign((size_t)N * N * nJ, -1);
   b.nbmap.assign((size_t)N * N * nJ, -1);
-  b.ccoff.resize(b.ncc); b.wcoff.resize(b.ncc);
+  b.ccoff.resize(b.ncc); b.wcoff.resize(b.ncc); b.ccpoff.resize(b.ncc);
   i64 co = 0, wo = 0;
   for (int q1 = 0; q1 < b.ncc; ++q1) {
     int JC = (q1 % (JTM + 1)) * 2;
     int PAR = q1 / (2 * JTM + 2);
     int TZ = (q1 % (2 * JTM + 2)) / (JTM + 1);
     b.ccJ[q1] = JC; b.ccT[q1] = TZ; b.ccP[q1] = PAR;
+    b.ccpoff[q1] = (i64)b.ccpair.size() / 2;
     int r = 0, nbc = 0;
     for (int i = 0; i < N; ++i)
       for (int j = 0; j < N; ++j) {
@@ -280,6 +283,8 @@ void build_basis(Basis& b, int N, const int* n, const int* l, const int* jj,
         size_t x = ((size_t)i * N + j) * nJ + JC / 2;
         if (con[i] == 0 && con[j] == 1) b.nbmap[x] = nbc++;
         b.rmap[x] = r++;
+        b.ccpair.push_back(i);
+        b.ccpair.push_back(j);
       }
     b.rlen[q1] = r; b.ncol[q1] = nbc;
     b.ccoff[q1] = co; co += (i64)r * nbc;
@@ -1050,6 +1055,7 @@ int magnus(Op& DG, const Op& G, double conv, Work& w) {
 #include "imsrg_adjoint.inc"
 #include "imsrg_interaction.inc"
 #include "imsrg_hf.inc"
+#include "imsrg_tensor.inc"
 
 }  // namespace
 
@@ -1419,6 +1425,104 @@ double orc_bf_2body(void* h, const double* A_, int Ah, const double* B_, int Bh,
   }
   if (parts) { parts[0] = s1; parts[1] = s2; parts[2] = s3; }
   return s1 + s2 + s3;
+}
+
+// ------------------------------------------------------------- tensors
+void* orc_tensor_new(void* h, int rank, int dpar) {
+  TLayout* t = new TLayout();
+  build_tensor(*t, *(Basis*)h, rank, dpar);
+  return t;
+}
+void orc_tensor_free(void* th) { delete (TLayout*)th; }
+// out: oplen, nblocks, ncc, cclen, wlen
+void orc_tensor_sizes(void* th, i64* out) {
+  const TLayout& t = *(TLayout*)th;
+  out[0] = t.oplen; out[1] = t.nblocks; out[2] = t.ncc; out[3] = t.cclen; out[4] = t.wlen;
+}
+// per block: info[7q..] = 2J1, 2J2, Tz, P1, P2, q1, q2; dims[6q..]; goff[9q..]
+void orc_tensor_blocks(void* th, int* info, int* dims, i64* goff) {
+  const TLayout& t = *(TLayout*)th;
+  for (int q = 0; q < t.nblocks; ++q) {
+    const int v[7] = {t.J1[q], t.J2[q], t.T[q], t.P1[q], t.P2[q], t.q1[q], t.q2[q]};
+    for (int k = 0; k < 7; ++k) info[7 * q + k] = v[k];
+    for (int k = 0; k < 6; ++k) dims[6 * q + k] = t.dim[6 * q + k];
+    for (int k = 0; k < 9; ++k) goff[9 * q + k] = t.goff[9 * q + k];
+  }
+}
+int orc_tensor_block_index_ref(int J1, int J2, int rank, int T, int P) {
+  return tensor_block_index_ref(J1, J2, rank, T, P);
+}
+double orc_xxxsixj(int J1, int J2, int R, int j4, int j5, int j6) { init_logfact(); return xxxsixj(J1, J2, R, j4, j5, j6); }
+double orc_ninej_ref(int a, int bb, int J1, int c, int d, int J2, int J3, int J4, int R) {
+  init_logfact();
+  return ninej_ref(a, bb, J1, c, d, J2, J3, J4, R);
+}
+#define TOPS(th) const TLayout& t = *(TLayout*)th; const Basis& b = *t.b; (void)b;
+double orc_tensor_elem(void* th, const double* op, int herm, int a, int bb, int c, int d, int J1, int J2) {
+  TOPS(th) return tensor_elem(a, bb, c, d, J1, J2, TOp{&t, (double*)op, herm});
+}
+double orc_f_tensor_elem(void* th, const double* op, int herm, int a, int c) {
+  TOPS(th) return f_tensor_elem(a, c, TOp{&t, (double*)op, herm});
+}
+void orc_ts111(void* th, const double* L, int Lh, const double* R, int Rh, double* RES, int RESh) {
+  TOPS(th) TOp res{&t, RES, RESh}; ts111(Op{&b, (double*)L, Lh}, TOp{&t, (double*)R, Rh}, res);
+}
+void orc_ts121(void* th, const double* L, int Lh, const double* R, int Rh, double* RES, int RESh) {
+  TOPS(th) TOp res{&t, RES, RESh}; ts121(Op{&b, (double*)L, Lh}, TOp{&t, (double*)R, Rh}, res);
+}
+void orc_ts211(void* th, const double* L, int Lh, const double* R, int Rh, double* RES, int RESh) {
+  TOPS(th)
+  std::vector<double> LCC(b.cclen, 0.0);
+  cross_coupled(Op{&b, (double*)L, Lh}, LCC.data());
+  TOp res{&t, RES, RESh};
+  ts211(LCC.data(), Lh, TOp{&t, (double*)R, Rh}, res);
+}
+void orc_ts122(void* th, const double* L, int Lh, const double* R, int Rh, double* RES, int RESh) {
+  TOPS(th) TOp res{&t, RES, RESh}; ts122(Op{&b, (double*)L, Lh}, TOp{&t, (double*)R, Rh}, res);
+}
+void orc_ts212(void* th, const double* L, int Lh, const double* R, int Rh, double* RES, int RESh) {
+  TOPS(th) TOp res{&t, RES, RESh}; ts212(Op{&b, (double*)L, Lh}, TOp{&t, (double*)R, Rh}, res);
+}
+void orc_ts222pphh(void* th, const double* L, int Lh, const double* R, int Rh, double* RES, int RESh, double* w1,
+                   double* w2) {
+  TOPS(th) TOp res{&t, RES, RESh}, W1{&t, w1, Rh}, W2{&t, w2, Rh};
+  ts222pphh(Op{&b, (double*)L, Lh}, TOp{&t, (double*)R, Rh}, res, W1, W2);
+}
+void orc_ts221(void* th, const double* w1, const double* w2, int pm, double* RES, int RESh) {
+  TOPS(th) TOp res{&t, RES, RESh};
+  ts221(TOp{&t, (double*)w1, 1}, TOp{&t, (double*)w2, 1}, pm, res);
+}
+void orc_ts222ph(void* th, const double* L, int Lh, const double* R, int Rh, double* RES, int RESh) {
+  TOPS(th)
+  std::vector<double> LCC(b.cclen, 0.0), CC(t.cclen, 0.0), Wx(t.wlen, 0.0), Wy(t.wlen, 0.0);
+  cross_coupled(Op{&b, (double*)L, Lh}, LCC.data());
+  TOp res{&t, RES, RESh};
+  ts222ph(LCC.data(), Lh, TOp{&t, (double*)R, Rh}, res, CC.data(), Wx.data(), Wy.data());
+}
+// calculate_single_pandya of every block (both terms) into CC (cclen doubles)
+void orc_tensor_pandya(void* th, const double* R, int Rh, double* CC) {
+  TOPS(th)
+  TOp r{&t, (double*)R, Rh};
+  for (int qx = 0; qx < t.ncc; ++qx) {
+    if (t.cJ4[qx] > 2 * b.JTM) continue;
+    if ((i64)t.r1[qx] * t.nb2[qx] > 0) single_pandya(r, qx, 1, CC);
+    if ((i64)t.nb1[qx] * t.r2[qx] > 0) single_pandya(r, qx, 2, CC);
+  }
+}
+void orc_ts_commutator(void* th, const double* L, int Lh, const double* R, int Rh, double* RES, int RESh) {
+  TOPS(th)
+  TOp res{&t, RES, RESh};
+  TWork w;
+  ts_commutator(Op{&b, (double*)L, Lh}, TOp{&t, (double*)R, Rh}, res, w);
+}
+int orc_bch_tensor(void* th, const double* G, int Gh, double* HS, int HSh, double conv, double* advals) {
+  TOPS(th)
+  TOp hs{&t, HS, HSh};
+  return bch_tensor(Op{&b, (double*)G, Gh}, hs, conv, advals);
+}
+double orc_tensor_frob_norm(void* th, const double* op) { TOPS(th) return tensor_frob_norm(TOp{&t, (double*)op, 1}); }
+void orc_copy_rank0_to_tensor(void* th, const double* A, int Ah, double* B) {
+  TOPS(th) TOp tb{&t, B, Ah}; copy_rank0_to_tensor(Op{&b, (double*)A, Ah}, tb);
 }
 
 }  // extern "C"

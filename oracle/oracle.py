@@ -68,6 +68,20 @@ def lib():
             "orc_hf_density": (None, [vp, dp, dp]), "orc_hf_gamma": (None, [vp, dp, dp, dp]),
             "orc_hf_finish": (None, [vp, dp, dp, dp, dp, dp]), "orc_hf_transform_2b": (None, [vp, dp, dp]),
             "orc_sqop_to_hf": (None, [vp, dp, dp]),
+            "orc_tensor_new": (vp, [vp, i, i]), "orc_tensor_free": (None, [vp]),
+            "orc_tensor_sizes": (None, [vp, lp]), "orc_tensor_blocks": (None, [vp, ip, ip, lp]),
+            "orc_tensor_block_index_ref": (i, [i] * 5), "orc_xxxsixj": (d, [i] * 6),
+            "orc_ninej_ref": (d, [i] * 9),
+            "orc_tensor_elem": (d, [vp, dp, i, i, i, i, i, i, i]), "orc_f_tensor_elem": (d, [vp, dp, i, i, i]),
+            "orc_ts111": (None, [vp, dp, i, dp, i, dp, i]), "orc_ts121": (None, [vp, dp, i, dp, i, dp, i]),
+            "orc_ts211": (None, [vp, dp, i, dp, i, dp, i]), "orc_ts122": (None, [vp, dp, i, dp, i, dp, i]),
+            "orc_ts212": (None, [vp, dp, i, dp, i, dp, i]),
+            "orc_ts222pphh": (None, [vp, dp, i, dp, i, dp, i, dp, dp]),
+            "orc_ts221": (None, [vp, dp, dp, i, dp, i]), "orc_ts222ph": (None, [vp, dp, i, dp, i, dp, i]),
+            "orc_tensor_pandya": (None, [vp, dp, i, dp]),
+            "orc_ts_commutator": (None, [vp, dp, i, dp, i, dp, i]),
+            "orc_bch_tensor": (i, [vp, dp, i, dp, i, d, dp]), "orc_tensor_frob_norm": (d, [vp, dp]),
+            "orc_copy_rank0_to_tensor": (None, [vp, dp, i, dp]),
         }
         for k, (r, a) in sig.items():
             f = getattr(L, k)
@@ -433,6 +447,118 @@ class OracleBasis:
         if getattr(self, "h", None):
             lib().orc_basis_free(self.h)
             self.h = None
+
+
+class OracleTensor:
+    """tensor operator layout of rank `rank` (2 lambda) and parity change
+    `dpar` (0 or 2) over an OracleBasis (allocate_tensor,
+    basic_IMSRG.f90:688-951), with the scalar-tensor commutator terms"""
+
+    def __init__(self, ob: OracleBasis, rank: int, dpar: int = 0):
+        self.ob = ob
+        self.rank, self.dpar = int(rank), int(dpar)
+        self.h = lib().orc_tensor_new(ob.h, self.rank, self.dpar)
+        s = np.zeros(5, np.int64)
+        lib().orc_tensor_sizes(self.h, s.ctypes.data_as(C.POINTER(C.c_int64)))
+        self.oplen, self.nblocks, self.ncc, self.cclen, self.wlen = (int(x) for x in s)
+        nb = self.nblocks
+        info, dims, goff = np.zeros(7 * nb, np.int32), np.zeros(6 * nb, np.int32), np.zeros(9 * nb, np.int64)
+        lib().orc_tensor_blocks(self.h, info.ctypes.data_as(C.POINTER(C.c_int)),
+                                dims.ctypes.data_as(C.POINTER(C.c_int)), goff.ctypes.data_as(C.POINTER(C.c_int64)))
+        self.info, self.dims, self.goff = info.reshape(nb, 7), dims.reshape(nb, 6), goff.reshape(nb, 9)
+
+    SEA1 = (0, 0, 0, 1, 2, 1, 2, 1, 2)
+    SEA2 = (0, 1, 2, 1, 2, 2, 0, 0, 1)
+
+    def shape(self, q, g):
+        return int(self.dims[q, self.SEA1[g]]), int(self.dims[q, 3 + self.SEA2[g]])
+
+    def elements(self):
+        """(q, g, ix, jx, (a, b), (c, d), 2J1, 2J2, flat offset) of every stored element"""
+        for q in range(self.nblocks):
+            J1, J2, _, _, _, q1, q2 = (int(x) for x in self.info[q])
+            if q2 < 0:
+                continue
+            for g in range(9):
+                n1, n2 = self.shape(q, g)
+                if n1 * n2 == 0:
+                    continue
+                rows, cols = self.ob.pairs(q1, self.SEA1[g]), self.ob.pairs(q2, self.SEA2[g])
+                for jx in range(n2):
+                    for ix in range(n1):
+                        yield q, g, ix, jx, tuple(rows[ix]), tuple(cols[jx]), J1, J2, \
+                            int(self.goff[q, g]) + ix + jx * n1
+
+    def zeros(self):
+        return np.zeros(self.oplen)
+
+    def tensor_elem(self, op, herm, a, b, c, d, J1, J2):
+        return lib().orc_tensor_elem(self.h, _dp(op), herm, a, b, c, d, J1, J2)
+
+    def f_tensor_elem(self, op, herm, a, b):
+        return lib().orc_f_tensor_elem(self.h, _dp(op), herm, a, b)
+
+    def _term(self, name, L, Lh, R, Rh, RES, RESh):
+        getattr(lib(), name)(self.h, _dp(L), Lh, _dp(R), Rh, _dp(RES), RESh)
+        return RES
+
+    def ts111(self, *a): return self._term("orc_ts111", *a)
+    def ts121(self, *a): return self._term("orc_ts121", *a)
+    def ts211(self, *a): return self._term("orc_ts211", *a)
+    def ts122(self, *a): return self._term("orc_ts122", *a)
+    def ts212(self, *a): return self._term("orc_ts212", *a)
+    def ts222ph(self, *a): return self._term("orc_ts222ph", *a)
+
+    def ts222pphh(self, L, Lh, R, Rh, RES, RESh, w1=None, w2=None):
+        w1 = self.zeros() if w1 is None else w1
+        w2 = self.zeros() if w2 is None else w2
+        lib().orc_ts222pphh(self.h, _dp(L), Lh, _dp(R), Rh, _dp(RES), RESh, _dp(w1), _dp(w2))
+        return RES, w1, w2
+
+    def ts221(self, w1, w2, pm, RES, RESh):
+        lib().orc_ts221(self.h, _dp(w1), _dp(w2), pm, _dp(RES), RESh)
+        return RES
+
+    def pandya(self, R, Rh):
+        CC = np.zeros(self.cclen)
+        lib().orc_tensor_pandya(self.h, _dp(R), Rh, _dp(CC))
+        return CC
+
+    def commutator(self, L, Lh, R, Rh, RESh):
+        RES = self.zeros()
+        lib().orc_ts_commutator(self.h, _dp(L), Lh, _dp(R), Rh, _dp(RES), RESh)
+        return RES
+
+    def bch(self, G, Gh, HS, HSh, conv=9.999999974752427e-07):
+        out = np.ascontiguousarray(HS, np.float64).copy()
+        adv = np.zeros(30)
+        it = lib().orc_bch_tensor(self.h, _dp(G), Gh, _dp(out), HSh, conv, _dp(adv))
+        return out, it, adv
+
+    def frob_norm(self, op):
+        return lib().orc_tensor_frob_norm(self.h, _dp(op))
+
+    def from_rank0(self, A, Ah):
+        B = self.zeros()
+        lib().orc_copy_rank0_to_tensor(self.h, _dp(A), Ah, _dp(B))
+        return B
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_tensor_free(self.h)
+            self.h = None
+
+
+def block_index_ref(J1, J2, rank, T, P):
+    return lib().orc_tensor_block_index_ref(J1, J2, rank, T, P)
+
+
+def xxxsixj(*a):
+    return lib().orc_xxxsixj(*a)
+
+
+def ninej_ref(*a):
+    return lib().orc_ninej_ref(*a)
 
 
 def sixj(*a):

@@ -57,3 +57,49 @@ def ljt_blocks(b):
     for a in range(b.norb):
         out.setdefault((int(b.l[a]), int(b.jj[a]), int(b.tz[a])), []).append(a)
     return list(out.values())
+
+
+def random_tensor(ot, b, herm, seed, sigma=1.0):
+    """random tensor operator in the flat tensor layout of `ot` (an
+    OracleTensor over basis b): selection rules of rank / parity / tz on the
+    one-body part, Pauli-forbidden elements zero, and the hermiticity
+    <cd J||T||ab J> = herm <ab J||T||cd J>, f(q,p) = herm (-1)^((jp-jq)/2)
+    f(p,q) that the reference's storage (both J1 = J2 orientations, full
+    fpp / fhh) assumes"""
+    rng = np.random.default_rng(seed)
+    op = ot.zeros()
+    A, P = b.nhole, b.npart
+    H, Pp = list(b.holes), list(b.parts)
+
+    def ok1(a, c):
+        return (b.tz[a] == b.tz[c] and (b.l[a] + b.l[c] + ot.dpar // 2) % 2 == 0
+                and abs(b.jj[a] - b.jj[c]) <= ot.rank <= b.jj[a] + b.jj[c])
+
+    for lst, off, n in ((H, b.off_fhh, A), (Pp, b.off_fpp, P)):
+        for i in range(n):
+            for k in range(i, n):
+                a, c = lst[i], lst[k]
+                if not ok1(a, c):
+                    continue
+                v = sigma * rng.normal()
+                ph = herm * (1 - 2 * ((abs(int(b.jj[a]) - int(b.jj[c])) // 2) % 2))
+                if i == k and ph < 0:
+                    v = 0.0
+                op[off + i + k * n] = v
+                op[off + k + i * n] = ph * v
+    for i, a in enumerate(Pp):
+        for k, c in enumerate(H):
+            if ok1(a, c):
+                op[b.off_fph + i + k * P] = sigma * rng.normal()
+    done = {}
+    for q, g, ix, jx, p1, p2, J1, J2, o in ot.elements():
+        if (p1[0] == p1[1] and (J1 // 2) % 2) or (p2[0] == p2[1] and (J2 // 2) % 2):
+            continue
+        if J1 == J2 and (p2, p1, J1) in done:
+            op[o] = herm * op[done[(p2, p1, J1)]]
+            continue
+        if J1 == J2 and p1 == p2 and herm < 0:
+            continue
+        op[o] = sigma * rng.normal()
+        done[(p1, p2, J1)] = o
+    return op
