@@ -29,7 +29,8 @@ HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 DEV_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-I", os.path.join(ROOT, "include"), "-I", CSRC]
 
-SOURCES = ["basis.cpp", "interaction.cpp", "hf.cpp", "kernels.hip", "kernels_adj.hip", "gemm.hip", "capi.hip"]
+SOURCES = ["basis.cpp", "interaction.cpp", "hf.cpp", "kernels.hip", "kernels_adj.hip", "gemm.hip", "tensor.hip",
+           "capi.hip"]
 
 
 def _compile(src: str) -> str:

@@ -3186,3 +3186,5 @@ int imsrg_profile_kernel(imsrg_basis B, int which, const double* L, int Lh, cons
 }
 
 }  // extern "C"
+
+#include "capi_tensor.inc"

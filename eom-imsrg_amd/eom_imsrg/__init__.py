@@ -11,8 +11,8 @@ from .orbits import hk_orbits, oxygen16_holes
 from . import commutators, synth
 from .magnus import Flow, REF_TOL
 from .batch import Batch
-from . import adjoint, interaction, tune
+from . import adjoint, interaction, tune, tensor
 from .adjoint import flow_energy
 
 __all__ = ["Basis", "SqOp", "CCMat", "CCWork", "Flow", "REF_TOL", "commutators", "synth", "hk_orbits",
-           "oxygen16_holes", "adjoint", "flow_energy", "tune", "Batch", "interaction", "_lib"]
+           "oxygen16_holes", "adjoint", "flow_energy", "tune", "Batch", "interaction", "tensor", "_lib"]

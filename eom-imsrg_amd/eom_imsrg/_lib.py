@@ -125,7 +125,25 @@ SIGNATURES = {
     "imsrg_set_term_streams": (_i, [_p, _i]),
     "imsrg_get_gemm_tile": (_i, [_p, _i]),
     "imsrg_profile_kernel": (_i, [_p, _i, _dp, _i, _dp, _i, _dp, _i, _dp, _dp, _dp, _dp, _dp, _i, _d, _p]),
+    "imsrg_tensor_create": (_i, [_p, _i, _i, C.POINTER(_p)]),
+    "imsrg_tensor_destroy": (_i, [_p]),
+    "imsrg_tensor_size": (C.c_int64, [_p, _i]),
+    "imsrg_tensor_blocks": (_i, [_p, _ip, _ip, _lp]),
+    "imsrg_ts_commutator_111": (_i, [_p, _dp, _i, _dp, _i, _dp, _i, _p]),
+    "imsrg_ts_commutator_121": (_i, [_p, _dp, _i, _dp, _i, _dp, _i, _p]),
+    "imsrg_ts_commutator_211": (_i, [_p, _dp, _i, _dp, _i, _dp, _i, _p]),
+    "imsrg_ts_commutator_122": (_i, [_p, _dp, _i, _dp, _i, _dp, _i, _p]),
+    "imsrg_ts_commutator_212": (_i, [_p, _dp, _i, _dp, _i, _dp, _i, _p]),
+    "imsrg_ts_commutator_222_pp_hh": (_i, [_p, _dp, _i, _dp, _i, _dp, _i, _dp, _dp, _p]),
+    "imsrg_ts_commutator_221": (_i, [_p, _dp, _dp, _i, _dp, _i, _p]),
+    "imsrg_calculate_single_pandya": (_i, [_p, _dp, _i, _dp, _p]),
+    "imsrg_ts_commutator_222_ph": (_i, [_p, _dp, _i, _dp, _dp, _i, _dp, _i, _p]),
+    "imsrg_ts_commutator": (_i, [_p, _dp, _i, _dp, _i, _dp, _i, _p]),
+    "imsrg_bch_tensor": (_i, [_p, _dp, _i, _dp, _i, C.c_double, _ip, _d, _p]),
+    "imsrg_tensor_frob_norm": (_i, [_p, _dp, _d, _p]),
+    "imsrg_tensor_add": (_i, [_p, _dp, C.c_double, _dp, C.c_double, _dp, _p]),
 }
+TSIZE = dict(OP=0, NBLOCKS=1, NCC=2, CC=3, W=4)
 
 GEMM = dict(PPPP=0, PPX=1, XX=2, PH=3)
 GENERATOR = dict(white=0, atan=1, w2=2)

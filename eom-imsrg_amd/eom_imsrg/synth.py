@@ -206,3 +206,93 @@ def me2j_values(orbits: dict, seed: int = 0, scale: float = 2.0) -> np.ndarray:
             v1 = 0.0
         out.extend((v0, v1, v1, v1))
     return np.array(out)
+
+
+# tgam g <-> its transposed partner in the J1 = J2 blocks (pppp, phph, hhhh
+# are their own; ppph <-> phpp, pphh <-> hhpp, phhh <-> hhph)
+_TMIRROR = (0, 7, 6, 3, 4, 8, 2, 1, 5)
+
+
+def tensor_random(layout, seed: int = 5, herm: int = 1, sigma: float = 1.0) -> np.ndarray:
+    """random tensor operator in the flat layout of a tensor.TensorLayout:
+    one-body elements N(0, sigma) where rank, parity and tz allow them (fpp
+    and fhh with f(q,p) = herm (-1)^((jp-jq)/2) f(p,q), as the commutators
+    mirror them); two-body blocks N(0, sigma), Pauli-forbidden pairs zero, and
+    in the J1 = J2 blocks <cd J||T||ab J> = herm <ab J||T||cd J> (the
+    orientation the storage keeps twice).  Draw order: one-body pairs, then
+    blocks q ascending, tgam 1..9 column by column (the mirrored halves are
+    overwritten, their draws still consumed)."""
+    b = layout.basis
+    rng = SplitMix64(seed)
+    op = np.zeros(layout.oplen)
+    rank, dp = layout.rank, layout.dpar // 2
+    l, jj, tz = np.asarray(b.l), np.asarray(b.jj), np.asarray(b.tz)
+
+    def ok1(x, y):
+        return (tz[x] == tz[y]) & ((l[x] + l[y] + dp) % 2 == 0) & (np.abs(jj[x] - jj[y]) <= rank) & \
+            (rank <= jj[x] + jj[y])
+
+    for lst, off in ((np.asarray(b.holes), b.off_fhh), (np.asarray(b.parts), b.off_fpp)):
+        n = len(lst)
+        if n == 0:
+            continue
+        X = sigma * rng.normal(n * n).reshape(n, n, order="F")
+        a, c = lst[:, None], lst[None, :]
+        ph = herm * (1 - 2 * ((np.abs(jj[a] - jj[c]) // 2) % 2))
+        X = np.where(ok1(a, c), X, 0.0)
+        up = np.triu(X)
+        M = up + (ph * up).T
+        np.fill_diagonal(M, np.where(np.diag(ph) > 0, np.diag(up), 0.0))
+        op[off: off + n * n] = M.reshape(-1, order="F")
+    P, A = len(b.parts), len(b.holes)
+    if P * A:
+        X = sigma * rng.normal(P * A).reshape(P, A, order="F")
+        X = np.where(ok1(np.asarray(b.parts)[:, None], np.asarray(b.holes)[None, :]), X, 0.0)
+        op[b.off_fph: b.off_fph + P * A] = X.reshape(-1, order="F")
+
+    def pauli_mask(pairs, J):
+        pr = np.asarray(pairs).reshape(-1, 2)
+        return ~((pr[:, 0] == pr[:, 1]) & ((J // 2) % 2 == 1))
+
+    blocks = {}
+    for q in range(layout.nblocks):
+        J1, J2, T, P1, P2, q1, q2 = (int(x) for x in layout.info[q])
+        if q2 < 0:
+            continue
+        for g in range(9):
+            n1, n2 = layout.shape(q, g)
+            if n1 * n2 == 0:
+                continue
+            X = sigma * rng.normal(n1 * n2).reshape(n1, n2, order="F")
+            rm = pauli_mask(b.pairs[q1][layout.SEA1[g]], J1)
+            cm = pauli_mask(b.pairs[q2][layout.SEA2[g]], J2)
+            X = X * rm[:, None] * cm[None, :]
+            blocks[(q, g)] = X
+    look = {}
+    for q in range(layout.nblocks):
+        J1, J2, T, P1 = (int(x) for x in layout.info[q, :4])
+        look[(J1, J2, T, P1)] = q
+    for q in range(layout.nblocks):
+        J1, J2, T, P1, P2, q1, q2 = (int(x) for x in layout.info[q])
+        if q2 < 0 or J1 != J2:
+            continue
+        qm = look[(J1, J1, T, P2)]
+        for g in range(9):
+            gm = _TMIRROR[g]
+            if (q, g) not in blocks or (qm, gm) not in blocks:
+                continue
+            if (qm, gm) < (q, g):
+                continue
+            X = blocks[(q, g)]
+            if (qm, gm) == (q, g):
+                up = np.triu(X)
+                M = up + herm * np.triu(X, 1).T
+                if herm < 0:
+                    np.fill_diagonal(M, 0.0)
+                blocks[(q, g)] = M
+            else:
+                blocks[(qm, gm)] = herm * X.T
+    for (q, g), X in blocks.items():
+        o = int(layout.goff[q, g])
+        op[o: o + X.size] = X.reshape(-1, order="F")
+    return op

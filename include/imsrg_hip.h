@@ -34,6 +34,7 @@ extern "C" {
 
 typedef struct imsrg_basis_s* imsrg_basis;
 typedef struct imsrg_flow_s* imsrg_flow;
+typedef struct imsrg_tensor_s* imsrg_tensor;
 typedef void* imsrg_stream;
 
 #define IMSRG_OK 0
@@ -414,6 +415,97 @@ int imsrg_set_term_streams(imsrg_basis b, int on);
 int imsrg_profile_kernel(imsrg_basis b, int which, const double* L, int Lherm, const double* R, int Rherm,
                          double* RES, int RESherm, double* w1, double* w2, double* LCC, double* RCC, double* WCC,
                          int iters, double* ms, imsrg_stream stream);
+
+/* ------------------------------------------------ tensor operators */
+/* A tensor operator of rank lambda = rank/2 and parity change dpar/2 (the
+ * reference's sq_op with op%rank > 0, both "multiplied by 2") is one flat
+ * FP64 device array:
+ *
+ *   [E0 | fhh (A x A) | fph (P x A) | fpp (P x P) |
+ *    for each block q: tgam1 .. tgam9 ]
+ *
+ * blocks in allocate_tensor's loop order (2J1 = 0 .. 2 jmax, 2J2 =
+ * max(|2J1 - rank|, 2J1) .. 2J1 + rank, Tz = -1..1, P1 = 0..1; P2 = P1, or
+ * 1 - P1 when dpar/2 is odd), tgam g = (pairs of class sea1(g) of channel
+ * (J1, Tz, P1)) x (pairs of class sea2(g) of channel (J2, Tz, P2)) with
+ * sea1 = (pp,pp,pp,ph,hh,ph,hh,ph,hh), sea2 = (pp,ph,hh,ph,hh,hh,pp,pp,ph),
+ * column-major (basic_IMSRG.f90:688-951, tblck(q)%tgam(g)%X).  Blocks with
+ * 2J2 > 2 jmax are empty.  fhh and fpp hold both triangles; the hole-particle
+ * one-body elements are herm (-1)^((ja-jb)/2) times the transposed fph. */
+#define IMSRG_TSIZE_OP 0      /* doubles per tensor operator */
+#define IMSRG_TSIZE_NBLOCKS 1 /* tensor blocks */
+#define IMSRG_TSIZE_NCC 2     /* tensor cross-coupled blocks (pandya_mat) */
+#define IMSRG_TSIZE_CC 3      /* doubles of a tensor cross-coupled set: CCX and CCR of every block */
+#define IMSRG_TSIZE_W 4       /* doubles of one 222_ph product set (Wx or Wy of every block) */
+/* allocate_tensor + allocate_small_tensor_CCMAT + store_6j_3halfint
+ * (basic_IMSRG.f90:688-951,2335-2411; cross_coupled.f90:498-700): rank and
+ * dpar twice lambda and twice the parity change (STOP 'RANK MUST BE
+ * MULTIPLIED BY 2' becomes IMSRG_EINVAL).  Host tables only; device tables
+ * on first use. */
+int imsrg_tensor_create(imsrg_basis b, int rank, int dpar, imsrg_tensor* out);
+int imsrg_tensor_destroy(imsrg_tensor t);
+int64_t imsrg_tensor_size(imsrg_tensor t, int what);
+/* per block q: info[7q..] = 2J1, 2J2, Tz, P1, P2, channel of J1, channel of
+ * J2 (-1: empty); dims[6q..] = npp1, nph1, nhh1, npp2, nph2, nhh2;
+ * goff[9q..] offset of tgam g */
+int imsrg_tensor_blocks(imsrg_tensor t, int* info, int* dims, int64_t* goff);
+/* TS_commutator_111, TS_commutators.f90:10-55: one-body part of [L1, R1],
+ * L scalar, R the tensor (assigns RES one-body) */
+int imsrg_ts_commutator_111(imsrg_tensor t, const double* L, int Lherm, const double* R, int Rherm, double* RES,
+                            int RESherm, imsrg_stream stream);
+/* TS_commutator_121, TS_commutators.f90:58-303: one-body part of [L1, R2]
+ * (accumulates; fhh / fpp mirrored with RES's herm) */
+int imsrg_ts_commutator_121(imsrg_tensor t, const double* L, int Lherm, const double* R, int Rherm, double* RES,
+                            int RESherm, imsrg_stream stream);
+/* TS_commutator_211, TS_commutators.f90:307-552: one-body part of
+ * -[R1, L2]; LCC = the scalar cross-coupled set of L
+ * (imsrg_calculate_cross_coupled) */
+int imsrg_ts_commutator_211(imsrg_tensor t, const double* LCC, int LCCherm, const double* R, int Rherm, double* RES,
+                            int RESherm, imsrg_stream stream);
+/* TS_commutator_122, TS_commutators.f90:555-663: two-body part of [L1, R2]
+ * (assigns RES two-body) */
+int imsrg_ts_commutator_122(imsrg_tensor t, const double* L, int Lherm, const double* R, int Rherm, double* RES,
+                            int RESherm, imsrg_stream stream);
+/* TS_commutator_212, TS_commutators.f90:666-806: two-body part of -[R1, L2]
+ * (accumulates) */
+int imsrg_ts_commutator_212(imsrg_tensor t, const double* L, int Lherm, const double* R, int Rherm, double* RES,
+                            int RESherm, imsrg_stream stream);
+/* TS_commutator_222_pp_hh, TS_commutators.f90:1070-1456 (accumulates; w1,
+ * w2: tensor-sized intermediates left for TS_commutator_221) */
+int imsrg_ts_commutator_222_pp_hh(imsrg_tensor t, const double* L, int Lherm, const double* R, int Rherm,
+                                  double* RES, int RESherm, double* w1, double* w2, imsrg_stream stream);
+/* TS_commutator_221, TS_commutators.f90:809-1067 (accumulates; after
+ * 222_pp_hh; pm = L%herm * R%herm) */
+int imsrg_ts_commutator_221(imsrg_tensor t, const double* w1, const double* w2, int pm, double* RES, int RESherm,
+                            imsrg_stream stream);
+/* calculate_single_pandya, cross_coupled.f90:1416-1634, of every block and
+ * both terms: RCC (IMSRG_TSIZE_CC doubles) <- CCX and CCR of the tensor R */
+int imsrg_calculate_single_pandya(imsrg_tensor t, const double* R, int Rherm, double* RCC, imsrg_stream stream);
+/* TS_commutator_222_ph, TS_commutators.f90:1459-1849 (accumulates): LCC the
+ * scalar cross-coupled set of L, RCC the tensor cross-coupled workspace
+ * (filled here, as the reference fills it block by block).  Uses the
+ * reference's factor LCC%herm on the Wx products (TS_commutators.f90:1506),
+ * which makes it the commutator for an antihermitian L (the BCH generator). */
+int imsrg_ts_commutator_222_ph(imsrg_tensor t, const double* LCC, int LCCherm, double* RCC, const double* R,
+                               int Rherm, double* RES, int RESherm, imsrg_stream stream);
+/* RES = [L, R]: the term sequence of one nested commutator of BCH_TENSOR
+ * (IMSRG_MAGNUS.f90:425-446) with its own workspaces */
+int imsrg_ts_commutator(imsrg_tensor t, const double* L, int Lherm, const double* R, int Rherm, double* RES,
+                        int RESherm, imsrg_stream stream);
+/* BCH_TENSOR, IMSRG_MAGNUS.f90:389-464: HS <- e^G HS e^-G for the tensor HS
+ * (in place), terms iw = 2..30 until mat_frob_norm(term) * coef < conv (the
+ * reference's conv = 1e-6 default-real literal is 9.999999974752427e-07).
+ * iterations = the last iw; advals (optional, 30 doubles) as there.
+ * Synchronous (one norm read per term). */
+int imsrg_bch_tensor(imsrg_tensor t, const double* G, int Gherm, double* HS, int HSherm, double conv,
+                     int* iterations, double* advals, imsrg_stream stream);
+/* mat_frob_norm, basic_IMSRG.f90:4056-4087, rank > 0 branch (returns the
+ * value; synchronises) */
+int imsrg_tensor_frob_norm(imsrg_tensor t, const double* op, double* value, imsrg_stream stream);
+/* add_sq_op / append_operator (basic_IMSRG.f90:3551-3611) on tensors:
+ * C = ax A + bx B */
+int imsrg_tensor_add(imsrg_tensor t, const double* A, double ax, const double* B, double bx, double* C,
+                     imsrg_stream stream);
 
 #ifdef __cplusplus
 }
