@@ -51,7 +51,8 @@ def _parse():
     ap.add_argument("--no-setup", action="store_true",
                     help="skip the Hamiltonian set-up timings (me2j reader, HO one-body part, HO / HF normal ordering)")
     ap.add_argument("--no-tune", action="store_true", help="keep the default GEMM tiles (no autotuner)")
-    ap.add_argument("--leg", choices=["gradient", "batch", "emax10", "setup"], default=None,
+    ap.add_argument("--no-tensor", action="store_true", help="skip the rank-2 tensor BCH leg")
+    ap.add_argument("--leg", choices=["gradient", "batch", "emax10", "setup", "tensor"], default=None,
                     help="run only this part and print its JSON object (for profiler runs)")
     ap.add_argument("--kernel-only", default=None,
                     help="launch one kernel class repeatedly and exit (for rocprofv3 counter runs)")
@@ -84,6 +85,9 @@ def main() -> int:
         return 0
     if a.leg == "setup":
         print(json.dumps(setup_bench(a.emax)))
+        return 0
+    if a.leg == "tensor":
+        print(json.dumps(tensor_bench(a.emax, not a.no_cpu)))
         return 0
     b = Basis.oxygen16(a.emax)
     tuned = None if a.no_tune else tune.tune(b)
@@ -207,6 +211,7 @@ def main() -> int:
     batch = None if a.no_batch else batch_bench(64, 6, a.batch_seq, a.batch_cpu_check)
     e10 = None if a.no_emax10 else emax10_bench(3, 1, not a.no_tune)
     setup = None if a.no_setup else setup_bench(a.emax)
+    tens = None if a.no_tensor else tensor_bench(a.emax, not a.no_cpu)
 
     # ---- roofline of the dominant kernel
     cand = {
@@ -253,6 +258,7 @@ def main() -> int:
         "batch": batch,
         "emax10": e10,
         "setup": setup,
+        "tensor": tens,
         "roofline": roof,
         "cpu_baseline": cpu,
     }
@@ -447,6 +453,91 @@ def emax10_bench(nsteps: int, warm: int, autotune: bool):
             "steps": nsteps, "bch_conv": 1e-10, "commutators_per_step": [int(x[2] + x[3] - 1) for x in log],
             "E0_after_steps": log[-1][0], "commutator_ms": round(cms.value, 3),
             "commutator_tflops": round(F / (cms.value / 1e3) / 1e12, 3), "gemm_tune": tuned}
+
+
+def tensor_bench(emax: int, cpu: bool):
+    """transform_sq_op_BCH of a rank-2 observable (BCH_TENSOR,
+    IMSRG_MAGNUS.f90:389-464) at eMax `emax`: one [Omega, O] per term
+    class and whole (hipEvents on the launch stream, mean of 5 after one
+    warm-up call), and the whole BCH_TENSOR.  Omega: the commutator kernels'
+    synthetic generator; O: synth.tensor_random (rank 4 = 2 lambda, even
+    parity, hermitian).  CPU: the oracle's commutator at eMax 4 (1 thread,
+    host BLAS) beside the GPU's at the same size."""
+    import numpy as np
+    import torch
+    from eom_imsrg import Basis, SqOp, CCMat, synth, tensor, commutators as cm
+
+    def run(emax_, reps, with_terms):
+        b = Basis.oxygen16(emax_)
+        tl = tensor.TensorLayout(b, 4, 0)
+        G = SqOp.from_numpy(b, synth.antihermitian(b, seed=1, sigma=0.05), -1)
+        O = tensor.TensorOp.from_numpy(tl, synth.tensor_random(tl, seed=5, herm=1), 1)
+        RES = tensor.TensorOp.zeros(tl, 1)
+        s = torch.cuda.current_stream()
+
+        def timed(fn):
+            fn()
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(s)
+            for _ in range(reps):
+                fn()
+            e1.record(s)
+            torch.cuda.synchronize()
+            return e0.elapsed_time(e1) / reps
+
+        out = {"orbits": b.norb, "tensor_elements": tl.oplen,
+               "commutator_ms": round(timed(lambda: tensor.TS_commutator(G, O, RES)), 3)}
+        if with_terms:
+            LCC = CCMat(b, -1)
+            cm.calculate_cross_coupled(G, LCC)
+            RCC = tensor.TensorCC(tl)
+            w1, w2 = tensor.TensorOp.zeros(tl), tensor.TensorOp.zeros(tl)
+            terms = {
+                "111": lambda: tensor.TS_commutator_111(G, O, RES),
+                "121": lambda: tensor.TS_commutator_121(G, O, RES),
+                "211": lambda: tensor.TS_commutator_211(LCC, O, RES),
+                "122": lambda: tensor.TS_commutator_122(G, O, RES),
+                "212": lambda: tensor.TS_commutator_212(G, O, RES),
+                "222_pp_hh": lambda: tensor.TS_commutator_222_pp_hh(G, O, RES, w1, w2),
+                "221": lambda: tensor.TS_commutator_221(w1, w2, -1, RES),
+                "single_pandya": lambda: tensor.calculate_single_pandya(O, RCC),
+                "222_ph": lambda: tensor.TS_commutator_222_ph(LCC, RCC, O, RES),
+            }
+            out["term_ms"] = {k: round(timed(f), 3) for k, f in terms.items()}
+            HS = O.clone()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            it, adv = tensor.BCH_TENSOR(G, HS)
+            torch.cuda.synchronize()
+            out["bch_tensor_s"] = round(time.perf_counter() - t0, 4)
+            out["bch_terms"] = int(it)
+            out["bch_last_term_norm"] = float(adv[it - 1])
+        return out
+
+    res = {"workload": f"bch_tensor_rank2_emax{emax}", "rank": 4, "dpar": 0, **run(emax, 5, True)}
+    small = run(4, 5, False)
+    res["emax4_commutator_ms"] = small["commutator_ms"]
+    if cpu:
+        sys.path.insert(0, ROOT)
+        from oracle.oracle import OracleBasis, OracleTensor, use_host_blas
+        from eom_imsrg import Basis as B_
+        use_host_blas()
+        b4 = B_.oxygen16(4)
+        o4 = OracleBasis(b4.n, b4.l, b4.jj, b4.tz, b4.hole)
+        ot = OracleTensor(o4, 4, 0)
+        tl4 = tensor.TensorLayout(b4, 4, 0)
+        G4 = synth.antihermitian(b4, seed=1, sigma=0.05)
+        O4 = synth.tensor_random(tl4, seed=5, herm=1)
+        ot.commutator(G4, -1, O4, 1, 1)
+        ts = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            ot.commutator(G4, -1, O4, 1, 1)
+            ts.append((time.perf_counter() - t0) * 1e3)
+        res["cpu_oracle_emax4_commutator_ms"] = round(float(np.median(ts)), 1)
+        res["gpu_over_cpu_emax4"] = round(res["cpu_oracle_emax4_commutator_ms"] / small["commutator_ms"], 1)
+    return res
 
 
 def setup_bench(emax: int):

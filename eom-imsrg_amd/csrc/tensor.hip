@@ -273,9 +273,37 @@ __device__ double ph_gather(const DevBasis& B, const DevTensor& T, const double*
     if (((B.ll[c] + B.ll[b]) & 1) != ((PAR + (T.dpar >> 1)) & 1)) continue;
     const double* W = pat < 8 ? Wx : Wy;
     const double phabcd = m1pow((ja + jb + jc + jd) / 2);
+    // the J3 / J4-independent part of the pattern's coefficient; the 9j is
+    // {u1 u2 J3; u4 u5 J4; J1 J2 rank} with (u1 u2) = (ja jd) or (jd ja)
+    // and (u4 u5) = (jb jc) or (jc jb)
+    double c0;
+    bool use34;
+    switch (pat) {
+      case 0: c0 = m1pow((jb + jd + J2) / 2) * Lh; use34 = true; break;
+      case 1: c0 = -m1pow((jc + ja + J1 + R) / 2) * Rh; use34 = false; break;
+      case 2: c0 = m1pow((jb + jc) / 2) * Lh; use34 = true; break;
+      case 3: c0 = m1pow((ja - jd + R) / 2) * Rh; use34 = false; break;
+      case 4: c0 = m1pow((ja + jd + J1 + J2) / 2) * Lh; use34 = true; break;
+      case 5: c0 = m1pow((jc - jb + J1 + J2 + R) / 2) * Rh; use34 = false; break;
+      case 6: c0 = m1pow((ja + jc + J1) / 2) * Lh; use34 = true; break;
+      case 7: c0 = m1pow((jb - jd + J2 + R) / 2) * Rh; use34 = false; break;
+      case 8: c0 = m1pow((jc + ja + J2) / 2) * Lh; use34 = false; break;
+      case 9: c0 = m1pow((jd - jb + J1 + R) / 2) * Rh; use34 = true; break;
+      case 10: c0 = -Lh; use34 = false; break;
+      case 11: c0 = phabcd * m1pow(R / 2) * Rh; use34 = true; break;
+      case 12: c0 = -phabcd * m1pow((J1 + J2) / 2) * Lh; use34 = false; break;
+      case 13: c0 = m1pow((J1 + J2 + R) / 2) * Rh; use34 = true; break;
+      case 14: c0 = m1pow((jb + jd + J1) / 2) * Lh; use34 = false; break;
+      default: c0 = m1pow((ja - jc + J2 + R) / 2) * Rh; use34 = true; break;
+    }
+    const bool t2 = pat & 1;                       // the (c d a b)-type target of a case
+    const bool sw12 = (pat < 8) ? t2 : !t2;
+    const int u1 = sw12 ? jd : ja, u2 = sw12 ? ja : jd;
+    const int u4 = t2 ? jc : jb, u5 = t2 ? jb : jc;
     for (int J3 = abs(jd - ja); J3 <= jd + ja; J3 += 2) {
       const int IX = B.rmap[((i64)d * B.N + a) * B.nJ + (J3 >> 1)];
-      const int J4lo = max(max(abs(J3 - R), J3), abs(jc - jb));
+      // Wy of a block with J3 = J4 is never formed (zero)
+      const int J4lo = max(max(max(abs(J3 - R), J3), abs(jc - jb)), pat < 8 ? J3 : J3 + 2);
       const int J4hi = min(min(J3 + R, jc + jb), 2 * B.JTM);
       for (int J4 = J4lo; J4 <= J4hi; J4 += 2) {
         const int qx = T.cclook[(J3 >> 1) * (R / 2 + 1) + ((J4 - J3) >> 1)] + 2 * Tz + PAR;
@@ -283,27 +311,8 @@ __device__ double ph_gather(const DevBasis& B, const DevTensor& T, const double*
         const double w = W[T.coff[qx * 3 + 2] + IX + (i64)JX * T.cblk[qx * 8 + 4]];
         if (!(fabs(w) > thr)) continue;
         const double p4 = sqrt((J3 + 1.0) * (J4 + 1.0)) * p12;
-        const double phi34 = m1pow((J3 + J4) / 2);
-        double v;
-        switch (pat) {
-          case 0: v = ninej(T, ja, jd, J3, jb, jc, J4, J1, J2) * m1pow((jb + jd + J2) / 2) * phi34 * Lh; break;
-          case 1: v = -ninej(T, jd, ja, J3, jc, jb, J4, J1, J2) * m1pow((jc + ja + J1 + R) / 2) * Rh; break;
-          case 2: v = ninej(T, ja, jd, J3, jb, jc, J4, J1, J2) * m1pow((jb + jc) / 2) * phi34 * Lh; break;
-          case 3: v = ninej(T, jd, ja, J3, jc, jb, J4, J1, J2) * m1pow((ja - jd + R) / 2) * Rh; break;
-          case 4: v = ninej(T, ja, jd, J3, jb, jc, J4, J1, J2) * m1pow((ja + jd + J1 + J2) / 2) * phi34 * Lh; break;
-          case 5: v = ninej(T, jd, ja, J3, jc, jb, J4, J1, J2) * m1pow((jc - jb + J1 + J2 + R) / 2) * Rh; break;
-          case 6: v = ninej(T, ja, jd, J3, jb, jc, J4, J1, J2) * m1pow((ja + jc + J1) / 2) * phi34 * Lh; break;
-          case 7: v = ninej(T, jd, ja, J3, jc, jb, J4, J1, J2) * m1pow((jb - jd + J2 + R) / 2) * Rh; break;
-          case 8: v = ninej(T, jd, ja, J3, jb, jc, J4, J1, J2) * m1pow((jc + ja + J2) / 2) * Lh; break;
-          case 9: v = ninej(T, ja, jd, J3, jc, jb, J4, J1, J2) * m1pow((jd - jb + J1 + R) / 2) * phi34 * Rh; break;
-          case 10: v = -ninej(T, jd, ja, J3, jb, jc, J4, J1, J2) * Lh; break;
-          case 11: v = ninej(T, ja, jd, J3, jc, jb, J4, J1, J2) * phabcd * phi34 * m1pow(R / 2) * Rh; break;
-          case 12: v = -ninej(T, jd, ja, J3, jb, jc, J4, J1, J2) * phabcd * m1pow((J1 + J2) / 2) * Lh; break;
-          case 13: v = ninej(T, ja, jd, J3, jc, jb, J4, J1, J2) * m1pow((J1 + J2 + R) / 2) * phi34 * Rh; break;
-          case 14: v = ninej(T, jd, ja, J3, jb, jc, J4, J1, J2) * m1pow((jb + jd + J1) / 2) * Lh; break;
-          default: v = ninej(T, ja, jd, J3, jc, jb, J4, J1, J2) * m1pow((ja - jc + J2 + R) / 2) * phi34 * Rh; break;
-        }
-        sm += p4 * v * w;
+        const double v = ninej(T, u1, u2, J3, u4, u5, J4, J1, J2) * (use34 ? (double)m1pow((J3 + J4) / 2) : 1.0);
+        sm += p4 * c0 * v * w;
       }
     }
   }

@@ -129,6 +129,8 @@ SIGNATURES = {
     "imsrg_tensor_destroy": (_i, [_p]),
     "imsrg_tensor_size": (C.c_int64, [_p, _i]),
     "imsrg_tensor_blocks": (_i, [_p, _ip, _ip, _lp]),
+    "imsrg_tensor_op_put": (_i, [_p, _dp, _i, _i, _d]),
+    "imsrg_tensor_op_get": (_i, [_p, _dp, _i, _i, _d]),
     "imsrg_ts_commutator_111": (_i, [_p, _dp, _i, _dp, _i, _dp, _i, _p]),
     "imsrg_ts_commutator_121": (_i, [_p, _dp, _i, _dp, _i, _dp, _i, _p]),
     "imsrg_ts_commutator_211": (_i, [_p, _dp, _i, _dp, _i, _dp, _i, _p]),

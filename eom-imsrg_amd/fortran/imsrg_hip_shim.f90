@@ -1,6 +1,7 @@
 ! Drop-in shim for n-parzuchowski/EOM-IMSRG: the reference's commutator,
 ! BCH / Magnus and flow routines with their own argument lists
 ! (commutators.f90:10-925, cross_coupled.f90:1190-1300,
+! TS_commutators.f90:10-1849, IMSRG_MAGNUS.f90:131-146,389-464,
 ! IMSRG_MAGNUS.f90:15-234,620-691,754-764, generators.f90:4-374,
 ! basic_IMSRG.f90:4056-4199), running in libimsrg_hip.so through the
 ! bind(C) module imsrg_hip_c.  Compiled with the reference (it uses
@@ -24,12 +25,21 @@ module imsrg_hip_shim
   public :: commutator_222_pp_hh_hip, commutator_221_hip, calculate_cross_coupled_hip, commutator_222_ph_hip
   public :: BCH_EXPAND_hip, MAGNUS_EXPAND_hip, euler_step_hip, magnus_decouple_hip
   public :: build_gs_white_hip, build_gs_atan_hip, build_gs_w2_hip, mbpt2_hip, mat_frob_norm_hip
+  public :: TS_commutator_111_hip, TS_commutator_121_hip, TS_commutator_211_hip, TS_commutator_122_hip
+  public :: TS_commutator_212_hip, TS_commutator_222_pp_hh_hip, TS_commutator_221_hip, TS_commutator_222_ph_hip
+  public :: BCH_TENSOR_hip, transform_sq_op_BCH_hip
 
   integer(c_int), parameter :: SZ_OP = 5, SZ_CC = 7, SZ_WCC = 8, SZ_NCC = 6
   type(c_ptr), save :: hb = c_null_ptr      ! imsrg_basis
   type(c_ptr), save :: hf = c_null_ptr      ! imsrg_flow
   type(c_ptr), save :: dop(6) = c_null_ptr  ! device operator mirrors: L, R, RES, w1, w2, spare
   type(c_ptr), save :: dcc(3) = c_null_ptr  ! device cc_mat mirrors: LCC, RCC, WCC
+  ! tensor operators: one library tensor type (rank, dpar) at a time
+  integer(c_int), parameter :: TSZ_OP = 0, TSZ_CC = 3
+  type(c_ptr), save :: ht = c_null_ptr      ! imsrg_tensor
+  integer, save :: ht_rank = -1, ht_dpar = -1
+  type(c_ptr), save :: dt(4) = c_null_ptr   ! device tensor mirrors: R, RES, w1, w2
+  type(c_ptr), save :: dtcc = c_null_ptr    ! device pandya_mat mirror (CCX and CCR of every block)
 
   interface
     function hipMemcpy(dst, src, bytes, kind) bind(C, name="hipMemcpy")
@@ -88,6 +98,7 @@ contains
     do k = 1, 3
       call hip_check(imsrg_free(dcc(k)), 'free')
     end do
+    call tensor_release()
     call hip_check(imsrg_flow_destroy(hf), 'flow_destroy')
     call hip_check(imsrg_basis_destroy(hb), 'basis_destroy')
     hb = c_null_ptr; hf = c_null_ptr
@@ -320,6 +331,190 @@ contains
     call hip_check(imsrg_mat_frob_norm(hb, dop(1), v, c_null_ptr), 'mat_frob_norm')
     mat_frob_norm_hip = v(1)
   end function mat_frob_norm_hip
+
+  ! ------------------------------------------------------------ tensors
+  subroutine tensor_release()
+    integer :: k
+    if (.not. c_associated(ht)) return
+    do k = 1, 4
+      call hip_check(imsrg_free(dt(k)), 'free')
+    end do
+    call hip_check(imsrg_free(dtcc), 'free')
+    call hip_check(imsrg_tensor_destroy(ht), 'tensor_destroy')
+    ht = c_null_ptr; ht_rank = -1; ht_dpar = -1
+  end subroutine tensor_release
+
+  ! the library's tensor layout of op's rank and parity (allocate_tensor)
+  subroutine tensor_ready(op)
+    type(sq_op), intent(in) :: op
+    integer :: k
+    if (c_associated(ht) .and. op%rank == ht_rank .and. op%dpar == ht_dpar) return
+    call tensor_release()
+    call hip_check(imsrg_tensor_create(hb, int(op%rank, c_int), int(op%dpar, c_int), ht), 'tensor_create')
+    ht_rank = op%rank; ht_dpar = op%dpar
+    do k = 1, 4
+      call hip_check(imsrg_alloc(imsrg_tensor_size(ht, TSZ_OP), dt(k)), 'alloc')
+    end do
+    call hip_check(imsrg_alloc(max(imsrg_tensor_size(ht, TSZ_CC), 1_c_int64_t), dtcc), 'alloc')
+  end subroutine tensor_ready
+
+  ! host tensor sq_op -> device mirror (E0, fhh, fph, fpp, tblck(q)%tgam(g)%X)
+  subroutine tensor_to_device(op, d)
+    type(sq_op), intent(in) :: op
+    type(c_ptr), intent(in) :: d
+    real(c_double) :: e0(1)
+    integer :: q, g
+    e0(1) = op%E0
+    call hip_check(imsrg_tensor_op_put(ht, d, -1_c_int, 0_c_int, e0), 'tensor_op_put')
+    call hip_check(imsrg_tensor_op_put(ht, d, -1_c_int, 1_c_int, op%fhh), 'tensor_op_put')
+    call hip_check(imsrg_tensor_op_put(ht, d, -1_c_int, 2_c_int, op%fph), 'tensor_op_put')
+    call hip_check(imsrg_tensor_op_put(ht, d, -1_c_int, 3_c_int, op%fpp), 'tensor_op_put')
+    do q = 1, op%nblocks
+      do g = 1, 9
+        if (size(op%tblck(q)%tgam(g)%X) > 0) call hip_check(imsrg_tensor_op_put(ht, d, int(q - 1, c_int), &
+            int(g - 1, c_int), op%tblck(q)%tgam(g)%X), 'tensor_op_put')
+      end do
+    end do
+  end subroutine tensor_to_device
+
+  subroutine tensor_from_device(d, op)
+    type(c_ptr), intent(in) :: d
+    type(sq_op), intent(inout) :: op
+    real(c_double) :: e0(1)
+    integer :: q, g
+    call hip_check(imsrg_tensor_op_get(ht, d, -1_c_int, 0_c_int, e0), 'tensor_op_get')
+    op%E0 = e0(1)
+    call hip_check(imsrg_tensor_op_get(ht, d, -1_c_int, 1_c_int, op%fhh), 'tensor_op_get')
+    call hip_check(imsrg_tensor_op_get(ht, d, -1_c_int, 2_c_int, op%fph), 'tensor_op_get')
+    call hip_check(imsrg_tensor_op_get(ht, d, -1_c_int, 3_c_int, op%fpp), 'tensor_op_get')
+    do q = 1, op%nblocks
+      do g = 1, 9
+        if (size(op%tblck(q)%tgam(g)%X) > 0) call hip_check(imsrg_tensor_op_get(ht, d, int(q - 1, c_int), &
+            int(g - 1, c_int), op%tblck(q)%tgam(g)%X), 'tensor_op_get')
+      end do
+    end do
+  end subroutine tensor_from_device
+
+  ! TS_commutators.f90: L scalar, R the tensor (the second argument), RES a tensor
+  subroutine TS_commutator_111_hip(L, R, RES, jbas)
+    type(sq_op) :: L, R, RES
+    type(spd) :: jbas
+    call tensor_ready(R)
+    call to_device(L, dop(1)); call tensor_to_device(R, dt(1)); call tensor_to_device(RES, dt(2))
+    call hip_check(imsrg_ts_commutator_111(ht, dop(1), int(L%herm, c_int), dt(1), int(R%herm, c_int), dt(2), &
+                                           int(RES%herm, c_int), c_null_ptr), 'ts_commutator_111')
+    call tensor_from_device(dt(2), RES)
+  end subroutine TS_commutator_111_hip
+
+  subroutine TS_commutator_121_hip(L, R, RES, jbas)
+    type(sq_op) :: L, R, RES
+    type(spd) :: jbas
+    call tensor_ready(R)
+    call to_device(L, dop(1)); call tensor_to_device(R, dt(1)); call tensor_to_device(RES, dt(2))
+    call hip_check(imsrg_ts_commutator_121(ht, dop(1), int(L%herm, c_int), dt(1), int(R%herm, c_int), dt(2), &
+                                           int(RES%herm, c_int), c_null_ptr), 'ts_commutator_121')
+    call tensor_from_device(dt(2), RES)
+  end subroutine TS_commutator_121_hip
+
+  subroutine TS_commutator_211_hip(LCC, R, RES, jbas)
+    type(cc_mat) :: LCC
+    type(sq_op) :: R, RES
+    type(spd) :: jbas
+    call tensor_ready(R)
+    call cc_copy(LCC, dcc(1), H2D)
+    call tensor_to_device(R, dt(1)); call tensor_to_device(RES, dt(2))
+    call hip_check(imsrg_ts_commutator_211(ht, dcc(1), int(LCC%herm, c_int), dt(1), int(R%herm, c_int), dt(2), &
+                                           int(RES%herm, c_int), c_null_ptr), 'ts_commutator_211')
+    call tensor_from_device(dt(2), RES)
+  end subroutine TS_commutator_211_hip
+
+  subroutine TS_commutator_122_hip(L, R, RES, jbas)
+    type(sq_op) :: L, R, RES
+    type(spd) :: jbas
+    call tensor_ready(R)
+    call to_device(L, dop(1)); call tensor_to_device(R, dt(1)); call tensor_to_device(RES, dt(2))
+    call hip_check(imsrg_ts_commutator_122(ht, dop(1), int(L%herm, c_int), dt(1), int(R%herm, c_int), dt(2), &
+                                           int(RES%herm, c_int), c_null_ptr), 'ts_commutator_122')
+    call tensor_from_device(dt(2), RES)
+  end subroutine TS_commutator_122_hip
+
+  subroutine TS_commutator_212_hip(L, R, RES, jbas)
+    type(sq_op) :: L, R, RES
+    type(spd) :: jbas
+    call tensor_ready(R)
+    call to_device(L, dop(1)); call tensor_to_device(R, dt(1)); call tensor_to_device(RES, dt(2))
+    call hip_check(imsrg_ts_commutator_212(ht, dop(1), int(L%herm, c_int), dt(1), int(R%herm, c_int), dt(2), &
+                                           int(RES%herm, c_int), c_null_ptr), 'ts_commutator_212')
+    call tensor_from_device(dt(2), RES)
+  end subroutine TS_commutator_212_hip
+
+  subroutine TS_commutator_222_pp_hh_hip(L, R, RES, w1, w2, jbas)
+    type(sq_op) :: L, R, RES, w1, w2
+    type(spd) :: jbas
+    call tensor_ready(R)
+    call to_device(L, dop(1)); call tensor_to_device(R, dt(1)); call tensor_to_device(RES, dt(2))
+    call hip_check(imsrg_ts_commutator_222_pp_hh(ht, dop(1), int(L%herm, c_int), dt(1), int(R%herm, c_int), dt(2), &
+                                                 int(RES%herm, c_int), dt(3), dt(4), c_null_ptr), 'ts_commutator_222_pp_hh')
+    call tensor_from_device(dt(2), RES)
+    call tensor_from_device(dt(3), w1)
+    call tensor_from_device(dt(4), w2)
+  end subroutine TS_commutator_222_pp_hh_hip
+
+  subroutine TS_commutator_221_hip(w1, w2, pm, RES, jbas)
+    type(sq_op) :: w1, w2, RES
+    integer, intent(in) :: pm
+    type(spd) :: jbas
+    call tensor_ready(RES)
+    call tensor_to_device(w1, dt(3)); call tensor_to_device(w2, dt(4)); call tensor_to_device(RES, dt(2))
+    call hip_check(imsrg_ts_commutator_221(ht, dt(3), dt(4), int(pm, c_int), dt(2), int(RES%herm, c_int), &
+                                           c_null_ptr), 'ts_commutator_221')
+    call tensor_from_device(dt(2), RES)
+  end subroutine TS_commutator_221_hip
+
+  ! RCC (the reference's pandya_mat workspace) is filled on the device only
+  subroutine TS_commutator_222_ph_hip(LCC, RCC, R, RES, jbas)
+    type(cc_mat) :: LCC
+    type(pandya_mat) :: RCC
+    type(sq_op) :: R, RES
+    type(spd) :: jbas
+    call tensor_ready(R)
+    call cc_copy(LCC, dcc(1), H2D)
+    call tensor_to_device(R, dt(1)); call tensor_to_device(RES, dt(2))
+    call hip_check(imsrg_ts_commutator_222_ph(ht, dcc(1), int(LCC%herm, c_int), dtcc, dt(1), int(R%herm, c_int), &
+                                              dt(2), int(RES%herm, c_int), c_null_ptr), 'ts_commutator_222_ph')
+    call tensor_from_device(dt(2), RES)
+  end subroutine TS_commutator_222_ph_hip
+
+  ! BCH_TENSOR, IMSRG_MAGNUS.f90:389-464: HS <- e^G HS e^-G for a tensor HS
+  subroutine BCH_TENSOR_hip(G, HS, jbas, quads)
+    type(sq_op) :: G, HS
+    type(spd) :: jbas
+    character(1) :: quads
+    real(c_double) :: advals(30)
+    integer(c_int) :: nt(1)
+    call tensor_ready(HS)
+    call to_device(G, dop(1)); call tensor_to_device(HS, dt(1))
+    call hip_check(imsrg_bch_tensor(ht, dop(1), int(G%herm, c_int), dt(1), int(HS%herm, c_int), &
+                                    dble(real(1e-6)), nt, advals, c_null_ptr), 'bch_tensor')
+    call tensor_from_device(dt(1), HS)
+  end subroutine BCH_TENSOR_hip
+
+  ! transform_sq_op_BCH, IMSRG_MAGNUS.f90:131-146
+  subroutine transform_sq_op_BCH_hip(Op, G, jbas, quads)
+    type(sq_op) :: Op, G
+    type(spd) :: jbas
+    character(1) :: quads
+    integer(c_int) :: nt(1)
+    if (Op%rank > 0) then
+      call BCH_TENSOR_hip(G, Op, jbas, quads)
+      return
+    end if
+    if (quads /= 'n') STOP 'imsrg_hip: BCH_EXPAND with quads /= n is not built'
+    call to_device(G, dop(1)); call to_device(Op, dop(2))
+    call hip_check(imsrg_transform_sq_op_bch(hf, dop(2), int(Op%herm, c_int), dop(1), dble(real(1e-6)), 40_c_int, &
+                                             nt, c_null_ptr), 'transform_sq_op_bch')
+    call from_device(dop(2), Op)
+  end subroutine transform_sq_op_BCH_hip
 
   ! ------------------------------------------------------------ the flow
   ! which of the library's generators build_generator is: applied once on

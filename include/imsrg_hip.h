@@ -449,6 +449,13 @@ int64_t imsrg_tensor_size(imsrg_tensor t, int what);
  * J2 (-1: empty); dims[6q..] = npp1, nph1, nhh1, npp2, nph2, nhh2;
  * goff[9q..] offset of tgam g */
 int imsrg_tensor_blocks(imsrg_tensor t, int* info, int* dims, int64_t* goff);
+/* copy one component of a host tensor sq_op to / from a device tensor
+ * operator (basic_IMSRG.f90:50-76, op%tblck(q)%tgam(g)%X): q = -1: g = 0
+ * E0, 1 fhh, 2 fph, 3 fpp; q >= 0: tgam(g+1) of block q (Fortran
+ * tblck(q+1)%tgam(g+1)).  Column-major, exactly the Fortran array.
+ * Synchronous. */
+int imsrg_tensor_op_put(imsrg_tensor t, double* op, int q, int g, const double* host);
+int imsrg_tensor_op_get(imsrg_tensor t, const double* op, int q, int g, double* host);
 /* TS_commutator_111, TS_commutators.f90:10-55: one-body part of [L1, R1],
  * L scalar, R the tensor (assigns RES one-body) */
 int imsrg_ts_commutator_111(imsrg_tensor t, const double* L, int Lherm, const double* R, int Rherm, double* RES,

@@ -201,3 +201,45 @@ def test_linearity_emax8():
         tensor.TS_commutator(G, tensor.TensorOp.from_numpy(tl, X, 1), RES)
         out.append(RES.numpy())
     _close(out[2], out[0] + 2.0 * out[1], "linearity", 1e-12)
+
+
+def _fixture(name):
+    import json
+    return json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", name)))
+
+
+def _check_sample(tl, b, X, fx, what):
+    n1 = b.off_fpp + b.npart * b.npart
+    _close(X[:n1], np.asarray(fx["one_body"]), what + " one-body")
+    _close(X[np.asarray(fx["idx"])], np.asarray(fx["val"]), what + " two-body sample")
+    n = tensor.mat_frob_norm(tensor.TensorOp.from_numpy(tl, X, 1))
+    assert abs(n - fx["norm"]) <= 1e-12 * fx["norm"], (what, n, fx["norm"])
+
+
+@pytest.mark.parametrize("rank,dpar", [(4, 0), (2, 2)])
+def test_commutator_emax6_fixture(rank, dpar):
+    """[L, R] at eMax 6 against the oracle's (tests/golden/make_tensor_fixtures.py):
+    one-body part, 4,096 seeded two-body elements, and the norm of the whole result"""
+    fx = _fixture("tensor_emax6.json")[f"rank{rank}_dpar{dpar}"]
+    b = Basis.oxygen16(6)
+    tl = tensor.TensorLayout(b, rank, dpar)
+    print(f"tensor commutator eMax 6 rank {rank} dpar {dpar} vs fixture", flush=True)
+    L = SqOp.from_numpy(b, synth.antihermitian(b, seed=11, sigma=0.3), -1)
+    R = tensor.TensorOp.from_numpy(tl, synth.tensor_random(tl, seed=12, herm=1), 1)
+    RES = tensor.TensorOp.zeros(tl, 1)
+    tensor.TS_commutator(L, R, RES)
+    torch.cuda.synchronize()
+    _check_sample(tl, b, RES.numpy(), fx, f"eMax 6 rank {rank}")
+
+
+def test_bch_tensor_emax4_fixture():
+    fx = _fixture("tensor_bch_emax4.json")
+    b = Basis.oxygen16(4)
+    tl = tensor.TensorLayout(b, 4, 0)
+    print("BCH_TENSOR eMax 4 vs fixture", flush=True)
+    G = SqOp.from_numpy(b, synth.antihermitian(b, seed=21, sigma=0.1), -1)
+    HS = tensor.TensorOp.from_numpy(tl, synth.tensor_random(tl, seed=22, herm=1), 1)
+    it, adv = tensor.BCH_TENSOR(G, HS)
+    assert it == fx["iterations"]
+    _close(adv[:it], np.asarray(fx["advals"]), "advals", 1e-9)
+    _check_sample(tl, b, HS.numpy(), fx, "BCH_TENSOR eMax 4")
