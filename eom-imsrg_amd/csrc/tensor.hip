@@ -36,6 +36,15 @@ __device__ __forceinline__ double ninej(const DevTensor& T, int a, int b, int J1
   return sm;
 }
 
+// the 9j {u1 u2 J3; u4 u5 J4; J1 J2 rank} sqrt((J3+1)(J4+1)) for J3 <= J4,
+// J1 <= J2 (host table)
+__device__ __forceinline__ double n9(const DevTensor& T, int u1, int u2, int J3, int u4, int u5, int J4, int J1,
+                                     int J2) {
+  const int nR = T.rank / 2 + 1;
+  return T.n9[(((((((i64)(J1 >> 1) * nR + ((J2 - J1) >> 1)) * T.nho + (u1 >> 1)) * T.nho + (u2 >> 1)) * T.nho +
+                   (u4 >> 1)) * T.nho + (u5 >> 1)) * T.nJ + (J3 >> 1)) * nR + ((J4 - J3) >> 1)];
+}
+
 __device__ __forceinline__ int tsub(int C1, int C2) {   // C = con + con + 1; tensor_elem's qx arithmetic
   int qx = C1 * C2;
   if (qx == 9) qx = 5;
@@ -242,7 +251,7 @@ __device__ double ph_gather(const DevBasis& B, const DevTensor& T, const double*
   if ((P == Q && ((J1 >> 1) & 1)) || (Rr == S && ((J2 >> 1) & 1))) return 0.0;
   const int R = T.rank;
   const double thr = (double)1e-6f;
-  const double p12 = sqrt((J1 + 1.0) * (J2 + 1.0));
+  const int nR = R / 2 + 1;
   double sm = 0.0;
   for (int pat = 0; pat < 16; ++pat) {
     int a, b, c, d;
@@ -300,22 +309,24 @@ __device__ double ph_gather(const DevBasis& B, const DevTensor& T, const double*
     const bool sw12 = (pat < 8) ? t2 : !t2;
     const int u1 = sw12 ? jd : ja, u2 = sw12 ? ja : jd;
     const int u4 = t2 ? jc : jb, u5 = t2 ? jb : jc;
+    double sp = 0.0;
     for (int J3 = abs(jd - ja); J3 <= jd + ja; J3 += 2) {
       const int IX = B.rmap[((i64)d * B.N + a) * B.nJ + (J3 >> 1)];
       // Wy of a block with J3 = J4 is never formed (zero)
       const int J4lo = max(max(max(abs(J3 - R), J3), abs(jc - jb)), pat < 8 ? J3 : J3 + 2);
       const int J4hi = min(min(J3 + R, jc + jb), 2 * B.JTM);
       for (int J4 = J4lo; J4 <= J4hi; J4 += 2) {
-        const int qx = T.cclook[(J3 >> 1) * (R / 2 + 1) + ((J4 - J3) >> 1)] + 2 * Tz + PAR;
+        const i64* wt = T.wtab + ((((J3 >> 1) * nR + ((J4 - J3) >> 1)) << 2) + 2 * Tz + PAR) * 2;
         const int JX = B.rmap[((i64)c * B.N + b) * B.nJ + (J4 >> 1)];
-        const double w = W[T.coff[qx * 3 + 2] + IX + (i64)JX * T.cblk[qx * 8 + 4]];
+        const double w = W[wt[0] + IX + JX * wt[1]];
         if (!(fabs(w) > thr)) continue;
-        const double p4 = sqrt((J3 + 1.0) * (J4 + 1.0)) * p12;
-        const double v = ninej(T, u1, u2, J3, u4, u5, J4, J1, J2) * (use34 ? (double)m1pow((J3 + J4) / 2) : 1.0);
-        sm += p4 * c0 * v * w;
+        const double v = n9(T, u1, u2, J3, u4, u5, J4, J1, J2) * w;
+        sp += (use34 && (((J3 + J4) >> 1) & 1)) ? -v : v;
       }
     }
+    sm += c0 * sp;
   }
+  sm *= sqrt((J1 + 1.0) * (J2 + 1.0));
   if (P == Q) sm *= 1.4142135623730951;
   if (Rr == S) sm *= 1.4142135623730951;
   return sm;

@@ -34,6 +34,12 @@ struct DevTensor {
   const int* cclook;               // [(2J3/2)*(rank/2+1) + (2J4-2J3)/2] -> qx of (J3, J4, Tz 0, PAR 0)
   const double* x6;                // {J1 J2 rank; j4 j5 j6}: [J1/2][J2/2][j4/2][j5/2][j6/2]
   const double* s6;                // {c d J2; b x J4}:       [c/2][d/2][b/2][x/2][J2/2][J4/2]
+  const i64* wtab;                 // [(((2J3/2)*(rank/2+1) + (2J4-2J3)/2)*4 + 2 Tz + PAR)*2]: offset of the
+                                   // product W of that cross-coupled block, its leading dimension
+  const double* n9;                // 9j {u1 u2 J3; u4 u5 J4; J1 J2 rank} * sqrt((J3+1)(J4+1)) of the
+                                   // 222_ph back-transform, J3 <= J4 <= J3 + rank, J1 <= J2 <= J1 + rank:
+                                   // [J1/2][(J2-J1)/2][u1/2][u2/2][u4/2][u5/2][J3/2][(J4-J3)/2]
+                                   // (one output block's (J1, J2) owns a contiguous slice)
   int nJ, nhx, nho;                // 2J/2 values; half-integers up to jmax + rank; up to jmax
 };
 
