@@ -90,7 +90,10 @@ struct imsrg_basis_s {
   Basis b;
   int path = IMSRG_PATH_GROUPED;
   bool dev_ready = false;
-  int tile_req[4] = {0, 0, 0, 0};   // GEMM tile shapes for pp1, k2, k3, ph (64x64, measured best)
+  // GEMM tile shapes for pp1, k2, k3, ph (64x64 until tuned) and the reverse-mode
+  // 222_pp_hh rounds and W' transpose (64x32 through LDS: the reverse sweep at
+  // eMax 8 takes 0.47 s with it, 0.50-0.51 s on the register-direct shapes)
+  int tile_req[6] = {0, 0, 0, 0, 3, 3};
   int plan_ver = 0;                 // bumped whenever a task list is rebuilt (captured graphs go stale)
   DevBasis D{};
   std::vector<void*> allocs;
@@ -144,7 +147,7 @@ struct imsrg_basis_s {
   // rounds of distinct outputs, the W' transpose products, quadruple
   // starts, the compact one-body map of 122
   bool adj_ready = false;
-  GemmFamily pb[6], phb;
+  GemmFamily pb[6], phb;   // reverse-mode 222_pp_hh rounds, the W' transpose
   std::vector<int> f1map_h, f1off_h;   // 122: one-body offset -> compact index, and back
   int* f1off = nullptr;
   int nf1 = 0;
@@ -1094,18 +1097,16 @@ int ensure_adj(imsrg_basis_s* B) {
     std::vector<int> fillp(B->pht0_h.begin(), B->pht0_h.end() - 1);
     for (size_t e = 0; e < term.size(); ++e) B->phterm_h[fillp[tid[e]]++] = term[e];
   }
-  // tile shapes of the transposed products (tile codes of imsrg_set_gemm_tile;
-  // 64x32 measured fastest for the reverse sweep at eMax 8: 0.494 -> 0.474 s)
-#ifndef IMSRG_ADJ_TILE
-#define IMSRG_ADJ_TILE 3
-#endif
-#ifndef IMSRG_ADJ_PH_TILE
-#define IMSRG_ADJ_PH_TILE 3
-#endif
-  for (GemmFamily* f : {&B->pb[0], &B->pb[1], &B->pb[2], &B->pb[3], &B->pb[4], &B->pb[5], &B->phb}) {
-    if ((rc = upload(B, f->probs_h, &f->probs))) return rc;
-    if ((rc = family_tasks(B, *f, f == &B->phb ? IMSRG_ADJ_PH_TILE : IMSRG_ADJ_TILE))) return rc;
+  // tile shapes of the transposed products (imsrg_set_gemm_tile families
+  // IMSRG_GEMM_ADJ / IMSRG_GEMM_ADJ_PH)
+  for (GemmFamily& f : B->pb) {
+    if ((rc = upload(B, f.probs_h, &f.probs))) return rc;
+    f.xcd_aware = (B->tile_req[4] & IMSRG_TILE_XCD) != 0;
+    if ((rc = family_tasks(B, f, B->tile_req[4] & 7))) return rc;
   }
+  if ((rc = upload(B, B->phb.probs_h, &B->phb.probs))) return rc;
+  B->phb.xcd_aware = (B->tile_req[5] & IMSRG_TILE_XCD) != 0;
+  if ((rc = family_tasks(B, B->phb, B->tile_req[5] & 7))) return rc;
   if ((rc = upload(B, B->phq_h, &B->phq)) || (rc = upload(B, B->phqs_h, &B->phqs)) ||
       (rc = upload(B, B->phl_h, &B->phl)) || (rc = upload(B, B->phtile_h, &B->phtile)) ||
       (rc = upload(B, B->pht0_h, &B->pht0)) || (rc = upload(B, B->phterm_h, &B->phterm)))
@@ -1876,13 +1877,16 @@ int flow_forward(imsrg_flow_s* F, const double* T, double ds, double bch_conv, d
   const i64 n = B->b.oplen;
   const size_t nb = (size_t)n * sizeof(double);
   int rc, nt = 0, nc = 0;
-  if (ckpt < 1) return fail(IMSRG_EINVAL, "checkpoint interval must be >= 1");
+  if (ckpt < 0 || maxsteps < 1) return fail(IMSRG_EINVAL, "checkpoint interval must be >= 0");
+  // ckpt 0: one segment holding every step's states and terms (the reverse
+  // sweep recomputes nothing; the buffers grow with the steps)
+  const bool keep_all = ckpt == 0;
+  if (keep_all) ckpt = maxsteps;
   if ((rc = ensure_adj(B))) return rc;
   for (double** p : {&F->gH, &F->gHS, &F->gG, &F->gDG, &F->gHSb, &F->gGb, &F->gHb, &F->gdOb, &F->getab})
     if ((rc = fbuf(F, p, n))) return rc;
-  if ((rc = grow(F, F->segG, (size_t)ckpt)) || (rc = grow(F, F->segH, (size_t)ckpt)) ||
-      (rc = grow(F, F->segE, (size_t)ckpt)))
-    return rc;
+  const size_t pre = keep_all ? 1 : (size_t)ckpt;
+  if ((rc = grow(F, F->segG, pre)) || (rc = grow(F, F->segH, pre)) || (rc = grow(F, F->segE, pre))) return rc;
   F->fw_ok = false;
   F->last_n0 = -1;
   F->lmrec.assign(ckpt, TermRec());
@@ -1911,6 +1915,9 @@ int flow_forward(imsrg_flow_s* F, const double* T, double ds, double bch_conv, d
       HIPCHK(hipMemcpyAsync(F->hckpts[c], F->gHS, nb, hipMemcpyDeviceToDevice, s));
       pidx = 0;
     }
+    if (keep_all && ((rc = grow(F, F->segG, (size_t)k + 1)) || (rc = grow(F, F->segH, (size_t)k)) ||
+                     (rc = grow(F, F->segE, (size_t)k))))
+      return rc;
     HIPCHK(hipMemcpyAsync(F->segG[k], F->gG, nb, hipMemcpyDeviceToDevice, s));
     HIPCHK(hipMemcpyAsync(F->segH[k], F->gHS, nb, hipMemcpyDeviceToDevice, s));
     HIPCHK(hipMemcpyAsync(F->segE[k], F->gDG, nb, hipMemcpyDeviceToDevice, s));
@@ -3059,8 +3066,22 @@ int imsrg_flow_grad(imsrg_flow F, const double* T, double ds, double bch_conv, d
 
 int imsrg_set_gemm_tile(imsrg_basis B, int family, int tile) {
   const int shape = tile & 7;
-  if (!B || family < 0 || family > 3 || shape > 5 || tile < 0 || tile > (IMSRG_TILE_XCD | 5))
+  if (!B || family < 0 || family > 5 || shape > 7 || tile < 0 || tile > (IMSRG_TILE_XCD | 7))
     return fail(IMSRG_EINVAL, "bad gemm family / tile");
+  if (family >= 4) {   // the reverse-mode families: every round / side
+    B->tile_req[family] = tile;
+    if (!B->adj_ready) return 0;   // applied when the reverse-mode plan is built
+    int rc = 0;
+    auto set = [&](GemmFamily& g) {
+      g.xcd_aware = (tile & IMSRG_TILE_XCD) != 0;
+      if (!rc) rc = family_tasks(B, g, shape);
+    };
+    if (family == 4)
+      for (GemmFamily& g : B->pb) set(g);
+    else
+      set(B->phb);
+    return rc;
+  }
   GemmFamily* f[4] = {&B->pp1, &B->k2, &B->k3, &B->ph};
   f[family]->xcd_aware = (tile & IMSRG_TILE_XCD) != 0;
   B->tile_req[family] = tile;
@@ -3095,7 +3116,7 @@ int imsrg_commutator_222_ph_noalloc(imsrg_basis B, const double* LCC, int Lh, co
 }
 
 int imsrg_get_gemm_tile(imsrg_basis B, int family) {
-  if (!B || family < 0 || family > 3) return -1;
+  if (!B || family < 0 || family > 5) return -1;
   return B->tile_req[family];
 }
 

@@ -5,7 +5,8 @@ dE0/d(input matrix elements) for the flow of magnus_decouple
 (IMSRG_MAGNUS.f90:15-128, white generator).  ``flow_energy`` exposes it to
 PyTorch: a ``torch.autograd.Function`` whose forward runs the flow from the
 vacuum operator T (normal ordered on the device) and whose backward runs the
-reverse sweep with Omega checkpointed every ``ckpt`` steps.
+reverse sweep with Omega checkpointed every ``ckpt`` steps (``ckpt=0``: every
+step kept in device memory, no recomputation).
 
 Gradients are with respect to the flat operator arrays; they are exact for
 perturbations that keep herm symmetry and the zero Pauli-forbidden (a,a)
@@ -75,7 +76,8 @@ def flow_grad(flow: Flow, T: torch.Tensor, ds: float = 0.5, bch_conv: float = RE
 def flow_forward(flow: Flow, T: torch.Tensor, ds: float = 0.5, bch_conv: float = REF_TOL,
                  mag_conv: float = REF_TOL, crit_tol: float = REF_TOL, maxsteps: int = 200, ckpt: int = 4):
     """run the flow from normal_order(T) keeping Omega every `ckpt` steps in
-    `flow`; returns (E0, steps)"""
+    `flow` (ckpt 0: every step's states and terms, so that flow_backward
+    recomputes nothing); returns (E0, steps)"""
     e0 = C.c_double()
     st = C.c_int()
     _lib.check(_lib.lib.imsrg_flow_forward(flow.handle, T.data_ptr(), float(ds), float(bch_conv), float(mag_conv),

@@ -30,7 +30,7 @@ from .basis import Basis
 FAMILIES = {"PPPP": ("PPPP", "COMMUTATOR"), "PPX": ("PPX", "COMMUTATOR"), "XX": ("XX", "COMMUTATOR"),
             "PH": ("PH", "COMMUTATOR")}
 XCD = 8
-CANDIDATES = [t | x for x in (0, XCD) for t in range(6)]
+CANDIDATES = [t | x for x in (0, XCD) for t in range(8)]
 SHIPPED = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuned_mi355x.json")
 
 

@@ -369,7 +369,9 @@ int imsrg_normal_order_vjp(imsrg_basis b, const double* Hb, double* Tb, imsrg_st
  * imsrg_flow_forward runs the flow, returns E0 and the step count and keeps
  * Omega every `ckpt` steps in the flow object; imsrg_flow_backward then
  * writes dE0/dT into Tb (overwritten), recomputing the steps between
- * checkpoints.  imsrg_flow_grad does both. */
+ * checkpoints.  ckpt = 0 keeps every step's states and nested-commutator
+ * terms instead (device memory of about (3 steps + commutators) operators,
+ * nothing recomputed).  imsrg_flow_grad does both. */
 int imsrg_flow_forward(imsrg_flow f, const double* T, double ds, double bch_conv, double mag_conv, double crit_tol,
                        int maxsteps, int ckpt, double* E0, int* steps, imsrg_stream stream);
 int imsrg_flow_backward(imsrg_flow f, double* Tb, imsrg_stream stream);
@@ -379,13 +381,17 @@ int imsrg_flow_grad(imsrg_flow f, const double* T, double ds, double bch_conv, d
 /* ------------------------------------------------ tuning */
 /* No counterpart in the reference: selects the MFMA tile shape of one GEMM
  * family (IMSRG_GEMM_*): tile 0 = 64x64, 1 = 128x64, 2 = 128x128, 3 = 64x32,
- * 4 = 32x64, 5 = 32x32, plus IMSRG_TILE_XCD to pack each channel's tiles onto
+ * 4 = 32x64, 5 = 32x32 (operands staged through LDS), 6 = 128x64, 7 =
+ * 64x64 (register-direct: each wave loads its own 64x32 / 32x32 block's
+ * operands, no LDS), plus IMSRG_TILE_XCD to pack each channel's tiles onto
  * one XCD (one L2). */
 #define IMSRG_TILE_XCD 8
 #define IMSRG_GEMM_PPPP 0   /* 222_pp_hh L1 R1 */
 #define IMSRG_GEMM_PPX 1    /* 222_pp_hh pp x (ph|hh) */
 #define IMSRG_GEMM_XX 2     /* 222_pp_hh (ph|hh)^T (ph|hh) */
 #define IMSRG_GEMM_PH 3     /* 222_ph W' = Lh LCC RCC^T - Rh RCC LCC^T */
+#define IMSRG_GEMM_ADJ 4    /* reverse mode: the transposed 222_pp_hh products */
+#define IMSRG_GEMM_ADJ_PH 5 /* reverse mode: the transpose of the W' product */
 int imsrg_set_gemm_tile(imsrg_basis b, int family, int tile);
 int imsrg_get_gemm_tile(imsrg_basis b, int family);
 /* No counterpart in the reference: on the grouped path, run the full

@@ -64,6 +64,40 @@ def test_commutator_vjp(gpu, emax, hL, hR):
         assert abs(rb @ dR - r2) <= 1e-10 * (abs(r2) + np.abs(x).sum() * 1e-3), (rb @ dR, r2)
 
 
+@pytest.mark.parametrize("tile", [0, 5, 6, 7, 6 | 8])
+def test_commutator_vjp_tile_shapes(gpu, tile):
+    """the transposed 222_pp_hh products and the W' transpose on every tile
+    shape (LDS-staged and register-direct, one kernel per transpose pair):
+    equal to the oracle's transposes at eMax 4 and to the default tile
+    shape's result at eMax 6 (blocks larger than the tiles)"""
+    from eom_imsrg import SqOp, _lib, adjoint
+    hL, hR, hres = -1, 1, 1
+    for emax in (4, 6):
+        b = basis(emax)
+        print(f"[commutator_vjp tiles] emax={emax} tile={tile}", flush=True)
+        L, R = _rand(b, hL, 3), _rand(b, hR, 4)
+        x = np.random.default_rng(5).normal(size=b.oplen)
+        Ld, Rd = SqOp.from_numpy(b, L, hL), SqOp.from_numpy(b, R, hR)
+        out = []
+        for t in (3, tile):
+            for fam in ("ADJ", "ADJ_PH"):
+                _lib.check(_lib.lib.imsrg_set_gemm_tile(b.handle, _lib.GEMM[fam], t))
+            Lb = torch.zeros(b.oplen, dtype=torch.float64, device="cuda")
+            Rb = torch.zeros_like(Lb)
+            adjoint.commutator_vjp(Ld, Rd, _t(x), hres, Lb, Rb)
+            torch.cuda.synchronize()
+            out.append((Lb.cpu().numpy(), Rb.cpu().numpy()))
+        for fam in ("ADJ", "ADJ_PH"):
+            _lib.check(_lib.lib.imsrg_set_gemm_tile(b.handle, _lib.GEMM[fam], 3))
+        (l0, r0), (l1, r1) = out
+        assert_rel(_sym(b, l1, hL), _sym(b, l0, hL), rtol=1e-11, what=f"Lbar tile {tile} vs 3")
+        assert_rel(_sym(b, r1, hR), _sym(b, r0, hR), rtol=1e-11, what=f"Rbar tile {tile} vs 3")
+        if emax == 4:
+            Lb_o, Rb_o = oracle_basis(emax).commutator_vjp(L, hL, R, hR, x, hres)
+            assert_rel(_sym(b, l1, hL), _sym(b, Lb_o, hL), rtol=1e-9, what="Lbar vs oracle")
+            assert_rel(_sym(b, r1, hR), _sym(b, Rb_o, hR), rtol=1e-9, what="Rbar vs oracle")
+
+
 def test_bch_magnus_white_normal_order_vjp(gpu):
     from eom_imsrg import Flow, SqOp, adjoint, synth
     emax = 4
@@ -111,14 +145,19 @@ def test_bch_magnus_white_normal_order_vjp(gpu):
 
 @pytest.mark.parametrize("emax", [2, 4])
 def test_flow_grad_vs_oracle(gpu, emax):
+    """the GPU reverse sweep with Omega every 4 steps (segments recomputed),
+    every 3 steps, and every step's states kept (ckpt 0, nothing recomputed)
+    against the oracle's reverse sweep"""
     from eom_imsrg import Flow, adjoint, synth
     b, o = basis(emax), oracle_basis(emax)
-    print(f"[flow_grad] emax={emax}", flush=True)
     T = synth.hamiltonian_vacuum(b, 0)
     e0_o, Tb_o, st_o = o.flow_grad(T, 0.5, REF, REF, REF, ckpt=4)
-    e0, Tb, st = adjoint.flow_grad(Flow(b, graphs=True), _t(T), ckpt=4)
-    assert st == st_o and abs(e0 - e0_o) < 1e-8
-    assert_rel(_sym(b, Tb.cpu().numpy(), 1), _sym(b, Tb_o, 1), rtol=1e-8, what="dE0/dT")
+    f = Flow(b, graphs=True)
+    for ck in (4, 3, 0):
+        print(f"[flow_grad] emax={emax} ckpt={ck}", flush=True)
+        e0, Tb, st = adjoint.flow_grad(f, _t(T), ckpt=ck)
+        assert st == st_o and abs(e0 - e0_o) < 1e-8
+        assert_rel(_sym(b, Tb.cpu().numpy(), 1), _sym(b, Tb_o, 1), rtol=1e-8, what=f"dE0/dT (ckpt {ck})")
 
 
 def test_flow_energy_autograd_and_central_differences(gpu):

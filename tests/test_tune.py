@@ -1,7 +1,9 @@
 """GEMM tile autotuner: the stored choice is keyed by device and basis,
 read back from the cache file (or the shipped MI355X table), and applied
 through imsrg_set_gemm_tile; on the GPU the measured choice must leave the
-commutator unchanged (tile shape and XCD packing reorder tiles, not sums)."""
+commutator unchanged to rounding (XCD packing and the LDS-staged shapes
+reorder tiles, not sums; the register-direct shapes also regroup each
+K sum, which moves the last bits)."""
 import json
 
 import numpy as np
@@ -59,16 +61,17 @@ def test_measured_choice_keeps_results(gpu, tmp_path, monkeypatch):
     got = SqOp.zeros(b, 1)
     cm.commutator(L, R, got)
     torch.cuda.synchronize()
-    assert_rel(got.numpy(), ref.numpy(), rtol=1e-14, what="commutator after tuning")
+    assert_rel(got.numpy(), ref.numpy(), rtol=1e-12, what="commutator after tuning")
     tune.apply(b, before)
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("tile", [0, 1, 2, 3, 4, 5])
+@pytest.mark.parametrize("tile", [0, 1, 2, 3, 4, 5, 6, 7])
 def test_every_tile_shape_matches_oracle(gpu, tile):
     """every GEMM family on one tile shape (64x64, 128x64, 128x128, 64x32,
-    32x64, 32x32, with XCD packing on odd shapes): the commutator equals the
-    oracle's, including eMax 6 blocks larger than the tiles"""
+    32x64, 32x32 through LDS, 128x64 and 64x64 register-direct, with XCD
+    packing on odd shapes): the commutator equals the oracle's, including
+    eMax 6 blocks larger than the tiles"""
     from eom_imsrg import SqOp, commutators as cm, synth, tune
     from conftest import oracle_basis
     b, o = basis(6), oracle_basis(6)
