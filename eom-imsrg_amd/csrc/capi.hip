@@ -113,6 +113,11 @@ struct imsrg_basis_s {
   std::vector<Partner> part_h;      // 122 partner lists (CSR by global pair id)
   std::vector<int> rowp_h, colp_h;
   Partner* partners = nullptr;
+  // the row lists again, entry k of pair gid at k * npairs + gid: the
+  // neighbouring rows that one wave's lanes hold read neighbouring records
+  std::vector<Partner> partT_h;
+  Partner* partT = nullptr;
+  int pstride = 0;
   int *rowp = nullptr, *colp = nullptr;
   std::vector<QuadItem> quad_h;   // 222_ph outputs, quadruple-major (grouped path)
   QuadItem* quad = nullptr;
@@ -463,6 +468,8 @@ int build_plan(imsrg_basis_s* B) {
   // then (b -> i, f(i,b)); i runs over the (l,j,tz) block of the replaced
   // orbit.  Phase / sqrt(2) of the resulting pair as v_elem applies them.
   {
+    // the partner terms address operator elements with 32-bit offsets
+    if (b.oplen >= ((i64)1 << 31)) return fail(IMSRG_EINVAL, "operator too large for 32-bit element offsets");
     B->part_h.clear();
     const size_t npairs = b.pairs.size();
     B->rowp_h.assign(npairs + 1, 0);
@@ -527,6 +534,13 @@ int build_plan(imsrg_basis_s* B) {
     }
     B->nf1 = (int)B->f1off_h.size();
     if (B->nf1 > kF1Max) return fail(IMSRG_EINVAL, "one-body partner set exceeds the LDS table of 122");
+    int lmax = 0;
+    for (size_t gi = 0; gi < npairs; ++gi) lmax = std::max(lmax, B->rowp_h[gi + 1] - B->rowp_h[gi]);
+    B->pstride = (int)npairs;
+    B->partT_h.assign((size_t)std::max(lmax, 1) * npairs, Partner{0, 0});
+    for (size_t gi = 0; gi < npairs; ++gi)
+      for (int k = 0; k < B->rowp_h[gi + 1] - B->rowp_h[gi]; ++k)
+        B->partT_h[(size_t)k * npairs + gi] = B->part_h[B->rowp_h[gi] + k];
   }
   // 222_ph outputs as (orbit quadruple, 2J) items, J fastest: every upper
   // element of every channel exactly once (checked against seg_tri)
@@ -671,6 +685,7 @@ int build_plan(imsrg_basis_s* B) {
       (rc = upload(B, B->fql_h, &B->fql)) || (rc = upload(B, B->fqi_h, &B->fqi)))
     return rc;
   if ((rc = upload(B, B->part_h, &B->partners))) return rc;
+  if ((rc = upload(B, B->partT_h, &B->partT))) return rc;
   if ((rc = upload(B, B->f1off_h, &B->f1off))) return rc;
   if ((rc = upload(B, B->rowp_h, &B->rowp))) return rc;
   if ((rc = upload(B, B->colp_h, &B->colp))) return rc;
@@ -801,7 +816,8 @@ void t_122(const Exec& X, const double* L, int Lh, const double* R, int Rh, doub
            hipStream_t s, bool mirror = true) {
   imsrg_basis_s* B = X.B;
   run(B, B->seg_tri, [&](const SegTask* t, int n) {
-    launch_comm122p(X.D, t, n, B->partners, B->rowp, B->colp, B->f1off, B->nf1, L, Lh, R, Rh, RES, RESh, s,
+    launch_comm122p(X.D, t, n, B->partners, B->partT, B->pstride, B->rowp, B->colp, B->f1off, B->nf1, L, Lh, R, Rh,
+                    RES, RESh, s,
                     mirror ? 1 : 0);
   });
 }
@@ -1186,7 +1202,8 @@ void comm_vjp(imsrg_basis_s* B, const double* L, int Lh, const double* R, int Rh
   // 122: two-body parts from the symmetrised cotangent (densified on
   // s122), one-body part by the partner sums
   if (fork) (void)hipStreamWaitEvent(s, B->tev[2], 0);
-  launch_122_vjp(D, B->seg_tri.dev, B->seg_tri.n, B->partners, B->rowp, B->colp, B->f1off, B->nf1, L, Lh, R, Rh,
+  launch_122_vjp(D, B->seg_tri.dev, B->seg_tri.n, B->partners, B->partT, B->pstride, B->rowp, B->colp, B->f1off,
+                 B->nf1, L, Lh, R, Rh,
                  B->cbar, RESb, RESh, Lb, Rb, s);
   // 222_ph: the Pandya transposes of the cross-coupled cotangents (the
   // scatter transpose into the W' cotangent U and the triangular-U GEMMs ran

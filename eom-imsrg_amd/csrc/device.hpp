@@ -123,21 +123,18 @@ __device__ __forceinline__ int blk_of(int lo, int hi) { return lo == 0 ? hi : (l
 // (1 in a stored block, Lh / Rh in the transposed one).  Base and stride
 // depend on the thread's IX / JX only, so they are formed once per thread
 // and each term costs its partner record and the two operand loads.
+// Offsets are 32-bit (an operator holds < 2^31 doubles, checked when the
+// plan is built).
 struct PartnerAddr {
-  i64 b0, b1, b2;   // base per class
+  int b0, b1, b2;   // base per class
   int s0, s1, s2;   // stride per class
   int t0, t1, t2;   // 1: element in the transposed stored block
 };
-// per-class value by arithmetic masks (no selects over member addresses,
-// which would make the compiler keep the struct in private memory)
-__device__ __forceinline__ i64 pick3(int cl, i64 v0, i64 v1, i64 v2) {
-  const i64 m0 = -(i64)(cl == 0), m1 = -(i64)(cl == 1), m2 = -(i64)(cl == 2);
-  return (v0 & m0) | (v1 & m1) | (v2 & m2);
-}
+__device__ __forceinline__ int pick3(int cl, int v0, int v1, int v2) { return cl == 0 ? v0 : (cl == 1 ? v1 : v2); }
 __device__ __forceinline__ i64 pa_off(const PartnerAddr A, int cl, int idx) {
-  return pick3(cl, A.b0, A.b1, A.b2) + (i64)idx * pick3(cl, A.s0, A.s1, A.s2);
+  return (i64)(unsigned)(pick3(cl, A.b0, A.b1, A.b2) + idx * pick3(cl, A.s0, A.s1, A.s2));
 }
-__device__ __forceinline__ bool pa_trans(const PartnerAddr A, int cl) { return pick3(cl, A.t0, A.t1, A.t2) != 0; }
+__device__ __forceinline__ int pa_trans(const PartnerAddr A, int cl) { return pick3(cl, A.t0, A.t1, A.t2); }
 
 // the row- and column-partner addressing of output (IX, JX) of the block of
 // classes (rc, cc), from the channel's block offsets go[6] and class
@@ -146,11 +143,11 @@ __device__ __forceinline__ void partner_addr(const i64* go, const int* dm, int r
                                              PartnerAddr& AR, PartnerAddr& AC) {
   // row partner at class cl: column JX of class cc; column partner at class
   // cl: row IX of class rc
-#define IMSRG_PA(cl, bR, sR, tR, bC, sC, tC)                                              \
-  if (cl <= cc) { bR = go[blk_of(cl, cc)] + (i64)JX * dm[cl]; sR = 1; tR = 0; }           \
-  else { bR = go[blk_of(cc, cl)] + JX; sR = dm[cc]; tR = 1; }                             \
-  if (rc <= cl) { bC = go[blk_of(rc, cl)] + IX; sC = dm[rc]; tC = 0; }                    \
-  else { bC = go[blk_of(cl, rc)] + (i64)IX * dm[cl]; sC = 1; tC = 1; }
+#define IMSRG_PA(cl, bR, sR, tR, bC, sC, tC)                                                  \
+  if (cl <= cc) { bR = (int)(go[blk_of(cl, cc)] + (i64)JX * dm[cl]); sR = 1; tR = 0; }       \
+  else { bR = (int)(go[blk_of(cc, cl)] + JX); sR = dm[cc]; tR = 1; }                          \
+  if (rc <= cl) { bC = (int)(go[blk_of(rc, cl)] + IX); sC = dm[rc]; tC = 0; }                 \
+  else { bC = (int)(go[blk_of(cl, rc)] + (i64)IX * dm[cl]); sC = 1; tC = 1; }
   IMSRG_PA(0, AR.b0, AR.s0, AR.t0, AC.b0, AC.s0, AC.t0)
   IMSRG_PA(1, AR.b1, AR.s1, AR.t1, AC.b1, AC.s1, AC.t1)
   IMSRG_PA(2, AR.b2, AR.s2, AR.t2, AC.b2, AC.s2, AC.t2)

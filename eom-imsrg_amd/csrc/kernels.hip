@@ -24,37 +24,43 @@ namespace imsrg {
 
 // one partner term: f (from LDS) times the operand elements r = R[o],
 // l = L[o], accumulated as sm + fl*vr - fr*vl (row partners) or
-// sm - fl*vr + fr*vl (column partners)
+// sm - fl*vr + fr*vl (column partners), vr = r * pre * (Rh if the element
+// is read from the transposed block), vl = l * pre * (Lh likewise), fl / fr
+// times Lh / Rh when the record's one-body element is read transposed.
+// Those herm factors and the record's sign are +-1: they combine into one
+// sign bit per product, applied to pre = pre_other * (1 or sqrt 2)
+// (pa1 / pa2).  lneg / rneg: Lh / Rh < 0.
 template <bool ROW>
 __device__ __forceinline__ double term122(double sm, const Partner p, double r, double l, const PartnerAddr A,
-                                          double pre_other, const double* s_f, int nf1, int Lh, int Rh) {
-  const int cl = (p.code >> 22) & 3;
-  double pre = ((p.code >> 26) & 1) ? 1.4142135623730951 : 1.0;
-  if ((p.code >> 25) & 1) pre = -pre;
-  pre = ROW ? pre * pre_other : pre_other * pre;
-  double vr, vl;
-  if (pa_trans(A, cl)) { vr = r * Rh * pre; vl = l * Lh * pre; }
-  else { vr = r * pre; vl = l * pre; }
-  double fl = s_f[p.fk], fr = s_f[nf1 + p.fk];
-  if ((p.code >> 24) & 1) { fl = fl * Lh; fr = fr * Rh; }
-  return ROW ? sm + fl * vr - fr * vl : sm - fl * vr + fr * vl;
+                                          double pa1, double pa2, const double* s_f, int nf1, int lneg, int rneg) {
+  const int code = p.code;
+  const int tr = pa_trans(A, (code >> 22) & 3);
+  const int fs = (code >> 24) & 1, neg = (code >> 25) & 1;
+  const double a = ((code >> 26) & 1) ? pa2 : pa1;
+  const int sr = neg ^ (fs & lneg) ^ (tr & rneg);   // sign of the R-operand product
+  const int sl = neg ^ (fs & rneg) ^ (tr & lneg);   // sign of the L-operand product
+  const double cr = (sr ^ (ROW ? 0 : 1)) ? -a : a, cl = (sl ^ (ROW ? 1 : 0)) ? -a : a;
+  sm = fma(s_f[p.fk] * cr, r, sm);
+  return fma(s_f[nf1 + p.fk] * cl, l, sm);
 }
 
-// the partner terms [x, xe) of one list, IMSRG_122_UNROLL records and their
-// operand loads in flight at a time; the terms are added in list order
+// the n partner terms of one list, record k at part[k * ps], IMSRG_122_UNROLL
+// records and their operand loads in flight at a time; the terms are added
+// in list order
 #ifndef IMSRG_122_UNROLL
 #define IMSRG_122_UNROLL 2
 #endif
 template <bool ROW>
-__device__ __forceinline__ double sum122(double sm, const Partner* part, int x, int xe, const PartnerAddr A,
-                                         double pre_other, const double* s_f, int nf1, const double* L, int Lh,
-                                         const double* R, int Rh) {
+__device__ __forceinline__ double sum122(double sm, const Partner* part, i64 ps, int n, const PartnerAddr A,
+                                         double pa1, double pa2, const double* s_f, int nf1, const double* L,
+                                         int lneg, const double* R, int rneg) {
   constexpr int U = IMSRG_122_UNROLL;
-  for (; x + U - 1 < xe; x += U) {
+  int x = 0;
+  for (; x + U - 1 < n; x += U) {
     Partner p[U];
     double r[U], l[U];
 #pragma unroll
-    for (int u = 0; u < U; ++u) p[u] = part[x + u];
+    for (int u = 0; u < U; ++u) p[u] = part[(x + u) * ps];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const i64 o = pa_off(A, (p[u].code >> 22) & 3, p[u].code & 0x3fffff);
@@ -62,18 +68,19 @@ __device__ __forceinline__ double sum122(double sm, const Partner* part, int x, 
       l[u] = L[o];
     }
 #pragma unroll
-    for (int u = 0; u < U; ++u) sm = term122<ROW>(sm, p[u], r[u], l[u], A, pre_other, s_f, nf1, Lh, Rh);
+    for (int u = 0; u < U; ++u) sm = term122<ROW>(sm, p[u], r[u], l[u], A, pa1, pa2, s_f, nf1, lneg, rneg);
   }
-  for (; x < xe; ++x) {
-    const Partner p = part[x];
+  for (; x < n; ++x) {
+    const Partner p = part[x * ps];
     const i64 o = pa_off(A, (p.code >> 22) & 3, p.code & 0x3fffff);
-    sm = term122<ROW>(sm, p, R[o], L[o], A, pre_other, s_f, nf1, Lh, Rh);
+    sm = term122<ROW>(sm, p, R[o], L[o], A, pa1, pa2, s_f, nf1, lneg, rneg);
   }
   return sm;
 }
 
 __global__ __launch_bounds__(256) void k_comm122p(DevBasis B, const SegTask* tasks, const Partner* part,
-                                                  const int* rowp, const int* colp, const int* f1off, int nf1,
+                                                  const Partner* partT, int pstride, const int* rowp,
+                                                  const int* colp, const int* f1off, int nf1,
                                                   const double* L, int Lh, const double* R, int Rh, double* RES,
                                                   int RESh, int mirror) {
   int inst;
@@ -108,20 +115,23 @@ __global__ __launch_bounds__(256) void k_comm122p(DevBasis B, const SegTask* tas
   PartnerAddr AR, AC;
   partner_addr(B.goff + q * 6, B.dims + q * 3, rc, cc, IX, JX, AR, AC);   // uniform (scalar) loads
   double sm = 0.0;
-  sm = sum122<true>(sm, part, rowp[gr], rowp[gr + 1], AR, pre_cd, s_f, nf1, L, Lh, R, Rh);
-  sm = sum122<false>(sm, part, colp[gc], colp[gc + 1], AC, pre_ab, s_f, nf1, L, Lh, R, Rh);
+  const int lneg = Lh < 0, rneg = Rh < 0;
+  sm = sum122<true>(sm, partT + gr, pstride, rowp[gr + 1] - rowp[gr], AR, pre_cd, pre_cd * 1.4142135623730951, s_f,
+                    nf1, L, lneg, R, rneg);
+  sm = sum122<false>(sm, part + colp[gc], 1, colp[gc + 1] - colp[gc], AC, pre_ab, pre_ab * 1.4142135623730951, s_f,
+                     nf1, L, lneg, R, rneg);
   sm = sm / sqrt(1.0 + (a == b ? 1.0 : 0.0)) / sqrt(1.0 + (c == d ? 1.0 : 0.0));
   const i64 base = B.goff[q * 6 + g];
   RES[base + IX + (i64)JX * n1] = sm;
   if (sq && mirror) RES[base + JX + (i64)IX * n1] = sm * RESh;
 }
 
-void launch_comm122p(const DevBasis& B, const SegTask* t, int nt, const Partner* part, const int* rowp,
-                     const int* colp, const int* f1off, int nf1, const double* L, int Lh, const double* R, int Rh,
-                     double* RES, int RESh, hipStream_t s, int mirror) {
+void launch_comm122p(const DevBasis& B, const SegTask* t, int nt, const Partner* part, const Partner* partT,
+                     int pstride, const int* rowp, const int* colp, const int* f1off, int nf1, const double* L, int Lh,
+                     const double* R, int Rh, double* RES, int RESh, hipStream_t s, int mirror) {
   if (nt <= 0) return;
-  hipLaunchKernelGGL(k_comm122p, dim3(nt, B.ninst), dim3(kTile), 2 * nf1 * sizeof(double), s, B, t, part, rowp, colp,
-                     f1off, nf1, L, Lh, R, Rh, RES, RESh, mirror);
+  hipLaunchKernelGGL(k_comm122p, dim3(nt, B.ninst), dim3(kTile), 2 * nf1 * sizeof(double), s, B, t, part, partT,
+                     pstride, rowp, colp, f1off, nf1, L, Lh, R, Rh, RES, RESh, mirror);
 }
 
 // ================================================== commutator_222_ph scatter
@@ -155,15 +165,17 @@ __device__ __forceinline__ double ph_pair(const DevBasis& B, const i64* wcoff, c
   // inside reads rows (k,i) / cols (j,l); outside rows (l,i) / cols (j,k).
   // 32-bit element offsets from uniform bases (WCC holds < 2^32 doubles,
   // checked at plan time)
-  const int N = B.N, JTM = B.JTM;
+  // The row tables are read with J' fastest (rmap): the consecutive J'
+  // terms of one quadruple, which the grouped form runs in neighbouring
+  // lanes, read neighbouring entries.
+  const int N = B.N, JTM = B.JTM, nJ = B.nJ, h = JP >> 1;
   const int a = INSIDE ? k : l, c = INSIDE ? l : k;
-  const int qx = JP / 2 + Tz * (JTM + 1) + 2 * PAR * (JTM + 1);
-  const unsigned t = (unsigned)(JP >> 1) * (unsigned)(N * N);
+  const int qx = h + Tz * (JTM + 1) + 2 * PAR * (JTM + 1);
   const unsigned wo = (unsigned)wcoff[qx], rr = (unsigned)rlen[qx];
-  const unsigned x1 = B.rmapJ[t + a * N + i];    // gik | gil
-  const unsigned y1 = B.rmapJ[t + j * N + c];    // rjl | rjk
-  const unsigned x2 = B.rmapJT[t + a * N + i];   // rik | ril
-  const unsigned y2 = B.rmapJ[t + c * N + j];    // gjl | gjk
+  const unsigned x1 = B.rmap[(a * N + i) * nJ + h];   // gik | gil
+  const unsigned y1 = B.rmap[(j * N + c) * nJ + h];   // rjl | rjk
+  const unsigned x2 = B.rmap[(i * N + a) * nJ + h];   // rik | ril
+  const unsigned y2 = B.rmap[(c * N + j) * nJ + h];   // gjl | gjk
   // W'(x1, y1) + phase W'(y2, x2), each read from the upper triangle
   const double A1 = x1 <= y1 ? WCC[wo + x1 + y1 * rr] : mpm * WCC[wo + y1 + x1 * rr];
   const double A2 = y2 <= x2 ? WCC[wo + y2 + x2 * rr] : mpm * WCC[wo + x2 + y2 * rr];
@@ -289,6 +301,11 @@ __global__ __launch_bounds__(256) void k_comm222ph_chunk(DevBasis B, const PhChu
   RES += (i64)inst * B.oplen;
   __shared__ int s_ijkl[kPhChunkItems], s_loff[kPhChunkItems], s_ioff[kPhChunkItems];
   __shared__ int s_jin[kPhChunkItems], s_nin[kPhChunkItems], s_jout[kPhChunkItems];
+  // per quadruple, from the orbit tables once: 2j of i, j, k, l (8 bits
+  // each), and the recoupling / output data (phase, Tz and parity of the
+  // inside and outside recouplings, the output channel's Tz / parity part,
+  // block g and the two (a,a) factors)
+  __shared__ int s_jq[kPhChunkItems], s_meta[kPhChunkItems];
   __shared__ double s_val[kPhChunkLanes];
   __shared__ i64 s_wcoff[kPhMaxCC];
   __shared__ int s_rlen[kPhMaxCC];
@@ -309,21 +326,33 @@ __global__ __launch_bounds__(256) void k_comm222ph_chunk(DevBasis B, const PhChu
     s_jin[tid] = r.jin;
     s_nin[tid] = r.ok ? r.nin : 0;
     s_jout[tid] = r.ok ? r.jout : -1;   // -1: no contribution at all
+    const int ji = B.jj[i], jj = B.jj[j], jk = B.jj[k], jl = B.jj[l];
+    const int ti = B.tz[i], tj = B.tz[j], tk = B.tz[k], tl = B.tz[l];
+    const int li = B.ll[i], lj = B.ll[j], lk = B.ll[k], ll = B.ll[l];
+    const int rc = B.con[i] + B.con[j], cc = B.con[k] + B.con[l];
+    const int g = (rc == 0) ? cc : (rc == 1 ? (cc == 1 ? 3 : 5) : 4);
+    s_jq[tid] = ji | (jj << 8) | (jk << 16) | (jl << 24);
+    // bit 0: phase (-1)^((ji+jj+jk+jl)/2) < 0; 1: Tz in; 2: parity in;
+    // 3: Tz out; 4: parity out; 5-7: output block g; 8: i == j; 9: k == l;
+    // 10+: the output channel's 2 (Tz + 1) + P
+    s_meta[tid] = (((ji + jj + jk + jl) / 2) & 1) | ((abs(ti - tk) / 2) << 1) | (((li + lk) & 1) << 2) |
+                  ((abs(ti - tl) / 2) << 3) | (((li + ll) & 1) << 4) | (g << 5) | ((i == j) << 8) |
+                  ((k == l) << 9) | ((2 * ((ti + tj) / 2 + 1) + ((li + lj) & 1)) << 10);
   }
   __syncthreads();
   for (int t = tid; t < c.nlanes; t += blockDim.x) {
     const int q = lds_floor(s_loff, c.nq, t);
-    const int u = s_ijkl[q];
+    const int u = s_ijkl[q], m = s_meta[q];
     const int i = u & 255, j = (u >> 8) & 255, k = (u >> 16) & 255, l = (u >> 24) & 255;
     const int r = t - s_loff[q];
-    const int phase = m1pow((B.jj[i] + B.jj[j] + B.jj[k] + B.jj[l]) / 2);
+    const int phase = (m & 1) ? -1 : 1;
     double v;
     if (r < s_nin[q])
-      v = ph_pair<true>(B, s_wcoff, s_rlen, WCC, mpm, phase, i, j, k, l, s_jin[q] + 2 * r,
-                        abs(B.tz[i] - B.tz[k]) / 2, (B.ll[i] + B.ll[k]) % 2);
+      v = ph_pair<true>(B, s_wcoff, s_rlen, WCC, mpm, phase, i, j, k, l, s_jin[q] + 2 * r, (m >> 1) & 1,
+                        (m >> 2) & 1);
     else
       v = ph_pair<false>(B, s_wcoff, s_rlen, WCC, mpm, phase, i, j, k, l, s_jout[q] + 2 * (r - s_nin[q]),
-                         abs(B.tz[i] - B.tz[l]) / 2, (B.ll[i] + B.ll[l]) % 2);
+                         (m >> 3) & 1, (m >> 4) & 1);
     s_val[t] = v;
   }
   __syncthreads();
@@ -334,7 +363,8 @@ __global__ __launch_bounds__(256) void k_comm222ph_chunk(DevBasis B, const PhChu
   const int u = it.ijkl;
   const int i = u & 255, j = (u >> 8) & 255, k = (u >> 16) & 255, l = (u >> 24) & 255;
   const int J = it.J;
-  const int ji = B.jj[i], jj = B.jj[j], jk = B.jj[k], jl = B.jj[l];
+  const int jq = s_jq[q], m = s_meta[q];
+  const int ji = jq & 255, jj = (jq >> 8) & 255, jk = (jq >> 16) & 255, jl = (jq >> 24) & 255;
   const double* v = s_val + s_loff[q];
   const int nin = s_nin[q], jin = s_jin[q], jout = s_jout[q];
   const int nout = (q + 1 < c.nq ? s_loff[q + 1] : c.nlanes) - s_loff[q] - nin;
@@ -343,15 +373,13 @@ __global__ __launch_bounds__(256) void k_comm222ph_chunk(DevBasis B, const PhChu
   for (int r = 0; r < nin; ++r) sm = sm + sg * v[r] * sixj_tri(B, ji, jj, J, jl, jk, jin + 2 * r);
   for (int r = 0; r < nout; ++r) sm = sm + 1.0 * v[nin + r] * sixj_tri(B, ji, jj, J, jk, jl, jout + 2 * r);
   // output position: channel from (J, Tz, parity), block from the pair classes
-  const int P = (B.ll[i] + B.ll[j]) & 1;
-  const int qc = 3 * J + 2 * ((B.tz[i] + B.tz[j]) / 2 + 1) + P;
-  const int rc = B.con[i] + B.con[j], cc = B.con[k] + B.con[l];
-  const int g = (rc == 0) ? cc : (rc == 1 ? (cc == 1 ? 3 : 5) : 4);
+  const int qc = 3 * J + (m >> 10);
+  const int g = (m >> 5) & 7;
   const int IX = B.xmap[((i64)i * B.N + j) * B.nJ + (J >> 1)];
   const int JX = B.xmap[((i64)k * B.N + l) * B.nJ + (J >> 1)];
   const int n1 = brows(B, qc, g);
-  const double pre = (i == j) ? IMSRG_INV_SQRT2_REF : 1.0;
-  const double pre2 = (k == l) ? IMSRG_INV_SQRT2_REF : 1.0;
+  const double pre = ((m >> 8) & 1) ? IMSRG_INV_SQRT2_REF : 1.0;
+  const double pre2 = ((m >> 9) & 1) ? IMSRG_INV_SQRT2_REF : 1.0;
   double* X = RES + B.goff[qc * 6 + g];
   const i64 o = IX + (i64)JX * n1;
   X[o] = X[o] + sm * pre * pre2;

@@ -131,9 +131,11 @@ void launch_hf_pack(const DevBasis& B, const SegTask* t, int nt, const double* o
                     hipStream_t s);
 void launch_hf_unpack(const DevBasis& B, const SegTask* t, int nt, const double* V, const i64* voff, double* op,
                       hipStream_t s);
-void launch_comm122p(const DevBasis& B, const SegTask* t, int nt, const Partner* part, const int* rowp,
-                     const int* colp, const int* f1off, int nf1, const double* L, int Lh, const double* R, int Rh,
-                     double* RES, int RESh, hipStream_t s, int mirror = 1);
+// part: the partner lists (CSR rowp / colp); partT: the row lists with
+// entry k of pair gid at k * pstride + gid
+void launch_comm122p(const DevBasis& B, const SegTask* t, int nt, const Partner* part, const Partner* partT,
+                     int pstride, const int* rowp, const int* colp, const int* f1off, int nf1, const double* L, int Lh,
+                     const double* R, int Rh, double* RES, int RESh, hipStream_t s, int mirror = 1);
 void launch_pandya(const DevBasis& B, const CCTask* t, int nt, const double* op, int herm, double* CC,
                    hipStream_t s);
 void launch_comm111(const DevBasis& B, const double* L, int Lh, const double* R, int Rh, double* RES,
@@ -212,9 +214,10 @@ void launch_ph_vjp(const DevBasis& B, const QuadItem* items, int nitems, const i
                    const int* qstart, const PairTask* tiles, int ntiles, const int* tstart, const int2* terms,
                    const double* RESb, int RESh, int Lh, int Rh, double* SB, double* CV, double* U, hipStream_t s);
 // both 122 transposes (two-body and one-body) in one pass
-void launch_122_vjp(const DevBasis& B, const SegTask* t, int nt, const Partner* part, const int* rowp, const int* colp,
-                    const int* f1off, int nf1, const double* L, int Lh, const double* R, int Rh, const double* cbar,
-                    const double* RESb, int RESh, double* Lb, double* Rb, hipStream_t s);
+void launch_122_vjp(const DevBasis& B, const SegTask* t, int nt, const Partner* part, const Partner* partT,
+                    int pstride, const int* rowp, const int* colp, const int* f1off, int nf1, const double* L, int Lh,
+                    const double* R, int Rh, const double* cbar, const double* RESb, int RESh, double* Lb, double* Rb,
+                    hipStream_t s);
 // the same transpose gathered per stored element over tasks of the blocks
 // a Pandya term reads (no atomics)
 void launch_pandya_vjp_gather(const DevBasis& B, const SegTask* t, int nt, const double* CCb, int herm, double* opb,

@@ -331,50 +331,51 @@ void launch_to_dense(const DevBasis& B, const SegTask* t, int nt, const double* 
 #ifndef IMSRG_122_UNROLL
 #define IMSRG_122_UNROLL 2
 #endif
+// The +-1 factors of a term (the record's sign, the herm factor of a
+// transposed element: ch for cbar, Rh / Lh for R / L, and Lh / Rh of a
+// transposed one-body element) combine into one sign bit per product,
+// applied to pre = pre_other * (1 or sqrt 2) (pa1 / pa2); chneg / lneg /
+// rneg: RESh / Lh / Rh < 0.
 template <bool ROW>
-__device__ __forceinline__ void vjp122(double& y1, double& y2, const Partner* part, int x, int xe,
-                                       const PartnerAddr A, double pre_other, double sb, int ch, const double* fL,
-                                       const double* fR, double* lbar, double* rbar, const double* cbar,
-                                       const double* L, int Lh, const double* R, int Rh) {
+__device__ __forceinline__ void vjp122(double& y1, double& y2, const Partner* part, i64 ps, int n,
+                                       const PartnerAddr A, double pa1, double pa2, double sb, int chneg,
+                                       const double* fL, const double* fR, double* lbar, double* rbar,
+                                       const double* cbar, const double* L, int lneg, const double* R, int rneg) {
   constexpr int U = IMSRG_122_UNROLL;
-  const double sg = ROW ? 1.0 : -1.0;
-  while (x < xe) {
-    const int nu = min(U, xe - x);
+  constexpr int col = ROW ? 0 : 1;
+  const bool ob = sb != 0.0;   // the one-body part of this element
+  int x = 0;
+  while (x < n) {
+    const int nu = min(U, n - x);
     Partner p[U];
     i64 o[U];
     double c[U], r[U], l[U];
 #pragma unroll
     for (int u = 0; u < U; ++u)
-      if (u < nu) p[u] = part[x + u];
+      if (u < nu) p[u] = part[(x + u) * ps];
 #pragma unroll
     for (int u = 0; u < U; ++u)
       if (u < nu) {
         o[u] = pa_off(A, (p[u].code >> 22) & 3, p[u].code & 0x3fffff);
         c[u] = cbar[o[u]];
-        if (sb != 0.0) { r[u] = R[o[u]]; l[u] = L[o[u]]; }
+        if (ob) { r[u] = R[o[u]]; l[u] = L[o[u]]; }
       }
 #pragma unroll
     for (int u = 0; u < U; ++u)
       if (u < nu) {
-        const int cl = (p[u].code >> 22) & 3;
-        double pre = ((p[u].code >> 26) & 1) ? 1.4142135623730951 : 1.0;
-        if ((p[u].code >> 25) & 1) pre = -pre;
-        pre = ROW ? pre * pre_other : pre_other * pre;
-        const bool lo = !pa_trans(A, cl);
-        const double v = lo ? c[u] * pre : c[u] * ch * pre;
-        const bool fs = (p[u].code >> 24) & 1;
+        const int code = p[u].code;
+        const int tr = pa_trans(A, (code >> 22) & 3);
+        const int fs = (code >> 24) & 1, neg = ((code >> 25) & 1) ^ col;
+        const double a = ((code >> 26) & 1) ? pa2 : pa1;
         const int fk = p[u].fk;
-        if (ROW) {
-          y1 += fR[fk] * (fs ? Rh : 1) * v;
-          y2 += fL[fk] * (fs ? Lh : 1) * v;
-        } else {
-          y1 -= fR[fk] * (fs ? Rh : 1) * v;
-          y2 -= fL[fk] * (fs ? Lh : 1) * v;
-        }
-        if (sb != 0.0) {
-          const double vr = lo ? r[u] * pre : r[u] * Rh * pre, vl = lo ? l[u] * pre : l[u] * Lh * pre;
-          unsafeAtomicAdd(lbar + fk, sg * sb * (fs ? Lh : 1) * vr);
-          unsafeAtomicAdd(rbar + fk, -sg * sb * (fs ? Rh : 1) * vl);
+        const int tc = neg ^ (tr & chneg);
+        y1 = fma(fR[fk] * ((tc ^ (fs & rneg)) ? -a : a), c[u], y1);
+        y2 = fma(fL[fk] * ((tc ^ (fs & lneg)) ? -a : a), c[u], y2);
+        if (ob) {
+          const double al = (neg ^ (tr & rneg) ^ (fs & lneg)) ? -a : a;
+          const double ar = (neg ^ 1 ^ (tr & lneg) ^ (fs & rneg)) ? -a : a;
+          unsafeAtomicAdd(lbar + fk, (sb * al) * r[u]);
+          unsafeAtomicAdd(rbar + fk, (sb * ar) * l[u]);
         }
       }
     x += nu;
@@ -382,7 +383,8 @@ __device__ __forceinline__ void vjp122(double& y1, double& y2, const Partner* pa
 }
 
 __global__ __launch_bounds__(256) void k_122_vjp(DevBasis B, const SegTask* tasks, int ntasks, const Partner* part,
-                                                 const int* rowp, const int* colp, const int* f1off, int nf1,
+                                                 const Partner* partT, int pstride, const int* rowp,
+                                                 const int* colp, const int* f1off, int nf1,
                                                  const double* L, int Lh, const double* R, int Rh,
                                                  const double* cbar, const double* RESb, int RESh, double* Lb,
                                                  double* Rb) {
@@ -423,8 +425,11 @@ __global__ __launch_bounds__(256) void k_122_vjp(DevBasis B, const SegTask* task
     PartnerAddr AR, AC;
     partner_addr(B.goff + q * 6, B.dims + q * 3, rc, cc, IX, JX, AR, AC);
     double y1 = 0.0, y2 = 0.0;
-    vjp122<true>(y1, y2, part, rowp[gr], rowp[gr + 1], AR, pre_cd, sb, ch, fL, fR, lbar, rbar, cbar, L, Lh, R, Rh);
-    vjp122<false>(y1, y2, part, colp[gc], colp[gc + 1], AC, pre_ab, sb, ch, fL, fR, lbar, rbar, cbar, L, Lh, R, Rh);
+    const int lneg = Lh < 0, rneg = Rh < 0, chneg = ch < 0;
+    vjp122<true>(y1, y2, partT + gr, pstride, rowp[gr + 1] - rowp[gr], AR, pre_cd, pre_cd * 1.4142135623730951, sb,
+                 chneg, fL, fR, lbar, rbar, cbar, L, lneg, R, rneg);
+    vjp122<false>(y1, y2, part + colp[gc], 1, colp[gc + 1] - colp[gc], AC, pre_ab, pre_ab * 1.4142135623730951, sb,
+                  chneg, fL, fR, lbar, rbar, cbar, L, lneg, R, rneg);
     y1 *= nrm;
     y2 *= nrm;
     const i64 o = B.goff[q * 6 + g] + IX + (i64)JX * n1;
@@ -449,13 +454,14 @@ __global__ __launch_bounds__(256) void k_122_vjp(DevBasis B, const SegTask* task
   }
 }
 
-void launch_122_vjp(const DevBasis& B, const SegTask* t, int nt, const Partner* part, const int* rowp, const int* colp,
-                    const int* f1off, int nf1, const double* L, int Lh, const double* R, int Rh, const double* cbar,
-                    const double* RESb, int RESh, double* Lb, double* Rb, hipStream_t s) {
+void launch_122_vjp(const DevBasis& B, const SegTask* t, int nt, const Partner* part, const Partner* partT,
+                    int pstride, const int* rowp, const int* colp, const int* f1off, int nf1, const double* L, int Lh,
+                    const double* R, int Rh, const double* cbar, const double* RESb, int RESh, double* Lb, double* Rb,
+                    hipStream_t s) {
   if (nt <= 0) return;
   const int grid = nt;
-  hipLaunchKernelGGL(k_122_vjp, dim3(grid), dim3(256), 4 * nf1 * sizeof(double), s, B, t, nt, part, rowp, colp, f1off,
-                     nf1, L, Lh, R, Rh, cbar, RESb, RESh, Lb, Rb);
+  hipLaunchKernelGGL(k_122_vjp, dim3(grid), dim3(256), 4 * nf1 * sizeof(double), s, B, t, nt, part, partT, pstride,
+                     rowp, colp, f1off, nf1, L, Lh, R, Rh, cbar, RESb, RESh, Lb, Rb);
 }
 
 // -------------------------------------------------- 222_ph transpose
