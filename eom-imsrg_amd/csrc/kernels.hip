@@ -159,23 +159,33 @@ __device__ __forceinline__ double sixj_tri(const DevBasis& B, int j1, int j2, in
 //   inside:  - (-1)^((jl+jk+J)/2) (W'(gik,rjl) + phase W'(gjl,rik)) {ji jj J; jl jk J'}
 //   outside: + (W'(gil,rjk) + phase W'(gjk,ril)) {ji jj J; jk jl J'}
 // ph_pair gives the W' combination of one J' term.
-template <bool INSIDE>
+// JFAST: the row tables read with J' fastest (rmap; the grouped form runs
+// the consecutive J' terms of one quadruple in neighbouring lanes, which
+// then read neighbouring entries), else with J' slowest (rmapJ / rmapJT;
+// the per-channel form runs neighbouring i in neighbouring lanes)
+template <bool INSIDE, bool JFAST>
 __device__ __forceinline__ double ph_pair(const DevBasis& B, const i64* wcoff, const int* rlen, const double* WCC,
                                           double mpm, int phase, int i, int j, int k, int l, int JP, int Tz, int PAR) {
   // inside reads rows (k,i) / cols (j,l); outside rows (l,i) / cols (j,k).
   // 32-bit element offsets from uniform bases (WCC holds < 2^32 doubles,
   // checked at plan time)
-  // The row tables are read with J' fastest (rmap): the consecutive J'
-  // terms of one quadruple, which the grouped form runs in neighbouring
-  // lanes, read neighbouring entries.
   const int N = B.N, JTM = B.JTM, nJ = B.nJ, h = JP >> 1;
   const int a = INSIDE ? k : l, c = INSIDE ? l : k;
   const int qx = h + Tz * (JTM + 1) + 2 * PAR * (JTM + 1);
   const unsigned wo = (unsigned)wcoff[qx], rr = (unsigned)rlen[qx];
-  const unsigned x1 = B.rmap[(a * N + i) * nJ + h];   // gik | gil
-  const unsigned y1 = B.rmap[(j * N + c) * nJ + h];   // rjl | rjk
-  const unsigned x2 = B.rmap[(i * N + a) * nJ + h];   // rik | ril
-  const unsigned y2 = B.rmap[(c * N + j) * nJ + h];   // gjl | gjk
+  unsigned x1, y1, x2, y2;
+  if (JFAST) {
+    x1 = B.rmap[(a * N + i) * nJ + h];   // gik | gil
+    y1 = B.rmap[(j * N + c) * nJ + h];   // rjl | rjk
+    x2 = B.rmap[(i * N + a) * nJ + h];   // rik | ril
+    y2 = B.rmap[(c * N + j) * nJ + h];   // gjl | gjk
+  } else {
+    const unsigned t = (unsigned)h * (unsigned)(N * N);
+    x1 = B.rmapJ[t + a * N + i];
+    y1 = B.rmapJ[t + j * N + c];
+    x2 = B.rmapJT[t + a * N + i];
+    y2 = B.rmapJ[t + c * N + j];
+  }
   // W'(x1, y1) + phase W'(y2, x2), each read from the upper triangle
   const double A1 = x1 <= y1 ? WCC[wo + x1 + y1 * rr] : mpm * WCC[wo + y1 + x1 * rr];
   const double A2 = y2 <= x2 ? WCC[wo + y2 + x2 * rr] : mpm * WCC[wo + x2 + y2 * rr];
@@ -188,7 +198,7 @@ __device__ __forceinline__ double ph_jsum(const DevBasis& B, const double* WCC, 
   const int ji = B.jj[i], jj = B.jj[j], jk = B.jj[k], jl = B.jj[l];
   const double sg = INSIDE ? -(double)m1pow((jl + jk + Jtot) / 2) : 1.0;
   for (int JP = jmin; JP <= jmax; JP += 2) {
-    const double v = ph_pair<INSIDE>(B, B.wcoff, B.rlen, WCC, mpm, phase, i, j, k, l, JP, Tz, PAR);
+    const double v = ph_pair<INSIDE, false>(B, B.wcoff, B.rlen, WCC, mpm, phase, i, j, k, l, JP, Tz, PAR);
     sm = sm + sg * v * (INSIDE ? sixj_tri(B, ji, jj, Jtot, jl, jk, JP) : sixj_tri(B, ji, jj, Jtot, jk, jl, JP));
   }
   return sm;
@@ -348,10 +358,10 @@ __global__ __launch_bounds__(256) void k_comm222ph_chunk(DevBasis B, const PhChu
     const int phase = (m & 1) ? -1 : 1;
     double v;
     if (r < s_nin[q])
-      v = ph_pair<true>(B, s_wcoff, s_rlen, WCC, mpm, phase, i, j, k, l, s_jin[q] + 2 * r, (m >> 1) & 1,
+      v = ph_pair<true, true>(B, s_wcoff, s_rlen, WCC, mpm, phase, i, j, k, l, s_jin[q] + 2 * r, (m >> 1) & 1,
                         (m >> 2) & 1);
     else
-      v = ph_pair<false>(B, s_wcoff, s_rlen, WCC, mpm, phase, i, j, k, l, s_jout[q] + 2 * (r - s_nin[q]),
+      v = ph_pair<false, true>(B, s_wcoff, s_rlen, WCC, mpm, phase, i, j, k, l, s_jout[q] + 2 * (r - s_nin[q]),
                          (m >> 3) & 1, (m >> 4) & 1);
     s_val[t] = v;
   }
