@@ -41,8 +41,8 @@ __device__ __forceinline__ double ninej(const DevTensor& T, int a, int b, int J1
 __device__ __forceinline__ double n9(const DevTensor& T, int u1, int u2, int J3, int u4, int u5, int J4, int J1,
                                      int J2) {
   const int nR = T.rank / 2 + 1;
-  return T.n9[(((((((i64)(J1 >> 1) * nR + ((J2 - J1) >> 1)) * T.nho + (u1 >> 1)) * T.nho + (u2 >> 1)) * T.nho +
-                   (u4 >> 1)) * T.nho + (u5 >> 1)) * T.nJ + (J3 >> 1)) * nR + ((J4 - J3) >> 1)];
+  return T.n9[(((((((i64)(u1 >> 1) * T.nho + (u2 >> 1)) * T.nho + (u4 >> 1)) * T.nho + (u5 >> 1)) * T.nJ +
+                   (J3 >> 1)) * nR + ((J4 - J3) >> 1)) * T.nJ + (J1 >> 1)) * nR + ((J2 - J1) >> 1)];
 }
 
 __device__ __forceinline__ int tsub(int C1, int C2) {   // C = con + con + 1; tensor_elem's qx arithmetic
@@ -332,13 +332,64 @@ __device__ double ph_gather(const DevBasis& B, const DevTensor& T, const double*
   return sm;
 }
 
-// one thread per stored two-body element: mode 0 TS_commutator_122
-// (TS_commutators.f90:555-663, assigns), 1 TS_commutator_212 (666-806), 2
-// the 222_ph back-transform (X = Wx, Y = Wy)
-template <int MODE>
-__global__ __launch_bounds__(256) void k_ts_two(DevBasis B, DevTensor T, const TSub* subs, const TElem* tasks,
-                                                const double* L, int Lh, const double* R, int Rh, const double* X,
-                                                const double* Y, double* RES) {
+// TS_commutator_122 of one output element (TS_commutators.f90:555-663)
+__device__ double ts122(const DevBasis& B, const DevTensor& T, const double* L, int Lh, const double* R, int Rh, int a,
+                        int b, int c, int d, int J1, int J2) {
+  double sm = 0.0;
+  const int sa = B.spblk[a], sbk = B.spblk[b], sc = B.spblk[c], sd = B.spblk[d];
+  for (int k = B.spstart[sa]; k < B.spstart[sa + 1]; ++k) {
+    const int i = B.spmem[k];
+    sm = sm + f_elem(B, L, Lh, a, i) * tensor_elem(B, T, R, Rh, i, b, c, d, J1, J2);
+  }
+  for (int k = B.spstart[sbk]; k < B.spstart[sbk + 1]; ++k) {
+    const int i = B.spmem[k];
+    sm = sm + f_elem(B, L, Lh, b, i) * tensor_elem(B, T, R, Rh, a, i, c, d, J1, J2);
+  }
+  for (int k = B.spstart[sc]; k < B.spstart[sc + 1]; ++k) {
+    const int i = B.spmem[k];
+    sm = sm - f_elem(B, L, Lh, i, c) * tensor_elem(B, T, R, Rh, a, b, i, d, J1, J2);
+  }
+  for (int k = B.spstart[sd]; k < B.spstart[sd + 1]; ++k) {
+    const int i = B.spmem[k];
+    sm = sm - f_elem(B, L, Lh, i, d) * tensor_elem(B, T, R, Rh, a, b, c, i, J1, J2);
+  }
+  sm = sm / sqrt(1.0 + (a == b)) / sqrt(1.0 + (c == d));
+  return sm;
+}
+
+// TS_commutator_212 of one output element (TS_commutators.f90:666-806)
+__device__ double ts212(const DevBasis& B, const DevTensor& T, const double* L, int Lh, const double* R, int Rh, int a,
+                        int b, int c, int d, int J1, int J2) {
+  const int rank = T.rank;
+  const int ja = B.jj[a], jb = B.jj[b], jc = B.jj[c], jd = B.jj[d];
+  const int ta = B.tz[a], tb = B.tz[b], tc = B.tz[c], td = B.tz[d];
+  const int mla = B.ll[a] & 1, mlb = B.ll[b] & 1, mlc = B.ll[c] & 1, mld = B.ll[d] & 1;
+  double sm = 0.0;
+  for (int i = 0; i < B.N; ++i) {
+    const int ji = B.jj[i], ti = B.tz[i], mli = (B.ll[i] + (T.dpar >> 1)) & 1;
+    double s1 = 0.0, s2 = 0.0, s3 = 0.0, s4 = 0.0;
+    if (ti == ta && mli == mla && dtri(ji, ja, rank))
+      s1 = -x6(T, J1, J2, ji, ja, jb) * f_tensor_elem(B, R, Rh, a, i) * v_elem(B, L, Lh, i, b, c, d, J2);
+    s1 = s1 * m1pow((ja + jb - J2) / 2);
+    if (ti == tb && mli == mlb && dtri(ji, jb, rank))
+      s2 = x6(T, J1, J2, ji, jb, ja) * f_tensor_elem(B, R, Rh, b, i) * v_elem(B, L, Lh, i, a, c, d, J2);
+    s2 = s2 * m1pow((J1 + J2) / 2);
+    if (ti == td && mli == mld && dtri(ji, jd, rank))
+      s3 = x6(T, J1, J2, jd, ji, jc) * f_tensor_elem(B, R, Rh, i, d) * v_elem(B, L, Lh, a, b, c, i, J1);
+    s3 = s3 * m1pow((jc + jd - J1) / 2);
+    if (ti == tc && mli == mlc && dtri(ji, jc, rank))
+      s4 = -x6(T, J1, J2, jc, ji, jd) * f_tensor_elem(B, R, Rh, i, c) * v_elem(B, L, Lh, a, b, d, i, J1);
+    s4 = s4 * m1pow((J2 + J1) / 2);
+    sm = sm + (s1 + s2 + s3 + s4);
+  }
+  sm = sm * sqrt((J1 + 1.0) * (J2 + 1.0) / (1.0 + (a == b)) / (1.0 + (c == d))) * m1pow(rank / 2);
+  return sm;
+}
+
+// TS_commutator_122 (assigns), one thread per stored two-body element in
+// tiles of 256 consecutive elements of a sub-block
+__global__ __launch_bounds__(256) void k_ts122(DevBasis B, DevTensor T, const TSub* subs, const TElem* tasks,
+                                               const double* L, int Lh, const double* R, int Rh, double* RES) {
   const TElem t = tasks[blockIdx.x];
   const TSub sb = subs[t.sb];
   const int e = t.start + threadIdx.x;
@@ -348,56 +399,34 @@ __global__ __launch_bounds__(256) void k_ts_two(DevBasis B, DevTensor T, const T
   unpack(B.pairs[sb.roff + IX], a, b);
   unpack(B.pairs[sb.coff + JX], c, d);
   const int J1 = T.blk[sb.q * 8], J2 = T.blk[sb.q * 8 + 1];
-  double* out = RES + sb.off + IX + (i64)JX * sb.n1;
-  if constexpr (MODE == 0) {
-    double sm = 0.0;
-    const int sa = B.spblk[a], sbk = B.spblk[b], sc = B.spblk[c], sd = B.spblk[d];
-    for (int k = B.spstart[sa]; k < B.spstart[sa + 1]; ++k) {
-      const int i = B.spmem[k];
-      sm = sm + f_elem(B, L, Lh, a, i) * tensor_elem(B, T, R, Rh, i, b, c, d, J1, J2);
-    }
-    for (int k = B.spstart[sbk]; k < B.spstart[sbk + 1]; ++k) {
-      const int i = B.spmem[k];
-      sm = sm + f_elem(B, L, Lh, b, i) * tensor_elem(B, T, R, Rh, a, i, c, d, J1, J2);
-    }
-    for (int k = B.spstart[sc]; k < B.spstart[sc + 1]; ++k) {
-      const int i = B.spmem[k];
-      sm = sm - f_elem(B, L, Lh, i, c) * tensor_elem(B, T, R, Rh, a, b, i, d, J1, J2);
-    }
-    for (int k = B.spstart[sd]; k < B.spstart[sd + 1]; ++k) {
-      const int i = B.spmem[k];
-      sm = sm - f_elem(B, L, Lh, i, d) * tensor_elem(B, T, R, Rh, a, b, c, i, J1, J2);
-    }
-    sm = sm / sqrt(1.0 + (a == b)) / sqrt(1.0 + (c == d));
-    *out = sm;
-  } else if constexpr (MODE == 1) {
-    const int rank = T.rank;
-    const int ja = B.jj[a], jb = B.jj[b], jc = B.jj[c], jd = B.jj[d];
-    const int ta = B.tz[a], tb = B.tz[b], tc = B.tz[c], td = B.tz[d];
-    const int mla = B.ll[a] & 1, mlb = B.ll[b] & 1, mlc = B.ll[c] & 1, mld = B.ll[d] & 1;
-    double sm = 0.0;
-    for (int i = 0; i < B.N; ++i) {
-      const int ji = B.jj[i], ti = B.tz[i], mli = (B.ll[i] + (T.dpar >> 1)) & 1;
-      double s1 = 0.0, s2 = 0.0, s3 = 0.0, s4 = 0.0;
-      if (ti == ta && mli == mla && dtri(ji, ja, rank))
-        s1 = -x6(T, J1, J2, ji, ja, jb) * f_tensor_elem(B, R, Rh, a, i) * v_elem(B, L, Lh, i, b, c, d, J2);
-      s1 = s1 * m1pow((ja + jb - J2) / 2);
-      if (ti == tb && mli == mlb && dtri(ji, jb, rank))
-        s2 = x6(T, J1, J2, ji, jb, ja) * f_tensor_elem(B, R, Rh, b, i) * v_elem(B, L, Lh, i, a, c, d, J2);
-      s2 = s2 * m1pow((J1 + J2) / 2);
-      if (ti == td && mli == mld && dtri(ji, jd, rank))
-        s3 = x6(T, J1, J2, jd, ji, jc) * f_tensor_elem(B, R, Rh, i, d) * v_elem(B, L, Lh, a, b, c, i, J1);
-      s3 = s3 * m1pow((jc + jd - J1) / 2);
-      if (ti == tc && mli == mlc && dtri(ji, jc, rank))
-        s4 = -x6(T, J1, J2, jc, ji, jd) * f_tensor_elem(B, R, Rh, i, c) * v_elem(B, L, Lh, a, b, d, i, J1);
-      s4 = s4 * m1pow((J2 + J1) / 2);
-      sm = sm + (s1 + s2 + s3 + s4);
-    }
-    sm = sm * sqrt((J1 + 1.0) * (J2 + 1.0) / (1.0 + (a == b)) / (1.0 + (c == d))) * m1pow(rank / 2);
-    *out += sm;
-  } else {
-    *out += ph_gather(B, T, X, Y, Lh, Rh, a, b, c, d, J1, J2);
-  }
+  RES[sb.off + IX + (i64)JX * sb.n1] = ts122(B, T, L, Lh, R, Rh, a, b, c, d, J1, J2);
+}
+
+// TS_commutator_212 (adds), one thread per element of the quadruple-ordered
+// output list (neighbouring lanes share the orbit quadruple and so the orbit
+// loop and its conditions)
+__global__ __launch_bounds__(256) void k_ts212(DevBasis B, DevTensor T, const TPh* tasks, i64 nt, const double* L,
+                                               int Lh, const double* R, int Rh, double* RES) {
+  const i64 e = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nt) return;
+  const TPh t = tasks[e];
+  const int u = t.abcd;
+  RES[t.off] += ts212(B, T, L, Lh, R, Rh, u & 255, (u >> 8) & 255, (u >> 16) & 255, (u >> 24) & 255, t.jj & 0xffff,
+                      t.jj >> 16);
+}
+
+// the 222_ph back-transform, one thread per output element of the
+// quadruple-ordered list: neighbouring lanes share the orbit quadruple, so
+// they run the same pattern and (J3, J4) loops and read the same W elements
+// and one run of 9j symbols
+__global__ __launch_bounds__(256) void k_ts_ph(DevBasis B, DevTensor T, const TPh* tasks, i64 nt, int Lh, int Rh,
+                                               const double* Wx, const double* Wy, double* RES) {
+  const i64 e = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nt) return;
+  const TPh t = tasks[e];
+  const int u = t.abcd;
+  RES[t.off] += ph_gather(B, T, Wx, Wy, Lh, Rh, u & 255, (u >> 8) & 255, (u >> 16) & 255, (u >> 24) & 255,
+                          t.jj & 0xffff, t.jj >> 16);
 }
 
 // calculate_single_pandya, cross_coupled.f90:1416-1634, one thread per
@@ -505,13 +534,22 @@ void launch_ts_one(int mode, const DevBasis& B, const DevTensor& T, const TOne* 
   else hipLaunchKernelGGL(k_ts_one<2>, g, dim3(256), 0, s, B, T, t, nt, L, Lh, X, Y, R, Rh, RES, RESh);
 }
 
-void launch_ts_two(int mode, const DevBasis& B, const DevTensor& T, const TSub* sb, const TElem* t, int nt,
-                   const double* L, int Lh, const double* R, int Rh, const double* X, const double* Y, double* RES,
-                   hipStream_t s) {
-  if (nt == 0) return;
-  if (mode == 0) hipLaunchKernelGGL(k_ts_two<0>, dim3(nt), dim3(256), 0, s, B, T, sb, t, L, Lh, R, Rh, X, Y, RES);
-  else if (mode == 1) hipLaunchKernelGGL(k_ts_two<1>, dim3(nt), dim3(256), 0, s, B, T, sb, t, L, Lh, R, Rh, X, Y, RES);
-  else hipLaunchKernelGGL(k_ts_two<2>, dim3(nt), dim3(256), 0, s, B, T, sb, t, L, Lh, R, Rh, X, Y, RES);
+void launch_ts122(const DevBasis& B, const DevTensor& T, const TSub* sb, const TElem* t, int nt, const double* L,
+                  int Lh, const double* R, int Rh, double* RES, hipStream_t s) {
+  if (nt > 0) hipLaunchKernelGGL(k_ts122, dim3(nt), dim3(256), 0, s, B, T, sb, t, L, Lh, R, Rh, RES);
+}
+
+void launch_ts212(const DevBasis& B, const DevTensor& T, const TPh* t, i64 nt, const double* L, int Lh,
+                  const double* R, int Rh, double* RES, hipStream_t s) {
+  if (nt > 0)
+    hipLaunchKernelGGL(k_ts212, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, s, B, T, t, nt, L, Lh, R, Rh, RES);
+}
+
+void launch_ts_ph(const DevBasis& B, const DevTensor& T, const TPh* t, i64 nt, int Lh, int Rh, const double* Wx,
+                  const double* Wy, double* RES, hipStream_t s) {
+  if (nt > 0)
+    hipLaunchKernelGGL(k_ts_ph, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, s, B, T, t, nt, Lh, Rh, Wx, Wy,
+                       RES);
 }
 
 void launch_ts_pandya(const DevBasis& B, const DevTensor& T, const TCC* t, int nt, const double* R, int Rh,

@@ -38,8 +38,8 @@ struct DevTensor {
                                    // product W of that cross-coupled block, its leading dimension
   const double* n9;                // 9j {u1 u2 J3; u4 u5 J4; J1 J2 rank} * sqrt((J3+1)(J4+1)) of the
                                    // 222_ph back-transform, J3 <= J4 <= J3 + rank, J1 <= J2 <= J1 + rank:
-                                   // [J1/2][(J2-J1)/2][u1/2][u2/2][u4/2][u5/2][J3/2][(J4-J3)/2]
-                                   // (one output block's (J1, J2) owns a contiguous slice)
+                                   // [u1/2][u2/2][u4/2][u5/2][J3/2][(J4-J3)/2][J1/2][(J2-J1)/2]
+                                   // (the outputs of one orbit quadruple read one contiguous run)
   int nJ, nhx, nho;                // 2J/2 values; half-integers up to jmax + rank; up to jmax
 };
 
@@ -47,6 +47,10 @@ struct DevTensor {
 struct TSub { int q, g, n1, n2; i64 off, roff, coff; };
 // 256 consecutive elements of sub-block sb
 struct TElem { int sb; int start; };
+// one output element of the 222_ph back-transform: its offset, its orbits
+// (a | b << 8 | c << 16 | d << 24) and 2J1 | 2J2 << 16; the list is ordered
+// by orbit quadruple, so the lanes of a wave share the quadruple's loops
+struct TPh { int off, abcd, jj; };
 // one-body output: cls 0 fhh (p <= q, mirrored), 1 fpp (p <= q, mirrored), 2 fph
 struct TOne { int cls, p, q, pad; };
 // 256 consecutive elements of CCX (term 1) or CCR (term 2) of block qx
@@ -58,11 +62,16 @@ void launch_ts111(const DevBasis& B, const DevTensor& T, const double* L, int Lh
 // cross-coupled set of L), 2: TS_commutator_221 (X = w1, Y = w2, pm = Lh)
 void launch_ts_one(int mode, const DevBasis& B, const DevTensor& T, const TOne* t, int nt, const double* L, int Lh,
                    const double* X, const double* Y, const double* R, int Rh, double* RES, int RESh, hipStream_t s);
-// mode 0: TS_commutator_122 (assigns), 1: TS_commutator_212, 2: the
-// back-transform of TS_commutator_222_ph from the products Wx / Wy (X, Y)
-void launch_ts_two(int mode, const DevBasis& B, const DevTensor& T, const TSub* sb, const TElem* t, int nt,
-                   const double* L, int Lh, const double* R, int Rh, const double* X, const double* Y, double* RES,
-                   hipStream_t s);
+// TS_commutator_122 (assigns) over element tiles
+void launch_ts122(const DevBasis& B, const DevTensor& T, const TSub* sb, const TElem* t, int nt, const double* L,
+                  int Lh, const double* R, int Rh, double* RES, hipStream_t s);
+// TS_commutator_212 (adds) over the quadruple-ordered output list
+void launch_ts212(const DevBasis& B, const DevTensor& T, const TPh* t, i64 nt, const double* L, int Lh,
+                  const double* R, int Rh, double* RES, hipStream_t s);
+// the back-transform of TS_commutator_222_ph from the products Wx / Wy, one
+// thread per listed output element
+void launch_ts_ph(const DevBasis& B, const DevTensor& T, const TPh* t, i64 nt, int Lh, int Rh, const double* Wx,
+                  const double* Wy, double* RES, hipStream_t s);
 // calculate_single_pandya of every listed (block, term) into CC
 void launch_ts_pandya(const DevBasis& B, const DevTensor& T, const TCC* t, int nt, const double* R, int Rh,
                       double* CC, hipStream_t s);
