@@ -118,8 +118,17 @@ __global__ __launch_bounds__(256) void k_comm122p(DevBasis B, const SegTask* tas
   const int lneg = Lh < 0, rneg = Rh < 0;
   sm = sum122<true>(sm, partT + gr, pstride, rowp[gr + 1] - rowp[gr], AR, pre_cd, pre_cd * 1.4142135623730951, s_f,
                     nf1, L, lneg, R, rneg);
-  sm = sum122<false>(sm, part + colp[gc], 1, colp[gc + 1] - colp[gc], AC, pre_ab, pre_ab * 1.4142135623730951, s_f,
-                     nf1, L, lneg, R, rneg);
+  // column partners: when every active lane of the wave holds an element of
+  // one column pair (the common case), the list is read through the scalar
+  // cache instead of once per lane
+  const int gc0 = __builtin_amdgcn_readfirstlane((int)gc);
+  if (__all((int)gc == gc0)) {
+    const int x0 = colp[gc0], nx = colp[gc0 + 1] - x0;
+    sm = sum122<false>(sm, part + x0, 1, nx, AC, pre_ab, pre_ab * 1.4142135623730951, s_f, nf1, L, lneg, R, rneg);
+  } else {
+    sm = sum122<false>(sm, part + colp[gc], 1, colp[gc + 1] - colp[gc], AC, pre_ab, pre_ab * 1.4142135623730951, s_f,
+                       nf1, L, lneg, R, rneg);
+  }
   sm = sm / sqrt(1.0 + (a == b ? 1.0 : 0.0)) / sqrt(1.0 + (c == d ? 1.0 : 0.0));
   const i64 base = B.goff[q * 6 + g];
   RES[base + IX + (i64)JX * n1] = sm;

@@ -428,8 +428,17 @@ __global__ __launch_bounds__(256) void k_122_vjp(DevBasis B, const SegTask* task
     const int lneg = Lh < 0, rneg = Rh < 0, chneg = ch < 0;
     vjp122<true>(y1, y2, partT + gr, pstride, rowp[gr + 1] - rowp[gr], AR, pre_cd, pre_cd * 1.4142135623730951, sb,
                  chneg, fL, fR, lbar, rbar, cbar, L, lneg, R, rneg);
-    vjp122<false>(y1, y2, part + colp[gc], 1, colp[gc + 1] - colp[gc], AC, pre_ab, pre_ab * 1.4142135623730951, sb,
-                  chneg, fL, fR, lbar, rbar, cbar, L, lneg, R, rneg);
+    // column partners through the scalar cache when the wave's active lanes
+    // share one column pair
+    const int gc0 = __builtin_amdgcn_readfirstlane((int)gc);
+    if (__all((int)gc == gc0)) {
+      const int x0 = colp[gc0], nx = colp[gc0 + 1] - x0;
+      vjp122<false>(y1, y2, part + x0, 1, nx, AC, pre_ab, pre_ab * 1.4142135623730951, sb, chneg, fL, fR, lbar, rbar,
+                    cbar, L, lneg, R, rneg);
+    } else {
+      vjp122<false>(y1, y2, part + colp[gc], 1, colp[gc + 1] - colp[gc], AC, pre_ab, pre_ab * 1.4142135623730951, sb,
+                    chneg, fL, fR, lbar, rbar, cbar, L, lneg, R, rneg);
+    }
     y1 *= nrm;
     y2 *= nrm;
     const i64 o = B.goff[q * 6 + g] + IX + (i64)JX * n1;
