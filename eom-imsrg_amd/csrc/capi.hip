@@ -93,7 +93,7 @@ struct imsrg_basis_s {
   // GEMM tile shapes for pp1, k2, k3, ph (64x64 until tuned) and the reverse-mode
   // 222_pp_hh rounds and W' transpose (64x32 through LDS: the reverse sweep at
   // eMax 8 takes 0.47 s with it, 0.50-0.51 s on the register-direct shapes)
-  int tile_req[6] = {0, 0, 0, 0, 3, 3};
+  int tile_req[7] = {0, 0, 0, 0, 5, 6, 11};
   int plan_ver = 0;                 // bumped whenever a task list is rebuilt (captured graphs go stale)
   DevBasis D{};
   std::vector<void*> allocs;
@@ -152,7 +152,7 @@ struct imsrg_basis_s {
   // rounds of distinct outputs, the W' transpose products, quadruple
   // starts, the compact one-body map of 122
   bool adj_ready = false;
-  GemmFamily pb[6], phb;   // reverse-mode 222_pp_hh rounds, the W' transpose
+  GemmFamily pb[6], ppb, phb;   // reverse-mode 222_pp_hh rounds, their pp outputs, the W' transpose
   std::vector<int> f1map_h, f1off_h;   // 122: one-body offset -> compact index, and back
   int* f1off = nullptr;
   int nf1 = 0;
@@ -995,6 +995,12 @@ const BwdProd kBwd[36] = {
     {3, 2, 5, 1, 0, 0, 5, 0, 'p', 'h', 'b', -1, 1}, {2, 5, 5, 1, 1, 1, 2, 0, 'b', 'h', 'p', -1, 1},
 };
 
+// kBwd products computed by the fused family ppb instead of the rounds: those
+// into the pp blocks, and the L_pp^T / R_pp^T products into the pb and ph blocks
+bool fused_pp(const BwdProd& x) {
+  return x.out_g == 0 || (x.a_slot <= 1 && x.a_g == 0 && x.ta == 1 && x.b_slot == 4 && (x.out_g == 1 || x.out_g == 2));
+}
+
 int ensure_adj(imsrg_basis_s* B) {
   if (B->adj_ready) return 0;
   int rc = ensure_dev(B);
@@ -1003,6 +1009,8 @@ int ensure_adj(imsrg_basis_s* B) {
   const std::vector<int> order = channel_order(b);
   static const int rcls[6] = {0, 0, 0, 1, 2, 1}, ccls[6] = {0, 1, 2, 1, 2, 2};
   for (GemmFamily& f : B->pb) { f.probs_h.clear(); f.nks.clear(); f.group_end.clear(); f.tile = 0; }
+  GemmFamily& pp = B->ppb;
+  pp.probs_h.clear(); pp.nks.clear(); pp.group_end.clear(); pp.tile = 0;
   for (int q : order) {
     const int dd[3] = {b.dims[q * 3], b.dims[q * 3 + 1], b.dims[q * 3 + 2]};
     auto dim = [&](char c) { return c == 'p' ? dd[0] : (c == 'b' ? dd[1] : dd[2]); };
@@ -1010,6 +1018,7 @@ int ensure_adj(imsrg_basis_s* B) {
     for (const BwdProd& x : kBwd) {
       const int m = dim(x.m), n = dim(x.n), k = dim(x.k);
       if (m == 0 || n == 0 || k == 0) continue;
+      if (fused_pp(x)) continue;    // in ppb below
       const int r = used[x.out_slot][x.out_g]++;
       GemmProb p{};
       p.a1 = b.goff[q * 6 + x.a_g]; p.b1 = b.goff[q * 6 + x.b_g]; p.c = b.goff[q * 6 + x.out_g];
@@ -1023,11 +1032,64 @@ int ensure_adj(imsrg_basis_s* B) {
     }
     (void)ccls;
     for (GemmFamily& f : B->pb) f.group_end.push_back((int)f.probs_h.size());
+    // The six kBwd products into the pp blocks of Lb and Rb as one problem
+    // each.  The row-p blocks pp | pb | ph of an operator are one contiguous
+    // np x (np + nb + nh) column-major matrix X_p (ld np), and with the
+    // herm symmetry R_pp^T = Rh R_pp, L_pp^T = Lh L_pp of the operators:
+    //   Lb_pp += Wb1_p R_p^T            = Rh (Wb1_pp R_pp + Rh Wb1_x R_x^T)
+    //   Rb_pp += L_pp^T Wb1_pp - Wb1_x L_x^T = Lh (L_pp Wb1_pp - Lh Wb1_x L_x^T)
+    // (x: the pb | ph columns, K = nb + nh): the first product untransposed,
+    // the second with op(B) transposed, the pp output read and written once.
+    const int np = dd[0], nx = dd[1] + dd[2];
+    if (np > 0) {
+      for (int side = 0; side < 2; ++side) {
+        GemmProb p{};
+        const i64 o0 = b.goff[q * 6 + 0], o1 = b.goff[q * 6 + 1];
+        p.a1 = o0; p.b1 = o0; p.c = o0;
+        p.sa1 = side ? 0 : 4; p.sb1 = side ? 4 : 1; p.sc = side ? 3 : 2;
+        p.m = np; p.n = np; p.k = np; p.lda = np; p.ldb = np; p.ldc = np;
+        p.ta1 = 0; p.tb1 = 0;
+        p.nprod = 1;
+        if (nx > 0) {
+          p.nprod = 2; p.k2 = nx; p.a2 = o1; p.b2 = o1; p.ta2 = 0; p.tb2 = 1;
+          p.sa2 = 4; p.sb2 = side ? 0 : 1;
+        }
+        // launched with ascale = Lh, s2scale = Rh: Lb al = Rh, second scale Rh;
+        // Rb al = Lh, second scale -Lh (bit6)
+        p.alpha = 1.0; p.beta = 1.0;
+        p.s2 = side ? -1.0 : 1.0;
+        p.mode = (unsigned char)(side ? (4 | 1 | 64) : (4 | 2));
+        pp.probs_h.push_back(p);
+        pp.nks.push_back(1);
+      }
+    }
+    // The transposed products that stream a whole pp block for the pb and
+    // ph outputs (kBwd's R_pp^T Wb1_pb, R_pp^T Wb1_ph and the L_pp ones):
+    //   Lb_x += -R_pp^T Wb1_x = -Rh R_pp Wb1_x,  Rb_x += L_pp^T Wb1_x = Lh L_pp Wb1_x
+    // with x = pb | ph (one np x (nb + nh) block, ld np): one product each.
+    if (np > 0 && nx > 0) {
+      for (int side = 0; side < 2; ++side) {
+        GemmProb p{};
+        const i64 o0 = b.goff[q * 6 + 0], o1 = b.goff[q * 6 + 1];
+        p.a1 = o0; p.b1 = o1; p.c = o1;
+        p.sa1 = side ? 0 : 1; p.sb1 = 4; p.sc = side ? 3 : 2;
+        p.m = np; p.n = nx; p.k = np; p.lda = np; p.ldb = np; p.ldc = np;
+        p.nprod = 1;
+        p.alpha = side ? 1.0 : -1.0; p.beta = 1.0;
+        p.mode = (unsigned char)(side ? (4 | 1) : (4 | 2));
+        pp.probs_h.push_back(p);
+        pp.nks.push_back(1);
+      }
+    }
+    pp.group_end.push_back((int)pp.probs_h.size());
   }
   // W' transpose per cross-coupled block, U the upper-triangular cotangent
   // of W' = Lh LCC RCC^T - Rh RCC LCC^T:
-  //   LCCb += Lh U RCC - Rh U^T RCC,   RCCb += Lh U^T LCC - Rh U LCC
-  // (alpha = Lh and s2 = -Rh Lh at launch; triangular A limits each tile's K)
+  //   LCCb += Lh U RCC - Rh U^T RCC = Lh (U RCC - Rh Lh U^T RCC),
+  //   RCCb += -Rh U LCC + Lh U^T LCC = -Rh (U LCC - Rh Lh U^T LCC)
+  // (ascale = Lh and s2scale = -Rh Lh at launch: al = Lh, resp. Lh s2scale;
+  // both sides take U then U^T, so one kernel specialised on the transposes
+  // runs the family; triangular A limits each tile's K)
   B->phb.probs_h.clear(); B->phb.nks.clear(); B->phb.group_end.clear(); B->phb.tile = 0;
   for (int q = 0; q < b.ncc; ++q) {
     const int r = b.rlen[q], nb = b.ncol[q];
@@ -1037,9 +1099,9 @@ int ensure_adj(imsrg_basis_s* B) {
         p.a1 = b.wcoff[q]; p.b1 = b.ccoff[q]; p.a2 = b.wcoff[q]; p.b2 = b.ccoff[q]; p.c = b.ccoff[q];
         p.m = r; p.n = nb; p.k = r;
         p.lda = r; p.ldb = r; p.ldc = r;
-        p.nprod = 2; p.ta1 = (unsigned char)side; p.tb1 = 0; p.ta2 = (unsigned char)(1 - side); p.tb2 = 0;
+        p.nprod = 2; p.ta1 = 0; p.tb1 = 0; p.ta2 = 1; p.tb2 = 0;
         p.sa1 = 4; p.sb1 = side ? 0 : 1; p.sa2 = 4; p.sb2 = side ? 0 : 1; p.sc = side ? 3 : 2;
-        p.alpha = 1.0; p.s2 = 1.0; p.beta = 1.0; p.mode = 1 | 4 | 8 | 16;
+        p.alpha = 1.0; p.s2 = 1.0; p.beta = 1.0; p.mode = (unsigned char)(1 | 4 | 8 | 16 | (side ? 2 : 0));
         B->phb.probs_h.push_back(p);
         B->phb.nks.push_back(1);
       }
@@ -1120,6 +1182,9 @@ int ensure_adj(imsrg_basis_s* B) {
     f.xcd_aware = (B->tile_req[4] & IMSRG_TILE_XCD) != 0;
     if ((rc = family_tasks(B, f, B->tile_req[4] & 7))) return rc;
   }
+  if ((rc = upload(B, B->ppb.probs_h, &B->ppb.probs))) return rc;
+  B->ppb.xcd_aware = (B->tile_req[6] & IMSRG_TILE_XCD) != 0;
+  if ((rc = family_tasks(B, B->ppb, B->tile_req[6] & 7))) return rc;
   if ((rc = upload(B, B->phb.probs_h, &B->phb.probs))) return rc;
   B->phb.xcd_aware = (B->tile_req[5] & IMSRG_TILE_XCD) != 0;
   if ((rc = family_tasks(B, B->phb, B->tile_req[5] & 7))) return rc;
@@ -1196,6 +1261,9 @@ void comm_vjp(imsrg_basis_s* B, const double* L, int Lh, const double* R, int Rh
   launch_221_vjp(D, B->one, (int)B->one_h.size(), Lh, Rh, RESb, RESh, B->Wb1, B->Wb2, s);
   Slots sp{};
   sp.p[0] = L; sp.p[1] = R; sp.p[2] = Lb; sp.p[3] = Rb; sp.p[4] = B->Wb1; sp.p[5] = B->Wb2;
+  if (B->ppb.tasks.n)
+    launch_gemm_tiled(B->ppb.tile, sp, B->ppb.probs, B->ppb.tasks.dev, B->ppb.tasks.grid, B->ppb.tasks.xmap, s,
+                      (double)Lh, (double)Rh, 1, B->ppb.sig);
   for (GemmFamily& f : B->pb)
     if (f.tasks.n) launch_gemm_tiled(f.tile, sp, f.probs, f.tasks.dev, f.tasks.grid, f.tasks.xmap, s, (double)Lh,
                                      (double)Rh, 1, f.sig);
@@ -3083,7 +3151,7 @@ int imsrg_flow_grad(imsrg_flow F, const double* T, double ds, double bch_conv, d
 
 int imsrg_set_gemm_tile(imsrg_basis B, int family, int tile) {
   const int shape = tile & 7;
-  if (!B || family < 0 || family > 5 || shape > 7 || tile < 0 || tile > (IMSRG_TILE_XCD | 7))
+  if (!B || family < 0 || family > 6 || shape > 7 || tile < 0 || tile > (IMSRG_TILE_XCD | 7))
     return fail(IMSRG_EINVAL, "bad gemm family / tile");
   if (family >= 4) {   // the reverse-mode families: every round / side
     B->tile_req[family] = tile;
@@ -3095,8 +3163,10 @@ int imsrg_set_gemm_tile(imsrg_basis B, int family, int tile) {
     };
     if (family == 4)
       for (GemmFamily& g : B->pb) set(g);
-    else
+    else if (family == 5)
       set(B->phb);
+    else
+      set(B->ppb);
     return rc;
   }
   GemmFamily* f[4] = {&B->pp1, &B->k2, &B->k3, &B->ph};
@@ -3133,7 +3203,7 @@ int imsrg_commutator_222_ph_noalloc(imsrg_basis B, const double* LCC, int Lh, co
 }
 
 int imsrg_get_gemm_tile(imsrg_basis B, int family) {
-  if (!B || family < 0 || family > 5) return -1;
+  if (!B || family < 0 || family > 6) return -1;
   return B->tile_req[family];
 }
 

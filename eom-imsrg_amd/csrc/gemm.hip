@@ -182,7 +182,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
     // leading dimensions packed in a2/b2's problem (lda2 = ldb2 = m here)
     if (pr.k2 > 0)
       mma_p<TBM, TBN, SIG, 2>(acc, As, Bs, slot(sl, pr.sa2, inst) + pr.a2, pr.m, pr.ta2, slot(sl, pr.sb2, inst) + pr.b2,
-                              pr.n, pr.tb2, pr.m, pr.n, 0, pr.k2, m0, n0, pr.s2 * s2scale);
+                              pr.n, pr.tb2, pr.m, pr.n, 0, pr.k2, m0, n0, pr.s2 * ((pr.mode & 64) ? ascale : s2scale));
     else
       mma_p<TBM, TBN, SIG, 2>(acc, As, Bs, slot(sl, pr.sa2, inst) + pr.a2, pr.lda, pr.ta2, slot(sl, pr.sb2, inst) + pr.b2,
                               pr.ldb, pr.tb2, pr.m, pr.n, (tri && !pr.ta2) ? max(kb, m0) : kb,
@@ -361,7 +361,7 @@ __global__ __launch_bounds__(256) void k_gemm_d(Slots sl, const GemmProb* probs,
   p2.ta = pr.ta2; p2.tb = pr.tb2;
   if (pr.k2 > 0) {
     // second product with its own K and leading dimensions m, n
-    p2.lda = pr.m; p2.ldb = pr.n; p2.k0 = 0; p2.k1 = pr.k2; p2.scale = pr.s2 * s2scale;
+    p2.lda = pr.m; p2.ldb = pr.n; p2.k0 = 0; p2.k1 = pr.k2; p2.scale = pr.s2 * ((pr.mode & 64) ? ascale : s2scale);
   } else {
     p2.lda = pr.lda; p2.ldb = pr.ldb;
     p2.k0 = (tri && !pr.ta2) ? max(kb, m0) : kb;

@@ -55,7 +55,9 @@ struct GemmProb {
   unsigned char sa1, sb1, sa2, sb2;
   unsigned char sc, mode, pad1, pad2;   // mode 0: alpha*ascale, s2*s2scale; else alpha is
                                         // scaled by ascale (bit0) and s2scale (bit1), s2 by
-                                        // s2scale (bit4); bit3: A upper triangular
+                                        // s2scale (bit4); bit3: A upper triangular;
+                                        // bit6: a second product with its own K
+                                        // (k2 > 0) scaled by ascale, not s2scale
   double alpha, s2, beta;
   int kchunk;        // > 0: split K in chunks of kchunk, chunk ks written to C + ks*cstride
   int k2;            // > 0: K of the second product when it differs (no split K then)

@@ -331,6 +331,9 @@ void launch_to_dense(const DevBasis& B, const SegTask* t, int nt, const double* 
 #ifndef IMSRG_122_UNROLL
 #define IMSRG_122_UNROLL 2
 #endif
+#ifndef IMSRG_122_COPIES
+#define IMSRG_122_COPIES 4
+#endif
 // The +-1 factors of a term (the record's sign, the herm factor of a
 // transposed element: ch for cbar, Rh / Lh for R / L, and Lh / Rh of a
 // transposed one-body element) combine into one sign bit per product,
@@ -387,19 +390,23 @@ __global__ __launch_bounds__(256) void k_122_vjp(DevBasis B, const SegTask* task
                                                  const int* colp, const int* f1off, int nf1,
                                                  const double* L, int Lh, const double* R, int Rh,
                                                  const double* cbar, const double* RESb, int RESh, double* Lb,
-                                                 double* Rb) {
+                                                 double* Rb, int ncopy) {
+  // one-body sums in ncopy copies (lanes 64 c / ncopy .. of each wave add to
+  // copy c): consecutive lanes often add to the same element at once (rows
+  // sharing their first orbit; the shared column list), and the LDS
+  // serialises such adds per copy
   extern __shared__ double sh[];
   double* fL = sh;
   double* fR = sh + nf1;
-  double* lbar = sh + 2 * nf1;
-  double* rbar = sh + 3 * nf1;
+  const int cs = 2 * nf1 + (ncopy > 1);   // stride of the copies
+  double* lbar = sh + 2 * nf1 + (((threadIdx.x & 63) * ncopy) >> 6) * cs;
+  double* rbar = lbar + nf1;
   for (int k = threadIdx.x; k < nf1; k += blockDim.x) {
     const int o = f1off[k];
     fL[k] = L[o];
     fR[k] = R[o];
-    lbar[k] = 0.0;
-    rbar[k] = 0.0;
   }
+  for (int k = threadIdx.x; k < ncopy * cs; k += blockDim.x) sh[2 * nf1 + k] = 0.0;
   __syncthreads();
   const int ch = RESh;
   for (int ti = xcd_contig(blockIdx.x, gridDim.x); ti < ntasks; ti += gridDim.x) {
@@ -458,8 +465,13 @@ __global__ __launch_bounds__(256) void k_122_vjp(DevBasis B, const SegTask* task
   }
   __syncthreads();
   for (int k = threadIdx.x; k < nf1; k += blockDim.x) {
-    if (lbar[k] != 0.0) aadd(Lb + f1off[k], lbar[k]);
-    if (rbar[k] != 0.0) aadd(Rb + f1off[k], rbar[k]);
+    double vl = 0.0, vr = 0.0;
+    for (int c = 0; c < ncopy; ++c) {
+      vl += sh[2 * nf1 + c * cs + k];
+      vr += sh[2 * nf1 + c * cs + nf1 + k];
+    }
+    if (vl != 0.0) aadd(Lb + f1off[k], vl);
+    if (vr != 0.0) aadd(Rb + f1off[k], vr);
   }
 }
 
@@ -469,8 +481,13 @@ void launch_122_vjp(const DevBasis& B, const SegTask* t, int nt, const Partner* 
                     hipStream_t s) {
   if (nt <= 0) return;
   const int grid = nt;
-  hipLaunchKernelGGL(k_122_vjp, dim3(grid), dim3(256), 4 * nf1 * sizeof(double), s, B, t, nt, part, partT, pstride,
-                     rowp, colp, f1off, nf1, L, Lh, R, Rh, cbar, RESb, RESh, Lb, Rb);
+  // as many copies of the one-body sums (4, 2 or 1) as 64 KB of LDS hold
+  int ncopy = IMSRG_122_COPIES;
+  auto bytes = [&](int nc) { return (2 * nf1 + nc * (2 * nf1 + (nc > 1))) * sizeof(double); };
+  while (ncopy > 1 && bytes(ncopy) > 65536) ncopy >>= 1;
+  const size_t lds = bytes(ncopy);
+  hipLaunchKernelGGL(k_122_vjp, dim3(grid), dim3(256), lds, s, B, t, nt, part, partT, pstride, rowp, colp, f1off,
+                     nf1, L, Lh, R, Rh, cbar, RESb, RESh, Lb, Rb, ncopy);
 }
 
 // -------------------------------------------------- 222_ph transpose

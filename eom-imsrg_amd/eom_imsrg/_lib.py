@@ -147,7 +147,7 @@ SIGNATURES = {
 }
 TSIZE = dict(OP=0, NBLOCKS=1, NCC=2, CC=3, W=4)
 
-GEMM = dict(PPPP=0, PPX=1, XX=2, PH=3, ADJ=4, ADJ_PH=5)
+GEMM = dict(PPPP=0, PPX=1, XX=2, PH=3, ADJ=4, ADJ_PH=5, ADJ_PP=6)
 GENERATOR = dict(white=0, atan=1, w2=2)
 KERNEL = dict(PPPP=0, PPHH_SMALL=1, PH_GEMM=2, PH_SCATTER=3, K122=4, PANDYA=5, ONEBODY=6, COMMUTATOR=7,
               PPPP_GEMM=8, COPY=9)

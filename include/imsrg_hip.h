@@ -392,6 +392,7 @@ int imsrg_flow_grad(imsrg_flow f, const double* T, double ds, double bch_conv, d
 #define IMSRG_GEMM_PH 3     /* 222_ph W' = Lh LCC RCC^T - Rh RCC LCC^T */
 #define IMSRG_GEMM_ADJ 4    /* reverse mode: the transposed 222_pp_hh products */
 #define IMSRG_GEMM_ADJ_PH 5 /* reverse mode: the transpose of the W' product */
+#define IMSRG_GEMM_ADJ_PP 6 /* reverse mode: the transposed products into the pp blocks */
 int imsrg_set_gemm_tile(imsrg_basis b, int family, int tile);
 int imsrg_get_gemm_tile(imsrg_basis b, int family);
 /* No counterpart in the reference: on the grouped path, run the full

@@ -66,8 +66,9 @@ def test_commutator_vjp(gpu, emax, hL, hR):
 
 @pytest.mark.parametrize("tile", [0, 5, 6, 7, 6 | 8])
 def test_commutator_vjp_tile_shapes(gpu, tile):
-    """the transposed 222_pp_hh products and the W' transpose on every tile
-    shape (LDS-staged and register-direct, one kernel per transpose pair):
+    """the transposed 222_pp_hh products (the pp-output problems and the
+    rounds of the others) and the W' transpose on every tile shape
+    (LDS-staged and register-direct, one kernel per transpose pair):
     equal to the oracle's transposes at eMax 4 and to the default tile
     shape's result at eMax 6 (blocks larger than the tiles)"""
     from eom_imsrg import SqOp, _lib, adjoint
@@ -80,15 +81,15 @@ def test_commutator_vjp_tile_shapes(gpu, tile):
         Ld, Rd = SqOp.from_numpy(b, L, hL), SqOp.from_numpy(b, R, hR)
         out = []
         for t in (3, tile):
-            for fam in ("ADJ", "ADJ_PH"):
+            for fam in ("ADJ", "ADJ_PH", "ADJ_PP"):
                 _lib.check(_lib.lib.imsrg_set_gemm_tile(b.handle, _lib.GEMM[fam], t))
             Lb = torch.zeros(b.oplen, dtype=torch.float64, device="cuda")
             Rb = torch.zeros_like(Lb)
             adjoint.commutator_vjp(Ld, Rd, _t(x), hres, Lb, Rb)
             torch.cuda.synchronize()
             out.append((Lb.cpu().numpy(), Rb.cpu().numpy()))
-        for fam in ("ADJ", "ADJ_PH"):
-            _lib.check(_lib.lib.imsrg_set_gemm_tile(b.handle, _lib.GEMM[fam], 3))
+        for fam, t in (("ADJ", 5), ("ADJ_PH", 6), ("ADJ_PP", 11)):
+            _lib.check(_lib.lib.imsrg_set_gemm_tile(b.handle, _lib.GEMM[fam], t))
         (l0, r0), (l1, r1) = out
         assert_rel(_sym(b, l1, hL), _sym(b, l0, hL), rtol=1e-11, what=f"Lbar tile {tile} vs 3")
         assert_rel(_sym(b, r1, hR), _sym(b, r0, hR), rtol=1e-11, what=f"Rbar tile {tile} vs 3")
