@@ -92,8 +92,6 @@ __device__ __forceinline__ double sixj(const DevBasis& B, int j1, int j2, int j3
   return B.six[idx];
 }
 
-// (row, col) of element t of the upper triangle (row <= col) of an n x n
-// block, enumerated column by column.
 // sum over the 64 lanes of a wave (every lane receives the total)
 __device__ __forceinline__ double wave_sum(double v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -154,6 +152,8 @@ __device__ __forceinline__ void partner_addr(const i64* go, const int* dm, int r
 #undef IMSRG_PA
 }
 
+// (row, col) of element t of the upper triangle (row <= col) of an n x n
+// block, enumerated column by column.
 __device__ __forceinline__ void upper_decode(i64 t, int& row, int& col) {
   int c = (int)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
   while ((i64)c * (c + 1) / 2 > t) --c;

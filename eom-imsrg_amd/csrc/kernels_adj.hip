@@ -339,7 +339,9 @@ void launch_to_dense(const DevBasis& B, const SegTask* t, int nt, const double* 
 // transposed one-body element) combine into one sign bit per product,
 // applied to pre = pre_other * (1 or sqrt 2) (pa1 / pa2); chneg / lneg /
 // rneg: RESh / Lh / Rh < 0.
-template <bool ROW>
+// RED: the list is the same for every lane of the wave (all active), so each
+// term's one-body sums are reduced across the wave and added by one lane
+template <bool ROW, bool RED = false>
 __device__ __forceinline__ void vjp122(double& y1, double& y2, const Partner* part, i64 ps, int n,
                                        const PartnerAddr A, double pa1, double pa2, double sb, int chneg,
                                        const double* fL, const double* fR, double* lbar, double* rbar,
@@ -362,6 +364,7 @@ __device__ __forceinline__ void vjp122(double& y1, double& y2, const Partner* pa
         o[u] = pa_off(A, (p[u].code >> 22) & 3, p[u].code & 0x3fffff);
         c[u] = cbar[o[u]];
         if (ob) { r[u] = R[o[u]]; l[u] = L[o[u]]; }
+        else if (RED) { r[u] = 0.0; l[u] = 0.0; }
       }
 #pragma unroll
     for (int u = 0; u < U; ++u)
@@ -374,7 +377,15 @@ __device__ __forceinline__ void vjp122(double& y1, double& y2, const Partner* pa
         const int tc = neg ^ (tr & chneg);
         y1 = fma(fR[fk] * ((tc ^ (fs & rneg)) ? -a : a), c[u], y1);
         y2 = fma(fL[fk] * ((tc ^ (fs & lneg)) ? -a : a), c[u], y2);
-        if (ob) {
+        if (RED) {
+          const double al = (neg ^ (tr & rneg) ^ (fs & lneg)) ? -a : a;
+          const double ar = (neg ^ 1 ^ (tr & lneg) ^ (fs & rneg)) ? -a : a;
+          const double vl = wave_sum((sb * al) * r[u]), vr = wave_sum((sb * ar) * l[u]);
+          if ((threadIdx.x & 63) == 0) {
+            unsafeAtomicAdd(lbar + fk, vl);
+            unsafeAtomicAdd(rbar + fk, vr);
+          }
+        } else if (ob) {
           const double al = (neg ^ (tr & rneg) ^ (fs & lneg)) ? -a : a;
           const double ar = (neg ^ 1 ^ (tr & lneg) ^ (fs & rneg)) ? -a : a;
           unsafeAtomicAdd(lbar + fk, (sb * al) * r[u]);
@@ -391,15 +402,15 @@ __global__ __launch_bounds__(256) void k_122_vjp(DevBasis B, const SegTask* task
                                                  const double* L, int Lh, const double* R, int Rh,
                                                  const double* cbar, const double* RESb, int RESh, double* Lb,
                                                  double* Rb, int ncopy) {
-  // one-body sums in ncopy copies (lanes 64 c / ncopy .. of each wave add to
-  // copy c): consecutive lanes often add to the same element at once (rows
-  // sharing their first orbit; the shared column list), and the LDS
-  // serialises such adds per copy
+  // one-body sums of the row lists in ncopy copies (lane k adds to copy
+  // k % ncopy): neighbouring lanes, rows that share their first orbit, often
+  // add to the same element at once, and the LDS serialises such adds per
+  // copy
   extern __shared__ double sh[];
   double* fL = sh;
   double* fR = sh + nf1;
   const int cs = 2 * nf1 + (ncopy > 1);   // stride of the copies
-  double* lbar = sh + 2 * nf1 + (((threadIdx.x & 63) * ncopy) >> 6) * cs;
+  double* lbar = sh + 2 * nf1 + (threadIdx.x & (ncopy - 1)) * cs;
   double* rbar = lbar + nf1;
   for (int k = threadIdx.x; k < nf1; k += blockDim.x) {
     const int o = f1off[k];
@@ -411,8 +422,10 @@ __global__ __launch_bounds__(256) void k_122_vjp(DevBasis B, const SegTask* task
   const int ch = RESh;
   for (int ti = xcd_contig(blockIdx.x, gridDim.x); ti < ntasks; ti += gridDim.x) {
     const SegTask t = tasks[ti];
-    const i64 e = (i64)t.start + threadIdx.x;
-    if (e >= t.cnt) continue;
+    // lanes past the task's end stay active (the wave-wide sums below) on
+    // its last element, with a zero cotangent and no stores
+    const bool valid = (i64)t.start + threadIdx.x < t.cnt;
+    const i64 e = valid ? (i64)t.start + threadIdx.x : t.cnt - 1;
     const int q = t.q, g = t.g;
     const int rc = row_cls(g), cc = col_cls(g);
     const int n1 = B.dims[q * 3 + rc];
@@ -427,7 +440,7 @@ __global__ __launch_bounds__(256) void k_122_vjp(DevBasis B, const SegTask* task
     const double pre_ab = (a == b) ? 1.4142135623730951 : 1.0;
     const double pre_cd = (c == d) ? 1.4142135623730951 : 1.0;
     const double nrm = 1.0 / (sqrt(1.0 + (a == b ? 1.0 : 0.0)) * sqrt(1.0 + (c == d ? 1.0 : 0.0)));
-    double sb = fold2(B, RESb, q, g, IX, JX, RESh);
+    double sb = valid ? fold2(B, RESb, q, g, IX, JX, RESh) : 0.0;
     sb = sb / sqrt(1.0 + (a == b ? 1.0 : 0.0)) / sqrt(1.0 + (c == d ? 1.0 : 0.0));
     PartnerAddr AR, AC;
     partner_addr(B.goff + q * 6, B.dims + q * 3, rc, cc, IX, JX, AR, AC);
@@ -440,12 +453,13 @@ __global__ __launch_bounds__(256) void k_122_vjp(DevBasis B, const SegTask* task
     const int gc0 = __builtin_amdgcn_readfirstlane((int)gc);
     if (__all((int)gc == gc0)) {
       const int x0 = colp[gc0], nx = colp[gc0 + 1] - x0;
-      vjp122<false>(y1, y2, part + x0, 1, nx, AC, pre_ab, pre_ab * 1.4142135623730951, sb, chneg, fL, fR, lbar, rbar,
-                    cbar, L, lneg, R, rneg);
+      vjp122<false, true>(y1, y2, part + x0, 1, nx, AC, pre_ab, pre_ab * 1.4142135623730951, sb, chneg, fL, fR, lbar,
+                          rbar, cbar, L, lneg, R, rneg);
     } else {
       vjp122<false>(y1, y2, part + colp[gc], 1, colp[gc + 1] - colp[gc], AC, pre_ab, pre_ab * 1.4142135623730951, sb,
                     chneg, fL, fR, lbar, rbar, cbar, L, lneg, R, rneg);
     }
+    if (!valid) continue;
     y1 *= nrm;
     y2 *= nrm;
     const i64 o = B.goff[q * 6 + g] + IX + (i64)JX * n1;
