@@ -340,7 +340,10 @@ int imsrg_batch_magnus_decouple(imsrg_flow f, double* HS, double* G, double ds, 
  * arrays and ACCUMULATE into the outputs (Lb, Rb, Gb, Hb, ETAb) unless
  * stated; they are exact for perturbations that keep the operators'
  * herm symmetry and the zero Pauli-forbidden (a,a) odd-J elements, which
- * is every operator of the flow.  Asynchronous on `stream` except the
+ * is every operator of the flow.  The operators passed in must themselves
+ * be (anti)hermitian as flagged (stored in both triangles, as every
+ * operator of the flow is): the transposes use X^T = herm X on their
+ * square blocks.  Asynchronous on `stream` except the
  * drivers that need the reference's data-dependent exits (bch / magnus /
  * flow), which synchronise like their forward counterparts. */
 
@@ -392,7 +395,7 @@ int imsrg_flow_grad(imsrg_flow f, const double* T, double ds, double bch_conv, d
 #define IMSRG_GEMM_PH 3     /* 222_ph W' = Lh LCC RCC^T - Rh RCC LCC^T */
 #define IMSRG_GEMM_ADJ 4    /* reverse mode: the transposed 222_pp_hh products */
 #define IMSRG_GEMM_ADJ_PH 5 /* reverse mode: the transpose of the W' product */
-#define IMSRG_GEMM_ADJ_PP 6 /* reverse mode: the transposed products into the pp blocks */
+#define IMSRG_GEMM_ADJ_PP 6 /* reverse mode: the fused transposed products that write or stream pp blocks */
 int imsrg_set_gemm_tile(imsrg_basis b, int family, int tile);
 int imsrg_get_gemm_tile(imsrg_basis b, int family);
 /* No counterpart in the reference: on the grouped path, run the full
