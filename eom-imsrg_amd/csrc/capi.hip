@@ -91,8 +91,10 @@ struct imsrg_basis_s {
   int path = IMSRG_PATH_GROUPED;
   bool dev_ready = false;
   // GEMM tile shapes for pp1, k2, k3, ph (64x64 until tuned) and the reverse-mode
-  // 222_pp_hh rounds and W' transpose (64x32 through LDS: the reverse sweep at
-  // eMax 8 takes 0.47 s with it, 0.50-0.51 s on the register-direct shapes)
+  // 222_pp_hh rounds (32x32 through LDS), W' transpose (register-direct 128x64)
+  // and fused pp products (64x32 through LDS, XCD packing): the fastest
+  // commutator transpose at eMax 8 (1.54 ms against 1.62 ms with 64x32, 64x32
+  // and register-direct 128x64)
   int tile_req[7] = {0, 0, 0, 0, 5, 6, 11};
   int plan_ver = 0;                 // bumped whenever a task list is rebuilt (captured graphs go stale)
   DevBasis D{};
@@ -152,7 +154,7 @@ struct imsrg_basis_s {
   // rounds of distinct outputs, the W' transpose products, quadruple
   // starts, the compact one-body map of 122
   bool adj_ready = false;
-  GemmFamily pb[6], ppb, phb;   // reverse-mode 222_pp_hh rounds, their pp outputs, the W' transpose
+  GemmFamily pb[6], ppb, phb;   // reverse-mode 222_pp_hh rounds, the fused pp products, the W' transpose
   std::vector<int> f1map_h, f1off_h;   // 122: one-body offset -> compact index, and back
   int* f1off = nullptr;
   int nf1 = 0;
@@ -1033,7 +1035,7 @@ int ensure_adj(imsrg_basis_s* B) {
     (void)ccls;
     for (GemmFamily& f : B->pb) f.group_end.push_back((int)f.probs_h.size());
     // The six kBwd products into the pp blocks of Lb and Rb as one problem
-    // each.  The row-p blocks pp | pb | ph of an operator are one contiguous
+    // per output.  The row-p blocks pp | pb | ph of an operator are one contiguous
     // np x (np + nb + nh) column-major matrix X_p (ld np), and with the
     // herm symmetry R_pp^T = Rh R_pp, L_pp^T = Lh L_pp of the operators:
     //   Lb_pp += Wb1_p R_p^T            = Rh (Wb1_pp R_pp + Rh Wb1_x R_x^T)
